@@ -1,0 +1,291 @@
+"""Python bindings (ctypes) over libcf1g.so, the MI355X phase library.
+
+The compute path is the HIP library; this module only marshals buffers. It
+never falls back to a CPU implementation: if libcf1g.so is missing or no
+gfx950 device is visible, every entry point raises.
+
+Method names follow the reference's phase functions (src/pipeline.rs:66-118):
+count_minimizer_histogram (P1), partition_minimizers_and_route (P2),
+build_mphf (P3), classify_vertices (P4).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcf1g.so")
+_LIB = None
+
+U64P = C.POINTER(C.c_uint64)
+U32P = C.POINTER(C.c_uint32)
+U16P = C.POINTER(C.c_uint16)
+U8P = C.POINTER(C.c_uint8)
+
+CF1G_OK = 0
+ERRORS = {-1: "CF1G_E_INVALID", -2: "CF1G_E_ORDER", -3: "CF1G_E_HIP", -4: "CF1G_E_NODEVICE", -5: "CF1G_E_INTERNAL"}
+NUM_BUCKETS = 65536
+
+# Every symbol include/cf1g.h declares.
+EXPORTS = [
+    "cf1g_create", "cf1g_destroy", "cf1g_last_error", "cf1g_get_info", "cf1g_load_records",
+    "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
+    "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
+    "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
+    "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_profiling", "cf1g_kernel_times",
+    "cf1g_device_keys",
+]
+
+
+class Cf1gError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CfParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("num_bins", C.c_uint32), ("m", C.c_uint32), ("p4_chunk", C.c_uint32),
+                ("p4_wgs_per_cu", C.c_uint32), ("p3_target_seg", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class CfInfo(C.Structure):
+    _fields_ = [("n_records", C.c_uint64), ("stream_len", C.c_uint64), ("n_kmers", C.c_uint64),
+                ("n_superkmers", C.c_uint64), ("n_distinct", C.c_uint64), ("n_state_words", C.c_uint64),
+                ("mphf_words", C.c_uint64), ("num_bins", C.c_uint32), ("seg_bits", C.c_uint32),
+                ("mphf_levels", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run `make -C cf1-rs_amd`)")
+        L = C.CDLL(LIB_PATH)
+        L.cf1g_create.argtypes = [C.POINTER(CfParams), C.POINTER(C.c_void_p)]
+        L.cf1g_destroy.argtypes = [C.c_void_p]
+        L.cf1g_last_error.restype = C.c_char_p
+        L.cf1g_last_error.argtypes = [C.c_void_p]
+        L.cf1g_get_info.argtypes = [C.c_void_p, C.POINTER(CfInfo)]
+        L.cf1g_load_records.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64]
+        L.cf1g_p1_histogram.argtypes = [C.c_void_p, U64P]
+        L.cf1g_p2_partition_route.argtypes = [C.c_void_p, U64P, C.c_uint32, U16P, U32P]
+        L.cf1g_p3_build.argtypes = [C.c_void_p, U64P]
+        L.cf1g_p4_classify.argtypes = [C.c_void_p]
+        L.cf1g_run_p1_p4.argtypes = [C.c_void_p, U64P]
+        L.cf1g_states_export.argtypes = [C.c_void_p, U64P, C.c_uint64]
+        L.cf1g_keys_export.argtypes = [C.c_void_p, U64P, C.c_uint64]
+        L.cf1g_mphf_export.argtypes = [C.c_void_p, U64P, U64P]
+        L.cf1g_mphf_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
+        L.cf1g_mphf_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.cf1g_annotate.argtypes = [C.c_void_p, U8P, C.c_uint64]
+        L.cf1g_record_starts.argtypes = [C.c_void_p, U64P, C.c_uint64]
+        L.cf1g_bins_export.argtypes = [C.c_void_p, U32P, U64P, U64P, C.c_uint64]
+        L.cf1g_p5_write.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64, C.c_char_p, U64P, C.c_char_p, C.c_char_p,
+                                    C.c_int, C.c_int, C.c_int, U64P]
+        L.cf1g_sync.argtypes = [C.c_void_p]
+        L.cf1g_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.cf1g_kernel_times.restype = C.c_size_t
+        L.cf1g_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]
+        L.cf1g_device_keys.restype = C.c_void_p
+        L.cf1g_device_keys.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def pack_records(seqs):
+    """list[bytes] -> (ascii uint8, rec_off uint64[n+1])."""
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    if seqs:
+        off[1:] = np.cumsum([len(s) for s in seqs], dtype=np.uint64)
+    a = np.frombuffer(b"".join(seqs), dtype=np.uint8).copy() if seqs else np.zeros(0, np.uint8)
+    if a.size == 0:
+        a = np.zeros(1, np.uint8)
+    return a, off
+
+
+def minimizer_len(k):
+    """src/params.rs:221-228."""
+    m = min(k // 2, 15)
+    if m % 2 == 0:
+        m = max(m - 1, 1)
+    return m
+
+
+class Cf1g:
+    """One device context: the P1..P4 phases on the loaded records."""
+
+    def __init__(self, k=31, num_bins=128, m=0, p4_chunk=0, p4_wgs_per_cu=0, p3_target_seg=0):
+        self.L = lib()
+        self.k = k
+        p = CfParams(k=k, num_bins=num_bins, m=m, p4_chunk=p4_chunk, p4_wgs_per_cu=p4_wgs_per_cu,
+                     p3_target_seg=p3_target_seg)
+        h = C.c_void_p()
+        rc = self.L.cf1g_create(C.byref(p), C.byref(h))
+        self.h = h
+        self.num_bins_req = num_bins
+        if rc != CF1G_OK:
+            msg = self.L.cf1g_last_error(h).decode() if h else "cf1g_create failed"
+            self.close()
+            raise Cf1gError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cf1g_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != CF1G_OK:
+            raise Cf1gError(rc, self.L.cf1g_last_error(self.h).decode())
+
+    # ---- input
+    def load_records(self, seqs=None, ascii=None, rec_off=None):
+        if seqs is not None:
+            ascii, rec_off = pack_records(seqs)
+        self._ascii = np.ascontiguousarray(ascii, np.uint8)
+        self._off = np.ascontiguousarray(rec_off, np.uint64)
+        self._check(self.L.cf1g_load_records(self.h, _p(self._ascii, U8P), _p(self._off, U64P), len(self._off) - 1))
+
+    # ---- phases (src/pipeline.rs:77-103)
+    def count_minimizer_histogram(self, want=True):
+        h = np.zeros(NUM_BUCKETS, np.uint64) if want else None
+        self._check(self.L.cf1g_p1_histogram(self.h, _p(h, U64P) if want else None))
+        return h
+
+    def partition_minimizers_and_route(self, hist=None, num_bins=None):
+        b2b = np.zeros(NUM_BUCKETS, np.uint16)
+        nb = C.c_uint32(0)
+        hp = _p(np.ascontiguousarray(hist, np.uint64), U64P) if hist is not None else None
+        self._check(self.L.cf1g_p2_partition_route(self.h, hp, num_bins or self.num_bins_req, _p(b2b, U16P), C.byref(nb)))
+        return b2b, nb.value
+
+    def build_mphf(self):
+        d = C.c_uint64(0)
+        self._check(self.L.cf1g_p3_build(self.h, C.byref(d)))
+        return d.value
+
+    def classify_vertices(self):
+        self._check(self.L.cf1g_p4_classify(self.h))
+
+    def run_p1_p4(self):
+        d = C.c_uint64(0)
+        self._check(self.L.cf1g_run_p1_p4(self.h, C.byref(d)))
+        return d.value
+
+    # ---- products
+    def info(self):
+        i = CfInfo()
+        self._check(self.L.cf1g_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in CfInfo._fields_}
+
+    def states(self):
+        n = self.info()["n_state_words"]
+        out = np.zeros(max(n, 1), np.uint64)
+        self._check(self.L.cf1g_states_export(self.h, _p(out, U64P), out.size))
+        return out[:n]
+
+    def keys(self):
+        inf = self.info()
+        wpk = 2 if self.k > 32 else 1
+        out = np.zeros(max(inf["n_distinct"] * wpk, 1), np.uint64)
+        self._check(self.L.cf1g_keys_export(self.h, _p(out, U64P), out.size))
+        out = out[: inf["n_distinct"] * wpk]
+        if wpk == 2:
+            return out[0::2].copy(), out[1::2].copy()
+        return out, np.zeros_like(out)
+
+    def mphf_export(self):
+        n = C.c_uint64(0)
+        self._check(self.L.cf1g_mphf_export(self.h, None, C.byref(n)))
+        out = np.zeros(n.value, np.uint64)
+        self._check(self.L.cf1g_mphf_export(self.h, _p(out, U64P), C.byref(n)))
+        return out
+
+    def mphf_lookup(self, lo, hi=None):
+        lo = np.ascontiguousarray(lo, np.uint64)
+        if self.k > 32:
+            keys = np.empty(2 * len(lo), np.uint64)
+            keys[0::2] = lo
+            keys[1::2] = np.ascontiguousarray(hi, np.uint64)
+        else:
+            keys = lo
+        out = np.zeros(max(len(lo), 1), np.uint64)
+        self._check(self.L.cf1g_mphf_lookup(self.h, _p(keys, U64P), len(lo), _p(out, U64P)))
+        return out[: len(lo)]
+
+    def annotate(self):
+        n = self.info()["stream_len"]
+        out = np.zeros(max(n, 1), np.uint8)
+        self._check(self.L.cf1g_annotate(self.h, _p(out, U8P), out.size))
+        return out[:n]
+
+    def record_starts(self):
+        n = self.info()["n_records"] + 1
+        out = np.zeros(n, np.uint64)
+        self._check(self.L.cf1g_record_starts(self.h, _p(out, U64P), n))
+        return out
+
+    def bins_export(self):
+        inf = self.info()
+        rw = C.c_uint32(0)
+        bs = np.zeros(inf["num_bins"] + 1, np.uint64)
+        self._check(self.L.cf1g_bins_export(self.h, C.byref(rw), _p(bs, U64P), None, 0))
+        recs = np.zeros(max(inf["n_superkmers"] * rw.value, 1), np.uint64)
+        self._check(self.L.cf1g_bins_export(self.h, None, None, _p(recs, U64P), recs.size))
+        return rw.value, bs, recs[: inf["n_superkmers"] * rw.value].reshape(-1, rw.value)
+
+    def p5_write(self, prefix, names=None, input_desc="", poly_n_stretch=True, track_short=True, threads=4):
+        """Host P5: extract_and_output + write_json (src/output.rs:152-354, 785-834)."""
+        if names is not None:
+            enc = [n.encode() for n in names]
+            blob = b"".join(enc)
+            noff = np.zeros(len(enc) + 1, np.uint64)
+            noff[1:] = np.cumsum([len(e) for e in enc])
+            nptr = _p(noff, U64P)
+        else:
+            blob, nptr = None, None
+        meta = np.zeros(5, np.uint64)
+        self._check(self.L.cf1g_p5_write(self.h, _p(self._ascii, U8P), _p(self._off, U64P), len(self._off) - 1, blob, nptr,
+                                         prefix.encode(), input_desc.encode(), int(poly_n_stretch), int(track_short),
+                                         threads, _p(meta, U64P)))
+        return dict(unitigs=int(meta[0]), vertices=int(meta[1]), max_len=int(meta[2]), min_len=int(meta[3]),
+                    sum_len=int(meta[4]))
+
+    # ---- measurement
+    def sync(self):
+        self._check(self.L.cf1g_sync(self.h))
+
+    def set_profiling(self, on):
+        self._check(self.L.cf1g_set_profiling(self.h, int(on)))
+
+    def kernel_times(self, reset=True):
+        n = self.L.cf1g_kernel_times(self.h, None, 0, 0)
+        buf = C.create_string_buffer(max(n, 1) + 256)
+        self.L.cf1g_kernel_times(self.h, buf, len(buf), int(reset))
+        res = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            res[name] = (int(cnt), float(ms))
+        return res
+
+
+def state_of(words, idx):
+    """5-bit state idx from 12-per-word packed words (src/state_vector.rs:36-42)."""
+    idx = np.asarray(idx, np.uint64)
+    w = words[(idx // 12).astype(np.int64)]
+    return ((w >> ((idx % 12) * 5)) & np.uint64(31)).astype(np.uint8)

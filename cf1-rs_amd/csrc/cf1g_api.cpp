@@ -1,0 +1,678 @@
+// cf1g context, phase orchestration and the extern "C" boundary.
+// Mirrors the phase sequence of run_pipeline (src/pipeline.rs:66-118); all
+// intermediates (packed stream, bins, keys, MPHF, states) stay in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cf1g.h"
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace cf1g {
+// kernels_p1p2.hip
+void launch_pack(hipStream_t, const uint8_t*, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t*, uint64_t*);
+void launch_p1_hist(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, unsigned long long*, unsigned long long*);
+void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
+                      uint64_t*, unsigned long long*, uint64_t*, uint64_t*);
+void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, const uint16_t*, int, const uint64_t*,
+                     unsigned long long*, uint64_t*, int, unsigned int*);
+// kernels_p3.hip
+uint64_t scan_temp_words(uint64_t n);
+void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*, uint64_t*);
+void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
+                     uint64_t*);
+void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
+                       uint64_t*, void*);
+void launch_p3_segsort(hipStream_t, bool, void*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, unsigned int*);
+void launch_p3_bigsort(hipStream_t, bool, void*, void*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*);
+void launch_p3_compact(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, void*);
+void launch_bbh_mark(hipStream_t, bool, const void*, uint64_t, int, uint64_t, uint64_t*, uint64_t*);
+void launch_bbh_filter(hipStream_t, bool, const void*, uint64_t, int, uint64_t, const uint64_t*, const uint64_t*, void*, uint64_t*);
+void launch_bbh_blocks(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t*, uint64_t*);
+void launch_bbh_rank(hipStream_t, uint64_t*, const uint64_t*, uint64_t);
+// kernels_p4.hip
+void launch_p4_classify(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const BbhMeta*, const uint64_t*,
+                        uint64_t*, unsigned int*, uint32_t, uint32_t, uint32_t);
+void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const BbhMeta*, const uint64_t*,
+                     const uint64_t*, uint8_t*);
+void launch_lookup(hipStream_t, bool, const void*, uint64_t, const BbhMeta*, const uint64_t*, uint64_t*);
+
+// ------------------------------------------------------------------ profiling
+namespace {
+struct ProfRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_ev_pool;
+hipEvent_t ev_get() {
+  if (!g_ev_pool.empty()) {
+    hipEvent_t e = g_ev_pool.back();
+    g_ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+thread_local hipEvent_t t_pending = nullptr;
+}  // namespace
+
+void prof_begin(const char* name, hipStream_t st) {
+  if (!g_prof_on) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  t_pending = ev_get();
+  hip_check(hipEventRecord(t_pending, st), "hipEventRecord");
+  (void)name;
+}
+void prof_end(const char* name, hipStream_t st) {
+  if (!g_prof_on || !t_pending) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  hipEvent_t e = ev_get();
+  hip_check(hipEventRecord(e, st), "hipEventRecord");
+  g_prof.push_back({name, t_pending, e});
+  t_pending = nullptr;
+}
+
+}  // namespace cf1g
+
+using namespace cf1g;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct cf1g_ctx {
+  cf1g_params prm{};
+  int k = 31, m = 15, w = 17;
+  bool wide = false;
+  int rw = 2;
+  size_t key_bytes = 8;
+  hipStream_t st = nullptr;
+  int device = 0;
+  int cu_count = 256;
+  std::string err;
+  int stage = 0;  // 1 loaded, 2 P1, 3 P2, 4 P3, 5 P4
+  std::map<std::string, DevBuf> bufs;
+
+  // sizes
+  uint64_t n_rec = 0, n_ascii = 0, n_pos = 0, n_words64 = 0;
+  std::vector<uint64_t> gstart_h;
+  uint64_t W = 0, S = 0, D = 0;
+  uint32_t num_bins = 0;
+  int sb = 0;
+  uint64_t n_segs = 0, n_chunks = 0;
+  uint32_t n_over = 0;
+  BbhMeta meta{};
+  uint64_t n_blocks = 0;
+  std::vector<std::pair<uint64_t*, uint64_t*>> lvl_ac;  // per level (A, C)
+
+  void* get(const std::string& name, size_t bytes) {
+    DevBuf& b = bufs[name];
+    if (bytes == 0) bytes = 8;
+    if (b.cap < bytes) {
+      if (b.p) hip_check(hipFree(b.p), "hipFree");
+      b.p = nullptr;
+      b.cap = 0;
+      size_t want = bytes + bytes / 8;  // headroom for repeated runs
+      hip_check(hipMalloc(&b.p, want), ("hipMalloc " + name).c_str());
+      b.cap = want;
+    }
+    return b.p;
+  }
+  template <typename T>
+  T* getT(const std::string& name, size_t count) {
+    return (T*)get(name, count * sizeof(T));
+  }
+  void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+  void zero(void* p, size_t bytes) { hip_check(hipMemsetAsync(p, 0, bytes, st), "hipMemsetAsync"); }
+  void d2h(void* dst, const void* src, size_t bytes) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "hipMemcpy D2H");
+    sync();
+  }
+  void h2d(void* dst, const void* src, size_t bytes) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpy H2D");
+    sync();
+  }
+};
+
+namespace {
+constexpr uint32_t CHUNK_RECORDS = 1024;
+constexpr uint64_t STREAM_PAD = 4096;
+
+struct Fail {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string& m) { throw Fail{code, m}; }
+
+template <typename F>
+int guarded(cf1g_ctx* c, F f) {
+  if (!c) return CF1G_E_INVALID;
+  try {
+    f();
+    return CF1G_OK;
+  } catch (const Fail& e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const HipError& e) {
+    c->err = e.what();
+    return CF1G_E_HIP;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return CF1G_E_INTERNAL;
+  }
+}
+void need(cf1g_ctx* c, int stage, const char* what) {
+  if (c->stage < stage) fail(CF1G_E_ORDER, std::string(what) + " called before its inputs exist");
+}
+int minimizer_len(int k) {  // src/params.rs:221-228
+  int m = std::min(k / 2, 15);
+  if (m % 2 == 0) m = std::max(m - 1, 1);
+  return m;
+}
+uint64_t level_bits(uint64_t remaining) {
+  uint64_t n = std::max<uint64_t>(2 * remaining, 1);
+  return (n + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS * BBH_BLOCK_BITS;
+}
+
+// ---------------------------------------------------------------- phases
+void do_p1(cf1g_ctx* c) {
+  hipStream_t st = c->st;
+  uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
+  uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
+  launch_pack(st, c->getT<uint8_t>("ascii", c->n_ascii + 8), c->getT<uint64_t>("rec_off", c->n_rec + 1),
+              c->getT<uint64_t>("gstart", c->n_rec + 1), c->n_rec, c->n_words64, seq, nm);
+  auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
+  auto* tot = c->getT<unsigned long long>("totals", 4);
+  c->zero(hist, CF1G_NUM_BUCKETS * 8);
+  c->zero(tot, 32);
+  launch_p1_hist(st, seq, nm, c->n_pos, c->k, c->m, hist, tot);
+  c->stage = 2;
+}
+
+void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
+  hipStream_t st = c->st;
+  if (num_bins < 1 || num_bins > CF1G_MAX_BINS) fail(CF1G_E_INVALID, "num_bins must be in [1, 4096]");
+  auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
+  auto* hpart = hist;
+  if (hist_in) {
+    hpart = c->getT<unsigned long long>("hist_in", CF1G_NUM_BUCKETS);
+    hip_check(hipMemcpyAsync(hpart, hist_in, CF1G_NUM_BUCKETS * 8, hipMemcpyHostToDevice, st), "H2D hist");
+  }
+  uint16_t* b2b = c->getT<uint16_t>("b2b", CF1G_NUM_BUCKETS);
+  uint64_t* brec = c->getT<uint64_t>("bin_records", CF1G_MAX_BINS + 1);
+  uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
+  auto* bcur = c->getT<unsigned long long>("bin_cursor", CF1G_MAX_BINS + 1);
+  uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
+  uint64_t* info = c->getT<uint64_t>("info", 4);
+  launch_partition(st, hpart, hist, num_bins, CHUNK_RECORDS, b2b, brec, bstart, bcur, coff, info);
+  uint64_t hinfo[4], htot[4];
+  hip_check(hipMemcpyAsync(hinfo, info, 32, hipMemcpyDeviceToHost, st), "D2H info");
+  hip_check(hipMemcpyAsync(htot, c->getT<uint64_t>("totals", 4), 32, hipMemcpyDeviceToHost, st), "D2H totals");
+  c->sync();
+  c->num_bins = (uint32_t)hinfo[0];
+  c->S = hinfo[1];
+  c->n_chunks = hinfo[2];
+  c->W = htot[0];
+  if (c->S != htot[1]) fail(CF1G_E_INTERNAL, "bin record total differs from the P1 super k-mer count");
+  uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
+  unsigned int* ovf = c->getT<unsigned int>("route_overflow", 2);
+  c->zero(ovf, 4);
+  launch_p2_route(st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), c->n_pos,
+                  c->k, c->m, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw, ovf);
+  unsigned int hov = 0;
+  c->d2h(&hov, ovf, 4);
+  if (hov) fail(CF1G_E_INTERNAL, "P2 routing overflowed a bin (bin sizes disagree with the histogram)");
+  c->stage = 3;
+}
+
+void bbhash_build(cf1g_ctx* c, const void* keys, uint64_t D) {
+  hipStream_t st = c->st;
+  const size_t kb = c->key_bytes;
+  void* surv[2] = {c->get("bbh_surv0", D * kb + 16), c->get("bbh_surv1", D * kb + 16)};
+  uint64_t* cnt = c->getT<uint64_t>("bbh_cnt", 2);
+  const void* cur = keys;
+  uint64_t n = D;
+  uint64_t blk = 0;
+  int l = 0;
+  c->lvl_ac.clear();
+  std::memset(&c->meta, 0, sizeof(c->meta));
+  while (n > 0) {
+    if (l >= BBH_MAX_LEVELS) fail(CF1G_E_INTERNAL, "BBHash exceeded the level limit");
+    uint64_t nb = level_bits(n);
+    uint64_t nw = nb / 64;
+    char an[32], cn[32];
+    std::snprintf(an, sizeof an, "bbhA%d", l);
+    std::snprintf(cn, sizeof cn, "bbhC%d", l);
+    uint64_t* A = c->getT<uint64_t>(an, nw);
+    uint64_t* Cc = c->getT<uint64_t>(cn, nw);
+    c->zero(A, nw * 8);
+    c->zero(Cc, nw * 8);
+    launch_bbh_mark(st, c->wide, cur, n, l, nb, A, Cc);
+    c->zero(cnt, 8);
+    void* nxt = surv[l & 1];
+    launch_bbh_filter(st, c->wide, cur, n, l, nb, A, Cc, nxt, cnt);
+    uint64_t nn = 0;
+    c->d2h(&nn, cnt, 8);
+    c->meta.level_bits[l] = nb;
+    c->meta.level_block_off[l] = blk;
+    blk += nb / BBH_BLOCK_BITS;
+    c->lvl_ac.push_back({A, Cc});
+    if (nn >= n && n > 64) fail(CF1G_E_INTERNAL, "BBHash level placed no key");
+    cur = nxt;
+    n = nn;
+    l++;
+  }
+  c->meta.num_levels = l;
+  c->meta.wide = c->wide ? 1 : 0;
+  c->meta.n_keys = D;
+  c->meta.level_block_off[l] = blk;
+  c->n_blocks = blk;
+  uint64_t* blocks = c->getT<uint64_t>("blocks", blk * 8 + 8);
+  uint64_t* popc = c->getT<uint64_t>("bbh_popc", blk + 1);
+  uint64_t* rank = c->getT<uint64_t>("bbh_rank", blk + 2);
+  for (int lv = 0; lv < l; lv++)
+    launch_bbh_blocks(st, c->lvl_ac[lv].first, c->lvl_ac[lv].second, c->meta.level_bits[lv] / BBH_BLOCK_BITS,
+                      c->meta.level_block_off[lv], blocks, popc);
+  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(blk + 1) + 16);
+  scan_excl_u64(st, popc, rank, blk, rank + blk, tmp);
+  launch_bbh_rank(st, blocks, rank, blk);
+  BbhMeta* dm = (BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta));
+  hip_check(hipMemcpyAsync(dm, &c->meta, sizeof(BbhMeta), hipMemcpyHostToDevice, st), "H2D meta");
+  uint64_t total = 0;
+  c->d2h(&total, rank + blk, 8);
+  if (D > 0 && total != D) fail(CF1G_E_INTERNAL, "BBHash rank total != D");
+}
+
+void do_p3(cf1g_ctx* c) {
+  hipStream_t st = c->st;
+  const size_t kb = c->key_bytes;
+  const int K = c->k;
+  // segment bits: mean segment of ~target k-mers, capped by LDS histogram
+  uint64_t target = c->prm.p3_target_seg ? c->prm.p3_target_seg : (c->wide ? 1024 : 2048);
+  int sb = 0;
+  while (sb < 12 && sb < 2 * K && (double)c->W / ((double)c->num_bins * (double)(1u << sb)) > (double)target) sb++;
+  c->sb = sb;
+  c->n_segs = (uint64_t)c->num_bins << sb;
+  const uint64_t ns = c->n_segs;
+  uint64_t* counts = c->getT<uint64_t>("seg_counts", ns + 1);
+  uint64_t* seg_off = c->getT<uint64_t>("seg_off", ns + 1);
+  uint64_t* cursor = c->getT<uint64_t>("seg_cursor", ns + 1);
+  uint64_t* ucount = c->getT<uint64_t>("seg_ucount", ns + 1);
+  uint64_t* out_off = c->getT<uint64_t>("seg_out_off", ns + 1);
+  uint32_t* over = c->getT<uint32_t>("seg_over", ns + 1);
+  unsigned int* n_over = c->getT<unsigned int>("seg_n_over", 2);
+  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
+  const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
+  const uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
+  const uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
+  c->zero(counts, (ns + 1) * 8);
+  launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts);
+  scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
+  hip_check(hipMemcpyAsync(cursor, seg_off, ns * 8, hipMemcpyDeviceToDevice, st), "D2D cursor");
+  uint64_t wcount = 0;
+  c->d2h(&wcount, seg_off + ns, 8);
+  if (wcount != c->W) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
+  void* occ = c->get("occ", c->W * kb + 16);
+  launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, cursor, occ);
+  c->zero(n_over, 4);
+  launch_p3_segsort(st, c->wide, occ, seg_off, ns, K, sb, ucount, over, n_over);
+  uint32_t nov = 0;
+  c->d2h(&nov, n_over, 4);
+  c->n_over = nov;
+  if (nov > 0) {
+    void* aux = c->get("occ_aux", c->W * kb + 16);
+    launch_p3_bigsort(st, c->wide, occ, aux, seg_off, over, nov, ucount);
+  }
+  scan_excl_u64(st, ucount, out_off, ns, out_off + ns, tmp);
+  uint64_t D = 0;
+  c->d2h(&D, out_off + ns, 8);
+  c->D = D;
+  void* keys = c->get("keys", D * kb + 16);
+  launch_p3_compact(st, c->wide, occ, seg_off, ucount, out_off, ns, keys);
+  bbhash_build(c, keys, D);
+  uint64_t nsw = (D + 11) / 12;
+  c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
+  c->sync();
+  c->stage = 4;
+}
+
+void do_p4(cf1g_ctx* c) {
+  hipStream_t st = c->st;
+  uint32_t chunk = c->prm.p4_chunk ? c->prm.p4_chunk : 8192;
+  uint64_t npos = c->n_pos > 1 ? c->n_pos - 1 : 0;
+  uint64_t nch = (npos + chunk - 1) / chunk;
+  if (nch > 0xFFFFFFFFULL) fail(CF1G_E_INVALID, "stream too long for the P4 chunk size");
+  uint32_t per_cu = c->prm.p4_wgs_per_cu ? c->prm.p4_wgs_per_cu : 8;
+  uint64_t grid = (uint64_t)c->cu_count * per_cu;
+  if (grid > nch) grid = nch;
+  unsigned int* ctr = c->getT<unsigned int>("p4_counter", 2);
+  c->zero(ctr, 4);
+  uint64_t nsw = (c->D + 11) / 12;
+  launch_p4_classify(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
+                     c->n_pos, c->k, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
+                     c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), c->getT<uint64_t>("states", nsw + 1), ctr, chunk,
+                     (uint32_t)nch, (uint32_t)(grid ? grid : 1));
+  c->sync();
+  c->stage = 5;
+}
+
+void check_k(const cf1g_params* p) {
+  if (!p) fail(CF1G_E_INVALID, "params is NULL");
+  if (p->k < 1 || p->k > 63 || p->k % 2 == 0) fail(CF1G_E_INVALID, "k must be odd and in [1, 63], got " + std::to_string(p->k));
+  if (p->num_bins < 1 || p->num_bins > CF1G_MAX_BINS) fail(CF1G_E_INVALID, "num_bins must be in [1, 4096]");
+}
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int cf1g_create(const cf1g_params* params, cf1g_ctx** out) {
+  if (!out) return CF1G_E_INVALID;
+  *out = nullptr;
+  cf1g_ctx* c = new cf1g_ctx();
+  int rc = guarded(c, [&]() {
+    check_k(params);
+    c->prm = *params;
+    c->k = (int)params->k;
+    c->m = params->m ? (int)params->m : minimizer_len(c->k);
+    if (c->m < 1 || c->m > 15 || c->m > c->k || c->m % 2 == 0) fail(CF1G_E_INVALID, "minimizer length must be odd, <= 15 and <= k");
+    c->w = c->k - c->m + 1;
+    c->wide = c->k > 32;
+    c->rw = c->wide ? 4 : 2;
+    c->key_bytes = c->wide ? 16 : 8;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) fail(CF1G_E_NODEVICE, "no HIP device visible");
+    hip_check(hipSetDevice(0), "hipSetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      fail(CF1G_E_NODEVICE, std::string("cf1g is built for gfx950, device is ") + prop.gcnArchName);
+    c->cu_count = prop.multiProcessorCount;
+    hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
+  });
+  if (rc != CF1G_OK) {
+    // keep the context so the caller can read the error
+    *out = c;
+    return rc;
+  }
+  *out = c;
+  return CF1G_OK;
+}
+
+void cf1g_destroy(cf1g_ctx* c) {
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto& kv : c->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char* cf1g_last_error(const cf1g_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cf1g_get_info(cf1g_ctx* c, cf1g_info* o) {
+  return guarded(c, [&]() {
+    if (!o) fail(CF1G_E_INVALID, "out is NULL");
+    std::memset(o, 0, sizeof *o);
+    o->n_records = c->n_rec;
+    o->stream_len = c->n_pos;
+    o->n_kmers = c->W;
+    o->n_superkmers = c->S;
+    o->n_distinct = c->D;
+    o->n_state_words = (c->D + 11) / 12;
+    o->mphf_words = 4 + 2 * BBH_MAX_LEVELS + c->n_blocks * 8;
+    o->num_bins = c->num_bins;
+    o->seg_bits = (uint32_t)c->sb;
+    o->mphf_levels = (uint32_t)c->meta.num_levels;
+    o->n_oversized = c->n_over;
+    o->k = (uint32_t)c->k;
+    o->m = (uint32_t)c->m;
+  });
+}
+
+int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec) {
+  return guarded(c, [&]() {
+    if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
+    if (!rec_off) fail(CF1G_E_INVALID, "rec_off is NULL");
+    for (uint64_t r = 0; r < n_rec; r++)
+      if (rec_off[r + 1] < rec_off[r]) fail(CF1G_E_INVALID, "rec_off must be non-decreasing");
+    const uint64_t base = rec_off[0];
+    c->n_rec = n_rec;
+    c->n_ascii = rec_off[n_rec] - base;
+    if (c->n_ascii > 0 && !ascii) fail(CF1G_E_INVALID, "ascii is NULL");
+    std::vector<uint64_t> off(n_rec + 1);
+    c->gstart_h.assign(n_rec + 1, 0);
+    for (uint64_t r = 0; r <= n_rec; r++) {
+      off[r] = rec_off[r] - base;
+      c->gstart_h[r] = off[r] + r + 1;
+    }
+    c->n_pos = c->n_ascii + n_rec + 1;
+    c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
+    uint8_t* da = c->getT<uint8_t>("ascii", c->n_ascii + 8);
+    if (c->n_ascii) hip_check(hipMemcpyAsync(da, ascii + base, c->n_ascii, hipMemcpyHostToDevice, c->st), "H2D ascii");
+    hip_check(hipMemcpyAsync(c->getT<uint64_t>("rec_off", n_rec + 1), off.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice, c->st),
+              "H2D rec_off");
+    hip_check(hipMemcpyAsync(c->getT<uint64_t>("gstart", n_rec + 1), c->gstart_h.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice,
+                             c->st),
+              "H2D gstart");
+    c->sync();
+    c->W = c->S = c->D = 0;
+    c->stage = 1;
+  });
+}
+
+int cf1g_p1_histogram(cf1g_ctx* c, uint64_t* out_hist) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_p1_histogram");
+    do_p1(c);
+    if (out_hist) c->d2h(out_hist, c->get("hist", CF1G_NUM_BUCKETS * 8), CF1G_NUM_BUCKETS * 8);
+    else c->sync();
+  });
+}
+
+int cf1g_p2_partition_route(cf1g_ctx* c, const uint64_t* hist, uint32_t num_bins, uint16_t* out_b2b, uint32_t* out_nb) {
+  return guarded(c, [&]() {
+    need(c, 2, "cf1g_p2_partition_route");
+    do_p2(c, hist, num_bins);
+    if (out_b2b) c->d2h(out_b2b, c->get("b2b", CF1G_NUM_BUCKETS * 2), CF1G_NUM_BUCKETS * 2);
+    else c->sync();
+    if (out_nb) *out_nb = c->num_bins;
+  });
+}
+
+int cf1g_p3_build(cf1g_ctx* c, uint64_t* out_D) {
+  return guarded(c, [&]() {
+    need(c, 3, "cf1g_p3_build");
+    do_p3(c);
+    if (out_D) *out_D = c->D;
+  });
+}
+
+int cf1g_p4_classify(cf1g_ctx* c) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_p4_classify");
+    do_p4(c);
+  });
+}
+
+int cf1g_run_p1_p4(cf1g_ctx* c, uint64_t* out_D) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_run_p1_p4");
+    do_p1(c);
+    do_p2(c, nullptr, c->prm.num_bins);
+    do_p3(c);
+    do_p4(c);
+    if (out_D) *out_D = c->D;
+  });
+}
+
+int cf1g_states_export(cf1g_ctx* c, uint64_t* words, uint64_t n_words) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_states_export");
+    uint64_t nsw = (c->D + 11) / 12;
+    if (!words || n_words < nsw) fail(CF1G_E_INVALID, "state buffer too small");
+    c->d2h(words, c->get("states", (nsw + 1) * 8), nsw * 8);
+  });
+}
+
+int cf1g_keys_export(cf1g_ctx* c, uint64_t* out, uint64_t n_words) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_keys_export");
+    uint64_t need_w = c->D * (c->key_bytes / 8);
+    if (!out || n_words < need_w) fail(CF1G_E_INVALID, "key buffer too small");
+    c->d2h(out, c->get("keys", c->D * c->key_bytes + 16), need_w * 8);
+  });
+}
+
+int cf1g_mphf_export(cf1g_ctx* c, uint64_t* buf, uint64_t* n_words) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_mphf_export");
+    if (!n_words) fail(CF1G_E_INVALID, "n_words is NULL");
+    uint64_t hdr = 4 + 2 * BBH_MAX_LEVELS;
+    uint64_t total = hdr + c->n_blocks * 8;
+    if (!buf) {
+      *n_words = total;
+      return;
+    }
+    if (*n_words < total) fail(CF1G_E_INVALID, "MPHF buffer too small");
+    buf[0] = (uint64_t)c->meta.num_levels;
+    buf[1] = c->meta.n_keys;
+    buf[2] = (uint64_t)c->meta.wide;
+    buf[3] = c->meta.level_block_off[c->meta.num_levels];
+    for (int l = 0; l < BBH_MAX_LEVELS; l++) {
+      buf[4 + l] = c->meta.level_bits[l];
+      buf[4 + BBH_MAX_LEVELS + l] = c->meta.level_block_off[l];
+    }
+    if (c->n_blocks) c->d2h(buf + hdr, c->get("blocks", c->n_blocks * 64 + 64), c->n_blocks * 64);
+    *n_words = total;
+  });
+}
+
+int cf1g_mphf_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* out) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_mphf_lookup");
+    if (n == 0) return;
+    if (!keys || !out) fail(CF1G_E_INVALID, "NULL buffer");
+    size_t kb = c->key_bytes;
+    void* dk = c->get("lookup_keys", n * kb);
+    uint64_t* dout = c->getT<uint64_t>("lookup_out", n);
+    hip_check(hipMemcpyAsync(dk, keys, n * kb, hipMemcpyHostToDevice, c->st), "H2D keys");
+    launch_lookup(c->st, c->wide, dk, n, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
+                  c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), dout);
+    c->d2h(out, dout, n * 8);
+  });
+}
+
+int cf1g_mphf_lookup_device(cf1g_ctx* c, const void* d_keys, uint64_t n, uint64_t* d_out) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_mphf_lookup_device");
+    launch_lookup(c->st, c->wide, d_keys, n, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
+                  c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), d_out);
+    c->sync();
+  });
+}
+
+int cf1g_annotate(cf1g_ctx* c, uint8_t* out, uint64_t n) {
+  return guarded(c, [&]() {
+    need(c, 5, "cf1g_annotate");
+    if (!out || n < c->n_pos) fail(CF1G_E_INVALID, "annotation buffer too small");
+    uint8_t* d = c->getT<uint8_t>("annot", c->n_pos + 8);
+    uint64_t nsw = (c->D + 11) / 12;
+    launch_annotate(c->st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
+                    c->n_pos, c->k, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
+                    c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), c->getT<uint64_t>("states", nsw + 1), d);
+    c->d2h(out, d, c->n_pos);
+  });
+}
+
+int cf1g_record_starts(cf1g_ctx* c, uint64_t* out, uint64_t n) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_record_starts");
+    if (!out || n < c->n_rec + 1) fail(CF1G_E_INVALID, "buffer too small");
+    std::memcpy(out, c->gstart_h.data(), (c->n_rec + 1) * 8);
+  });
+}
+
+int cf1g_bins_export(cf1g_ctx* c, uint32_t* rec_words, uint64_t* bin_start, uint64_t* records, uint64_t n_words) {
+  return guarded(c, [&]() {
+    need(c, 3, "cf1g_bins_export");
+    if (rec_words) *rec_words = (uint32_t)c->rw;
+    if (bin_start) c->d2h(bin_start, c->get("bin_start", (CF1G_MAX_BINS + 1) * 8), (c->num_bins + 1) * 8);
+    if (records) {
+      uint64_t nw = c->S * (uint64_t)c->rw;
+      if (n_words < nw) fail(CF1G_E_INVALID, "record buffer too small");
+      c->d2h(records, c->get("rec", nw * 8 + 8), nw * 8);
+    }
+  });
+}
+
+int cf1g_sync(cf1g_ctx* c) {
+  return guarded(c, [&]() { c->sync(); });
+}
+
+int cf1g_set_profiling(cf1g_ctx* c, int on) {
+  return guarded(c, [&]() {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_on = on != 0;
+  });
+}
+
+size_t cf1g_kernel_times(cf1g_ctx* c, char* buf, size_t cap, int reset) {
+  if (!c) return 0;
+  std::string txt;
+  try {
+    c->sync();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    std::map<std::string, std::pair<uint64_t, double>> agg;
+    std::vector<std::string> order;
+    for (auto& r : g_prof) {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+      if (!agg.count(r.name)) order.push_back(r.name);
+      agg[r.name].first += 1;
+      agg[r.name].second += ms;
+    }
+    for (auto& nme : order) {
+      char line[256];
+      std::snprintf(line, sizeof line, "%s %llu %.6f\n", nme.c_str(), (unsigned long long)agg[nme].first, agg[nme].second);
+      txt += line;
+    }
+    if (reset) {
+      for (auto& r : g_prof) {
+        g_ev_pool.push_back(r.a);
+        g_ev_pool.push_back(r.b);
+      }
+      g_prof.clear();
+    }
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return 0;
+  }
+  if (buf && cap > 0) {
+    size_t n = std::min(cap - 1, txt.size());
+    std::memcpy(buf, txt.data(), n);
+    buf[n] = 0;
+  }
+  return txt.size() + 1;
+}
+
+const void* cf1g_device_keys(cf1g_ctx* c) {
+  if (!c || c->stage < 4) return nullptr;
+  return c->get("keys", c->D * c->key_bytes + 16);
+}
+
+}  // extern "C"

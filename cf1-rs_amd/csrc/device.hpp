@@ -1,0 +1,258 @@
+// Device-side primitives shared by the cf1g kernels (gfx950 / CDNA4).
+//
+// Sequence layout in HBM ("flat stream" G): one separator, then every record
+// followed by one separator, padded with separators:
+//     G = N r0 N r1 N ... r_{n-1} N N...
+// Bases are 2-bit (A0 C1 G2 T3), 32 per u64 word, MSB-first (base g sits at
+// bits 63-2*(g%32)..62-2*(g%32) of word g/32). A parallel bit array marks
+// placeholders (non-ACGT input and separators), 64 per word, MSB-first.
+// A k-mer at g is valid iff its K mask bits are clear; record ends behave
+// exactly like an N (classify.rs:76-77, output.rs:517-518).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cf1g {
+
+struct Key128 {
+  uint64_t lo, hi;
+};
+
+__host__ __device__ __forceinline__ bool key_eq(uint64_t a, uint64_t b) { return a == b; }
+__host__ __device__ __forceinline__ bool key_eq(const Key128& a, const Key128& b) { return a.lo == b.lo && a.hi == b.hi; }
+__host__ __device__ __forceinline__ bool key_lt(uint64_t a, uint64_t b) { return a < b; }
+__host__ __device__ __forceinline__ bool key_lt(const Key128& a, const Key128& b) {
+  return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+}
+__host__ __device__ __forceinline__ uint64_t key_lo(uint64_t a) { return a; }
+__host__ __device__ __forceinline__ uint64_t key_hi(uint64_t) { return 0; }
+__host__ __device__ __forceinline__ uint64_t key_lo(const Key128& a) { return a.lo; }
+__host__ __device__ __forceinline__ uint64_t key_hi(const Key128& a) { return a.hi; }
+
+// ------------------------------------------------------------ stream access
+// 64 bits of the MSB-first bit stream starting at bit position `bit`.
+__device__ __forceinline__ uint64_t stream_bits64(const uint64_t* __restrict__ w, uint64_t bit) {
+  uint64_t wi = bit >> 6;
+  uint32_t sh = (uint32_t)(bit & 63);
+  uint64_t a = w[wi];
+  uint64_t b = w[wi + 1];
+  return (a << sh) | ((b >> (63 - sh)) >> 1);
+}
+// Forward k-mer (K <= 32) whose first base is at g: first base in the high bits.
+__device__ __forceinline__ uint64_t kmer64_at(const uint64_t* __restrict__ seq, uint64_t g, int K) {
+  return stream_bits64(seq, 2 * g) >> (64 - 2 * K);
+}
+// Forward k-mer for 32 < K <= 63 as (hi, lo) of the 2K-bit value.
+__device__ __forceinline__ Key128 kmer128_at(const uint64_t* __restrict__ seq, uint64_t g, int K) {
+  Key128 r;
+  r.lo = stream_bits64(seq, 2 * g + 2 * (uint64_t)K - 64);
+  r.hi = stream_bits64(seq, 2 * g) >> (128 - 2 * K);
+  return r;
+}
+__device__ __forceinline__ uint32_t base_at(const uint64_t* __restrict__ seq, uint64_t g) {
+  return (uint32_t)((seq[g >> 5] >> (62 - 2 * (g & 31))) & 3);
+}
+__device__ __forceinline__ bool is_n(const uint64_t* __restrict__ nmask, uint64_t g) {
+  return (nmask[g >> 6] >> (63 - (g & 63))) & 1;
+}
+// True iff bases [g, g+K) hold no placeholder (1 <= K <= 63).
+__device__ __forceinline__ bool window_clear(const uint64_t* __restrict__ nmask, uint64_t g, int K) {
+  return (stream_bits64(nmask, g) >> (64 - K)) == 0;
+}
+
+// ------------------------------------------------------------ reverse complement
+__device__ __forceinline__ uint64_t swap_pairs(uint64_t x) {
+  return ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+}
+// Reverse complement of a 2K-bit k-mer (K <= 32): complement = XOR 3 per
+// base; reversing bit order then swapping each bit pair reverses the bases.
+__device__ __forceinline__ uint64_t rc64(uint64_t x, int K) {
+  return swap_pairs(__brevll(~x)) >> (64 - 2 * K);
+}
+__device__ __forceinline__ Key128 rc128(const Key128& x, int K) {
+  uint64_t nh = swap_pairs(__brevll(~x.lo));  // reversed value, high word
+  uint64_t nl = swap_pairs(__brevll(~x.hi));
+  uint32_t s = 128 - 2 * K;  // 2..62 for K in [33,63]
+  Key128 r;
+  r.lo = (nl >> s) | (nh << (64 - s));
+  r.hi = nh >> s;
+  return r;
+}
+__device__ __forceinline__ uint64_t kmer_min(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ Key128 kmer_min(const Key128& a, const Key128& b) { return key_lt(b, a) ? b : a; }
+
+// Top `sb` bits of a 2K-bit key (sb <= 2K, sb <= 32).
+__device__ __forceinline__ uint32_t key_top(uint64_t k, int K2, int sb) {
+  return sb == 0 ? 0u : (uint32_t)(k >> (K2 - sb));
+}
+__device__ __forceinline__ uint32_t key_top(const Key128& k, int K2, int sb) {
+  if (sb == 0) return 0u;
+  int s = K2 - sb;  // shift of the 128-bit value
+  if (s >= 64) return (uint32_t)(k.hi >> (s - 64));
+  return (uint32_t)((k.hi << (64 - s)) | (k.lo >> s));
+}
+// 8-bit radix digit at bit `sh` (sh multiple of 8).
+__device__ __forceinline__ uint32_t key_digit(uint64_t k, int sh) { return (uint32_t)(k >> sh) & 255u; }
+__device__ __forceinline__ uint32_t key_digit(const Key128& k, int sh) {
+  return (uint32_t)(sh >= 64 ? (k.hi >> (sh - 64)) : (k.lo >> sh)) & 255u;
+}
+
+// ------------------------------------------------------------ hashes
+// Minimizer order: fmix32 (a bijection on u32) of the canonical m-mer.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ uint32_t mmer_order(uint32_t canon) { return fmix32(canon + 0x9E3779B9u); }
+
+// xxh3_64 (seed 0) of the 8 little-endian bytes of v: the XXH3 4..8-byte
+// path (bitflip of default-secret bytes 8..23, rrmxmx finalizer).
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+__device__ __forceinline__ uint64_t xxh3_u64le(uint64_t v) {
+  uint64_t in64 = (v >> 32) + (v << 32);
+  uint64_t h = in64 ^ (0x1cad21f72c81017cULL ^ 0xdb979083e96dd4deULL);
+  h ^= rotl64(h, 49) ^ rotl64(h, 24);
+  h *= 0x9FB21C651E98DF25ULL;
+  h ^= (h >> 35) + 8;
+  h *= 0x9FB21C651E98DF25ULL;
+  return h ^ (h >> 28);
+}
+__device__ __forceinline__ uint32_t bucket_hash(uint64_t v) { return (uint32_t)(xxh3_u64le(v) >> 48); }
+
+// BBHash level hash (splitmix64 finalizer; per-level seeds from a Weyl sequence).
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t bbh_seed(int level) {
+  return mix64(0x243F6A8885A308D3ULL + (uint64_t)(level + 1) * 0x9E3779B97F4A7C15ULL);
+}
+__device__ __forceinline__ uint64_t bbh_hash(uint64_t k, uint64_t seed) { return mix64(k ^ seed); }
+__device__ __forceinline__ uint64_t bbh_hash(const Key128& k, uint64_t seed) { return mix64(k.lo ^ mix64(k.hi ^ seed)); }
+
+constexpr int BBH_MAX_LEVELS = 64;
+constexpr uint64_t BBH_BLOCK_BITS = 448;  // 7 words of bits + 1 rank word = one 64-byte line
+
+struct BbhMeta {
+  int32_t num_levels;
+  int32_t wide;
+  uint64_t n_keys;
+  uint64_t level_bits[BBH_MAX_LEVELS];
+  uint64_t level_block_off[BBH_MAX_LEVELS + 1];
+};
+
+// Exact rank of a present key; 0 for a key that misses every level
+// (the reference maps absent keys to 0, mphf.rs:269).
+template <typename KeyT>
+__device__ __forceinline__ uint64_t bbh_lookup(const BbhMeta* __restrict__ meta, const uint64_t* __restrict__ blocks, const KeyT& key) {
+  int L = meta->num_levels;
+  for (int l = 0; l < L; l++) {
+    uint64_t nb = meta->level_bits[l];
+    uint64_t pos = __umul64hi(bbh_hash(key, bbh_seed(l)), nb);
+    uint64_t q = pos / BBH_BLOCK_BITS;
+    uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
+    const uint64_t* B = blocks + (meta->level_block_off[l] + q) * 8;
+    uint32_t wi = b >> 6;
+    uint64_t wd = B[1 + wi];
+    if ((wd >> (b & 63)) & 1) {
+      uint64_t r = B[0] + (uint64_t)__popcll(wd & ((1ULL << (b & 63)) - 1));
+      for (uint32_t t = 0; t < wi; t++) r += (uint64_t)__popcll(B[1 + t]);
+      return r;
+    }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ DFA (state.rs / classify.rs)
+// 5-bit codes (state.rs:92-113): 0 unvisited, 3 MIMO, 4+b MISO, 8+f SIMO,
+// 12..15 outputted, 16+4f+b SISO.
+enum : uint32_t { CLS_SISO = 0, CLS_MISO = 1, CLS_SIMO = 2, CLS_MIMO = 3 };
+__host__ __device__ __forceinline__ uint32_t st_class(uint32_t s) {  // STATE_CLASS_TABLE state.rs:249-270
+  if (s == 3 || s == 15) return CLS_MIMO;
+  if ((s >= 4 && s <= 7) || s == 13) return CLS_MISO;
+  if ((s >= 8 && s <= 11) || s == 14) return CLS_SIMO;
+  return CLS_SISO;
+}
+__host__ __device__ __forceinline__ uint32_t st_front(uint32_t s) {  // DECODE_TABLE front
+  if (s >= 8 && s <= 11) return s - 8;
+  if (s >= 16) return (s >> 2) & 3;
+  return 0;
+}
+__host__ __device__ __forceinline__ uint32_t st_back(uint32_t s) {
+  if (s >= 4 && s <= 7) return s - 4;
+  if (s >= 16) return s & 3;
+  return 0;
+}
+__host__ __device__ __forceinline__ bool st_dead_end(uint32_t s) { return s != 0 && st_class(s) == CLS_MIMO; }
+// from_vertex for a visited, not-outputted vertex of class c.
+__host__ __device__ __forceinline__ uint32_t st_encode(uint32_t c, uint32_t f, uint32_t b) {
+  switch (c) {
+    case CLS_SISO: return 16u | (f << 2) | b;
+    case CLS_MISO: return 4u | b;
+    case CLS_SIMO: return 8u | f;
+    default: return 3u;
+  }
+}
+
+// One observation of a k-mer occurrence (the classify.rs:147-518 cases).
+enum : uint32_t { OBS_MIMO = 0, OBS_LEFT = 1, OBS_RIGHT = 2, OBS_INTERNAL = 3 };
+// New state for old state s under an observation; returns s when nothing
+// changes. Restates process_isolated/loop/leftmost/rightmost/internal.
+__host__ __device__ __forceinline__ uint32_t dfa_next(uint32_t s, uint32_t type, bool fwd, uint32_t pb, uint32_t nb) {
+  if (st_dead_end(s)) return s;
+  if (type == OBS_MIMO) return 3u;
+  uint32_t cls = st_class(s), f = st_front(s), b = st_back(s);
+  bool vis = s != 0;
+  if (type == OBS_LEFT) {  // classify.rs:199-278
+    if (fwd) {
+      if (!vis) return st_encode(CLS_MISO, 0, nb);
+      if (cls == CLS_SISO) return b == nb ? st_encode(CLS_MISO, f, b) : 3u;
+      if (cls == CLS_MISO) return b != nb ? 3u : s;
+      return 3u;
+    }
+    uint32_t cn = 3 - nb;
+    if (!vis) return st_encode(CLS_SIMO, cn, 0);
+    if (cls == CLS_SISO) return f == cn ? st_encode(CLS_SIMO, f, b) : 3u;
+    if (cls == CLS_MISO) return 3u;
+    return f != cn ? 3u : s;
+  }
+  if (type == OBS_RIGHT) {  // classify.rs:297-376
+    if (fwd) {
+      if (!vis) return st_encode(CLS_SIMO, pb, 0);
+      if (cls == CLS_SISO) return f == pb ? st_encode(CLS_SIMO, f, b) : 3u;
+      if (cls == CLS_MISO) return 3u;
+      return f != pb ? 3u : s;
+    }
+    uint32_t cp = 3 - pb;
+    if (!vis) return st_encode(CLS_MISO, 0, cp);
+    if (cls == CLS_SISO) return b == cp ? st_encode(CLS_MISO, f, b) : 3u;
+    if (cls == CLS_MISO) return b != cp ? 3u : s;
+    return 3u;
+  }
+  // OBS_INTERNAL, classify.rs:406-511: in-label a, out-label c in canonical orientation.
+  uint32_t a = fwd ? pb : 3 - nb;
+  uint32_t c = fwd ? nb : 3 - pb;
+  if (!vis) return st_encode(CLS_SISO, a, c);
+  if (cls == CLS_SISO) {
+    if (f == a && b == c) return s;
+    if (f != a && b != c) return 3u;
+    if (f != a) return st_encode(CLS_MISO, f, b);
+    return st_encode(CLS_SIMO, f, b);
+  }
+  if (cls == CLS_MISO) return b != c ? 3u : s;
+  return f != a ? 3u : s;
+}
+
+// ------------------------------------------------------------ wave helpers (wave64)
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  uint32_t l = __lane_id();
+  return l == 0 ? 0ULL : (~0ULL >> (64 - l));
+}
+
+}  // namespace cf1g

@@ -1,0 +1,156 @@
+/*
+ * cf1g — MI355X (gfx950) phase API for the cf1-rs P1..P4 hot path.
+ *
+ * A drop-in for the phase calls of cf1-rs run_pipeline (src/pipeline.rs:66-118):
+ * each entry point replaces one reference function, cited beside it. Plain C
+ * types only; no exceptions or panics cross this boundary: every call returns
+ * CF1G_OK or a negative error code, and cf1g_last_error(ctx) describes it.
+ *
+ * One context owns one HIP stream and all device buffers (sequence, bins,
+ * keys, MPHF, states). It is not re-entrant; phases must run in order
+ * (load -> P1 -> P2 -> P3 -> P4) and a call out of order returns
+ * CF1G_E_ORDER. Every call returns after its GPU work has completed.
+ *
+ * k-mer keys cross the boundary as u64 for k <= 32 and as (lo, hi) u64 pairs
+ * (the little-endian u128 layout of the reference's Kmer storage,
+ * src/kmer.rs:407-460) for 33 <= k <= 63.
+ */
+#ifndef CF1G_H
+#define CF1G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF1G_OK 0
+#define CF1G_E_INVALID (-1)   /* bad argument (e.g. even k: src/params.rs:167-170) */
+#define CF1G_E_ORDER (-2)     /* phase called before its inputs exist */
+#define CF1G_E_HIP (-3)       /* HIP runtime error, including device OOM */
+#define CF1G_E_NODEVICE (-4)  /* no usable gfx950 device */
+#define CF1G_E_INTERNAL (-5)  /* invariant violated (reported, never silent) */
+
+#define CF1G_NUM_BUCKETS 65536 /* NUM_HISTOGRAM_BUCKETS, src/minimizer.rs:8 */
+#define CF1G_MAX_BINS 4096
+
+typedef struct cf1g_ctx cf1g_ctx;
+
+typedef struct cf1g_params {
+  uint32_t k;              /* odd, 1..63 (src/params.rs:167-170) */
+  uint32_t num_bins;       /* --num-bins (src/params.rs:58-60); 1..4096 */
+  uint32_t m;              /* minimizer length; 0 = min(k/2,15) made odd (src/params.rs:221-228) */
+  uint32_t p4_chunk;       /* P4 work-stealing chunk (k-mer positions); 0 = tuned */
+  uint32_t p4_wgs_per_cu;  /* P4 persistent work-groups per CU; 0 = tuned */
+  uint32_t p3_target_seg;  /* P3 target k-mers per sort segment; 0 = tuned */
+  uint32_t reserved[2];
+} cf1g_params;
+
+typedef struct cf1g_info {
+  uint64_t n_records;      /* records loaded */
+  uint64_t stream_len;     /* flat stream length (bases + separators) */
+  uint64_t n_kmers;        /* W: valid k-mer occurrences */
+  uint64_t n_superkmers;   /* S: super k-mers routed */
+  uint64_t n_distinct;     /* D: distinct canonical k-mers */
+  uint64_t n_state_words;  /* ceil(D/12) */
+  uint64_t mphf_words;     /* size of cf1g_mphf_export in u64 words */
+  uint32_t num_bins;       /* bins after partitioning (<= requested) */
+  uint32_t seg_bits;       /* P3 segment bits per bin */
+  uint32_t mphf_levels;
+  uint32_t n_oversized;    /* P3 segments sorted through global memory */
+  uint32_t k, m;
+} cf1g_info;
+
+/* Create / destroy a context (device 0 of the current process). */
+int cf1g_create(const cf1g_params* params, cf1g_ctx** out);
+void cf1g_destroy(cf1g_ctx* ctx);
+const char* cf1g_last_error(const cf1g_ctx* ctx);
+int cf1g_get_info(cf1g_ctx* ctx, cf1g_info* out);
+
+/* Input records (the FASTA records P1/P2/P4 re-read: src/minimizer.rs:44-89,
+ * src/superkmer.rs:44-91, src/classify.rs:540-597). ascii[rec_off[r] ..
+ * rec_off[r+1]) is record r; any byte outside ACGTacgt is a placeholder
+ * (src/dna.rs:81-92). Host buffers are copied to HBM. */
+int cf1g_load_records(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec);
+
+/* P1 — count_minimizer_histogram (src/minimizer.rs:27-98): one count per super
+ * k-mer at bucket_hash(minimizer) (src/minimizer.rs:18-21). out_hist may be
+ * NULL; else it receives CF1G_NUM_BUCKETS counts. */
+int cf1g_p1_histogram(cf1g_ctx* ctx, uint64_t* out_hist);
+
+/* P2 — partition_minimizers (src/minimizer.rs:116-144) + route_superkmers
+ * (src/superkmer.rs:21-101). hist == NULL uses P1's device histogram;
+ * otherwise the caller's CF1G_NUM_BUCKETS counts are partitioned. Optional
+ * outputs: the bucket->bin table (CF1G_NUM_BUCKETS u16, Partitioning,
+ * src/minimizer.rs:262-293) and the resulting bin count. */
+int cf1g_p2_partition_route(cf1g_ctx* ctx, const uint64_t* hist, uint32_t num_bins, uint16_t* out_bucket_to_bin,
+                            uint32_t* out_num_bins);
+
+/* P3 — Mphf::<K>::build (src/mphf.rs:32-43): per-bin dedup (dedup_bin,
+ * src/mphf.rs:323-363) then the BBHash over all D keys; also allocates the
+ * zeroed state vector (AtomicStateVector::new, src/state_vector.rs:17-24).
+ * out_D (may be NULL) = Mphf::total_kmers (src/mphf.rs:278-280). */
+int cf1g_p3_build(cf1g_ctx* ctx, uint64_t* out_D);
+
+/* P4 — classify_vertices::<K> (src/classify.rs:524-601). */
+int cf1g_p4_classify(cf1g_ctx* ctx);
+
+/* P1..P4 in order on the loaded records; out_D may be NULL. */
+int cf1g_run_p1_p4(cf1g_ctx* ctx, uint64_t* out_D);
+
+/* ---- products consumed by the host P5 */
+
+/* AtomicStateVector words (12 x 5-bit states per u64, src/state_vector.rs:7-42). */
+int cf1g_states_export(cf1g_ctx* ctx, uint64_t* words, uint64_t n_words);
+/* The distinct canonical keys in bin order, each bin sorted (the order of
+ * cf1_dedup_all.bin, src/mphf.rs:120-132): D u64, or D (lo, hi) pairs. */
+int cf1g_keys_export(cf1g_ctx* ctx, uint64_t* out, uint64_t n_words);
+/* Serialised BBHash: header (4 + 2*64 words: levels, D, wide, blocks, level
+ * bits[64], level block offsets[64]) then 8-word blocks. With buf == NULL only
+ * *n_words is set. The host lookup over this buffer equals Mphf::hash
+ * (src/mphf.rs:268-270) on the device. */
+int cf1g_mphf_export(cf1g_ctx* ctx, uint64_t* buf, uint64_t* n_words);
+/* Batch Mphf::hash (src/mphf.rs:268-270): keys in host memory (u64 or pairs). */
+int cf1g_mphf_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
+/* Same on device-resident keys and output (n keys). */
+int cf1g_mphf_lookup_device(cf1g_ctx* ctx, const void* d_keys, uint64_t n, uint64_t* d_out);
+/* Per stream position (cf1g_info.stream_len bytes): bit0 valid k-mer,
+ * bit1 forward (DirectedKmer::dir, src/directed_kmer.rs:26-36), bits 2-3 state
+ * class (StateClass, src/state.rs:6-11) — what AnnotatedKmer::new computes per
+ * k-mer for P5 (src/directed_kmer.rs:81-96). */
+int cf1g_annotate(cf1g_ctx* ctx, uint8_t* out, uint64_t n);
+/* Stream position of record r's first base (n_rec + 1 entries; the last is the
+ * stream length). */
+int cf1g_record_starts(cf1g_ctx* ctx, uint64_t* out, uint64_t n);
+/* Routed bin records (P2 product, fixed-width form of the bin record format
+ * of src/superkmer.rs:181-202): words per record (2 or 4) via *rec_words,
+ * bin boundaries (num_bins + 1 record offsets) and the records. Any output
+ * may be NULL. */
+int cf1g_bins_export(cf1g_ctx* ctx, uint32_t* rec_words, uint64_t* bin_start, uint64_t* records, uint64_t n_words);
+
+/* ---- host P5 (stays on the host, per the reference's phase split)
+ * extract_and_output + write_json (src/output.rs:152-354, 785-834): writes
+ * prefix.cf_seg / .cf_seq / .json for the records given to cf1g_load_records
+ * (same ascii/rec_off), using the GPU's per-position annotation, states and
+ * MPHF. names_blob[name_off[r] .. name_off[r+1]) is record r's id (may be
+ * NULL). out_meta5 (may be NULL): unitigs, vertices, max, min, sum length. */
+int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, const char* names_blob,
+                  const uint64_t* name_off, const char* out_prefix, const char* input_desc, int poly_n_stretch,
+                  int track_short_seqs, int threads, uint64_t* out_meta5);
+
+/* ---- measurement */
+int cf1g_sync(cf1g_ctx* ctx);
+/* Enable per-kernel HIP-event timing on the context stream (0/1). */
+int cf1g_set_profiling(cf1g_ctx* ctx, int on);
+/* Kernel timing summary as text lines "name count total_ms" ('\n'-separated);
+ * returns the needed buffer size (including the NUL). Resets when reset != 0. */
+size_t cf1g_kernel_times(cf1g_ctx* ctx, char* buf, size_t cap, int reset);
+/* Device-resident copies for benchmarks that time from HBM-resident input:
+ * the device pointers of the loaded ASCII buffer and of the keys array. */
+const void* cf1g_device_keys(cf1g_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CF1G_H */

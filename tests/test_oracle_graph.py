@@ -1,0 +1,118 @@
+"""Oracle P3..P5 against the brute-force graph oracle (oracle/brute.py) and
+closed forms: dedup = sorted set of canonical k-mers per bin; BBHash is a
+bijection onto [0, D); per-k-mer DFA states equal the observation-set join;
+unitigs are the maximal unitigs; tilings spell the records; results do not
+depend on thread count or on CAS failures (the join is order-invariant)."""
+import os
+import random
+import tempfile
+
+import numpy as np
+import pytest
+
+import brute
+from conftest import rand_seq, rc
+from oracle import RefPipeline, state_of
+
+
+def tiny_inputs(seed):
+    r = random.Random(seed)
+    k = r.choice([1, 3, 5, 7, 9, 11, 13, 31, 33])
+    seqs = []
+    for _ in range(r.randint(1, 5)):
+        s = rand_seq(r, r.randint(0, 150), pn=r.choice([0, 0.02, 0.1]), plow=0.05)
+        if r.random() < 0.3 and len(s) > 10:
+            h = s[: len(s) // 2].replace("N", "A")
+            s = h + rc(h)
+        if r.random() < 0.2:
+            s = "A" * r.randint(0, 60) + s
+        seqs.append(s)
+    return k, seqs
+
+
+def label(v, k):
+    return "".join("ACGT"[(v >> (2 * (k - 1 - t))) & 3] for t in range(k))
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_against_brute_force(seed):
+    k, seqs = tiny_inputs(seed)
+    p = RefPipeline(k, num_bins=random.Random(seed).choice([1, 3, 16, 128]), threads=2)
+    names = [f"s{i}" for i in range(len(seqs))]
+    p.load([s.encode() for s in seqs], names)
+    d = p.run_p1_p4()
+    st = brute.states(seqs, k)
+    assert d == len(st)
+    lo, hi = p.keys()
+    # per bin: sorted and unique; globally: exactly the canonical k-mer set
+    off = np.concatenate([[0], np.cumsum(p.per_bin_distinct())]).astype(int)
+    vals = [int(l) | (int(h) << 64) for l, h in zip(lo, hi)]
+    for b in range(len(off) - 1):
+        seg = vals[off[b]:off[b + 1]]
+        assert seg == sorted(set(seg))
+    assert sorted(label(v, k) for v in vals) == sorted(st)
+    idx = p.mphf_lookup(lo, hi if k > 32 else None)
+    assert sorted(idx.tolist()) == list(range(d))
+    sv = state_of(p.states(), idx)
+    for v, s in zip(vals, sv):
+        assert st[label(v, k)] == s
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "o")
+        p.p5(pre)
+        seg = {}
+        for line in open(pre + ".cf_seg"):
+            i, s = line.rstrip("\n").split("\t")
+            seg[int(i)] = s
+        bsegs, btil = brute.unitigs(seqs, k)
+        assert {min(s, brute.rc(s)) for s in seg.values()} == bsegs
+        meta = p.meta()
+        assert meta["unitigs"] == len(bsegs) and meta["vertices"] == d
+        # every tiling spells its record (uppercase, N gaps skipped)
+        lines = open(pre + ".cf_seq").read().splitlines()
+        byname = {ln.split("\t")[0].split("_Sequence:")[1]: ln.split("\t")[1] for ln in lines}
+        for i, s in enumerate(seqs):
+            if len(s) < k:
+                assert f"s{i}" not in byname
+                continue
+            toks = byname[f"s{i}"].split()
+            pos_tokens = [t for t in toks if not t.startswith("N")]
+            bt = btil[i]
+            assert len(pos_tokens) == len(bt)
+            for t, (cu, a, b) in zip(pos_tokens, bt):
+                u = seg[int(t[:-1])]
+                spelled = u if t[-1] == "+" else brute.rc(u)
+                assert spelled == s[a:b + k].upper()
+
+
+def test_thread_and_cas_invariance():
+    r = random.Random(3)
+    seqs = [rand_seq(r, 3000, pn=0.002).encode() for _ in range(6)]
+    seqs.append(b"ACGTT" * 400)
+    res = []
+    for threads, fails in ((1, 0), (8, 0), (8, 5000)):
+        p = RefPipeline(31, num_bins=32, threads=threads)
+        p.load(seqs)
+        p.p1(); p.p2(); p.p3()
+        if fails:
+            p.force_cas_failures(fails)
+        p.p4()
+        if fails:
+            assert p.forced_cas_failures_left() <= 0  # failures happened and were retried
+        res.append((p.keys()[0].copy(), p.mphf_export().copy(), p.states().copy()))
+    for a in res[1:]:
+        for x, y in zip(res[0], a):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_order_invariance_of_records():
+    r = random.Random(4)
+    seqs = [rand_seq(r, 2000, pn=0.003).encode() for _ in range(8)]
+    out = []
+    for order in (seqs, seqs[::-1]):
+        p = RefPipeline(21, num_bins=16, threads=4)
+        p.load(order)
+        p.run_p1_p4()
+        lo, _ = p.keys()
+        idx = p.mphf_lookup(lo)
+        out.append(dict(zip(lo.tolist(), state_of(p.states(), idx).tolist())))
+    assert out[0] == out[1]
