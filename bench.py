@@ -1,0 +1,273 @@
+#!/usr/bin/env python3
+"""Benchmark: distinct k-mers/s over phases P1..P4 on one MI355X.
+
+Workload: the genome-like input (24 records, 300 Mbp, Zipf-like lengths, GC
+41% with drift, 5% interspersed repeat families, poly-N runs; seed 3), k=31.
+A step is one pass of P1..P4 over the records, which are resident in HBM as
+ASCII when the timed region starts (the host->device copy is not timed).
+value = D / seconds-per-step (D = distinct canonical k-mers).
+
+Also reported on the same JSON line:
+  roofline     : the dominant kernel's algorithmic bytes per launch over its
+                 mean launch time (HIP events on the context stream during the
+                 timed steps), against 8 TB/s HBM.
+  cpu_baseline : the multithreaded C++ CPU oracle (the restatement of the
+                 reference's P1..P4) timed on a bounded sample of the same
+                 genome, on this host's cores.
+  lookup       : BBHash lookups/s over the distinct keys.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cf1-rs_amd"))
+
+METRIC = "distinct k-mers/s phases 1-4 at k=31 vs CPU; learned-index vs BBHash lookups/s"
+HBM_PEAK_GBS = 8000.0
+
+PHASE_OF = {
+    "pack": "P1", "p1_minimizer_hist": "P1",
+    "p2_partition": "P2", "p2_route": "P2",
+    "p3_count": "P3a", "p3_scatter": "P3a", "p3_segsort": "P3a", "p3_bigsort": "P3a", "p3_compact": "P3a",
+    "scan_tile": "scan", "scan_add": "scan",
+    "p3b_bbh_size": "P3b", "p3b_bbh_build": "P3b", "p3b_bbh_big": "P3b",
+    "p4_classify": "P4",
+}
+
+
+def mphf_level_stats(mphf_words):
+    """Keys placed per level (summed over segments) of the exported MPHF and
+    the mean number of levels a lookup of a present key probes."""
+    import numpy as np
+
+    ns = int(mphf_words[5])
+    D = int(mphf_words[6])
+    base = 16 + 65536 // 4
+    key_off = mphf_words[base:base + ns + 1].astype(np.int64)
+    blk_off = mphf_words[base + ns + 1:base + 2 * (ns + 1)].astype(np.int64)
+    blocks = mphf_words[base + 2 * (ns + 1):].reshape(-1, 8)
+    words = blocks[:, 1:].reshape(-1)
+    pc = np.concatenate([[0], np.cumsum(np.bitwise_count(words).astype(np.int64))])
+    t = np.diff(key_off)
+    start = blk_off[:-1] * 7
+    off = np.zeros(ns, np.int64)
+    placed = []
+    left = int(t.sum())
+    while left > 0 and len(placed) < 64:
+        n = ((np.maximum(2 * t, 1) + 63) // 64) * 64
+        live = t > 0
+        a = start + off // 64
+        b = start + (off + n) // 64
+        got = int((pc[np.where(live, b, 0)] - pc[np.where(live, a, 0)]).sum())
+        placed.append(got)
+        left -= got
+        off = off + n
+        t = (9 * t + 19) // 20
+    e_probe = sum((l + 1) * p for l, p in enumerate(placed)) / max(D, 1)
+    return placed, e_probe
+
+
+def algorithmic_bytes(info, kb, rw, placed):
+    """Algorithmic HBM bytes per kernel over one step (random accesses = one
+    64-byte line each)."""
+    NG = info["stream_len"]
+    W, S, D = info["n_kmers"], info["n_superkmers"], info["n_distinct"]
+    seq = NG / 4 + NG / 8  # packed bases + placeholder mask
+    rec = S * rw * 8
+    rem = [D]
+    for p in placed:
+        rem.append(rem[-1] - p)
+    e_probe = sum((l + 1) * p for l, p in enumerate(placed)) / max(D, 1)
+    return {
+        "pack": NG + seq,
+        "p1_minimizer_hist": seq,
+        "p2_route": seq + rec,
+        "p3_count": rec,
+        "p3_scatter": rec + W * kb,
+        "p3_segsort": W * kb + D * kb,
+        "p3_compact": 2 * D * kb,
+        "p3b_bbh_size": D * kb,
+        "p3b_bbh_build": D * kb + info["mphf_words"] * 8,
+        "p4_classify": seq + 64.0 * W * (e_probe + 1.0),
+    }
+
+
+def cpu_baseline(ascii_, rec_off, k, num_bins, sample_bases, threads):
+    """The CPU oracle on the first `sample_bases` bases of the same records."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    from oracle import RefPipeline
+
+    seqs, tot = [], 0
+    for i in range(len(rec_off) - 1):
+        a, b = int(rec_off[i]), int(rec_off[i + 1])
+        take = min(b - a, sample_bases - tot)
+        if take <= 0:
+            break
+        seqs.append(ascii_[a:a + take].tobytes())
+        tot += take
+    p = RefPipeline(k, num_bins=num_bins, threads=threads)
+    p.load(seqs)
+    t = time.perf_counter()
+    d = p.run_p1_p4()
+    dt = time.perf_counter() - t
+    return {"value": d / dt, "unit": "distinct k-mers/s", "cores": threads, "kind": "port",
+            "sample": f"first {tot / 1e6:.0f} Mbp of the same genome ({len(seqs)} records), D={d}, {dt:.2f} s for P1-P4",
+            "phases_s": {k2: round(v, 3) for k2, v in p.times().items() if k2 != "p5"}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--k", type=int, default=31)
+    ap.add_argument("--num-bins", type=int, default=128)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cpu-sample-mbp", type=float, default=60.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lookups", type=int, default=1 << 28)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every run uses one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import cf1g
+    from cf1g import synth
+
+    t = time.perf_counter()
+    ascii_, rec_off, names = synth.genome_300m(scale=args.scale)
+    print(f"[bench] generated {len(ascii_) / 1e6:.0f} Mbp in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    g = cf1g.Cf1g(k=args.k, num_bins=args.num_bins)
+    g.load_records(ascii=ascii_, rec_off=rec_off)
+    for i in range(args.warmup):
+        g.run_p1_p4()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    # per-phase wall times (separate, untimed pass through the phase API)
+    ph = {}
+    t = time.perf_counter(); g.count_minimizer_histogram(want=False); ph["P1"] = time.perf_counter() - t
+    t = time.perf_counter(); g.partition_minimizers_and_route(); ph["P2"] = time.perf_counter() - t
+    t = time.perf_counter(); g.build_mphf(); ph["P3"] = time.perf_counter() - t
+    t = time.perf_counter(); g.classify_vertices(); ph["P4"] = time.perf_counter() - t
+
+    g.set_profiling(True)
+    g.kernel_times(reset=True)
+    g.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        D = g.run_p1_p4()
+    g.sync()
+    t1 = time.perf_counter()
+    kt = g.kernel_times(reset=True)
+    g.set_profiling(False)
+    print(f"[bench] {args.steps} timed steps done", file=sys.stderr, flush=True)
+    ms = (t1 - t0) * 1000.0 / args.steps
+    info = g.info()
+    kb = 16 if args.k > 32 else 8
+    rw = 4 if args.k > 32 else 2
+    placed, e_probe = mphf_level_stats(g.mphf_export())
+    ab = algorithmic_bytes(info, kb, rw, placed)
+    kern = {}
+    for name, (cnt, tot) in kt.items():
+        per_step = cnt / args.steps
+        kern[name] = {"launches_per_step": per_step, "ms_per_step": tot / args.steps, "mean_launch_ms": tot / cnt}
+        if name in ab:
+            byt = ab[name] / per_step  # per launch
+            kern[name]["gbs"] = byt / (tot / cnt * 1e-3) / 1e9
+    dom = max((n for n in kern if n in ab), key=lambda n: kern[n]["ms_per_step"])
+    dk = kern[dom]
+    achieved = dk["gbs"]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": ab[dom] / dk["launches_per_step"], "mean_launch_ms": round(dk["mean_launch_ms"], 4)}
+    phases = {}
+    for name, v in kern.items():
+        phases.setdefault(PHASE_OF.get(name, name), 0.0)
+        phases[PHASE_OF.get(name, name)] += v["ms_per_step"]
+
+    # BBHash lookups over device-resident present keys (uniform sample)
+    lookup = None
+    if args.lookups > 0 and D > 0:
+        import ctypes as C
+        lookup = bbhash_lookup_rate(g, D, args.lookups, kb)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(ascii_, rec_off, args.k, args.num_bins, int(args.cpu_sample_mbp * 1e6), threads)
+
+    out = {
+        "metric": METRIC,
+        "value": D / (ms / 1000.0),
+        "unit": "distinct k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"genome-like {len(ascii_) / 1e6:.0f} Mbp, 24 records, k={args.k}, {args.num_bins} bins, seed 3",
+                   "k": args.k, "bases": int(len(ascii_)), "records": int(len(rec_off) - 1),
+                   "kmer_occurrences": info["n_kmers"], "superkmers": info["n_superkmers"], "distinct_kmers": D,
+                   "num_bins": info["num_bins"], "seg_bits": info["seg_bits"], "mphf_big_segments": info["mphf_big_segments"],
+                   "mphf_mean_levels_probed": round(e_probe, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (D / (ms / 1000.0)) / cpu["value"] if cpu else None,
+        "phase_wall_ms": {k2: round(v * 1000, 3) for k2, v in ph.items()},
+        "phase_kernel_ms": {k2: round(v, 3) for k2, v in phases.items()},
+        "kernels": {k2: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k2, v in kern.items()},
+        "lookup": lookup,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+def bbhash_lookup_rate(g, D, n, kb):
+    """Lookups/s of the BBHash on n keys drawn uniformly from the present keys,
+    all resident in HBM."""
+    import ctypes as C
+
+    import numpy as np
+
+    lo, hi = g.keys()
+    rng = np.random.default_rng(5)
+    idx = rng.integers(0, D, n)
+    if kb == 8:
+        q = lo[idx]
+    else:
+        q = np.empty(2 * n, np.uint64)
+        q[0::2], q[1::2] = lo[idx], hi[idx]
+    L = g.L
+    hip = C.CDLL("libamdhip64.so")
+    dq, dout = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(dq), C.c_size_t(q.nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dout), C.c_size_t(n * 8)) == 0
+    assert hip.hipMemcpy(dq, q.ctypes.data_as(C.c_void_p), C.c_size_t(q.nbytes), 1) == 0
+    g._check(L.cf1g_mphf_lookup_device(g.h, dq, n, dout))  # warm
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        g._check(L.cf1g_mphf_lookup_device(g.h, dq, n, dout))
+    dt = (time.perf_counter() - t) / reps
+    out = np.empty(min(n, 1 << 20), np.uint64)
+    assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(out.nbytes), 2) == 0
+    ok = bool(np.all(out == g.mphf_lookup(lo[idx[:len(out)]], hi[idx[:len(out)]] if kb == 16 else None)))
+    hip.hipFree(dq)
+    hip.hipFree(dout)
+    return {"bbhash_lookups_per_s": n / dt, "n": n, "checked_against_host_batch": ok, "learned_lookups_per_s": None}
+
+
+if __name__ == "__main__":
+    main()

@@ -52,7 +52,7 @@ class CfInfo(C.Structure):
     _fields_ = [("n_records", C.c_uint64), ("stream_len", C.c_uint64), ("n_kmers", C.c_uint64),
                 ("n_superkmers", C.c_uint64), ("n_distinct", C.c_uint64), ("n_state_words", C.c_uint64),
                 ("mphf_words", C.c_uint64), ("num_bins", C.c_uint32), ("seg_bits", C.c_uint32),
-                ("mphf_levels", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32)]
+                ("mphf_big_segments", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32)]
 
 
 def lib():

@@ -35,16 +35,15 @@ void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, cons
 void launch_p3_segsort(hipStream_t, bool, void*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, unsigned int*);
 void launch_p3_bigsort(hipStream_t, bool, void*, void*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*);
 void launch_p3_compact(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, void*);
-void launch_bbh_mark(hipStream_t, bool, const void*, uint64_t, int, uint64_t, uint64_t*, uint64_t*);
-void launch_bbh_filter(hipStream_t, bool, const void*, uint64_t, int, uint64_t, const uint64_t*, const uint64_t*, void*, uint64_t*);
-void launch_bbh_blocks(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t*, uint64_t*);
-void launch_bbh_rank(hipStream_t, uint64_t*, const uint64_t*, uint64_t);
+void launch_bbh_seg(hipStream_t, bool, int, const void*, const uint64_t*, uint64_t, uint64_t*, const uint64_t*, uint64_t*, uint32_t*,
+                    unsigned int*, unsigned int*);
+void launch_bbh_big(hipStream_t, bool, int, const void*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*, const uint64_t*,
+                    uint64_t*, uint64_t*, uint64_t*, unsigned int*);
 // kernels_p4.hip
-void launch_p4_classify(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const BbhMeta*, const uint64_t*,
-                        uint64_t*, unsigned int*, uint32_t, uint32_t, uint32_t);
-void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const BbhMeta*, const uint64_t*,
-                     const uint64_t*, uint8_t*);
-void launch_lookup(hipStream_t, bool, const void*, uint64_t, const BbhMeta*, const uint64_t*, uint64_t*);
+void launch_p4_classify(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, uint64_t*, unsigned int*,
+                        uint32_t, uint32_t, uint32_t);
+void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint8_t*);
+void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*);
 
 // ------------------------------------------------------------------ profiling
 namespace {
@@ -115,9 +114,8 @@ struct cf1g_ctx {
   int sb = 0;
   uint64_t n_segs = 0, n_chunks = 0;
   uint32_t n_over = 0;
-  BbhMeta meta{};
   uint64_t n_blocks = 0;
-  std::vector<std::pair<uint64_t*, uint64_t*>> lvl_ac;  // per level (A, C)
+  uint32_t n_big = 0;
 
   void* get(const std::string& name, size_t bytes) {
     DevBuf& b = bufs[name];
@@ -135,6 +133,17 @@ struct cf1g_ctx {
   template <typename T>
   T* getT(const std::string& name, size_t count) {
     return (T*)get(name, count * sizeof(T));
+  }
+  MphfView view() {
+    MphfView v;
+    v.b2b = getT<uint16_t>("b2b", CF1G_NUM_BUCKETS);
+    v.key_off = getT<uint64_t>("seg_out_off", n_segs + 1);
+    v.blk_off = getT<uint64_t>("bbh_blk_off", n_segs + 1);
+    v.blocks = getT<uint64_t>("blocks", n_blocks * 8 + 8);
+    v.K = k;
+    v.m = m;
+    v.sb = sb;
+    return v;
   }
   void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
   void zero(void* p, size_t bytes) { hip_check(hipMemsetAsync(p, 0, bytes, st), "hipMemsetAsync"); }
@@ -239,62 +248,43 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->stage = 3;
 }
 
+// P3b: the minimizer-partitioned BBHash, one segment per work-group (see
+// device.hpp). Size pass, scan of block counts, build pass.
 void bbhash_build(cf1g_ctx* c, const void* keys, uint64_t D) {
   hipStream_t st = c->st;
-  const size_t kb = c->key_bytes;
-  void* surv[2] = {c->get("bbh_surv0", D * kb + 16), c->get("bbh_surv1", D * kb + 16)};
-  uint64_t* cnt = c->getT<uint64_t>("bbh_cnt", 2);
-  const void* cur = keys;
-  uint64_t n = D;
-  uint64_t blk = 0;
-  int l = 0;
-  c->lvl_ac.clear();
-  std::memset(&c->meta, 0, sizeof(c->meta));
-  while (n > 0) {
-    if (l >= BBH_MAX_LEVELS) fail(CF1G_E_INTERNAL, "BBHash exceeded the level limit");
-    uint64_t nb = level_bits(n);
-    uint64_t nw = nb / 64;
-    char an[32], cn[32];
-    std::snprintf(an, sizeof an, "bbhA%d", l);
-    std::snprintf(cn, sizeof cn, "bbhC%d", l);
-    uint64_t* A = c->getT<uint64_t>(an, nw);
-    uint64_t* Cc = c->getT<uint64_t>(cn, nw);
-    c->zero(A, nw * 8);
-    c->zero(Cc, nw * 8);
-    launch_bbh_mark(st, c->wide, cur, n, l, nb, A, Cc);
-    c->zero(cnt, 8);
-    void* nxt = surv[l & 1];
-    launch_bbh_filter(st, c->wide, cur, n, l, nb, A, Cc, nxt, cnt);
-    uint64_t nn = 0;
-    c->d2h(&nn, cnt, 8);
-    c->meta.level_bits[l] = nb;
-    c->meta.level_block_off[l] = blk;
-    blk += nb / BBH_BLOCK_BITS;
-    c->lvl_ac.push_back({A, Cc});
-    if (nn >= n && n > 64) fail(CF1G_E_INTERNAL, "BBHash level placed no key");
-    cur = nxt;
-    n = nn;
-    l++;
+  const uint64_t ns = c->n_segs;
+  const uint64_t* key_off = c->getT<uint64_t>("seg_out_off", ns + 1);
+  uint64_t* nblk = c->getT<uint64_t>("bbh_seg_nblk", ns + 1);
+  uint64_t* blk_off = c->getT<uint64_t>("bbh_blk_off", ns + 1);
+  uint32_t* big = c->getT<uint32_t>("bbh_big", ns + 1);
+  unsigned int* flags = c->getT<unsigned int>("bbh_flags", 4);  // [0] n_big, [1] error
+  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
+  c->zero(flags, 16);
+  launch_bbh_seg(st, c->wide, 0, keys, key_off, ns, nblk, nullptr, nullptr, big, flags, flags + 1);
+  unsigned int hf[2];
+  c->d2h(hf, flags, 8);
+  c->n_big = hf[0];
+  uint64_t* scratch = nullptr;
+  uint64_t* alive = nullptr;
+  uint64_t alive_words = D / 64 + 2;
+  if (c->n_big) {
+    scratch = c->getT<uint64_t>("bbh_scratch", 4 * (D / 64 + 2) + 8 * (ns + 1) + 1024);
+    alive = c->getT<uint64_t>("bbh_alive", alive_words);
+    hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
+    launch_bbh_big(st, c->wide, 0, keys, key_off, big, c->n_big, nblk, nullptr, nullptr, scratch, alive, flags + 1);
   }
-  c->meta.num_levels = l;
-  c->meta.wide = c->wide ? 1 : 0;
-  c->meta.n_keys = D;
-  c->meta.level_block_off[l] = blk;
-  c->n_blocks = blk;
-  uint64_t* blocks = c->getT<uint64_t>("blocks", blk * 8 + 8);
-  uint64_t* popc = c->getT<uint64_t>("bbh_popc", blk + 1);
-  uint64_t* rank = c->getT<uint64_t>("bbh_rank", blk + 2);
-  for (int lv = 0; lv < l; lv++)
-    launch_bbh_blocks(st, c->lvl_ac[lv].first, c->lvl_ac[lv].second, c->meta.level_bits[lv] / BBH_BLOCK_BITS,
-                      c->meta.level_block_off[lv], blocks, popc);
-  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(blk + 1) + 16);
-  scan_excl_u64(st, popc, rank, blk, rank + blk, tmp);
-  launch_bbh_rank(st, blocks, rank, blk);
-  BbhMeta* dm = (BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta));
-  hip_check(hipMemcpyAsync(dm, &c->meta, sizeof(BbhMeta), hipMemcpyHostToDevice, st), "H2D meta");
+  scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
   uint64_t total = 0;
-  c->d2h(&total, rank + blk, 8);
-  if (D > 0 && total != D) fail(CF1G_E_INTERNAL, "BBHash rank total != D");
+  c->d2h(&total, blk_off + ns, 8);
+  c->n_blocks = total;
+  uint64_t* blocks = c->getT<uint64_t>("blocks", total * 8 + 8);
+  launch_bbh_seg(st, c->wide, 1, keys, key_off, ns, nblk, blk_off, blocks, big, flags, flags + 1);
+  if (c->n_big) {
+    hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
+    launch_bbh_big(st, c->wide, 1, keys, key_off, big, c->n_big, nblk, blk_off, blocks, scratch, alive, flags + 1);
+  }
+  c->d2h(hf, flags, 8);
+  if (hf[1]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit");
 }
 
 void do_p3(cf1g_ctx* c) {
@@ -302,9 +292,11 @@ void do_p3(cf1g_ctx* c) {
   const size_t kb = c->key_bytes;
   const int K = c->k;
   // segment bits: mean segment of ~target k-mers, capped by LDS histogram
+  // Segment bits: the fewest (<= 12, <= 2K) that bring the mean segment to
+  // <= target k-mers. The MPHF partition depends on it (it is exported).
   uint64_t target = c->prm.p3_target_seg ? c->prm.p3_target_seg : (c->wide ? 1024 : 2048);
   int sb = 0;
-  while (sb < 12 && sb < 2 * K && (double)c->W / ((double)c->num_bins * (double)(1u << sb)) > (double)target) sb++;
+  while (sb < 12 && sb < 2 * K && c->W > (uint64_t)c->num_bins * (1ULL << sb) * target) sb++;
   c->sb = sb;
   c->n_segs = (uint64_t)c->num_bins << sb;
   const uint64_t ns = c->n_segs;
@@ -363,9 +355,8 @@ void do_p4(cf1g_ctx* c) {
   c->zero(ctr, 4);
   uint64_t nsw = (c->D + 11) / 12;
   launch_p4_classify(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
-                     c->n_pos, c->k, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
-                     c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), c->getT<uint64_t>("states", nsw + 1), ctr, chunk,
-                     (uint32_t)nch, (uint32_t)(grid ? grid : 1));
+                     c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), ctr, chunk, (uint32_t)nch,
+                     (uint32_t)(grid ? grid : 1));
   c->sync();
   c->stage = 5;
 }
@@ -434,10 +425,10 @@ int cf1g_get_info(cf1g_ctx* c, cf1g_info* o) {
     o->n_superkmers = c->S;
     o->n_distinct = c->D;
     o->n_state_words = (c->D + 11) / 12;
-    o->mphf_words = 4 + 2 * BBH_MAX_LEVELS + c->n_blocks * 8;
+    o->mphf_words = c->stage >= 4 ? 16 + CF1G_NUM_BUCKETS / 4 + 2 * (c->n_segs + 1) + c->n_blocks * 8 : 0;
     o->num_bins = c->num_bins;
     o->seg_bits = (uint32_t)c->sb;
-    o->mphf_levels = (uint32_t)c->meta.num_levels;
+    o->mphf_big_segments = c->n_big;
     o->n_oversized = c->n_over;
     o->k = (uint32_t)c->k;
     o->m = (uint32_t)c->m;
@@ -542,22 +533,31 @@ int cf1g_mphf_export(cf1g_ctx* c, uint64_t* buf, uint64_t* n_words) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_mphf_export");
     if (!n_words) fail(CF1G_E_INVALID, "n_words is NULL");
-    uint64_t hdr = 4 + 2 * BBH_MAX_LEVELS;
-    uint64_t total = hdr + c->n_blocks * 8;
+    const uint64_t ns = c->n_segs;
+    const uint64_t hdr = 16, tab = CF1G_NUM_BUCKETS / 4;
+    const uint64_t total = hdr + tab + 2 * (ns + 1) + c->n_blocks * 8;
     if (!buf) {
       *n_words = total;
       return;
     }
     if (*n_words < total) fail(CF1G_E_INVALID, "MPHF buffer too small");
-    buf[0] = (uint64_t)c->meta.num_levels;
-    buf[1] = c->meta.n_keys;
-    buf[2] = (uint64_t)c->meta.wide;
-    buf[3] = c->meta.level_block_off[c->meta.num_levels];
-    for (int l = 0; l < BBH_MAX_LEVELS; l++) {
-      buf[4 + l] = c->meta.level_bits[l];
-      buf[4 + BBH_MAX_LEVELS + l] = c->meta.level_block_off[l];
-    }
-    if (c->n_blocks) c->d2h(buf + hdr, c->get("blocks", c->n_blocks * 64 + 64), c->n_blocks * 64);
+    std::memset(buf, 0, hdr * 8);
+    buf[0] = 0x6366316D70686631ULL;  // "cf1mphf1"
+    buf[1] = (uint64_t)c->k;
+    buf[2] = (uint64_t)c->m;
+    buf[3] = (uint64_t)c->sb;
+    buf[4] = c->num_bins;
+    buf[5] = ns;
+    buf[6] = c->D;
+    buf[7] = c->n_blocks;
+    buf[8] = c->wide ? 1 : 0;
+    MphfView v = c->view();
+    hip_check(hipMemcpyAsync(buf + hdr, v.b2b, CF1G_NUM_BUCKETS * 2, hipMemcpyDeviceToHost, c->st), "D2H");
+    hip_check(hipMemcpyAsync(buf + hdr + tab, v.key_off, (ns + 1) * 8, hipMemcpyDeviceToHost, c->st), "D2H");
+    hip_check(hipMemcpyAsync(buf + hdr + tab + ns + 1, v.blk_off, (ns + 1) * 8, hipMemcpyDeviceToHost, c->st), "D2H");
+    if (c->n_blocks)
+      hip_check(hipMemcpyAsync(buf + hdr + tab + 2 * (ns + 1), v.blocks, c->n_blocks * 64, hipMemcpyDeviceToHost, c->st), "D2H");
+    c->sync();
     *n_words = total;
   });
 }
@@ -571,8 +571,7 @@ int cf1g_mphf_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* ou
     void* dk = c->get("lookup_keys", n * kb);
     uint64_t* dout = c->getT<uint64_t>("lookup_out", n);
     hip_check(hipMemcpyAsync(dk, keys, n * kb, hipMemcpyHostToDevice, c->st), "H2D keys");
-    launch_lookup(c->st, c->wide, dk, n, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
-                  c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), dout);
+    launch_lookup(c->st, c->wide, dk, n, c->view(), dout);
     c->d2h(out, dout, n * 8);
   });
 }
@@ -580,8 +579,7 @@ int cf1g_mphf_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* ou
 int cf1g_mphf_lookup_device(cf1g_ctx* c, const void* d_keys, uint64_t n, uint64_t* d_out) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_mphf_lookup_device");
-    launch_lookup(c->st, c->wide, d_keys, n, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
-                  c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), d_out);
+    launch_lookup(c->st, c->wide, d_keys, n, c->view(), d_out);
     c->sync();
   });
 }
@@ -593,8 +591,7 @@ int cf1g_annotate(cf1g_ctx* c, uint8_t* out, uint64_t n) {
     uint8_t* d = c->getT<uint8_t>("annot", c->n_pos + 8);
     uint64_t nsw = (c->D + 11) / 12;
     launch_annotate(c->st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
-                    c->n_pos, c->k, (const BbhMeta*)c->get("bbh_meta", sizeof(BbhMeta)),
-                    c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), c->getT<uint64_t>("states", nsw + 1), d);
+                    c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), d);
     c->d2h(out, d, c->n_pos);
   });
 }

@@ -138,32 +138,89 @@ __device__ __forceinline__ uint64_t bbh_hash(const Key128& k, uint64_t seed) { r
 constexpr int BBH_MAX_LEVELS = 64;
 constexpr uint64_t BBH_BLOCK_BITS = 448;  // 7 words of bits + 1 rank word = one 64-byte line
 
-struct BbhMeta {
-  int32_t num_levels;
-  int32_t wide;
-  uint64_t n_keys;
-  uint64_t level_bits[BBH_MAX_LEVELS];
-  uint64_t level_block_off[BBH_MAX_LEVELS + 1];
+// ------------------------------------------------------------ MPHF (minimizer-partitioned BBHash)
+// The distinct keys are partitioned exactly as P3a left them: segment
+// (bin, top sb key bits), bin = bucket_to_bin[bucket_hash(minimizer value)].
+// Each segment holds its own BBHash: level l has n_l = roundup64(max(2 t_l, 1))
+// bits with t_0 = D_s (the segment's key count) and t_{l+1} = ceil(9 t_l / 20);
+// a key lands at pos = mulhi64(hash(key, l), n_l) and is placed at the first
+// level where no other remaining key lands on the same bit. Levels are
+// concatenated bit-contiguously in 64-byte blocks {u64 rank_before, u64 bits[7]}
+// (rank local to the segment). value = key_off[segment] + local rank.
+__host__ __device__ __forceinline__ uint64_t bbh_level_n(uint64_t t) {
+  uint64_t n = 2 * t;
+  if (n == 0) n = 1;
+  return (n + 63) & ~63ULL;
+}
+__host__ __device__ __forceinline__ uint64_t bbh_next_t(uint64_t t) { return (9 * t + 19) / 20; }
+
+struct MphfView {
+  const uint16_t* b2b;      // 65536 bucket -> bin
+  const uint64_t* key_off;  // n_segs + 1
+  const uint64_t* blk_off;  // n_segs + 1 (in blocks)
+  const uint64_t* blocks;   // 8 words per block
+  int K, m, sb;
 };
 
-// Exact rank of a present key; 0 for a key that misses every level
-// (the reference maps absent keys to 0, mphf.rs:269).
+// 2K-bit value shifted right by s, low 32 bits (s < 128).
+__device__ __forceinline__ uint32_t key_shr32(uint64_t k, int s) { return (uint32_t)(k >> s); }
+__device__ __forceinline__ uint32_t key_shr32(const Key128& k, int s) {
+  if (s >= 64) return (uint32_t)(k.hi >> (s - 64));
+  if (s == 0) return (uint32_t)k.lo;
+  return (uint32_t)((k.lo >> s) | (k.hi << (64 - s)));
+}
+__device__ __forceinline__ uint64_t rc_any(uint64_t x, int K) { return rc64(x, K); }
+__device__ __forceinline__ Key128 rc_any(const Key128& x, int K) { return rc128(x, K); }
+
+// Minimizer value of a k-mer given in canonical form (the value is the same
+// for both orientations): the canonical m-mer of least order among its w m-mers.
 template <typename KeyT>
-__device__ __forceinline__ uint64_t bbh_lookup(const BbhMeta* __restrict__ meta, const uint64_t* __restrict__ blocks, const KeyT& key) {
-  int L = meta->num_levels;
-  for (int l = 0; l < L; l++) {
-    uint64_t nb = meta->level_bits[l];
-    uint64_t pos = __umul64hi(bbh_hash(key, bbh_seed(l)), nb);
-    uint64_t q = pos / BBH_BLOCK_BITS;
-    uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
-    const uint64_t* B = blocks + (meta->level_block_off[l] + q) * 8;
-    uint32_t wi = b >> 6;
-    uint64_t wd = B[1 + wi];
+__device__ __forceinline__ uint32_t minimizer_value(const KeyT& c, int K, int m) {
+  const KeyT r = rc_any(c, K);
+  const uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
+  const int w = K - m + 1;
+  uint32_t best = 0xFFFFFFFFu, bestv = 0;
+  for (int j = 0; j < w; j++) {
+    uint32_t f = key_shr32(c, 2 * (K - m - j)) & M;
+    uint32_t b = key_shr32(r, 2 * j) & M;
+    uint32_t cv = f < b ? f : b;
+    uint32_t o = mmer_order(cv);
+    if (o < best) {
+      best = o;
+      bestv = cv;
+    }
+  }
+  return bestv;
+}
+template <typename KeyT>
+__device__ __forceinline__ uint64_t mphf_segment(const MphfView& v, const KeyT& c) {
+  uint32_t bin = v.b2b[bucket_hash((uint64_t)minimizer_value(c, v.K, v.m))];
+  return ((uint64_t)bin << v.sb) | key_top(c, 2 * v.K, v.sb);
+}
+// Exact value of a present key; for an absent key, some value (0 if it
+// misses every level — the reference maps absent keys to 0, mphf.rs:269).
+template <typename KeyT>
+__device__ __forceinline__ uint64_t mphf_lookup(const MphfView& v, const KeyT& c) {
+  const uint64_t seg = mphf_segment(v, c);
+  const uint64_t ko = v.key_off[seg];
+  uint64_t t = v.key_off[seg + 1] - ko;
+  const uint64_t* base = v.blocks + v.blk_off[seg] * 8;
+  uint64_t off = 0;
+  for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
+    const uint64_t n = bbh_level_n(t);
+    const uint64_t pos = off + __umul64hi(bbh_hash(c, bbh_seed(l)), n);
+    const uint64_t q = pos / BBH_BLOCK_BITS;
+    const uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
+    const uint64_t* B = base + q * 8;
+    const uint32_t wi = b >> 6;
+    const uint64_t wd = B[1 + wi];
     if ((wd >> (b & 63)) & 1) {
-      uint64_t r = B[0] + (uint64_t)__popcll(wd & ((1ULL << (b & 63)) - 1));
-      for (uint32_t t = 0; t < wi; t++) r += (uint64_t)__popcll(B[1 + t]);
+      uint64_t r = ko + B[0] + (uint64_t)__popcll(wd & ((1ULL << (b & 63)) - 1));
+      for (uint32_t u = 0; u < wi; u++) r += (uint64_t)__popcll(B[1 + u]);
       return r;
     }
+    off += n;
+    t = bbh_next_t(t);
   }
   return 0;
 }

@@ -16,35 +16,85 @@ inline uint64_t h_mix64(uint64_t z) {
 }
 inline uint64_t h_seed(int level) { return h_mix64(0x243F6A8885A308D3ULL + (uint64_t)(level + 1) * 0x9E3779B97F4A7C15ULL); }
 
-// View over a cf1g_mphf_export buffer.
+inline uint32_t h_fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+inline uint64_t h_xxh3_u64le(uint64_t v) {
+  uint64_t in64 = (v >> 32) + (v << 32);
+  uint64_t h = in64 ^ (0x1cad21f72c81017cULL ^ 0xdb979083e96dd4deULL);
+  h ^= ((h << 49) | (h >> 15)) ^ ((h << 24) | (h >> 40));
+  h *= 0x9FB21C651E98DF25ULL;
+  h ^= (h >> 35) + 8;
+  h *= 0x9FB21C651E98DF25ULL;
+  return h ^ (h >> 28);
+}
+inline u128 h_rc(u128 v, int K);
+
+// View over a cf1g_mphf_export buffer (the minimizer-partitioned BBHash of
+// the device, same lookup).
 struct HostMphf {
-  const uint64_t* buf = nullptr;
-  int levels = 0;
+  int K = 0, m = 0, sb = 0;
   bool wide = false;
-  const uint64_t* level_bits = nullptr;
-  const uint64_t* level_off = nullptr;
+  uint64_t n_segs = 0;
+  const uint16_t* b2b = nullptr;
+  const uint64_t* key_off = nullptr;
+  const uint64_t* blk_off = nullptr;
   const uint64_t* blocks = nullptr;
   void attach(const uint64_t* b) {
-    buf = b;
-    levels = (int)b[0];
-    wide = b[2] != 0;
-    level_bits = b + 4;
-    level_off = b + 4 + 64;
-    blocks = b + 4 + 128;
+    K = (int)b[1];
+    m = (int)b[2];
+    sb = (int)b[3];
+    n_segs = b[5];
+    wide = b[8] != 0;
+    b2b = (const uint16_t*)(b + 16);
+    key_off = b + 16 + 65536 / 4;
+    blk_off = key_off + n_segs + 1;
+    blocks = blk_off + n_segs + 1;
   }
-  uint64_t lookup(uint64_t lo, uint64_t hi) const {
-    for (int l = 0; l < levels; l++) {
+  uint64_t segment(u128 c) const {
+    u128 r = h_rc(c, K);
+    uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
+    uint32_t best = 0xFFFFFFFFu, bv = 0;
+    for (int j = 0; j < K - m + 1; j++) {
+      uint32_t f = (uint32_t)(c >> (2 * (K - m - j))) & M;
+      uint32_t b = (uint32_t)(r >> (2 * j)) & M;
+      uint32_t cv = f < b ? f : b;
+      uint32_t o = h_fmix32(cv + 0x9E3779B9u);
+      if (o < best) {
+        best = o;
+        bv = cv;
+      }
+    }
+    uint32_t bin = b2b[h_xxh3_u64le((uint64_t)bv) >> 48];
+    uint32_t sub = sb ? (uint32_t)(c >> (2 * K - sb)) : 0;
+    return ((uint64_t)bin << sb) | sub;
+  }
+  uint64_t lookup(u128 c) const {
+    uint64_t seg = segment(c);
+    uint64_t ko = key_off[seg], t = key_off[seg + 1] - ko;
+    const uint64_t* base = blocks + blk_off[seg] * 8;
+    uint64_t off = 0;
+    uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64);
+    for (int l = 0; l < 64 && t > 0; l++) {
+      uint64_t n = (2 * t + 63) & ~63ULL;
       uint64_t s = h_seed(l);
       uint64_t h = wide ? h_mix64(lo ^ h_mix64(hi ^ s)) : h_mix64(lo ^ s);
-      uint64_t pos = (uint64_t)(((u128)h * level_bits[l]) >> 64);
+      uint64_t pos = off + (uint64_t)(((u128)h * n) >> 64);
       uint64_t q = pos / 448, b = pos % 448;
-      const uint64_t* B = blocks + (level_off[l] + q) * 8;
+      const uint64_t* B = base + q * 8;
       uint64_t wd = B[1 + b / 64];
       if ((wd >> (b % 64)) & 1) {
-        uint64_t r = B[0] + (uint64_t)__builtin_popcountll(wd & ((1ULL << (b % 64)) - 1));
-        for (uint64_t t = 0; t < b / 64; t++) r += (uint64_t)__builtin_popcountll(B[1 + t]);
-        return r;
+        uint64_t rk = ko + B[0] + (uint64_t)__builtin_popcountll(wd & ((1ULL << (b % 64)) - 1));
+        for (uint64_t u = 0; u < b / 64; u++) rk += (uint64_t)__builtin_popcountll(B[1 + u]);
+        return rk;
       }
+      off += n;
+      t = (9 * t + 19) / 20;
     }
     return 0;
   }

@@ -107,7 +107,7 @@ struct P5 {
     u128 rs = h_rc(xs, K), re = h_rc(xe, K);
     u128 cs = xs < rs ? xs : rs, ce = xe < re ? xe : re;
     u128 mf = cs < ce ? cs : ce;
-    uint64_t id = mphf.lookup((uint64_t)mf, (uint64_t)(mf >> 64));
+    uint64_t id = mphf.lookup(mf);
     bool fwd = xs < re;
     us.push_back({id, fwd, st.idx, en.idx});
     uint8_t s = get_state(id);

@@ -581,65 +581,213 @@ __global__ void __launch_bounds__(256) k_p3_compact(const KeyT* __restrict__ occ
 }
 
 // ------------------------------------------------------------------ BBHash build
-template <typename KeyT>
-__global__ void __launch_bounds__(256) k_bbh_mark(const KeyT* __restrict__ keys, uint64_t n, uint64_t seed, uint64_t nbits,
-                                                  unsigned long long* __restrict__ A, unsigned long long* __restrict__ Cc) {
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint64_t pos = __umul64hi(bbh_hash(keys[i], seed), nbits);
-    unsigned long long bit = 1ULL << (pos & 63);
-    unsigned long long old = atomicOr(&A[pos >> 6], bit);
-    if (old & bit) atomicOr(&Cc[pos >> 6], bit);
+// One work-group per P3a segment builds that segment's BBHash (see
+// device.hpp) from its sorted distinct keys: keys live in registers, the
+// level's seen/collision bit arrays in LDS. MODE 0 only measures the
+// segment's block count; after a scan of those counts MODE 1 rebuilds the
+// same levels and writes the blocks with their segment-local rank words.
+// Segments with more keys than fit in registers (> CAP) are handed to
+// k_bbh_big through big_list.
+template <int MODE, typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_bbh_seg(const KeyT* __restrict__ keys, const uint64_t* __restrict__ key_off,
+                                                   uint64_t n_segs, uint64_t* __restrict__ seg_nblk,
+                                                   const uint64_t* __restrict__ blk_off, uint64_t* __restrict__ blocks,
+                                                   uint32_t* __restrict__ big_list, unsigned int* __restrict__ n_big,
+                                                   unsigned int* __restrict__ err) {
+  constexpr int KPT = SegCfg<KeyT>::KPT;
+  constexpr int CAP = RK_WG * KPT;
+  constexpr int SEEN_W = (2 * CAP) / 64 + 1;
+  constexpr int REG_W = (4 * CAP + 8192) / 64;
+  __shared__ uint64_t seen[SEEN_W];
+  __shared__ uint64_t col[SEEN_W];
+  __shared__ uint64_t region[MODE ? REG_W : 1];
+  __shared__ uint64_t sh8[8];
+  const uint64_t seg = blockIdx.x;
+  if (seg >= n_segs) return;
+  const uint64_t ko = key_off[seg];
+  const uint64_t D = key_off[seg + 1] - ko;
+  if (D == 0) {
+    if (MODE == 0 && threadIdx.x == 0) seg_nblk[seg] = 0;
+    return;
   }
-}
-template <typename KeyT>
-__global__ void __launch_bounds__(256) k_bbh_filter(const KeyT* __restrict__ keys, uint64_t n, uint64_t seed, uint64_t nbits,
-                                                    const uint64_t* __restrict__ A, const uint64_t* __restrict__ Cc,
-                                                    KeyT* __restrict__ out, unsigned long long* __restrict__ out_n) {
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t iters = (n + stride - 1) / stride;
-  for (uint64_t it = 0; it < iters; it++) {
-    uint64_t i = i0 + it * stride;
-    bool surv = false;
-    KeyT k{};
-    if (i < n) {
-      k = keys[i];
-      uint64_t pos = __umul64hi(bbh_hash(k, seed), nbits);
-      uint64_t wd = A[pos >> 6] & ~Cc[pos >> 6];
-      surv = !((wd >> (pos & 63)) & 1);
+  if (D > (uint64_t)CAP) {
+    if (MODE == 0 && threadIdx.x == 0) {
+      big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
+      seg_nblk[seg] = 0;
     }
-    uint64_t bal = __ballot(surv);
-    if (bal) {
-      uint32_t cntw = (uint32_t)__popcll(bal);
-      uint32_t leader = (uint32_t)(__ffsll((long long)bal) - 1);
-      unsigned long long base = 0;
-      if (__lane_id() == leader) base = atomicAdd(out_n, (unsigned long long)cntw);
-      base = __shfl(base, (int)leader, 64);
-      if (surv) out[base + (uint64_t)__popcll(bal & lanemask_lt())] = k;
-    }
+    return;
   }
-}
-// Level words A & ~C packed into 64-byte blocks {rank, 7 words}; per-block
-// popcounts go to popc[] for the rank scan.
-__global__ void __launch_bounds__(256) k_bbh_blocks(const uint64_t* __restrict__ A, const uint64_t* __restrict__ Cc,
-                                                    uint64_t nblk, uint64_t blk_off, uint64_t* __restrict__ blocks,
-                                                    uint64_t* __restrict__ popc) {
-  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nblk) return;
-  uint64_t* B = blocks + (blk_off + j) * 8;
-  uint64_t c = 0;
+  KeyT key[KPT];
+  uint32_t alive = 0;
 #pragma unroll
-  for (int t = 0; t < 7; t++) {
-    uint64_t x = A[j * 7 + t] & ~Cc[j * 7 + t];
-    B[1 + t] = x;
-    c += (uint64_t)__popcll(x);
+  for (int s = 0; s < KPT; s++) {
+    uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+    if (e < D) {
+      key[s] = keys[ko + e];
+      alive |= 1u << s;
+    }
   }
-  popc[blk_off + j] = c;
+  if (MODE)
+    for (int i = threadIdx.x; i < REG_W; i += RK_WG) region[i] = 0;
+  uint64_t off = 0, t = D, r = D;
+  int l = 0;
+  while (r > 0) {
+    const uint64_t n = bbh_level_n(t);
+    if (l >= BBH_MAX_LEVELS || off + n > (uint64_t)REG_W * 64) {
+      if (threadIdx.x == 0) atomicOr(err, 1u);
+      break;
+    }
+    const uint32_t nw = (uint32_t)(n >> 6);
+    for (uint32_t i = threadIdx.x; i < nw; i += RK_WG) {
+      seen[i] = 0;
+      col[i] = 0;
+    }
+    __syncthreads();
+    const uint64_t seed = bbh_seed(l);
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      if ((alive >> s) & 1) {
+        uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
+        unsigned long long bit = 1ULL << (pos & 63);
+        unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
+        if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+      }
+    }
+    __syncthreads();
+    if (MODE)
+      for (uint32_t i = threadIdx.x; i < nw; i += RK_WG) region[(off >> 6) + i] = seen[i] & ~col[i];
+    uint64_t cnt = 0;
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      if ((alive >> s) & 1) {
+        uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
+        if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) alive &= ~(1u << s);
+        else cnt++;
+      }
+    }
+    block_excl_scan_u64(cnt, sh8, &r);  // r = keys still unplaced (barriers inside)
+    off += n;
+    t = bbh_next_t(t);
+    l++;
+  }
+  const uint64_t nblk = (off + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;
+  if (MODE == 0) {
+    if (threadIdx.x == 0) seg_nblk[seg] = nblk;
+    return;
+  }
+  __syncthreads();
+  // nblk <= REG_W*64/448 + 1 < 256: one block per thread
+  uint64_t w[7];
+  uint64_t pc = 0;
+  const uint64_t j = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 7; u++) {
+    uint64_t q = j * 7 + u;
+    w[u] = (j < nblk && q < (uint64_t)REG_W) ? region[q] : 0;
+    pc += (uint64_t)__popcll(w[u]);
+  }
+  uint64_t tot;
+  uint64_t rank = block_excl_scan_u64(j < nblk ? pc : 0, sh8, &tot);
+  if (j < nblk) {
+    uint64_t* B = blocks + (blk_off[seg] + j) * 8;
+    B[0] = rank;
+#pragma unroll
+    for (int u = 0; u < 7; u++) B[1 + u] = w[u];
+  }
 }
-__global__ void __launch_bounds__(256) k_bbh_rank(uint64_t* __restrict__ blocks, const uint64_t* __restrict__ rank, uint64_t nblk) {
-  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nblk) blocks[j * 8] = rank[j];
+
+// Segments above the register capacity: the same levels, with seen/collision
+// bits in global scratch (device-scope atomics) and per-key alive bits in a
+// global bitmap. One work-group per big segment.
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int MODE, typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ keys, const uint64_t* __restrict__ key_off,
+                                                   const uint32_t* __restrict__ big_list, uint64_t* __restrict__ seg_nblk,
+                                                   const uint64_t* __restrict__ blk_off, uint64_t* __restrict__ blocks,
+                                                   uint64_t* __restrict__ scratch, uint64_t* __restrict__ alive_g,
+                                                   unsigned int* __restrict__ err) {
+  __shared__ uint64_t sh8[8];
+  __shared__ unsigned long long carry;
+  const uint32_t seg = big_list[blockIdx.x];
+  const uint64_t ko = key_off[seg];
+  const uint64_t D = key_off[seg + 1] - ko;
+  uint64_t* seen = scratch + 4 * (ko / 64) + 8 * (uint64_t)seg;
+  uint64_t* col = seen + (D / 32 + 2);
+  uint64_t* region = blocks + (MODE ? blk_off[seg] * 8 : 0);
+  if (MODE) {
+    uint64_t nwords = (blk_off[seg + 1] - blk_off[seg]) * 8;
+    for (uint64_t i = threadIdx.x; i < nwords; i += RK_WG) region[i] = 0;
+  }
+  uint64_t off = 0, t = D, r = D;
+  int l = 0;
+  while (r > 0) {
+    const uint64_t n = bbh_level_n(t);
+    if (l >= BBH_MAX_LEVELS) {
+      if (threadIdx.x == 0) atomicOr(err, 2u);
+      break;
+    }
+    const uint64_t nw = n >> 6;
+    for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) {
+      seen[i] = 0;
+      col[i] = 0;
+    }
+    __threadfence();
+    __syncthreads();
+    const uint64_t seed = bbh_seed(l);
+    for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
+      uint64_t gi = ko + e;
+      if (!((ld_agent(&alive_g[gi >> 6]) >> (gi & 63)) & 1)) continue;
+      uint64_t pos = __umul64hi(bbh_hash(keys[gi], seed), n);
+      unsigned long long bit = 1ULL << (pos & 63);
+      unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
+      if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+    }
+    __threadfence();
+    __syncthreads();
+    if (MODE) {
+      for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) {
+        uint64_t q = (off >> 6) + i;
+        region[(q / 7) * 8 + 1 + (q % 7)] = ld_agent(&seen[i]) & ~ld_agent(&col[i]);
+      }
+    }
+    uint64_t cnt = 0;
+    for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
+      uint64_t gi = ko + e;
+      if (!((ld_agent(&alive_g[gi >> 6]) >> (gi & 63)) & 1)) continue;
+      uint64_t pos = __umul64hi(bbh_hash(keys[gi], seed), n);
+      if (((ld_agent(&seen[pos >> 6]) & ~ld_agent(&col[pos >> 6])) >> (pos & 63)) & 1)
+        atomicAnd((unsigned long long*)&alive_g[gi >> 6], ~(1ULL << (gi & 63)));
+      else
+        cnt++;
+    }
+    __threadfence();
+    block_excl_scan_u64(cnt, sh8, &r);
+    off += n;
+    t = bbh_next_t(t);
+    l++;
+  }
+  const uint64_t nblk = (off + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;
+  if (MODE == 0) {
+    if (threadIdx.x == 0) seg_nblk[seg] = nblk;
+    return;
+  }
+  __threadfence();
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint64_t j0 = 0; j0 < nblk; j0 += RK_WG) {
+    uint64_t j = j0 + threadIdx.x;
+    uint64_t pc = 0;
+    if (j < nblk)
+      for (int u = 0; u < 7; u++) pc += (uint64_t)__popcll(ld_agent(&region[j * 8 + 1 + u]));
+    uint64_t tot;
+    uint64_t rk = block_excl_scan_u64(pc, sh8, &tot);
+    if (j < nblk) region[j * 8] = carry + rk;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -700,36 +848,47 @@ void launch_p3_compact(hipStream_t st, bool wide, const void* occ, const uint64_
     CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(256), 0, st, (const Key128*)occ, seg_off,
                 ucount, out_off, n_segs, (Key128*)keys);
 }
-void launch_bbh_mark(hipStream_t st, bool wide, const void* keys, uint64_t n, int level, uint64_t nbits, uint64_t* A, uint64_t* Cc) {
-  if (n == 0) return;
-  unsigned g = grid_for(n, 256, 8192);
-  if (!wide)
-    CF1G_LAUNCH("p3b_bbh_mark", k_bbh_mark<uint64_t>, dim3(g), dim3(256), 0, st, (const uint64_t*)keys, n, bbh_seed(level), nbits,
-                (unsigned long long*)A, (unsigned long long*)Cc);
-  else
-    CF1G_LAUNCH("p3b_bbh_mark", k_bbh_mark<Key128>, dim3(g), dim3(256), 0, st, (const Key128*)keys, n, bbh_seed(level), nbits,
-                (unsigned long long*)A, (unsigned long long*)Cc);
+void launch_bbh_seg(hipStream_t st, bool wide, int mode, const void* keys, const uint64_t* key_off, uint64_t n_segs,
+                    uint64_t* seg_nblk, const uint64_t* blk_off, uint64_t* blocks, uint32_t* big_list, unsigned int* n_big,
+                    unsigned int* err) {
+  if (n_segs == 0) return;
+  dim3 g((unsigned)n_segs), b(RK_WG);
+  if (!wide) {
+    if (mode == 0)
+      CF1G_LAUNCH("p3b_bbh_size", (k_bbh_seg<0, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, n_segs, seg_nblk, blk_off,
+                  blocks, big_list, n_big, err);
+    else
+      CF1G_LAUNCH("p3b_bbh_build", (k_bbh_seg<1, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, n_segs, seg_nblk, blk_off,
+                  blocks, big_list, n_big, err);
+  } else {
+    if (mode == 0)
+      CF1G_LAUNCH("p3b_bbh_size", (k_bbh_seg<0, Key128>), g, b, 0, st, (const Key128*)keys, key_off, n_segs, seg_nblk, blk_off,
+                  blocks, big_list, n_big, err);
+    else
+      CF1G_LAUNCH("p3b_bbh_build", (k_bbh_seg<1, Key128>), g, b, 0, st, (const Key128*)keys, key_off, n_segs, seg_nblk, blk_off,
+                  blocks, big_list, n_big, err);
+  }
 }
-void launch_bbh_filter(hipStream_t st, bool wide, const void* keys, uint64_t n, int level, uint64_t nbits, const uint64_t* A,
-                       const uint64_t* Cc, void* out, uint64_t* out_n) {
-  if (n == 0) return;
-  unsigned g = grid_for(n, 256, 8192);
-  if (!wide)
-    CF1G_LAUNCH("p3b_bbh_filter", k_bbh_filter<uint64_t>, dim3(g), dim3(256), 0, st, (const uint64_t*)keys, n, bbh_seed(level),
-                nbits, A, Cc, (uint64_t*)out, (unsigned long long*)out_n);
-  else
-    CF1G_LAUNCH("p3b_bbh_filter", k_bbh_filter<Key128>, dim3(g), dim3(256), 0, st, (const Key128*)keys, n, bbh_seed(level), nbits,
-                A, Cc, (Key128*)out, (unsigned long long*)out_n);
-}
-void launch_bbh_blocks(hipStream_t st, const uint64_t* A, const uint64_t* Cc, uint64_t nblk, uint64_t blk_off, uint64_t* blocks,
-                       uint64_t* popc) {
-  if (nblk == 0) return;
-  CF1G_LAUNCH("p3b_bbh_blocks", k_bbh_blocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, A, Cc, nblk, blk_off, blocks,
-              popc);
-}
-void launch_bbh_rank(hipStream_t st, uint64_t* blocks, const uint64_t* rank, uint64_t nblk) {
-  if (nblk == 0) return;
-  CF1G_LAUNCH("p3b_bbh_rank", k_bbh_rank, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, blocks, rank, nblk);
+void launch_bbh_big(hipStream_t st, bool wide, int mode, const void* keys, const uint64_t* key_off, const uint32_t* big_list,
+                    uint32_t n_big, uint64_t* seg_nblk, const uint64_t* blk_off, uint64_t* blocks, uint64_t* scratch,
+                    uint64_t* alive_g, unsigned int* err) {
+  if (n_big == 0) return;
+  dim3 g(n_big), b(RK_WG);
+  if (!wide) {
+    if (mode == 0)
+      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<0, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, big_list, seg_nblk, blk_off,
+                  blocks, scratch, alive_g, err);
+    else
+      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<1, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, big_list, seg_nblk, blk_off,
+                  blocks, scratch, alive_g, err);
+  } else {
+    if (mode == 0)
+      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<0, Key128>), g, b, 0, st, (const Key128*)keys, key_off, big_list, seg_nblk, blk_off,
+                  blocks, scratch, alive_g, err);
+    else
+      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<1, Key128>), g, b, 0, st, (const Key128*)keys, key_off, big_list, seg_nblk, blk_off,
+                  blocks, scratch, alive_g, err);
+  }
 }
 
 }  // namespace cf1g

@@ -90,8 +90,7 @@ __device__ __forceinline__ void state_join(unsigned long long* __restrict__ stat
 
 template <typename KeyT>
 __global__ void __launch_bounds__(256) k_p4_classify(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                     uint64_t n_pos, int K, const BbhMeta* __restrict__ meta,
-                                                     const uint64_t* __restrict__ blocks, unsigned long long* __restrict__ states,
+                                                     uint64_t n_pos, int K, const MphfView mv, unsigned long long* __restrict__ states,
                                                      unsigned int* __restrict__ next_chunk, uint32_t chunk, uint32_t n_chunks) {
   __shared__ uint32_t s_c;
   for (;;) {
@@ -108,7 +107,7 @@ __global__ void __launch_bounds__(256) k_p4_classify(const uint64_t* __restrict_
       bool fwd;
       uint32_t type, pb, nb;
       if (!observe<KeyT>(seq, nmask, g, K, canon, fwd, type, pb, nb)) continue;
-      uint64_t idx = bbh_lookup<KeyT>(meta, blocks, canon);
+      uint64_t idx = mphf_lookup<KeyT>(mv, canon);
       state_join(states, idx, type, fwd, pb, nb);
     }
   }
@@ -117,8 +116,8 @@ __global__ void __launch_bounds__(256) k_p4_classify(const uint64_t* __restrict_
 // Per stream position: bit0 valid k-mer, bit1 FWD, bits2-3 state class.
 template <typename KeyT>
 __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t n_pos,
-                                                  int K, const BbhMeta* __restrict__ meta, const uint64_t* __restrict__ blocks,
-                                                  const uint64_t* __restrict__ states, uint8_t* __restrict__ out) {
+                                                  int K, const MphfView mv, const uint64_t* __restrict__ states,
+                                                  uint8_t* __restrict__ out) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_pos) return;
   uint8_t a = 0;
@@ -126,7 +125,7 @@ __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ s
     KeyT x = KOps<KeyT>::at(seq, g, K);
     KeyT r = KOps<KeyT>::rc(x, K);
     KeyT c = kmer_min(x, r);
-    uint64_t idx = bbh_lookup<KeyT>(meta, blocks, c);
+    uint64_t idx = mphf_lookup<KeyT>(mv, c);
     uint32_t s = (uint32_t)(states[idx / 12] >> ((idx % 12) * 5)) & 31u;
     a = (uint8_t)(1u | (key_eq(x, c) ? 2u : 0u) | (st_class(s) << 2));
   }
@@ -134,42 +133,41 @@ __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ s
 }
 
 template <typename KeyT>
-__global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, uint64_t n, const BbhMeta* __restrict__ meta,
-                                                const uint64_t* __restrict__ blocks, uint64_t* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, uint64_t n, const MphfView mv,
+                                                uint64_t* __restrict__ out) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    out[i] = bbh_lookup<KeyT>(meta, blocks, keys[i]);
+    out[i] = mphf_lookup<KeyT>(mv, keys[i]);
 }
 
 void launch_p4_classify(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
-                        const BbhMeta* meta, const uint64_t* blocks, uint64_t* states, unsigned int* next_chunk, uint32_t chunk,
-                        uint32_t n_chunks, uint32_t grid) {
+                        const MphfView& mv, uint64_t* states, unsigned int* next_chunk, uint32_t chunk, uint32_t n_chunks,
+                        uint32_t grid) {
   if (n_chunks == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p4_classify", k_p4_classify<uint64_t>, dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, meta, blocks,
+    CF1G_LAUNCH("p4_classify", k_p4_classify<uint64_t>, dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, mv,
                 (unsigned long long*)states, next_chunk, chunk, n_chunks);
   else
-    CF1G_LAUNCH("p4_classify", k_p4_classify<Key128>, dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, meta, blocks,
+    CF1G_LAUNCH("p4_classify", k_p4_classify<Key128>, dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, mv,
                 (unsigned long long*)states, next_chunk, chunk, n_chunks);
 }
 void launch_annotate(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
-                     const BbhMeta* meta, const uint64_t* blocks, const uint64_t* states, uint8_t* out) {
+                     const MphfView& mv, const uint64_t* states, uint8_t* out) {
   if (n_pos == 0) return;
   dim3 grid((unsigned)((n_pos + 255) / 256));
   if (!wide)
-    CF1G_LAUNCH("p5_annotate", k_annotate<uint64_t>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, meta, blocks, states, out);
+    CF1G_LAUNCH("p5_annotate", k_annotate<uint64_t>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, mv, states, out);
   else
-    CF1G_LAUNCH("p5_annotate", k_annotate<Key128>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, meta, blocks, states, out);
+    CF1G_LAUNCH("p5_annotate", k_annotate<Key128>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, mv, states, out);
 }
-void launch_lookup(hipStream_t st, bool wide, const void* keys, uint64_t n, const BbhMeta* meta, const uint64_t* blocks,
-                   uint64_t* out) {
+void launch_lookup(hipStream_t st, bool wide, const void* keys, uint64_t n, const MphfView& mv, uint64_t* out) {
   if (n == 0) return;
   uint64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
   if (!wide)
-    CF1G_LAUNCH("bbh_lookup", k_lookup<uint64_t>, dim3((unsigned)g), dim3(256), 0, st, (const uint64_t*)keys, n, meta, blocks, out);
+    CF1G_LAUNCH("bbh_lookup", k_lookup<uint64_t>, dim3((unsigned)g), dim3(256), 0, st, (const uint64_t*)keys, n, mv, out);
   else
-    CF1G_LAUNCH("bbh_lookup", k_lookup<Key128>, dim3((unsigned)g), dim3(256), 0, st, (const Key128*)keys, n, meta, blocks, out);
+    CF1G_LAUNCH("bbh_lookup", k_lookup<Key128>, dim3((unsigned)g), dim3(256), 0, st, (const Key128*)keys, n, mv, out);
 }
 
 }  // namespace cf1g

@@ -57,7 +57,7 @@ typedef struct cf1g_info {
   uint64_t mphf_words;     /* size of cf1g_mphf_export in u64 words */
   uint32_t num_bins;       /* bins after partitioning (<= requested) */
   uint32_t seg_bits;       /* P3 segment bits per bin */
-  uint32_t mphf_levels;
+  uint32_t mphf_big_segments; /* MPHF segments built through global memory */
   uint32_t n_oversized;    /* P3 segments sorted through global memory */
   uint32_t k, m;
 } cf1g_info;
@@ -106,10 +106,12 @@ int cf1g_states_export(cf1g_ctx* ctx, uint64_t* words, uint64_t n_words);
 /* The distinct canonical keys in bin order, each bin sorted (the order of
  * cf1_dedup_all.bin, src/mphf.rs:120-132): D u64, or D (lo, hi) pairs. */
 int cf1g_keys_export(cf1g_ctx* ctx, uint64_t* out, uint64_t n_words);
-/* Serialised BBHash: header (4 + 2*64 words: levels, D, wide, blocks, level
- * bits[64], level block offsets[64]) then 8-word blocks. With buf == NULL only
- * *n_words is set. The host lookup over this buffer equals Mphf::hash
- * (src/mphf.rs:268-270) on the device. */
+/* Serialised MPHF (minimizer-partitioned BBHash): 16 header words
+ * (magic, k, m, segment bits, bins, segments, D, blocks, wide), the
+ * bucket->bin table (65536 u16), per-segment key offsets and block offsets
+ * (segments + 1 words each), then 8-word blocks {rank, 7 bit words}. With
+ * buf == NULL only *n_words is set. A host lookup over this buffer equals
+ * Mphf::hash (src/mphf.rs:268-270) on the device. */
 int cf1g_mphf_export(cf1g_ctx* ctx, uint64_t* buf, uint64_t* n_words);
 /* Batch Mphf::hash (src/mphf.rs:268-270): keys in host memory (u64 or pairs). */
 int cf1g_mphf_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);

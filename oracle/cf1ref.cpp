@@ -35,73 +35,96 @@ void parallel_for(size_t n_tasks, int threads, const std::function<void(size_t)>
   for (auto& th : pool) th.join();
 }
 
-// ------------------------------------------------------------------ BBHash build
-void bbhash_build(BBHash& out, const uint64_t* lo, const uint64_t* hi, uint64_t n, bool wide, int threads) {
-  out = BBHash();
-  out.wide = wide;
-  out.n_keys = n;
-  std::vector<uint64_t> cur_lo(lo, lo + n), cur_hi;
-  if (wide) cur_hi.assign(hi, hi + n);
-  std::vector<std::vector<uint64_t>> level_words;
-  uint64_t blk = 0;
+// ------------------------------------------------------------------ MPHF build
+static void build_segment(const uint64_t* lo, const uint64_t* hi, uint64_t D, bool wide, std::vector<uint64_t>& blocks_out) {
+  std::vector<char> alive(D, 1);
+  std::vector<uint64_t> region;  // level words, bit-contiguous
+  uint64_t off = 0, t = D, r = D;
   int l = 0;
-  while (!cur_lo.empty()) {
-    if (l >= BBH_MAX_LEVELS) throw std::runtime_error("bbhash: too many levels");
-    uint64_t m = cur_lo.size();
-    uint64_t nb = bbh_level_bits(m);
-    uint64_t nw = nb / 64;
-    std::vector<std::atomic<uint64_t>> A(nw), C(nw);
-    for (uint64_t i = 0; i < nw; i++) {
-      A[i].store(0, std::memory_order_relaxed);
-      C[i].store(0, std::memory_order_relaxed);
+  while (r > 0) {
+    if (l >= BBH_MAX_LEVELS) throw std::runtime_error("MPHF: too many levels");
+    uint64_t n = bbh_level_bits(t), nw = n / 64;
+    std::vector<uint64_t> seen(nw, 0), col(nw, 0);
+    for (uint64_t i = 0; i < D; i++) {
+      if (!alive[i]) continue;
+      uint64_t pos = mulhi64(bbh_hash(lo[i], wide ? hi[i] : 0, wide, l), n);
+      uint64_t bit = 1ULL << (pos % 64);
+      if (seen[pos / 64] & bit) col[pos / 64] |= bit;
+      seen[pos / 64] |= bit;
     }
-    size_t chunk = 1 << 16;
-    size_t ntask = (m + chunk - 1) / chunk;
-    // mark pass: seen / collision bits
-    parallel_for(ntask, threads, [&](size_t t) {
-      uint64_t a = t * chunk, b = std::min<uint64_t>(m, a + chunk);
-      for (uint64_t i = a; i < b; i++) {
-        uint64_t pos = mulhi64(bbh_hash(cur_lo[i], wide ? cur_hi[i] : 0, wide, l), nb);
-        uint64_t bit = 1ULL << (pos % 64);
-        uint64_t old = A[pos / 64].fetch_or(bit, std::memory_order_relaxed);
-        if (old & bit) C[pos / 64].fetch_or(bit, std::memory_order_relaxed);
-      }
-    });
-    std::vector<uint64_t> lw(nw);
-    for (uint64_t i = 0; i < nw; i++) lw[i] = A[i].load() & ~C[i].load();
-    // survivors, in input order
-    std::vector<uint64_t> nlo, nhi;
-    for (uint64_t i = 0; i < m; i++) {
-      uint64_t pos = mulhi64(bbh_hash(cur_lo[i], wide ? cur_hi[i] : 0, wide, l), nb);
-      if (!((lw[pos / 64] >> (pos % 64)) & 1)) {
-        nlo.push_back(cur_lo[i]);
-        if (wide) nhi.push_back(cur_hi[i]);
-      }
+    for (uint64_t i = 0; i < nw; i++) region.push_back(seen[i] & ~col[i]);
+    uint64_t left = 0;
+    for (uint64_t i = 0; i < D; i++) {
+      if (!alive[i]) continue;
+      uint64_t pos = mulhi64(bbh_hash(lo[i], wide ? hi[i] : 0, wide, l), n);
+      if (((seen[pos / 64] & ~col[pos / 64]) >> (pos % 64)) & 1) alive[i] = 0;
+      else left++;
     }
-    out.level_bits[l] = nb;
-    out.level_block_off[l] = blk;
-    blk += nb / BBH_BLOCK_BITS;
-    level_words.push_back(std::move(lw));
-    cur_lo.swap(nlo);
-    cur_hi.swap(nhi);
+    r = left;
+    off += n;
+    t = bbh_next_t(t);
     l++;
   }
-  out.num_levels = l;
-  out.level_block_off[l] = blk;
-  out.blocks.assign(blk * 8, 0);
+  uint64_t nblk = (off + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;
+  blocks_out.assign(nblk * 8, 0);
   uint64_t rank = 0;
-  for (int lv = 0; lv < l; lv++) {
-    uint64_t nblk = out.level_bits[lv] / BBH_BLOCK_BITS;
-    for (uint64_t j = 0; j < nblk; j++) {
-      uint64_t* B = &out.blocks[(out.level_block_off[lv] + j) * 8];
-      B[0] = rank;
-      for (int t = 0; t < 7; t++) {
-        B[1 + t] = level_words[lv][j * 7 + t];
-        rank += (uint64_t)__builtin_popcountll(B[1 + t]);
-      }
+  for (uint64_t j = 0; j < nblk; j++) {
+    blocks_out[j * 8] = rank;
+    for (int u = 0; u < 7; u++) {
+      uint64_t q = j * 7 + (uint64_t)u;
+      uint64_t w = q < region.size() ? region[q] : 0;
+      blocks_out[j * 8 + 1 + (uint64_t)u] = w;
+      rank += (uint64_t)__builtin_popcountll(w);
     }
   }
-  if (rank != n) throw std::runtime_error("bbhash: rank total mismatch");
+  if (rank != D) throw std::runtime_error("MPHF: rank total mismatch");
+}
+
+void mphf_build(Mphf& out, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
+                const std::vector<uint16_t>& b2b, int K, int m, uint64_t W, uint64_t target, int threads) {
+  out = Mphf();
+  out.K = K;
+  out.m = m;
+  out.wide = K > 32;
+  out.n_keys = lo.size();
+  out.b2b = b2b;
+  const uint64_t bins = per_bin.size();
+  out.sb = segment_bits(W, bins, K, target);
+  const int sb = out.sb;
+  const uint64_t ns = bins << sb;
+  out.key_off.assign(ns + 1, 0);
+  // segment boundaries inside each sorted bin
+  uint64_t pos = 0;
+  for (uint64_t b = 0; b < bins; b++) {
+    uint64_t end = pos + per_bin[b];
+    uint64_t i = pos;
+    for (uint64_t s = 0; s < (1ULL << sb); s++) {
+      out.key_off[(b << sb) | s] = i;
+      while (i < end) {
+        u128 c = ((u128)hi[i] << 64) | lo[i];
+        uint64_t top = sb ? (uint64_t)(c >> (2 * K - sb)) : 0;
+        if (top != s) break;
+        i++;
+      }
+    }
+    if (i != end) throw std::runtime_error("MPHF: keys of a bin are not sorted");
+    pos = end;
+  }
+  out.key_off[ns] = pos;
+  std::vector<std::vector<uint64_t>> blk(ns);
+  parallel_for(ns, threads, [&](size_t s) {
+    uint64_t a = out.key_off[s], d = out.key_off[s + 1] - a;
+    if (d) build_segment(lo.data() + a, hi.data() + a, d, out.wide, blk[s]);
+  });
+  out.blk_off.assign(ns + 1, 0);
+  uint64_t tot = 0;
+  for (uint64_t s = 0; s < ns; s++) {
+    out.blk_off[s] = tot;
+    tot += blk[s].size() / 8;
+  }
+  out.blk_off[ns] = tot;
+  out.blocks.assign(tot * 8, 0);
+  for (uint64_t s = 0; s < ns; s++) std::copy(blk[s].begin(), blk[s].end(), out.blocks.begin() + (ptrdiff_t)(out.blk_off[s] * 8));
 }
 
 // ------------------------------------------------------------------ minimizer scan
@@ -221,7 +244,9 @@ class PipelineBase {
   std::vector<std::vector<uint8_t>> bins;  // reference bin record format
   std::vector<uint64_t> keys_lo, keys_hi;  // consolidated dedup keys
   std::vector<uint64_t> per_bin_distinct;
-  BBHash mphf;
+  Mphf mphf;
+  uint64_t target_seg = 0;  // 0 = default (2048 for K <= 32, 1024 above)
+  uint64_t n_kmers = 0;     // W, valid k-mer occurrences decoded in P3a
   AtomicStateVector states;
   PhaseTimes times;
   P5Meta meta;
@@ -232,7 +257,10 @@ class PipelineBase {
   void p2_partition();
   virtual void p2_route() = 0;
   virtual void p3a() = 0;
-  void p3b() { bbhash_build(mphf, keys_lo.data(), keys_hi.data(), keys_lo.size(), k > 32, threads); }
+  void p3b() {
+    uint64_t tgt = target_seg ? target_seg : (k > 32 ? 1024 : 2048);
+    mphf_build(mphf, keys_lo, keys_hi, per_bin_distinct, bucket_to_bin, k, m, n_kmers, tgt, threads);
+  }
   virtual void p4() = 0;
   virtual void p5(bool poly_n_stretch) = 0;
   virtual uint64_t kmer_from_ascii(const char* s, uint64_t* hi) = 0;
@@ -360,6 +388,7 @@ class Pipeline : public PipelineBase {
   // canonicalise, sort, dedup; bins concatenated in order (mphf.rs:120-132).
   void p3a() override {
     std::vector<std::vector<typename KM::Storage>> per((size_t)num_bins);
+    std::vector<uint64_t> occ((size_t)num_bins, 0);
     parallel_for((size_t)num_bins, threads, [&](size_t b) {
       const std::vector<uint8_t>& d = bins[b];
       auto& out = per[b];
@@ -372,9 +401,12 @@ class Pipeline : public PipelineBase {
           for (size_t i = 0; i + K <= bc; i++) out.push_back(KM::from_packed_2bit(packed, i).canonical().bits);
         p += (bc + 3) / 4;
       }
+      occ[b] = out.size();
       std::sort(out.begin(), out.end());
       out.erase(std::unique(out.begin(), out.end()), out.end());
     });
+    n_kmers = 0;
+    for (auto x : occ) n_kmers += x;
     keys_lo.clear();
     keys_hi.clear();
     per_bin_distinct.assign((size_t)num_bins, 0);
@@ -1002,24 +1034,36 @@ uint64_t ref_per_bin_distinct(void* hp, uint64_t* out) {
   if (out) std::memcpy(out, h->p->per_bin_distinct.data(), n * 8);
   return n;
 }
-// MPHF structure: header words then blocks. Returns total u64 words.
-uint64_t ref_mphf_export(void* hp, uint64_t* out) {
-  Handle* h = (Handle*)hp;
-  const BBHash& b = h->p->mphf;
-  uint64_t hdr = 4 + 2 * BBH_MAX_LEVELS;
-  uint64_t total = hdr + b.blocks.size();
+// MPHF export (same layout as the device export): 16 header words, the
+// bucket->bin table as 65536 u16, key offsets, block offsets, blocks.
+static uint64_t export_mphf(const Mphf& b, uint64_t D, uint64_t bins, uint64_t* out) {
+  uint64_t ns = b.key_off.empty() ? 0 : b.key_off.size() - 1;
+  uint64_t total = 16 + 65536 / 4 + 2 * (ns + 1) + b.blocks.size();
   if (out) {
-    out[0] = (uint64_t)b.num_levels;
-    out[1] = b.n_keys;
-    out[2] = b.wide ? 1 : 0;
-    out[3] = b.level_block_off[b.num_levels];
-    for (int l = 0; l < BBH_MAX_LEVELS; l++) {
-      out[4 + l] = b.level_bits[l];
-      out[4 + BBH_MAX_LEVELS + l] = b.level_block_off[l];
-    }
-    std::memcpy(out + hdr, b.blocks.data(), b.blocks.size() * 8);
+    std::memset(out, 0, 16 * 8);
+    out[0] = 0x6366316D70686631ULL;
+    out[1] = (uint64_t)b.K;
+    out[2] = (uint64_t)b.m;
+    out[3] = (uint64_t)b.sb;
+    out[4] = bins;
+    out[5] = ns;
+    out[6] = D;
+    out[7] = b.blocks.size() / 8;
+    out[8] = b.wide ? 1 : 0;
+    std::memcpy(out + 16, b.b2b.data(), 65536 * 2);
+    std::memcpy(out + 16 + 16384, b.key_off.data(), (ns + 1) * 8);
+    std::memcpy(out + 16 + 16384 + ns + 1, b.blk_off.data(), (ns + 1) * 8);
+    std::memcpy(out + 16 + 16384 + 2 * (ns + 1), b.blocks.data(), b.blocks.size() * 8);
   }
   return total;
+}
+uint64_t ref_mphf_export(void* hp, uint64_t* out) {
+  Handle* h = (Handle*)hp;
+  return export_mphf(h->p->mphf, h->p->keys_lo.size(), (uint64_t)h->p->num_bins, out);
+}
+int ref_set_target_seg(void* hp, uint64_t t) {
+  ((Handle*)hp)->p->target_seg = t;
+  return 0;
 }
 int ref_mphf_lookup(void* hp, const uint64_t* lo, const uint64_t* hi, uint64_t n, uint64_t* out) {
   Handle* h = (Handle*)hp;
@@ -1190,34 +1234,5 @@ void ref_sv_free(void* p) { delete (AtomicStateVector*)p; }
 int ref_sv_get(void* p, uint64_t i) { return ((AtomicStateVector*)p)->get(i); }
 int ref_sv_cas(void* p, uint64_t i, int e, int n) { return ((AtomicStateVector*)p)->compare_and_swap(i, (uint8_t)e, (uint8_t)n) ? 1 : 0; }
 uint64_t ref_sv_word(void* p, uint64_t w) { return ((AtomicStateVector*)p)->data[w].load(); }
-
-// BBHash over caller keys (tests of the MPHF alone).
-void* ref_bbh_build(const uint64_t* lo, const uint64_t* hi, uint64_t n, int wide, int threads) {
-  BBHash* b = new BBHash();
-  bbhash_build(*b, lo, hi, n, wide != 0, threads);
-  return b;
-}
-void ref_bbh_free(void* p) { delete (BBHash*)p; }
-int ref_bbh_lookup(void* p, const uint64_t* lo, const uint64_t* hi, uint64_t n, uint64_t* out) {
-  BBHash* b = (BBHash*)p;
-  for (uint64_t i = 0; i < n; i++) out[i] = b->lookup(lo[i], hi ? hi[i] : 0);
-  return 0;
-}
-uint64_t ref_bbh_export(void* p, uint64_t* out) {
-  BBHash* b = (BBHash*)p;
-  uint64_t hdr = 4 + 2 * BBH_MAX_LEVELS;
-  if (out) {
-    out[0] = (uint64_t)b->num_levels;
-    out[1] = b->n_keys;
-    out[2] = b->wide ? 1 : 0;
-    out[3] = b->level_block_off[b->num_levels];
-    for (int l = 0; l < BBH_MAX_LEVELS; l++) {
-      out[4 + l] = b->level_bits[l];
-      out[4 + BBH_MAX_LEVELS + l] = b->level_block_off[l];
-    }
-    std::memcpy(out + hdr, b->blocks.data(), b->blocks.size() * 8);
-  }
-  return hdr + b->blocks.size();
-}
 
 }  // extern "C"

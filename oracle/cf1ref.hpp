@@ -428,16 +428,23 @@ inline uint32_t fmix32(uint32_t h) {
 }
 inline uint32_t mmer_order(uint32_t canon) { return fmix32(canon + 0x9E3779B9u); }
 
-// ---------------------------------------------------------------- BBHash
+// ---------------------------------------------------------------- MPHF
 // The reference builds ph::fmph (ph 0.11.0, gamma = relative_level_size 200%,
-// mphf.rs:155-160). ph is absent; the project's BBHash (the same structure on
-// CPU and GPU, exact values parity-unpinned against ph; graph output is
-// invariant to the MPHF once unitig ids are canonicalised):
-//   level l holds n_l = roundup(max(2*r_l, 1), 448) bits for r_l remaining keys;
-//   a key lands at pos = mulhi64(hash(key, l), n_l); a key is placed at level l
-//   iff no other remaining key lands on pos; others go to level l+1;
-//   storage = 64-byte blocks {u64 rank_before, u64 bits[7]} (448 bits each),
-//   levels concatenated; value = rank of the key's bit among all set bits.
+// mphf.rs:155-160). ph is absent; the project's MPHF (the same structure on
+// CPU and GPU; exact values parity-unpinned against ph; graph output is
+// invariant to the MPHF once unitig ids are canonicalised) is a
+// minimizer-partitioned BBHash:
+//   segment of key c = (bin, s): bin = bucket_to_bin[bucket_hash(v(c))] with
+//   v(c) the minimizer value of c; s = top sb bits of c. The consolidated keys
+//   (bins in order, each sorted) are already grouped by segment.
+//   sb = fewest bits (<= 12, <= 2K) with W <= bins * 2^sb * target
+//   (target 2048 for K <= 32, 1024 above; W = valid k-mer occurrences).
+//   per segment with D_s keys, level l has n_l = roundup64(max(2 t_l, 1)) bits,
+//   t_0 = D_s, t_{l+1} = ceil(9 t_l / 20); a key lands on
+//   off_l + mulhi64(hash(key, l), n_l) and is placed at the first level where
+//   no other remaining key of the segment lands on that bit;
+//   levels are bit-contiguous in 64-byte blocks {u64 rank_before, u64 bits[7]}
+//   (ranks local to the segment); value = key_off[segment] + local rank.
 constexpr int BBH_MAX_LEVELS = 64;
 constexpr uint64_t BBH_BLOCK_BITS = 448;
 inline uint64_t mix64(uint64_t z) {
@@ -452,40 +459,82 @@ inline uint64_t bbh_hash(uint64_t lo, uint64_t hi, bool wide, int level) {
   return mix64(lo ^ mix64(hi ^ s));
 }
 inline uint64_t mulhi64(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) >> 64); }
-inline uint64_t bbh_level_bits(uint64_t remaining) {
-  uint64_t n = std::max<uint64_t>(2 * remaining, 1);
-  return (n + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS * BBH_BLOCK_BITS;
+inline uint64_t bbh_level_bits(uint64_t t) { return (std::max<uint64_t>(2 * t, 1) + 63) / 64 * 64; }
+inline uint64_t bbh_next_t(uint64_t t) { return (9 * t + 19) / 20; }
+
+// Reverse complement of a 2K-bit value, base by base.
+inline u128 rc_value(u128 v, int K) {
+  u128 r = 0;
+  for (int t = 0; t < K; t++) {
+    r = (r << 2) | (u128)(3 - (uint64_t)(v & 3));
+    v >>= 2;
+  }
+  return r;
+}
+// Minimizer value of a k-mer from its canonical form: the canonical m-mer of
+// least order among its w = K-m+1 m-mers.
+inline uint32_t minimizer_value(u128 c, int K, int m) {
+  u128 r = rc_value(c, K);
+  uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
+  uint32_t best = 0xFFFFFFFFu, bv = 0;
+  for (int j = 0; j + m <= K; j++) {
+    uint32_t f = (uint32_t)(c >> (2 * (K - m - j))) & M;
+    uint32_t b = (uint32_t)(r >> (2 * j)) & M;
+    uint32_t cv = std::min(f, b);
+    uint32_t o = mmer_order(cv);
+    if (o < best) {
+      best = o;
+      bv = cv;
+    }
+  }
+  return bv;
+}
+inline int segment_bits(uint64_t W, uint64_t bins, int K, uint64_t target) {
+  int sb = 0;
+  while (sb < 12 && sb < 2 * K && W > bins * (1ULL << sb) * target) sb++;
+  return sb;
 }
 
-struct BBHash {
-  int num_levels = 0;
-  uint64_t n_keys = 0;
-  uint64_t level_bits[BBH_MAX_LEVELS] = {0};
-  uint64_t level_block_off[BBH_MAX_LEVELS + 1] = {0};
-  std::vector<uint64_t> blocks;  // 8 words per block
+struct Mphf {
+  int K = 0, m = 0, sb = 0;
   bool wide = false;
-
+  uint64_t n_keys = 0;
+  std::vector<uint16_t> b2b;
+  std::vector<uint64_t> key_off, blk_off, blocks;
+  uint64_t segment(u128 c) const {
+    uint32_t v = minimizer_value(c, K, m);
+    uint64_t bin = b2b[bucket_hash(v)];
+    uint64_t sub = sb ? (uint64_t)(c >> (2 * K - sb)) : 0;
+    return (bin << sb) | sub;
+  }
   uint64_t lookup(uint64_t lo, uint64_t hi) const {
-    for (int l = 0; l < num_levels; l++) {
-      uint64_t pos = mulhi64(bbh_hash(lo, hi, wide, l), level_bits[l]);
-      uint64_t blk = level_block_off[l] + pos / BBH_BLOCK_BITS;
+    u128 c = ((u128)hi << 64) | lo;
+    uint64_t seg = segment(c);
+    uint64_t ko = key_off[seg], t = key_off[seg + 1] - ko;
+    const uint64_t* base = blocks.data() + blk_off[seg] * 8;
+    uint64_t off = 0;
+    for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
+      uint64_t n = bbh_level_bits(t);
+      uint64_t pos = off + mulhi64(bbh_hash(lo, hi, wide, l), n);
+      const uint64_t* B = base + (pos / BBH_BLOCK_BITS) * 8;
       uint64_t b = pos % BBH_BLOCK_BITS;
-      const uint64_t* B = &blocks[blk * 8];
       uint64_t wd = B[1 + b / 64];
       if ((wd >> (b % 64)) & 1) {
-        uint64_t r = B[0];
-        for (uint64_t t = 0; t < b / 64; t++) r += (uint64_t)__builtin_popcountll(B[1 + t]);
-        r += (uint64_t)__builtin_popcountll(wd & ((1ULL << (b % 64)) - 1));
-        return r;
+        uint64_t r = ko + B[0];
+        for (uint64_t u = 0; u < b / 64; u++) r += (uint64_t)__builtin_popcountll(B[1 + u]);
+        return r + (uint64_t)__builtin_popcountll(wd & ((1ULL << (b % 64)) - 1));
       }
+      off += n;
+      t = bbh_next_t(t);
     }
     return 0;  // absent key: the reference maps absent keys to 0 (mphf.rs:269)
   }
 };
 
-// Build from distinct keys given as (lo, hi) pairs (hi == 0 and wide == false
-// for K <= 32). Threads split each level's key range.
-void bbhash_build(BBHash& out, const uint64_t* lo, const uint64_t* hi, uint64_t n, bool wide, int threads);
+// Build over the consolidated keys (bins in order, each bin sorted) given the
+// per-bin distinct counts. Threads split the segments.
+void mphf_build(Mphf& out, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
+                const std::vector<uint16_t>& b2b, int K, int m, uint64_t W, uint64_t target, int threads);
 
 // ---------------------------------------------------------------- threading
 void parallel_for(size_t n_tasks, int threads, const std::function<void(size_t)>& fn);

@@ -69,12 +69,7 @@ def lib():
         L.ref_sv_cas.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int]
         L.ref_sv_word.restype = C.c_uint64
         L.ref_sv_word.argtypes = [C.c_void_p, C.c_uint64]
-        L.ref_bbh_build.restype = C.c_void_p
-        L.ref_bbh_build.argtypes = [U64P, U64P, C.c_uint64, C.c_int, C.c_int]
-        L.ref_bbh_free.argtypes = [C.c_void_p]
-        L.ref_bbh_lookup.argtypes = [C.c_void_p, U64P, U64P, C.c_uint64, U64P]
-        L.ref_bbh_export.restype = C.c_uint64
-        L.ref_bbh_export.argtypes = [C.c_void_p, U64P]
+        L.ref_set_target_seg.argtypes = [C.c_void_p, C.c_uint64]
         _LIB = L
     return _LIB
 
@@ -97,12 +92,14 @@ def pack_records(seqs):
 class RefPipeline:
     """The oracle pipeline for one input and one k (P1..P5)."""
 
-    def __init__(self, k, num_bins=128, threads=1):
+    def __init__(self, k, num_bins=128, threads=1, target_seg=0):
         self.L = lib()
         self.k = k
         self.h = self.L.ref_create(k, num_bins, threads)
         if not self.h:
             raise ValueError("invalid k / num_bins")
+        if target_seg:
+            self.L.ref_set_target_seg(self.h, target_seg)
 
     def __del__(self):
         try:
@@ -214,31 +211,3 @@ def state_of(words, idx):
     idx = np.asarray(idx, np.uint64)
     w = words[(idx // 12).astype(np.int64)]
     return ((w >> ((idx % 12) * 5)) & 31).astype(np.uint8)
-
-
-class RefBBHash:
-    def __init__(self, lo, hi=None, threads=1):
-        self.L = lib()
-        lo = np.ascontiguousarray(lo, np.uint64)
-        self.wide = hi is not None
-        hip = _p(np.ascontiguousarray(hi, np.uint64), U64P) if self.wide else None
-        self.h = self.L.ref_bbh_build(_p(lo, U64P), hip, len(lo), int(self.wide), threads)
-
-    def __del__(self):
-        try:
-            self.L.ref_bbh_free(self.h)
-        except Exception:
-            pass
-
-    def lookup(self, lo, hi=None):
-        lo = np.ascontiguousarray(lo, np.uint64)
-        out = np.zeros(max(len(lo), 1), np.uint64)
-        hip = _p(np.ascontiguousarray(hi, np.uint64), U64P) if hi is not None else None
-        self.L.ref_bbh_lookup(self.h, _p(lo, U64P), hip, len(lo), _p(out, U64P))
-        return out[: len(lo)]
-
-    def export(self):
-        n = self.L.ref_bbh_export(self.h, None)
-        out = np.zeros(n, np.uint64)
-        self.L.ref_bbh_export(self.h, _p(out, U64P))
-        return out

@@ -56,16 +56,16 @@ def kmers_of(strs, k):
     return sorted(out)
 
 
-def run_pair(seqs, k, num_bins=128, names=None, p5=True, bins=True, threads=4):
+def run_pair(seqs, k, num_bins=128, names=None, p5=True, bins=True, threads=4, target_seg=0):
     names = names or [f"r{i}" for i in range(len(seqs))]
-    ref = RefPipeline(k, num_bins=num_bins, threads=threads)
+    ref = RefPipeline(k, num_bins=num_bins, threads=threads, target_seg=target_seg)
     ref.load(seqs, names)
     h_ref = ref.p1()
     b2b_ref, nb_ref = ref.p2()
     d_ref = ref.p3()
     ref.p4()
 
-    g = cf1g.Cf1g(k=k, num_bins=num_bins)
+    g = cf1g.Cf1g(k=k, num_bins=num_bins, p3_target_seg=target_seg)
     g.load_records(seqs)
     h = g.count_minimizer_histogram()
     np.testing.assert_array_equal(h, h_ref, err_msg="P1 histogram")
@@ -159,6 +159,15 @@ def test_genome_5m_scaled():
 def test_genome_300m_shape_scaled():
     a, off, names = synth.genome_300m(scale=0.001)
     run_pair(synth.records_list(a, off), 31, num_bins=32, names=names, bins=False)
+
+
+@pytest.mark.parametrize("k", [15, 31, 63])
+def test_mphf_segment_sizes(k):
+    seqs = random_genome(seed=77, n_rec=4, max_len=30000)
+    # one segment per bin, > register capacity: global-memory MPHF build
+    run_pair(seqs, k, num_bins=2, target_seg=1 << 24, p5=False, bins=False)
+    # 2^12 segments per bin, mostly tiny
+    run_pair(seqs, k, num_bins=8, target_seg=4, p5=False, bins=False)
 
 
 def test_mphf_lookup_batch():
