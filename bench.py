@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lookups", type=int, default=1 << 28)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run device-to-device copies of this size (MiB) to scale HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every run uses one MI355X); "
@@ -142,6 +144,8 @@ def main():
     import cf1g
     from cf1g import synth
 
+    if args.copy_calibration_mb:
+        copy_calibration(args.copy_calibration_mb << 20)
     t = time.perf_counter()
     ascii_, rec_off, names = synth.genome_300m(scale=args.scale)
     print(f"[bench] generated {len(ascii_) / 1e6:.0f} Mbp in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
@@ -232,6 +236,24 @@ def main():
     }
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def copy_calibration(nbytes, reps=3):
+    """hipMemcpy device-to-device of nbytes (a plain copy kernel of known
+    size: reads and writes nbytes each) for scaling FETCH_SIZE/WRITE_SIZE."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy calibration: {reps} x {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
 def bbhash_lookup_rate(g, D, n, kb):

@@ -31,17 +31,16 @@ void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*,
 void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
                      uint64_t*);
 void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
-                       uint64_t*, void*);
-void launch_p3_segsort(hipStream_t, bool, void*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, unsigned int*);
-void launch_p3_bigsort(hipStream_t, bool, void*, void*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*);
-void launch_p3_compact(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, void*);
-void launch_bbh_seg(hipStream_t, bool, int, const void*, const uint64_t*, uint64_t, uint64_t*, const uint64_t*, uint64_t*, uint32_t*,
-                    unsigned int*, unsigned int*);
-void launch_bbh_big(hipStream_t, bool, int, const void*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*, const uint64_t*,
-                    uint64_t*, uint64_t*, uint64_t*, unsigned int*);
-// kernels_p4.hip
-void launch_p4_classify(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, uint64_t*, unsigned int*,
-                        uint32_t, uint32_t, uint32_t);
+                       uint64_t*, void*, uint16_t*);
+void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint64_t*, uint32_t*,
+                     unsigned int*, uint32_t*, uint64_t*);
+void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
+                       unsigned int*, uint32_t*);
+void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
+                    uint32_t, uint64_t*, uint64_t*, uint64_t*, uint64_t*, unsigned int*);
+void launch_p3_compact(hipStream_t, bool, const void*, const uint16_t*, const uint16_t*, const uint64_t*, const uint64_t*,
+                       const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, int, int, void*, uint16_t*, uint64_t*);
+void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
 void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint8_t*);
 void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*);
 
@@ -248,50 +247,14 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->stage = 3;
 }
 
-// P3b: the minimizer-partitioned BBHash, one segment per work-group (see
-// device.hpp). Size pass, scan of block counts, build pass.
-void bbhash_build(cf1g_ctx* c, const void* keys, uint64_t D) {
-  hipStream_t st = c->st;
-  const uint64_t ns = c->n_segs;
-  const uint64_t* key_off = c->getT<uint64_t>("seg_out_off", ns + 1);
-  uint64_t* nblk = c->getT<uint64_t>("bbh_seg_nblk", ns + 1);
-  uint64_t* blk_off = c->getT<uint64_t>("bbh_blk_off", ns + 1);
-  uint32_t* big = c->getT<uint32_t>("bbh_big", ns + 1);
-  unsigned int* flags = c->getT<unsigned int>("bbh_flags", 4);  // [0] n_big, [1] error
-  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
-  c->zero(flags, 16);
-  launch_bbh_seg(st, c->wide, 0, keys, key_off, ns, nblk, nullptr, nullptr, big, flags, flags + 1);
-  unsigned int hf[2];
-  c->d2h(hf, flags, 8);
-  c->n_big = hf[0];
-  uint64_t* scratch = nullptr;
-  uint64_t* alive = nullptr;
-  uint64_t alive_words = D / 64 + 2;
-  if (c->n_big) {
-    scratch = c->getT<uint64_t>("bbh_scratch", 4 * (D / 64 + 2) + 8 * (ns + 1) + 1024);
-    alive = c->getT<uint64_t>("bbh_alive", alive_words);
-    hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
-    launch_bbh_big(st, c->wide, 0, keys, key_off, big, c->n_big, nblk, nullptr, nullptr, scratch, alive, flags + 1);
-  }
-  scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
-  uint64_t total = 0;
-  c->d2h(&total, blk_off + ns, 8);
-  c->n_blocks = total;
-  uint64_t* blocks = c->getT<uint64_t>("blocks", total * 8 + 8);
-  launch_bbh_seg(st, c->wide, 1, keys, key_off, ns, nblk, blk_off, blocks, big, flags, flags + 1);
-  if (c->n_big) {
-    hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
-    launch_bbh_big(st, c->wide, 1, keys, key_off, big, c->n_big, nblk, blk_off, blocks, scratch, alive, flags + 1);
-  }
-  c->d2h(hf, flags, 8);
-  if (hf[1]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit");
-}
-
+// P3: count, scan, scatter (keys + observation masks), per-segment sort /
+// dedup / mask join / BBHash into padded block slots, big-segment sorts and
+// BBHash, scans of the distinct counts (MPHF key offsets) and block counts,
+// then one compaction of keys, masks (to MPHF-value order) and blocks.
 void do_p3(cf1g_ctx* c) {
   hipStream_t st = c->st;
   const size_t kb = c->key_bytes;
   const int K = c->k;
-  // segment bits: mean segment of ~target k-mers, capped by LDS histogram
   // Segment bits: the fewest (<= 12, <= 2K) that bring the mean segment to
   // <= target k-mers. The MPHF partition depends on it (it is exported).
   uint64_t target = c->prm.p3_target_seg ? c->prm.p3_target_seg : (c->wide ? 1024 : 2048);
@@ -304,14 +267,18 @@ void do_p3(cf1g_ctx* c) {
   uint64_t* seg_off = c->getT<uint64_t>("seg_off", ns + 1);
   uint64_t* cursor = c->getT<uint64_t>("seg_cursor", ns + 1);
   uint64_t* ucount = c->getT<uint64_t>("seg_ucount", ns + 1);
-  uint64_t* out_off = c->getT<uint64_t>("seg_out_off", ns + 1);
+  uint64_t* key_off = c->getT<uint64_t>("seg_out_off", ns + 1);
+  uint64_t* nblk = c->getT<uint64_t>("bbh_seg_nblk", ns + 1);
+  uint64_t* blk_off = c->getT<uint64_t>("bbh_blk_off", ns + 1);
   uint32_t* over = c->getT<uint32_t>("seg_over", ns + 1);
-  unsigned int* n_over = c->getT<unsigned int>("seg_n_over", 2);
+  uint32_t* big = c->getT<uint32_t>("bbh_big", ns + 1);
+  unsigned int* flags = c->getT<unsigned int>("p3_flags", 4);  // [0] n_over, [1] n_big, [2] error bits
   uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
   const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   const uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
   const uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
   c->zero(counts, (ns + 1) * 8);
+  c->zero(flags, 16);
   launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts);
   scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
   hip_check(hipMemcpyAsync(cursor, seg_off, ns * 8, hipMemcpyDeviceToDevice, st), "D2D cursor");
@@ -319,44 +286,55 @@ void do_p3(cf1g_ctx* c) {
   c->d2h(&wcount, seg_off + ns, 8);
   if (wcount != c->W) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
   void* occ = c->get("occ", c->W * kb + 16);
-  launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, cursor, occ);
-  c->zero(n_over, 4);
-  launch_p3_segsort(st, c->wide, occ, seg_off, ns, K, sb, ucount, over, n_over);
-  uint32_t nov = 0;
-  c->d2h(&nov, n_over, 4);
-  c->n_over = nov;
-  if (nov > 0) {
+  uint16_t* occ_obs = c->getT<uint16_t>("occ_obs", c->W + 8);
+  // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
+  uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
+  launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, cursor, occ, occ_obs);
+  launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad);
+  unsigned int hf[3];
+  c->d2h(hf, flags, 12);
+  c->n_over = hf[0];
+  uint16_t* aux_obs = nullptr;
+  if (hf[0] > 0) {
     void* aux = c->get("occ_aux", c->W * kb + 16);
-    launch_p3_bigsort(st, c->wide, occ, aux, seg_off, over, nov, ucount);
+    aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
+    launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over, hf[0], ucount, flags, big);
+    c->d2h(hf, flags, 12);
   }
-  scan_excl_u64(st, ucount, out_off, ns, out_off + ns, tmp);
-  uint64_t D = 0;
-  c->d2h(&D, out_off + ns, 8);
+  c->n_big = hf[1];
+  if (c->n_big) {
+    // segments above LDS capacity: seen / collision words in scratch (at
+    // 4*floor(seg_off/64) + 8*seg, 2*(D_s/32 + 2) words), alive bits per
+    // occurrence position
+    aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
+    uint64_t* scratch = c->getT<uint64_t>("bbh_scratch", 4 * (c->W / 64 + 2) + 8 * (ns + 1) + 1024);
+    const uint64_t alive_words = c->W / 64 + 2;
+    uint64_t* alive = c->getT<uint64_t>("bbh_alive", alive_words);
+    hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
+    launch_bbh_big(st, c->wide, occ, occ_obs, aux_obs, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, flags + 2);
+  }
+  scan_excl_u64(st, ucount, key_off, ns, key_off + ns, tmp);
+  scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
+  uint64_t tot[2] = {0, 0};
+  c->d2h(&tot[0], key_off + ns, 8);
+  c->d2h(&tot[1], blk_off + ns, 8);
+  const uint64_t D = tot[0];
   c->D = D;
+  c->n_blocks = tot[1];
+  const uint64_t nsw = (D + 11) / 12;
   void* keys = c->get("keys", D * kb + 16);
-  launch_p3_compact(st, c->wide, occ, seg_off, ucount, out_off, ns, keys);
-  bbhash_build(c, keys, D);
-  uint64_t nsw = (D + 11) / 12;
-  c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
-  c->sync();
+  uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
+  uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
+  launch_p3_compact(st, c->wide, occ, occ_obs, aux_obs, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, keys, vmask, blocks);
+  c->d2h(hf, flags, 12);
+  if (hf[2]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit (flags " + std::to_string(hf[2]) + ")");
   c->stage = 4;
 }
 
+// P4: the joined masks, already in MPHF-value order, become packed states.
 void do_p4(cf1g_ctx* c) {
-  hipStream_t st = c->st;
-  uint32_t chunk = c->prm.p4_chunk ? c->prm.p4_chunk : 8192;
-  uint64_t npos = c->n_pos > 1 ? c->n_pos - 1 : 0;
-  uint64_t nch = (npos + chunk - 1) / chunk;
-  if (nch > 0xFFFFFFFFULL) fail(CF1G_E_INVALID, "stream too long for the P4 chunk size");
-  uint32_t per_cu = c->prm.p4_wgs_per_cu ? c->prm.p4_wgs_per_cu : 8;
-  uint64_t grid = (uint64_t)c->cu_count * per_cu;
-  if (grid > nch) grid = nch;
-  unsigned int* ctr = c->getT<unsigned int>("p4_counter", 2);
-  c->zero(ctr, 4);
   uint64_t nsw = (c->D + 11) / 12;
-  launch_p4_classify(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
-                     c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), ctr, chunk, (uint32_t)nch,
-                     (uint32_t)(grid ? grid : 1));
+  launch_p4_pack(c->st, c->getT<uint16_t>("vmask", 12 * nsw + 16), c->D, c->getT<uint64_t>("states", nsw + 1));
   c->sync();
   c->stage = 5;
 }

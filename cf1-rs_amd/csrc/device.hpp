@@ -197,6 +197,28 @@ __device__ __forceinline__ uint64_t mphf_segment(const MphfView& v, const KeyT& 
   uint32_t bin = v.b2b[bucket_hash((uint64_t)minimizer_value(c, v.K, v.m))];
   return ((uint64_t)bin << v.sb) | key_top(c, 2 * v.K, v.sb);
 }
+// Rank of a present key inside its segment (segment already known).
+template <typename KeyT>
+__device__ __forceinline__ uint64_t mphf_local_rank(const uint64_t* __restrict__ base, uint64_t t, const KeyT& c) {
+  uint64_t off = 0;
+  for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
+    const uint64_t n = bbh_level_n(t);
+    const uint64_t pos = off + __umul64hi(bbh_hash(c, bbh_seed(l)), n);
+    const uint64_t q = pos / BBH_BLOCK_BITS;
+    const uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
+    const uint64_t* B = base + q * 8;
+    const uint32_t wi = b >> 6;
+    const uint64_t wd = B[1 + wi];
+    if ((wd >> (b & 63)) & 1) {
+      uint64_t r = B[0] + (uint64_t)__popcll(wd & ((1ULL << (b & 63)) - 1));
+      for (uint32_t u = 0; u < wi; u++) r += (uint64_t)__popcll(B[1 + u]);
+      return r;
+    }
+    off += n;
+    t = bbh_next_t(t);
+  }
+  return ~0ULL;
+}
 // Exact value of a present key; for an absent key, some value (0 if it
 // misses every level — the reference maps absent keys to 0, mphf.rs:269).
 template <typename KeyT>
@@ -235,74 +257,45 @@ __host__ __device__ __forceinline__ uint32_t st_class(uint32_t s) {  // STATE_CL
   if ((s >= 8 && s <= 11) || s == 14) return CLS_SIMO;
   return CLS_SISO;
 }
-__host__ __device__ __forceinline__ uint32_t st_front(uint32_t s) {  // DECODE_TABLE front
-  if (s >= 8 && s <= 11) return s - 8;
-  if (s >= 16) return (s >> 2) & 3;
-  return 0;
-}
-__host__ __device__ __forceinline__ uint32_t st_back(uint32_t s) {
-  if (s >= 4 && s <= 7) return s - 4;
-  if (s >= 16) return s & 3;
-  return 0;
-}
-__host__ __device__ __forceinline__ bool st_dead_end(uint32_t s) { return s != 0 && st_class(s) == CLS_MIMO; }
-// from_vertex for a visited, not-outputted vertex of class c.
-__host__ __device__ __forceinline__ uint32_t st_encode(uint32_t c, uint32_t f, uint32_t b) {
-  switch (c) {
-    case CLS_SISO: return 16u | (f << 2) | b;
-    case CLS_MISO: return 4u | b;
-    case CLS_SIMO: return 8u | f;
-    default: return 3u;
-  }
-}
-
-// One observation of a k-mer occurrence (the classify.rs:147-518 cases).
+// Occurrence types of classify.rs:65-177: isolated or MIMO self loop,
+// leftmost, rightmost (also the inverted-repeat self loop), internal.
 enum : uint32_t { OBS_MIMO = 0, OBS_LEFT = 1, OBS_RIGHT = 2, OBS_INTERNAL = 3 };
-// New state for old state s under an observation; returns s when nothing
-// changes. Restates process_isolated/loop/leftmost/rightmost/internal.
-__host__ __device__ __forceinline__ uint32_t dfa_next(uint32_t s, uint32_t type, bool fwd, uint32_t pb, uint32_t nb) {
-  if (st_dead_end(s)) return s;
-  if (type == OBS_MIMO) return 3u;
-  uint32_t cls = st_class(s), f = st_front(s), b = st_back(s);
-  bool vis = s != 0;
-  if (type == OBS_LEFT) {  // classify.rs:199-278
-    if (fwd) {
-      if (!vis) return st_encode(CLS_MISO, 0, nb);
-      if (cls == CLS_SISO) return b == nb ? st_encode(CLS_MISO, f, b) : 3u;
-      if (cls == CLS_MISO) return b != nb ? 3u : s;
-      return 3u;
-    }
-    uint32_t cn = 3 - nb;
-    if (!vis) return st_encode(CLS_SIMO, cn, 0);
-    if (cls == CLS_SISO) return f == cn ? st_encode(CLS_SIMO, f, b) : 3u;
-    if (cls == CLS_MISO) return 3u;
-    return f != cn ? 3u : s;
+// ------------------------------------------------------------ observation masks
+// The DFA of classify.rs:147-518 is a join: a state is a function of the set
+// of in-labels and the set of out-labels observed for the k-mer (canonical
+// orientation; "none" marks a missing neighbour or a forced MIMO), and every
+// transition adds one (in, out) pair. An observation is therefore a 10-bit
+// mask (bits 0-4 in-label A,C,G,T,none; bits 5-9 out-label) and the joined
+// state of a k-mer is the OR of its occurrences' masks.
+constexpr uint32_t LBL_NONE = 4;
+__host__ __device__ __forceinline__ uint32_t obs_mask(uint32_t type, bool fwd, uint32_t pb, uint32_t nb) {
+  uint32_t a, b;
+  if (type == OBS_MIMO) {
+    a = LBL_NONE;
+    b = LBL_NONE;
+  } else if (type == OBS_LEFT) {
+    a = fwd ? LBL_NONE : 3 - nb;
+    b = fwd ? nb : LBL_NONE;
+  } else if (type == OBS_RIGHT) {
+    a = fwd ? pb : LBL_NONE;
+    b = fwd ? LBL_NONE : 3 - pb;
+  } else {
+    a = fwd ? pb : 3 - nb;
+    b = fwd ? nb : 3 - pb;
   }
-  if (type == OBS_RIGHT) {  // classify.rs:297-376
-    if (fwd) {
-      if (!vis) return st_encode(CLS_SIMO, pb, 0);
-      if (cls == CLS_SISO) return f == pb ? st_encode(CLS_SIMO, f, b) : 3u;
-      if (cls == CLS_MISO) return 3u;
-      return f != pb ? 3u : s;
-    }
-    uint32_t cp = 3 - pb;
-    if (!vis) return st_encode(CLS_MISO, 0, cp);
-    if (cls == CLS_SISO) return b == cp ? st_encode(CLS_MISO, f, b) : 3u;
-    if (cls == CLS_MISO) return b != cp ? 3u : s;
-    return 3u;
-  }
-  // OBS_INTERNAL, classify.rs:406-511: in-label a, out-label c in canonical orientation.
-  uint32_t a = fwd ? pb : 3 - nb;
-  uint32_t c = fwd ? nb : 3 - pb;
-  if (!vis) return st_encode(CLS_SISO, a, c);
-  if (cls == CLS_SISO) {
-    if (f == a && b == c) return s;
-    if (f != a && b != c) return 3u;
-    if (f != a) return st_encode(CLS_MISO, f, b);
-    return st_encode(CLS_SIMO, f, b);
-  }
-  if (cls == CLS_MISO) return b != c ? 3u : s;
-  return f != a ? 3u : s;
+  return (1u << a) | (1u << (5 + b));
+}
+// State::from_vertex (state.rs:92-113) of the joined label sets.
+__host__ __device__ __forceinline__ uint32_t mask_state(uint32_t m) {
+  if (m == 0) return 0;
+  const uint32_t I = m & 31u, O = (m >> 5) & 31u;
+  const bool si = I != 0 && (I & (I - 1)) == 0 && I != 16u;
+  const bool so = O != 0 && (O & (O - 1)) == 0 && O != 16u;
+  const uint32_t f = si ? (uint32_t)__builtin_ctz(I) : 0u, b = so ? (uint32_t)__builtin_ctz(O) : 0u;
+  if (si && so) return 16u | (f << 2) | b;
+  if (so) return 4u | b;
+  if (si) return 8u | f;
+  return 3u;
 }
 
 // ------------------------------------------------------------ wave helpers (wave64)

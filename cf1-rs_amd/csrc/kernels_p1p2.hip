@@ -135,8 +135,11 @@ __global__ void __launch_bounds__(MZ_WG) k_p1_hist(const uint64_t* __restrict__ 
 }
 
 // P2 route: each super k-mer [i, i+n) becomes one fixed-size record in its
-// bin: its n-1+K bases MSB-first in RW words, base count in the low byte of
-// the last word (bin record format of superkmer.rs:181-202, fixed width).
+// bin (the bin record of superkmer.rs:181-202 in fixed width): its L = n-1+K
+// bases plus one flanking base on each side, MSB-first in RW words; the last
+// word's low byte holds L and bits 8/9 say whether the left/right flank is a
+// real base (not a placeholder or record end). The flanks carry the
+// neighbours P4's observations need (classify.rs:76-77).
 template <int RW>
 __global__ void __launch_bounds__(MZ_WG) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                     uint64_t n_pos, int K, int m, const uint16_t* __restrict__ b2b,
@@ -181,17 +184,20 @@ __global__ void __launch_bounds__(MZ_WG) k_p2_route(const uint64_t* __restrict__
       atomicOr(overflow, 1u);
       continue;
     }
+    // bases [i-1, i+L]: the super k-mer with one flanking base each side
+    const uint32_t LS = L + 2;
     uint64_t wv[RW];
 #pragma unroll
     for (int t = 0; t < RW; t++) {
       int lo_bit = 64 * t;  // record bit offset of this word
-      uint64_t x = stream_bits64(seq, 2 * i + (uint64_t)lo_bit);
-      int keep = (int)(2 * L) - lo_bit;  // record bits living in this word
+      uint64_t x = stream_bits64(seq, 2 * (i - 1) + (uint64_t)lo_bit);
+      int keep = (int)(2 * LS) - lo_bit;  // record bits living in this word
       if (keep <= 0) x = 0;
       else if (keep < 64) x &= ~0ULL << (64 - keep);
       wv[t] = x;
     }
-    wv[RW - 1] |= (uint64_t)L;
+    const uint64_t lf = is_n(nmask, i - 1) ? 0 : 1, rf = is_n(nmask, i + L) ? 0 : 1;
+    wv[RW - 1] |= (uint64_t)L | (lf << 8) | (rf << 9);
     uint64_t* dst = rec + out * RW;
 #pragma unroll
     for (int t = 0; t < RW; t++) dst[t] = wv[t];

@@ -3,13 +3,18 @@
 // replacement for ph::fmph, mphf.rs:144-190).
 //
 // P3a layout: the k-mers of bin b are split by the top `sb` bits of the
-// canonical key into 2^sb segments (b, s). Segments are counted (k_p3_count),
-// scattered into one occurrence array (k_p3_scatter), then each segment is
-// sorted and deduplicated by one work-group (LDS radix sort, or an LDS
-// bitmap when only <= 18 key bits vary); segments too large for LDS go to a
-// global-memory radix sort (k_p3_bigsort). Uniques are compacted in
-// (bin, segment) order, so every bin's keys come out sorted, bins
-// concatenated: the order of the reference's cf1_dedup_all.bin.
+// canonical key into 2^sb segments (b, s). Segments are counted (k_p3_count)
+// and scattered into one occurrence array with each occurrence's 10-bit
+// observation mask (k_p3_scatter). k_p3_dedup sorts and deduplicates one
+// segment per work-group (LDS bucket sort, LDS radix sort for uneven
+// buckets, or a direct-address table when <= 13 key bits vary), OR-joins
+// the masks of equal keys and sizes the segment's BBHash; segments above
+// the LDS capacity take a global-memory radix sort (k_p3_bigsort). Keys are
+// then compacted in (bin, segment) order, so every bin's keys come out
+// sorted, bins concatenated: the order of the reference's
+// cf1_dedup_all.bin. The BBHash of each segment is built in the dedup
+// kernel (k_bbh_big for segments above register capacity) into padded block
+// slots that the compaction packs densely.
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -149,20 +154,45 @@ struct Roll<Key128> {
   __device__ Key128 canon() const { return key_lt(r, x) ? r : x; }
 };
 
-// Iterate the canonical k-mers of one record.
-template <typename KeyT, int RW, typename F>
-__device__ __forceinline__ void for_each_rec_kmer(const uint64_t* __restrict__ rec, uint64_t idx, int K, F f) {
+// Iterate the k-mer occurrences of one record: f(canonical, observation mask).
+// Record positions: 0 = left flank, 1..L = super k-mer, L+1 = right flank.
+// The observation of k-mer j follows classify.rs:65-177: neighbours are the
+// bases at j and j+K+1 (valid inside the record, or by the flank flags),
+// isolated -> MIMO, no right -> rightmost, self loop (canonical of the next
+// k-mer equal) -> MIMO or rightmost, else leftmost / internal.
+template <typename KeyT, int RW, bool OBS, typename F>
+__device__ __forceinline__ void for_each_rec_occ(const uint64_t* __restrict__ rec, uint64_t idx, int K, F f) {
   uint64_t w[RW];
 #pragma unroll
   for (int t = 0; t < RW; t++) w[t] = rec[idx * RW + t];
-  int L = (int)(w[RW - 1] & 255);
-  if (L > 32 * RW - 4) L = 32 * RW - 4;  // record capacity (bases above the count byte)
+  const uint64_t tail = w[RW - 1];
+  int L = (int)(tail & 255);
+  if (L > 32 * RW - 6) L = 32 * RW - 6;  // record capacity (bases above the flag bits)
+  const bool lf = (tail >> 8) & 1, rf = (tail >> 9) & 1;
+  const int n = L - K + 1;
+  if (n <= 0) return;
   Roll<KeyT> ro;
   ro.init(K);
-  for (int t = 0; t < K - 1; t++) ro.push(rec_base<RW>(w, t));
-  for (int t = K - 1; t < L; t++) {
-    ro.push(rec_base<RW>(w, t));
-    f(ro.canon());
+  for (int t = 1; t <= K; t++) ro.push(rec_base<RW>(w, t));
+  for (int j = 0; j < n; j++) {
+    const KeyT c = ro.canon();
+    if (OBS) {
+      const bool pv = j > 0 || lf, nv = j < n - 1 || rf;
+      const uint32_t pb = rec_base<RW>(w, j), nb = rec_base<RW>(w, j + K + 1);
+      const bool fwd = key_eq(ro.x, c);
+      Roll<KeyT> nx = ro;
+      nx.push(nb);
+      uint32_t type;
+      if (!pv && !nv) type = OBS_MIMO;
+      else if (!nv) type = OBS_RIGHT;
+      else if (key_eq(nx.canon(), c)) type = (!pv || key_eq(ro.x, nx.x)) ? OBS_MIMO : OBS_RIGHT;
+      else type = pv ? OBS_INTERNAL : OBS_LEFT;
+      f(c, obs_mask(type, fwd, pb, nb));
+      ro = nx;
+    } else {
+      f(c, 0u);
+      if (j + 1 < n) ro.push(rec_base<RW>(w, j + K + 1));
+    }
   }
 }
 
@@ -201,7 +231,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
   for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
   __syncthreads();
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_kmer<KeyT, RW>(rec, r, K, [&](const KeyT& c) { atomicAdd(&hist[key_top(c, 2 * K, sb)], 1u); });
+    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[key_top(c, 2 * K, sb)], 1u); });
   __syncthreads();
   uint64_t* dst = counts + (uint64_t)bin * ns;
   for (int s = threadIdx.x; s < ns; s += P3_WG)
@@ -211,7 +241,8 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
 template <typename KeyT, int RW>
 __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
                                                       const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
-                                                      int K, int sb, uint64_t* __restrict__ cursor, KeyT* __restrict__ occ) {
+                                                      int K, int sb, uint64_t* __restrict__ cursor, KeyT* __restrict__ occ,
+                                                      uint16_t* __restrict__ occ_obs) {
   __shared__ uint32_t hist[1 << P3_MAX_SB];
   __shared__ unsigned long long gb[1 << P3_MAX_SB];
   int bin;
@@ -221,17 +252,18 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
   __syncthreads();
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_kmer<KeyT, RW>(rec, r, K, [&](const KeyT& c) { atomicAdd(&hist[key_top(c, 2 * K, sb)], 1u); });
+    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[key_top(c, 2 * K, sb)], 1u); });
   __syncthreads();
   uint64_t* cur = cursor + (uint64_t)bin * ns;
   for (int s = threadIdx.x; s < ns; s += P3_WG)
     if (hist[s]) gb[s] = atomicAdd((unsigned long long*)&cur[s], (unsigned long long)hist[s]);
   __syncthreads();
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_kmer<KeyT, RW>(rec, r, K, [&](const KeyT& c) {
+    for_each_rec_occ<KeyT, RW, true>(rec, r, K, [&](const KeyT& c, uint32_t m) {
       uint32_t s = key_top(c, 2 * K, sb);
-      uint32_t slot = atomicSub(&hist[s], 1u) - 1u;
-      occ[gb[s] + slot] = c;
+      uint64_t at = gb[s] + (atomicSub(&hist[s], 1u) - 1u);
+      occ[at] = c;
+      occ_obs[at] = (uint16_t)m;
     });
 }
 
@@ -314,326 +346,57 @@ __device__ __forceinline__ void block_or2(uint64_t& a, uint64_t& b, uint64_t* sh
   __syncthreads();
 }
 
-// ------------------------------------------------------------------ segment sort
+// ------------------------------------------------------------------ segment dedup
+// Per-segment capacity of the register/LDS paths: keys are held KPT per
+// thread. Larger segments take the global-memory paths (k_p3_bigsort,
+// k_bbh_big).
 template <typename KeyT>
 struct SegCfg;
 template <>
 struct SegCfg<uint64_t> {
-  static constexpr int KPT = 32;
+  static constexpr int KPT = 16;
 };
 template <>
 struct SegCfg<Key128> {
-  static constexpr int KPT = 16;
+  static constexpr int KPT = 8;
 };
-constexpr int BM_MAX_BITS = 18;  // LDS bitmap dedup when <= 18 low key bits vary
+constexpr int BM_MAX_BITS = 13;   // direct-address mask table when <= 13 low key bits vary
+constexpr int BK_BITS = 12;       // bucket-sort buckets (next 12 key bits below the segment prefix)
+constexpr int BK_N = 1 << BK_BITS;
+constexpr uint32_t BK_MAX = 24;   // larger buckets (duplicate-heavy) switch to the LDS radix sort
 
+// A segment's MPHF is built through global memory (k_bbh_big) when its
+// distinct keys exceed the register capacity, or when it was sorted by
+// k_p3_bigsort (more occurrences than capacity and too many varying key bits
+// for the direct-address table).
 template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_segsort(KeyT* __restrict__ occ, const uint64_t* __restrict__ seg_off, uint64_t n_segs,
-                                                      int K2, int sb, uint64_t* __restrict__ ucount,
-                                                      uint32_t* __restrict__ over_list, unsigned int* __restrict__ n_over) {
-  constexpr int KPT = SegCfg<KeyT>::KPT;
-  constexpr int CAP = RK_WG * KPT;
-  __shared__ KeyT buf[CAP];
-  __shared__ uint32_t cnt[4 * 256];
-  __shared__ uint64_t sh8[8];
-  const uint64_t seg = blockIdx.x;
-  if (seg >= n_segs) return;
-  const uint64_t start = seg_off[seg];
-  const uint64_t n = seg_off[seg + 1] - start;
-  const int rbits = K2 - sb;  // low bits that may vary in this segment
-  if (n == 0) {
-    if (threadIdx.x == 0) ucount[seg] = 0;
-    return;
-  }
-  if (sizeof(KeyT) == 8 && rbits <= BM_MAX_BITS) {
-    // ---- bitmap dedup: any number of duplicates, sorted output.
-    uint64_t* bm = (uint64_t*)buf;  // 2^18 bits = 32 KB <= sizeof(buf)
-    const uint32_t nwords = rbits <= 6 ? 1u : (1u << (rbits - 6));
-    for (uint32_t i = threadIdx.x; i < nwords; i += RK_WG) bm[i] = 0;
-    __syncthreads();
-    const uint64_t lowmask = (rbits >= 64) ? ~0ULL : ((1ULL << rbits) - 1);
-    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) {
-      uint64_t low = key_lo(occ[start + i]) & lowmask;
-      atomicOr((unsigned long long*)&bm[low >> 6], 1ULL << (low & 63));
-    }
-    __syncthreads();
-    const uint32_t wpt = (nwords + RK_WG - 1) / RK_WG;
-    const uint32_t w0 = threadIdx.x * wpt;
-    uint64_t c = 0;
-    for (uint32_t q = 0; q < wpt; q++)
-      if (w0 + q < nwords) c += (uint64_t)__popcll(bm[w0 + q]);
-    uint64_t tot;
-    uint64_t o = block_excl_scan_u64(c, sh8, &tot);
-    const uint64_t prefix = (uint64_t)(seg & ((1ULL << sb) - 1)) << rbits;
-    for (uint32_t q = 0; q < wpt; q++) {
-      if (w0 + q >= nwords) break;
-      uint64_t x = bm[w0 + q];
-      while (x) {
-        int b = __ffsll((long long)x) - 1;
-        x &= x - 1;
-        uint64_t kv = prefix | ((uint64_t)(w0 + q) * 64 + (uint64_t)b);
-        KeyT kk;
-        if constexpr (sizeof(KeyT) == 8) kk = (KeyT)kv;
-        occ[start + o++] = kk;
-      }
-    }
-    if (threadIdx.x == 0) ucount[seg] = tot;
-    return;
-  }
-  if (n > (uint64_t)CAP) {
-    if (threadIdx.x == 0) {
-      unsigned int idx = atomicAdd(n_over, 1u);
-      over_list[idx] = (uint32_t)seg;
-      ucount[seg] = 0;
-    }
-    return;
-  }
-  // ---- LDS LSD radix sort of <= CAP keys.
-  const int kpt = (int)((n + RK_WG - 1) / RK_WG);
-  const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
-  KeyT key[KPT];
-  uint32_t rank[KPT];
-  const KeyT k0 = occ[start];
-  uint64_t xl = 0, xh = 0;
-#pragma unroll
-  for (int s = 0; s < KPT; s++) {
-    if (s < kpt) {
-      uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
-      if (e < n) {
-        key[s] = occ[start + e];
-        key_xor_or(xl, xh, key[s], k0);
-      } else {
-        key[s] = key_pad<KeyT>();
-      }
-    }
-  }
-  block_or2(xl, xh, sh8);
-  const int vb = varying_bits(xl, xh);
-  if (vb == 0) {
-    if (threadIdx.x == 0) {
-      occ[start] = k0;
-      ucount[seg] = 1;
-    }
-    return;
-  }
-  const int passes = (vb + 7) >> 3;
-  for (int p = 0; p < passes; p++) {
-    const int sh = 8 * p;
-    for (int i = threadIdx.x; i < 4 * 256; i += RK_WG) cnt[i] = 0;
-    __syncthreads();
-    rank_tile<KeyT, KPT>(key, kpt, sh, rank, cnt);
-    __syncthreads();
-    rank_offsets(cnt, sh8, nullptr);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < KPT; s++)
-      if (s < kpt) buf[cnt[wv * 256 + key_digit(key[s], sh)] + rank[s]] = key[s];
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < KPT; s++)
-      if (s < kpt) key[s] = buf[(uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane];
-    __syncthreads();
-  }
-  // buf[0..n) is sorted; dedup in thread-contiguous ranges.
-  const uint64_t e0 = (uint64_t)threadIdx.x * kpt;
-  uint64_t c = 0;
-  for (int q = 0; q < kpt; q++) {
-    uint64_t e = e0 + q;
-    if (e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) c++;
-  }
-  uint64_t tot;
-  uint64_t o = block_excl_scan_u64(c, sh8, &tot);
-  for (int q = 0; q < kpt; q++) {
-    uint64_t e = e0 + q;
-    if (e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) occ[start + o++] = buf[e];
-  }
-  if (threadIdx.x == 0) ucount[seg] = tot;
+__device__ __forceinline__ bool seg_is_big(uint64_t n, uint64_t u, int rbits) {
+  constexpr uint64_t CAP = (uint64_t)RK_WG * SegCfg<KeyT>::KPT;
+  const bool table = sizeof(KeyT) == 8 && rbits <= BM_MAX_BITS;
+  return u > CAP || (n > CAP && !table);
 }
 
-// Oversized segments: one work-group per segment runs a stable LSD radix
-// sort through global memory (ping-pong with `aux`), tile by tile, then
-// deduplicates into occ[start...].
-template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, KeyT* __restrict__ aux,
-                                                      const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ over_list,
-                                                      uint64_t* __restrict__ ucount) {
-  constexpr int KPT = SegCfg<KeyT>::KPT;
-  constexpr int CAP = RK_WG * KPT;
-  __shared__ uint32_t cnt[4 * 256];
-  __shared__ unsigned long long dbase[256];
-  __shared__ uint32_t ttot[256];
-  __shared__ uint64_t sh8[8];
-  __shared__ KeyT carry;
-  const uint32_t seg = over_list[blockIdx.x];
-  const uint64_t start = seg_off[seg];
-  const uint64_t n = seg_off[seg + 1] - start;
-  const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
-  KeyT* src = occ + start;
-  KeyT* dst = aux + start;
-  const KeyT k0 = src[0];
-  uint64_t xl = 0, xh = 0;
-  for (uint64_t i = threadIdx.x; i < n; i += RK_WG) key_xor_or(xl, xh, src[i], k0);
-  block_or2(xl, xh, sh8);
-  const int vb = varying_bits(xl, xh);
-  const int passes = (vb + 7) >> 3;
-  KeyT key[KPT];
-  uint32_t rank[KPT];
-  for (int p = 0; p < passes; p++) {
-    const int sh = 8 * p;
-    // digit histogram of the whole segment
-    for (int i = threadIdx.x; i < 256; i += RK_WG) ttot[i] = 0;
-    __syncthreads();
-    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) atomicAdd(&ttot[key_digit(src[i], sh)], 1u);
-    __syncthreads();
-    {
-      uint64_t ex = block_excl_scan_u64(ttot[threadIdx.x], sh8, nullptr);
-      dbase[threadIdx.x] = ex;
-    }
-    __syncthreads();
-    for (uint64_t t0 = 0; t0 < n; t0 += CAP) {
-      uint64_t tn = n - t0 < (uint64_t)CAP ? n - t0 : (uint64_t)CAP;
-      int kpt = (int)((tn + RK_WG - 1) / RK_WG);
-#pragma unroll
-      for (int s = 0; s < KPT; s++) {
-        if (s < kpt) {
-          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
-          key[s] = e < tn ? src[t0 + e] : key_pad<KeyT>();
-        }
-      }
-      for (int i = threadIdx.x; i < 4 * 256; i += RK_WG) cnt[i] = 0;
-      __syncthreads();
-      rank_tile<KeyT, KPT>(key, kpt, sh, rank, cnt);
-      __syncthreads();
-      rank_offsets(cnt, sh8, ttot);
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < KPT; s++) {
-        if (s < kpt) {
-          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
-          if (e < tn) {
-            uint32_t d = key_digit(key[s], sh);
-            uint32_t local = cnt[wv * 256 + d] + rank[s];
-            // position inside the tile's digit-d run = local - (start of digit d in tile)
-            uint32_t dstart = cnt[d];  // wave 0 base of digit d == tile start of digit d
-            dst[dbase[d] + (local - dstart)] = key[s];
-          }
-        }
-      }
-      __syncthreads();
-      dbase[threadIdx.x] += ttot[threadIdx.x];
-      __syncthreads();
-    }
-    KeyT* t = src;
-    src = dst;
-    dst = t;
-    __syncthreads();
-  }
-  // dedup sorted src into occ[start...] (src may be occ itself: writes trail reads)
-  KeyT* out = occ + start;
-  uint64_t u = 0;
-  for (uint64_t t0 = 0; t0 < n; t0 += CAP) {
-    uint64_t tn = n - t0 < (uint64_t)CAP ? n - t0 : (uint64_t)CAP;
-    int kpt = (int)((tn + RK_WG - 1) / RK_WG);
-    // thread-contiguous: thread t owns [t*kpt, (t+1)*kpt)
-#pragma unroll
-    for (int s = 0; s < KPT; s++) {
-      uint64_t e = (uint64_t)threadIdx.x * kpt + s;
-      if (s < kpt && e < tn) key[s] = src[t0 + e];
-    }
-    KeyT prev_carry = carry;
-    __syncthreads();
-    uint64_t c = 0;
-#pragma unroll
-    for (int s = 0; s < KPT; s++) {
-      uint64_t e = (uint64_t)threadIdx.x * kpt + s;
-      if (s < kpt && e < tn) {
-        bool first;
-        if (s > 0) first = !key_eq(key[s], key[s - 1]);
-        else if (e == 0) first = (t0 == 0) || !key_eq(key[s], prev_carry);
-        else first = !key_eq(key[s], src[t0 + e - 1]);
-        rank[s] = first ? 1u : 0u;
-        c += first;
-      }
-    }
-    uint64_t tot;
-    uint64_t o = block_excl_scan_u64(c, sh8, &tot);
-    // all reads of this tile happened before the scan's barriers
-#pragma unroll
-    for (int s = 0; s < KPT; s++) {
-      uint64_t e = (uint64_t)threadIdx.x * kpt + s;
-      if (s < kpt && e < tn && rank[s]) out[u + o++] = key[s];
-      if (s < kpt && e + 1 == tn) carry = key[s];
-    }
-    u += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) ucount[seg] = u;
-}
+// Each segment builds its BBHash blocks into a padded slot of
+// floor(n/64) + 16 blocks (n = the segment's occurrences) at block offset
+// floor(seg_off/64) + 16*seg; k_p3_compact then packs them densely. The
+// slot bounds the region: sum_l roundup64(2 t_l) over t_{l+1} = ceil(9 t_l/20)
+// is < 3.7u + 67L bits, i.e. < u/121 + 10 blocks for L <= 64 levels.
+constexpr uint64_t PAD_BLOCKS = 16;
+__device__ __forceinline__ uint64_t pad_slot_off(uint64_t seg_start, uint64_t seg) { return seg_start / 64 + PAD_BLOCKS * seg; }
+__device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_occ / 64 + PAD_BLOCKS; }
 
-template <typename KeyT>
-__global__ void __launch_bounds__(256) k_p3_compact(const KeyT* __restrict__ occ, const uint64_t* __restrict__ seg_off,
-                                                    const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ out_off,
-                                                    uint64_t n_segs, KeyT* __restrict__ keys) {
-  uint64_t seg = blockIdx.x;
-  if (seg >= n_segs) return;
-  uint64_t u = ucount[seg], s = seg_off[seg], o = out_off[seg];
-  for (uint64_t i = threadIdx.x; i < u; i += 256) keys[o + i] = occ[s + i];
-}
-
-// ------------------------------------------------------------------ BBHash build
-// One work-group per P3a segment builds that segment's BBHash (see
-// device.hpp) from its sorted distinct keys: keys live in registers, the
-// level's seen/collision bit arrays in LDS. MODE 0 only measures the
-// segment's block count; after a scan of those counts MODE 1 rebuilds the
-// same levels and writes the blocks with their segment-local rank words.
-// Segments with more keys than fit in registers (> CAP) are handed to
-// k_bbh_big through big_list.
-template <int MODE, typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_bbh_seg(const KeyT* __restrict__ keys, const uint64_t* __restrict__ key_off,
-                                                   uint64_t n_segs, uint64_t* __restrict__ seg_nblk,
-                                                   const uint64_t* __restrict__ blk_off, uint64_t* __restrict__ blocks,
-                                                   uint32_t* __restrict__ big_list, unsigned int* __restrict__ n_big,
-                                                   unsigned int* __restrict__ err) {
-  constexpr int KPT = SegCfg<KeyT>::KPT;
-  constexpr int CAP = RK_WG * KPT;
-  constexpr int SEEN_W = (2 * CAP) / 64 + 1;
-  constexpr int REG_W = (4 * CAP + 8192) / 64;
-  __shared__ uint64_t seen[SEEN_W];
-  __shared__ uint64_t col[SEEN_W];
-  __shared__ uint64_t region[MODE ? REG_W : 1];
-  __shared__ uint64_t sh8[8];
-  const uint64_t seg = blockIdx.x;
-  if (seg >= n_segs) return;
-  const uint64_t ko = key_off[seg];
-  const uint64_t D = key_off[seg + 1] - ko;
-  if (D == 0) {
-    if (MODE == 0 && threadIdx.x == 0) seg_nblk[seg] = 0;
-    return;
-  }
-  if (D > (uint64_t)CAP) {
-    if (MODE == 0 && threadIdx.x == 0) {
-      big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
-      seg_nblk[seg] = 0;
-    }
-    return;
-  }
-  KeyT key[KPT];
-  uint32_t alive = 0;
-#pragma unroll
-  for (int s = 0; s < KPT; s++) {
-    uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
-    if (e < D) {
-      key[s] = keys[ko + e];
-      alive |= 1u << s;
-    }
-  }
-  if (MODE)
-    for (int i = threadIdx.x; i < REG_W; i += RK_WG) region[i] = 0;
+// ---- BBHash levels over keys held in registers (see device.hpp), key slot s
+// of thread t being element s*RK_WG + t. Returns the region size in bits;
+// MODE 1 also writes the level words into `region` and each element's bit
+// position into ppos[element] (LDS).
+template <int MODE, typename KeyT, int KPT>
+__device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seen, uint64_t* col, uint64_t* region,
+                               uint64_t max_bits, uint32_t* ppos, uint64_t* sh8, unsigned int* err) {
   uint64_t off = 0, t = D, r = D;
   int l = 0;
   while (r > 0) {
     const uint64_t n = bbh_level_n(t);
-    if (l >= BBH_MAX_LEVELS || off + n > (uint64_t)REG_W * 64) {
+    if (l >= BBH_MAX_LEVELS || off + n > max_bits) {
       if (threadIdx.x == 0) atomicOr(err, 1u);
       break;
     }
@@ -661,8 +424,12 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_seg(const KeyT* __restrict__ keys
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
         uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
-        if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) alive &= ~(1u << s);
-        else cnt++;
+        if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
+          alive &= ~(1u << s);
+          if (MODE) ppos[s * RK_WG + threadIdx.x] = (uint32_t)(off + pos);
+        } else {
+          cnt++;
+        }
       }
     }
     block_excl_scan_u64(cnt, sh8, &r);  // r = keys still unplaced (barriers inside)
@@ -670,61 +437,526 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_seg(const KeyT* __restrict__ keys
     t = bbh_next_t(t);
     l++;
   }
-  const uint64_t nblk = (off + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;
-  if (MODE == 0) {
-    if (threadIdx.x == 0) seg_nblk[seg] = nblk;
+  return off;
+}
+
+// Sort + dedup of one segment, then the size of its BBHash. Each occurrence
+// carries its 10-bit observation mask; the masks of equal keys are OR-joined
+// (the DFA join of classify.rs:147-518), and the segment's BBHash is built.
+// Output in place: sorted distinct keys at occ[start...], joined masks in
+// local-rank order at obs[start...], their count in ucount, the BBHash
+// blocks in the segment's padded slot of blocks_pad and their count in
+// nblk. Oversized segments go to over_list (sorted by k_p3_bigsort);
+// segments whose distinct keys exceed the register capacity go to big_list
+// (BBHash built by k_bbh_big).
+template <typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+                                                    const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K2, int sb,
+                                                    uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
+                                                    uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
+                                                    uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad) {
+  constexpr int KPT = SegCfg<KeyT>::KPT;
+  constexpr int CAP = RK_WG * KPT;
+  constexpr int SEEN_W = (2 * CAP) / 64 + 1;
+  __shared__ KeyT buf[CAP];
+  __shared__ uint16_t mbuf[CAP];
+  __shared__ uint32_t cnt[BK_N];
+  __shared__ uint64_t seen[SEEN_W];
+  __shared__ uint64_t col[SEEN_W];
+  __shared__ uint64_t sh8[8];
+  unsigned int* n_over = flags;
+  unsigned int* n_big = flags + 1;
+  unsigned int* err = flags + 2;
+  const uint64_t seg = blockIdx.x;
+  if (seg >= n_segs) return;
+  const uint64_t start = seg_off[seg];
+  const uint64_t n = seg_off[seg + 1] - start;
+  const int rbits = K2 - sb;  // low bits that may vary in this segment
+  uint64_t u = 0;             // distinct keys written to occ[start...]
+  if (n == 0) {
+    if (threadIdx.x == 0) {
+      ucount[seg] = 0;
+      nblk[seg] = 0;
+    }
     return;
   }
+  if (sizeof(KeyT) == 8 && rbits <= BM_MAX_BITS) {
+    // ---- direct-address table of masks: any number of duplicates, sorted output.
+    uint32_t* tab = (uint32_t*)buf;  // 2^13 u32 = 32 KB = sizeof(buf)
+    const uint32_t nt = 1u << rbits;
+    for (uint32_t i = threadIdx.x; i < nt; i += RK_WG) tab[i] = 0;
+    __syncthreads();
+    const uint64_t lowmask = (1ULL << rbits) - 1;
+    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) atomicOr(&tab[key_lo(occ[start + i]) & lowmask], (uint32_t)obs[start + i]);
+    __syncthreads();
+    const uint32_t per = (nt + RK_WG - 1) / RK_WG;
+    const uint32_t t0 = threadIdx.x * per;
+    uint64_t c = 0;
+    for (uint32_t q = 0; q < per; q++)
+      if (t0 + q < nt && tab[t0 + q]) c++;
+    uint64_t tot;
+    uint64_t o = block_excl_scan_u64(c, sh8, &tot);
+    const uint64_t prefix = (uint64_t)(seg & ((1ULL << sb) - 1)) << rbits;
+    for (uint32_t q = 0; q < per; q++) {
+      if (t0 + q >= nt) break;
+      uint32_t m = tab[t0 + q];
+      if (!m) continue;
+      if constexpr (sizeof(KeyT) == 8) occ[start + o] = (KeyT)(prefix | (uint64_t)(t0 + q));
+      obs[start + o] = (uint16_t)m;
+      o++;
+    }
+    u = tot;
+  } else if (n > (uint64_t)CAP) {
+    if (threadIdx.x == 0) {
+      over_list[atomicAdd(n_over, 1u)] = (uint32_t)seg;
+      ucount[seg] = 0;
+      nblk[seg] = 0;
+    }
+    return;
+  } else {
+    // ---- bucket sort on the 12 key bits below the segment prefix.
+    const int kpt = (int)((n + RK_WG - 1) / RK_WG);
+    KeyT key[KPT];
+    uint32_t msk[KPT];
+    uint32_t aux[KPT];
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+      if (s < kpt) {
+        if (e < n) {
+          key[s] = occ[start + e];
+          msk[s] = obs[start + e];
+        } else {
+          key[s] = key_pad<KeyT>();
+          msk[s] = 0;
+        }
+      }
+    }
+    const int bsh = rbits > BK_BITS ? rbits - BK_BITS : 0;
+    for (int i = threadIdx.x; i < BK_N; i += RK_WG) cnt[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+      if (s < kpt && e < n) aux[s] = atomicAdd(&cnt[key_shr32(key[s], bsh) & (BK_N - 1)], 1u);
+    }
+    __syncthreads();
+    constexpr int CPT = BK_N / RK_WG;  // counters per thread
+    uint64_t loc = 0, mx = 0;
+    for (int q = 0; q < CPT; q++) {
+      uint32_t v = cnt[threadIdx.x * CPT + q];
+      loc += v;
+      mx = v > mx ? v : mx;
+    }
+    uint64_t dummy = 0;
+    block_or2(mx, dummy, sh8);  // OR of maxima >= max: a conservative "some bucket is large"
+    if (mx <= BK_MAX) {
+      uint64_t run = block_excl_scan_u64(loc, sh8, nullptr);
+      for (int q = 0; q < CPT; q++) {
+        uint32_t v = cnt[threadIdx.x * CPT + q];
+        cnt[threadIdx.x * CPT + q] = (uint32_t)run;
+        run += v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+        if (s < kpt && e < n) {
+          uint32_t at = cnt[key_shr32(key[s], bsh) & (BK_N - 1)] + aux[s];
+          buf[at] = key[s];
+          mbuf[at] = (uint16_t)msk[s];
+        }
+      }
+      __syncthreads();
+      for (int q = 0; q < CPT; q++) {  // insertion sort inside each bucket
+        const int b = threadIdx.x * CPT + q;
+        const uint32_t a0 = cnt[b], a1 = (b + 1 < BK_N) ? cnt[b + 1] : (uint32_t)n;
+        for (uint32_t i = a0 + 1; i < a1; i++) {
+          KeyT kv = buf[i];
+          uint16_t mv = mbuf[i];
+          uint32_t j = i;
+          while (j > a0 && key_lt(kv, buf[j - 1])) {
+            buf[j] = buf[j - 1];
+            mbuf[j] = mbuf[j - 1];
+            j--;
+          }
+          buf[j] = kv;
+          mbuf[j] = mv;
+        }
+      }
+      __syncthreads();
+    } else {
+      // ---- LDS LSD radix sort (buckets too uneven for insertion sorts).
+      // Keys are reloaded in the ranking's element order e = wave*64*kpt +
+      // slot*64 + lane, so that the pads (e >= n) come after every key and
+      // stay there: they tie with keys whose varying bits are all ones.
+      const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
+      uint64_t xl = 0, xh = 0;
+      const KeyT k0 = occ[start];
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if (s < kpt) {
+          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
+          key[s] = e < n ? occ[start + e] : key_pad<KeyT>();
+          msk[s] = e < n ? (uint32_t)obs[start + e] : 0u;
+          if (e < n) key_xor_or(xl, xh, key[s], k0);
+        }
+      }
+      block_or2(xl, xh, sh8);
+      const int passes = (varying_bits(xl, xh) + 7) >> 3;
+      for (int p = 0; p < passes; p++) {
+        const int sh = 8 * p;
+        for (int i = threadIdx.x; i < 4 * 256; i += RK_WG) cnt[i] = 0;
+        __syncthreads();
+        rank_tile<KeyT, KPT>(key, kpt, sh, aux, cnt);
+        __syncthreads();
+        rank_offsets(cnt, sh8, nullptr);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < KPT; s++) {
+          if (s < kpt) {
+            uint32_t at = cnt[wv * 256 + key_digit(key[s], sh)] + aux[s];
+            buf[at] = key[s];
+            mbuf[at] = (uint16_t)msk[s];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < KPT; s++) {
+          if (s < kpt) {
+            uint32_t e = (uint32_t)(wv * 64 * kpt + s * 64 + lane);
+            key[s] = buf[e];
+            msk[s] = mbuf[e];
+          }
+        }
+        __syncthreads();
+      }
+      if (passes == 0) {  // one distinct key
+#pragma unroll
+        for (int s = 0; s < KPT; s++) {
+          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
+          if (s < kpt) {
+            buf[e] = key[s];
+            mbuf[e] = (uint16_t)msk[s];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // buf[0..n) sorted; a run's head joins the masks of its run.
+    const uint64_t e0 = (uint64_t)threadIdx.x * kpt;
+    uint64_t c = 0;
+    for (int q = 0; q < kpt; q++) {
+      uint64_t e = e0 + q;
+      if (e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) c++;
+    }
+    uint64_t tot;
+    uint64_t o = block_excl_scan_u64(c, sh8, &tot);
+    for (int q = 0; q < kpt; q++) {
+      uint64_t e = e0 + q;
+      if (e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) {
+        uint32_t m = mbuf[e];
+        for (uint64_t f = e + 1; f < n && key_eq(buf[f], buf[e]); f++) m |= mbuf[f];
+        occ[start + o] = buf[e];
+        obs[start + o] = (uint16_t)m;
+        o++;
+      }
+    }
+    u = tot;
+  }
+  // ---- BBHash of the segment, into its padded block slot; then the joined
+  // masks are permuted in place into local-rank order (obs[start + rank]).
+  if (threadIdx.x == 0) ucount[seg] = u;
+  if (seg_is_big<KeyT>(n, u, rbits)) {
+    if (threadIdx.x == 0) {
+      big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
+      nblk[seg] = 0;
+    }
+    return;
+  }
+  constexpr int REG_W = (4 * CAP + 8192) / 64;
+  static_assert(REG_W * 8 <= (int)sizeof(buf) && CAP <= BK_N && REG_W / 7 + 2 <= SEEN_W, "LDS reuse fits");
+  __threadfence_block();
+  __syncthreads();  // every read of buf / cnt above is done: they are reused below
+  // LDS reuse: region (level words) in buf, masks in mbuf, bit positions in
+  // cnt, block ranks in seen (free once the levels are done).
+  uint64_t* region = (uint64_t*)buf;
+  uint32_t* ppos = cnt;
+  uint64_t* brank = seen;
+  KeyT key[KPT];
+  uint32_t alive = 0;
+#pragma unroll
+  for (int s = 0; s < KPT; s++) {
+    uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+    if (e < u) {
+      key[s] = occ[start + e];
+      mbuf[e] = obs[start + e];
+      alive |= 1u << s;
+    }
+  }
+  for (int i = threadIdx.x; i < REG_W; i += RK_WG) region[i] = 0;
+  const uint64_t slot = pad_slot_blocks(n);
+  const uint64_t max_bits = slot * BBH_BLOCK_BITS < (uint64_t)REG_W * 64 ? slot * BBH_BLOCK_BITS : (uint64_t)REG_W * 64;
+  const uint64_t bits = bbh_levels<1, KeyT, KPT>(key, alive, u, seen, col, region, max_bits, ppos, sh8, err);
+  const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < RK_WG
   __syncthreads();
-  // nblk <= REG_W*64/448 + 1 < 256: one block per thread
+  const uint64_t j = threadIdx.x;
   uint64_t w[7];
   uint64_t pc = 0;
-  const uint64_t j = threadIdx.x;
 #pragma unroll
-  for (int u = 0; u < 7; u++) {
-    uint64_t q = j * 7 + u;
-    w[u] = (j < nblk && q < (uint64_t)REG_W) ? region[q] : 0;
-    pc += (uint64_t)__popcll(w[u]);
+  for (int q = 0; q < 7; q++) {
+    uint64_t x = j * 7 + q;
+    w[q] = (j < nb && x < (uint64_t)REG_W) ? region[x] : 0;
+    pc += (uint64_t)__popcll(w[q]);
   }
   uint64_t tot;
-  uint64_t rank = block_excl_scan_u64(j < nblk ? pc : 0, sh8, &tot);
-  if (j < nblk) {
-    uint64_t* B = blocks + (blk_off[seg] + j) * 8;
-    B[0] = rank;
+  const uint64_t rk = block_excl_scan_u64(j < nb ? pc : 0, sh8, &tot);
+  if (j < nb) {
+    brank[j] = rk;
+    uint64_t* B = blocks_pad + (pad_slot_off(start, seg) + j) * 8;
+    B[0] = rk;
 #pragma unroll
-    for (int u = 0; u < 7; u++) B[1 + u] = w[u];
+    for (int q = 0; q < 7; q++) B[1 + q] = w[q];
+  }
+  if (threadIdx.x == 0) nblk[seg] = nb;
+  __syncthreads();
+  for (uint64_t e = threadIdx.x; e < u; e += RK_WG) {
+    const uint32_t p = ppos[e];
+    const uint32_t qb = p / (uint32_t)BBH_BLOCK_BITS, b = p - qb * (uint32_t)BBH_BLOCK_BITS;
+    uint64_t r = brank[qb];
+    const uint32_t wi = b >> 6;
+    for (uint32_t q = 0; q < wi; q++) r += (uint64_t)__popcll(region[qb * 7 + q]);
+    r += (uint64_t)__popcll(region[qb * 7 + wi] & ((1ULL << (b & 63)) - 1));
+    if (r >= u) atomicOr(err, 8u);
+    else obs[start + r] = mbuf[e];  // every read of obs happened before bbh_levels' barriers
   }
 }
 
-// Segments above the register capacity: the same levels, with seen/collision
-// bits in global scratch (device-scope atomics) and per-key alive bits in a
-// global bitmap. One work-group per big segment.
+// Oversized segments: one work-group per segment runs a stable LSD radix
+// sort of (key, mask) through global memory (ping-pong with aux), tile by
+// tile, then deduplicates (OR-joining masks) into occ/obs[start...]. Their
+// MPHF is built by k_bbh_big (big_list).
+template <typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, uint16_t* __restrict__ obs, KeyT* __restrict__ aux,
+                                                      uint16_t* __restrict__ aux_obs, const uint64_t* __restrict__ seg_off,
+                                                      const uint32_t* __restrict__ over_list, uint64_t* __restrict__ ucount,
+                                                      unsigned int* __restrict__ flags, uint32_t* __restrict__ big_list) {
+  constexpr int KPT = SegCfg<KeyT>::KPT;
+  constexpr int CAP = RK_WG * KPT;
+  __shared__ uint32_t cnt[4 * 256];
+  __shared__ unsigned long long dbase[256];
+  __shared__ uint32_t ttot[256];
+  __shared__ uint64_t sh8[8];
+  __shared__ KeyT carry;
+  const uint32_t seg = over_list[blockIdx.x];
+  const uint64_t start = seg_off[seg];
+  const uint64_t n = seg_off[seg + 1] - start;
+  const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
+  KeyT* src = occ + start;
+  KeyT* dst = aux + start;
+  uint16_t* msrc = obs + start;
+  uint16_t* mdst = aux_obs + start;
+  const KeyT k0 = src[0];
+  uint64_t xl = 0, xh = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += RK_WG) key_xor_or(xl, xh, src[i], k0);
+  block_or2(xl, xh, sh8);
+  const int vb = varying_bits(xl, xh);
+  const int passes = (vb + 7) >> 3;
+  KeyT key[KPT];
+  uint32_t msk[KPT];
+  uint32_t rank[KPT];
+  for (int p = 0; p < passes; p++) {
+    const int sh = 8 * p;
+    for (int i = threadIdx.x; i < 256; i += RK_WG) ttot[i] = 0;
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) atomicAdd(&ttot[key_digit(src[i], sh)], 1u);
+    __syncthreads();
+    {
+      uint64_t ex = block_excl_scan_u64(ttot[threadIdx.x], sh8, nullptr);
+      dbase[threadIdx.x] = ex;
+    }
+    __syncthreads();
+    for (uint64_t t0 = 0; t0 < n; t0 += CAP) {
+      uint64_t tn = n - t0 < (uint64_t)CAP ? n - t0 : (uint64_t)CAP;
+      int kpt = (int)((tn + RK_WG - 1) / RK_WG);
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if (s < kpt) {
+          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
+          key[s] = e < tn ? src[t0 + e] : key_pad<KeyT>();
+          msk[s] = e < tn ? msrc[t0 + e] : 0u;
+        }
+      }
+      for (int i = threadIdx.x; i < 4 * 256; i += RK_WG) cnt[i] = 0;
+      __syncthreads();
+      rank_tile<KeyT, KPT>(key, kpt, sh, rank, cnt);
+      __syncthreads();
+      rank_offsets(cnt, sh8, ttot);
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if (s < kpt) {
+          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
+          if (e < tn) {
+            uint32_t d = key_digit(key[s], sh);
+            uint32_t local = cnt[wv * 256 + d] + rank[s];
+            uint64_t at = dbase[d] + (local - cnt[d]);  // cnt[d]: tile start of digit d
+            dst[at] = key[s];
+            mdst[at] = (uint16_t)msk[s];
+          }
+        }
+      }
+      __syncthreads();
+      dbase[threadIdx.x] += ttot[threadIdx.x];
+      __syncthreads();
+    }
+    KeyT* t = src;
+    src = dst;
+    dst = t;
+    uint16_t* mt = msrc;
+    msrc = mdst;
+    mdst = mt;
+    __threadfence_block();
+    __syncthreads();
+  }
+  // Dedup sorted src into occ/obs[start...] (src may be occ itself: writes
+  // trail reads). Thread t owns elements [t*kpt, (t+1)*kpt) of each tile; a
+  // run head reads ahead through its whole run to join the masks. Positions
+  // before the current tile may already hold uniques, so the key preceding a
+  // tile is carried over from the previous tile.
+  KeyT* out = occ + start;
+  uint16_t* mout = obs + start;
+  uint64_t u = 0;
+  for (uint64_t t0 = 0; t0 < n; t0 += CAP) {
+    uint64_t tn = n - t0 < (uint64_t)CAP ? n - t0 : (uint64_t)CAP;
+    int kpt = (int)((tn + RK_WG - 1) / RK_WG);
+    const KeyT prevc = carry;
+    uint64_t c = 0;
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      uint64_t e = (uint64_t)threadIdx.x * kpt + s;
+      rank[s] = 0;
+      if (s < kpt && e < tn) {
+        key[s] = src[t0 + e];
+        bool head = (t0 + e == 0) || !key_eq(key[s], e == 0 ? prevc : src[t0 + e - 1]);
+        if (head) {
+          uint32_t m = msrc[t0 + e];
+          for (uint64_t f = t0 + e + 1; f < n && key_eq(src[f], key[s]); f++) m |= msrc[f];
+          msk[s] = m;
+          rank[s] = 1;
+          c++;
+        }
+      }
+    }
+    uint64_t tot;
+    uint64_t o = block_excl_scan_u64(c, sh8, &tot);  // all reads of this tile precede its barriers
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      uint64_t e = (uint64_t)threadIdx.x * kpt + s;
+      if (rank[s]) {
+        out[u + o] = key[s];
+        mout[u + o] = (uint16_t)msk[s];
+        o++;
+      }
+      if (s < kpt && e + 1 == tn) carry = key[s];
+    }
+    u += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ucount[seg] = u;
+    big_list[atomicAdd(flags + 1, 1u)] = seg;
+  }
+}
+
+// ------------------------------------------------------------------ compaction
+// One work-group per segment copies the segment's distinct keys to the dense
+// key order (key_off), its masks (local-rank order: obs, or aux_obs for the
+// segments of k_bbh_big) to their MPHF values in vmask, and its BBHash blocks
+// from the padded slot to the dense block order (blk_off).
+template <typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ occ, const uint16_t* __restrict__ obs,
+                                                      const uint16_t* __restrict__ aux_obs, const uint64_t* __restrict__ seg_off,
+                                                      const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
+                                                      const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks_pad,
+                                                      uint64_t n_segs, int K2, int sb, KeyT* __restrict__ keys,
+                                                      uint16_t* __restrict__ vmask, uint64_t* __restrict__ blocks) {
+  const uint64_t seg = blockIdx.x;
+  if (seg >= n_segs) return;
+  const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
+  if (u == 0) return;
+  const bool big = seg_is_big<KeyT>(seg_off[seg + 1] - s0, u, K2 - sb);
+  const uint16_t* m = big ? aux_obs : obs;
+  for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
+    keys[ko + i] = occ[s0 + i];
+    vmask[ko + i] = m[s0 + i];
+  }
+  const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
+  const uint64_t* src = blocks_pad + pad_slot_off(s0, seg) * 8;
+  for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) blocks[b0 * 8 + i] = src[i];
+}
+
+// ------------------------------------------------------------------ BBHash of big segments
+// Segments above the register capacity, one work-group each: the same levels
+// as bbh_levels with keys read from occ (sorted, distinct, in place). LDSB:
+// the seen / collision / alive bitmaps sit in LDS (segments of <= MED_CAP
+// distinct keys); otherwise in global scratch (seen / collision words of the
+// segment at 4*floor(seg_off/64) + 8*seg, alive bits per occurrence position)
+// with device-scope atomics. Blocks go to the padded slot; each key's mask is
+// written to aux_obs[seg_off + local rank].
+constexpr uint64_t MED_CAP = 1ULL << 17;
+constexpr int MED_W = (int)(MED_CAP / 64);
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
   return __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <int MODE, typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ keys, const uint64_t* __restrict__ key_off,
-                                                   const uint32_t* __restrict__ big_list, uint64_t* __restrict__ seg_nblk,
-                                                   const uint64_t* __restrict__ blk_off, uint64_t* __restrict__ blocks,
+template <bool LDSB, typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ, const uint16_t* __restrict__ obs,
+                                                   uint16_t* __restrict__ aux_obs, const uint64_t* __restrict__ seg_off,
+                                                   const uint64_t* __restrict__ ucount, const uint32_t* __restrict__ big_list,
+                                                   uint64_t* __restrict__ nblk, uint64_t* __restrict__ blocks_pad,
                                                    uint64_t* __restrict__ scratch, uint64_t* __restrict__ alive_g,
                                                    unsigned int* __restrict__ err) {
   __shared__ uint64_t sh8[8];
   __shared__ unsigned long long carry;
   const uint32_t seg = big_list[blockIdx.x];
-  const uint64_t ko = key_off[seg];
-  const uint64_t D = key_off[seg + 1] - ko;
-  uint64_t* seen = scratch + 4 * (ko / 64) + 8 * (uint64_t)seg;
-  uint64_t* col = seen + (D / 32 + 2);
-  uint64_t* region = blocks + (MODE ? blk_off[seg] * 8 : 0);
-  if (MODE) {
-    uint64_t nwords = (blk_off[seg + 1] - blk_off[seg]) * 8;
-    for (uint64_t i = threadIdx.x; i < nwords; i += RK_WG) region[i] = 0;
+  const uint64_t so = seg_off[seg];
+  const uint64_t n_occ = seg_off[seg + 1] - so;
+  const uint64_t D = ucount[seg];
+  if (LDSB != (D <= MED_CAP)) return;  // the other variant builds it
+  const KeyT* keys = occ + so;
+  uint64_t* seen;
+  uint64_t* col;
+  uint64_t* alive;
+  uint64_t abase;  // bit index of key 0 in alive
+  if constexpr (LDSB) {
+    __shared__ uint64_t l_alive[MED_W];
+    __shared__ uint64_t l_seen[2 * MED_W + 1];
+    __shared__ uint64_t l_col[2 * MED_W + 1];
+    seen = l_seen;
+    col = l_col;
+    alive = l_alive;
+    abase = 0;
+    for (uint64_t i = threadIdx.x; i < (D + 63) / 64; i += RK_WG) {
+      uint64_t rem = D - 64 * i;
+      alive[i] = rem >= 64 ? ~0ULL : ((1ULL << rem) - 1);
+    }
+  } else {
+    seen = scratch + 4 * (so / 64) + 8 * (uint64_t)seg;
+    col = seen + (D / 32 + 2);
+    alive = alive_g;
+    abase = so;
   }
+  auto ld = [&](const uint64_t* p) -> uint64_t {
+    if constexpr (LDSB) return *p;
+    else return ld_agent(p);
+  };
+  uint64_t* region = blocks_pad + pad_slot_off(so, seg) * 8;
+  const uint64_t slot = pad_slot_blocks(n_occ);
   uint64_t off = 0, t = D, r = D;
   int l = 0;
   while (r > 0) {
     const uint64_t n = bbh_level_n(t);
-    if (l >= BBH_MAX_LEVELS) {
+    if (l >= BBH_MAX_LEVELS || (off + n + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS > slot) {
       if (threadIdx.x == 0) atomicOr(err, 2u);
       break;
     }
@@ -733,60 +965,64 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ keys
       seen[i] = 0;
       col[i] = 0;
     }
-    __threadfence();
+    if constexpr (!LDSB) __threadfence();
     __syncthreads();
     const uint64_t seed = bbh_seed(l);
     for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
-      uint64_t gi = ko + e;
-      if (!((ld_agent(&alive_g[gi >> 6]) >> (gi & 63)) & 1)) continue;
-      uint64_t pos = __umul64hi(bbh_hash(keys[gi], seed), n);
+      const uint64_t gi = abase + e;
+      if (!((ld(&alive[gi >> 6]) >> (gi & 63)) & 1)) continue;
+      uint64_t pos = __umul64hi(bbh_hash(keys[e], seed), n);
       unsigned long long bit = 1ULL << (pos & 63);
       unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
       if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
     }
-    __threadfence();
+    if constexpr (!LDSB) __threadfence();
     __syncthreads();
-    if (MODE) {
-      for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) {
-        uint64_t q = (off >> 6) + i;
-        region[(q / 7) * 8 + 1 + (q % 7)] = ld_agent(&seen[i]) & ~ld_agent(&col[i]);
-      }
+    for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) {
+      uint64_t q = (off >> 6) + i;
+      region[(q / 7) * 8 + 1 + (q % 7)] = ld(&seen[i]) & ~ld(&col[i]);
     }
     uint64_t cnt = 0;
     for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
-      uint64_t gi = ko + e;
-      if (!((ld_agent(&alive_g[gi >> 6]) >> (gi & 63)) & 1)) continue;
-      uint64_t pos = __umul64hi(bbh_hash(keys[gi], seed), n);
-      if (((ld_agent(&seen[pos >> 6]) & ~ld_agent(&col[pos >> 6])) >> (pos & 63)) & 1)
-        atomicAnd((unsigned long long*)&alive_g[gi >> 6], ~(1ULL << (gi & 63)));
+      const uint64_t gi = abase + e;
+      if (!((ld(&alive[gi >> 6]) >> (gi & 63)) & 1)) continue;
+      uint64_t pos = __umul64hi(bbh_hash(keys[e], seed), n);
+      if (((ld(&seen[pos >> 6]) & ~ld(&col[pos >> 6])) >> (pos & 63)) & 1)
+        atomicAnd((unsigned long long*)&alive[gi >> 6], ~(1ULL << (gi & 63)));
       else
         cnt++;
     }
-    __threadfence();
-    block_excl_scan_u64(cnt, sh8, &r);
+    if constexpr (!LDSB) __threadfence();
+    block_excl_scan_u64(cnt, sh8, &r);  // barriers inside
     off += n;
     t = bbh_next_t(t);
     l++;
   }
-  const uint64_t nblk = (off + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;
-  if (MODE == 0) {
-    if (threadIdx.x == 0) seg_nblk[seg] = nblk;
-    return;
-  }
+  const uint64_t nb = (off + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;
+  // zero the tail of the last block
+  for (uint64_t q = (off >> 6) + threadIdx.x; q < nb * 7; q += RK_WG) region[(q / 7) * 8 + 1 + (q % 7)] = 0;
   __threadfence();
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (uint64_t j0 = 0; j0 < nblk; j0 += RK_WG) {
+  for (uint64_t j0 = 0; j0 < nb; j0 += RK_WG) {
     uint64_t j = j0 + threadIdx.x;
     uint64_t pc = 0;
-    if (j < nblk)
-      for (int u = 0; u < 7; u++) pc += (uint64_t)__popcll(ld_agent(&region[j * 8 + 1 + u]));
+    if (j < nb)
+      for (int q = 0; q < 7; q++) pc += (uint64_t)__popcll(ld_agent(&region[j * 8 + 1 + q]));
     uint64_t tot;
     uint64_t rk = block_excl_scan_u64(pc, sh8, &tot);
-    if (j < nblk) region[j * 8] = carry + rk;
+    if (j < nb) region[j * 8] = carry + rk;
     __syncthreads();
     if (threadIdx.x == 0) carry += tot;
     __syncthreads();
+  }
+  if (threadIdx.x == 0) nblk[seg] = nb;
+  __threadfence();
+  __syncthreads();
+  for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
+    uint64_t rk = mphf_local_rank<KeyT>(region, D, keys[e]);
+    if (rk >= D) atomicOr(err, 4u);
+    else aux_obs[so + rk] = obs[so + e];
   }
 }
 
@@ -809,86 +1045,68 @@ void launch_p3_count(hipStream_t st, bool wide, const uint64_t* rec, const uint6
                 num_bins, chunk_records, K, sb, counts);
 }
 void launch_p3_scatter(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
-                       int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* cursor, void* occ) {
+                       int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* cursor, void* occ,
+                       uint16_t* occ_obs) {
   if (n_chunks == 0) return;
   if (!wide)
     CF1G_LAUNCH("p3_scatter", (k_p3_scatter<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, cursor, (uint64_t*)occ);
+                chunk_off, num_bins, chunk_records, K, sb, cursor, (uint64_t*)occ, occ_obs);
   else
     CF1G_LAUNCH("p3_scatter", (k_p3_scatter<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, cursor, (Key128*)occ);
+                chunk_off, num_bins, chunk_records, K, sb, cursor, (Key128*)occ, occ_obs);
 }
-void launch_p3_segsort(hipStream_t st, bool wide, void* occ, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
-                       uint64_t* ucount, uint32_t* over_list, unsigned int* n_over) {
+void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
+                     uint64_t* ucount, uint64_t* nblk, uint32_t* over_list, unsigned int* flags, uint32_t* big_list,
+                     uint64_t* blocks_pad) {
   if (n_segs == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_segsort", k_p3_segsort<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (uint64_t*)occ, seg_off,
-                n_segs, 2 * K, sb, ucount, over_list, n_over);
+    CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (uint64_t*)occ, obs, seg_off, n_segs,
+                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad);
   else
-    CF1G_LAUNCH("p3_segsort", k_p3_segsort<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (Key128*)occ, seg_off, n_segs,
-                2 * K, sb, ucount, over_list, n_over);
+    CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (Key128*)occ, obs, seg_off, n_segs,
+                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad);
 }
-void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, void* aux, const uint64_t* seg_off, const uint32_t* over_list,
-                       uint32_t n_over, uint64_t* ucount) {
+void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void* aux, uint16_t* aux_obs, const uint64_t* seg_off,
+                       const uint32_t* over_list, uint32_t n_over, uint64_t* ucount, unsigned int* flags, uint32_t* big_list) {
   if (n_over == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_bigsort", k_p3_bigsort<uint64_t>, dim3(n_over), dim3(RK_WG), 0, st, (uint64_t*)occ, (uint64_t*)aux,
-                seg_off, over_list, ucount);
+    CF1G_LAUNCH("p3_bigsort", k_p3_bigsort<uint64_t>, dim3(n_over), dim3(RK_WG), 0, st, (uint64_t*)occ, obs, (uint64_t*)aux, aux_obs,
+                seg_off, over_list, ucount, flags, big_list);
   else
-    CF1G_LAUNCH("p3_bigsort", k_p3_bigsort<Key128>, dim3(n_over), dim3(RK_WG), 0, st, (Key128*)occ, (Key128*)aux, seg_off,
-                over_list, ucount);
+    CF1G_LAUNCH("p3_bigsort", k_p3_bigsort<Key128>, dim3(n_over), dim3(RK_WG), 0, st, (Key128*)occ, obs, (Key128*)aux, aux_obs,
+                seg_off, over_list, ucount, flags, big_list);
 }
-void launch_p3_compact(hipStream_t st, bool wide, const void* occ, const uint64_t* seg_off, const uint64_t* ucount,
-                       const uint64_t* out_off, uint64_t n_segs, void* keys) {
-  if (n_segs == 0) return;
-  if (!wide)
-    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(256), 0, st, (const uint64_t*)occ, seg_off,
-                ucount, out_off, n_segs, (uint64_t*)keys);
-  else
-    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(256), 0, st, (const Key128*)occ, seg_off,
-                ucount, out_off, n_segs, (Key128*)keys);
-}
-void launch_bbh_seg(hipStream_t st, bool wide, int mode, const void* keys, const uint64_t* key_off, uint64_t n_segs,
-                    uint64_t* seg_nblk, const uint64_t* blk_off, uint64_t* blocks, uint32_t* big_list, unsigned int* n_big,
-                    unsigned int* err) {
-  if (n_segs == 0) return;
-  dim3 g((unsigned)n_segs), b(RK_WG);
-  if (!wide) {
-    if (mode == 0)
-      CF1G_LAUNCH("p3b_bbh_size", (k_bbh_seg<0, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, n_segs, seg_nblk, blk_off,
-                  blocks, big_list, n_big, err);
-    else
-      CF1G_LAUNCH("p3b_bbh_build", (k_bbh_seg<1, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, n_segs, seg_nblk, blk_off,
-                  blocks, big_list, n_big, err);
-  } else {
-    if (mode == 0)
-      CF1G_LAUNCH("p3b_bbh_size", (k_bbh_seg<0, Key128>), g, b, 0, st, (const Key128*)keys, key_off, n_segs, seg_nblk, blk_off,
-                  blocks, big_list, n_big, err);
-    else
-      CF1G_LAUNCH("p3b_bbh_build", (k_bbh_seg<1, Key128>), g, b, 0, st, (const Key128*)keys, key_off, n_segs, seg_nblk, blk_off,
-                  blocks, big_list, n_big, err);
-  }
-}
-void launch_bbh_big(hipStream_t st, bool wide, int mode, const void* keys, const uint64_t* key_off, const uint32_t* big_list,
-                    uint32_t n_big, uint64_t* seg_nblk, const uint64_t* blk_off, uint64_t* blocks, uint64_t* scratch,
-                    uint64_t* alive_g, unsigned int* err) {
+// Both variants are launched over the whole big list; each builds the
+// segments of its size class and returns at once for the others.
+void launch_bbh_big(hipStream_t st, bool wide, const void* occ, const uint16_t* obs, uint16_t* aux_obs, const uint64_t* seg_off,
+                    const uint64_t* ucount, const uint32_t* big_list, uint32_t n_big, uint64_t* nblk, uint64_t* blocks_pad,
+                    uint64_t* scratch, uint64_t* alive_g, unsigned int* err) {
   if (n_big == 0) return;
   dim3 g(n_big), b(RK_WG);
   if (!wide) {
-    if (mode == 0)
-      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<0, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, big_list, seg_nblk, blk_off,
-                  blocks, scratch, alive_g, err);
-    else
-      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<1, uint64_t>), g, b, 0, st, (const uint64_t*)keys, key_off, big_list, seg_nblk, blk_off,
-                  blocks, scratch, alive_g, err);
+    CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, aux_obs, seg_off, ucount, big_list,
+                nblk, blocks_pad, scratch, alive_g, err);
+    if (alive_g)
+      CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, aux_obs, seg_off, ucount,
+                  big_list, nblk, blocks_pad, scratch, alive_g, err);
   } else {
-    if (mode == 0)
-      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<0, Key128>), g, b, 0, st, (const Key128*)keys, key_off, big_list, seg_nblk, blk_off,
-                  blocks, scratch, alive_g, err);
-    else
-      CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<1, Key128>), g, b, 0, st, (const Key128*)keys, key_off, big_list, seg_nblk, blk_off,
-                  blocks, scratch, alive_g, err);
+    CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, Key128>), g, b, 0, st, (const Key128*)occ, obs, aux_obs, seg_off, ucount, big_list,
+                nblk, blocks_pad, scratch, alive_g, err);
+    if (alive_g)
+      CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, Key128>), g, b, 0, st, (const Key128*)occ, obs, aux_obs, seg_off, ucount, big_list,
+                  nblk, blocks_pad, scratch, alive_g, err);
   }
+}
+void launch_p3_compact(hipStream_t st, bool wide, const void* occ, const uint16_t* obs, const uint16_t* aux_obs, const uint64_t* seg_off,
+                       const uint64_t* ucount, const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad,
+                       uint64_t n_segs, int K, int sb, void* keys, uint16_t* vmask, uint64_t* blocks) {
+  if (n_segs == 0) return;
+  if (!wide)
+    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)occ, obs, aux_obs,
+                seg_off, ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (uint64_t*)keys, vmask, blocks);
+  else
+    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)occ, obs, aux_obs,
+                seg_off, ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (Key128*)keys, vmask, blocks);
 }
 
 }  // namespace cf1g

@@ -1,17 +1,16 @@
-// P4: lock-free DFA vertex classification (classify.rs:25-601) and the
-// per-position annotation consumed by the host P5 (directed_kmer.rs:70-109).
+// P4: DFA vertex classification (classify.rs:25-601) into the packed state
+// vector (state_vector.rs:7-71), and the per-position annotation consumed by
+// the host P5 (directed_kmer.rs:70-109).
 //
-// One lane per k-mer position of the flat stream. Each valid occurrence
-// derives its observation from its two neighbours (classify.rs:65-141 reduces
-// to: isolated / leftmost / rightmost / internal, with the self-loop cases of
-// :143-177), looks its canonical k-mer up in the BBHash and joins the
-// observation into the 5-bit state with a 64-bit CAS loop on the packed word
-// (state_vector.rs:47-70). The join is commutative and idempotent, so the
-// final states do not depend on the interleaving.
-//
-// Scheduling: a persistent grid; work-groups take fixed chunks of stream
-// positions from an atomic counter (work stealing across records of any
-// length; a chunk reads up to K bases past its end, the k-1 overlap).
+// Every occurrence's observation (isolated / leftmost / rightmost / internal
+// with the self-loop cases, classify.rs:65-177) was computed in P3a from the
+// routed super k-mers and OR-joined per distinct k-mer while P3a sorted and
+// deduplicated the segment; P3b placed each joined mask at the k-mer's MPHF
+// value. The reference reaches the same state by one CAS per occurrence,
+// because its threads meet the occurrences in sequence order; the join is
+// order-free, so P4 here is a streaming pass that encodes each joined mask
+// as its 5-bit state (State::from_vertex, state.rs:92-113) and packs 12 per
+// word.
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -42,75 +41,22 @@ struct KOps<Key128> {
   }
 };
 
-// Observation of the valid k-mer at g; returns false when g holds no valid k-mer.
-template <typename KeyT>
-__device__ __forceinline__ bool observe(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t g, int K,
-                                       KeyT& canon, bool& fwd, uint32_t& type, uint32_t& pb, uint32_t& nb) {
-  if (!window_clear(nmask, g, K)) return false;
-  const bool pv = !is_n(nmask, g - 1);
-  const bool nv = !is_n(nmask, g + K);
-  KeyT x = KOps<KeyT>::at(seq, g, K);
-  KeyT r = KOps<KeyT>::rc(x, K);
-  canon = kmer_min(x, r);
-  fwd = key_eq(x, canon);
-  pb = base_at(seq, g - 1);
-  nb = base_at(seq, g + K);
-  if (!pv && !nv) {
-    type = OBS_MIMO;  // isolated (classify.rs:147-158)
-  } else if (!nv) {
-    type = OBS_RIGHT;  // classify.rs:83-86, :124-125
-  } else {
-    KeyT x2 = x, r2 = r;
-    KOps<KeyT>::roll(x2, r2, nb, K);
-    KeyT c2 = kmer_min(x2, r2);
-    if (key_eq(c2, canon)) {  // self loop (classify.rs:143-145, 160-177)
-      type = (!pv || key_eq(x, x2)) ? OBS_MIMO : OBS_RIGHT;
-    } else {
-      type = pv ? OBS_INTERNAL : OBS_LEFT;
-    }
+// One thread per state word: the 12 joined masks of values [12w, 12w+12)
+// (three 8-byte loads; the mask array is value-ordered by P3) become 5-bit
+// states, packed 12 per word with bits 60-63 zero (state_vector.rs:36-71).
+__global__ void __launch_bounds__(256) k_p4_pack(const uint64_t* __restrict__ vmask, uint64_t D, uint64_t n_words,
+                                                 uint64_t* __restrict__ states) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  uint64_t m[3] = {vmask[3 * w], vmask[3 * w + 1], vmask[3 * w + 2]};
+  const uint64_t valid = D - 12 * w;  // >= 1
+  uint64_t out = 0;
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    const uint32_t mk = (uint32_t)(m[i >> 2] >> (16 * (i & 3))) & 0xFFFFu;
+    if ((uint64_t)i < valid) out |= (uint64_t)mask_state(mk) << (5 * i);
   }
-  return true;
-}
-
-__device__ __forceinline__ void state_join(unsigned long long* __restrict__ states, uint64_t idx, uint32_t type, bool fwd,
-                                           uint32_t pb, uint32_t nb) {
-  unsigned long long* wp = states + idx / 12;
-  const uint32_t sh = (uint32_t)(idx % 12) * 5;
-  unsigned long long old = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (;;) {
-    uint32_t s = (uint32_t)(old >> sh) & 31u;
-    uint32_t ns = dfa_next(s, type, fwd, pb, nb);
-    if (ns == s) return;
-    unsigned long long nw = (old & ~(31ULL << sh)) | ((unsigned long long)ns << sh);
-    unsigned long long prev = atomicCAS(wp, old, nw);
-    if (prev == old) return;
-    old = prev;
-  }
-}
-
-template <typename KeyT>
-__global__ void __launch_bounds__(256) k_p4_classify(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                     uint64_t n_pos, int K, const MphfView mv, unsigned long long* __restrict__ states,
-                                                     unsigned int* __restrict__ next_chunk, uint32_t chunk, uint32_t n_chunks) {
-  __shared__ uint32_t s_c;
-  for (;;) {
-    if (threadIdx.x == 0) s_c = atomicAdd(next_chunk, 1u);
-    __syncthreads();
-    const uint32_t c = s_c;
-    __syncthreads();
-    if (c >= n_chunks) break;
-    uint64_t g0 = 1 + (uint64_t)c * chunk;
-    uint64_t g1 = g0 + chunk;
-    if (g1 > n_pos) g1 = n_pos;
-    for (uint64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
-      KeyT canon;
-      bool fwd;
-      uint32_t type, pb, nb;
-      if (!observe<KeyT>(seq, nmask, g, K, canon, fwd, type, pb, nb)) continue;
-      uint64_t idx = mphf_lookup<KeyT>(mv, canon);
-      state_join(states, idx, type, fwd, pb, nb);
-    }
-  }
+  states[w] = out;
 }
 
 // Per stream position: bit0 valid k-mer, bit1 FWD, bits2-3 state class.
@@ -140,16 +86,10 @@ __global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, u
     out[i] = mphf_lookup<KeyT>(mv, keys[i]);
 }
 
-void launch_p4_classify(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
-                        const MphfView& mv, uint64_t* states, unsigned int* next_chunk, uint32_t chunk, uint32_t n_chunks,
-                        uint32_t grid) {
-  if (n_chunks == 0) return;
-  if (!wide)
-    CF1G_LAUNCH("p4_classify", k_p4_classify<uint64_t>, dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, mv,
-                (unsigned long long*)states, next_chunk, chunk, n_chunks);
-  else
-    CF1G_LAUNCH("p4_classify", k_p4_classify<Key128>, dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, mv,
-                (unsigned long long*)states, next_chunk, chunk, n_chunks);
+void launch_p4_pack(hipStream_t st, const uint16_t* vmask, uint64_t D, uint64_t* states) {
+  const uint64_t nw = (D + 11) / 12;
+  if (nw == 0) return;
+  CF1G_LAUNCH("p4_classify", k_p4_pack, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (const uint64_t*)vmask, D, nw, states);
 }
 void launch_annotate(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
                      const MphfView& mv, const uint64_t* states, uint8_t* out) {

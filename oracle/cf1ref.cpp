@@ -1228,6 +1228,82 @@ int ref_state_tables(uint8_t* cls32, uint8_t* out32, uint8_t* dead32, uint8_t* f
   }
   return 0;
 }
+// One DFA step as the reference's process_* functions compute it for a
+// state `s` (classify.rs:147-518): type 0 isolated, 1 leftmost, 2 rightmost,
+// 3 internal, 4 self loop with the leftmost or direct-repeat rule (MIMO),
+// 5 self loop of an inverted repeat (handled as rightmost with prev). Bases
+// are 2-bit codes; fwd = direction of the occurrence. Returns the new state.
+int ref_dfa_step(int s, int type, int fwd, int pb, int nb) {
+  AtomicStateVector sv(1);
+  if (s) sv.compare_and_swap(0, 0, (uint8_t)s);
+  // a one-key pipeline context is not needed: restate the branch bodies on sv
+  struct Local {
+    static uint8_t with_class(const Vertex& v, StateClass c) {
+      Vertex x = v;
+      x.state_class = c;
+      return state_from_vertex(x);
+    }
+  };
+  uint8_t st = sv.get(0);
+  if (st_is_dead_end(st)) return st;
+  if (type == 0 || type == 4) return state_from_vertex(v_mimo());
+  if (type == 5) type = 2;
+  uint8_t P = (uint8_t)pb, N = (uint8_t)nb, ns = st;
+  if (type == 1) {  // process_leftmost_kmer :199-278
+    if (fwd) {
+      if (!st_is_visited(st)) ns = state_from_vertex(v_miso(N));
+      else {
+        Vertex v = st_decode(st);
+        if (v.state_class == SISO) ns = v.back == N ? Local::with_class(v, MISO) : Local::with_class(v, MIMO);
+        else if (v.state_class == MISO) ns = v.back != N ? Local::with_class(v, MIMO) : st;
+        else ns = Local::with_class(v, MIMO);
+      }
+    } else {
+      uint8_t cn = complement_base(N);
+      if (!st_is_visited(st)) ns = state_from_vertex(v_simo(cn));
+      else {
+        Vertex v = st_decode(st);
+        if (v.state_class == SISO) ns = v.front == cn ? Local::with_class(v, SIMO) : Local::with_class(v, MIMO);
+        else if (v.state_class == MISO) ns = Local::with_class(v, MIMO);
+        else ns = v.front != cn ? Local::with_class(v, MIMO) : st;
+      }
+    }
+  } else if (type == 2) {  // process_rightmost_kmer :297-376
+    if (fwd) {
+      if (!st_is_visited(st)) ns = state_from_vertex(v_simo(P));
+      else {
+        Vertex v = st_decode(st);
+        if (v.state_class == SISO) ns = v.front == P ? Local::with_class(v, SIMO) : Local::with_class(v, MIMO);
+        else if (v.state_class == MISO) ns = Local::with_class(v, MIMO);
+        else ns = v.front != P ? Local::with_class(v, MIMO) : st;
+      }
+    } else {
+      uint8_t cp = complement_base(P);
+      if (!st_is_visited(st)) ns = state_from_vertex(v_miso(cp));
+      else {
+        Vertex v = st_decode(st);
+        if (v.state_class == SISO) ns = v.back == cp ? Local::with_class(v, MISO) : Local::with_class(v, MIMO);
+        else if (v.state_class == MISO) ns = v.back != cp ? Local::with_class(v, MIMO) : st;
+        else ns = Local::with_class(v, MIMO);
+      }
+    }
+  } else {  // process_internal_kmer :406-511
+    uint8_t a = fwd ? P : complement_base(N), c = fwd ? N : complement_base(P);
+    if (!st_is_visited(st)) ns = state_from_vertex(v_siso(a, c));
+    else {
+      Vertex v = st_decode(st);
+      if (v.state_class == SISO) {
+        if (v.front == a && v.back == c) ns = st;
+        else if (v.front != a && v.back != c) ns = Local::with_class(v, MIMO);
+        else if (v.front != a) ns = Local::with_class(v, MISO);
+        else ns = Local::with_class(v, SIMO);
+      } else if (v.state_class == MISO) ns = v.back != c ? Local::with_class(v, MIMO) : st;
+      else ns = v.front != a ? Local::with_class(v, MIMO) : st;
+    }
+  }
+  return ns;
+}
+
 // Stand-alone AtomicStateVector for the state_vector.rs unit tests.
 void* ref_sv_new(uint64_t n) { return new AtomicStateVector(n); }
 void ref_sv_free(void* p) { delete (AtomicStateVector*)p; }

@@ -70,6 +70,7 @@ def lib():
         L.ref_sv_word.restype = C.c_uint64
         L.ref_sv_word.argtypes = [C.c_void_p, C.c_uint64]
         L.ref_set_target_seg.argtypes = [C.c_void_p, C.c_uint64]
+        L.ref_dfa_step.argtypes = [C.c_int] * 5
         _LIB = L
     return _LIB
 

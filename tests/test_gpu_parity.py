@@ -22,6 +22,7 @@ pytestmark = pytest.mark.gpu
 
 
 def decode_gpu_bins(rw, bin_start, recs):
+    """Super k-mer strings of the device bin records (flanking bases dropped)."""
     out = []
     for b in range(len(bin_start) - 1):
         strs = []
@@ -32,7 +33,8 @@ def decode_gpu_bins(rw, bin_start, recs):
             for x in w:
                 v = (v << 64) | int(x)
             tot = 64 * rw
-            strs.append("".join("ACGT"[(v >> (tot - 2 * (t + 1))) & 3] for t in range(L)))
+            # position 0 is the left flanking base; the super k-mer is 1..L
+            strs.append("".join("ACGT"[(v >> (tot - 2 * (t + 1))) & 3] for t in range(1, L + 1)))
         out.append(strs)
     return out
 
@@ -79,7 +81,8 @@ def run_pair(seqs, k, num_bins=128, names=None, p5=True, bins=True, threads=4, t
         for b in range(nb_ref):
             # Record boundaries may differ only if chunking differed; the k-mer
             # multiset per bin is what P3 consumes.
-            assert kmers_of(gb[b], k) == kmers_of(decode_ref_bin(ref.bin_bytes(b)), k), f"bin {b}"
+            same = kmers_of(gb[b], k) == kmers_of(decode_ref_bin(ref.bin_bytes(b)), k)
+            assert same, f"bin {b}: routed k-mers differ"
     d = g.build_mphf()
     assert d == d_ref, "D"
     lo, hi = g.keys()
