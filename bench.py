@@ -31,9 +31,8 @@ HBM_PEAK_GBS = 8000.0
 PHASE_OF = {
     "pack": "P1", "p1_minimizer_hist": "P1",
     "p2_partition": "P2", "p2_route": "P2",
-    "p3_count": "P3a", "p3_scatter": "P3a", "p3_segsort": "P3a", "p3_bigsort": "P3a", "p3_compact": "P3a",
-    "scan_tile": "scan", "scan_add": "scan",
-    "p3b_bbh_size": "P3b", "p3b_bbh_build": "P3b", "p3b_bbh_big": "P3b",
+    "p3_count": "P3", "p3_scatter": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
+    "scan_tile": "P3", "scan_add": "P3", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",
 }
 
@@ -70,28 +69,34 @@ def mphf_level_stats(mphf_words):
     return placed, e_probe
 
 
-def algorithmic_bytes(info, kb, rw, placed):
-    """Algorithmic HBM bytes per kernel over one step (random accesses = one
-    64-byte line each)."""
+def algorithmic_bytes(info, kb, rw):
+    """Algorithmic HBM bytes per kernel over one step: what each kernel's
+    function must read and write at least once (inputs, outputs, no
+    intermediate re-reads).
+      pack      : ASCII in, 2-bit bases + placeholder bits out
+      p1        : packed stream in (histogram output is 512 KiB)
+      p2_route  : packed stream in, fixed-width super k-mer records out
+      p3_count  : records in
+      p3_scatter: records in, W (key, mask) occurrences out
+      p3_dedup  : W occurrences in, D distinct (key, mask) + BBHash blocks out
+      p3_compact: D (key, mask) + blocks in, the same out (dense)
+      p4        : D masks in, ceil(D/12) state words out
+    """
     NG = info["stream_len"]
     W, S, D = info["n_kmers"], info["n_superkmers"], info["n_distinct"]
-    seq = NG / 4 + NG / 8  # packed bases + placeholder mask
+    B = info["mphf_words"] * 8
+    seq = NG / 4 + NG / 8
     rec = S * rw * 8
-    rem = [D]
-    for p in placed:
-        rem.append(rem[-1] - p)
-    e_probe = sum((l + 1) * p for l, p in enumerate(placed)) / max(D, 1)
+    ob = kb + 2
     return {
         "pack": NG + seq,
         "p1_minimizer_hist": seq,
         "p2_route": seq + rec,
         "p3_count": rec,
-        "p3_scatter": rec + W * kb,
-        "p3_segsort": W * kb + D * kb,
-        "p3_compact": 2 * D * kb,
-        "p3b_bbh_size": D * kb,
-        "p3b_bbh_build": D * kb + info["mphf_words"] * 8,
-        "p4_classify": seq + 64.0 * W * (e_probe + 1.0),
+        "p3_scatter": rec + W * ob,
+        "p3_dedup": W * ob + D * ob + B,
+        "p3_compact": 2 * (D * ob + B),
+        "p4_classify": 2 * D + 8 * ((D + 11) // 12),
     }
 
 
@@ -177,7 +182,7 @@ def main():
     kb = 16 if args.k > 32 else 8
     rw = 4 if args.k > 32 else 2
     placed, e_probe = mphf_level_stats(g.mphf_export())
-    ab = algorithmic_bytes(info, kb, rw, placed)
+    ab = algorithmic_bytes(info, kb, rw)
     kern = {}
     for name, (cnt, tot) in kt.items():
         per_step = cnt / args.steps
@@ -191,6 +196,8 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": ab[dom] / dk["launches_per_step"], "mean_launch_ms": round(dk["mean_launch_ms"], 4)}
+    tot_bytes = sum(ab.values())
+    roof["pipeline"] = {"algorithmic_bytes_per_step": tot_bytes, "achieved": round(tot_bytes / (ms * 1e-3) / 1e9, 1)}
     phases = {}
     for name, v in kern.items():
         phases.setdefault(PHASE_OF.get(name, name), 0.0)

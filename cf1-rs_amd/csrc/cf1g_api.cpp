@@ -157,7 +157,9 @@ struct cf1g_ctx {
 };
 
 namespace {
-constexpr uint32_t CHUNK_RECORDS = 1024;
+// Records per work-group in the P3 count / scatter passes: long enough that
+// each (work-group, segment) run of the scatter spans whole cache lines.
+constexpr uint32_t CHUNK_RECORDS = 16384;
 constexpr uint64_t STREAM_PAD = 4096;
 
 struct Fail {
@@ -255,11 +257,11 @@ void do_p3(cf1g_ctx* c) {
   hipStream_t st = c->st;
   const size_t kb = c->key_bytes;
   const int K = c->k;
-  // Segment bits: the fewest (<= 12, <= 2K) that bring the mean segment to
-  // <= target k-mers. The MPHF partition depends on it (it is exported).
-  uint64_t target = c->prm.p3_target_seg ? c->prm.p3_target_seg : (c->wide ? 1024 : 2048);
+  // Segment bits: the fewest (<= 12) that bring the mean segment to <= target
+  // k-mer occurrences. The MPHF partition depends on it (it is exported).
+  uint64_t target = c->prm.p3_target_seg ? c->prm.p3_target_seg : 1536;
   int sb = 0;
-  while (sb < 12 && sb < 2 * K && c->W > (uint64_t)c->num_bins * (1ULL << sb) * target) sb++;
+  while (sb < 12 && c->W > (uint64_t)c->num_bins * (1ULL << sb) * target) sb++;
   c->sb = sb;
   c->n_segs = (uint64_t)c->num_bins << sb;
   const uint64_t ns = c->n_segs;

@@ -81,16 +81,6 @@ __device__ __forceinline__ Key128 rc128(const Key128& x, int K) {
 __device__ __forceinline__ uint64_t kmer_min(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ Key128 kmer_min(const Key128& a, const Key128& b) { return key_lt(b, a) ? b : a; }
 
-// Top `sb` bits of a 2K-bit key (sb <= 2K, sb <= 32).
-__device__ __forceinline__ uint32_t key_top(uint64_t k, int K2, int sb) {
-  return sb == 0 ? 0u : (uint32_t)(k >> (K2 - sb));
-}
-__device__ __forceinline__ uint32_t key_top(const Key128& k, int K2, int sb) {
-  if (sb == 0) return 0u;
-  int s = K2 - sb;  // shift of the 128-bit value
-  if (s >= 64) return (uint32_t)(k.hi >> (s - 64));
-  return (uint32_t)((k.hi << (64 - s)) | (k.lo >> s));
-}
 // 8-bit radix digit at bit `sh` (sh multiple of 8).
 __device__ __forceinline__ uint32_t key_digit(uint64_t k, int sh) { return (uint32_t)(k >> sh) & 255u; }
 __device__ __forceinline__ uint32_t key_digit(const Key128& k, int sh) {
@@ -134,13 +124,19 @@ __host__ __device__ __forceinline__ uint64_t bbh_seed(int level) {
 }
 __device__ __forceinline__ uint64_t bbh_hash(uint64_t k, uint64_t seed) { return mix64(k ^ seed); }
 __device__ __forceinline__ uint64_t bbh_hash(const Key128& k, uint64_t seed) { return mix64(k.lo ^ mix64(k.hi ^ seed)); }
+// Segment of a key inside its bin: the top sb bits of the level hash with
+// seed bbh_seed(-1) (spreads each bin evenly over its 2^sb segments).
+template <typename KeyT>
+__device__ __forceinline__ uint32_t seg_sub(const KeyT& k, int sb) {
+  return sb ? (uint32_t)(bbh_hash(k, bbh_seed(-1)) >> (64 - sb)) : 0u;
+}
 
 constexpr int BBH_MAX_LEVELS = 64;
 constexpr uint64_t BBH_BLOCK_BITS = 448;  // 7 words of bits + 1 rank word = one 64-byte line
 
 // ------------------------------------------------------------ MPHF (minimizer-partitioned BBHash)
 // The distinct keys are partitioned exactly as P3a left them: segment
-// (bin, top sb key bits), bin = bucket_to_bin[bucket_hash(minimizer value)].
+// (bin, seg_sub(key)), bin = bucket_to_bin[bucket_hash(minimizer value)].
 // Each segment holds its own BBHash: level l has n_l = roundup64(max(2 t_l, 1))
 // bits with t_0 = D_s (the segment's key count) and t_{l+1} = ceil(9 t_l / 20);
 // a key lands at pos = mulhi64(hash(key, l), n_l) and is placed at the first
@@ -195,7 +191,7 @@ __device__ __forceinline__ uint32_t minimizer_value(const KeyT& c, int K, int m)
 template <typename KeyT>
 __device__ __forceinline__ uint64_t mphf_segment(const MphfView& v, const KeyT& c) {
   uint32_t bin = v.b2b[bucket_hash((uint64_t)minimizer_value(c, v.K, v.m))];
-  return ((uint64_t)bin << v.sb) | key_top(c, 2 * v.K, v.sb);
+  return ((uint64_t)bin << v.sb) | seg_sub(c, v.sb);
 }
 // Rank of a present key inside its segment (segment already known).
 template <typename KeyT>

@@ -71,7 +71,9 @@ struct HostMphf {
       }
     }
     uint32_t bin = b2b[h_xxh3_u64le((uint64_t)bv) >> 48];
-    uint32_t sub = sb ? (uint32_t)(c >> (2 * K - sb)) : 0;
+    uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64), s = h_seed(-1);
+    uint64_t h = wide ? h_mix64(lo ^ h_mix64(hi ^ s)) : h_mix64(lo ^ s);
+    uint32_t sub = sb ? (uint32_t)(h >> (64 - sb)) : 0;
     return ((uint64_t)bin << sb) | sub;
   }
   uint64_t lookup(u128 c) const {

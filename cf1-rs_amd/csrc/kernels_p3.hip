@@ -2,19 +2,17 @@
 // (mphf.rs:323-363 dedup_bin) and the BBHash build (the project's
 // replacement for ph::fmph, mphf.rs:144-190).
 //
-// P3a layout: the k-mers of bin b are split by the top `sb` bits of the
-// canonical key into 2^sb segments (b, s). Segments are counted (k_p3_count)
-// and scattered into one occurrence array with each occurrence's 10-bit
-// observation mask (k_p3_scatter). k_p3_dedup sorts and deduplicates one
-// segment per work-group (LDS bucket sort, LDS radix sort for uneven
-// buckets, or a direct-address table when <= 13 key bits vary), OR-joins
-// the masks of equal keys and sizes the segment's BBHash; segments above
-// the LDS capacity take a global-memory radix sort (k_p3_bigsort). Keys are
-// then compacted in (bin, segment) order, so every bin's keys come out
-// sorted, bins concatenated: the order of the reference's
-// cf1_dedup_all.bin. The BBHash of each segment is built in the dedup
-// kernel (k_bbh_big for segments above register capacity) into padded block
-// slots that the compaction packs densely.
+// P3 layout: the k-mers of bin b are split into 2^sb segments (b, s) by the
+// top sb bits of a 64-bit mix of the canonical key (seg_sub), so segments
+// are evenly sized. Segments are counted (k_p3_count) and scattered into one
+// occurrence array with each occurrence's 10-bit observation mask
+// (k_p3_scatter). k_p3_dedup sorts and deduplicates one segment per
+// work-group (LDS bucket sort, LDS radix sort for uneven buckets, or a
+// direct-address table for keys of <= 12 bits), OR-joins the masks of equal
+// keys and builds the segment's BBHash into a padded block slot; segments
+// above the LDS capacity take a global-memory radix sort (k_p3_bigsort) and
+// k_bbh_big. k_p3_compact then packs keys (in (bin, segment, key) order: the
+// MPHF key order), the masks (to MPHF-value order) and the blocks densely.
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -231,7 +229,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
   for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
   __syncthreads();
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[key_top(c, 2 * K, sb)], 1u); });
+    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
   __syncthreads();
   uint64_t* dst = counts + (uint64_t)bin * ns;
   for (int s = threadIdx.x; s < ns; s += P3_WG)
@@ -252,7 +250,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
   __syncthreads();
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[key_top(c, 2 * K, sb)], 1u); });
+    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
   __syncthreads();
   uint64_t* cur = cursor + (uint64_t)bin * ns;
   for (int s = threadIdx.x; s < ns; s += P3_WG)
@@ -260,7 +258,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   __syncthreads();
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
     for_each_rec_occ<KeyT, RW, true>(rec, r, K, [&](const KeyT& c, uint32_t m) {
-      uint32_t s = key_top(c, 2 * K, sb);
+      uint32_t s = seg_sub(c, sb);
       uint64_t at = gb[s] + (atomicSub(&hist[s], 1u) - 1u);
       occ[at] = c;
       occ_obs[at] = (uint16_t)m;
@@ -354,25 +352,24 @@ template <typename KeyT>
 struct SegCfg;
 template <>
 struct SegCfg<uint64_t> {
-  static constexpr int KPT = 16;
+  static constexpr int KPT = 8;
 };
 template <>
 struct SegCfg<Key128> {
   static constexpr int KPT = 8;
 };
-constexpr int BM_MAX_BITS = 13;   // direct-address mask table when <= 13 low key bits vary
-constexpr int BK_BITS = 12;       // bucket-sort buckets (next 12 key bits below the segment prefix)
+constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12 bits (k <= 5)
+constexpr int BK_BITS = 11;       // bucket-sort buckets (top 11 key bits)
 constexpr int BK_N = 1 << BK_BITS;
 constexpr uint32_t BK_MAX = 24;   // larger buckets (duplicate-heavy) switch to the LDS radix sort
 
-// A segment's MPHF is built through global memory (k_bbh_big) when its
-// distinct keys exceed the register capacity, or when it was sorted by
-// k_p3_bigsort (more occurrences than capacity and too many varying key bits
-// for the direct-address table).
+// A segment's MPHF is built by k_bbh_big when its distinct keys exceed the
+// register capacity, or when it was sorted by k_p3_bigsort (more
+// occurrences than capacity, keys too wide for the direct-address table).
 template <typename KeyT>
-__device__ __forceinline__ bool seg_is_big(uint64_t n, uint64_t u, int rbits) {
+__device__ __forceinline__ bool seg_is_big(uint64_t n, uint64_t u, int K2) {
   constexpr uint64_t CAP = (uint64_t)RK_WG * SegCfg<KeyT>::KPT;
-  const bool table = sizeof(KeyT) == 8 && rbits <= BM_MAX_BITS;
+  const bool table = sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS;
   return u > CAP || (n > CAP && !table);
 }
 
@@ -471,8 +468,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   if (seg >= n_segs) return;
   const uint64_t start = seg_off[seg];
   const uint64_t n = seg_off[seg + 1] - start;
-  const int rbits = K2 - sb;  // low bits that may vary in this segment
-  uint64_t u = 0;             // distinct keys written to occ[start...]
+  uint64_t u = 0;  // distinct keys written to occ[start...]
   if (n == 0) {
     if (threadIdx.x == 0) {
       ucount[seg] = 0;
@@ -480,14 +476,15 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
     return;
   }
-  if (sizeof(KeyT) == 8 && rbits <= BM_MAX_BITS) {
-    // ---- direct-address table of masks: any number of duplicates, sorted output.
-    uint32_t* tab = (uint32_t*)buf;  // 2^13 u32 = 32 KB = sizeof(buf)
-    const uint32_t nt = 1u << rbits;
+  if (sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS) {
+    // ---- direct-address table of masks over all 2^K2 keys: any number of
+    // duplicates, sorted output.
+    uint32_t* tab = (uint32_t*)buf;
+    static_assert((1 << BM_MAX_BITS) * 4 <= (int)sizeof(buf), "table fits");
+    const uint32_t nt = 1u << K2;
     for (uint32_t i = threadIdx.x; i < nt; i += RK_WG) tab[i] = 0;
     __syncthreads();
-    const uint64_t lowmask = (1ULL << rbits) - 1;
-    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) atomicOr(&tab[key_lo(occ[start + i]) & lowmask], (uint32_t)obs[start + i]);
+    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) atomicOr(&tab[key_lo(occ[start + i])], (uint32_t)obs[start + i]);
     __syncthreads();
     const uint32_t per = (nt + RK_WG - 1) / RK_WG;
     const uint32_t t0 = threadIdx.x * per;
@@ -496,12 +493,11 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       if (t0 + q < nt && tab[t0 + q]) c++;
     uint64_t tot;
     uint64_t o = block_excl_scan_u64(c, sh8, &tot);
-    const uint64_t prefix = (uint64_t)(seg & ((1ULL << sb) - 1)) << rbits;
     for (uint32_t q = 0; q < per; q++) {
       if (t0 + q >= nt) break;
       uint32_t m = tab[t0 + q];
       if (!m) continue;
-      if constexpr (sizeof(KeyT) == 8) occ[start + o] = (KeyT)(prefix | (uint64_t)(t0 + q));
+      if constexpr (sizeof(KeyT) == 8) occ[start + o] = (KeyT)(t0 + q);
       obs[start + o] = (uint16_t)m;
       o++;
     }
@@ -514,7 +510,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
     return;
   } else {
-    // ---- bucket sort on the 12 key bits below the segment prefix.
+    // ---- bucket sort on the top BK_BITS key bits.
     const int kpt = (int)((n + RK_WG - 1) / RK_WG);
     KeyT key[KPT];
     uint32_t msk[KPT];
@@ -532,7 +528,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
         }
       }
     }
-    const int bsh = rbits > BK_BITS ? rbits - BK_BITS : 0;
+    const int bsh = K2 > BK_BITS ? K2 - BK_BITS : 0;
     for (int i = threadIdx.x; i < BK_N; i += RK_WG) cnt[i] = 0;
     __syncthreads();
 #pragma unroll
@@ -667,7 +663,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   // ---- BBHash of the segment, into its padded block slot; then the joined
   // masks are permuted in place into local-rank order (obs[start + rank]).
   if (threadIdx.x == 0) ucount[seg] = u;
-  if (seg_is_big<KeyT>(n, u, rbits)) {
+  if (seg_is_big<KeyT>(n, u, K2)) {
     if (threadIdx.x == 0) {
       big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
       nblk[seg] = 0;
@@ -885,7 +881,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ o
   if (seg >= n_segs) return;
   const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
   if (u == 0) return;
-  const bool big = seg_is_big<KeyT>(seg_off[seg + 1] - s0, u, K2 - sb);
+  const bool big = seg_is_big<KeyT>(seg_off[seg + 1] - s0, u, K2);
   const uint16_t* m = big ? aux_obs : obs;
   for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
     keys[ko + i] = occ[s0 + i];

@@ -80,7 +80,7 @@ static void build_segment(const uint64_t* lo, const uint64_t* hi, uint64_t D, bo
   if (rank != D) throw std::runtime_error("MPHF: rank total mismatch");
 }
 
-void mphf_build(Mphf& out, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
+void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
                 const std::vector<uint16_t>& b2b, int K, int m, uint64_t W, uint64_t target, int threads) {
   out = Mphf();
   out.K = K;
@@ -89,28 +89,45 @@ void mphf_build(Mphf& out, const std::vector<uint64_t>& lo, const std::vector<ui
   out.n_keys = lo.size();
   out.b2b = b2b;
   const uint64_t bins = per_bin.size();
-  out.sb = segment_bits(W, bins, K, target);
+  out.sb = segment_bits(W, bins, target);
   const int sb = out.sb;
   const uint64_t ns = bins << sb;
   out.key_off.assign(ns + 1, 0);
-  // segment boundaries inside each sorted bin
-  uint64_t pos = 0;
-  for (uint64_t b = 0; b < bins; b++) {
-    uint64_t end = pos + per_bin[b];
-    uint64_t i = pos;
-    for (uint64_t s = 0; s < (1ULL << sb); s++) {
-      out.key_off[(b << sb) | s] = i;
-      while (i < end) {
-        u128 c = ((u128)hi[i] << 64) | lo[i];
-        uint64_t top = sb ? (uint64_t)(c >> (2 * K - sb)) : 0;
-        if (top != s) break;
-        i++;
-      }
+  // regroup each sorted bin by segment (a stable counting sort keeps the key
+  // order inside a segment)
+  std::vector<uint64_t> bin_start(bins + 1, 0);
+  for (uint64_t b = 0; b < bins; b++) bin_start[b + 1] = bin_start[b] + per_bin[b];
+  if (bin_start[bins] != lo.size()) throw std::runtime_error("MPHF: per-bin counts do not cover the keys");
+  std::vector<uint64_t> nlo(lo.size()), nhi(hi.size());
+  std::vector<uint64_t> cnt(ns + 1, 0);
+  parallel_for(bins, threads, [&](size_t b) {
+    const uint64_t a = bin_start[b], e = bin_start[b + 1];
+    uint64_t* c = cnt.data() + (b << sb);
+    for (uint64_t i = a; i < e; i++) {
+      if (i > a && !(((u128)hi[i - 1] << 64 | lo[i - 1]) < ((u128)hi[i] << 64 | lo[i])))
+        throw std::runtime_error("MPHF: keys of a bin are not sorted and distinct");
+      c[seg_sub(lo[i], hi[i], out.wide, sb)]++;
     }
-    if (i != end) throw std::runtime_error("MPHF: keys of a bin are not sorted");
-    pos = end;
+    std::vector<uint64_t> at(1ULL << sb);
+    uint64_t run = a;
+    for (uint64_t s = 0; s < (1ULL << sb); s++) {
+      at[s] = run;
+      run += c[s];
+    }
+    for (uint64_t i = a; i < e; i++) {
+      uint64_t d = at[seg_sub(lo[i], hi[i], out.wide, sb)]++;
+      nlo[d] = lo[i];
+      nhi[d] = hi[i];
+    }
+  });
+  uint64_t pos = 0;
+  for (uint64_t s = 0; s < ns; s++) {
+    out.key_off[s] = pos;
+    pos += cnt[s];
   }
   out.key_off[ns] = pos;
+  lo.swap(nlo);
+  hi.swap(nhi);
   std::vector<std::vector<uint64_t>> blk(ns);
   parallel_for(ns, threads, [&](size_t s) {
     uint64_t a = out.key_off[s], d = out.key_off[s + 1] - a;
@@ -245,7 +262,7 @@ class PipelineBase {
   std::vector<uint64_t> keys_lo, keys_hi;  // consolidated dedup keys
   std::vector<uint64_t> per_bin_distinct;
   Mphf mphf;
-  uint64_t target_seg = 0;  // 0 = default (2048 for K <= 32, 1024 above)
+  uint64_t target_seg = 0;  // 0 = default (1536)
   uint64_t n_kmers = 0;     // W, valid k-mer occurrences decoded in P3a
   AtomicStateVector states;
   PhaseTimes times;
@@ -258,7 +275,7 @@ class PipelineBase {
   virtual void p2_route() = 0;
   virtual void p3a() = 0;
   void p3b() {
-    uint64_t tgt = target_seg ? target_seg : (k > 32 ? 1024 : 2048);
+    uint64_t tgt = target_seg ? target_seg : 1536;
     mphf_build(mphf, keys_lo, keys_hi, per_bin_distinct, bucket_to_bin, k, m, n_kmers, tgt, threads);
   }
   virtual void p4() = 0;

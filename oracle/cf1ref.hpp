@@ -435,10 +435,12 @@ inline uint32_t mmer_order(uint32_t canon) { return fmix32(canon + 0x9E3779B9u);
 // invariant to the MPHF once unitig ids are canonicalised) is a
 // minimizer-partitioned BBHash:
 //   segment of key c = (bin, s): bin = bucket_to_bin[bucket_hash(v(c))] with
-//   v(c) the minimizer value of c; s = top sb bits of c. The consolidated keys
-//   (bins in order, each sorted) are already grouped by segment.
-//   sb = fewest bits (<= 12, <= 2K) with W <= bins * 2^sb * target
-//   (target 2048 for K <= 32, 1024 above; W = valid k-mer occurrences).
+//   v(c) the minimizer value of c; s = top sb bits of seg_hash(c) (the level
+//   hash below with seed bbh_seed(-1)), which spreads every bin evenly over
+//   its 2^sb segments. The MPHF key order is (bin, segment, key): each bin's
+//   sorted distinct keys, stably regrouped by segment.
+//   sb = fewest bits (<= 12) with W <= bins * 2^sb * target
+//   (target 1536; W = valid k-mer occurrences).
 //   per segment with D_s keys, level l has n_l = roundup64(max(2 t_l, 1)) bits,
 //   t_0 = D_s, t_{l+1} = ceil(9 t_l / 20); a key lands on
 //   off_l + mulhi64(hash(key, l), n_l) and is placed at the first level where
@@ -489,11 +491,12 @@ inline uint32_t minimizer_value(u128 c, int K, int m) {
   }
   return bv;
 }
-inline int segment_bits(uint64_t W, uint64_t bins, int K, uint64_t target) {
+inline int segment_bits(uint64_t W, uint64_t bins, uint64_t target) {
   int sb = 0;
-  while (sb < 12 && sb < 2 * K && W > bins * (1ULL << sb) * target) sb++;
+  while (sb < 12 && W > bins * (1ULL << sb) * target) sb++;
   return sb;
 }
+inline uint64_t seg_sub(uint64_t lo, uint64_t hi, bool wide, int sb) { return sb ? bbh_hash(lo, hi, wide, -1) >> (64 - sb) : 0; }
 
 struct Mphf {
   int K = 0, m = 0, sb = 0;
@@ -504,8 +507,7 @@ struct Mphf {
   uint64_t segment(u128 c) const {
     uint32_t v = minimizer_value(c, K, m);
     uint64_t bin = b2b[bucket_hash(v)];
-    uint64_t sub = sb ? (uint64_t)(c >> (2 * K - sb)) : 0;
-    return (bin << sb) | sub;
+    return (bin << sb) | seg_sub((uint64_t)c, (uint64_t)(c >> 64), wide, sb);
   }
   uint64_t lookup(uint64_t lo, uint64_t hi) const {
     u128 c = ((u128)hi << 64) | lo;
@@ -532,8 +534,9 @@ struct Mphf {
 };
 
 // Build over the consolidated keys (bins in order, each bin sorted) given the
-// per-bin distinct counts. Threads split the segments.
-void mphf_build(Mphf& out, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
+// per-bin distinct counts; lo/hi are regrouped in place into the MPHF key
+// order (bin, segment, key). Threads split the segments.
+void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
                 const std::vector<uint16_t>& b2b, int K, int m, uint64_t W, uint64_t target, int threads);
 
 // ---------------------------------------------------------------- threading
