@@ -20,11 +20,17 @@
 namespace cf1g {
 // kernels_p1p2.hip
 void launch_pack(hipStream_t, const uint8_t*, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t*, uint64_t*);
-void launch_p1_hist(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, unsigned long long*, unsigned long long*);
+uint64_t p1_tiles(uint64_t);
+uint64_t p1_tile_positions();
+uint64_t p1_slices(uint64_t);
+void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, uint32_t*, uint32_t*, uint64_t*,
+               unsigned long long*, unsigned long long*);
+void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
+void launch_p2_count(hipStream_t, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, const uint16_t*, int, uint64_t*);
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
                       uint64_t*, unsigned long long*, uint64_t*, uint64_t*);
-void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, const uint16_t*, int, const uint64_t*,
-                     unsigned long long*, uint64_t*, int, unsigned int*);
+void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, int,
+                     const uint16_t*, int, const uint64_t*, const uint64_t*, uint64_t*, int, unsigned int*);
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
 void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*, uint64_t*);
@@ -209,7 +215,12 @@ void do_p1(cf1g_ctx* c) {
   auto* tot = c->getT<unsigned long long>("totals", 4);
   c->zero(hist, CF1G_NUM_BUCKETS * 8);
   c->zero(tot, 32);
-  launch_p1_hist(st, seq, nm, c->n_pos, c->k, c->m, hist, tot);
+  // super k-mer entries: one slot of tile-size per scan tile; partial
+  // histograms per slice of tiles
+  const uint64_t nt = p1_tiles(c->n_pos), nsl = p1_slices(c->n_pos);
+  launch_p1(st, seq, nm, c->n_pos, c->k, c->m, c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1),
+            c->getT<uint32_t>("sk_tile_cnt", nt + 1), c->getT<uint32_t>("sk_tile_w", nt + 1),
+            c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1), c->getT<uint64_t>("p1_sums", 2 * nsl + 2), hist, tot);
   c->stage = 2;
 }
 
@@ -241,11 +252,24 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   unsigned int* ovf = c->getT<unsigned int>("route_overflow", 2);
   c->zero(ovf, 4);
-  launch_p2_route(st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), c->n_pos,
-                  c->k, c->m, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw, ovf);
+  // count records per (bin, chunk), scan bin-major, write
+  const uint64_t nt = p1_tiles(c->n_pos);
+  uint64_t chunk_tiles = 0, n_chunks = 0;
+  p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
+  const uint64_t nm_ = (uint64_t)c->num_bins * n_chunks;
+  uint64_t* mat = c->getT<uint64_t>("p2_mat", nm_ + 1);
+  uint64_t* moff = c->getT<uint64_t>("p2_off", nm_ + 1);
+  const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
+  const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
+  launch_p2_count(st, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, b2b, (int)c->num_bins, mat);
+  scan_excl_u64(st, mat, moff, nm_, moff + nm_, c->getT<uint64_t>("scan_tmp", scan_temp_words(nm_ + 1) + 16));
+  launch_p2_route(st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), ent, tcnt,
+                  c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, moff, bstart, rec, c->rw, ovf);
+  uint64_t routed = 0;
   unsigned int hov = 0;
+  if (nm_) c->d2h(&routed, moff + nm_, 8);
   c->d2h(&hov, ovf, 4);
-  if (hov) fail(CF1G_E_INTERNAL, "P2 routing overflowed a bin (bin sizes disagree with the histogram)");
+  if (hov || routed != c->S) fail(CF1G_E_INTERNAL, "P2 routing disagrees with the bin sizes from the histogram");
   c->stage = 3;
 }
 

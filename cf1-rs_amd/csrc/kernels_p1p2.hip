@@ -47,160 +47,337 @@ __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii,
   nmask[t] = nm;
 }
 
-// ------------------------------------------------------------------ minimizer tiles
+// ------------------------------------------------------------------ minimizer scan (P1)
 // Project minimizer scheme (same on the CPU oracle): order of an m-mer =
 // fmix32(canonical m-mer + c); the minimizer of the k-mer at i is the leftmost
 // least-order m-mer in [i, i+w); a super k-mer is a maximal run of valid
-// k-mers sharing the minimizer position; it routes by bucket_hash(value).
-constexpr int MZ_WG = 256;
-constexpr int MZ_T = 1024;         // k-mer positions per tile
-constexpr int MZ_MAXW = 64;        // w <= 49 for k <= 63
-constexpr int MZ_NJ = MZ_T + 2 * MZ_MAXW;
-constexpr int MZ_PPT = MZ_T / MZ_WG;
+// k-mers sharing the minimizer position, at most w long; it routes by
+// bucket_hash(value).
+//
+// k_p1_scan walks tiles of SK_T k-mer positions (work-groups loop over
+// tiles). Per tile it computes the m-mer orders of positions [t0-1, t0+T+2w)
+// into LDS (8 consecutive m-mers per thread from one 64-bit window and its
+// reverse complement), each position's minimizer index (sliding window) and
+// validity (no placeholder in its K bases), and emits one entry per super
+// k-mer starting in the tile into the tile's slot of the entry array:
+//   entry = start position << 24 | length (k-mers) << 16 | bucket.
+// tile_cnt[tile] = entries of the tile, tile_w[tile] = its valid k-mers.
+constexpr int SK_WG = 256;
+constexpr int SK_T = 2048;  // k-mer positions per tile
+constexpr int SK_Q = 8;     // consecutive m-mers / positions per thread step
+constexpr int SK_MAXW = 64;  // w <= 49 for k <= 63
+constexpr int SK_NJ = SK_T + 2 * SK_MAXW;
 constexpr int MAX_BINS = 4096;
 
-struct MzTile {
-  uint32_t ord[MZ_NJ];
-  uint32_t can[MZ_NJ];
-  uint16_t pidx[MZ_T + MZ_MAXW + 1];
-  uint8_t valid[MZ_T + MZ_MAXW + 1];
+// LDS index of tile slot j: a gap after every 8 slots, so that lanes working
+// on 8 consecutive slots each (stride 9 words apart) hit distinct banks.
+__device__ __forceinline__ int PJ(int j) { return j + (j >> 3); }
+constexpr int SK_NJP = SK_NJ + SK_NJ / 8 + 8;
+constexpr int SK_NPP = (SK_T + SK_MAXW + 8) * 9 / 8 + 8;
+constexpr int SK_BRK = (SK_T + 2 * SK_MAXW) / 8 + 16;  // bytes of break flags (bit j%8 of byte j/8)
+struct SkTile {
+  uint32_t ord[SK_NJP];
+  uint32_t can[SK_NJP];
+  uint16_t pidx[SK_NPP];
+  uint8_t valid[SK_NPP];
+  uint64_t brk[SK_BRK / 8];
+  uint64_t sh[4];
 };
 
-// Fills the tile for k-mer positions [t0, t0+MZ_T): orders of m-mers at
-// [t0-1, t0-1+NJ), minimizer positions (relative to t0-1) and validity of
-// k-mers at [t0-1, t0+T+w).
-__device__ __forceinline__ void mz_fill(MzTile& S, const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                        uint64_t t0, int K, int m, int w) {
-  const uint64_t base = t0 - 1;
-  const int nj = MZ_T + 2 * w;
-  for (int j = threadIdx.x; j < nj; j += MZ_WG) {
-    uint64_t g = base + (uint64_t)j;
-    uint32_t f = (uint32_t)(stream_bits64(seq, 2 * g) >> (64 - 2 * m));
-    uint32_t r = (uint32_t)rc64(f, m);
-    uint32_t c = f < r ? f : r;
-    S.can[j] = c;
-    S.ord[j] = mmer_order(c);
+__device__ __forceinline__ uint64_t sk_entry(uint64_t pos, uint32_t n, uint32_t bucket) {
+  return (pos << 24) | ((uint64_t)n << 16) | (uint64_t)bucket;
+}
+__device__ __forceinline__ uint64_t sk_pos(uint64_t e) { return e >> 24; }
+__device__ __forceinline__ uint32_t sk_len(uint64_t e) { return (uint32_t)(e >> 16) & 255u; }
+__device__ __forceinline__ uint32_t sk_bucket(uint64_t e) { return (uint32_t)e & 0xFFFFu; }
+
+__device__ __forceinline__ uint64_t sk_scan_excl(uint64_t v, uint64_t* sh4, uint64_t* total) {
+  uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
+  uint64_t x = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up(x, d, 64);
+    if ((int)lane >= d) x += y;
+  }
+  if (lane == 63) sh4[wv] = x;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+  for (int q = 0; q < SK_WG / 64; q++) {
+    uint64_t t = sh4[q];
+    if (q < (int)wv) off += t;
+    tot += t;
   }
   __syncthreads();
-  const int np = MZ_T + w + 1;
-  for (int j = threadIdx.x; j < np; j += MZ_WG) {
-    uint64_t g = base + (uint64_t)j;
-    uint32_t best = S.ord[j];
-    int bi = j;
-    for (int q = 1; q < w; q++) {
-      uint32_t o = S.ord[j + q];
-      if (o < best) { best = o; bi = j + q; }
-    }
-    S.pidx[j] = (uint16_t)bi;
-    S.valid[j] = window_clear(nmask, g, K) ? 1 : 0;
-  }
-  __syncthreads();
+  if (total) *total = tot;
+  return off + x - v;
 }
+__device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-__device__ __forceinline__ bool mz_is_start(const MzTile& S, int j) {  // j relative to t0-1, j >= 1
-  return S.valid[j] && (!S.valid[j - 1] || S.pidx[j] != S.pidx[j - 1]);
-}
-
-// P1: histogram of super k-mers per bucket; also counts valid k-mers (W) and
-// super k-mers (S).
-__global__ void __launch_bounds__(MZ_WG) k_p1_hist(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                   uint64_t n_pos, int K, int m, unsigned long long* __restrict__ hist,
-                                                   unsigned long long* __restrict__ totals) {
-  __shared__ MzTile S;
-  __shared__ uint32_t red[2];
+__global__ void __launch_bounds__(SK_WG) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+                                                   uint64_t n_pos, uint64_t n_tiles, int K, int m,
+                                                   uint64_t* __restrict__ entries, uint32_t* __restrict__ tile_cnt,
+                                                   uint32_t* __restrict__ tile_w) {
+  __shared__ SkTile S;
   const int w = K - m + 1;
-  uint64_t t0 = 1 + (uint64_t)blockIdx.x * MZ_T;
-  if (threadIdx.x < 2) red[threadIdx.x] = 0;
-  mz_fill(S, seq, nmask, t0, K, m, w);
-  uint32_t nw = 0, ns = 0;
-  for (int q = 0; q < MZ_PPT; q++) {
-    int li = threadIdx.x + q * MZ_WG;
-    uint64_t i = t0 + (uint64_t)li;
-    if (i >= n_pos) break;
-    int j = li + 1;
-    if (!S.valid[j]) continue;
-    nw++;
-    if (mz_is_start(S, j)) {
-      ns++;
-      atomicAdd(&hist[bucket_hash((uint64_t)S.can[S.pidx[j]])], 1ULL);
+  const uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
+  uint8_t* brk8 = (uint8_t*)S.brk;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint64_t t0 = 1 + tile * SK_T;  // first k-mer position of the tile
+    const uint64_t B = t0 - 1;            // tile slot 0 <-> stream position B
+    // (1) m-mer orders of slots [0, T + 2w)
+    const int nj = SK_T + 2 * w;
+    for (int j0 = threadIdx.x * SK_Q; j0 < nj; j0 += SK_WG * SK_Q) {
+      const uint64_t x = stream_bits64(seq, 2 * (B + (uint64_t)j0));
+      const uint64_t rx = rc64(x, 32);
+#pragma unroll
+      for (int q = 0; q < SK_Q; q++) {
+        const uint32_t f = (uint32_t)(x >> (64 - 2 * m - 2 * q)) & M;
+        const uint32_t r = (uint32_t)(rx >> (2 * q)) & M;
+        const uint32_t c = f < r ? f : r;
+        if (j0 + q < nj) {
+          S.can[PJ(j0 + q)] = c;
+          S.ord[PJ(j0 + q)] = mmer_order(c);
+        }
+      }
     }
-  }
-  atomicAdd(&red[0], nw);
-  atomicAdd(&red[1], ns);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(&totals[0], (unsigned long long)red[0]);
-    atomicAdd(&totals[1], (unsigned long long)red[1]);
+    __syncthreads();
+    // (2) minimizer slot and validity of the k-mers at slots [0, T + w + 1).
+    // Window minima of (order, slot): for 8 consecutive windows [q, q+w-1]
+    // (w >= 8), min(suffix min of [q, 7), min of the common core [7, w),
+    // prefix min of [w, q+w)); the slot in the low bits makes ties pick the
+    // leftmost m-mer.
+    const int np = SK_T + w + 1;
+    for (int j0 = threadIdx.x * SK_Q; j0 < np; j0 += SK_WG * SK_Q) {
+      const uint64_t g = B + (uint64_t)j0;
+      const uint64_t n0 = stream_bits64(nmask, g), n1 = stream_bits64(nmask, g + 64);
+      auto key = [&](int u) -> uint64_t { return ((uint64_t)S.ord[PJ(j0 + u)] << 16) | (uint64_t)(j0 + u); };
+      uint64_t win[SK_Q];
+      if (w >= SK_Q) {
+        uint64_t core = ~0ULL;
+#pragma unroll 4
+        for (int u = SK_Q - 1; u < w; u++) core = min64(core, key(u));
+        uint64_t suf = ~0ULL, pre = ~0ULL;
+        uint64_t L[SK_Q], R[SK_Q];
+#pragma unroll
+        for (int q = SK_Q - 1; q >= 0; q--) {
+          L[q] = suf;  // min over [q, 7) after the update below
+          if (q < SK_Q - 1) {
+            suf = min64(suf, key(q));
+            L[q] = suf;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < SK_Q; q++) {
+          R[q] = pre;  // min over [w, w+q)
+          pre = min64(pre, key(w + q));
+        }
+#pragma unroll
+        for (int q = 0; q < SK_Q; q++) win[q] = min64(min64(L[q], core), R[q]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < SK_Q; q++) {
+          uint64_t b = ~0ULL;
+          for (int u = 0; u < w; u++) b = min64(b, key(q + u));
+          win[q] = b;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < SK_Q; q++) {
+        const int j = j0 + q;
+        if (j < np) {
+          // bases [g+q, g+q+K) of the N mask: bits q .. q+K-1 of n0:n1
+          const uint64_t hi = q ? ((n0 << q) | (n1 >> (64 - q))) : n0;
+          S.pidx[PJ(j)] = (uint16_t)(win[q] & 0xFFFF);
+          S.valid[PJ(j)] = (hi >> (64 - K)) == 0 ? 1 : 0;
+        }
+      }
+    }
+    __syncthreads();
+    // (3) break flags of slots [1, T + w]: slot j breaks a run unless it and
+    // slot j-1 are valid with the same minimizer slot. Byte j/8, bit j%8.
+    for (int j0 = threadIdx.x * SK_Q; j0 < np; j0 += SK_WG * SK_Q) {
+      uint32_t bits = 0;
+#pragma unroll
+      for (int q = 0; q < SK_Q; q++) {
+        const int j = j0 + q;
+        bool br = true;
+        if (j >= 1 && j < np)
+          br = !(S.valid[PJ(j)] && S.valid[PJ(j - 1)] && S.pidx[PJ(j)] == S.pidx[PJ(j - 1)]);
+        bits |= (uint32_t)br << q;
+      }
+      brk8[j0 >> 3] = (uint8_t)bits;
+    }
+    __syncthreads();
+    // (4) super k-mer starts at positions t0 + [0, T) (slot j = 1 + li): a
+    // valid slot with its break flag set. Its length runs to the next break.
+    uint32_t starts = 0, valid_cnt = 0;
+    const int li0 = threadIdx.x * SK_Q;
+#pragma unroll
+    for (int q = 0; q < SK_Q; q++) {
+      const int j = li0 + q + 1;
+      if (t0 + (uint64_t)(li0 + q) >= n_pos || !S.valid[PJ(j)]) continue;
+      valid_cnt++;
+      if ((brk8[j >> 3] >> (j & 7)) & 1) starts++;
+    }
+    uint64_t wt;
+    sk_scan_excl(valid_cnt, S.sh, &wt);
+    uint64_t tot;
+    uint64_t o = sk_scan_excl(starts, S.sh, &tot);
+    uint64_t* dst = entries + tile * SK_T;
+#pragma unroll
+    for (int q = 0; q < SK_Q; q++) {
+      const int j = li0 + q + 1;
+      if (t0 + (uint64_t)(li0 + q) >= n_pos || !S.valid[PJ(j)]) continue;
+      if (!((brk8[j >> 3] >> (j & 7)) & 1)) continue;
+      // next break after j: bits j+1 .. of the break flags (within w <= 49)
+      const int wi = (j + 1) >> 6, sh = (j + 1) & 63;
+      const uint64_t lo = S.brk[wi], hi = S.brk[wi + 1];
+      const uint64_t after = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+      const uint32_t n = (uint32_t)__builtin_ctzll(after) + 1;
+      dst[o++] = sk_entry(t0 + (uint64_t)(li0 + q), n, bucket_hash((uint64_t)S.can[PJ((int)S.pidx[PJ(j)])]));
+    }
+    if (threadIdx.x == 0) {
+      tile_cnt[tile] = (uint32_t)tot;
+      tile_w[tile] = (uint32_t)wt;
+    }
+    __syncthreads();  // LDS reuse by the next tile
   }
 }
 
-// P2 route: each super k-mer [i, i+n) becomes one fixed-size record in its
-// bin (the bin record of superkmer.rs:181-202 in fixed width): its L = n-1+K
+// P1 histogram from the entries, without device atomics: work-group (slice,
+// range r) counts the buckets in [r*16384, (r+1)*16384) of a slice of tiles
+// in LDS (each wave takes one tile at a time) and writes its partial counts
+// to part[(slice*4 + r)*16384 ...]; the r = 0 work-groups write the slice's
+// W and S to sums[2*slice ...]. k_p1_reduce adds the partials up.
+constexpr int HB_RANGE = 16384;
+constexpr int HB_SLICES = 512;
+__global__ void __launch_bounds__(256) k_p1_hist(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
+                                                 const uint32_t* __restrict__ tile_w, uint64_t n_tiles, uint64_t tiles_per_slice,
+                                                 uint32_t* __restrict__ part, uint64_t* __restrict__ sums) {
+  __shared__ uint32_t h[HB_RANGE];
+  __shared__ uint64_t red[8];
+  const uint32_t r = blockIdx.y;
+  for (int i = threadIdx.x; i < HB_RANGE; i += 256) h[i] = 0;
+  __syncthreads();
+  const uint64_t ta = (uint64_t)blockIdx.x * tiles_per_slice;
+  uint64_t tb = ta + tiles_per_slice;
+  if (tb > n_tiles) tb = n_tiles;
+  const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
+  uint64_t s = 0, wsum = 0;
+  for (uint64_t t = ta + wv; t < tb; t += 4) {
+    const uint32_t c = tile_cnt[t];
+    const uint64_t* e = entries + t * SK_T;
+    for (uint32_t i = lane; i < c; i += 64) {
+      const uint32_t b = sk_bucket(e[i]);
+      if ((b / HB_RANGE) == r) atomicAdd(&h[b % HB_RANGE], 1u);
+    }
+    s += c;
+    wsum += tile_w[t];
+  }
+  if (lane == 0) {
+    red[wv] = s;
+    red[4 + wv] = wsum;
+  }
+  __syncthreads();
+  uint32_t* dst = part + ((uint64_t)blockIdx.x * (65536 / HB_RANGE) + r) * HB_RANGE;
+  for (int i = threadIdx.x; i < HB_RANGE; i += 256) dst[i] = h[i];
+  if (r == 0 && threadIdx.x == 0) {
+    sums[2 * blockIdx.x] = red[4] + red[5] + red[6] + red[7];
+    sums[2 * blockIdx.x + 1] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+// hist[b] = sum over slices of the partial counts; totals = {W, S}.
+__global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ part, const uint64_t* __restrict__ sums,
+                                                   uint32_t n_slices, unsigned long long* __restrict__ hist,
+                                                   unsigned long long* __restrict__ totals) {
+  const uint32_t b = blockIdx.x * 256 + threadIdx.x;  // grid covers 65536 buckets
+  uint64_t acc = 0;
+  for (uint32_t sl = 0; sl < n_slices; sl++) acc += part[(uint64_t)sl * 65536 + b];
+  hist[b] = acc;
+  if (b < 2) {
+    uint64_t t = 0;
+    for (uint32_t sl = 0; sl < n_slices; sl++) t += sums[2 * sl + b];
+    totals[b] = t;
+  }
+}
+
+// ------------------------------------------------------------------ route (P2)
+// Chunks of `chunk_tiles` tiles. k_p2_count writes each chunk's records per
+// bin into mat[bin * n_chunks + chunk]; an exclusive scan of mat (bin-major)
+// gives every (bin, chunk) run its first record; k_p2_route writes the
+// records. Each super k-mer [i, i+n) becomes one fixed-size record in its bin
+// (the bin record of superkmer.rs:181-202 in fixed width): its L = n-1+K
 // bases plus one flanking base on each side, MSB-first in RW words; the last
 // word's low byte holds L and bits 8/9 say whether the left/right flank is a
 // real base (not a placeholder or record end). The flanks carry the
 // neighbours P4's observations need (classify.rs:76-77).
-template <int RW>
-__global__ void __launch_bounds__(MZ_WG) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                    uint64_t n_pos, int K, int m, const uint16_t* __restrict__ b2b,
-                                                    int num_bins, const uint64_t* __restrict__ bin_start,
-                                                    unsigned long long* __restrict__ bin_cursor, uint64_t* __restrict__ rec,
-                                                    unsigned int* __restrict__ overflow) {
-  __shared__ MzTile S;
+__global__ void __launch_bounds__(256) k_p2_count(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
+                                                  uint64_t n_tiles, uint64_t chunk_tiles, uint64_t n_chunks,
+                                                  const uint16_t* __restrict__ b2b, int num_bins, uint64_t* __restrict__ mat) {
   __shared__ uint32_t cnt[MAX_BINS];
-  __shared__ unsigned long long gbase[MAX_BINS];
-  const int w = K - m + 1;
-  uint64_t t0 = 1 + (uint64_t)blockIdx.x * MZ_T;
-  for (int b = threadIdx.x; b < num_bins; b += MZ_WG) cnt[b] = 0;
-  mz_fill(S, seq, nmask, t0, K, m, w);
-  int my_bin[MZ_PPT];
-  uint32_t my_rank[MZ_PPT];
-  for (int q = 0; q < MZ_PPT; q++) {
-    my_bin[q] = -1;
-    int li = threadIdx.x + q * MZ_WG;
-    uint64_t i = t0 + (uint64_t)li;
-    if (i >= n_pos) continue;
-    int j = li + 1;
-    if (!mz_is_start(S, j)) continue;
-    int bin = b2b[bucket_hash((uint64_t)S.can[S.pidx[j]])];
-    my_bin[q] = bin;
-    my_rank[q] = atomicAdd(&cnt[bin], 1u);
+  const uint64_t chunk = blockIdx.x;
+  for (int b = threadIdx.x; b < num_bins; b += 256) cnt[b] = 0;
+  __syncthreads();
+  const uint64_t ta = chunk * chunk_tiles;
+  uint64_t tb = ta + chunk_tiles;
+  if (tb > n_tiles) tb = n_tiles;
+  for (uint64_t t = ta; t < tb; t++) {
+    const uint32_t c = tile_cnt[t];
+    const uint64_t* e = entries + t * SK_T;
+    for (uint32_t i = threadIdx.x; i < c; i += 256) atomicAdd(&cnt[b2b[sk_bucket(e[i])]], 1u);
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < num_bins; b += MZ_WG)
-    if (cnt[b]) gbase[b] = atomicAdd(&bin_cursor[b], (unsigned long long)cnt[b]);
+  for (int b = threadIdx.x; b < num_bins; b += 256) mat[(uint64_t)b * n_chunks + chunk] = cnt[b];
+}
+
+template <int RW>
+__global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+                                                  const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
+                                                  uint64_t n_tiles, uint64_t chunk_tiles, uint64_t n_chunks, int K,
+                                                  const uint16_t* __restrict__ b2b, int num_bins,
+                                                  const uint64_t* __restrict__ off, const uint64_t* __restrict__ bin_start,
+                                                  uint64_t* __restrict__ rec, unsigned int* __restrict__ overflow) {
+  __shared__ uint32_t cnt[MAX_BINS];
+  __shared__ uint64_t base[MAX_BINS];
+  const uint64_t chunk = blockIdx.x;
+  for (int b = threadIdx.x; b < num_bins; b += 256) {
+    cnt[b] = 0;
+    base[b] = off[(uint64_t)b * n_chunks + chunk];
+  }
   __syncthreads();
-  for (int q = 0; q < MZ_PPT; q++) {
-    if (my_bin[q] < 0) continue;
-    int li = threadIdx.x + q * MZ_WG;
-    int j = li + 1;
-    uint32_t p = S.pidx[j];
-    int n = 1;
-    while (n < w && S.valid[j + n] && S.pidx[j + n] == p) n++;
-    uint32_t L = (uint32_t)(n + K - 1);
-    uint64_t i = t0 + (uint64_t)li;
-    uint64_t out = bin_start[my_bin[q]] + gbase[my_bin[q]] + my_rank[q];
-    if (out >= bin_start[my_bin[q] + 1]) {  // never write past the bin
-      atomicOr(overflow, 1u);
-      continue;
-    }
-    // bases [i-1, i+L]: the super k-mer with one flanking base each side
-    const uint32_t LS = L + 2;
-    uint64_t wv[RW];
+  const uint64_t ta = chunk * chunk_tiles;
+  uint64_t tb = ta + chunk_tiles;
+  if (tb > n_tiles) tb = n_tiles;
+  for (uint64_t t = ta; t < tb; t++) {
+    const uint32_t c = tile_cnt[t];
+    const uint64_t* e = entries + t * SK_T;
+    for (uint32_t i = threadIdx.x; i < c; i += 256) {
+      const uint64_t en = e[i];
+      const uint32_t bin = b2b[sk_bucket(en)];
+      const uint64_t out = base[bin] + atomicAdd(&cnt[bin], 1u);
+      if (out >= bin_start[bin + 1] || out < bin_start[bin]) {  // never write outside the bin
+        atomicOr(overflow, 1u);
+        continue;
+      }
+      const uint64_t pos = sk_pos(en);
+      const uint32_t L = sk_len(en) + (uint32_t)K - 1;
+      // bases [pos-1, pos+L]: the super k-mer with one flanking base each side
+      const uint32_t LS = L + 2;
+      uint64_t wv[RW];
 #pragma unroll
-    for (int t = 0; t < RW; t++) {
-      int lo_bit = 64 * t;  // record bit offset of this word
-      uint64_t x = stream_bits64(seq, 2 * (i - 1) + (uint64_t)lo_bit);
-      int keep = (int)(2 * LS) - lo_bit;  // record bits living in this word
-      if (keep <= 0) x = 0;
-      else if (keep < 64) x &= ~0ULL << (64 - keep);
-      wv[t] = x;
-    }
-    const uint64_t lf = is_n(nmask, i - 1) ? 0 : 1, rf = is_n(nmask, i + L) ? 0 : 1;
-    wv[RW - 1] |= (uint64_t)L | (lf << 8) | (rf << 9);
-    uint64_t* dst = rec + out * RW;
+      for (int q = 0; q < RW; q++) {
+        const int lo_bit = 64 * q;  // record bit offset of this word
+        uint64_t x = stream_bits64(seq, 2 * (pos - 1) + (uint64_t)lo_bit);
+        const int keep = (int)(2 * LS) - lo_bit;  // record bits living in this word
+        if (keep <= 0) x = 0;
+        else if (keep < 64) x &= ~0ULL << (64 - keep);
+        wv[q] = x;
+      }
+      const uint64_t lf = is_n(nmask, pos - 1) ? 0 : 1, rf = is_n(nmask, pos + L) ? 0 : 1;
+      wv[RW - 1] |= (uint64_t)L | (lf << 8) | (rf << 9);
+      uint64_t* dst = rec + out * RW;
 #pragma unroll
-    for (int t = 0; t < RW; t++) dst[t] = wv[t];
+      for (int q = 0; q < RW; q++) dst[q] = wv[q];
+    }
   }
 }
 
@@ -367,11 +544,52 @@ void launch_pack(hipStream_t st, const uint8_t* ascii, const uint64_t* rec_off, 
   dim3 grid((unsigned)((n_words64 + 255) / 256));
   CF1G_LAUNCH("pack", k_pack, grid, dim3(256), 0, st, ascii, rec_off, gstart, n_rec, n_words64, seq, nmask);
 }
-void launch_p1_hist(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m,
-                    unsigned long long* hist, unsigned long long* totals) {
-  if (n_pos <= 1) return;
-  dim3 grid((unsigned)((n_pos - 1 + MZ_T - 1) / MZ_T));
-  CF1G_LAUNCH("p1_minimizer_hist", k_p1_hist, grid, dim3(MZ_WG), 0, st, seq, nmask, n_pos, K, m, hist, totals);
+uint64_t p1_tiles(uint64_t n_pos) { return n_pos > 1 ? (n_pos - 1 + SK_T - 1) / SK_T : 0; }
+uint64_t p1_tile_positions() { return SK_T; }
+uint64_t p1_slices(uint64_t n_pos) {
+  const uint64_t nt = p1_tiles(n_pos);
+  return nt < (uint64_t)HB_SLICES ? nt : (uint64_t)HB_SLICES;
+}
+void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m, uint64_t* entries,
+               uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
+               unsigned long long* totals) {
+  const uint64_t nt = p1_tiles(n_pos);
+  if (nt == 0) return;
+  const uint64_t grid = nt < 4096 ? nt : 4096;
+  CF1G_LAUNCH("p1_scan", k_p1_scan, dim3((unsigned)grid), dim3(SK_WG), 0, st, seq, nmask, n_pos, nt, K, m, entries, tile_cnt, tile_w);
+  const uint64_t slices = p1_slices(n_pos);
+  const uint64_t per = (nt + slices - 1) / slices;
+  const uint64_t used = (nt + per - 1) / per;
+  CF1G_LAUNCH("p1_hist", k_p1_hist, dim3((unsigned)used, 65536 / HB_RANGE), dim3(256), 0, st, entries, tile_cnt, tile_w, nt, per,
+              part, sums);
+  CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 256), dim3(256), 0, st, part, sums, (uint32_t)used, hist, totals);
+}
+// Route chunking: chunk_tiles tiles per chunk, keeping the count matrix at
+// <= 2^22 entries.
+void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint64_t* n_chunks) {
+  const uint64_t nt = p1_tiles(n_pos);
+  uint64_t ct = 16;
+  while (nt > 0 && ((nt + ct - 1) / ct) * num_bins > (1ULL << 22)) ct *= 2;
+  *chunk_tiles = ct;
+  *n_chunks = (nt + ct - 1) / ct;
+}
+void launch_p2_count(hipStream_t st, const uint64_t* entries, const uint32_t* tile_cnt, uint64_t n_pos, uint64_t chunk_tiles,
+                     uint64_t n_chunks, const uint16_t* b2b, int num_bins, uint64_t* mat) {
+  if (n_chunks == 0) return;
+  CF1G_LAUNCH("p2_count", k_p2_count, dim3((unsigned)n_chunks), dim3(256), 0, st, entries, tile_cnt, p1_tiles(n_pos), chunk_tiles,
+              n_chunks, b2b, num_bins, mat);
+}
+void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint32_t* tile_cnt,
+                     uint64_t n_pos, uint64_t chunk_tiles, uint64_t n_chunks, int K, const uint16_t* b2b, int num_bins,
+                     const uint64_t* off, const uint64_t* bin_start, uint64_t* rec, int rw, unsigned int* overflow) {
+  if (n_chunks == 0) return;
+  const uint64_t nt = p1_tiles(n_pos);
+  if (rw == 2)
+    CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), 0, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
+                n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
+  else
+    CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), 0, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
+                n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
 }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,
                       uint32_t chunk_records, uint16_t* b2b, uint64_t* bin_records, uint64_t* bin_start,
@@ -379,17 +597,4 @@ void launch_partition(hipStream_t st, const unsigned long long* hist, const unsi
   CF1G_LAUNCH("p2_partition", k_partition, dim3(1), dim3(PT_WG), 0, st, hist, hist_true, num_bins, chunk_records, b2b,
               bin_records, bin_start, bin_cursor, chunk_off, info);
 }
-void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m,
-                     const uint16_t* b2b, int num_bins, const uint64_t* bin_start, unsigned long long* bin_cursor,
-                     uint64_t* rec, int rw, unsigned int* overflow) {
-  if (n_pos <= 1) return;
-  dim3 grid((unsigned)((n_pos - 1 + MZ_T - 1) / MZ_T));
-  if (rw == 2)
-    CF1G_LAUNCH("p2_route", k_p2_route<2>, grid, dim3(MZ_WG), 0, st, seq, nmask, n_pos, K, m, b2b, num_bins, bin_start,
-                bin_cursor, rec, overflow);
-  else
-    CF1G_LAUNCH("p2_route", k_p2_route<4>, grid, dim3(MZ_WG), 0, st, seq, nmask, n_pos, K, m, b2b, num_bins, bin_start,
-                bin_cursor, rec, overflow);
-}
-
 }  // namespace cf1g
