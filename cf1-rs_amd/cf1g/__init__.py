@@ -33,8 +33,14 @@ EXPORTS = [
     "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_profiling", "cf1g_kernel_times",
-    "cf1g_device_keys",
+    "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
+    "cf1g_lidx_gradients", "cf1g_classify_via", "cf1g_select_index", "cf1g_selected_index",
 ]
+
+# Learned-index records (the layouts of include/cf1g.h).
+ROOT_DTYPE = np.dtype([("A", "<f4"), ("B", "<f4"), ("lo", "<u4"), ("hi", "<u4")])
+LEAF_DTYPE = np.dtype([("base", "<u8"), ("q", "<u8"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"), ("ehi", "<u2"),
+                       ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
 
 
 class Cf1gError(RuntimeError):
@@ -53,6 +59,24 @@ class CfInfo(C.Structure):
                 ("n_superkmers", C.c_uint64), ("n_distinct", C.c_uint64), ("n_state_words", C.c_uint64),
                 ("mphf_words", C.c_uint64), ("num_bins", C.c_uint32), ("seg_bits", C.c_uint32),
                 ("mphf_big_segments", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32)]
+
+
+class LidxParams(C.Structure):
+    _fields_ = [("leaf_bits", C.c_uint32), ("root_bits", C.c_uint32), ("epochs", C.c_uint32), ("lr", C.c_float),
+                ("max_window", C.c_uint32), ("untrained", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class LidxInfo(C.Structure):
+    _fields_ = [("n_keys", C.c_uint64), ("n_leaves", C.c_uint64), ("n_pieces", C.c_uint64),
+                ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("tie_keys", C.c_uint64),
+                ("max_err_lo", C.c_uint32), ("max_err_hi", C.c_uint32), ("mean_window", C.c_double),
+                ("mean_abs_err", C.c_double), ("root_loss", C.c_double * 2), ("leaf_loss", C.c_double * 2),
+                ("sort_ms", C.c_double), ("train_ms", C.c_double)]
+
+
+class LidxShape(C.Structure):
+    _fields_ = [("n_keys", C.c_uint64), ("n_pieces", C.c_uint32), ("n_leaves", C.c_uint32), ("root_shift", C.c_int32),
+                ("reserved", C.c_uint32)]
 
 
 def lib():
@@ -78,6 +102,16 @@ def lib():
         L.cf1g_mphf_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
         L.cf1g_mphf_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
         L.cf1g_annotate.argtypes = [C.c_void_p, U8P, C.c_uint64]
+        L.cf1g_lidx_train.argtypes = [C.c_void_p, C.POINTER(LidxParams), C.POINTER(LidxInfo)]
+        L.cf1g_lidx_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
+        L.cf1g_lidx_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.cf1g_lidx_export.argtypes = [C.c_void_p, C.POINTER(LidxShape), U64P, C.c_void_p, C.c_void_p]
+        F32P = C.POINTER(C.c_float)
+        F64P = C.POINTER(C.c_double)
+        L.cf1g_lidx_gradients.argtypes = [C.c_void_p, C.c_int, F32P, F32P, C.c_uint64, F64P, F64P, F64P, U64P, F32P, F32P]
+        L.cf1g_classify_via.argtypes = [C.c_void_p, C.c_int]
+        L.cf1g_select_index.argtypes = [C.c_void_p, C.c_int]
+        L.cf1g_selected_index.argtypes = [C.c_void_p]
         L.cf1g_record_starts.argtypes = [C.c_void_p, U64P, C.c_uint64]
         L.cf1g_bins_export.argtypes = [C.c_void_p, U32P, U64P, U64P, C.c_uint64]
         L.cf1g_p5_write.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64, C.c_char_p, U64P, C.c_char_p, C.c_char_p,
@@ -282,6 +316,60 @@ class Cf1g:
             name, cnt, ms = line.split()
             res[name] = (int(cnt), float(ms))
         return res
+
+
+def _lidx_methods():
+    """Learned rank index methods of Cf1g (kept together here)."""
+
+    def lidx_train(self, leaf_bits=0, root_bits=0, epochs=0, lr=0.0, max_window=0, untrained=False):
+        p = LidxParams(leaf_bits, root_bits, epochs, lr, max_window, 1 if untrained else 0, (C.c_uint32 * 2)())
+        o = LidxInfo()
+        self._check(self.L.cf1g_lidx_train(self.h, C.byref(p), C.byref(o)))
+        d = {f: getattr(o, f) for f, _ in LidxInfo._fields_}
+        d["root_loss"] = list(o.root_loss)
+        d["leaf_loss"] = list(o.leaf_loss)
+        return d
+
+    def lidx_lookup(self, keys):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        out = np.zeros(max(len(keys), 1), np.uint64)
+        self._check(self.L.cf1g_lidx_lookup(self.h, _p(keys, U64P), len(keys), _p(out, U64P)))
+        return out[:len(keys)]
+
+    def lidx_export(self):
+        sh = LidxShape()
+        self._check(self.L.cf1g_lidx_export(self.h, C.byref(sh), None, None, None))
+        keys = np.zeros(max(sh.n_keys, 1), np.uint64)
+        root = np.zeros(sh.n_pieces, ROOT_DTYPE)
+        leaf = np.zeros(sh.n_leaves, LEAF_DTYPE)
+        self._check(self.L.cf1g_lidx_export(self.h, None, _p(keys, U64P), root.ctypes.data_as(C.c_void_p),
+                                            leaf.ctypes.data_as(C.c_void_p)))
+        return {"keys": keys[:sh.n_keys], "root": root, "leaf": leaf, "shift": sh.root_shift}
+
+    def lidx_gradients(self, level, w, c):
+        w = np.ascontiguousarray(w, np.float32)
+        c = np.ascontiguousarray(c, np.float32)
+        n = len(w)
+        gw, gc, sq = np.zeros(n), np.zeros(n), np.zeros(n)
+        seg = np.zeros(n + 1, np.uint64)
+        mu, isd = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        F32P = C.POINTER(C.c_float)
+        F64P = C.POINTER(C.c_double)
+        self._check(self.L.cf1g_lidx_gradients(self.h, level, _p(w, F32P), _p(c, F32P), n, _p(gw, F64P), _p(gc, F64P),
+                                               _p(sq, F64P), _p(seg, U64P), _p(mu, F32P), _p(isd, F32P)))
+        return {"gw": gw, "gc": gc, "sq": sq, "seg": seg, "mu": mu, "isd": isd}
+
+    def classify_via(self, index):
+        self._check(self.L.cf1g_classify_via(self.h, index))
+
+    def select_index(self, index):
+        self._check(self.L.cf1g_select_index(self.h, index))
+
+    for f in (lidx_train, lidx_lookup, lidx_export, lidx_gradients, classify_via, select_index):
+        setattr(Cf1g, f.__name__, f)
+
+
+_lidx_methods()
 
 
 def state_of(words, idx):

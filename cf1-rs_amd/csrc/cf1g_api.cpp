@@ -16,6 +16,7 @@
 #include "../../include/cf1g.h"
 #include "device.hpp"
 #include "launch.hpp"
+#include "lidx.hpp"
 
 namespace cf1g {
 // kernels_p1p2.hip
@@ -39,14 +40,35 @@ void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const 
 void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
                        uint64_t*, void*, uint16_t*);
 void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint64_t*, uint32_t*,
-                     unsigned int*, uint32_t*, uint64_t*);
+                     unsigned int*, uint32_t*, uint64_t*, uint16_t*);
 void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
                        unsigned int*, uint32_t*);
 void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
                     uint32_t, uint64_t*, uint64_t*, uint64_t*, uint64_t*, unsigned int*);
-void launch_p3_compact(hipStream_t, bool, const void*, const uint16_t*, const uint16_t*, const uint64_t*, const uint64_t*,
-                       const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, int, int, void*, uint16_t*, uint64_t*);
+void launch_p3_compact(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*,
+                       const uint64_t*, const uint64_t*, uint64_t, int, int, void*, uint16_t*, uint64_t*);
 void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
+uint64_t sort_tiles(uint64_t);
+void launch_sort_hist(hipStream_t, const uint64_t*, uint64_t, int, uint64_t*);
+void launch_sort_scatter(hipStream_t, const uint64_t*, uint64_t*, uint64_t, int, const uint64_t*);
+void launch_lidx_stats_init(hipStream_t, int, const SegCtx&, uint64_t, const LidxTrainBufs&);
+void launch_lidx_fwd_bwd(hipStream_t, int, const SegCtx&, uint64_t, const LidxTrainBufs&);
+void launch_lidx_update(hipStream_t, uint64_t, float, const LidxTrainBufs&);
+void launch_lidx_pieces(hipStream_t, const uint64_t*, uint64_t, int, uint32_t, uint32_t, uint64_t*, uint32_t*);
+void launch_lidx_root_final(hipStream_t, uint32_t, uint64_t, uint32_t, const uint64_t*, const uint32_t*, const LidxTrainBufs&,
+                            RootPiece*);
+void launch_lidx_leaf_bounds(hipStream_t, const uint64_t*, uint64_t, int, uint32_t, const RootPiece*, uint64_t*);
+void launch_lidx_leaf_final(hipStream_t, const uint64_t*, uint64_t, uint32_t, const uint64_t*, const LidxTrainBufs&, uint32_t,
+                            LeafRec*, uint64_t*);
+void launch_lidx_fingerprints(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, uint16_t*);
+void launch_lidx_bbh2rank(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, const MphfView&, uint32_t*);
+void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*);
+void launch_count_ties(hipStream_t, const uint16_t*, uint64_t, unsigned long long*);
+void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, uint64_t,
+                         const LidxView&, const MphfView&, uint8_t*, unsigned int*);
+void launch_states_pack(hipStream_t, const uint8_t*, uint64_t, uint64_t*);
+void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&,
+                          const uint64_t*, uint8_t*);
 void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint8_t*);
 void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*);
 
@@ -121,6 +143,11 @@ struct cf1g_ctx {
   uint32_t n_over = 0;
   uint64_t n_blocks = 0;
   uint32_t n_big = 0;
+  // learned index
+  bool lidx_ready = false, lidx_fallback = false, lidx_states = false;
+  uint32_t lidx_R = 0, lidx_L = 0;
+  int lidx_s = 0;
+  int use_index = 0;
 
   void* get(const std::string& name, size_t bytes) {
     DevBuf& b = bufs[name];
@@ -148,6 +175,19 @@ struct cf1g_ctx {
     v.K = k;
     v.m = m;
     v.sb = sb;
+    return v;
+  }
+  LidxView lview() {
+    LidxView v;
+    v.root = getT<RootPiece>("lidx_root", lidx_R + 1);
+    v.leaf = getT<LeafRec>("lidx_leaf", lidx_L + 1);
+    v.keys = getT<uint64_t>("lidx_keys", D + 1);
+    v.fp = getT<uint16_t>("lidx_fp", D + 128);
+    v.bbh2rank = lidx_fallback ? getT<uint32_t>("lidx_bbh2rank", D + 1) : nullptr;
+    v.D = D;
+    v.L = lidx_L;
+    v.R = lidx_R;
+    v.s = lidx_s;
     return v;
   }
   void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
@@ -316,14 +356,15 @@ void do_p3(cf1g_ctx* c) {
   // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
   uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
   launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, cursor, occ, occ_obs);
-  launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad);
+  // joined masks in MPHF-rank order per segment (occ_obs keeps key order)
+  uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
+  launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, vpad);
   unsigned int hf[3];
   c->d2h(hf, flags, 12);
   c->n_over = hf[0];
-  uint16_t* aux_obs = nullptr;
   if (hf[0] > 0) {
     void* aux = c->get("occ_aux", c->W * kb + 16);
-    aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
+    uint16_t* aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
     launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over, hf[0], ucount, flags, big);
     c->d2h(hf, flags, 12);
   }
@@ -332,12 +373,11 @@ void do_p3(cf1g_ctx* c) {
     // segments above LDS capacity: seen / collision words in scratch (at
     // 4*floor(seg_off/64) + 8*seg, 2*(D_s/32 + 2) words), alive bits per
     // occurrence position
-    aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
     uint64_t* scratch = c->getT<uint64_t>("bbh_scratch", 4 * (c->W / 64 + 2) + 8 * (ns + 1) + 1024);
     const uint64_t alive_words = c->W / 64 + 2;
     uint64_t* alive = c->getT<uint64_t>("bbh_alive", alive_words);
     hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
-    launch_bbh_big(st, c->wide, occ, occ_obs, aux_obs, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, flags + 2);
+    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, flags + 2);
   }
   scan_excl_u64(st, ucount, key_off, ns, key_off + ns, tmp);
   scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
@@ -351,9 +391,12 @@ void do_p3(cf1g_ctx* c) {
   void* keys = c->get("keys", D * kb + 16);
   uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
-  launch_p3_compact(st, c->wide, occ, occ_obs, aux_obs, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, keys, vmask, blocks);
+  launch_p3_compact(st, c->wide, occ, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, keys, vmask, blocks);
+  c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
   c->d2h(hf, flags, 12);
   if (hf[2]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit (flags " + std::to_string(hf[2]) + ")");
+  c->lidx_ready = c->lidx_states = false;
+  c->use_index = 0;
   c->stage = 4;
 }
 
@@ -363,6 +406,156 @@ void do_p4(cf1g_ctx* c) {
   launch_p4_pack(c->st, c->getT<uint16_t>("vmask", 12 * nsw + 16), c->D, c->getT<uint64_t>("states", nsw + 1));
   c->sync();
   c->stage = 5;
+}
+
+// ---------------------------------------------------------------- learned index
+LidxTrainBufs lidx_bufs(cf1g_ctx* c, uint64_t n_segs) {
+  LidxTrainBufs b;
+  b.mu = c->getT<float>("lidx_mu", n_segs + 1);
+  b.isd = c->getT<float>("lidx_isd", n_segs + 1);
+  b.w = c->getT<float>("lidx_w", n_segs + 1);
+  b.c = c->getT<float>("lidx_c", n_segs + 1);
+  b.gw = c->getT<double>("lidx_gw", n_segs + 1);
+  b.gc = c->getT<double>("lidx_gc", n_segs + 1);
+  b.loss = c->getT<double>("lidx_loss", n_segs + 1);
+  b.resid = c->getT<float>("lidx_resid", c->D + 1);
+  b.seg_of_tile = c->getT<uint32_t>("lidx_tile_seg", (c->D + 255) / 256 + 1);
+  return b;
+}
+double lidx_mse(cf1g_ctx* c, const LidxTrainBufs& b, uint64_t n_segs) {
+  std::vector<double> l(n_segs);
+  c->d2h(l.data(), b.loss, n_segs * 8);
+  double t = 0;
+  for (double v : l) t += v;
+  return c->D ? t / (double)c->D : 0.0;
+}
+// Stats, endpoint-interpolation start, then epochs of forward / backward /
+// update; loss[0] = MSE at the start, loss[1] = after the last update.
+void lidx_train_level(cf1g_ctx* c, int level, const SegCtx& sc, uint64_t n_segs, uint32_t epochs, float lr, bool train,
+                      double loss[2]) {
+  const LidxTrainBufs b = lidx_bufs(c, n_segs);
+  launch_lidx_stats_init(c->st, level, sc, n_segs, b);
+  launch_lidx_fwd_bwd(c->st, level, sc, n_segs, b);
+  loss[0] = loss[1] = lidx_mse(c, b, n_segs);
+  if (!train) return;
+  for (uint32_t e = 0; e < epochs; e++) {
+    launch_lidx_update(c->st, n_segs, lr, b);
+    launch_lidx_fwd_bwd(c->st, level, sc, n_segs, b);
+  }
+  loss[1] = lidx_mse(c, b, n_segs);
+}
+
+void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* info) {
+  if (c->wide) fail(CF1G_E_INVALID, "the learned index takes u64 keys (k <= 31)");
+  cf1g_lidx_params p{};
+  if (prm) p = *prm;
+  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 20, root_bits = p.root_bits ? p.root_bits : 12;
+  const uint32_t epochs = p.epochs ? p.epochs : 4, max_window = p.max_window ? p.max_window : 256;
+  const float lr = p.lr > 0 ? p.lr : 0.5f;
+  if (leaf_bits > 26 || root_bits > 20) fail(CF1G_E_INVALID, "leaf_bits <= 26 and root_bits <= 20");
+  hipStream_t st = c->st;
+  const uint64_t D = c->D;
+  const int K2 = 2 * c->k;
+  auto t0 = std::chrono::steady_clock::now();
+  // ---- global sort of the distinct keys (LSD, 8-bit digits)
+  uint64_t* ka = c->getT<uint64_t>("lidx_keys", D + 1);
+  uint64_t* kt = c->getT<uint64_t>("lidx_tmp", D + 1);
+  if (D) hip_check(hipMemcpyAsync(ka, c->get("keys", D * 8 + 16), D * 8, hipMemcpyDeviceToDevice, st), "D2D keys");
+  const uint64_t nt = sort_tiles(D);
+  uint64_t* mat = c->getT<uint64_t>("lidx_mat", 256 * nt + 1);
+  uint64_t* off = c->getT<uint64_t>("lidx_off", 256 * nt + 1);
+  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(256 * nt + 1) + 16);
+  uint64_t* src = ka;
+  uint64_t* dst = kt;
+  for (int sh = 0; sh < K2 && D; sh += 8) {
+    launch_sort_hist(st, src, D, sh, mat);
+    scan_excl_u64(st, mat, off, 256 * nt, off + 256 * nt, tmp);
+    launch_sort_scatter(st, src, dst, D, sh, off);
+    std::swap(src, dst);
+  }
+  if (src != ka) hip_check(hipMemcpyAsync(ka, src, D * 8, hipMemcpyDeviceToDevice, st), "D2D sorted");
+  c->sync();
+  auto t1 = std::chrono::steady_clock::now();
+  // ---- root pieces
+  const uint32_t rb = std::min<uint32_t>(root_bits, (uint32_t)K2);
+  const uint32_t R = 1u << rb;
+  const int s = K2 - (int)rb;
+  uint32_t lb = 0;
+  while (lb < leaf_bits && (2ULL << lb) <= std::max<uint64_t>(D, 1)) lb++;
+  const uint32_t L = 1u << lb;
+  uint64_t* pcs = c->getT<uint64_t>("lidx_p", R + 1);
+  uint32_t* lo = c->getT<uint32_t>("lidx_lo", R + 1);
+  launch_lidx_pieces(st, ka, D, s, R, L, pcs, lo);
+  SegCtx sc0{ka, pcs, lo, D, L, s};
+  double rl[2] = {0, 0}, ll[2] = {0, 0};
+  lidx_train_level(c, 0, sc0, R, epochs, lr, !p.untrained, rl);
+  RootPiece* root = c->getT<RootPiece>("lidx_root", R + 1);
+  launch_lidx_root_final(st, R, D, L, pcs, lo, lidx_bufs(c, R), root);
+  // ---- leaves
+  uint64_t* q = c->getT<uint64_t>("lidx_q", L + 1);
+  launch_lidx_leaf_bounds(st, ka, D, s, L, root, q);
+  SegCtx sc1{ka, q, nullptr, D, L, s};
+  lidx_train_level(c, 1, sc1, L, epochs, lr, !p.untrained, ll);
+  LeafRec* leaf = c->getT<LeafRec>("lidx_leaf", L + 1);
+  uint64_t* sabs = c->getT<uint64_t>("lidx_sabs", L + 1);
+  launch_lidx_leaf_final(st, ka, D, L, q, lidx_bufs(c, L), max_window, leaf, sabs);
+  uint16_t* fp = c->getT<uint16_t>("lidx_fp", D + 128);
+  c->zero(fp, (D + 128) * 2);
+  launch_lidx_fingerprints(st, ka, L, q, leaf, fp);
+  // ---- host-side summary of the leaves; fallback table when needed
+  std::vector<LeafRec> hl(L);
+  std::vector<uint64_t> hs(L);
+  c->d2h(hl.data(), leaf, (size_t)L * sizeof(LeafRec));
+  c->d2h(hs.data(), sabs, (size_t)L * 8);
+  uint64_t fb_leaves = 0, fb_keys = 0, win_sum = 0, win_keys = 0, abs_sum = 0;
+  uint32_t mlo = 0, mhi = 0;
+  for (uint32_t j = 0; j < L; j++) {
+    const uint64_t n = (j + 1 < L ? hl[j + 1].q : D) - hl[j].q;
+    abs_sum += hs[j];
+    if (!n) continue;
+    if (hl[j].flags & 1) {
+      fb_leaves++;
+      fb_keys += n;
+    } else {
+      win_sum += n * (uint64_t)(hl[j].elo + hl[j].ehi + 1);
+      win_keys += n;
+      mlo = std::max<uint32_t>(mlo, hl[j].elo);
+      mhi = std::max<uint32_t>(mhi, hl[j].ehi);
+    }
+  }
+  c->lidx_R = R;
+  c->lidx_L = L;
+  c->lidx_s = s;
+  c->lidx_fallback = fb_leaves > 0;
+  if (c->lidx_fallback) launch_lidx_bbh2rank(st, ka, L, q, leaf, c->view(), c->getT<uint32_t>("lidx_bbh2rank", D + 1));
+  auto* tie = c->getT<unsigned long long>("lidx_ties", 2);
+  c->zero(tie, 8);
+  launch_count_ties(st, fp, D, tie);
+  unsigned long long ties = 0;
+  c->d2h(&ties, tie, 8);
+  auto t2 = std::chrono::steady_clock::now();
+  c->lidx_ready = true;
+  c->lidx_states = false;
+  if (info) {
+    cf1g_lidx_info o{};
+    o.n_keys = D;
+    o.n_leaves = L;
+    o.n_pieces = R;
+    o.fallback_leaves = fb_leaves;
+    o.fallback_keys = fb_keys;
+    o.tie_keys = ties;
+    o.max_err_lo = mlo;
+    o.max_err_hi = mhi;
+    o.mean_window = win_keys ? (double)win_sum / (double)win_keys : 0.0;
+    o.mean_abs_err = D ? (double)abs_sum / (double)D : 0.0;
+    o.root_loss[0] = rl[0];
+    o.root_loss[1] = rl[1];
+    o.leaf_loss[0] = ll[0];
+    o.leaf_loss[1] = ll[1];
+    o.sort_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    o.train_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    *info = o;
+  }
 }
 
 void check_k(const cf1g_params* p) {
@@ -520,7 +713,7 @@ int cf1g_states_export(cf1g_ctx* c, uint64_t* words, uint64_t n_words) {
     need(c, 4, "cf1g_states_export");
     uint64_t nsw = (c->D + 11) / 12;
     if (!words || n_words < nsw) fail(CF1G_E_INVALID, "state buffer too small");
-    c->d2h(words, c->get("states", (nsw + 1) * 8), nsw * 8);
+    c->d2h(words, c->get(c->use_index == 1 ? "states_lidx" : "states", (nsw + 1) * 8), nsw * 8);
   });
 }
 
@@ -594,8 +787,12 @@ int cf1g_annotate(cf1g_ctx* c, uint8_t* out, uint64_t n) {
     if (!out || n < c->n_pos) fail(CF1G_E_INVALID, "annotation buffer too small");
     uint8_t* d = c->getT<uint8_t>("annot", c->n_pos + 8);
     uint64_t nsw = (c->D + 11) / 12;
-    launch_annotate(c->st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
-                    c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), d);
+    if (c->use_index == 1)
+      launch_annotate_lidx(c->st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), c->n_pos,
+                           c->k, c->lview(), c->view(), c->getT<uint64_t>("states_lidx", nsw + 1), d);
+    else
+      launch_annotate(c->st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
+                      c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), d);
     c->d2h(out, d, c->n_pos);
   });
 }
@@ -675,5 +872,115 @@ const void* cf1g_device_keys(cf1g_ctx* c) {
   if (!c || c->stage < 4) return nullptr;
   return c->get("keys", c->D * c->key_bytes + 16);
 }
+
+// ---------------------------------------------------------------- learned index
+int cf1g_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* params, cf1g_lidx_info* out) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_lidx_train");
+    do_lidx_train(c, params, out);
+  });
+}
+
+int cf1g_lidx_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* out) {
+  return guarded(c, [&]() {
+    if (!c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_lidx_lookup called before cf1g_lidx_train");
+    if (n == 0) return;
+    if (!keys || !out) fail(CF1G_E_INVALID, "NULL buffer");
+    uint64_t* dk = c->getT<uint64_t>("lookup_in", n + 1);
+    uint64_t* dout = c->getT<uint64_t>("lookup_out", n + 1);
+    c->h2d(dk, keys, n * 8);
+    launch_lidx_lookup(c->st, dk, n, c->lview(), c->view(), dout);
+    c->d2h(out, dout, n * 8);
+  });
+}
+
+int cf1g_lidx_lookup_device(cf1g_ctx* c, const void* d_keys, uint64_t n, uint64_t* d_out) {
+  return guarded(c, [&]() {
+    if (!c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_lidx_lookup_device called before cf1g_lidx_train");
+    if (n == 0) return;
+    if (!d_keys || !d_out) fail(CF1G_E_INVALID, "NULL buffer");
+    launch_lidx_lookup(c->st, (const uint64_t*)d_keys, n, c->lview(), c->view(), d_out);
+    c->sync();
+  });
+}
+
+int cf1g_lidx_export(cf1g_ctx* c, cf1g_lidx_shape* shape, uint64_t* sorted_keys, void* root, void* leaves) {
+  return guarded(c, [&]() {
+    if (!c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_lidx_export called before cf1g_lidx_train");
+    if (shape) {
+      *shape = cf1g_lidx_shape{};
+      shape->n_keys = c->D;
+      shape->n_pieces = c->lidx_R;
+      shape->n_leaves = c->lidx_L;
+      shape->root_shift = c->lidx_s;
+    }
+    const LidxView v = c->lview();
+    if (sorted_keys && c->D) c->d2h(sorted_keys, v.keys, c->D * 8);
+    if (root) c->d2h(root, v.root, (size_t)c->lidx_R * sizeof(RootPiece));
+    if (leaves) c->d2h(leaves, v.leaf, (size_t)c->lidx_L * sizeof(LeafRec));
+  });
+}
+
+int cf1g_lidx_gradients(cf1g_ctx* c, int level, const float* w, const float* cc, uint64_t n_segs, double* out_gw, double* out_gc,
+                        double* out_sq, uint64_t* out_seg, float* out_mu, float* out_isd) {
+  return guarded(c, [&]() {
+    if (!c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_lidx_gradients called before cf1g_lidx_train");
+    if (level != 0 && level != 1) fail(CF1G_E_INVALID, "level is 0 (root) or 1 (leaves)");
+    const uint64_t ns = level == 0 ? c->lidx_R : c->lidx_L;
+    if (n_segs != ns || !w || !cc) fail(CF1G_E_INVALID, "one (w, c) pair per segment is required");
+    uint64_t* ka = c->getT<uint64_t>("lidx_keys", c->D + 1);
+    SegCtx sc{ka, level == 0 ? c->getT<uint64_t>("lidx_p", ns + 1) : c->getT<uint64_t>("lidx_q", ns + 1),
+              level == 0 ? c->getT<uint32_t>("lidx_lo", ns + 1) : nullptr, c->D, c->lidx_L, c->lidx_s};
+    const LidxTrainBufs b = lidx_bufs(c, ns);
+    launch_lidx_stats_init(c->st, level, sc, ns, b);
+    c->h2d(b.w, w, ns * 4);
+    c->h2d(b.c, cc, ns * 4);
+    launch_lidx_fwd_bwd(c->st, level, sc, ns, b);
+    if (out_gw) c->d2h(out_gw, b.gw, ns * 8);
+    if (out_gc) c->d2h(out_gc, b.gc, ns * 8);
+    if (out_sq) c->d2h(out_sq, b.loss, ns * 8);
+    if (out_seg) c->d2h(out_seg, sc.seg, (ns + 1) * 8);
+    if (out_mu) c->d2h(out_mu, b.mu, ns * 4);
+    if (out_isd) c->d2h(out_isd, b.isd, ns * 4);
+  });
+}
+
+int cf1g_classify_via(cf1g_ctx* c, int index) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_classify_via");
+    if (c->wide) fail(CF1G_E_INVALID, "classification through an index takes u64 keys (k <= 31)");
+    if (index == 1 && !c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_classify_via(1) called before cf1g_lidx_train");
+    if (index != 0 && index != 1) fail(CF1G_E_INVALID, "index is 0 (BBHash) or 1 (learned)");
+    const uint64_t D = c->D, nsw = (D + 11) / 12;
+    uint8_t* st8 = c->getT<uint8_t>("st8", D + 8);
+    c->zero(st8, D + 8);
+    unsigned int* err = c->getT<unsigned int>("via_err", 2);
+    c->zero(err, 4);
+    LidxView v{};
+    if (index == 1) v = c->lview();
+    v.D = D;
+    launch_classify_via(c->st, index == 1, c->getT<uint64_t>("keys", D + 2), c->getT<uint16_t>("occ_obs", c->W + 8),
+                        c->getT<uint64_t>("seg_off", c->n_segs + 1), c->getT<uint64_t>("seg_out_off", c->n_segs + 1), c->n_segs, v,
+                        c->view(), st8, err);
+    launch_states_pack(c->st, st8, D, c->getT<uint64_t>(index == 1 ? "states_lidx" : "states", nsw + 1));
+    unsigned int he = 0;
+    c->d2h(&he, err, 4);
+    if (he) fail(CF1G_E_INTERNAL, "a distinct k-mer was not found through the index");
+    if (index == 1) c->lidx_states = true;
+  });
+}
+
+int cf1g_select_index(cf1g_ctx* c, int index) {
+  return guarded(c, [&]() {
+    if (index == 0) {
+      c->use_index = 0;
+      return;
+    }
+    if (index != 1) fail(CF1G_E_INVALID, "index is 0 (BBHash) or 1 (learned)");
+    if (!c->lidx_states) fail(CF1G_E_ORDER, "cf1g_select_index(1) needs cf1g_classify_via(ctx, 1)");
+    c->use_index = 1;
+  });
+}
+int cf1g_selected_index(const cf1g_ctx* c) { return c ? c->use_index : 0; }
 
 }  // extern "C"

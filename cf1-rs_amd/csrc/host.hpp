@@ -3,7 +3,11 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
+
+#include "lidx.hpp"
 
 namespace cf1g {
 
@@ -99,6 +103,31 @@ struct HostMphf {
       t = (9 * t + 19) / 20;
     }
     return 0;
+  }
+};
+
+// Host lookup through the learned index (u64 keys): the root routes the key
+// to its leaf with the device's FP32 arithmetic, then an exact search over
+// the leaf's keys gives the rank (the same map as the device lookup).
+struct HostLidx {
+  const uint64_t* keys = nullptr;
+  uint64_t D = 0;
+  const RootPiece* root = nullptr;
+  const LeafRec* leaf = nullptr;
+  uint32_t L = 0, R = 0;
+  int s = 0;
+  uint64_t lookup(uint64_t key) const {
+    if (!D || (key >> s) >= R) return 0;
+    const uint32_t r = (uint32_t)(key >> s);
+    const RootPiece& P = root[r];
+    const float x = (float)(key - ((uint64_t)r << s));
+    const float f = std::floor(std::fmaf(P.A, x, P.B));
+    const float span = (float)(P.hi - P.lo);
+    const float cl = f < 0.f ? 0.f : (f > span ? span : f);
+    const uint32_t j = P.lo + (uint32_t)cl;
+    const uint64_t qa = leaf[j].q, qb = j + 1 < L ? leaf[j + 1].q : D;
+    const uint64_t* it = std::lower_bound(keys + qa, keys + qb, key);
+    return (it != keys + qb && *it == key) ? (uint64_t)(it - keys) : 0;
   }
 };
 

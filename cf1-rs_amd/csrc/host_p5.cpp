@@ -83,6 +83,7 @@ struct P5 {
   int K;
   const uint8_t* annot;  // per stream position
   HostMphf mphf;
+  const HostLidx* lidx = nullptr;  // lookups through the learned index when set
   std::atomic<uint64_t>* states;
 
   AK at(uint64_t g0, uint64_t idx) const {
@@ -107,7 +108,7 @@ struct P5 {
     u128 rs = h_rc(xs, K), re = h_rc(xe, K);
     u128 cs = xs < rs ? xs : rs, ce = xe < re ? xe : re;
     u128 mf = cs < ce ? cs : ce;
-    uint64_t id = mphf.lookup(mf);
+    uint64_t id = lidx ? lidx->lookup((uint64_t)mf) : mphf.lookup(mf);
     bool fwd = xs < re;
     us.push_back({id, fwd, st.idx, en.idx});
     uint8_t s = get_state(id);
@@ -271,6 +272,27 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
   if ((rc = cf1g_mphf_export(ctx, mbuf.data(), &mw))) return rc;
   std::vector<uint64_t> gstart(n_rec + 1);
   if ((rc = cf1g_record_starts(ctx, gstart.data(), gstart.size()))) return rc;
+  // learned index selected: unitig ids are ranks in the sorted key order
+  HostLidx hl;
+  std::vector<uint64_t> lkeys;
+  std::vector<RootPiece> lroot;
+  std::vector<LeafRec> lleaf;
+  const bool learned = cf1g_selected_index(ctx) == 1;
+  if (learned) {
+    cf1g_lidx_shape sh;
+    if ((rc = cf1g_lidx_export(ctx, &sh, nullptr, nullptr, nullptr))) return rc;
+    lkeys.resize(sh.n_keys + 1);
+    lroot.resize(sh.n_pieces);
+    lleaf.resize(sh.n_leaves);
+    if ((rc = cf1g_lidx_export(ctx, nullptr, lkeys.data(), lroot.data(), lleaf.data()))) return rc;
+    hl.keys = lkeys.data();
+    hl.D = sh.n_keys;
+    hl.root = lroot.data();
+    hl.leaf = lleaf.data();
+    hl.L = sh.n_leaves;
+    hl.R = sh.n_pieces;
+    hl.s = sh.root_shift;
+  }
 
   std::vector<std::atomic<uint64_t>> st(words.size());
   for (size_t i = 0; i < words.size(); i++) st[i].store(words[i], std::memory_order_relaxed);
@@ -278,6 +300,7 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
   p5.K = K;
   p5.annot = annot.data();
   p5.mphf.attach(mbuf.data());
+  if (learned) p5.lidx = &hl;
   p5.states = st.data();
 
   const uint64_t base = rec_off[0];

@@ -13,6 +13,7 @@
 // above the LDS capacity take a global-memory radix sort (k_p3_bigsort) and
 // k_bbh_big. k_p3_compact then packs keys (in (bin, segment, key) order: the
 // MPHF key order), the masks (to MPHF-value order) and the blocks densely.
+#include "block.hpp"
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -22,31 +23,6 @@ namespace cf1g {
 constexpr int SC_WG = 256;
 constexpr int SC_IPT = 16;
 constexpr int SC_TILE = SC_WG * SC_IPT;
-
-__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x) {
-  uint32_t lane = __lane_id();
-  for (int d = 1; d < 64; d <<= 1) {
-    uint64_t y = __shfl_up(x, d, 64);
-    if ((int)lane >= d) x += y;
-  }
-  return x;
-}
-// Exclusive scan across a 256-thread block; *total gets the block sum.
-__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* sh4, uint64_t* total) {
-  uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
-  uint64_t x = wave_incl_scan_u64(v);
-  if (lane == 63) sh4[wv] = x;
-  __syncthreads();
-  uint64_t off = 0, tot = 0;
-  for (int q = 0; q < 4; q++) {
-    uint64_t t = sh4[q];
-    if (q < (int)wv) off += t;
-    tot += t;
-  }
-  __syncthreads();
-  if (total) *total = tot;
-  return off + x - v;
-}
 
 __global__ void __launch_bounds__(SC_WG) k_scan_tile(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n,
                                                      uint64_t* __restrict__ block_sums) {
@@ -265,85 +241,6 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
     });
 }
 
-// ------------------------------------------------------------------ tile ranking
-// A tile of up to 256*KPT keys lives in registers, wave-blocked: element
-// e = wave*(64*kpt) + slot*64 + lane. rank_tile computes, for one 8-bit
-// digit, each element's rank among equal digits of its wave (stable, in
-// element order) and the per-wave digit counts in cnt[wave*256 + d].
-constexpr int RK_WG = 256;
-
-template <typename KeyT, int KPT>
-__device__ __forceinline__ void rank_tile(const KeyT (&key)[KPT], int kpt, int sh, uint32_t (&rank)[KPT], uint32_t* cnt) {
-  const uint32_t wv = threadIdx.x >> 6;
-  const uint64_t lt = lanemask_lt();
-  uint32_t* wc = cnt + wv * 256;
-#pragma unroll
-  for (int s = 0; s < KPT; s++) {
-    if (s < kpt) {
-      uint32_t d = key_digit(key[s], sh);
-      uint64_t peers = ~0ULL;
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        uint64_t bal = __ballot((d >> b) & 1);
-        peers &= ((d >> b) & 1) ? bal : ~bal;
-      }
-      uint32_t below = (uint32_t)__popcll(peers & lt);
-      uint32_t base = wc[d];
-      rank[s] = base + below;
-      __builtin_amdgcn_wave_barrier();
-      if (below == 0) wc[d] = base + (uint32_t)__popcll(peers);
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-}
-// Turn per-wave counts into scatter bases: cnt[w*256+d] = start of (w, d).
-// Optional: tile totals per digit into tot[d] (when tot != nullptr).
-__device__ __forceinline__ void rank_offsets(uint32_t* cnt, uint64_t* sh4, uint32_t* tot) {
-  uint32_t d = threadIdx.x;  // 256 threads == 256 digits
-  uint32_t c0 = cnt[d], c1 = cnt[256 + d], c2 = cnt[512 + d], c3 = cnt[768 + d];
-  uint32_t t = c0 + c1 + c2 + c3;
-  uint64_t ex = block_excl_scan_u64(t, sh4, nullptr);
-  cnt[d] = (uint32_t)ex;
-  cnt[256 + d] = (uint32_t)ex + c0;
-  cnt[512 + d] = (uint32_t)ex + c0 + c1;
-  cnt[768 + d] = (uint32_t)ex + c0 + c1 + c2;
-  if (tot) tot[d] = t;
-}
-
-template <typename KeyT>
-__device__ __forceinline__ KeyT key_pad();
-template <>
-__device__ __forceinline__ uint64_t key_pad<uint64_t>() { return ~0ULL; }
-template <>
-__device__ __forceinline__ Key128 key_pad<Key128>() { return {~0ULL, ~0ULL}; }
-
-__device__ __forceinline__ void key_xor_or(uint64_t& acc_lo, uint64_t& acc_hi, uint64_t k, uint64_t k0) { acc_lo |= k ^ k0; }
-__device__ __forceinline__ void key_xor_or(uint64_t& acc_lo, uint64_t& acc_hi, const Key128& k, const Key128& k0) {
-  acc_lo |= k.lo ^ k0.lo;
-  acc_hi |= k.hi ^ k0.hi;
-}
-__device__ __forceinline__ int varying_bits(uint64_t lo, uint64_t hi) {
-  if (hi) return 128 - __clzll(hi);
-  if (lo) return 64 - __clzll(lo);
-  return 0;
-}
-// Block OR-reduce of two words.
-__device__ __forceinline__ void block_or2(uint64_t& a, uint64_t& b, uint64_t* sh8) {
-  for (int d = 32; d >= 1; d >>= 1) {
-    a |= __shfl_xor(a, d, 64);
-    b |= __shfl_xor(b, d, 64);
-  }
-  uint32_t wv = threadIdx.x >> 6;
-  if (__lane_id() == 0) {
-    sh8[wv] = a;
-    sh8[4 + wv] = b;
-  }
-  __syncthreads();
-  a = sh8[0] | sh8[1] | sh8[2] | sh8[3];
-  b = sh8[4] | sh8[5] | sh8[6] | sh8[7];
-  __syncthreads();
-}
-
 // ------------------------------------------------------------------ segment dedup
 // Per-segment capacity of the register/LDS paths: keys are held KPT per
 // thread. Larger segments take the global-memory paths (k_p3_bigsort,
@@ -440,10 +337,10 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
 // Sort + dedup of one segment, then the size of its BBHash. Each occurrence
 // carries its 10-bit observation mask; the masks of equal keys are OR-joined
 // (the DFA join of classify.rs:147-518), and the segment's BBHash is built.
-// Output in place: sorted distinct keys at occ[start...], joined masks in
-// local-rank order at obs[start...], their count in ucount, the BBHash
-// blocks in the segment's padded slot of blocks_pad and their count in
-// nblk. Oversized segments go to over_list (sorted by k_p3_bigsort);
+// Output in place: sorted distinct keys at occ[start...] with their joined
+// masks at obs[start...], their count in ucount, the masks again in
+// local-rank order at vpad[start...], the BBHash blocks in the segment's
+// padded slot of blocks_pad and their count in nblk. Oversized segments go to over_list (sorted by k_p3_bigsort);
 // segments whose distinct keys exceed the register capacity go to big_list
 // (BBHash built by k_bbh_big).
 template <typename KeyT>
@@ -451,7 +348,8 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
                                                     const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K2, int sb,
                                                     uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
                                                     uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
-                                                    uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad) {
+                                                    uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad,
+                                                    uint16_t* __restrict__ vpad) {
   constexpr int KPT = SegCfg<KeyT>::KPT;
   constexpr int CAP = RK_WG * KPT;
   constexpr int SEEN_W = (2 * CAP) / 64 + 1;
@@ -660,8 +558,8 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
     u = tot;
   }
-  // ---- BBHash of the segment, into its padded block slot; then the joined
-  // masks are permuted in place into local-rank order (obs[start + rank]).
+  // ---- BBHash of the segment, into its padded block slot; the joined masks
+  // also go to vpad[start + local rank] (obs keeps them in key order).
   if (threadIdx.x == 0) ucount[seg] = u;
   if (seg_is_big<KeyT>(n, u, K2)) {
     if (threadIdx.x == 0) {
@@ -724,7 +622,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     for (uint32_t q = 0; q < wi; q++) r += (uint64_t)__popcll(region[qb * 7 + q]);
     r += (uint64_t)__popcll(region[qb * 7 + wi] & ((1ULL << (b & 63)) - 1));
     if (r >= u) atomicOr(err, 8u);
-    else obs[start + r] = mbuf[e];  // every read of obs happened before bbh_levels' barriers
+    else vpad[start + r] = mbuf[e];
   }
 }
 
@@ -867,12 +765,12 @@ __global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, ui
 
 // ------------------------------------------------------------------ compaction
 // One work-group per segment copies the segment's distinct keys to the dense
-// key order (key_off), its masks (local-rank order: obs, or aux_obs for the
-// segments of k_bbh_big) to their MPHF values in vmask, and its BBHash blocks
-// from the padded slot to the dense block order (blk_off).
+// key order (key_off), its masks (local-rank order, vpad) to their MPHF
+// values in vmask, and its BBHash blocks from the padded slot to the dense
+// block order (blk_off).
 template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ occ, const uint16_t* __restrict__ obs,
-                                                      const uint16_t* __restrict__ aux_obs, const uint64_t* __restrict__ seg_off,
+__global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ occ, const uint16_t* __restrict__ vpad,
+                                                      const uint64_t* __restrict__ seg_off,
                                                       const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
                                                       const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks_pad,
                                                       uint64_t n_segs, int K2, int sb, KeyT* __restrict__ keys,
@@ -881,11 +779,9 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ o
   if (seg >= n_segs) return;
   const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
   if (u == 0) return;
-  const bool big = seg_is_big<KeyT>(seg_off[seg + 1] - s0, u, K2);
-  const uint16_t* m = big ? aux_obs : obs;
   for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
     keys[ko + i] = occ[s0 + i];
-    vmask[ko + i] = m[s0 + i];
+    vmask[ko + i] = vpad[s0 + i];
   }
   const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
   const uint64_t* src = blocks_pad + pad_slot_off(s0, seg) * 8;
@@ -899,7 +795,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ o
 // distinct keys); otherwise in global scratch (seen / collision words of the
 // segment at 4*floor(seg_off/64) + 8*seg, alive bits per occurrence position)
 // with device-scope atomics. Blocks go to the padded slot; each key's mask is
-// written to aux_obs[seg_off + local rank].
+// written to vpad[seg_off + local rank].
 constexpr uint64_t MED_CAP = 1ULL << 17;
 constexpr int MED_W = (int)(MED_CAP / 64);
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
@@ -907,7 +803,7 @@ __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
 }
 template <bool LDSB, typename KeyT>
 __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ, const uint16_t* __restrict__ obs,
-                                                   uint16_t* __restrict__ aux_obs, const uint64_t* __restrict__ seg_off,
+                                                   uint16_t* __restrict__ vpad, const uint64_t* __restrict__ seg_off,
                                                    const uint64_t* __restrict__ ucount, const uint32_t* __restrict__ big_list,
                                                    uint64_t* __restrict__ nblk, uint64_t* __restrict__ blocks_pad,
                                                    uint64_t* __restrict__ scratch, uint64_t* __restrict__ alive_g,
@@ -1018,7 +914,7 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
   for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
     uint64_t rk = mphf_local_rank<KeyT>(region, D, keys[e]);
     if (rk >= D) atomicOr(err, 4u);
-    else aux_obs[so + rk] = obs[so + e];
+    else vpad[so + rk] = obs[so + e];
   }
 }
 
@@ -1053,14 +949,14 @@ void launch_p3_scatter(hipStream_t st, bool wide, const uint64_t* rec, const uin
 }
 void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
                      uint64_t* ucount, uint64_t* nblk, uint32_t* over_list, unsigned int* flags, uint32_t* big_list,
-                     uint64_t* blocks_pad) {
+                     uint64_t* blocks_pad, uint16_t* vpad) {
   if (n_segs == 0) return;
   if (!wide)
     CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (uint64_t*)occ, obs, seg_off, n_segs,
-                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad);
+                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
   else
     CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (Key128*)occ, obs, seg_off, n_segs,
-                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad);
+                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
 }
 void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void* aux, uint16_t* aux_obs, const uint64_t* seg_off,
                        const uint32_t* over_list, uint32_t n_over, uint64_t* ucount, unsigned int* flags, uint32_t* big_list) {
@@ -1074,35 +970,35 @@ void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void
 }
 // Both variants are launched over the whole big list; each builds the
 // segments of its size class and returns at once for the others.
-void launch_bbh_big(hipStream_t st, bool wide, const void* occ, const uint16_t* obs, uint16_t* aux_obs, const uint64_t* seg_off,
+void launch_bbh_big(hipStream_t st, bool wide, const void* occ, const uint16_t* obs, uint16_t* vpad, const uint64_t* seg_off,
                     const uint64_t* ucount, const uint32_t* big_list, uint32_t n_big, uint64_t* nblk, uint64_t* blocks_pad,
                     uint64_t* scratch, uint64_t* alive_g, unsigned int* err) {
   if (n_big == 0) return;
   dim3 g(n_big), b(RK_WG);
   if (!wide) {
-    CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, aux_obs, seg_off, ucount, big_list,
+    CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, vpad, seg_off, ucount, big_list,
                 nblk, blocks_pad, scratch, alive_g, err);
     if (alive_g)
-      CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, aux_obs, seg_off, ucount,
+      CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, vpad, seg_off, ucount,
                   big_list, nblk, blocks_pad, scratch, alive_g, err);
   } else {
-    CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, Key128>), g, b, 0, st, (const Key128*)occ, obs, aux_obs, seg_off, ucount, big_list,
+    CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, Key128>), g, b, 0, st, (const Key128*)occ, obs, vpad, seg_off, ucount, big_list,
                 nblk, blocks_pad, scratch, alive_g, err);
     if (alive_g)
-      CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, Key128>), g, b, 0, st, (const Key128*)occ, obs, aux_obs, seg_off, ucount, big_list,
+      CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, Key128>), g, b, 0, st, (const Key128*)occ, obs, vpad, seg_off, ucount, big_list,
                   nblk, blocks_pad, scratch, alive_g, err);
   }
 }
-void launch_p3_compact(hipStream_t st, bool wide, const void* occ, const uint16_t* obs, const uint16_t* aux_obs, const uint64_t* seg_off,
-                       const uint64_t* ucount, const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad,
-                       uint64_t n_segs, int K, int sb, void* keys, uint16_t* vmask, uint64_t* blocks) {
+void launch_p3_compact(hipStream_t st, bool wide, const void* occ, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount,
+                       const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad, uint64_t n_segs, int K, int sb,
+                       void* keys, uint16_t* vmask, uint64_t* blocks) {
   if (n_segs == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)occ, obs, aux_obs,
-                seg_off, ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (uint64_t*)keys, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)occ, vpad, seg_off,
+                ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (uint64_t*)keys, vmask, blocks);
   else
-    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)occ, obs, aux_obs,
-                seg_off, ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (Key128*)keys, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)occ, vpad, seg_off,
+                ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (Key128*)keys, vmask, blocks);
 }
 
 }  // namespace cf1g

@@ -141,6 +141,65 @@ int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, 
                   const uint64_t* name_off, const char* out_prefix, const char* input_desc, int poly_n_stretch,
                   int track_short_seqs, int threads, uint64_t* out_meta5);
 
+/* ---- learned rank index (no reference counterpart: an alternative to
+ * Mphf::hash, src/mphf.rs:268-276, whose values are the ranks of the keys in
+ * the globally sorted distinct k-mers — a monotone minimal perfect map).
+ * u64 keys only (k <= 31). Trained on the GPU after P3. */
+typedef struct cf1g_lidx_params {
+  uint32_t leaf_bits;  /* log2 leaves; 0 = 20 (capped at floor(log2 D)) */
+  uint32_t root_bits;  /* log2 root pieces; 0 = 12 (capped at 2k) */
+  uint32_t epochs;     /* gradient-descent epochs per level; 0 = 4 */
+  float lr;            /* step size on standardised inputs; 0 = 0.5 */
+  uint32_t max_window; /* leaves with a wider search window resolve through the BBHash; 0 = 256 */
+  uint32_t untrained;  /* 1: keep the endpoint interpolation (the method without training) */
+  uint32_t reserved[2];
+} cf1g_lidx_params;
+
+typedef struct cf1g_lidx_info {
+  uint64_t n_keys, n_leaves, n_pieces;
+  uint64_t fallback_leaves, fallback_keys; /* leaves (and their keys) resolved through the BBHash */
+  uint64_t tie_keys;                       /* keys whose fingerprint ties a neighbour's */
+  uint32_t max_err_lo, max_err_hi;         /* recorded error bounds (non-fallback leaves) */
+  double mean_window;                      /* mean search window over keys of non-fallback leaves */
+  double mean_abs_err;                     /* mean |rank - prediction| over all keys */
+  double root_loss[2], leaf_loss[2];       /* MSE before the first and after the last epoch */
+  double sort_ms, train_ms;                /* sort of the keys; whole training (sort included) */
+} cf1g_lidx_info;
+
+/* Sort the distinct keys, train root and leaves (forward / backward / update
+ * kernels), record error bounds, build the search arrays. Needs P3. */
+int cf1g_lidx_train(cf1g_ctx* ctx, const cf1g_lidx_params* params, cf1g_lidx_info* out);
+/* Exact rank of each key (0 for an absent key, like src/mphf.rs:269): host
+ * buffers / device-resident buffers. */
+int cf1g_lidx_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
+int cf1g_lidx_lookup_device(cf1g_ctx* ctx, const void* d_keys, uint64_t n, uint64_t* d_out);
+/* The trained model: its shape, the sorted keys (D), root pieces (16 bytes
+ * each: f32 A, f32 B, u32 lo, u32 hi; piece = key >> root_shift) and leaf
+ * records (32 bytes each: u64 base, u64 first rank, f32 slope, f32 intercept,
+ * u16 err_lo, u16 err_hi, u8 shift, u8 flags, u16). Any output may be NULL. */
+typedef struct cf1g_lidx_shape {
+  uint64_t n_keys;
+  uint32_t n_pieces, n_leaves;
+  int32_t root_shift;
+  uint32_t reserved;
+} cf1g_lidx_shape;
+int cf1g_lidx_export(cf1g_ctx* ctx, cf1g_lidx_shape* shape, uint64_t* sorted_keys, void* root, void* leaves);
+/* Gradient check: with the caller's parameters (w, c in standardised
+ * coordinates, one pair per segment; level 0 = root pieces, 1 = leaves) run
+ * the forward and backward kernels and return per-segment gradients and
+ * summed squared residuals, plus the segment starts (n_segs + 1) and the
+ * standardisation (mu, 1/deviation) the kernels used. Outputs may be NULL. */
+int cf1g_lidx_gradients(cf1g_ctx* ctx, int level, const float* w, const float* c, uint64_t n_segs, double* out_gw, double* out_gc,
+                        double* out_sq, uint64_t* out_seg, float* out_mu, float* out_isd);
+/* Classification through an index (0 = BBHash, 1 = learned): every distinct
+ * k-mer is looked up and its joined state stored at that slot of a state
+ * vector in the index's order. */
+int cf1g_classify_via(cf1g_ctx* ctx, int index);
+/* Which index cf1g_annotate, cf1g_states_export and cf1g_p5_write use
+ * (0 = BBHash, the default; 1 = learned, after cf1g_classify_via(ctx, 1)). */
+int cf1g_select_index(cf1g_ctx* ctx, int index);
+int cf1g_selected_index(const cf1g_ctx* ctx);
+
 /* ---- measurement */
 int cf1g_sync(cf1g_ctx* ctx);
 /* Enable per-kernel HIP-event timing on the context stream (0/1). */
