@@ -14,7 +14,10 @@ Also reported on the same JSON line:
   cpu_baseline : the multithreaded C++ CPU oracle (the restatement of the
                  reference's P1..P4) timed on a bounded sample of the same
                  genome, on this host's cores.
-  lookup       : BBHash lookups/s over the distinct keys.
+  lookup       : learned rank index vs BBHash lookups/s over 4 x 2^28 uniform
+                 lookups of present keys (every learned rank verified), the
+                 index's training summary, and the classification pass
+                 through each index.
 """
 import argparse
 import json
@@ -135,7 +138,8 @@ def main():
     ap.add_argument("--cpu-sample-mbp", type=float, default=60.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--lookups", type=int, default=1 << 28)
+    ap.add_argument("--lookups", type=int, default=1 << 28, help="lookups per batch (0: skip the lookup benchmark)")
+    ap.add_argument("--lookup-batches", type=int, default=4)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (MiB) to scale HBM byte counters")
     args = ap.parse_args()
@@ -203,11 +207,10 @@ def main():
         phases.setdefault(PHASE_OF.get(name, name), 0.0)
         phases[PHASE_OF.get(name, name)] += v["ms_per_step"]
 
-    # BBHash lookups over device-resident present keys (uniform sample)
+    # learned index vs BBHash lookups over device-resident present keys
     lookup = None
-    if args.lookups > 0 and D > 0:
-        import ctypes as C
-        lookup = bbhash_lookup_rate(g, D, args.lookups, kb)
+    if args.lookups > 0 and D > 0 and kb == 8:
+        lookup = lookup_bench(g, D, args.lookups, args.lookup_batches)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -263,39 +266,74 @@ def copy_calibration(nbytes, reps=3):
     print(f"[bench] copy calibration: {reps} x {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
 
 
-def bbhash_lookup_rate(g, D, n, kb):
-    """Lookups/s of the BBHash on n keys drawn uniformly from the present keys,
-    all resident in HBM."""
+def lookup_bench(g, D, batch, n_batches, seed=5):
+    """The learned index against the BBHash on the same n_batches x batch
+    lookups of present keys (uniform over the distinct keys, seeded), keys
+    and results resident in HBM. Lookups/s from the lookup kernels' HIP-event
+    times. Every learned rank is checked against the drawn index (ranks are
+    positions in the sorted keys); the BBHash values are checked against the
+    BBHash of the sorted keys."""
     import ctypes as C
 
     import numpy as np
 
-    lo, hi = g.keys()
-    rng = np.random.default_rng(5)
-    idx = rng.integers(0, D, n)
-    if kb == 8:
-        q = lo[idx]
-    else:
-        q = np.empty(2 * n, np.uint64)
-        q[0::2], q[1::2] = lo[idx], hi[idx]
-    L = g.L
-    hip = C.CDLL("libamdhip64.so")
-    dq, dout = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(dq), C.c_size_t(q.nbytes)) == 0
-    assert hip.hipMalloc(C.byref(dout), C.c_size_t(n * 8)) == 0
-    assert hip.hipMemcpy(dq, q.ctypes.data_as(C.c_void_p), C.c_size_t(q.nbytes), 1) == 0
-    g._check(L.cf1g_mphf_lookup_device(g.h, dq, n, dout))  # warm
-    reps = 3
     t = time.perf_counter()
-    for _ in range(reps):
-        g._check(L.cf1g_mphf_lookup_device(g.h, dq, n, dout))
-    dt = (time.perf_counter() - t) / reps
-    out = np.empty(min(n, 1 << 20), np.uint64)
-    assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(out.nbytes), 2) == 0
-    ok = bool(np.all(out == g.mphf_lookup(lo[idx[:len(out)]], hi[idx[:len(out)]] if kb == 16 else None)))
-    hip.hipFree(dq)
-    hip.hipFree(dout)
-    return {"bbhash_lookups_per_s": n / dt, "n": n, "checked_against_host_batch": ok, "learned_lookups_per_s": None}
+    info = g.lidx_train()
+    train_wall = time.perf_counter() - t
+    print(f"[bench] learned index trained in {train_wall:.2f} s", file=sys.stderr, flush=True)
+    keys = g.lidx_export()["keys"]
+    hip = C.CDLL("libamdhip64.so")
+    L = g.L
+    dq, dout, dall = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(dq), C.c_size_t(batch * 8)) == 0
+    assert hip.hipMalloc(C.byref(dout), C.c_size_t(batch * 8)) == 0
+    assert hip.hipMalloc(C.byref(dall), C.c_size_t(max(D, 1) * 8)) == 0
+    # BBHash values of the sorted keys (for checking the BBHash batches)
+    assert hip.hipMemcpy(dall, keys.ctypes.data_as(C.c_void_p), C.c_size_t(D * 8), 1) == 0
+    vall = np.empty(D, np.uint64)
+    g._check(L.cf1g_mphf_lookup_device(g.h, dall, D, dall))
+    assert hip.hipMemcpy(vall.ctypes.data_as(C.c_void_p), dall, C.c_size_t(D * 8), 2) == 0
+    rng = np.random.default_rng(seed)
+    out = np.empty(batch, np.uint64)
+    errs = {"learned": 0, "bbhash": 0}
+    g.set_profiling(True)
+    g.kernel_times(reset=True)
+    for bi in range(n_batches):
+        idx = rng.integers(0, D, batch).astype(np.uint64)
+        q = keys[idx]
+        assert hip.hipMemcpy(dq, q.ctypes.data_as(C.c_void_p), C.c_size_t(batch * 8), 1) == 0
+        g._check(L.cf1g_lidx_lookup_device(g.h, dq, batch, dout))
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
+        errs["learned"] += int(np.count_nonzero(out != idx))
+        g._check(L.cf1g_mphf_lookup_device(g.h, dq, batch, dout))
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
+        errs["bbhash"] += int(np.count_nonzero(out != vall[idx]))
+        print(f"[bench] lookup batch {bi + 1}/{n_batches} done", file=sys.stderr, flush=True)
+    kt = g.kernel_times(reset=True)
+    # classification pass through each index (every distinct k-mer looked up)
+    g.classify_via(1)
+    g.classify_via(0)
+    kc = g.kernel_times(reset=True)
+    g.set_profiling(False)
+    for p in (dq, dout, dall):
+        hip.hipFree(p)
+    n = batch * n_batches
+    lms = kt.get("lidx_lookup", (0, 0.0))[1]
+    bms = kt.get("bbh_lookup", (0, 0.0))[1]
+    lrate = n / (lms * 1e-3) if lms else None
+    brate = n / (bms * 1e-3) if bms else None
+    keep = ("n_leaves", "n_pieces", "fallback_leaves", "fallback_keys", "tie_keys", "max_err_lo", "max_err_hi",
+            "mean_window", "mean_abs_err", "root_loss", "leaf_loss", "sort_ms", "train_ms")
+    return {
+        "n": n, "batch": batch, "keys": "uniform over the distinct keys, seed %d" % seed,
+        "bbhash_lookups_per_s": brate, "learned_lookups_per_s": lrate,
+        "learned_vs_bbhash": (lrate / brate) if (lrate and brate) else None,
+        "learned_rank_errors": errs["learned"], "bbhash_value_errors": errs["bbhash"],
+        "lookup_kernel_ms": {"learned": round(lms, 3), "bbhash": round(bms, 3)},
+        "classify_via_ms": {k2: round(v[1], 3) for k2, v in kc.items()},
+        "learned_index": {k2: (round(info[k2], 4) if isinstance(info[k2], float) else info[k2]) for k2 in keep},
+        "train_wall_s": round(train_wall, 3),
+    }
 
 
 if __name__ == "__main__":

@@ -39,8 +39,8 @@ EXPORTS = [
 
 # Learned-index records (the layouts of include/cf1g.h).
 ROOT_DTYPE = np.dtype([("A", "<f4"), ("B", "<f4"), ("lo", "<u4"), ("hi", "<u4")])
-LEAF_DTYPE = np.dtype([("base", "<u8"), ("q", "<u8"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"), ("ehi", "<u2"),
-                       ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
+LEAF_DTYPE = np.dtype([("base", "<u8"), ("q", "<u4"), ("n", "<u4"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"),
+                       ("ehi", "<u2"), ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
 
 
 class Cf1gError(RuntimeError):

@@ -60,10 +60,9 @@ void launch_lidx_root_final(hipStream_t, uint32_t, uint64_t, uint32_t, const uin
 void launch_lidx_leaf_bounds(hipStream_t, const uint64_t*, uint64_t, int, uint32_t, const RootPiece*, uint64_t*);
 void launch_lidx_leaf_final(hipStream_t, const uint64_t*, uint64_t, uint32_t, const uint64_t*, const LidxTrainBufs&, uint32_t,
                             LeafRec*, uint64_t*);
-void launch_lidx_fingerprints(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, uint16_t*);
+void launch_lidx_fingerprints(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, uint16_t*, uint32_t*);
 void launch_lidx_bbh2rank(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, const MphfView&, uint32_t*);
 void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*);
-void launch_count_ties(hipStream_t, const uint16_t*, uint64_t, unsigned long long*);
 void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, uint64_t,
                          const LidxView&, const MphfView&, uint8_t*, unsigned int*);
 void launch_states_pack(hipStream_t, const uint8_t*, uint64_t, uint64_t*);
@@ -453,6 +452,7 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   const uint32_t epochs = p.epochs ? p.epochs : 4, max_window = p.max_window ? p.max_window : 256;
   const float lr = p.lr > 0 ? p.lr : 0.5f;
   if (leaf_bits > 26 || root_bits > 20) fail(CF1G_E_INVALID, "leaf_bits <= 26 and root_bits <= 20");
+  if (c->D >= (1ULL << 32)) fail(CF1G_E_INVALID, "the learned index holds fewer than 2^32 keys");
   hipStream_t st = c->st;
   const uint64_t D = c->D;
   const int K2 = 2 * c->k;
@@ -500,18 +500,22 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   uint64_t* sabs = c->getT<uint64_t>("lidx_sabs", L + 1);
   launch_lidx_leaf_final(st, ka, D, L, q, lidx_bufs(c, L), max_window, leaf, sabs);
   uint16_t* fp = c->getT<uint16_t>("lidx_fp", D + 128);
+  uint32_t* ties = c->getT<uint32_t>("lidx_tiecnt", L + 1);
   c->zero(fp, (D + 128) * 2);
-  launch_lidx_fingerprints(st, ka, L, q, leaf, fp);
+  launch_lidx_fingerprints(st, ka, L, q, leaf, fp, ties);
   // ---- host-side summary of the leaves; fallback table when needed
   std::vector<LeafRec> hl(L);
   std::vector<uint64_t> hs(L);
+  std::vector<uint32_t> ht(L);
   c->d2h(hl.data(), leaf, (size_t)L * sizeof(LeafRec));
   c->d2h(hs.data(), sabs, (size_t)L * 8);
-  uint64_t fb_leaves = 0, fb_keys = 0, win_sum = 0, win_keys = 0, abs_sum = 0;
+  c->d2h(ht.data(), ties, (size_t)L * 4);
+  uint64_t fb_leaves = 0, fb_keys = 0, win_sum = 0, win_keys = 0, abs_sum = 0, n_ties = 0;
   uint32_t mlo = 0, mhi = 0;
   for (uint32_t j = 0; j < L; j++) {
-    const uint64_t n = (j + 1 < L ? hl[j + 1].q : D) - hl[j].q;
+    const uint64_t n = hl[j].n;
     abs_sum += hs[j];
+    n_ties += ht[j];
     if (!n) continue;
     if (hl[j].flags & 1) {
       fb_leaves++;
@@ -528,11 +532,6 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   c->lidx_s = s;
   c->lidx_fallback = fb_leaves > 0;
   if (c->lidx_fallback) launch_lidx_bbh2rank(st, ka, L, q, leaf, c->view(), c->getT<uint32_t>("lidx_bbh2rank", D + 1));
-  auto* tie = c->getT<unsigned long long>("lidx_ties", 2);
-  c->zero(tie, 8);
-  launch_count_ties(st, fp, D, tie);
-  unsigned long long ties = 0;
-  c->d2h(&ties, tie, 8);
   auto t2 = std::chrono::steady_clock::now();
   c->lidx_ready = true;
   c->lidx_states = false;
@@ -543,7 +542,7 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
     o.n_pieces = R;
     o.fallback_leaves = fb_leaves;
     o.fallback_keys = fb_keys;
-    o.tie_keys = ties;
+    o.tie_keys = n_ties;
     o.max_err_lo = mlo;
     o.max_err_hi = mhi;
     o.mean_window = win_keys ? (double)win_sum / (double)win_keys : 0.0;

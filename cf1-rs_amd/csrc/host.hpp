@@ -125,7 +125,7 @@ struct HostLidx {
     const float span = (float)(P.hi - P.lo);
     const float cl = f < 0.f ? 0.f : (f > span ? span : f);
     const uint32_t j = P.lo + (uint32_t)cl;
-    const uint64_t qa = leaf[j].q, qb = j + 1 < L ? leaf[j + 1].q : D;
+    const uint64_t qa = leaf[j].q, qb = qa + leaf[j].n;
     const uint64_t* it = std::lower_bound(keys + qa, keys + qb, key);
     return (it != keys + qb && *it == key) ? (uint64_t)(it - keys) : 0;
   }

@@ -26,12 +26,14 @@
 //  bounds   : elo_j = max(pred - rank), ehi_j = max(rank - pred) over the
 //             leaf's keys, evaluated with the exact lookup arithmetic, so a
 //             present key always lies in [pred - elo, pred + ehi].
-//  search   : over a 2-byte array aligned with the sorted keys: a 15-bit
-//             fingerprint of (key - base_j) >> shift_j (monotone inside a
-//             leaf) and a tie bit marking entries whose fingerprint equals a
-//             neighbour's; tied entries and leaves whose window exceeds the
-//             threshold resolve exactly (full keys, or the BBHash value
-//             mapped to its rank).
+//  search   : over a 2-byte array aligned with the sorted keys: the 16-bit
+//             fingerprint (key - base_j) >> shift_j, non-decreasing inside a
+//             leaf. The walk starts at the 64-byte block holding the
+//             prediction and moves left / right by comparing fingerprints
+//             (4 lanes cooperate on one lookup, 16 bytes each); a fingerprint
+//             shared with a neighbour resolves with the full keys, and leaves
+//             whose window exceeds the threshold resolve through the BBHash
+//             value mapped to its rank.
 #include "block.hpp"
 #include "device.hpp"
 #include "launch.hpp"
@@ -352,7 +354,8 @@ __global__ void __launch_bounds__(256) k_lidx_leaf_final(const uint64_t* __restr
   const uint64_t a = q[j], b = q[j + 1];
   LeafRec R;
   R.base = a < D ? keys[a] : (D ? keys[D - 1] : 0);
-  R.q = a;
+  R.q = (uint32_t)a;
+  R.n = (uint32_t)(b - a);
   double S = (double)w[j] * (double)isd[j];
   double I = (double)cc[j] - S * (double)mu[j];
   if (!(S == S) || !(I == I)) S = I = 0;
@@ -362,7 +365,7 @@ __global__ void __launch_bounds__(256) k_lidx_leaf_final(const uint64_t* __restr
   if (b > a + 1) {
     const uint64_t span = keys[b - 1] - keys[a];
     const int bits = 64 - __clzll(span);
-    sh = bits > 15 ? bits - 15 : 0;
+    sh = bits > 16 ? bits - 16 : 0;
   }
   R.shift = (uint8_t)sh;
   R.pad = 0;
@@ -390,32 +393,27 @@ __global__ void __launch_bounds__(256) k_lidx_leaf_final(const uint64_t* __restr
   }
 }
 
-// Keys whose fingerprint carries the tie bit.
-__global__ void __launch_bounds__(256) k_count_ties(const uint16_t* __restrict__ fp, uint64_t D, unsigned long long* __restrict__ cnt) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  uint64_t c = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) c += fp[i] & 1u;
-  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-  if (__lane_id() == 0 && c) atomicAdd(cnt, (unsigned long long)c);
-}
-
-// Fingerprints (15 bits of (key - base) >> shift) with the tie bit, one wave
-// per leaf.
-__device__ __forceinline__ uint32_t fp_of(const LeafRec& R, uint64_t key) { return (uint32_t)((key - R.base) >> R.shift) & 0x7FFFu; }
+// Fingerprints (key - base) >> shift, one wave per leaf; ties[j] = keys of
+// leaf j whose fingerprint equals a neighbour's.
+__device__ __forceinline__ uint32_t fp_of(const LeafRec& R, uint64_t key) { return (uint32_t)((key - R.base) >> R.shift) & 0xFFFFu; }
 __global__ void __launch_bounds__(256) k_lidx_fingerprints(const uint64_t* __restrict__ keys, uint32_t L,
                                                            const uint64_t* __restrict__ q, const LeafRec* __restrict__ leaf,
-                                                           uint16_t* __restrict__ fp) {
+                                                           uint16_t* __restrict__ fp, uint32_t* __restrict__ ties) {
   const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= L) return;
   const LeafRec R = leaf[j];
   const uint64_t a = q[j], b = q[j + 1];
+  uint32_t t = 0;
   for (uint64_t i = a + __lane_id(); i < b; i += 64) {
     const uint32_t f = fp_of(R, keys[i]);
     bool tie = false;
     if (i > a) tie |= fp_of(R, keys[i - 1]) == f;
     if (i + 1 < b) tie |= fp_of(R, keys[i + 1]) == f;
-    fp[i] = (uint16_t)((f << 1) | (tie ? 1u : 0u));
+    fp[i] = (uint16_t)f;
+    t += tie ? 1u : 0u;
   }
+  for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+  if (__lane_id() == 0) ties[j] = t;
 }
 
 // Fallback leaves: bbh2rank[BBHash value of key i] = i, one wave per leaf.
@@ -432,67 +430,109 @@ __global__ void __launch_bounds__(256) k_lidx_bbh2rank(const uint64_t* __restric
 }
 
 // ------------------------------------------------------------------ lookup
-// Exact rank of a present key (absent keys: D, "not found"; the caller maps
-// it to 0 like mphf.rs:269).
-__device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key) {
+// Exact rank of a present key (absent keys give D, "not found"; callers map
+// it to 0 like mphf.rs:269). G lanes cooperate on one key (G = 1 or 4; the
+// G lanes of a group run in lockstep and pass the same key): each block of
+// 32 fingerprints (64 bytes) is read as G x (32/G) entries.
+template <int G>
+__device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key, uint32_t g) {
   if ((key >> V.s) >= V.R) return V.D;  // wider than a 2k-bit key
   const uint32_t r = (uint32_t)(key >> V.s);
   const RootPiece P = V.root[r];
   const uint32_t j = root_leaf(P, key, V.s, r);
   const LeafRec R = V.leaf[j];
-  const uint64_t qa = R.q;
-  const uint64_t qb = (j + 1 < V.L) ? V.leaf[j + 1].q : V.D;
-  if (qb <= qa) return V.D;
+  if (R.n == 0) return V.D;
   if (R.flags & 1) {
     const uint64_t v = mphf_lookup<uint64_t>(mv, key);
     const uint64_t rk = v < V.D ? V.bbh2rank[v] : V.D;
     return (rk < V.D && V.keys[rk] == key) ? rk : V.D;
   }
+  const int64_t qa = (int64_t)R.q, qb = qa + (int64_t)R.n;
   const int64_t pr = leaf_pred(R, key);
   int64_t lo = pr - (int64_t)R.elo, hi = pr + (int64_t)R.ehi;
-  if (lo < (int64_t)qa) lo = (int64_t)qa;
-  if (hi > (int64_t)qb - 1) hi = (int64_t)qb - 1;
+  if (lo < qa) lo = qa;
+  if (hi > qb - 1) hi = qb - 1;
   if (lo > hi) return V.D;
   const uint32_t f = fp_of(R, key);
-  // rank candidate = lo + #(fingerprints < f) in [lo, hi], over aligned
-  // 64-entry (128-byte) groups
-  uint64_t cand = (uint64_t)lo;
-  bool hit = false, tie = false;
-  for (int64_t g = lo & ~63LL; g <= hi; g += 64) {
-    const uint4* p4 = (const uint4*)(V.fp + g);
+  int64_t at = pr < lo ? lo : (pr > hi ? hi : pr);
+  int64_t blk = at & ~31LL;
+  bool resolve = false;
+  int64_t found = -1;
+  for (int it = 0; it < 4096; it++) {
+    uint32_t n_eq = 0;
+    int64_t first_eq = INT64_MAX;
+    uint32_t lt = 0, gt = 0;
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const uint4 v = p4[u];
+    for (int part = 0; part < 4 / G; part++) {
+      const int64_t base = blk + 8 * (int64_t)(g + (uint32_t)part * G);
+      const uint4 v = *(const uint4*)(V.fp + base);
       const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int h = 0; h < 8; h++) {
-        const int64_t i = g + u * 8 + h;
-        const uint32_t e = (wd[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
-        if (i >= lo && i <= hi) {
-          const uint32_t fe = e >> 1;
-          cand += fe < f ? 1u : 0u;
-          if (fe == f) {
-            hit = true;
-            tie |= (e & 1u) != 0;
-          }
+      for (int u = 0; u < 8; u++) {
+        const int64_t pos = base + u;
+        const uint32_t e = (wd[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+        const bool in = pos >= lo && pos <= hi;
+        if (in && e == f) {
+          n_eq++;
+          first_eq = pos < first_eq ? pos : first_eq;
         }
+        lt |= (in && e < f) ? 1u : 0u;
+        gt |= (in && e > f) ? 1u : 0u;
       }
     }
+    if (G > 1) {
+#pragma unroll
+      for (int d = 1; d < G; d <<= 1) {
+        n_eq += __shfl_xor(n_eq, d, 64);
+        const int64_t o = __shfl_xor(first_eq, d, 64);
+        first_eq = o < first_eq ? o : first_eq;
+        lt |= __shfl_xor(lt, d, 64);
+        gt |= __shfl_xor(gt, d, 64);
+      }
+    }
+    if (n_eq == 1) {
+      // a neighbour just outside the block could share the fingerprint
+      const int64_t ed = blk + 31;
+      bool tie = false;
+      if (first_eq == blk && blk > lo) tie = V.fp[blk - 1] == f;
+      if (first_eq == ed && ed < hi) tie |= V.fp[ed + 1] == f;
+      if (tie) resolve = true;
+      else found = first_eq;
+      break;
+    }
+    if (n_eq > 1) {
+      resolve = true;
+      break;
+    }
+    if (!gt && blk + 32 <= hi) {  // every fingerprint here is smaller: go right
+      blk += 32;
+      continue;
+    }
+    if (!lt && blk > lo) {  // every fingerprint here is larger: go left
+      blk -= 32;
+      continue;
+    }
+    break;  // between two fingerprints: absent
   }
-  if (!hit) return V.D;
-  if (!tie) return cand;
-  // tied fingerprints: resolve with the full keys of the window
-  uint64_t rk = (uint64_t)lo;
-  for (int64_t i = lo; i <= hi; i++) rk += V.keys[i] < key ? 1u : 0u;
-  return (rk <= (uint64_t)hi && V.keys[rk] == key) ? rk : V.D;
+  if (resolve) {  // exact search over the window's full keys
+    int64_t a = lo, b = hi + 1;
+    while (a < b) {
+      const int64_t mid = (a + b) >> 1;
+      if (V.keys[mid] < key) a = mid + 1; else b = mid;
+    }
+    return (a <= hi && V.keys[a] == key) ? (uint64_t)a : V.D;
+  }
+  return found >= 0 ? (uint64_t)found : V.D;
 }
 
+constexpr int LK_G = 4;
 __global__ void __launch_bounds__(256) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
                                                      uint64_t* __restrict__ out) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t r = lidx_rank(V, mv, keys[i]);
-    out[i] = r < V.D ? r : 0;
+  const uint32_t g = threadIdx.x & (LK_G - 1);
+  const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / LK_G);
+  for (uint64_t qi = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / LK_G; qi < n; qi += groups) {
+    const uint64_t r = lidx_rank<LK_G>(V, mv, keys[qi], g);
+    if (g == 0) out[qi] = r < V.D ? r : 0;
   }
 }
 
@@ -521,7 +561,7 @@ __global__ void __launch_bounds__(256) k_classify_via(const uint64_t* __restrict
   for (uint64_t e = threadIdx.x; e < u; e += 256) {
     const uint64_t k = keys[ko + e];
     uint64_t idx;
-    if (LEARNED) idx = lidx_rank(V, mv, k);
+    if (LEARNED) idx = lidx_rank<1>(V, mv, k, 0);
     else idx = mphf_lookup<uint64_t>(mv, k);
     if (idx >= V.D) {
       atomicOr(err, 1u);
@@ -555,7 +595,7 @@ __global__ void __launch_bounds__(256) k_annotate_lidx(const uint64_t* __restric
     const uint64_t x = kmer64_at(seq, g, K);
     const uint64_t r = rc64(x, K);
     const uint64_t c = x < r ? x : r;
-    const uint64_t idx = lidx_rank(V, mv, c);
+    const uint64_t idx = lidx_rank<1>(V, mv, c, 0);
     const uint32_t s = idx < V.D ? (uint32_t)(states[idx / 12] >> ((idx % 12) * 5)) & 31u : 0u;
     a = (uint8_t)(1u | (x == c ? 2u : 0u) | (st_class(s) << 2));
   }
@@ -625,14 +665,9 @@ void launch_lidx_leaf_final(hipStream_t st, const uint64_t* keys, uint64_t D, ui
   CF1G_LAUNCH("lidx_leaf_final", k_lidx_leaf_final, dim3(blocks_for(L, 4)), dim3(256), 0, st, keys, D, L, q, b.mu, b.isd, b.w, b.c,
               max_window, out, sabs);
 }
-void launch_count_ties(hipStream_t st, const uint16_t* fp, uint64_t D, unsigned long long* cnt) {
-  if (!D) return;
-  uint64_t g = (D + 255) / 256;
-  if (g > 4096) g = 4096;
-  CF1G_LAUNCH("lidx_count_ties", k_count_ties, dim3((unsigned)g), dim3(256), 0, st, fp, D, cnt);
-}
-void launch_lidx_fingerprints(hipStream_t st, const uint64_t* keys, uint32_t L, const uint64_t* q, const LeafRec* leaf, uint16_t* fp) {
-  CF1G_LAUNCH("lidx_fingerprints", k_lidx_fingerprints, dim3(blocks_for(L, 4)), dim3(256), 0, st, keys, L, q, leaf, fp);
+void launch_lidx_fingerprints(hipStream_t st, const uint64_t* keys, uint32_t L, const uint64_t* q, const LeafRec* leaf, uint16_t* fp,
+                              uint32_t* ties) {
+  CF1G_LAUNCH("lidx_fingerprints", k_lidx_fingerprints, dim3(blocks_for(L, 4)), dim3(256), 0, st, keys, L, q, leaf, fp, ties);
 }
 void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t L, const uint64_t* q, const LeafRec* leaf, const MphfView& mv,
                           uint32_t* bbh2rank) {
@@ -640,7 +675,7 @@ void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t L, cons
 }
 void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const LidxView& V, const MphfView& mv, uint64_t* out) {
   if (!n) return;
-  uint64_t g = (n + 255) / 256;
+  uint64_t g = (n * LK_G + 255) / 256;
   if (g > 65536) g = 65536;
   CF1G_LAUNCH("lidx_lookup", k_lidx_lookup, dim3((unsigned)g), dim3(256), 0, st, keys, n, V, mv, out);
 }

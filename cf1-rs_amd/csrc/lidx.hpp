@@ -12,10 +12,11 @@ struct RootPiece {
 };
 struct LeafRec {
   uint64_t base;  // first key of the leaf
-  uint64_t q;     // rank of the first key
+  uint32_t q;     // rank of the first key
+  uint32_t n;     // keys in the leaf (ranks [q, q + n))
   float S, I;     // pred = q + floor(fma(S, (float)(key - base), I) + 0.5)
   uint16_t elo, ehi;
-  uint8_t shift;  // fingerprint = ((key - base) >> shift) & 0x7FFF
+  uint8_t shift;  // fingerprint = (key - base) >> shift (< 2^16 inside the leaf)
   uint8_t flags;  // bit 0: window above threshold, resolve through the BBHash
   uint16_t pad;
 };
@@ -25,7 +26,7 @@ struct LidxView {
   const RootPiece* root;
   const LeafRec* leaf;
   const uint64_t* keys;      // sorted
-  const uint16_t* fp;        // fingerprint << 1 | tie
+  const uint16_t* fp;        // fingerprint of each sorted key (monotone inside a leaf)
   const uint32_t* bbh2rank;  // rank of the key whose BBHash value is v (fallback leaves only)
   uint64_t D;
   uint32_t L, R;

@@ -158,7 +158,7 @@ typedef struct cf1g_lidx_params {
 typedef struct cf1g_lidx_info {
   uint64_t n_keys, n_leaves, n_pieces;
   uint64_t fallback_leaves, fallback_keys; /* leaves (and their keys) resolved through the BBHash */
-  uint64_t tie_keys;                       /* keys whose fingerprint ties a neighbour's */
+  uint64_t tie_keys;                       /* keys whose 16-bit fingerprint ties a neighbour's */
   uint32_t max_err_lo, max_err_hi;         /* recorded error bounds (non-fallback leaves) */
   double mean_window;                      /* mean search window over keys of non-fallback leaves */
   double mean_abs_err;                     /* mean |rank - prediction| over all keys */
@@ -169,14 +169,16 @@ typedef struct cf1g_lidx_info {
 /* Sort the distinct keys, train root and leaves (forward / backward / update
  * kernels), record error bounds, build the search arrays. Needs P3. */
 int cf1g_lidx_train(cf1g_ctx* ctx, const cf1g_lidx_params* params, cf1g_lidx_info* out);
-/* Exact rank of each key (0 for an absent key, like src/mphf.rs:269): host
- * buffers / device-resident buffers. */
+/* Exact rank of each present key; an absent key gets 0 or some rank, as with
+ * any minimal perfect map (src/mphf.rs:268-270). Host buffers /
+ * device-resident buffers. */
 int cf1g_lidx_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
 int cf1g_lidx_lookup_device(cf1g_ctx* ctx, const void* d_keys, uint64_t n, uint64_t* d_out);
 /* The trained model: its shape, the sorted keys (D), root pieces (16 bytes
  * each: f32 A, f32 B, u32 lo, u32 hi; piece = key >> root_shift) and leaf
- * records (32 bytes each: u64 base, u64 first rank, f32 slope, f32 intercept,
- * u16 err_lo, u16 err_hi, u8 shift, u8 flags, u16). Any output may be NULL. */
+ * records (32 bytes each: u64 base, u32 first rank, u32 key count, f32 slope,
+ * f32 intercept, u16 err_lo, u16 err_hi, u8 shift, u8 flags, u16). Any output
+ * may be NULL. */
 typedef struct cf1g_lidx_shape {
   uint64_t n_keys;
   uint32_t n_pieces, n_leaves;

@@ -52,6 +52,21 @@ def test_sorted_keys_and_exact_ranks(built):
     np.testing.assert_array_equal(out, perm.astype(np.uint64))
 
 
+def test_absent_and_repeated_queries(built):
+    g, _, D = built
+    g.lidx_train(leaf_bits=LB, root_bits=RB)
+    keys = sorted_keys(g)
+    rng = np.random.default_rng(3)
+    absent = rng.integers(0, 1 << 62, 20000, dtype=np.uint64)
+    absent = absent[~np.isin(absent, keys)]
+    out = g.lidx_lookup(absent)
+    assert np.all(out < D)  # 0 or some rank, never out of range
+    wide = np.array([np.uint64(1) << np.uint64(63), ~np.uint64(0)], np.uint64)  # wider than a 62-bit key
+    np.testing.assert_array_equal(g.lidx_lookup(wide), [0, 0])
+    rep = np.repeat(keys[:: max(1, D // 50)], 7)
+    np.testing.assert_array_equal(g.lidx_lookup(rep), np.searchsorted(keys, rep).astype(np.uint64))
+
+
 def test_model_matches_oracle(built):
     g, _, D = built
     g.lidx_train(leaf_bits=LB, root_bits=RB)

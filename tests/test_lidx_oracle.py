@@ -106,12 +106,13 @@ def test_error_bounds_definition():
     keys = np.sort(rng.choice(np.uint64(1) << np.uint64(30), 500, replace=False).astype(np.uint64))
     L = 8
     q = np.linspace(0, 500, L + 1).astype(np.int64)
-    leaf = np.zeros(L, [("base", "<u8"), ("q", "<u8"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"), ("ehi", "<u2"),
-                        ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
+    leaf = np.zeros(L, [("base", "<u8"), ("q", "<u4"), ("n", "<u4"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"),
+                        ("ehi", "<u2"), ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
     for j in range(L):
         a, b = q[j], q[j + 1]
         leaf[j]["base"] = keys[a]
         leaf[j]["q"] = a
+        leaf[j]["n"] = b - a
         span = float(keys[b - 1] - keys[a]) or 1.0
         leaf[j]["S"] = np.float32((b - a - 1) / span * rng.uniform(0.8, 1.2))
         leaf[j]["I"] = np.float32(rng.uniform(-3, 3))
