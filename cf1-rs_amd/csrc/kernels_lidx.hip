@@ -28,10 +28,10 @@
 //             present key always lies in [pred - elo, pred + ehi].
 //  search   : over a 2-byte array aligned with the sorted keys: the 16-bit
 //             fingerprint (key - base_j) >> shift_j, non-decreasing inside a
-//             leaf. The walk starts at the 64-byte block holding the
-//             prediction and moves left / right by comparing fingerprints
-//             (4 lanes cooperate on one lookup, 16 bytes each); a fingerprint
-//             shared with a neighbour resolves with the full keys, and leaves
+//             leaf. The walk starts at the 16-byte chunk holding the
+//             prediction and moves left / right by comparing fingerprints;
+//             a fingerprint shared with a neighbour resolves with the full
+//             keys, and leaves
 //             whose window exceeds the threshold resolve through the BBHash
 //             value mapped to its rank.
 #include "block.hpp"
@@ -431,11 +431,11 @@ __global__ void __launch_bounds__(256) k_lidx_bbh2rank(const uint64_t* __restric
 
 // ------------------------------------------------------------------ lookup
 // Exact rank of a present key (absent keys give D, "not found"; callers map
-// it to 0 like mphf.rs:269). G lanes cooperate on one key (G = 1 or 4; the
-// G lanes of a group run in lockstep and pass the same key): each block of
-// 32 fingerprints (64 bytes) is read as G x (32/G) entries.
-template <int G>
-__device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key, uint32_t g) {
+// it to 0 like mphf.rs:269). One lane per key: the walk reads the 16-byte
+// chunk of 8 fingerprints holding the prediction and steps one chunk left
+// or right while every fingerprint in it is larger / smaller than the key's
+// (adjacent chunks usually share the cache line).
+__device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key) {
   if ((key >> V.s) >= V.R) return V.D;  // wider than a 2k-bit key
   const uint32_t r = (uint32_t)(key >> V.s);
   const RootPiece P = V.root[r];
@@ -454,48 +454,30 @@ __device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView&
   if (hi > qb - 1) hi = qb - 1;
   if (lo > hi) return V.D;
   const uint32_t f = fp_of(R, key);
-  int64_t at = pr < lo ? lo : (pr > hi ? hi : pr);
-  int64_t blk = at & ~31LL;
+  int64_t c = (pr < lo ? lo : (pr > hi ? hi : pr)) & ~7LL;
   bool resolve = false;
   int64_t found = -1;
-  for (int it = 0; it < 4096; it++) {
-    uint32_t n_eq = 0;
-    int64_t first_eq = INT64_MAX;
-    uint32_t lt = 0, gt = 0;
+  for (int it = 0; it < 16384; it++) {
+    const uint4 v = *(const uint4*)(V.fp + c);
+    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+    uint32_t n_eq = 0, lt = 0, gt = 0;
+    int64_t first_eq = -1;
 #pragma unroll
-    for (int part = 0; part < 4 / G; part++) {
-      const int64_t base = blk + 8 * (int64_t)(g + (uint32_t)part * G);
-      const uint4 v = *(const uint4*)(V.fp + base);
-      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int64_t pos = base + u;
-        const uint32_t e = (wd[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-        const bool in = pos >= lo && pos <= hi;
-        if (in && e == f) {
-          n_eq++;
-          first_eq = pos < first_eq ? pos : first_eq;
-        }
-        lt |= (in && e < f) ? 1u : 0u;
-        gt |= (in && e > f) ? 1u : 0u;
-      }
-    }
-    if (G > 1) {
-#pragma unroll
-      for (int d = 1; d < G; d <<= 1) {
-        n_eq += __shfl_xor(n_eq, d, 64);
-        const int64_t o = __shfl_xor(first_eq, d, 64);
-        first_eq = o < first_eq ? o : first_eq;
-        lt |= __shfl_xor(lt, d, 64);
-        gt |= __shfl_xor(gt, d, 64);
-      }
+    for (int u = 7; u >= 0; u--) {
+      const int64_t pos = c + u;
+      const uint32_t e = (wd[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+      const bool in = pos >= lo && pos <= hi;
+      const bool eq = in && e == f;
+      n_eq += eq ? 1u : 0u;
+      first_eq = eq ? pos : first_eq;
+      lt |= (in && e < f) ? 1u : 0u;
+      gt |= (in && e > f) ? 1u : 0u;
     }
     if (n_eq == 1) {
-      // a neighbour just outside the block could share the fingerprint
-      const int64_t ed = blk + 31;
+      // a neighbour just outside the chunk could share the fingerprint
       bool tie = false;
-      if (first_eq == blk && blk > lo) tie = V.fp[blk - 1] == f;
-      if (first_eq == ed && ed < hi) tie |= V.fp[ed + 1] == f;
+      if (first_eq == c && c > lo) tie = V.fp[c - 1] == f;
+      if (first_eq == c + 7 && c + 7 < hi) tie |= V.fp[c + 8] == f;
       if (tie) resolve = true;
       else found = first_eq;
       break;
@@ -504,12 +486,12 @@ __device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView&
       resolve = true;
       break;
     }
-    if (!gt && blk + 32 <= hi) {  // every fingerprint here is smaller: go right
-      blk += 32;
+    if (!gt && c + 8 <= hi) {  // every fingerprint here is smaller: go right
+      c += 8;
       continue;
     }
-    if (!lt && blk > lo) {  // every fingerprint here is larger: go left
-      blk -= 32;
+    if (!lt && c > lo) {  // every fingerprint here is larger: go left
+      c -= 8;
       continue;
     }
     break;  // between two fingerprints: absent
@@ -525,14 +507,12 @@ __device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView&
   return found >= 0 ? (uint64_t)found : V.D;
 }
 
-constexpr int LK_G = 4;
 __global__ void __launch_bounds__(256) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
                                                      uint64_t* __restrict__ out) {
-  const uint32_t g = threadIdx.x & (LK_G - 1);
-  const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / LK_G);
-  for (uint64_t qi = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / LK_G; qi < n; qi += groups) {
-    const uint64_t r = lidx_rank<LK_G>(V, mv, keys[qi], g);
-    if (g == 0) out[qi] = r < V.D ? r : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t r = lidx_rank(V, mv, keys[i]);
+    out[i] = r < V.D ? r : 0;
   }
 }
 
@@ -561,7 +541,7 @@ __global__ void __launch_bounds__(256) k_classify_via(const uint64_t* __restrict
   for (uint64_t e = threadIdx.x; e < u; e += 256) {
     const uint64_t k = keys[ko + e];
     uint64_t idx;
-    if (LEARNED) idx = lidx_rank<1>(V, mv, k, 0);
+    if (LEARNED) idx = lidx_rank(V, mv, k);
     else idx = mphf_lookup<uint64_t>(mv, k);
     if (idx >= V.D) {
       atomicOr(err, 1u);
@@ -595,7 +575,7 @@ __global__ void __launch_bounds__(256) k_annotate_lidx(const uint64_t* __restric
     const uint64_t x = kmer64_at(seq, g, K);
     const uint64_t r = rc64(x, K);
     const uint64_t c = x < r ? x : r;
-    const uint64_t idx = lidx_rank<1>(V, mv, c, 0);
+    const uint64_t idx = lidx_rank(V, mv, c);
     const uint32_t s = idx < V.D ? (uint32_t)(states[idx / 12] >> ((idx % 12) * 5)) & 31u : 0u;
     a = (uint8_t)(1u | (x == c ? 2u : 0u) | (st_class(s) << 2));
   }
@@ -675,7 +655,7 @@ void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t L, cons
 }
 void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const LidxView& V, const MphfView& mv, uint64_t* out) {
   if (!n) return;
-  uint64_t g = (n * LK_G + 255) / 256;
+  uint64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
   CF1G_LAUNCH("lidx_lookup", k_lidx_lookup, dim3((unsigned)g), dim3(256), 0, st, keys, n, V, mv, out);
 }
