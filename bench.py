@@ -32,10 +32,10 @@ METRIC = "distinct k-mers/s phases 1-4 at k=31 vs CPU; learned-index vs BBHash l
 HBM_PEAK_GBS = 8000.0
 
 PHASE_OF = {
-    "pack": "P1", "p1_minimizer_hist": "P1",
-    "p2_partition": "P2", "p2_route": "P2",
+    "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
+    "p2_partition": "P2", "p2_count": "P2", "p2_route": "P2",
     "p3_count": "P3", "p3_scatter": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
-    "scan_tile": "P3", "scan_add": "P3", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
+    "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",
 }
 
@@ -75,15 +75,20 @@ def mphf_level_stats(mphf_words):
 def algorithmic_bytes(info, kb, rw):
     """Algorithmic HBM bytes per kernel over one step: what each kernel's
     function must read and write at least once (inputs, outputs, no
-    intermediate re-reads).
+    intermediate re-reads). NG = stream positions, W / S / D = k-mer
+    occurrences / super k-mers / distinct k-mers, B = BBHash bytes, rec = S
+    fixed-width records, ob = key + 2-byte mask.
       pack      : ASCII in, 2-bit bases + placeholder bits out
-      p1        : packed stream in (histogram output is 512 KiB)
-      p2_route  : packed stream in, fixed-width super k-mer records out
+      p1_scan   : packed stream in, one 8-byte entry per super k-mer out
+      p1_hist   : entries in (the 512 KiB histogram is noise)
+      p2_count  : entries in
+      p2_route  : entries in, the super k-mers' bases in, records out
       p3_count  : records in
       p3_scatter: records in, W (key, mask) occurrences out
-      p3_dedup  : W occurrences in, D distinct (key, mask) + BBHash blocks out
-      p3_compact: D (key, mask) + blocks in, the same out (dense)
-      p4        : D masks in, ceil(D/12) state words out
+      p3_dedup  : W occurrences in; D distinct (key, mask), D rank-ordered
+                  masks and the BBHash blocks out
+      p3_compact: D keys, D masks, B in; the same out (dense)
+      p4_classify: D masks in, ceil(D/12) state words out
     """
     NG = info["stream_len"]
     W, S, D = info["n_kmers"], info["n_superkmers"], info["n_distinct"]
@@ -93,14 +98,32 @@ def algorithmic_bytes(info, kb, rw):
     ob = kb + 2
     return {
         "pack": NG + seq,
-        "p1_minimizer_hist": seq,
-        "p2_route": seq + rec,
+        "p1_scan": seq + 8 * S,
+        "p1_hist": 8 * S,
+        "p2_count": 8 * S,
+        "p2_route": 8 * S + 2 * rec,
         "p3_count": rec,
         "p3_scatter": rec + W * ob,
-        "p3_dedup": W * ob + D * ob + B,
-        "p3_compact": 2 * (D * ob + B),
+        "p3_dedup": W * ob + D * ob + 2 * D + B,
+        "p3_compact": 2 * (D * kb + 2 * D + B),
         "p4_classify": 2 * D + 8 * ((D + 11) // 12),
     }
+
+
+def profiled_traffic(kernel, workload_tag):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the counter's reading for a 1 GiB device copy) from the
+    committed counter summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload_tag}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_dispatch:"):
+            for tok in line.split(":", 1)[1].split():
+                name, val = tok.split("=")
+                if name == kernel:
+                    return float(val), os.path.relpath(path, ROOT)
+    return None, os.path.relpath(path, ROOT)
 
 
 def cpu_baseline(ascii_, rec_off, k, num_bins, sample_bases, threads):
@@ -197,8 +220,10 @@ def main():
     dom = max((n for n in kern if n in ab), key=lambda n: kern[n]["ms_per_step"])
     dk = kern[dom]
     achieved = dk["gbs"]
+    tag = "genome300m_k%d" % args.k if args.scale == 1.0 and args.num_bins == 128 else None
+    traffic, tsrc = profiled_traffic(dom, tag) if tag else (None, None)
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
             "bytes_per_launch": ab[dom] / dk["launches_per_step"], "mean_launch_ms": round(dk["mean_launch_ms"], 4)}
     tot_bytes = sum(ab.values())
     roof["pipeline"] = {"algorithmic_bytes_per_step": tot_bytes, "achieved": round(tot_bytes / (ms * 1e-3) / 1e9, 1)}
@@ -233,6 +258,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"genome-like {len(ascii_) / 1e6:.0f} Mbp, 24 records, k={args.k}, {args.num_bins} bins, seed 3",
                    "k": args.k, "bases": int(len(ascii_)), "records": int(len(rec_off) - 1),
+                   "stream_len": info["stream_len"], "mphf_words": info["mphf_words"],
                    "kmer_occurrences": info["n_kmers"], "superkmers": info["n_superkmers"], "distinct_kmers": D,
                    "num_bins": info["num_bins"], "seg_bits": info["seg_bits"], "mphf_big_segments": info["mphf_big_segments"],
                    "mphf_mean_levels_probed": round(e_probe, 3)},
