@@ -35,10 +35,12 @@ void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
 void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*, uint64_t*);
-void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
+int p3_part_bits(int);
+void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int, uint64_t*,
                      uint64_t*);
 void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
-                       uint64_t*, void*, uint16_t*);
+                       const uint64_t*, void*, uint16_t*);
+void launch_p3_split(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, uint32_t, int, void*, uint16_t*);
 void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint64_t*, uint32_t*,
                      unsigned int*, uint32_t*, uint64_t*, uint16_t*);
 void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
@@ -202,9 +204,9 @@ struct cf1g_ctx {
 };
 
 namespace {
-// Records per work-group in the P3 count / scatter passes: long enough that
-// each (work-group, segment) run of the scatter spans whole cache lines.
-constexpr uint32_t CHUNK_RECORDS = 16384;
+// Records per work-group in the P3 count / scatter passes (~37K k-mers, so
+// each (chunk, part) run of the scatter is ~hundreds of keys).
+constexpr uint32_t CHUNK_RECORDS = 4096;
 constexpr uint64_t STREAM_PAD = 4096;
 
 struct Fail {
@@ -330,7 +332,6 @@ void do_p3(cf1g_ctx* c) {
   const uint64_t ns = c->n_segs;
   uint64_t* counts = c->getT<uint64_t>("seg_counts", ns + 1);
   uint64_t* seg_off = c->getT<uint64_t>("seg_off", ns + 1);
-  uint64_t* cursor = c->getT<uint64_t>("seg_cursor", ns + 1);
   uint64_t* ucount = c->getT<uint64_t>("seg_ucount", ns + 1);
   uint64_t* key_off = c->getT<uint64_t>("seg_out_off", ns + 1);
   uint64_t* nblk = c->getT<uint64_t>("bbh_seg_nblk", ns + 1);
@@ -344,17 +345,30 @@ void do_p3(cf1g_ctx* c) {
   const uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
   c->zero(counts, (ns + 1) * 8);
   c->zero(flags, 16);
-  launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts);
+  // per-(bin, part, chunk) totals -> scan -> each chunk's run in its part
+  const int pb = p3_part_bits(sb);
+  const uint64_t nm = ((uint64_t)1 << pb) * c->n_chunks;
+  uint64_t* pmat = c->getT<uint64_t>("p3_pmat", nm + 1);
+  uint64_t* poff = c->getT<uint64_t>("p3_poff", nm + 1);
+  launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts, pmat);
   scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
-  hip_check(hipMemcpyAsync(cursor, seg_off, ns * 8, hipMemcpyDeviceToDevice, st), "D2D cursor");
+  uint64_t* tmp2 = c->getT<uint64_t>("scan_tmp2", scan_temp_words(nm + 1) + 16);
+  scan_excl_u64(st, pmat, poff, nm, poff + nm, tmp2);
   uint64_t wcount = 0;
   c->d2h(&wcount, seg_off + ns, 8);
   if (wcount != c->W) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
   void* occ = c->get("occ", c->W * kb + 16);
   uint16_t* occ_obs = c->getT<uint16_t>("occ_obs", c->W + 8);
+  if (pb == sb) {
+    launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, occ, occ_obs);
+  } else {
+    void* part = c->get("occ_aux", c->W * kb + 16);
+    uint16_t* part_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
+    launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, part, part_obs);
+    launch_p3_split(st, c->wide, part, part_obs, seg_off, c->num_bins, sb, occ, occ_obs);
+  }
   // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
   uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
-  launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, cursor, occ, occ_obs);
   // joined masks in MPHF-rank order per segment (occ_obs keeps key order)
   uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
   launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, vpad);

@@ -193,10 +193,19 @@ __device__ __forceinline__ bool p3_chunk(const uint64_t* __restrict__ chunk_off,
   return true;
 }
 
+// Two-level scatter of the occurrences. The 2^sb segments of a bin form
+// 2^pb parts of 2^(sb-pb) consecutive segments (pb = min(sb, P3_PART_BITS)).
+// k_p3_count counts k-mers per (bin, segment) and per (bin, part, chunk);
+// k_p3_scatter writes each chunk's occurrences into its (part, chunk) run
+// (runs of ~hundreds of keys, so the stores fill whole lines); k_p3_split
+// then moves each part's keys into its segments with one work-group per
+// part. A chunk is one work-group's record range of one bin.
+constexpr int P3_PART_BITS = 6;
+
 template <typename KeyT, int RW>
 __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
                                                     const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
-                                                    int K, int sb, uint64_t* __restrict__ counts) {
+                                                    int K, int sb, int pb, uint64_t* __restrict__ counts, uint64_t* __restrict__ mat) {
   __shared__ uint32_t hist[1 << P3_MAX_SB];
   int bin;
   uint64_t r0, r1;
@@ -210,35 +219,162 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
   uint64_t* dst = counts + (uint64_t)bin * ns;
   for (int s = threadIdx.x; s < ns; s += P3_WG)
     if (hist[s]) atomicAdd((unsigned long long*)&dst[s], (unsigned long long)hist[s]);
+  // per-part totals of this chunk at mat[P * chunk_off[bin] + part * chunks_of_bin + chunk]
+  const int np = 1 << pb, spp = 1 << (sb - pb);
+  const uint64_t c0 = chunk_off[bin], nch = chunk_off[bin + 1] - c0, c = (uint64_t)blockIdx.x - c0;
+  for (int p = threadIdx.x; p < np; p += P3_WG) {
+    uint64_t t = 0;
+    for (int q = 0; q < spp; q++) t += hist[p * spp + q];
+    mat[(uint64_t)np * c0 + (uint64_t)p * nch + c] = t;
+  }
 }
 
+// Tiles staged in LDS and counting-sorted by destination before the stores,
+// so that consecutive lanes write consecutive addresses of one run.
+template <typename KeyT>
+struct StageCfg {
+  static constexpr int TILE = sizeof(KeyT) == 8 ? 2048 : 1024;
+};
+
+// Within a staged tile of tn elements with destination ids dst_id[i] < nd:
+// rk[i] = rank among equal ids, cnt / start = per-id counts / tile offsets,
+// inv[start[id] + rk] = i. Barriers inside; cnt must be zero on entry.
+__device__ __forceinline__ void stage_sort(uint32_t tn, const uint8_t* dst_id, uint16_t* rk, uint16_t* inv, uint32_t* cnt,
+                                           uint32_t* start, int nd) {
+  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) rk[i] = (uint16_t)atomicAdd(&cnt[dst_id[i]], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64) {  // exclusive scan of <= 64 counts in one wave
+    const uint32_t v = (int)threadIdx.x < nd ? cnt[threadIdx.x] : 0u;
+    uint32_t x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if ((int)threadIdx.x >= d) x += y;
+    }
+    if ((int)threadIdx.x < nd) start[threadIdx.x] = x - v;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) inv[start[dst_id[i]] + rk[i]] = (uint16_t)i;
+  __syncthreads();
+}
+
+// Pass A: each chunk's occurrences go to its (part, chunk) runs. Records are
+// taken 256 at a time (one per thread); their k-mers fill the LDS tile at the
+// scanned k-mer offsets (a round with more k-mers than a tile is generated
+// in tile windows), then the tile is sorted by part and stored.
 template <typename KeyT, int RW>
 __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
                                                       const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
-                                                      int K, int sb, uint64_t* __restrict__ cursor, KeyT* __restrict__ occ,
+                                                      int K, int sb, int pb, const uint64_t* __restrict__ moff, KeyT* __restrict__ occ,
                                                       uint16_t* __restrict__ occ_obs) {
-  __shared__ uint32_t hist[1 << P3_MAX_SB];
-  __shared__ unsigned long long gb[1 << P3_MAX_SB];
+  constexpr int TILE = StageCfg<KeyT>::TILE;
+  __shared__ KeyT tk[TILE];
+  __shared__ uint16_t tm[TILE];
+  __shared__ uint8_t tp[TILE];
+  __shared__ uint16_t rk[TILE];
+  __shared__ uint16_t inv[TILE];
+  __shared__ uint32_t cnt[1 << P3_PART_BITS], start[1 << P3_PART_BITS];
+  __shared__ uint64_t base[1 << P3_PART_BITS];
+  __shared__ uint64_t sh4[4];
   int bin;
   uint64_t r0, r1;
   if (!p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
-  const int ns = 1 << sb;
-  for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
+  const int np = 1 << pb, psh = sb - pb;
+  const uint64_t c0 = chunk_off[bin], nch = chunk_off[bin + 1] - c0, c = (uint64_t)blockIdx.x - c0;
+  for (int p = threadIdx.x; p < np; p += P3_WG) {
+    cnt[p] = 0;
+    base[p] = moff[(uint64_t)np * c0 + (uint64_t)p * nch + c];
+  }
   __syncthreads();
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
+  for (uint64_t rr = r0; rr < r1; rr += P3_WG) {
+    const uint64_t r = rr + threadIdx.x;
+    uint32_t n = 0;
+    if (r < r1) {
+      int L = (int)(rec[r * RW + RW - 1] & 255);
+      if (L > 32 * RW - 6) L = 32 * RW - 6;
+      n = L >= K ? (uint32_t)(L - K + 1) : 0u;
+    }
+    uint64_t T;
+    const uint64_t o = block_excl_scan_u64(n, sh4, &T);
+    for (uint64_t w0 = 0; w0 < T; w0 += TILE) {
+      const uint32_t tn = (uint32_t)(T - w0 < (uint64_t)TILE ? T - w0 : (uint64_t)TILE);
+      if (n && o < w0 + tn && o + n > w0) {
+        uint64_t j = o;
+        for_each_rec_occ<KeyT, RW, true>(rec, r, K, [&](const KeyT& key, uint32_t m) {
+          if (j >= w0 && j < w0 + tn) {
+            const uint32_t t = (uint32_t)(j - w0);
+            tk[t] = key;
+            tm[t] = (uint16_t)m;
+            tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
+          }
+          j++;
+        });
+      }
+      __syncthreads();
+      stage_sort(tn, tp, rk, inv, cnt, start, np);
+      for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
+        const uint32_t e = inv[i], p = tp[e];
+        const uint64_t at = base[p] + (i - start[p]);
+        occ[at] = tk[e];
+        occ_obs[at] = tm[e];
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < np; p += P3_WG) {
+        base[p] += cnt[p];
+        cnt[p] = 0;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Pass B: one work-group per (bin, part) moves the part's keys [seg_off[s0],
+// seg_off[s0 + spp]) from src to their segments in dst, a staged tile at a
+// time.
+template <typename KeyT>
+__global__ void __launch_bounds__(P3_WG) k_p3_split(const KeyT* __restrict__ src, const uint16_t* __restrict__ src_obs,
+                                                    const uint64_t* __restrict__ seg_off, int sb, int pb, KeyT* __restrict__ dst,
+                                                    uint16_t* __restrict__ dst_obs) {
+  constexpr int TILE = StageCfg<KeyT>::TILE;
+  constexpr int MAXS = 1 << (P3_MAX_SB - P3_PART_BITS);
+  __shared__ KeyT tk[TILE];
+  __shared__ uint16_t tm[TILE];
+  __shared__ uint8_t tp[TILE];
+  __shared__ uint16_t rk[TILE];
+  __shared__ uint16_t inv[TILE];
+  __shared__ uint32_t cnt[MAXS], start[MAXS];
+  __shared__ uint64_t base[MAXS];
+  const int spp = 1 << (sb - pb);  // <= 64: the split runs only when sb > pb = P3_PART_BITS
+  const uint64_t s0 = (uint64_t)blockIdx.x * spp;  // first segment of the part (bin-major)
+  for (int t = threadIdx.x; t < spp; t += P3_WG) {
+    cnt[t] = 0;
+    base[t] = seg_off[s0 + t];
+  }
   __syncthreads();
-  uint64_t* cur = cursor + (uint64_t)bin * ns;
-  for (int s = threadIdx.x; s < ns; s += P3_WG)
-    if (hist[s]) gb[s] = atomicAdd((unsigned long long*)&cur[s], (unsigned long long)hist[s]);
-  __syncthreads();
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_occ<KeyT, RW, true>(rec, r, K, [&](const KeyT& c, uint32_t m) {
-      uint32_t s = seg_sub(c, sb);
-      uint64_t at = gb[s] + (atomicSub(&hist[s], 1u) - 1u);
-      occ[at] = c;
-      occ_obs[at] = (uint16_t)m;
-    });
+  const uint64_t a = seg_off[s0], b = seg_off[s0 + spp];
+  const uint32_t mask = (uint32_t)spp - 1;
+  for (uint64_t t0 = a; t0 < b; t0 += TILE) {
+    const uint32_t tn = (uint32_t)(b - t0 < (uint64_t)TILE ? b - t0 : (uint64_t)TILE);
+    for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
+      const KeyT key = src[t0 + i];
+      tk[i] = key;
+      tm[i] = src_obs[t0 + i];
+      tp[i] = (uint8_t)(seg_sub(key, sb) & mask);
+    }
+    __syncthreads();
+    stage_sort(tn, tp, rk, inv, cnt, start, spp);
+    for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
+      const uint32_t e = inv[i], t = tp[e];
+      const uint64_t at = base[t] + (i - start[t]);
+      dst[at] = tk[e];
+      dst_obs[at] = tm[e];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < spp; t += P3_WG) {
+      base[t] += cnt[t];
+      cnt[t] = 0;
+    }
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------ segment dedup
@@ -280,53 +416,66 @@ __device__ __forceinline__ uint64_t pad_slot_off(uint64_t seg_start, uint64_t se
 __device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_occ / 64 + PAD_BLOCKS; }
 
 // ---- BBHash levels over keys held in registers (see device.hpp), key slot s
-// of thread t being element s*RK_WG + t. Returns the region size in bits;
-// MODE 1 also writes the level words into `region` and each element's bit
-// position into ppos[element] (LDS).
-template <int MODE, typename KeyT, int KPT>
-__device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seen, uint64_t* col, uint64_t* region,
-                               uint64_t max_bits, uint32_t* ppos, uint64_t* sh8, unsigned int* err) {
-  uint64_t off = 0, t = D, r = D;
+// of thread t being element s*RK_WG + t. Writes the level words into
+// `region` and each element's bit position into ppos[element] (LDS); returns
+// the region size in bits. The seen / collision bitmaps are double-buffered
+// (A for even levels, B for odd ones), so a level takes two barriers: after
+// marking, and after placing (which also clears the next level's buffers and
+// tells whether any key is left). seenA / colA must be clear for level 0.
+template <typename KeyT, int KPT>
+__device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
+                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, unsigned int* err) {
+  uint64_t off = 0, t = D;
   int l = 0;
-  while (r > 0) {
+  uint64_t* seen = seenA;
+  uint64_t* col = colA;
+  uint64_t* nseen = seenB;
+  uint64_t* ncol = colB;
+  bool any = D > 0;
+  while (any) {
     const uint64_t n = bbh_level_n(t);
     if (l >= BBH_MAX_LEVELS || off + n > max_bits) {
       if (threadIdx.x == 0) atomicOr(err, 1u);
       break;
     }
-    const uint32_t nw = (uint32_t)(n >> 6);
-    for (uint32_t i = threadIdx.x; i < nw; i += RK_WG) {
-      seen[i] = 0;
-      col[i] = 0;
-    }
-    __syncthreads();
     const uint64_t seed = bbh_seed(l);
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
-        uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
-        unsigned long long bit = 1ULL << (pos & 63);
-        unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
+        const uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
+        const unsigned long long bit = 1ULL << (pos & 63);
+        const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
         if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
       }
     }
     __syncthreads();
-    if (MODE)
-      for (uint32_t i = threadIdx.x; i < nw; i += RK_WG) region[(off >> 6) + i] = seen[i] & ~col[i];
-    uint64_t cnt = 0;
+    const uint32_t nw = (uint32_t)(n >> 6);
+    for (uint32_t i = threadIdx.x; i < nw; i += RK_WG) region[(off >> 6) + i] = seen[i] & ~col[i];
+    const uint32_t nw2 = (uint32_t)(bbh_level_n(bbh_next_t(t)) >> 6);
+    for (uint32_t i = threadIdx.x; i < nw2; i += RK_WG) {
+      nseen[i] = 0;
+      ncol[i] = 0;
+    }
+    int left = 0;
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
-        uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
+        const uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
         if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
           alive &= ~(1u << s);
-          if (MODE) ppos[s * RK_WG + threadIdx.x] = (uint32_t)(off + pos);
+          ppos[s * RK_WG + threadIdx.x] = (uint32_t)(off + pos);
         } else {
-          cnt++;
+          left = 1;
         }
       }
     }
-    block_excl_scan_u64(cnt, sh8, &r);  // r = keys still unplaced (barriers inside)
+    any = __syncthreads_or(left) != 0;
+    uint64_t* ts = seen;
+    seen = nseen;
+    nseen = ts;
+    ts = col;
+    col = ncol;
+    ncol = ts;
     off += n;
     t = bbh_next_t(t);
     l++;
@@ -356,8 +505,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   __shared__ KeyT buf[CAP];
   __shared__ uint16_t mbuf[CAP];
   __shared__ uint32_t cnt[BK_N];
-  __shared__ uint64_t seen[SEEN_W];
-  __shared__ uint64_t col[SEEN_W];
+  __shared__ uint64_t seenA[SEEN_W], colA[SEEN_W], seenB[SEEN_W], colB[SEEN_W];
   __shared__ uint64_t sh8[8];
   unsigned int* n_over = flags;
   unsigned int* n_big = flags + 1;
@@ -366,7 +514,8 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   if (seg >= n_segs) return;
   const uint64_t start = seg_off[seg];
   const uint64_t n = seg_off[seg + 1] - start;
-  uint64_t u = 0;  // distinct keys written to occ[start...]
+  uint64_t u = 0;          // distinct keys written to occ[start...]
+  bool from_lds = false;   // the distinct keys / masks are also compacted in buf / mbuf
   if (n == 0) {
     if (threadIdx.x == 0) {
       ucount[seg] = 0;
@@ -442,9 +591,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       loc += v;
       mx = v > mx ? v : mx;
     }
-    uint64_t dummy = 0;
-    block_or2(mx, dummy, sh8);  // OR of maxima >= max: a conservative "some bucket is large"
-    if (mx <= BK_MAX) {
+    if (!__syncthreads_or(mx > BK_MAX)) {
       uint64_t run = block_excl_scan_u64(loc, sh8, nullptr);
       for (int q = 0; q < CPT; q++) {
         uint32_t v = cnt[threadIdx.x * CPT + q];
@@ -537,26 +684,39 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
         __syncthreads();
       }
     }
-    // buf[0..n) sorted; a run's head joins the masks of its run.
+    // buf[0..n) sorted; a run's head joins the masks of its run. Heads are
+    // read into registers, then written back compacted in place (and to
+    // occ / obs) once every thread has read (the scan's barriers).
     const uint64_t e0 = (uint64_t)threadIdx.x * kpt;
-    uint64_t c = 0;
-    for (int q = 0; q < kpt; q++) {
-      uint64_t e = e0 + q;
-      if (e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) c++;
+    KeyT hk[KPT];
+    uint32_t hm[KPT];
+    uint32_t hbits = 0, c = 0;
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const uint64_t e = e0 + q;
+      if (q < kpt && e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) {
+        uint32_t m = mbuf[e];
+        for (uint64_t f = e + 1; f < n && key_eq(buf[f], buf[e]); f++) m |= mbuf[f];
+        hk[q] = buf[e];
+        hm[q] = m;
+        hbits |= 1u << q;
+        c++;
+      }
     }
     uint64_t tot;
     uint64_t o = block_excl_scan_u64(c, sh8, &tot);
-    for (int q = 0; q < kpt; q++) {
-      uint64_t e = e0 + q;
-      if (e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) {
-        uint32_t m = mbuf[e];
-        for (uint64_t f = e + 1; f < n && key_eq(buf[f], buf[e]); f++) m |= mbuf[f];
-        occ[start + o] = buf[e];
-        obs[start + o] = (uint16_t)m;
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      if ((hbits >> q) & 1) {
+        buf[o] = hk[q];
+        mbuf[o] = (uint16_t)hm[q];
+        occ[start + o] = hk[q];
+        obs[start + o] = (uint16_t)hm[q];
         o++;
       }
     }
     u = tot;
+    from_lds = true;
   }
   // ---- BBHash of the segment, into its padded block slot; the joined masks
   // also go to vpad[start + local rank] (obs keeps them in key order).
@@ -571,27 +731,37 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   constexpr int REG_W = (4 * CAP + 8192) / 64;
   static_assert(REG_W * 8 <= (int)sizeof(buf) && CAP <= BK_N && REG_W / 7 + 2 <= SEEN_W, "LDS reuse fits");
   __threadfence_block();
-  __syncthreads();  // every read of buf / cnt above is done: they are reused below
-  // LDS reuse: region (level words) in buf, masks in mbuf, bit positions in
-  // cnt, block ranks in seen (free once the levels are done).
+  __syncthreads();  // compacted keys (or, on the table path, occ / obs) are complete
+  // LDS reuse: region (level words) in buf once the keys are in registers,
+  // bit positions in cnt, block ranks in seenA (free once the levels are done).
   uint64_t* region = (uint64_t*)buf;
   uint32_t* ppos = cnt;
-  uint64_t* brank = seen;
+  uint64_t* brank = seenA;
   KeyT key[KPT];
   uint32_t alive = 0;
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
     if (e < u) {
-      key[s] = occ[start + e];
-      mbuf[e] = obs[start + e];
+      if (from_lds) {
+        key[s] = buf[e];
+      } else {
+        key[s] = occ[start + e];
+        mbuf[e] = obs[start + e];
+      }
       alive |= 1u << s;
     }
   }
+  __syncthreads();  // buf becomes the region
   for (int i = threadIdx.x; i < REG_W; i += RK_WG) region[i] = 0;
+  for (int i = threadIdx.x; i < SEEN_W; i += RK_WG) {
+    seenA[i] = 0;
+    colA[i] = 0;
+  }
+  __syncthreads();
   const uint64_t slot = pad_slot_blocks(n);
   const uint64_t max_bits = slot * BBH_BLOCK_BITS < (uint64_t)REG_W * 64 ? slot * BBH_BLOCK_BITS : (uint64_t)REG_W * 64;
-  const uint64_t bits = bbh_levels<1, KeyT, KPT>(key, alive, u, seen, col, region, max_bits, ppos, sh8, err);
+  const uint64_t bits = bbh_levels<KeyT, KPT>(key, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, err);
   const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < RK_WG
   __syncthreads();
   const uint64_t j = threadIdx.x;
@@ -926,26 +1096,41 @@ static inline unsigned grid_for(uint64_t n, unsigned wg, unsigned cap) {
   return (unsigned)g;
 }
 
+int p3_part_bits(int sb) { return sb < P3_PART_BITS ? sb : P3_PART_BITS; }
 void launch_p3_count(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
-                     int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* counts) {
+                     int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* counts, uint64_t* mat) {
   if (n_chunks == 0) return;
+  const int pb = p3_part_bits(sb);
   if (!wide)
     CF1G_LAUNCH("p3_count", (k_p3_count<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start, chunk_off,
-                num_bins, chunk_records, K, sb, counts);
+                num_bins, chunk_records, K, sb, pb, counts, mat);
   else
     CF1G_LAUNCH("p3_count", (k_p3_count<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start, chunk_off,
-                num_bins, chunk_records, K, sb, counts);
+                num_bins, chunk_records, K, sb, pb, counts, mat);
 }
 void launch_p3_scatter(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
-                       int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* cursor, void* occ,
+                       int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, const uint64_t* moff, void* occ,
                        uint16_t* occ_obs) {
   if (n_chunks == 0) return;
+  const int pb = p3_part_bits(sb);
   if (!wide)
     CF1G_LAUNCH("p3_scatter", (k_p3_scatter<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, cursor, (uint64_t*)occ, occ_obs);
+                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (uint64_t*)occ, occ_obs);
   else
     CF1G_LAUNCH("p3_scatter", (k_p3_scatter<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, cursor, (Key128*)occ, occ_obs);
+                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (Key128*)occ, occ_obs);
+}
+void launch_p3_split(hipStream_t st, bool wide, const void* src, const uint16_t* src_obs, const uint64_t* seg_off, uint32_t num_bins,
+                     int sb, void* dst, uint16_t* dst_obs) {
+  const int pb = p3_part_bits(sb);
+  const uint64_t parts = (uint64_t)num_bins << pb;
+  if (parts == 0) return;
+  if (!wide)
+    CF1G_LAUNCH("p3_split", k_p3_split<uint64_t>, dim3((unsigned)parts), dim3(P3_WG), 0, st, (const uint64_t*)src, src_obs, seg_off,
+                sb, pb, (uint64_t*)dst, dst_obs);
+  else
+    CF1G_LAUNCH("p3_split", k_p3_split<Key128>, dim3((unsigned)parts), dim3(P3_WG), 0, st, (const Key128*)src, src_obs, seg_off, sb,
+                pb, (Key128*)dst, dst_obs);
 }
 void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
                      uint64_t* ucount, uint64_t* nblk, uint32_t* over_list, unsigned int* flags, uint32_t* big_list,
