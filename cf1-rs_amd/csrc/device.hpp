@@ -113,22 +113,36 @@ __device__ __forceinline__ uint64_t xxh3_u64le(uint64_t v) {
 }
 __device__ __forceinline__ uint32_t bucket_hash(uint64_t v) { return (uint32_t)(xxh3_u64le(v) >> 48); }
 
-// BBHash level hash (splitmix64 finalizer; per-level seeds from a Weyl sequence).
-__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+// splitmix64 finalizer; seeds from a Weyl sequence.
+__host__ __device__ constexpr uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
-__host__ __device__ __forceinline__ uint64_t bbh_seed(int level) {
-  return mix64(0x243F6A8885A308D3ULL + (uint64_t)(level + 1) * 0x9E3779B97F4A7C15ULL);
-}
-__device__ __forceinline__ uint64_t bbh_hash(uint64_t k, uint64_t seed) { return mix64(k ^ seed); }
-__device__ __forceinline__ uint64_t bbh_hash(const Key128& k, uint64_t seed) { return mix64(k.lo ^ mix64(k.hi ^ seed)); }
-// Segment of a key inside its bin: the top sb bits of the level hash with
-// seed bbh_seed(-1) (spreads each bin evenly over its 2^sb segments).
+__host__ __device__ constexpr uint64_t bbh_seed(int i) { return mix64(0x243F6A8885A308D3ULL + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL); }
+constexpr uint64_t BBH_S1 = bbh_seed(0), BBH_S2 = bbh_seed(1);
+constexpr uint64_t FIB_MUL = 0x9E3779B97F4A7C15ULL, WIDE_MUL = 0xC2B2AE3D27D4EB4FULL;
+
+// Segment of a key inside its bin: the top sb bits of a multiplicative
+// (Fibonacci) hash, which spreads each bin evenly over its 2^sb segments.
+__device__ __forceinline__ uint64_t seg_hash(uint64_t k) { return k * FIB_MUL; }
+__device__ __forceinline__ uint64_t seg_hash(const Key128& k) { return (k.lo ^ (k.hi * WIDE_MUL)) * FIB_MUL; }
 template <typename KeyT>
 __device__ __forceinline__ uint32_t seg_sub(const KeyT& k, int sb) {
-  return sb ? (uint32_t)(bbh_hash(k, bbh_seed(-1)) >> (64 - sb)) : 0u;
+  return sb ? (uint32_t)(seg_hash(k) >> (64 - sb)) : 0u;
+}
+// BBHash level hashes: one 64-bit state per key and level, g_l = h1 + l * d
+// (mod 2^64), h1 = mix64 of the key with seed S1, d = mix64(h1 ^ S2) | 1.
+// The position among n < 2^32 bits is floor(x * n / 2^32) with
+// x = fmix32(hi32(g_l) ^ lo32(g_l)); the fmix32 step keeps the levels
+// independent (top bits of g_l alone would keep two keys with close h1 and
+// close d on one bit at every level).
+__device__ __forceinline__ uint64_t bbh_h1(uint64_t k) { return mix64(k ^ BBH_S1); }
+__device__ __forceinline__ uint64_t bbh_h1(const Key128& k) { return mix64(k.lo ^ mix64(k.hi ^ BBH_S1)); }
+__device__ __forceinline__ uint64_t bbh_d(uint64_t h1) { return mix64(h1 ^ BBH_S2) | 1ULL; }
+__device__ __forceinline__ uint32_t bbh_pos(uint64_t g, uint32_t n) {
+  const uint32_t x = fmix32((uint32_t)(g >> 32) ^ (uint32_t)g);
+  return __umulhi(x, n);
 }
 
 constexpr int BBH_MAX_LEVELS = 64;
@@ -139,7 +153,7 @@ constexpr uint64_t BBH_BLOCK_BITS = 448;  // 7 words of bits + 1 rank word = one
 // (bin, seg_sub(key)), bin = bucket_to_bin[bucket_hash(minimizer value)].
 // Each segment holds its own BBHash: level l has n_l = roundup64(max(2 t_l, 1))
 // bits with t_0 = D_s (the segment's key count) and t_{l+1} = ceil(9 t_l / 20);
-// a key lands at pos = mulhi64(hash(key, l), n_l) and is placed at the first
+// a key lands at pos = bbh_pos(g_l(key), n_l) and is placed at the first
 // level where no other remaining key lands on the same bit. Levels are
 // concatenated bit-contiguously in 64-byte blocks {u64 rank_before, u64 bits[7]}
 // (rank local to the segment). value = key_off[segment] + local rank.
@@ -197,9 +211,11 @@ __device__ __forceinline__ uint64_t mphf_segment(const MphfView& v, const KeyT& 
 template <typename KeyT>
 __device__ __forceinline__ uint64_t mphf_local_rank(const uint64_t* __restrict__ base, uint64_t t, const KeyT& c) {
   uint64_t off = 0;
-  for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
+  uint64_t h = bbh_h1(c);
+  const uint64_t d = bbh_d(h);
+  for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++, h += d) {
     const uint64_t n = bbh_level_n(t);
-    const uint64_t pos = off + __umul64hi(bbh_hash(c, bbh_seed(l)), n);
+    const uint64_t pos = off + bbh_pos(h, (uint32_t)n);
     const uint64_t q = pos / BBH_BLOCK_BITS;
     const uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
     const uint64_t* B = base + q * 8;
@@ -224,9 +240,11 @@ __device__ __forceinline__ uint64_t mphf_lookup(const MphfView& v, const KeyT& c
   uint64_t t = v.key_off[seg + 1] - ko;
   const uint64_t* base = v.blocks + v.blk_off[seg] * 8;
   uint64_t off = 0;
-  for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
+  uint64_t h = bbh_h1(c);
+  const uint64_t d = bbh_d(h);
+  for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++, h += d) {
     const uint64_t n = bbh_level_n(t);
-    const uint64_t pos = off + __umul64hi(bbh_hash(c, bbh_seed(l)), n);
+    const uint64_t pos = off + bbh_pos(h, (uint32_t)n);
     const uint64_t q = pos / BBH_BLOCK_BITS;
     const uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
     const uint64_t* B = base + q * 8;

@@ -18,7 +18,17 @@ inline uint64_t h_mix64(uint64_t z) {
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
-inline uint64_t h_seed(int level) { return h_mix64(0x243F6A8885A308D3ULL + (uint64_t)(level + 1) * 0x9E3779B97F4A7C15ULL); }
+inline uint64_t h_seed(int i) { return h_mix64(0x243F6A8885A308D3ULL + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL); }
+// Segment and level hashes of the device (device.hpp): Fibonacci hash for
+// the segment; state h1 + l * d folded through fmix32 for BBHash level l.
+inline uint64_t h_seg_hash(uint64_t lo, uint64_t hi, bool wide) {
+  return (lo ^ (wide ? hi * 0xC2B2AE3D27D4EB4FULL : 0)) * 0x9E3779B97F4A7C15ULL;
+}
+inline uint64_t h_bbh_h1(uint64_t lo, uint64_t hi, bool wide) {
+  const uint64_t s = h_seed(0);
+  return wide ? h_mix64(lo ^ h_mix64(hi ^ s)) : h_mix64(lo ^ s);
+}
+inline uint64_t h_bbh_d(uint64_t h1) { return h_mix64(h1 ^ h_seed(1)) | 1ULL; }
 
 inline uint32_t h_fmix32(uint32_t h) {
   h ^= h >> 16;
@@ -75,8 +85,7 @@ struct HostMphf {
       }
     }
     uint32_t bin = b2b[h_xxh3_u64le((uint64_t)bv) >> 48];
-    uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64), s = h_seed(-1);
-    uint64_t h = wide ? h_mix64(lo ^ h_mix64(hi ^ s)) : h_mix64(lo ^ s);
+    const uint64_t h = h_seg_hash((uint64_t)c, (uint64_t)(c >> 64), wide);
     uint32_t sub = sb ? (uint32_t)(h >> (64 - sb)) : 0;
     return ((uint64_t)bin << sb) | sub;
   }
@@ -86,11 +95,11 @@ struct HostMphf {
     const uint64_t* base = blocks + blk_off[seg] * 8;
     uint64_t off = 0;
     uint64_t lo = (uint64_t)c, hi = (uint64_t)(c >> 64);
-    for (int l = 0; l < 64 && t > 0; l++) {
+    uint64_t h = h_bbh_h1(lo, hi, wide);
+    const uint64_t d = h_bbh_d(h);
+    for (int l = 0; l < 64 && t > 0; l++, h += d) {
       uint64_t n = (2 * t + 63) & ~63ULL;
-      uint64_t s = h_seed(l);
-      uint64_t h = wide ? h_mix64(lo ^ h_mix64(hi ^ s)) : h_mix64(lo ^ s);
-      uint64_t pos = off + (uint64_t)(((u128)h * n) >> 64);
+      uint64_t pos = off + (((uint64_t)h_fmix32((uint32_t)(h >> 32) ^ (uint32_t)h) * n) >> 32);
       uint64_t q = pos / 448, b = pos % 448;
       const uint64_t* B = base + q * 8;
       uint64_t wd = B[1 + b / 64];

@@ -425,6 +425,16 @@ __device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_o
 template <typename KeyT, int KPT>
 __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
                                uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, unsigned int* err) {
+  uint64_t h[KPT], d[KPT];  // level state h1 + l * d of each slot (device.hpp)
+#pragma unroll
+  for (int s = 0; s < KPT; s++) {
+    if ((alive >> s) & 1) {
+      h[s] = bbh_h1(key[s]);
+      d[s] = bbh_d(h[s]);
+    } else {
+      h[s] = d[s] = 0;
+    }
+  }
   uint64_t off = 0, t = D;
   int l = 0;
   uint64_t* seen = seenA;
@@ -438,11 +448,12 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
       if (threadIdx.x == 0) atomicOr(err, 1u);
       break;
     }
-    const uint64_t seed = bbh_seed(l);
+    uint32_t pp[KPT];
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
-        const uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
+        const uint32_t pos = bbh_pos(h[s], (uint32_t)n);
+        pp[s] = pos;
         const unsigned long long bit = 1ULL << (pos & 63);
         const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
         if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
@@ -460,7 +471,7 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
-        const uint64_t pos = __umul64hi(bbh_hash(key[s], seed), n);
+        const uint32_t pos = pp[s];
         if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
           alive &= ~(1u << s);
           ppos[s * RK_WG + threadIdx.x] = (uint32_t)(off + pos);
@@ -470,6 +481,8 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
       }
     }
     any = __syncthreads_or(left) != 0;
+#pragma unroll
+    for (int s = 0; s < KPT; s++) h[s] += d[s];
     uint64_t* ts = seen;
     seen = nseen;
     nseen = ts;
@@ -1018,7 +1031,7 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
   int l = 0;
   while (r > 0) {
     const uint64_t n = bbh_level_n(t);
-    if (l >= BBH_MAX_LEVELS || (off + n + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS > slot) {
+    if (l >= BBH_MAX_LEVELS || (n >> 32) != 0 || (off + n + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS > slot) {
       if (threadIdx.x == 0) atomicOr(err, 2u);
       break;
     }
@@ -1029,11 +1042,11 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
     }
     if constexpr (!LDSB) __threadfence();
     __syncthreads();
-    const uint64_t seed = bbh_seed(l);
     for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
       const uint64_t gi = abase + e;
       if (!((ld(&alive[gi >> 6]) >> (gi & 63)) & 1)) continue;
-      uint64_t pos = __umul64hi(bbh_hash(keys[e], seed), n);
+      const uint64_t h1 = bbh_h1(keys[e]);
+      const uint64_t pos = bbh_pos(h1 + (uint64_t)l * bbh_d(h1), (uint32_t)n);
       unsigned long long bit = 1ULL << (pos & 63);
       unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
       if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
@@ -1048,7 +1061,8 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
     for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
       const uint64_t gi = abase + e;
       if (!((ld(&alive[gi >> 6]) >> (gi & 63)) & 1)) continue;
-      uint64_t pos = __umul64hi(bbh_hash(keys[e], seed), n);
+      const uint64_t h1 = bbh_h1(keys[e]);
+      const uint64_t pos = bbh_pos(h1 + (uint64_t)l * bbh_d(h1), (uint32_t)n);
       if (((ld(&seen[pos >> 6]) & ~ld(&col[pos >> 6])) >> (pos & 63)) & 1)
         atomicAnd((unsigned long long*)&alive[gi >> 6], ~(1ULL << (gi & 63)));
       else

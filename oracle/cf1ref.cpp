@@ -43,11 +43,12 @@ static void build_segment(const uint64_t* lo, const uint64_t* hi, uint64_t D, bo
   int l = 0;
   while (r > 0) {
     if (l >= BBH_MAX_LEVELS) throw std::runtime_error("MPHF: too many levels");
+    if (t >= (1ULL << 31)) throw std::runtime_error("MPHF: segment too large");
     uint64_t n = bbh_level_bits(t), nw = n / 64;
     std::vector<uint64_t> seen(nw, 0), col(nw, 0);
     for (uint64_t i = 0; i < D; i++) {
       if (!alive[i]) continue;
-      uint64_t pos = mulhi64(bbh_hash(lo[i], wide ? hi[i] : 0, wide, l), n);
+      uint64_t pos = bbh_pos(lo[i], wide ? hi[i] : 0, wide, l, n);
       uint64_t bit = 1ULL << (pos % 64);
       if (seen[pos / 64] & bit) col[pos / 64] |= bit;
       seen[pos / 64] |= bit;
@@ -56,7 +57,7 @@ static void build_segment(const uint64_t* lo, const uint64_t* hi, uint64_t D, bo
     uint64_t left = 0;
     for (uint64_t i = 0; i < D; i++) {
       if (!alive[i]) continue;
-      uint64_t pos = mulhi64(bbh_hash(lo[i], wide ? hi[i] : 0, wide, l), n);
+      uint64_t pos = bbh_pos(lo[i], wide ? hi[i] : 0, wide, l, n);
       if (((seen[pos / 64] & ~col[pos / 64]) >> (pos % 64)) & 1) alive[i] = 0;
       else left++;
     }

@@ -435,16 +435,16 @@ inline uint32_t mmer_order(uint32_t canon) { return fmix32(canon + 0x9E3779B9u);
 // invariant to the MPHF once unitig ids are canonicalised) is a
 // minimizer-partitioned BBHash:
 //   segment of key c = (bin, s): bin = bucket_to_bin[bucket_hash(v(c))] with
-//   v(c) the minimizer value of c; s = top sb bits of seg_hash(c) (the level
-//   hash below with seed bbh_seed(-1)), which spreads every bin evenly over
-//   its 2^sb segments. The MPHF key order is (bin, segment, key): each bin's
+//   v(c) the minimizer value of c; s = top sb bits of seg_hash(c) =
+//   (lo ^ hi * 0xC2B2AE3D27D4EB4F) * 0x9E3779B97F4A7C15 (a Fibonacci hash),
+//   which spreads every bin evenly over its 2^sb segments. The MPHF key order is (bin, segment, key): each bin's
 //   sorted distinct keys, stably regrouped by segment.
 //   sb = fewest bits (<= 12) with W <= bins * 2^sb * target
 //   (target 1536; W = valid k-mer occurrences).
 //   per segment with D_s keys, level l has n_l = roundup64(max(2 t_l, 1)) bits,
-//   t_0 = D_s, t_{l+1} = ceil(9 t_l / 20); a key lands on
-//   off_l + mulhi64(hash(key, l), n_l) and is placed at the first level where
-//   no other remaining key of the segment lands on that bit;
+//   t_0 = D_s, t_{l+1} = ceil(9 t_l / 20); a key lands on off_l + bbh_pos(key,
+//   l, n_l) (below) and is placed at the first level where no other remaining
+//   key of the segment lands on that bit;
 //   levels are bit-contiguous in 64-byte blocks {u64 rank_before, u64 bits[7]}
 //   (ranks local to the segment); value = key_off[segment] + local rank.
 constexpr int BBH_MAX_LEVELS = 64;
@@ -454,13 +454,32 @@ inline uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
-inline uint64_t bbh_seed(int level) { return mix64(0x243F6A8885A308D3ULL + (uint64_t)(level + 1) * 0x9E3779B97F4A7C15ULL); }
-inline uint64_t bbh_hash(uint64_t lo, uint64_t hi, bool wide, int level) {
-  uint64_t s = bbh_seed(level);
+inline uint64_t bbh_seed(int i) { return mix64(0x243F6A8885A308D3ULL + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL); }
+// Level hashes of a key come from one 64-bit state per key and level,
+// g_l = h1 + l * d (mod 2^64), with h1 = mix64 of the key with seed
+// bbh_seed(0) and d = mix64(h1 ^ bbh_seed(1)) | 1. The level position among
+// n bits (n < 2^32) is floor(x * n / 2^32) with x = fmix32(hi32(g_l) ^
+// lo32(g_l)). The fmix32 step is what keeps levels independent: reading the
+// position straight from the top bits of g_l would keep two keys whose h1 and
+// d are both close together on one bit at every level.
+inline uint64_t bbh_h1(uint64_t lo, uint64_t hi, bool wide) {
+  uint64_t s = bbh_seed(0);
   if (!wide) return mix64(lo ^ s);
   return mix64(lo ^ mix64(hi ^ s));
 }
-inline uint64_t mulhi64(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) >> 64); }
+inline uint64_t bbh_d(uint64_t h1) { return mix64(h1 ^ bbh_seed(1)) | 1ULL; }
+inline uint64_t bbh_pos_g(uint64_t g, uint64_t n) {
+  uint32_t x = fmix32((uint32_t)(g >> 32) ^ (uint32_t)g);
+  return ((uint64_t)x * n) >> 32;
+}
+inline uint64_t bbh_pos(uint64_t lo, uint64_t hi, bool wide, int level, uint64_t n) {
+  uint64_t h1 = bbh_h1(lo, hi, wide);
+  return bbh_pos_g(h1 + (uint64_t)level * bbh_d(h1), n);
+}
+// Segment hash: multiplicative (Fibonacci) hash of the key.
+inline uint64_t seg_hash(uint64_t lo, uint64_t hi, bool wide) {
+  return (lo ^ (wide ? hi * 0xC2B2AE3D27D4EB4FULL : 0)) * 0x9E3779B97F4A7C15ULL;
+}
 inline uint64_t bbh_level_bits(uint64_t t) { return (std::max<uint64_t>(2 * t, 1) + 63) / 64 * 64; }
 inline uint64_t bbh_next_t(uint64_t t) { return (9 * t + 19) / 20; }
 
@@ -496,7 +515,7 @@ inline int segment_bits(uint64_t W, uint64_t bins, uint64_t target) {
   while (sb < 12 && W > bins * (1ULL << sb) * target) sb++;
   return sb;
 }
-inline uint64_t seg_sub(uint64_t lo, uint64_t hi, bool wide, int sb) { return sb ? bbh_hash(lo, hi, wide, -1) >> (64 - sb) : 0; }
+inline uint64_t seg_sub(uint64_t lo, uint64_t hi, bool wide, int sb) { return sb ? seg_hash(lo, hi, wide) >> (64 - sb) : 0; }
 
 struct Mphf {
   int K = 0, m = 0, sb = 0;
@@ -517,7 +536,7 @@ struct Mphf {
     uint64_t off = 0;
     for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
       uint64_t n = bbh_level_bits(t);
-      uint64_t pos = off + mulhi64(bbh_hash(lo, hi, wide, l), n);
+      uint64_t pos = off + bbh_pos(lo, hi, wide, l, n);
       const uint64_t* B = base + (pos / BBH_BLOCK_BITS) * 8;
       uint64_t b = pos % BBH_BLOCK_BITS;
       uint64_t wd = B[1 + b / 64];
