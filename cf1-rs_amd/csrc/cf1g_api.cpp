@@ -42,11 +42,11 @@ void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, cons
                        const uint64_t*, void*, uint16_t*);
 void launch_p3_split(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, uint32_t, int, void*, uint16_t*);
 void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint64_t*, uint32_t*,
-                     unsigned int*, uint32_t*, uint64_t*, uint16_t*);
+                     unsigned int*, uint32_t*, uint64_t*, void*, uint16_t*);
 void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
                        unsigned int*, uint32_t*);
 void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
-                    uint32_t, uint64_t*, uint64_t*, uint64_t*, uint64_t*, unsigned int*);
+                    uint32_t, uint64_t*, uint64_t*, uint64_t*, uint64_t*, void*, unsigned int*);
 void launch_p3_compact(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*,
                        const uint64_t*, const uint64_t*, uint64_t, int, int, void*, uint16_t*, uint64_t*);
 void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
@@ -65,8 +65,7 @@ void launch_lidx_leaf_final(hipStream_t, const uint64_t*, uint64_t, uint32_t, co
 void launch_lidx_fingerprints(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, uint16_t*, uint32_t*);
 void launch_lidx_bbh2rank(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, const MphfView&, uint32_t*);
 void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*);
-void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, uint64_t,
-                         const LidxView&, const MphfView&, uint8_t*, unsigned int*);
+void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, const LidxView&, const MphfView&, uint8_t*, unsigned int*);
 void launch_states_pack(hipStream_t, const uint8_t*, uint64_t, uint64_t*);
 void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&,
                           const uint64_t*, uint8_t*);
@@ -369,9 +368,12 @@ void do_p3(cf1g_ctx* c) {
   }
   // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
   uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
-  // joined masks in MPHF-rank order per segment (occ_obs keeps key order)
+  // distinct keys and joined masks in local-rank order per segment (occ_aux is
+  // free once P3 pass B has run; k_p3_bigsort's use of it is limited to the
+  // oversized segments, which k_p3_dedup skips)
+  void* kout = c->get("occ_aux", c->W * kb + 16);
   uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
-  launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, vpad);
+  launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, kout, vpad);
   unsigned int hf[3];
   c->d2h(hf, flags, 12);
   c->n_over = hf[0];
@@ -390,7 +392,7 @@ void do_p3(cf1g_ctx* c) {
     const uint64_t alive_words = c->W / 64 + 2;
     uint64_t* alive = c->getT<uint64_t>("bbh_alive", alive_words);
     hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
-    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, flags + 2);
+    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, kout, flags + 2);
   }
   scan_excl_u64(st, ucount, key_off, ns, key_off + ns, tmp);
   scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
@@ -404,7 +406,7 @@ void do_p3(cf1g_ctx* c) {
   void* keys = c->get("keys", D * kb + 16);
   uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
-  launch_p3_compact(st, c->wide, occ, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, keys, vmask, blocks);
+  launch_p3_compact(st, c->wide, kout, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, keys, vmask, blocks);
   c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
   c->d2h(hf, flags, 12);
   if (hf[2]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit (flags " + std::to_string(hf[2]) + ")");
@@ -972,9 +974,8 @@ int cf1g_classify_via(cf1g_ctx* c, int index) {
     LidxView v{};
     if (index == 1) v = c->lview();
     v.D = D;
-    launch_classify_via(c->st, index == 1, c->getT<uint64_t>("keys", D + 2), c->getT<uint16_t>("occ_obs", c->W + 8),
-                        c->getT<uint64_t>("seg_off", c->n_segs + 1), c->getT<uint64_t>("seg_out_off", c->n_segs + 1), c->n_segs, v,
-                        c->view(), st8, err);
+    launch_classify_via(c->st, index == 1, c->getT<uint64_t>("keys", D + 2), c->getT<uint16_t>("vmask", 12 * nsw + 16), v, c->view(),
+                        st8, err);
     launch_states_pack(c->st, st8, D, c->getT<uint64_t>(index == 1 ? "states_lidx" : "states", nsw + 1));
     unsigned int he = 0;
     c->d2h(&he, err, 4);

@@ -527,19 +527,16 @@ __global__ void __launch_bounds__(256) k_count_mismatch(const uint64_t* __restri
 }
 
 // ------------------------------------------------------------------ classification through an index
-// Every distinct k-mer (P3 key order, its joined mask in key order) is looked
-// up through the index and its 5-bit state stored as a byte at that slot;
-// k_states_pack packs the bytes 12 per word.
+// Every distinct k-mer (P3 keys with their joined masks, both in MPHF value
+// order) is looked up through the index and its 5-bit state stored as a byte
+// at that slot; k_states_pack packs the bytes 12 per word.
 template <bool LEARNED>
-__global__ void __launch_bounds__(256) k_classify_via(const uint64_t* __restrict__ keys, const uint16_t* __restrict__ obs,
-                                                      const uint64_t* __restrict__ seg_off, const uint64_t* __restrict__ key_off,
-                                                      uint64_t n_segs, const LidxView V, const MphfView mv, uint8_t* __restrict__ st8,
+__global__ void __launch_bounds__(256) k_classify_via(const uint64_t* __restrict__ keys, const uint16_t* __restrict__ vmask,
+                                                      const LidxView V, const MphfView mv, uint8_t* __restrict__ st8,
                                                       unsigned int* __restrict__ err) {
-  const uint64_t sg = blockIdx.x;
-  if (sg >= n_segs) return;
-  const uint64_t ko = key_off[sg], u = key_off[sg + 1] - ko, s0 = seg_off[sg];
-  for (uint64_t e = threadIdx.x; e < u; e += 256) {
-    const uint64_t k = keys[ko + e];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V.D; v += stride) {
+    const uint64_t k = keys[v];
     uint64_t idx;
     if (LEARNED) idx = lidx_rank(V, mv, k);
     else idx = mphf_lookup<uint64_t>(mv, k);
@@ -547,7 +544,7 @@ __global__ void __launch_bounds__(256) k_classify_via(const uint64_t* __restrict
       atomicOr(err, 1u);
       continue;
     }
-    st8[idx] = (uint8_t)mask_state(obs[s0 + e]);
+    st8[idx] = (uint8_t)mask_state(vmask[v]);
   }
 }
 __global__ void __launch_bounds__(256) k_states_pack(const uint8_t* __restrict__ st8, uint64_t D, uint64_t n_words,
@@ -665,16 +662,15 @@ void launch_count_mismatch(hipStream_t st, const uint64_t* a, const uint64_t* b,
   if (g > 16384) g = 16384;
   CF1G_LAUNCH("count_mismatch", k_count_mismatch, dim3((unsigned)g), dim3(256), 0, st, a, b, n, bad);
 }
-void launch_classify_via(hipStream_t st, bool learned, const uint64_t* keys, const uint16_t* obs, const uint64_t* seg_off,
-                         const uint64_t* key_off, uint64_t n_segs, const LidxView& V, const MphfView& mv, uint8_t* st8,
-                         unsigned int* err) {
-  if (!n_segs) return;
+void launch_classify_via(hipStream_t st, bool learned, const uint64_t* keys, const uint16_t* vmask, const LidxView& V,
+                         const MphfView& mv, uint8_t* st8, unsigned int* err) {
+  if (!V.D) return;
+  uint64_t g = (V.D + 255) / 256;
+  if (g > 65536) g = 65536;
   if (learned)
-    CF1G_LAUNCH("p4_classify_lidx", k_classify_via<true>, dim3((unsigned)n_segs), dim3(256), 0, st, keys, obs, seg_off, key_off, n_segs,
-                V, mv, st8, err);
+    CF1G_LAUNCH("p4_classify_lidx", k_classify_via<true>, dim3((unsigned)g), dim3(256), 0, st, keys, vmask, V, mv, st8, err);
   else
-    CF1G_LAUNCH("p4_classify_bbh", k_classify_via<false>, dim3((unsigned)n_segs), dim3(256), 0, st, keys, obs, seg_off, key_off, n_segs,
-                V, mv, st8, err);
+    CF1G_LAUNCH("p4_classify_bbh", k_classify_via<false>, dim3((unsigned)g), dim3(256), 0, st, keys, vmask, V, mv, st8, err);
 }
 void launch_states_pack(hipStream_t st, const uint8_t* st8, uint64_t D, uint64_t* states) {
   const uint64_t nw = (D + 11) / 12;

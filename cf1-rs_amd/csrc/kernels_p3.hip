@@ -1,18 +1,18 @@
-// P3 kernels: per-bin canonical k-mer extraction, sort and dedup
-// (mphf.rs:323-363 dedup_bin) and the BBHash build (the project's
-// replacement for ph::fmph, mphf.rs:144-190).
+// P3 kernels: per-bin canonical k-mer extraction, dedup (mphf.rs:323-363
+// dedup_bin) and the BBHash build (the project's replacement for ph::fmph,
+// mphf.rs:144-190).
 //
 // P3 layout: the k-mers of bin b are split into 2^sb segments (b, s) by the
-// top sb bits of a 64-bit mix of the canonical key (seg_sub), so segments
-// are evenly sized. Segments are counted (k_p3_count) and scattered into one
-// occurrence array with each occurrence's 10-bit observation mask
-// (k_p3_scatter). k_p3_dedup sorts and deduplicates one segment per
-// work-group (LDS bucket sort, LDS radix sort for uneven buckets, or a
-// direct-address table for keys of <= 12 bits), OR-joins the masks of equal
-// keys and builds the segment's BBHash into a padded block slot; segments
-// above the LDS capacity take a global-memory radix sort (k_p3_bigsort) and
-// k_bbh_big. k_p3_compact then packs keys (in (bin, segment, key) order: the
-// MPHF key order), the masks (to MPHF-value order) and the blocks densely.
+// top sb bits of a multiplicative hash of the canonical key (seg_sub), so
+// segments are evenly sized. Segments are counted (k_p3_count) and scattered
+// into one occurrence array with each occurrence's 10-bit observation mask
+// (k_p3_scatter, k_p3_split). k_p3_dedup deduplicates one segment per
+// work-group (LDS hash table, or a direct-address table for keys of <= 12
+// bits), OR-joins the masks of equal keys and builds the segment's BBHash
+// into a padded block slot, writing keys and masks in MPHF value order;
+// segments above the LDS capacity take a global-memory radix sort
+// (k_p3_bigsort) and k_bbh_big. k_p3_compact then packs keys and masks (both
+// in value order) and the blocks densely.
 #include "block.hpp"
 #include "device.hpp"
 #include "launch.hpp"
@@ -392,9 +392,6 @@ struct SegCfg<Key128> {
   static constexpr int KPT = 8;
 };
 constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12 bits (k <= 5)
-constexpr int BK_BITS = 11;       // bucket-sort buckets (top 11 key bits)
-constexpr int BK_N = 1 << BK_BITS;
-constexpr uint32_t BK_MAX = 24;   // larger buckets (duplicate-heavy) switch to the LDS radix sort
 
 // A segment's MPHF is built by k_bbh_big when its distinct keys exceed the
 // register capacity, or when it was sorted by k_p3_bigsort (more
@@ -418,13 +415,44 @@ __device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_o
 // ---- BBHash levels over keys held in registers (see device.hpp), key slot s
 // of thread t being element s*RK_WG + t. Writes the level words into
 // `region` and each element's bit position into ppos[element] (LDS); returns
-// the region size in bits. The seen / collision bitmaps are double-buffered
-// (A for even levels, B for odd ones), so a level takes two barriers: after
-// marking, and after placing (which also clears the next level's buffers and
-// tells whether any key is left). seenA / colA must be clear for level 0.
+// the region size in bits. While more than TAIL_KEYS keys remain a level is
+// block-wide: the seen / collision bitmaps are double-buffered (A for even
+// levels, B for odd ones), so it takes two barriers, after marking and after
+// placing (which also clears the next level's buffers and counts the keys
+// left). The last TAIL_KEYS or fewer keys are listed in LDS and wave 0
+// finishes their levels alone, one key per lane, without block barriers (the
+// LDS operations of one wave execute in order). seenA / colA must be clear
+// for level 0.
+constexpr int TAIL_KEYS = 64;
+struct BbhTail {
+  uint64_t h[TAIL_KEYS], d[TAIL_KEYS];
+  uint32_t e[TAIL_KEYS];
+  uint32_t left[2][RK_WG / 64];
+  uint64_t off;
+};
+// Sum over the wave of a per-lane count in [0, 15].
+__device__ __forceinline__ uint32_t wave_sum4(uint32_t c) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) s += (uint32_t)__popcll(__ballot((c >> b) & 1)) << b;
+  return s;
+}
+// Exclusive prefix over the lanes below this one of a per-lane count in [0, 15].
+__device__ __forceinline__ uint32_t wave_prefix4(uint32_t c) {
+  const uint64_t below = (1ULL << __lane_id()) - 1;
+  uint32_t s = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) s += (uint32_t)__popcll(__ballot((c >> b) & 1) & below) << b;
+  return s;
+}
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <typename KeyT, int KPT>
 __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
-                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, unsigned int* err) {
+                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhTail* tl, unsigned int* err) {
   uint64_t h[KPT], d[KPT];  // level state h1 + l * d of each slot (device.hpp)
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
@@ -435,18 +463,19 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
       h[s] = d[s] = 0;
     }
   }
+  const uint32_t wv = threadIdx.x >> 6;
   uint64_t off = 0, t = D;
   int l = 0;
   uint64_t* seen = seenA;
   uint64_t* col = colA;
   uint64_t* nseen = seenB;
   uint64_t* ncol = colB;
-  bool any = D > 0;
-  while (any) {
+  uint32_t total = (uint32_t)D;
+  while (total > TAIL_KEYS) {
     const uint64_t n = bbh_level_n(t);
     if (l >= BBH_MAX_LEVELS || off + n > max_bits) {
       if (threadIdx.x == 0) atomicOr(err, 1u);
-      break;
+      return off;
     }
     uint32_t pp[KPT];
 #pragma unroll
@@ -467,7 +496,6 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
       nseen[i] = 0;
       ncol[i] = 0;
     }
-    int left = 0;
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
@@ -475,12 +503,15 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
         if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
           alive &= ~(1u << s);
           ppos[s * RK_WG + threadIdx.x] = (uint32_t)(off + pos);
-        } else {
-          left = 1;
         }
       }
     }
-    any = __syncthreads_or(left) != 0;
+    const uint32_t wl = wave_sum4((uint32_t)__popc(alive));
+    if (__lane_id() == 0) tl->left[l & 1][wv] = wl;
+    __syncthreads();
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < RK_WG / 64; w++) total += tl->left[l & 1][w];
 #pragma unroll
     for (int s = 0; s < KPT; s++) h[s] += d[s];
     uint64_t* ts = seen;
@@ -493,32 +524,114 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
     t = bbh_next_t(t);
     l++;
   }
-  return off;
+  if (total == 0) return off;
+  // ---- tail: list the remaining keys (wave counts from the last level, or
+  // a fresh count when the segment starts small), then wave 0 alone.
+  if (l == 0) {
+    const uint32_t wl = wave_sum4((uint32_t)__popc(alive));
+    if (__lane_id() == 0) tl->left[0][wv] = wl;
+    __syncthreads();
+  }
+  const int lb = l == 0 ? 0 : ((l - 1) & 1);
+  uint32_t base = 0;
+  for (uint32_t w = 0; w < wv; w++) base += tl->left[lb][w];
+  base += wave_prefix4((uint32_t)__popc(alive));
+#pragma unroll
+  for (int s = 0; s < KPT; s++) {
+    if ((alive >> s) & 1) {
+      tl->h[base] = h[s];
+      tl->d[base] = d[s];
+      tl->e[base] = (uint32_t)(s * RK_WG + threadIdx.x);
+      base++;
+    }
+  }
+  __syncthreads();
+  if (wv == 0) {
+    const uint32_t lane = __lane_id();
+    bool a = lane < total;
+    uint64_t hh = a ? tl->h[lane] : 0;
+    const uint64_t dd = a ? tl->d[lane] : 0;
+    const uint32_t ee = a ? tl->e[lane] : 0;
+    for (;;) {  // seen / col are clear for this level's n
+      const uint64_t n = bbh_level_n(t);
+      if (l >= BBH_MAX_LEVELS || off + n > max_bits) {
+        if (lane == 0) atomicOr(err, 1u);
+        break;
+      }
+      uint32_t pos = 0;
+      if (a) {
+        pos = bbh_pos(hh, (uint32_t)n);
+        const unsigned long long bit = 1ULL << (pos & 63);
+        const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
+        if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+      }
+      wave_lds_order();
+      const uint32_t nw = (uint32_t)(n >> 6);
+      for (uint32_t i = lane; i < nw; i += 64) region[(off >> 6) + i] = seen[i] & ~col[i];
+      if (a && (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1)) {
+        ppos[ee] = (uint32_t)(off + pos);
+        a = false;
+      }
+      wave_lds_order();
+      const uint32_t nw2 = (uint32_t)(bbh_level_n(bbh_next_t(t)) >> 6);
+      for (uint32_t i = lane; i < nw2; i += 64) {
+        seen[i] = 0;
+        col[i] = 0;
+      }
+      wave_lds_order();
+      hh += dd;
+      off += n;
+      t = bbh_next_t(t);
+      l++;
+      if (__ballot(a) == 0) break;
+    }
+    if (lane == 0) tl->off = off;
+  }
+  __syncthreads();
+  return tl->off;
 }
 
-// Sort + dedup of one segment, then the size of its BBHash. Each occurrence
-// carries its 10-bit observation mask; the masks of equal keys are OR-joined
-// (the DFA join of classify.rs:147-518), and the segment's BBHash is built.
-// Output in place: sorted distinct keys at occ[start...] with their joined
-// masks at obs[start...], their count in ucount, the masks again in
-// local-rank order at vpad[start...], the BBHash blocks in the segment's
-// padded slot of blocks_pad and their count in nblk. Oversized segments go to over_list (sorted by k_p3_bigsort);
-// segments whose distinct keys exceed the register capacity go to big_list
-// (BBHash built by k_bbh_big).
+// Slot of a key in its segment's dedup hash table of 2^tb entries: the bits
+// of seg_hash just below the segment bits (uniform inside a segment).
+template <typename KeyT>
+__device__ __forceinline__ uint32_t tab_slot(const KeyT& k, int sb, int tb) {
+  return (uint32_t)((seg_hash(k) << sb) >> (64 - tb));
+}
+
+// Dedup of one segment and its BBHash. Each occurrence carries its 10-bit
+// observation mask; the masks of equal keys are OR-joined (the DFA join of
+// classify.rs:147-518). The distinct keys are found with an LDS hash table
+// (linear probing, entries = representative occurrence + 1 | joined mask
+// << 12); which occurrence represents a key depends on timing, but the set of
+// keys, their joined masks and the BBHash built over them do not. Output: the
+// distinct keys in MPHF value order at kout[start + local rank], their masks
+// at vpad[start + local rank], their count in ucount, the BBHash blocks in
+// the segment's padded slot of blocks_pad and their count in nblk. Segments
+// with more occurrences than CAP go to over_list (sorted by k_p3_bigsort);
+// segments whose distinct keys exceed CAP go to big_list (BBHash built by
+// k_bbh_big). Keys of <= 12 bits (k <= 6) use a direct-address mask table
+// instead, which takes any number of occurrences.
 template <typename KeyT>
 __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
                                                     const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K2, int sb,
                                                     uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
                                                     uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
                                                     uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad,
-                                                    uint16_t* __restrict__ vpad) {
+                                                    KeyT* __restrict__ kout, uint16_t* __restrict__ vpad) {
   constexpr int KPT = SegCfg<KeyT>::KPT;
   constexpr int CAP = RK_WG * KPT;
+  constexpr int TAB_BITS = 12;
   constexpr int SEEN_W = (2 * CAP) / 64 + 1;
+  constexpr int REG_W = (4 * CAP + 8192) / 64;
+  static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
+  static_assert(REG_W * 8 <= (int)sizeof(KeyT) * CAP && (CAP + REG_W) * 4 <= (4 << TAB_BITS), "LDS reuse fits");
+  static_assert((1 << BM_MAX_BITS) <= (1 << TAB_BITS) && REG_W / 7 + 2 <= SEEN_W, "LDS reuse fits");
+  // buf: keys for the table's comparisons | region (level words) | staged keys
+  // tab: hash table (or direct-address mask table) | ppos + word ranks | staged masks
   __shared__ KeyT buf[CAP];
-  __shared__ uint16_t mbuf[CAP];
-  __shared__ uint32_t cnt[BK_N];
+  __shared__ uint32_t tab[1 << TAB_BITS];
   __shared__ uint64_t seenA[SEEN_W], colA[SEEN_W], seenB[SEEN_W], colB[SEEN_W];
+  __shared__ BbhTail tail;
   __shared__ uint64_t sh8[8];
   unsigned int* n_over = flags;
   unsigned int* n_big = flags + 1;
@@ -527,8 +640,6 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   if (seg >= n_segs) return;
   const uint64_t start = seg_off[seg];
   const uint64_t n = seg_off[seg + 1] - start;
-  uint64_t u = 0;          // distinct keys written to occ[start...]
-  bool from_lds = false;   // the distinct keys / masks are also compacted in buf / mbuf
   if (n == 0) {
     if (threadIdx.x == 0) {
       ucount[seg] = 0;
@@ -536,11 +647,13 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
     return;
   }
+  KeyT key[KPT];
+  uint32_t msk[KPT];
+  uint32_t alive = 0;
+  uint64_t u = 0;  // distinct keys
   if (sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS) {
     // ---- direct-address table of masks over all 2^K2 keys: any number of
-    // duplicates, sorted output.
-    uint32_t* tab = (uint32_t*)buf;
-    static_assert((1 << BM_MAX_BITS) * 4 <= (int)sizeof(buf), "table fits");
+    // occurrences; distinct keys come out sorted at occ / obs[start...].
     const uint32_t nt = 1u << K2;
     for (uint32_t i = threadIdx.x; i < nt; i += RK_WG) tab[i] = 0;
     __syncthreads();
@@ -562,6 +675,25 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       o++;
     }
     u = tot;
+    if (threadIdx.x == 0) ucount[seg] = u;
+    if (seg_is_big<KeyT>(n, u, K2)) {
+      if (threadIdx.x == 0) {
+        big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
+        nblk[seg] = 0;
+      }
+      return;
+    }
+    __threadfence_block();
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      const uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+      if (e < u) {
+        key[s] = occ[start + e];
+        msk[s] = obs[start + e];
+        alive |= 1u << s;
+      }
+    }
   } else if (n > (uint64_t)CAP) {
     if (threadIdx.x == 0) {
       over_list[atomicAdd(n_over, 1u)] = (uint32_t)seg;
@@ -570,202 +702,59 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
     return;
   } else {
-    // ---- bucket sort on the top BK_BITS key bits.
-    const int kpt = (int)((n + RK_WG - 1) / RK_WG);
-    KeyT key[KPT];
-    uint32_t msk[KPT];
-    uint32_t aux[KPT];
+    // ---- hash-table dedup: table of 2^tb >= 2n entries (load <= 1/2).
+    int tb = 6;
+    while ((1ull << tb) < 2 * n) tb++;
+    const uint32_t tmask = (1u << tb) - 1;
+    uint32_t pend = 0;
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
-      uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
-      if (s < kpt) {
-        if (e < n) {
-          key[s] = occ[start + e];
-          msk[s] = obs[start + e];
-        } else {
-          key[s] = key_pad<KeyT>();
-          msk[s] = 0;
-        }
-      }
-    }
-    const int bsh = K2 > BK_BITS ? K2 - BK_BITS : 0;
-    for (int i = threadIdx.x; i < BK_N; i += RK_WG) cnt[i] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < KPT; s++) {
-      uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
-      if (s < kpt && e < n) aux[s] = atomicAdd(&cnt[key_shr32(key[s], bsh) & (BK_N - 1)], 1u);
-    }
-    __syncthreads();
-    constexpr int CPT = BK_N / RK_WG;  // counters per thread
-    uint64_t loc = 0, mx = 0;
-    for (int q = 0; q < CPT; q++) {
-      uint32_t v = cnt[threadIdx.x * CPT + q];
-      loc += v;
-      mx = v > mx ? v : mx;
-    }
-    if (!__syncthreads_or(mx > BK_MAX)) {
-      uint64_t run = block_excl_scan_u64(loc, sh8, nullptr);
-      for (int q = 0; q < CPT; q++) {
-        uint32_t v = cnt[threadIdx.x * CPT + q];
-        cnt[threadIdx.x * CPT + q] = (uint32_t)run;
-        run += v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < KPT; s++) {
-        uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
-        if (s < kpt && e < n) {
-          uint32_t at = cnt[key_shr32(key[s], bsh) & (BK_N - 1)] + aux[s];
-          buf[at] = key[s];
-          mbuf[at] = (uint16_t)msk[s];
-        }
-      }
-      __syncthreads();
-      for (int q = 0; q < CPT; q++) {  // insertion sort inside each bucket
-        const int b = threadIdx.x * CPT + q;
-        const uint32_t a0 = cnt[b], a1 = (b + 1 < BK_N) ? cnt[b + 1] : (uint32_t)n;
-        for (uint32_t i = a0 + 1; i < a1; i++) {
-          KeyT kv = buf[i];
-          uint16_t mv = mbuf[i];
-          uint32_t j = i;
-          while (j > a0 && key_lt(kv, buf[j - 1])) {
-            buf[j] = buf[j - 1];
-            mbuf[j] = mbuf[j - 1];
-            j--;
-          }
-          buf[j] = kv;
-          mbuf[j] = mv;
-        }
-      }
-      __syncthreads();
-    } else {
-      // ---- LDS LSD radix sort (buckets too uneven for insertion sorts).
-      // Keys are reloaded in the ranking's element order e = wave*64*kpt +
-      // slot*64 + lane, so that the pads (e >= n) come after every key and
-      // stay there: they tie with keys whose varying bits are all ones.
-      const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
-      uint64_t xl = 0, xh = 0;
-      const KeyT k0 = occ[start];
-#pragma unroll
-      for (int s = 0; s < KPT; s++) {
-        if (s < kpt) {
-          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
-          key[s] = e < n ? occ[start + e] : key_pad<KeyT>();
-          msk[s] = e < n ? (uint32_t)obs[start + e] : 0u;
-          if (e < n) key_xor_or(xl, xh, key[s], k0);
-        }
-      }
-      block_or2(xl, xh, sh8);
-      const int passes = (varying_bits(xl, xh) + 7) >> 3;
-      for (int p = 0; p < passes; p++) {
-        const int sh = 8 * p;
-        for (int i = threadIdx.x; i < 4 * 256; i += RK_WG) cnt[i] = 0;
-        __syncthreads();
-        rank_tile<KeyT, KPT>(key, kpt, sh, aux, cnt);
-        __syncthreads();
-        rank_offsets(cnt, sh8, nullptr);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < KPT; s++) {
-          if (s < kpt) {
-            uint32_t at = cnt[wv * 256 + key_digit(key[s], sh)] + aux[s];
-            buf[at] = key[s];
-            mbuf[at] = (uint16_t)msk[s];
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < KPT; s++) {
-          if (s < kpt) {
-            uint32_t e = (uint32_t)(wv * 64 * kpt + s * 64 + lane);
-            key[s] = buf[e];
-            msk[s] = mbuf[e];
-          }
-        }
-        __syncthreads();
-      }
-      if (passes == 0) {  // one distinct key
-#pragma unroll
-        for (int s = 0; s < KPT; s++) {
-          uint64_t e = (uint64_t)wv * 64 * kpt + (uint64_t)s * 64 + lane;
-          if (s < kpt) {
-            buf[e] = key[s];
-            mbuf[e] = (uint16_t)msk[s];
-          }
-        }
-        __syncthreads();
-      }
-    }
-    // buf[0..n) sorted; a run's head joins the masks of its run. Heads are
-    // read into registers, then written back compacted in place (and to
-    // occ / obs) once every thread has read (the scan's barriers).
-    const uint64_t e0 = (uint64_t)threadIdx.x * kpt;
-    KeyT hk[KPT];
-    uint32_t hm[KPT];
-    uint32_t hbits = 0, c = 0;
-#pragma unroll
-    for (int q = 0; q < KPT; q++) {
-      const uint64_t e = e0 + q;
-      if (q < kpt && e < n && (e == 0 || !key_eq(buf[e], buf[e - 1]))) {
-        uint32_t m = mbuf[e];
-        for (uint64_t f = e + 1; f < n && key_eq(buf[f], buf[e]); f++) m |= mbuf[f];
-        hk[q] = buf[e];
-        hm[q] = m;
-        hbits |= 1u << q;
-        c++;
-      }
-    }
-    uint64_t tot;
-    uint64_t o = block_excl_scan_u64(c, sh8, &tot);
-#pragma unroll
-    for (int q = 0; q < KPT; q++) {
-      if ((hbits >> q) & 1) {
-        buf[o] = hk[q];
-        mbuf[o] = (uint16_t)hm[q];
-        occ[start + o] = hk[q];
-        obs[start + o] = (uint16_t)hm[q];
-        o++;
-      }
-    }
-    u = tot;
-    from_lds = true;
-  }
-  // ---- BBHash of the segment, into its padded block slot; the joined masks
-  // also go to vpad[start + local rank] (obs keeps them in key order).
-  if (threadIdx.x == 0) ucount[seg] = u;
-  if (seg_is_big<KeyT>(n, u, K2)) {
-    if (threadIdx.x == 0) {
-      big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
-      nblk[seg] = 0;
-    }
-    return;
-  }
-  constexpr int REG_W = (4 * CAP + 8192) / 64;
-  static_assert(REG_W * 8 <= (int)sizeof(buf) && CAP <= BK_N && REG_W / 7 + 2 <= SEEN_W, "LDS reuse fits");
-  __threadfence_block();
-  __syncthreads();  // compacted keys (or, on the table path, occ / obs) are complete
-  // LDS reuse: region (level words) in buf once the keys are in registers,
-  // bit positions in cnt, block ranks in seenA (free once the levels are done).
-  uint64_t* region = (uint64_t*)buf;
-  uint32_t* ppos = cnt;
-  uint64_t* brank = seenA;
-  KeyT key[KPT];
-  uint32_t alive = 0;
-#pragma unroll
-  for (int s = 0; s < KPT; s++) {
-    uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
-    if (e < u) {
-      if (from_lds) {
-        key[s] = buf[e];
-      } else {
+      const uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+      if (e < n) {
         key[s] = occ[start + e];
-        mbuf[e] = obs[start + e];
+        msk[s] = obs[start + e];
+        buf[e] = key[s];
+        pend |= 1u << s;
       }
-      alive |= 1u << s;
     }
+    for (uint32_t i = threadIdx.x; i <= tmask; i += RK_WG) tab[i] = 0;
+    __syncthreads();
+    uint32_t hs[KPT];
+#pragma unroll
+    for (int s = 0; s < KPT; s++) hs[s] = ((pend >> s) & 1) ? tab_slot(key[s], sb, tb) : 0u;
+    while (pend) {
+      uint32_t old[KPT];
+#pragma unroll
+      for (int s = 0; s < KPT; s++)
+        if ((pend >> s) & 1) old[s] = atomicCAS(&tab[hs[s]], 0u, (uint32_t)(s * RK_WG + threadIdx.x + 1) | (msk[s] << 12));
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if ((pend >> s) & 1) {
+          if (old[s] == 0) {
+            alive |= 1u << s;
+            pend &= ~(1u << s);
+          } else if (key_eq(buf[(old[s] & 0xFFFu) - 1], key[s])) {
+            atomicOr(&tab[hs[s]], msk[s] << 12);
+            pend &= ~(1u << s);
+          } else {
+            hs[s] = (hs[s] + 1) & tmask;
+          }
+        }
+      }
+    }
+    __syncthreads();  // every mask is joined
+#pragma unroll
+    for (int s = 0; s < KPT; s++)
+      if ((alive >> s) & 1) msk[s] = tab[hs[s]] >> 12;
+    uint64_t tot;
+    block_excl_scan_u64((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab and buf are free after it
+    u = tot;
+    if (threadIdx.x == 0) ucount[seg] = u;
   }
-  __syncthreads();  // buf becomes the region
+  // ---- BBHash of the segment, into its padded block slot.
+  uint64_t* region = (uint64_t*)buf;
+  uint32_t* ppos = tab;
+  uint32_t* wrank = tab + CAP;
   for (int i = threadIdx.x; i < REG_W; i += RK_WG) region[i] = 0;
   for (int i = threadIdx.x; i < SEEN_W; i += RK_WG) {
     seenA[i] = 0;
@@ -774,38 +763,61 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   __syncthreads();
   const uint64_t slot = pad_slot_blocks(n);
   const uint64_t max_bits = slot * BBH_BLOCK_BITS < (uint64_t)REG_W * 64 ? slot * BBH_BLOCK_BITS : (uint64_t)REG_W * 64;
-  const uint64_t bits = bbh_levels<KeyT, KPT>(key, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, err);
+  const uint64_t bits = bbh_levels<KeyT, KPT>(key, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, &tail, err);
   const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < RK_WG
   __syncthreads();
+  // blocks {rank before, 7 words}; word ranks for the local ranks below
   const uint64_t j = threadIdx.x;
   uint64_t w[7];
   uint64_t pc = 0;
 #pragma unroll
   for (int q = 0; q < 7; q++) {
-    uint64_t x = j * 7 + q;
+    const uint64_t x = j * 7 + q;
     w[q] = (j < nb && x < (uint64_t)REG_W) ? region[x] : 0;
     pc += (uint64_t)__popcll(w[q]);
   }
   uint64_t tot;
   const uint64_t rk = block_excl_scan_u64(j < nb ? pc : 0, sh8, &tot);
   if (j < nb) {
-    brank[j] = rk;
     uint64_t* B = blocks_pad + (pad_slot_off(start, seg) + j) * 8;
     B[0] = rk;
+    uint64_t run = rk;
 #pragma unroll
-    for (int q = 0; q < 7; q++) B[1 + q] = w[q];
+    for (int q = 0; q < 7; q++) {
+      B[1 + q] = w[q];
+      if (j * 7 + q < (uint64_t)REG_W) wrank[j * 7 + q] = (uint32_t)run;
+      run += (uint64_t)__popcll(w[q]);
+    }
   }
   if (threadIdx.x == 0) nblk[seg] = nb;
   __syncthreads();
-  for (uint64_t e = threadIdx.x; e < u; e += RK_WG) {
-    const uint32_t p = ppos[e];
-    const uint32_t qb = p / (uint32_t)BBH_BLOCK_BITS, b = p - qb * (uint32_t)BBH_BLOCK_BITS;
-    uint64_t r = brank[qb];
-    const uint32_t wi = b >> 6;
-    for (uint32_t q = 0; q < wi; q++) r += (uint64_t)__popcll(region[qb * 7 + q]);
-    r += (uint64_t)__popcll(region[qb * 7 + wi] & ((1ULL << (b & 63)) - 1));
-    if (r >= u) atomicOr(err, 8u);
-    else vpad[start + r] = mbuf[e];
+  // local rank of each distinct key, then keys and masks staged in rank order
+  uint32_t rr[KPT];
+#pragma unroll
+  for (int s = 0; s < KPT; s++) {
+    rr[s] = ~0u;
+    if ((alive >> s) & 1) {
+      const uint32_t p = ppos[s * RK_WG + threadIdx.x];
+      const uint32_t wi = p >> 6;
+      const uint32_t r = p < bits ? wrank[wi] + (uint32_t)__popcll(region[wi] & ((1ULL << (p & 63)) - 1)) : ~0u;
+      if (r < u) rr[s] = r;
+      else atomicOr(err, 8u);
+    }
+  }
+  __syncthreads();
+  KeyT* stage_k = buf;
+  uint16_t* stage_m = (uint16_t*)tab;
+#pragma unroll
+  for (int s = 0; s < KPT; s++) {
+    if (rr[s] != ~0u) {
+      stage_k[rr[s]] = key[s];
+      stage_m[rr[s]] = (uint16_t)msk[s];
+    }
+  }
+  __syncthreads();
+  for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
+    kout[start + i] = stage_k[i];
+    vpad[start + i] = stage_m[i];
   }
 }
 
@@ -947,12 +959,11 @@ __global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, ui
 }
 
 // ------------------------------------------------------------------ compaction
-// One work-group per segment copies the segment's distinct keys to the dense
-// key order (key_off), its masks (local-rank order, vpad) to their MPHF
-// values in vmask, and its BBHash blocks from the padded slot to the dense
-// block order (blk_off).
+// One work-group per segment copies the segment's distinct keys and masks
+// (local-rank order, kout / vpad) to their MPHF values in keys / vmask, and
+// its BBHash blocks from the padded slot to the dense block order (blk_off).
 template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ occ, const uint16_t* __restrict__ vpad,
+__global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ kout, const uint16_t* __restrict__ vpad,
                                                       const uint64_t* __restrict__ seg_off,
                                                       const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
                                                       const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks_pad,
@@ -963,7 +974,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ o
   const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
   if (u == 0) return;
   for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
-    keys[ko + i] = occ[s0 + i];
+    keys[ko + i] = kout[s0 + i];
     vmask[ko + i] = vpad[s0 + i];
   }
   const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
@@ -977,8 +988,8 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ o
 // the seen / collision / alive bitmaps sit in LDS (segments of <= MED_CAP
 // distinct keys); otherwise in global scratch (seen / collision words of the
 // segment at 4*floor(seg_off/64) + 8*seg, alive bits per occurrence position)
-// with device-scope atomics. Blocks go to the padded slot; each key's mask is
-// written to vpad[seg_off + local rank].
+// with device-scope atomics. Blocks go to the padded slot; each key and its
+// mask are written to kout / vpad[seg_off + local rank].
 constexpr uint64_t MED_CAP = 1ULL << 17;
 constexpr int MED_W = (int)(MED_CAP / 64);
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
@@ -990,7 +1001,7 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
                                                    const uint64_t* __restrict__ ucount, const uint32_t* __restrict__ big_list,
                                                    uint64_t* __restrict__ nblk, uint64_t* __restrict__ blocks_pad,
                                                    uint64_t* __restrict__ scratch, uint64_t* __restrict__ alive_g,
-                                                   unsigned int* __restrict__ err) {
+                                                   KeyT* __restrict__ kout, unsigned int* __restrict__ err) {
   __shared__ uint64_t sh8[8];
   __shared__ unsigned long long carry;
   const uint32_t seg = big_list[blockIdx.x];
@@ -1096,9 +1107,14 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
   __threadfence();
   __syncthreads();
   for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
-    uint64_t rk = mphf_local_rank<KeyT>(region, D, keys[e]);
-    if (rk >= D) atomicOr(err, 4u);
-    else vpad[so + rk] = obs[so + e];
+    const KeyT kv = keys[e];
+    uint64_t rk = mphf_local_rank<KeyT>(region, D, kv);
+    if (rk >= D) {
+      atomicOr(err, 4u);
+    } else {
+      kout[so + rk] = kv;
+      vpad[so + rk] = obs[so + e];
+    }
   }
 }
 
@@ -1148,14 +1164,14 @@ void launch_p3_split(hipStream_t st, bool wide, const void* src, const uint16_t*
 }
 void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
                      uint64_t* ucount, uint64_t* nblk, uint32_t* over_list, unsigned int* flags, uint32_t* big_list,
-                     uint64_t* blocks_pad, uint16_t* vpad) {
+                     uint64_t* blocks_pad, void* kout, uint16_t* vpad) {
   if (n_segs == 0) return;
   if (!wide)
     CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (uint64_t*)occ, obs, seg_off, n_segs,
-                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
+                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, (uint64_t*)kout, vpad);
   else
     CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (Key128*)occ, obs, seg_off, n_segs,
-                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
+                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, (Key128*)kout, vpad);
 }
 void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void* aux, uint16_t* aux_obs, const uint64_t* seg_off,
                        const uint32_t* over_list, uint32_t n_over, uint64_t* ucount, unsigned int* flags, uint32_t* big_list) {
@@ -1171,32 +1187,32 @@ void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void
 // segments of its size class and returns at once for the others.
 void launch_bbh_big(hipStream_t st, bool wide, const void* occ, const uint16_t* obs, uint16_t* vpad, const uint64_t* seg_off,
                     const uint64_t* ucount, const uint32_t* big_list, uint32_t n_big, uint64_t* nblk, uint64_t* blocks_pad,
-                    uint64_t* scratch, uint64_t* alive_g, unsigned int* err) {
+                    uint64_t* scratch, uint64_t* alive_g, void* kout, unsigned int* err) {
   if (n_big == 0) return;
   dim3 g(n_big), b(RK_WG);
   if (!wide) {
     CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, vpad, seg_off, ucount, big_list,
-                nblk, blocks_pad, scratch, alive_g, err);
+                nblk, blocks_pad, scratch, alive_g, (uint64_t*)kout, err);
     if (alive_g)
       CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, uint64_t>), g, b, 0, st, (const uint64_t*)occ, obs, vpad, seg_off, ucount,
-                  big_list, nblk, blocks_pad, scratch, alive_g, err);
+                  big_list, nblk, blocks_pad, scratch, alive_g, (uint64_t*)kout, err);
   } else {
     CF1G_LAUNCH("p3b_bbh_big", (k_bbh_big<true, Key128>), g, b, 0, st, (const Key128*)occ, obs, vpad, seg_off, ucount, big_list,
-                nblk, blocks_pad, scratch, alive_g, err);
+                nblk, blocks_pad, scratch, alive_g, (Key128*)kout, err);
     if (alive_g)
       CF1G_LAUNCH("p3b_bbh_huge", (k_bbh_big<false, Key128>), g, b, 0, st, (const Key128*)occ, obs, vpad, seg_off, ucount, big_list,
-                  nblk, blocks_pad, scratch, alive_g, err);
+                  nblk, blocks_pad, scratch, alive_g, (Key128*)kout, err);
   }
 }
-void launch_p3_compact(hipStream_t st, bool wide, const void* occ, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount,
+void launch_p3_compact(hipStream_t st, bool wide, const void* kout, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount,
                        const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad, uint64_t n_segs, int K, int sb,
                        void* keys, uint16_t* vmask, uint64_t* blocks) {
   if (n_segs == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)occ, vpad, seg_off,
+    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)kout, vpad, seg_off,
                 ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (uint64_t*)keys, vmask, blocks);
   else
-    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)occ, vpad, seg_off,
+    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)kout, vpad, seg_off,
                 ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (Key128*)keys, vmask, blocks);
 }
 

@@ -1,6 +1,7 @@
 // cf1ref — CPU oracle pipeline (P1..P5) and its C entry points for ctypes.
 // TEST INFRASTRUCTURE ONLY (see cf1ref.hpp): loaded by tests/, the smoke check
 // and bench.py's cpu_baseline leg; never by the product.
+#include <atomic>
 #include "cf1ref.hpp"
 
 #include <chrono>
@@ -94,8 +95,7 @@ void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi,
   const int sb = out.sb;
   const uint64_t ns = bins << sb;
   out.key_off.assign(ns + 1, 0);
-  // regroup each sorted bin by segment (a stable counting sort keeps the key
-  // order inside a segment)
+  // group each sorted bin by segment (stable counting sort)
   std::vector<uint64_t> bin_start(bins + 1, 0);
   for (uint64_t b = 0; b < bins; b++) bin_start[b + 1] = bin_start[b] + per_bin[b];
   if (bin_start[bins] != lo.size()) throw std::runtime_error("MPHF: per-bin counts do not cover the keys");
@@ -143,6 +143,26 @@ void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi,
   out.blk_off[ns] = tot;
   out.blocks.assign(tot * 8, 0);
   for (uint64_t s = 0; s < ns; s++) std::copy(blk[s].begin(), blk[s].end(), out.blocks.begin() + (ptrdiff_t)(out.blk_off[s] * 8));
+  // Value order: key i moves to its MPHF value; the values of each segment
+  // must be a permutation of the segment's key range.
+  std::vector<uint64_t> vlo(lo.size()), vhi(hi.size());
+  std::vector<uint8_t> hit(lo.size(), 0);
+  std::atomic<bool> bad{false};
+  parallel_for(ns, threads, [&](size_t s) {
+    for (uint64_t i = out.key_off[s]; i < out.key_off[s + 1]; i++) {
+      const uint64_t v = out.lookup(lo[i], hi[i]);
+      if (v < out.key_off[s] || v >= out.key_off[s + 1] || hit[v]) {
+        bad = true;
+        return;
+      }
+      hit[v] = 1;
+      vlo[v] = lo[i];
+      vhi[v] = hi[i];
+    }
+  });
+  if (bad) throw std::runtime_error("MPHF: values are not a permutation of the keys");
+  lo.swap(vlo);
+  hi.swap(vhi);
 }
 
 // ------------------------------------------------------------------ minimizer scan

@@ -437,8 +437,9 @@ inline uint32_t mmer_order(uint32_t canon) { return fmix32(canon + 0x9E3779B9u);
 //   segment of key c = (bin, s): bin = bucket_to_bin[bucket_hash(v(c))] with
 //   v(c) the minimizer value of c; s = top sb bits of seg_hash(c) =
 //   (lo ^ hi * 0xC2B2AE3D27D4EB4F) * 0x9E3779B97F4A7C15 (a Fibonacci hash),
-//   which spreads every bin evenly over its 2^sb segments. The MPHF key order is (bin, segment, key): each bin's
-//   sorted distinct keys, stably regrouped by segment.
+//   which spreads every bin evenly over its 2^sb segments. Segments are
+//   numbered (bin << sb) | s and hold consecutive value ranges key_off[seg]..;
+//   the distinct keys are stored in value order (keys[v] has value v).
 //   sb = fewest bits (<= 12) with W <= bins * 2^sb * target
 //   (target 1536; W = valid k-mer occurrences).
 //   per segment with D_s keys, level l has n_l = roundup64(max(2 t_l, 1)) bits,
@@ -553,8 +554,8 @@ struct Mphf {
 };
 
 // Build over the consolidated keys (bins in order, each bin sorted) given the
-// per-bin distinct counts; lo/hi are regrouped in place into the MPHF key
-// order (bin, segment, key). Threads split the segments.
+// per-bin distinct counts; lo/hi are reordered in place into MPHF value
+// order (lo[v], hi[v] = the key whose value is v). Threads split the segments.
 void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi, const std::vector<uint64_t>& per_bin,
                 const std::vector<uint16_t>& b2b, int K, int m, uint64_t W, uint64_t target, int threads);
 
