@@ -8,43 +8,84 @@
 namespace cf1g {
 
 // ------------------------------------------------------------------ pack
-// One thread per 64 stream positions: one N-mask word and two sequence words.
+// Per 64 stream positions: one N-mask word and two sequence words.
 // gstart[r] = stream position of record r's first base; gstart[n_rec] = the
 // position just past the last separator. Non-ACGT bytes and separators are
 // placeholders (dna.rs:64-92); their 2-bit code is 0.
+
+// 2-bit code of an ACGT byte (either case: A 0, C 1, G 2, T 3), 4 for any
+// other byte.
+__device__ __forceinline__ uint32_t base_code(uint32_t c) {
+  // upper-cased letter minus 'A' in {0 (A), 2 (C), 6 (G), 19 (T)}
+  const uint32_t d = (c & 0xDFu) - 0x41u;
+  const uint32_t ok = (uint32_t)(d < 32u) & (0x80045u >> (d & 31u));
+  return (ok & 1u) ? ((c >> 1) ^ (c >> 2)) & 3u : 4u;
+}
+// bit i of x -> bit 2i
+__device__ __forceinline__ uint64_t spread32(uint32_t v) {
+  uint64_t x = v;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFULL;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFULL;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+  x = (x | (x << 2)) & 0x3333333333333333ULL;
+  x = (x | (x << 1)) & 0x5555555555555555ULL;
+  return x;
+}
+// One wave packs 64 consecutive 64-position words. Step q reads positions
+// 64q + lane (one coalesced byte per lane); the ballots of the three code
+// bits over the lanes are word q's N-mask and base bits, kept by lane q.
+// Each lane then interleaves its own word's bits.
 __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii, const uint64_t* __restrict__ rec_off,
                                               const uint64_t* __restrict__ gstart, uint64_t n_rec, uint64_t n_words64,
                                               uint64_t* __restrict__ seq, uint64_t* __restrict__ nmask) {
-  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_words64) return;
-  uint64_t g0 = t * 64;
-  // largest r with gstart[r] <= g0 (r = n_rec when past the last record)
+  const uint64_t w0 = (((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64;
+  if (w0 >= n_words64) return;  // whole waves
+  const uint32_t lane = __lane_id();
+  const uint32_t nq = n_words64 - w0 < 64 ? (uint32_t)(n_words64 - w0) : 64u;
+  const uint64_t gw = w0 * 64;  // first position of the wave
+  // largest r with gstart[r] <= gw (r = n_rec when past the last record)
   uint64_t lo = 0, hi = n_rec + 1;
   while (hi - lo > 1) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (gstart[mid] <= g0) lo = mid; else hi = mid;
+    const uint64_t mid = (lo + hi) >> 1;
+    if (gstart[mid] <= gw) lo = mid; else hi = mid;
   }
+  uint64_t wn = 0, wh = 0, wl = 0;  // ballots of word `lane`
+  auto emit = [&](uint32_t code, uint32_t q) {
+    const uint64_t bn = __ballot(code >> 2), bh = __ballot((code >> 1) & 1u), bl = __ballot(code & 1u);
+    const bool mine = lane == q;
+    wn = mine ? bn : wn;
+    wh = mine ? bh : wh;
+    wl = mine ? bl : wl;
+    __builtin_amdgcn_sched_barrier(0);  // consume each step's ballots before the next step's
+  };
   uint64_t r = lo;
-  uint64_t s0 = 0, s1 = 0, nm = 0;
-  for (int q = 0; q < 64; q++) {
-    uint64_t g = g0 + (uint64_t)q;
-    while (r < n_rec && g >= gstart[r + 1]) r++;
-    uint32_t code = 4;
-    if (r < n_rec && g >= gstart[r] && g + 1 < gstart[r + 1]) {
-      uint8_t c = ascii[rec_off[r] + (g - gstart[r])];
-      uint8_t u = c & 0xDF;  // upper-case letters
-      code = (u == 'A') ? 0u : (u == 'C') ? 1u : (u == 'G') ? 2u : (u == 'T') ? 3u : 4u;
-      if (c < 'A') code = 4;
+  if (nq == 64 && r < n_rec && gw >= gstart[r] && gw + 64 * 64 < gstart[r + 1]) {
+    // the wave's positions are all bases of record r: contiguous bytes
+    // (gw < gstart[0] on the first wave: position 0 is a separator)
+    const uint8_t* p = ascii + rec_off[r] + (gw - gstart[r]) + lane;
+    for (uint32_t q0 = 0; q0 < 64; q0 += 16) {
+      uint8_t c[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) c[u] = p[(q0 + u) * 64];
+#pragma unroll
+      for (int u = 0; u < 16; u++) emit(base_code(c[u]), q0 + u);
     }
-    uint64_t isn = code >> 2;
-    uint64_t b = code & 3;
-    nm |= isn << (63 - q);
-    if (q < 32) s0 |= b << (62 - 2 * q);
-    else s1 |= b << (62 - 2 * (q - 32));
+  } else {
+    uint64_t g = gw + lane;
+    for (uint32_t q = 0; q < nq; q++, g += 64) {
+      while (r < n_rec && g >= gstart[r + 1]) r++;
+      uint32_t code = 4;
+      if (r < n_rec && g >= gstart[r] && g + 1 < gstart[r + 1]) code = base_code(ascii[rec_off[r] + (g - gstart[r])]);
+      emit(code, q);
+    }
   }
-  seq[2 * t] = s0;
-  seq[2 * t + 1] = s1;
-  nmask[t] = nm;
+  if (lane < nq) {
+    // base i of the word (lane i of its step) -> bits 63-2i (high), 62-2i (low)
+    seq[2 * (w0 + lane)] = (spread32(__builtin_bitreverse32((uint32_t)wh)) << 1) | spread32(__builtin_bitreverse32((uint32_t)wl));
+    seq[2 * (w0 + lane) + 1] =
+        (spread32(__builtin_bitreverse32((uint32_t)(wh >> 32))) << 1) | spread32(__builtin_bitreverse32((uint32_t)(wl >> 32)));
+    nmask[w0 + lane] = __builtin_bitreverse64(wn);
+  }
 }
 
 // ------------------------------------------------------------------ minimizer scan (P1)
@@ -541,7 +582,8 @@ __global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* _
 void launch_pack(hipStream_t st, const uint8_t* ascii, const uint64_t* rec_off, const uint64_t* gstart, uint64_t n_rec,
                  uint64_t n_words64, uint64_t* seq, uint64_t* nmask) {
   if (n_words64 == 0) return;
-  dim3 grid((unsigned)((n_words64 + 255) / 256));
+  const uint64_t waves = (n_words64 + 63) / 64;  // 64 words per wave, 4 waves per work-group
+  dim3 grid((unsigned)((waves + 3) / 4));
   CF1G_LAUNCH("pack", k_pack, grid, dim3(256), 0, st, ascii, rec_off, gstart, n_rec, n_words64, seq, nmask);
 }
 uint64_t p1_tiles(uint64_t n_pos) { return n_pos > 1 ? (n_pos - 1 + SK_T - 1) / SK_T : 0; }

@@ -173,6 +173,48 @@ __device__ __forceinline__ void for_each_rec_occ(const uint64_t* __restrict__ re
 constexpr int P3_WG = 256;
 constexpr int P3_MAX_SB = 12;
 
+// ---- one lane per k-mer for u64-key records (RW = 2, K <= 31). The record
+// is the 128-bit value W = w0:w1, position t at bits [126-2t, 127-2t]; k-mer
+// j spans positions j+1..j+K.
+__device__ __forceinline__ uint64_t rec2_bits(uint64_t w0, uint64_t w1, int sh) {  // low 64 bits of W >> sh, 0 < sh < 128
+  return sh >= 64 ? (w0 >> (sh - 64)) : ((w0 << (64 - sh)) | (w1 >> sh));
+}
+__device__ __forceinline__ int rec2_kmers(uint64_t w1, int K) {
+  int L = (int)(w1 & 255);
+  if (L > 58) L = 58;
+  return L >= K ? L - K + 1 : 0;
+}
+// Canonical k-mer j of a record of n k-mers and (OBS) its observation mask,
+// the same as the j-th call of for_each_rec_occ.
+template <bool OBS>
+__device__ __forceinline__ uint64_t rec2_occ(uint64_t w0, uint64_t w1, int j, int n, int K, uint64_t mask, uint32_t& m) {
+  const int sh = 126 - 2 * (j + K);
+  const uint64_t x = rec2_bits(w0, w1, sh) & mask;
+  const uint64_t r = rc64(x, K);
+  const uint64_t c = x < r ? x : r;
+  if (OBS) {
+    const bool lf = (w1 >> 8) & 1, rf = (w1 >> 9) & 1;
+    const bool pv = j > 0 || lf, nv = j < n - 1 || rf;
+    const uint32_t pb = (uint32_t)rec2_bits(w0, w1, 126 - 2 * j) & 3u;
+    const uint32_t nb = (uint32_t)rec2_bits(w0, w1, sh - 2) & 3u;
+    const uint64_t nx = ((x << 2) | nb) & mask;
+    const uint64_t nr = (r >> 2) | ((uint64_t)(3 - nb) << (2 * K - 2));
+    const uint64_t nc = nx < nr ? nx : nr;
+    uint32_t type;
+    if (!pv && !nv) type = OBS_MIMO;
+    else if (!nv) type = OBS_RIGHT;
+    else if (nc == c) type = (!pv || x == nx) ? OBS_MIMO : OBS_RIGHT;
+    else type = pv ? OBS_INTERNAL : OBS_LEFT;
+    m = obs_mask(type, x == c, pb, nb);
+  }
+  return c;
+}
+// Round-local k-mer -> record map for RECS_PER_ROUND records of at most
+// OWN_KMERS k-mers each (P2 emits super k-mers of at most w <= 17 k-mers for
+// K <= 31; a round with longer records takes the per-record loop).
+constexpr int OWN_KMERS = 18;
+
+
 // Work-group -> (bin, record range) for the per-bin passes.
 __device__ __forceinline__ bool p3_chunk(const uint64_t* __restrict__ chunk_off, const uint64_t* __restrict__ bin_start,
                                          int num_bins, uint32_t chunk_records, int& bin, uint64_t& r0, uint64_t& r1) {
@@ -213,8 +255,37 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
   const int ns = 1 << sb;
   for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
   __syncthreads();
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-    for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
+  if constexpr (RW == 2) {
+    __shared__ uint8_t own[P3_WG * OWN_KMERS];
+    __shared__ uint16_t oo[P3_WG];
+    __shared__ uint64_t sh4[4];
+    const uint64_t mask = (1ULL << (2 * K)) - 1;
+    for (uint64_t rr = r0; rr < r1; rr += P3_WG) {
+      const uint64_t r = rr + threadIdx.x;
+      const int n = r < r1 ? rec2_kmers(rec[r * 2 + 1], K) : 0;
+      uint64_t T;
+      const uint32_t o = (uint32_t)block_excl_scan_u64((uint64_t)n, sh4, &T);
+      if (__syncthreads_or(n > OWN_KMERS)) {
+        if (n) for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
+        continue;
+      }
+      oo[threadIdx.x] = (uint16_t)o;
+      for (int q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
+      __syncthreads();
+      for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += P3_WG) {
+        const uint32_t rl = own[g];
+        const uint64_t* wp = rec + (rr + rl) * 2;
+        const uint64_t w0 = wp[0], w1 = wp[1];
+        uint32_t m;
+        const uint64_t c = rec2_occ<false>(w0, w1, (int)(g - oo[rl]), rec2_kmers(w1, K), K, mask, m);
+        atomicAdd(&hist[seg_sub(c, sb)], 1u);
+      }
+      __syncthreads();
+    }
+  } else {
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
+      for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
+  }
   __syncthreads();
   uint64_t* dst = counts + (uint64_t)bin * ns;
   for (int s = threadIdx.x; s < ns; s += P3_WG)
@@ -275,10 +346,13 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   __shared__ uint32_t cnt[1 << P3_PART_BITS], start[1 << P3_PART_BITS];
   __shared__ uint64_t base[1 << P3_PART_BITS];
   __shared__ uint64_t sh4[4];
+  __shared__ uint8_t own[RW == 2 ? P3_WG * OWN_KMERS : 1];
+  __shared__ uint16_t oo[RW == 2 ? P3_WG : 1];
   int bin;
   uint64_t r0, r1;
   if (!p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
   const int np = 1 << pb, psh = sb - pb;
+  const uint64_t kmask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
   const uint64_t c0 = chunk_off[bin], nch = chunk_off[bin + 1] - c0, c = (uint64_t)blockIdx.x - c0;
   for (int p = threadIdx.x; p < np; p += P3_WG) {
     cnt[p] = 0;
@@ -295,9 +369,32 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
     }
     uint64_t T;
     const uint64_t o = block_excl_scan_u64(n, sh4, &T);
+    bool per_kmer = false;
+    if constexpr (RW == 2) {
+      per_kmer = !__syncthreads_or(n > OWN_KMERS);
+      if (per_kmer) {
+        oo[threadIdx.x] = (uint16_t)o;
+        for (uint32_t q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
+      }
+    }
     for (uint64_t w0 = 0; w0 < T; w0 += TILE) {
       const uint32_t tn = (uint32_t)(T - w0 < (uint64_t)TILE ? T - w0 : (uint64_t)TILE);
-      if (n && o < w0 + tn && o + n > w0) {
+      if constexpr (RW == 2) {
+        if (per_kmer) {
+          __syncthreads();  // owner map complete
+          for (uint32_t t = threadIdx.x; t < tn; t += P3_WG) {
+            const uint32_t g = (uint32_t)w0 + t, rl = own[g];
+            const uint64_t* wp = rec + (rr + rl) * 2;
+            const uint64_t x0 = wp[0], x1 = wp[1];
+            uint32_t m;
+            const uint64_t key = rec2_occ<true>(x0, x1, (int)(g - oo[rl]), rec2_kmers(x1, K), K, kmask, m);
+            tk[t] = (KeyT)key;
+            tm[t] = (uint16_t)m;
+            tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
+          }
+        }
+      }
+      if (!per_kmer && n && o < w0 + tn && o + n > w0) {
         uint64_t j = o;
         for_each_rec_occ<KeyT, RW, true>(rec, r, K, [&](const KeyT& key, uint32_t m) {
           if (j >= w0 && j < w0 + tn) {
@@ -419,14 +516,18 @@ __device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_o
 // block-wide: the seen / collision bitmaps are double-buffered (A for even
 // levels, B for odd ones), so it takes two barriers, after marking and after
 // placing (which also clears the next level's buffers and counts the keys
-// left). The last TAIL_KEYS or fewer keys are listed in LDS and wave 0
-// finishes their levels alone, one key per lane, without block barriers (the
-// LDS operations of one wave execute in order). seenA / colA must be clear
-// for level 0.
-constexpr int TAIL_KEYS = 64;
-struct BbhTail {
-  uint64_t h[TAIL_KEYS], d[TAIL_KEYS];
-  uint32_t e[TAIL_KEYS];
+// left). The last TAIL_KEYS or fewer keys go to a dense list in LDS (level
+// state, step, element) that wave 0 walks 64 keys at a time, with no block
+// barriers (the LDS operations of one wave execute in order); each level
+// compacts the unplaced keys to the front of the list. seenA / colA must be
+// clear for level 0.
+constexpr int TAIL_KEYS = 256;
+struct BbhList {
+  uint64_t* h;  // level state
+  uint64_t* d;  // step
+  uint16_t* e;  // element
+};
+struct BbhShared {
   uint32_t left[2][RK_WG / 64];
   uint64_t off;
 };
@@ -452,7 +553,8 @@ __device__ __forceinline__ void wave_lds_order() {
 
 template <typename KeyT, int KPT>
 __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
-                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhTail* tl, unsigned int* err) {
+                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhList L, BbhShared* sh,
+                               unsigned int* err) {
   uint64_t h[KPT], d[KPT];  // level state h1 + l * d of each slot (device.hpp)
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
@@ -507,11 +609,11 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
       }
     }
     const uint32_t wl = wave_sum4((uint32_t)__popc(alive));
-    if (__lane_id() == 0) tl->left[l & 1][wv] = wl;
+    if (__lane_id() == 0) sh->left[l & 1][wv] = wl;
     __syncthreads();
     total = 0;
 #pragma unroll
-    for (int w = 0; w < RK_WG / 64; w++) total += tl->left[l & 1][w];
+    for (int w = 0; w < RK_WG / 64; w++) total += sh->left[l & 1][w];
 #pragma unroll
     for (int s = 0; s < KPT; s++) h[s] += d[s];
     uint64_t* ts = seen;
@@ -529,38 +631,35 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
   // a fresh count when the segment starts small), then wave 0 alone.
   if (l == 0) {
     const uint32_t wl = wave_sum4((uint32_t)__popc(alive));
-    if (__lane_id() == 0) tl->left[0][wv] = wl;
+    if (__lane_id() == 0) sh->left[0][wv] = wl;
     __syncthreads();
   }
   const int lb = l == 0 ? 0 : ((l - 1) & 1);
   uint32_t base = 0;
-  for (uint32_t w = 0; w < wv; w++) base += tl->left[lb][w];
+  for (uint32_t w = 0; w < wv; w++) base += sh->left[lb][w];
   base += wave_prefix4((uint32_t)__popc(alive));
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     if ((alive >> s) & 1) {
-      tl->h[base] = h[s];
-      tl->d[base] = d[s];
-      tl->e[base] = (uint32_t)(s * RK_WG + threadIdx.x);
+      L.h[base] = h[s];
+      L.d[base] = d[s];
+      L.e[base] = (uint16_t)(s * RK_WG + threadIdx.x);
       base++;
     }
   }
   __syncthreads();
   if (wv == 0) {
     const uint32_t lane = __lane_id();
-    bool a = lane < total;
-    uint64_t hh = a ? tl->h[lane] : 0;
-    const uint64_t dd = a ? tl->d[lane] : 0;
-    const uint32_t ee = a ? tl->e[lane] : 0;
+    const uint64_t below = (1ULL << lane) - 1;
+    uint32_t cnt = total;
     for (;;) {  // seen / col are clear for this level's n
       const uint64_t n = bbh_level_n(t);
       if (l >= BBH_MAX_LEVELS || off + n > max_bits) {
         if (lane == 0) atomicOr(err, 1u);
         break;
       }
-      uint32_t pos = 0;
-      if (a) {
-        pos = bbh_pos(hh, (uint32_t)n);
+      for (uint32_t i = lane; i < cnt; i += 64) {
+        const uint32_t pos = bbh_pos(L.h[i], (uint32_t)n);
         const unsigned long long bit = 1ULL << (pos & 63);
         const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
         if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
@@ -568,9 +667,34 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
       wave_lds_order();
       const uint32_t nw = (uint32_t)(n >> 6);
       for (uint32_t i = lane; i < nw; i += 64) region[(off >> 6) + i] = seen[i] & ~col[i];
-      if (a && (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1)) {
-        ppos[ee] = (uint32_t)(off + pos);
-        a = false;
+      // place; unplaced keys move to the front with the next level's state
+      // (a key's new index never exceeds its old one, and each 64-key chunk
+      // is read before it is written)
+      uint32_t kept = 0;
+      for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        const bool act = i < cnt;
+        uint64_t hh = 0, dd = 0;
+        uint32_t ee = 0, pos = 0;
+        bool placed = false;
+        if (act) {
+          hh = L.h[i];
+          dd = L.d[i];
+          ee = L.e[i];
+          pos = bbh_pos(hh, (uint32_t)n);
+          placed = ((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1;
+        }
+        if (placed) ppos[ee] = (uint32_t)(off + pos);
+        const bool keep = act && !placed;
+        const uint64_t km = __ballot(keep);
+        wave_lds_order();
+        if (keep) {
+          const uint32_t j = kept + (uint32_t)__popcll(km & below);
+          L.h[j] = hh + dd;
+          L.d[j] = dd;
+          L.e[j] = (uint16_t)ee;
+        }
+        kept += (uint32_t)__popcll(km);
       }
       wave_lds_order();
       const uint32_t nw2 = (uint32_t)(bbh_level_n(bbh_next_t(t)) >> 6);
@@ -579,16 +703,16 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
         col[i] = 0;
       }
       wave_lds_order();
-      hh += dd;
       off += n;
       t = bbh_next_t(t);
       l++;
-      if (__ballot(a) == 0) break;
+      cnt = kept;
+      if (cnt == 0) break;
     }
-    if (lane == 0) tl->off = off;
+    if (lane == 0) sh->off = off;
   }
   __syncthreads();
-  return tl->off;
+  return sh->off;
 }
 
 // Slot of a key in its segment's dedup hash table of 2^tb entries: the bits
@@ -626,12 +750,13 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
   static_assert(REG_W * 8 <= (int)sizeof(KeyT) * CAP && (CAP + REG_W) * 4 <= (4 << TAB_BITS), "LDS reuse fits");
   static_assert((1 << BM_MAX_BITS) <= (1 << TAB_BITS) && REG_W / 7 + 2 <= SEEN_W, "LDS reuse fits");
-  // buf: keys for the table's comparisons | region (level words) | staged keys
+  static_assert(REG_W * 8 + TAIL_KEYS * 18 <= (int)sizeof(KeyT) * CAP && CAP <= 65536, "tail list fits behind the region");
+  // buf: keys for the table's comparisons | region (level words) + tail list | staged keys
   // tab: hash table (or direct-address mask table) | ppos + word ranks | staged masks
   __shared__ KeyT buf[CAP];
   __shared__ uint32_t tab[1 << TAB_BITS];
   __shared__ uint64_t seenA[SEEN_W], colA[SEEN_W], seenB[SEEN_W], colB[SEEN_W];
-  __shared__ BbhTail tail;
+  __shared__ BbhShared bsh;
   __shared__ uint64_t sh8[8];
   unsigned int* n_over = flags;
   unsigned int* n_big = flags + 1;
@@ -763,7 +888,11 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   __syncthreads();
   const uint64_t slot = pad_slot_blocks(n);
   const uint64_t max_bits = slot * BBH_BLOCK_BITS < (uint64_t)REG_W * 64 ? slot * BBH_BLOCK_BITS : (uint64_t)REG_W * 64;
-  const uint64_t bits = bbh_levels<KeyT, KPT>(key, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, &tail, err);
+  BbhList tl;
+  tl.h = region + REG_W;
+  tl.d = tl.h + TAIL_KEYS;
+  tl.e = (uint16_t*)(tl.d + TAIL_KEYS);
+  const uint64_t bits = bbh_levels<KeyT, KPT>(key, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
   const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < RK_WG
   __syncthreads();
   // blocks {rank before, 7 words}; word ranks for the local ranks below
