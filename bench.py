@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 PHASE_OF = {
     "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
     "p2_partition": "P2", "p2_count": "P2", "p2_route": "P2",
-    "p3_count": "P3", "p3_scatter": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
+    "p3_count": "P3", "p3_scatter": "P3", "p3_split": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
     "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",
 }
@@ -84,10 +84,12 @@ def algorithmic_bytes(info, kb, rw):
       p2_count  : entries in
       p2_route  : entries in, the super k-mers' bases in, records out
       p3_count  : records in
-      p3_scatter: records in, W (key, mask) occurrences out
-      p3_dedup  : W occurrences in; D distinct (key, mask), D rank-ordered
-                  masks and the BBHash blocks out
-      p3_compact: D keys, D masks, B in; the same out (dense)
+      p3_scatter: records in, W (key, mask) occurrences out (64 parts per bin)
+      p3_split  : W occurrences in, the same out (segments)
+      p3_dedup  : W occurrences in; D distinct keys and D masks (both in
+                  local-rank order) and the BBHash blocks out
+      p3_compact: D masks, B in; the same out (dense; the dense key array
+                  is compacted only when a caller asks for the keys)
       p4_classify: D masks in, ceil(D/12) state words out
     """
     NG = info["stream_len"]
@@ -104,8 +106,9 @@ def algorithmic_bytes(info, kb, rw):
         "p2_route": 8 * S + 2 * rec,
         "p3_count": rec,
         "p3_scatter": rec + W * ob,
-        "p3_dedup": W * ob + D * ob + 2 * D + B,
-        "p3_compact": 2 * (D * kb + 2 * D + B),
+        "p3_split": 2 * W * ob,
+        "p3_dedup": W * ob + D * ob + B,
+        "p3_compact": 2 * (2 * D + B),
         "p4_classify": 2 * D + 8 * ((D + 11) // 12),
     }
 

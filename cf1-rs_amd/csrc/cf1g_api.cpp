@@ -145,6 +145,7 @@ struct cf1g_ctx {
   uint32_t n_big = 0;
   // learned index
   bool lidx_ready = false, lidx_fallback = false, lidx_states = false;
+  bool keys_ready = false;  // dense keys compacted since the last P3
   uint32_t lidx_R = 0, lidx_L = 0;
   int lidx_s = 0;
   int use_index = 0;
@@ -403,16 +404,33 @@ void do_p3(cf1g_ctx* c) {
   c->D = D;
   c->n_blocks = tot[1];
   const uint64_t nsw = (D + 11) / 12;
-  void* keys = c->get("keys", D * kb + 16);
   uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
-  launch_p3_compact(st, c->wide, kout, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, keys, vmask, blocks);
+  // masks and blocks now; the dense key array (which the reference's MPHF
+  // does not keep) is compacted from kout on first use, dense_keys()
+  launch_p3_compact(st, c->wide, kout, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, nullptr, vmask, blocks);
+  c->keys_ready = false;
   c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
   c->d2h(hf, flags, 12);
   if (hf[2]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit (flags " + std::to_string(hf[2]) + ")");
   c->lidx_ready = c->lidx_states = false;
   c->use_index = 0;
   c->stage = 4;
+}
+
+// The distinct keys in MPHF value order (keys[v] has value v), compacted
+// from the per-segment rank-order runs P3 left in occ_aux on first use.
+void* dense_keys(cf1g_ctx* c) {
+  const uint64_t kb = c->key_bytes;
+  void* keys = c->get("keys", c->D * kb + 16);
+  if (!c->keys_ready) {
+    const uint64_t ns = c->n_segs;
+    launch_p3_compact(c->st, c->wide, c->get("occ_aux", c->W * kb + 16), nullptr, c->getT<uint64_t>("seg_off", ns + 1),
+                      c->getT<uint64_t>("seg_ucount", ns + 1), c->getT<uint64_t>("seg_out_off", ns + 1), nullptr, nullptr, ns, c->k,
+                      c->sb, keys, nullptr, nullptr);
+    c->keys_ready = true;
+  }
+  return keys;
 }
 
 // P4: the joined masks, already in MPHF-value order, become packed states.
@@ -476,7 +494,7 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   // ---- global sort of the distinct keys (LSD, 8-bit digits)
   uint64_t* ka = c->getT<uint64_t>("lidx_keys", D + 1);
   uint64_t* kt = c->getT<uint64_t>("lidx_tmp", D + 1);
-  if (D) hip_check(hipMemcpyAsync(ka, c->get("keys", D * 8 + 16), D * 8, hipMemcpyDeviceToDevice, st), "D2D keys");
+  if (D) hip_check(hipMemcpyAsync(ka, dense_keys(c), D * 8, hipMemcpyDeviceToDevice, st), "D2D keys");
   const uint64_t nt = sort_tiles(D);
   uint64_t* mat = c->getT<uint64_t>("lidx_mat", 256 * nt + 1);
   uint64_t* off = c->getT<uint64_t>("lidx_off", 256 * nt + 1);
@@ -737,7 +755,7 @@ int cf1g_keys_export(cf1g_ctx* c, uint64_t* out, uint64_t n_words) {
     need(c, 4, "cf1g_keys_export");
     uint64_t need_w = c->D * (c->key_bytes / 8);
     if (!out || n_words < need_w) fail(CF1G_E_INVALID, "key buffer too small");
-    c->d2h(out, c->get("keys", c->D * c->key_bytes + 16), need_w * 8);
+    c->d2h(out, dense_keys(c), need_w * 8);
   });
 }
 
@@ -885,7 +903,9 @@ size_t cf1g_kernel_times(cf1g_ctx* c, char* buf, size_t cap, int reset) {
 
 const void* cf1g_device_keys(cf1g_ctx* c) {
   if (!c || c->stage < 4) return nullptr;
-  return c->get("keys", c->D * c->key_bytes + 16);
+  const void* p = nullptr;
+  if (guarded(c, [&]() { p = dense_keys(c); }) != CF1G_OK) return nullptr;
+  return p;
 }
 
 // ---------------------------------------------------------------- learned index
@@ -974,7 +994,7 @@ int cf1g_classify_via(cf1g_ctx* c, int index) {
     LidxView v{};
     if (index == 1) v = c->lview();
     v.D = D;
-    launch_classify_via(c->st, index == 1, c->getT<uint64_t>("keys", D + 2), c->getT<uint16_t>("vmask", 12 * nsw + 16), v, c->view(),
+    launch_classify_via(c->st, index == 1, (const uint64_t*)dense_keys(c), c->getT<uint16_t>("vmask", 12 * nsw + 16), v, c->view(),
                         st8, err);
     launch_states_pack(c->st, st8, D, c->getT<uint64_t>(index == 1 ? "states_lidx" : "states", nsw + 1));
     unsigned int he = 0;

@@ -120,7 +120,7 @@ __host__ __device__ constexpr uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 __host__ __device__ constexpr uint64_t bbh_seed(int i) { return mix64(0x243F6A8885A308D3ULL + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL); }
-constexpr uint64_t BBH_S1 = bbh_seed(0), BBH_S2 = bbh_seed(1);
+constexpr uint64_t BBH_S1 = bbh_seed(0);
 constexpr uint64_t FIB_MUL = 0x9E3779B97F4A7C15ULL, WIDE_MUL = 0xC2B2AE3D27D4EB4FULL;
 
 // Segment of a key inside its bin: the top sb bits of a multiplicative
@@ -132,14 +132,15 @@ __device__ __forceinline__ uint32_t seg_sub(const KeyT& k, int sb) {
   return sb ? (uint32_t)(seg_hash(k) >> (64 - sb)) : 0u;
 }
 // BBHash level hashes: one 64-bit state per key and level, g_l = h1 + l * d
-// (mod 2^64), h1 = mix64 of the key with seed S1, d = mix64(h1 ^ S2) | 1.
-// The position among n < 2^32 bits is floor(x * n / 2^32) with
-// x = fmix32(hi32(g_l) ^ lo32(g_l)); the fmix32 step keeps the levels
-// independent (top bits of g_l alone would keep two keys with close h1 and
-// close d on one bit at every level).
-__device__ __forceinline__ uint64_t bbh_h1(uint64_t k) { return mix64(k ^ BBH_S1); }
-__device__ __forceinline__ uint64_t bbh_h1(const Key128& k) { return mix64(k.lo ^ mix64(k.hi ^ BBH_S1)); }
-__device__ __forceinline__ uint64_t bbh_d(uint64_t h1) { return mix64(h1 ^ BBH_S2) | 1ULL; }
+// (mod 2^64), h1 = (key ^ S1) * M1 (wide keys: (lo ^ hi * WIDE_MUL ^ S1) *
+// M1), d = rotl(h1, 21) | 1. The position among n < 2^32 bits is
+// floor(x * n / 2^32) with x = fmix32(hi32(g_l) ^ lo32(g_l)); the fmix32 step
+// keeps the levels independent (top bits of g_l alone would keep two keys
+// with close h1 and close d on one bit at every level).
+constexpr uint64_t BBH_M1 = 0xD6E8FEB86659FD93ULL;
+__device__ __forceinline__ uint64_t bbh_h1(uint64_t k) { return (k ^ BBH_S1) * BBH_M1; }
+__device__ __forceinline__ uint64_t bbh_h1(const Key128& k) { return (k.lo ^ (k.hi * WIDE_MUL) ^ BBH_S1) * BBH_M1; }
+__device__ __forceinline__ uint64_t bbh_d(uint64_t h1) { return ((h1 << 21) | (h1 >> 43)) | 1ULL; }
 __device__ __forceinline__ uint32_t bbh_pos(uint64_t g, uint32_t n) {
   const uint32_t x = fmix32((uint32_t)(g >> 32) ^ (uint32_t)g);
   return __umulhi(x, n);

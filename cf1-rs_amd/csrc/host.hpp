@@ -25,10 +25,10 @@ inline uint64_t h_seg_hash(uint64_t lo, uint64_t hi, bool wide) {
   return (lo ^ (wide ? hi * 0xC2B2AE3D27D4EB4FULL : 0)) * 0x9E3779B97F4A7C15ULL;
 }
 inline uint64_t h_bbh_h1(uint64_t lo, uint64_t hi, bool wide) {
-  const uint64_t s = h_seed(0);
-  return wide ? h_mix64(lo ^ h_mix64(hi ^ s)) : h_mix64(lo ^ s);
+  const uint64_t s = h_seed(0), m1 = 0xD6E8FEB86659FD93ULL;
+  return wide ? (lo ^ (hi * 0xC2B2AE3D27D4EB4FULL) ^ s) * m1 : (lo ^ s) * m1;
 }
-inline uint64_t h_bbh_d(uint64_t h1) { return h_mix64(h1 ^ h_seed(1)) | 1ULL; }
+inline uint64_t h_bbh_d(uint64_t h1) { return ((h1 << 21) | (h1 >> 43)) | 1ULL; }
 
 inline uint32_t h_fmix32(uint32_t h) {
   h ^= h >> 16;

@@ -449,14 +449,34 @@ __global__ void __launch_bounds__(P3_WG) k_p3_split(const KeyT* __restrict__ src
   __syncthreads();
   const uint64_t a = seg_off[s0], b = seg_off[s0 + spp];
   const uint32_t mask = (uint32_t)spp - 1;
+  // the next tile is loaded into registers while the current one is sorted
+  // and stored
+  constexpr int PT = TILE / P3_WG;
+  KeyT nk[PT];
+  uint16_t nm[PT];
+  auto fetch = [&](uint64_t t0) {
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const uint64_t i = t0 + (uint64_t)q * P3_WG + threadIdx.x;
+      if (i < b) {
+        nk[q] = src[i];
+        nm[q] = src_obs[i];
+      }
+    }
+  };
+  if (a < b) fetch(a);
   for (uint64_t t0 = a; t0 < b; t0 += TILE) {
     const uint32_t tn = (uint32_t)(b - t0 < (uint64_t)TILE ? b - t0 : (uint64_t)TILE);
-    for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
-      const KeyT key = src[t0 + i];
-      tk[i] = key;
-      tm[i] = src_obs[t0 + i];
-      tp[i] = (uint8_t)(seg_sub(key, sb) & mask);
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const uint32_t i = (uint32_t)q * P3_WG + threadIdx.x;
+      if (i < tn) {
+        tk[i] = nk[q];
+        tm[i] = nm[q];
+        tp[i] = (uint8_t)(seg_sub(nk[q], sb) & mask);
+      }
     }
+    if (t0 + TILE < b) fetch(t0 + TILE);
     __syncthreads();
     stage_sort(tn, tp, rk, inv, cnt, start, spp);
     for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
@@ -1088,9 +1108,10 @@ __global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, ui
 }
 
 // ------------------------------------------------------------------ compaction
-// One work-group per segment copies the segment's distinct keys and masks
-// (local-rank order, kout / vpad) to their MPHF values in keys / vmask, and
-// its BBHash blocks from the padded slot to the dense block order (blk_off).
+// One work-group per segment copies the segment's masks (local-rank order,
+// vpad) to their MPHF values in vmask and its BBHash blocks from the padded
+// slot to the dense block order (blk_off); or, with keys != nullptr, only its
+// distinct keys (local-rank order, kout) to their MPHF values in keys.
 template <typename KeyT>
 __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ kout, const uint16_t* __restrict__ vpad,
                                                       const uint64_t* __restrict__ seg_off,
@@ -1102,10 +1123,11 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ k
   if (seg >= n_segs) return;
   const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
   if (u == 0) return;
-  for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
-    keys[ko + i] = kout[s0 + i];
-    vmask[ko + i] = vpad[s0 + i];
+  if (keys) {
+    for (uint64_t i = threadIdx.x; i < u; i += RK_WG) keys[ko + i] = kout[s0 + i];
+    return;
   }
+  for (uint64_t i = threadIdx.x; i < u; i += RK_WG) vmask[ko + i] = vpad[s0 + i];
   const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
   const uint64_t* src = blocks_pad + pad_slot_off(s0, seg) * 8;
   for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) blocks[b0 * 8 + i] = src[i];
