@@ -13,21 +13,26 @@ __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x) {
   }
   return x;
 }
-// Exclusive scan across a 256-thread block; *total gets the block sum.
-__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* sh4, uint64_t* total) {
+// Exclusive scan across a block of WG threads (sh: WG/64 words); *total
+// gets the block sum.
+template <int WG>
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* sh, uint64_t* total) {
   uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
   uint64_t x = wave_incl_scan_u64(v);
-  if (lane == 63) sh4[wv] = x;
+  if (lane == 63) sh[wv] = x;
   __syncthreads();
   uint64_t off = 0, tot = 0;
-  for (int q = 0; q < 4; q++) {
-    uint64_t t = sh4[q];
+  for (int q = 0; q < WG / 64; q++) {
+    uint64_t t = sh[q];
     if (q < (int)wv) off += t;
     tot += t;
   }
   __syncthreads();
   if (total) *total = tot;
   return off + x - v;
+}
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* sh4, uint64_t* total) {
+  return block_excl_scan<256>(v, sh4, total);
 }
 
 // ------------------------------------------------------------------ tile ranking

@@ -509,13 +509,17 @@ struct SegCfg<Key128> {
   static constexpr int KPT = 8;
 };
 constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12 bits (k <= 5)
+// k_p3_dedup: DD_WG threads, DD_KPT occurrences each, at most DD_CAP per segment
+constexpr int DD_WG = 256;
+constexpr int DD_KPT = 8;
+constexpr int DD_CAP = DD_WG * DD_KPT;
 
 // A segment's MPHF is built by k_bbh_big when its distinct keys exceed the
 // register capacity, or when it was sorted by k_p3_bigsort (more
 // occurrences than capacity, keys too wide for the direct-address table).
 template <typename KeyT>
 __device__ __forceinline__ bool seg_is_big(uint64_t n, uint64_t u, int K2) {
-  constexpr uint64_t CAP = (uint64_t)RK_WG * SegCfg<KeyT>::KPT;
+  constexpr uint64_t CAP = DD_CAP;
   const bool table = sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS;
   return u > CAP || (n > CAP && !table);
 }
@@ -530,9 +534,9 @@ __device__ __forceinline__ uint64_t pad_slot_off(uint64_t seg_start, uint64_t se
 __device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_occ / 64 + PAD_BLOCKS; }
 
 // ---- BBHash levels over keys held in registers (see device.hpp), key slot s
-// of thread t being element s*RK_WG + t. Writes the level words into
-// `region` and each element's bit position into ppos[element] (LDS); returns
-// the region size in bits. While more than TAIL_KEYS keys remain a level is
+// of thread t being element s*WG + t. Writes the level words into `region`
+// and each element's bit position into ppos[element] (LDS); returns the
+// region size in bits. While more than TAIL_KEYS keys remain a level is
 // block-wide: the seen / collision bitmaps are double-buffered (A for even
 // levels, B for odd ones), so it takes two barriers, after marking and after
 // placing (which also clears the next level's buffers and counts the keys
@@ -547,8 +551,9 @@ struct BbhList {
   uint64_t* d;  // step
   uint16_t* e;  // element
 };
+template <int WG>
 struct BbhShared {
-  uint32_t left[2][RK_WG / 64];
+  uint32_t left[2][WG / 64];
   uint64_t off;
 };
 // Sum over the wave of a per-lane count in [0, 15].
@@ -571,20 +576,12 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
 }
 
-template <typename KeyT, int KPT>
-__device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
-                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhList L, BbhShared* sh,
-                               unsigned int* err) {
-  uint64_t h[KPT], d[KPT];  // level state h1 + l * d of each slot (device.hpp)
-#pragma unroll
-  for (int s = 0; s < KPT; s++) {
-    if ((alive >> s) & 1) {
-      h[s] = bbh_h1(key[s]);
-      d[s] = bbh_d(h[s]);
-    } else {
-      h[s] = d[s] = 0;
-    }
-  }
+template <int KPT, int WG>
+__device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
+                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhList L,
+                               BbhShared<WG>* sh, unsigned int* err) {
+  // level l state of a slot: h1 + l * bbh_d(h1) (device.hpp), recomputed per
+  // level so that only h1 stays in registers
   const uint32_t wv = threadIdx.x >> 6;
   uint64_t off = 0, t = D;
   int l = 0;
@@ -603,7 +600,7 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if ((alive >> s) & 1) {
-        const uint32_t pos = bbh_pos(h[s], (uint32_t)n);
+        const uint32_t pos = bbh_pos(h1[s] + (uint64_t)l * bbh_d(h1[s]), (uint32_t)n);
         pp[s] = pos;
         const unsigned long long bit = 1ULL << (pos & 63);
         const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
@@ -612,9 +609,9 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
     }
     __syncthreads();
     const uint32_t nw = (uint32_t)(n >> 6);
-    for (uint32_t i = threadIdx.x; i < nw; i += RK_WG) region[(off >> 6) + i] = seen[i] & ~col[i];
+    for (uint32_t i = threadIdx.x; i < nw; i += WG) region[(off >> 6) + i] = seen[i] & ~col[i];
     const uint32_t nw2 = (uint32_t)(bbh_level_n(bbh_next_t(t)) >> 6);
-    for (uint32_t i = threadIdx.x; i < nw2; i += RK_WG) {
+    for (uint32_t i = threadIdx.x; i < nw2; i += WG) {
       nseen[i] = 0;
       ncol[i] = 0;
     }
@@ -624,7 +621,7 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
         const uint32_t pos = pp[s];
         if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
           alive &= ~(1u << s);
-          ppos[s * RK_WG + threadIdx.x] = (uint32_t)(off + pos);
+          ppos[s * WG + threadIdx.x] = (uint32_t)(off + pos);
         }
       }
     }
@@ -633,9 +630,7 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
     __syncthreads();
     total = 0;
 #pragma unroll
-    for (int w = 0; w < RK_WG / 64; w++) total += sh->left[l & 1][w];
-#pragma unroll
-    for (int s = 0; s < KPT; s++) h[s] += d[s];
+    for (int w = 0; w < WG / 64; w++) total += sh->left[l & 1][w];
     uint64_t* ts = seen;
     seen = nseen;
     nseen = ts;
@@ -661,9 +656,10 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     if ((alive >> s) & 1) {
-      L.h[base] = h[s];
-      L.d[base] = d[s];
-      L.e[base] = (uint16_t)(s * RK_WG + threadIdx.x);
+      const uint64_t d = bbh_d(h1[s]);
+      L.h[base] = h1[s] + (uint64_t)l * d;
+      L.d[base] = d;
+      L.e[base] = (uint16_t)(s * WG + threadIdx.x);
       base++;
     }
   }
@@ -736,48 +732,57 @@ __device__ uint64_t bbh_levels(const KeyT (&key)[KPT], uint32_t alive, uint64_t 
 }
 
 // Slot of a key in its segment's dedup hash table of 2^tb entries: the bits
-// of seg_hash just below the segment bits (uniform inside a segment).
+// of seg_hash just below the segment bits (uniform inside a segment); the
+// next 10 bits are the key's fingerprint in its table entry.
 template <typename KeyT>
-__device__ __forceinline__ uint32_t tab_slot(const KeyT& k, int sb, int tb) {
-  return (uint32_t)((seg_hash(k) << sb) >> (64 - tb));
+__device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t& slot, uint32_t& fp) {
+  const uint64_t h = seg_hash(k) << sb;
+  slot = (uint32_t)(h >> (64 - tb));
+  fp = (uint32_t)((h << tb) >> 54);
 }
 
-// Dedup of one segment and its BBHash. Each occurrence carries its 10-bit
-// observation mask; the masks of equal keys are OR-joined (the DFA join of
-// classify.rs:147-518). The distinct keys are found with an LDS hash table
-// (linear probing, entries = representative occurrence + 1 | joined mask
-// << 12); which occurrence represents a key depends on timing, but the set of
-// keys, their joined masks and the BBHash built over them do not. Output: the
-// distinct keys in MPHF value order at kout[start + local rank], their masks
-// at vpad[start + local rank], their count in ucount, the BBHash blocks in
-// the segment's padded slot of blocks_pad and their count in nblk. Segments
-// with more occurrences than CAP go to over_list (sorted by k_p3_bigsort);
-// segments whose distinct keys exceed CAP go to big_list (BBHash built by
-// k_bbh_big). Keys of <= 12 bits (k <= 6) use a direct-address mask table
-// instead, which takes any number of occurrences.
+// Dedup of one segment and its BBHash, DD_WG threads holding DD_KPT
+// occurrences each (element e = slot * DD_WG + thread). Each occurrence
+// carries its 10-bit observation mask; the masks of equal keys are OR-joined
+// (the DFA join of classify.rs:147-518). The distinct keys are found with an
+// LDS hash table (linear probing, entries = representative occurrence + 1 |
+// joined mask << 12 | 10-bit key fingerprint << 22; keys are compared, from
+// occ, only when the fingerprints match); which occurrence represents a key
+// depends on timing, but the set of keys, their joined masks and the BBHash
+// built over them do not. Output: the distinct keys in MPHF value order at
+// kout[start + local rank], their masks at vpad[start + local rank], their
+// count in ucount, the BBHash blocks in the segment's padded slot of
+// blocks_pad and their count in nblk. Segments with more occurrences than
+// DD_CAP go to over_list (sorted by k_p3_bigsort); segments whose distinct
+// keys exceed DD_CAP go to big_list (BBHash built by k_bbh_big). Keys of <= 12
+// bits (k <= 6) use a direct-address mask table instead, which takes any
+// number of occurrences.
 template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+__global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
                                                     const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K2, int sb,
                                                     uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
                                                     uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
                                                     uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad,
                                                     KeyT* __restrict__ kout, uint16_t* __restrict__ vpad) {
-  constexpr int KPT = SegCfg<KeyT>::KPT;
-  constexpr int CAP = RK_WG * KPT;
+  constexpr int WG = DD_WG;
+  constexpr int KPT = DD_KPT;
+  constexpr int CAP = DD_CAP;
   constexpr int TAB_BITS = 12;
   constexpr int SEEN_W = (2 * CAP) / 64 + 1;
   constexpr int REG_W = (4 * CAP + 8192) / 64;
+  constexpr int TAB_BYTES = (4 << TAB_BITS) > (int)sizeof(KeyT) * CAP ? (4 << TAB_BITS) : (int)sizeof(KeyT) * CAP;
+  constexpr int AUX_BYTES = REG_W * 8 + TAIL_KEYS * 18;
   static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
-  static_assert(REG_W * 8 <= (int)sizeof(KeyT) * CAP && (CAP + REG_W) * 4 <= (4 << TAB_BITS), "LDS reuse fits");
-  static_assert((1 << BM_MAX_BITS) <= (1 << TAB_BITS) && REG_W / 7 + 2 <= SEEN_W, "LDS reuse fits");
-  static_assert(REG_W * 8 + TAIL_KEYS * 18 <= (int)sizeof(KeyT) * CAP && CAP <= 65536, "tail list fits behind the region");
-  // buf: keys for the table's comparisons | region (level words) + tail list | staged keys
-  // tab: hash table (or direct-address mask table) | ppos + word ranks | staged masks
-  __shared__ KeyT buf[CAP];
-  __shared__ uint32_t tab[1 << TAB_BITS];
+  static_assert((CAP + REG_W) * 4 <= TAB_BYTES && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
+  static_assert(REG_W / 7 + 2 <= SEEN_W && CAP * 2 <= AUX_BYTES && CAP <= 65536, "LDS reuse fits");
+  // tab: hash table (or direct-address mask table) | ppos + word ranks | staged keys
+  // aux: region (level words) + tail list | staged masks
+  __shared__ __align__(16) uint8_t tab_b[TAB_BYTES];
+  __shared__ __align__(16) uint8_t aux_b[AUX_BYTES];
   __shared__ uint64_t seenA[SEEN_W], colA[SEEN_W], seenB[SEEN_W], colB[SEEN_W];
-  __shared__ BbhShared bsh;
-  __shared__ uint64_t sh8[8];
+  __shared__ BbhShared<WG> bsh;
+  __shared__ uint64_t sh8[WG / 64];
+  uint32_t* tab = (uint32_t*)tab_b;
   unsigned int* n_over = flags;
   unsigned int* n_big = flags + 1;
   unsigned int* err = flags + 2;
@@ -800,17 +805,17 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     // ---- direct-address table of masks over all 2^K2 keys: any number of
     // occurrences; distinct keys come out sorted at occ / obs[start...].
     const uint32_t nt = 1u << K2;
-    for (uint32_t i = threadIdx.x; i < nt; i += RK_WG) tab[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nt; i += WG) tab[i] = 0;
     __syncthreads();
-    for (uint64_t i = threadIdx.x; i < n; i += RK_WG) atomicOr(&tab[key_lo(occ[start + i])], (uint32_t)obs[start + i]);
+    for (uint64_t i = threadIdx.x; i < n; i += WG) atomicOr(&tab[key_lo(occ[start + i])], (uint32_t)obs[start + i]);
     __syncthreads();
-    const uint32_t per = (nt + RK_WG - 1) / RK_WG;
+    const uint32_t per = (nt + WG - 1) / WG;
     const uint32_t t0 = threadIdx.x * per;
     uint64_t c = 0;
     for (uint32_t q = 0; q < per; q++)
       if (t0 + q < nt && tab[t0 + q]) c++;
     uint64_t tot;
-    uint64_t o = block_excl_scan_u64(c, sh8, &tot);
+    uint64_t o = block_excl_scan<WG>(c, sh8, &tot);
     for (uint32_t q = 0; q < per; q++) {
       if (t0 + q >= nt) break;
       uint32_t m = tab[t0 + q];
@@ -832,7 +837,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
-      const uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+      const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < u) {
         key[s] = occ[start + e];
         msk[s] = obs[start + e];
@@ -854,31 +859,37 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     uint32_t pend = 0;
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
-      const uint64_t e = (uint64_t)s * RK_WG + threadIdx.x;
+      const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < n) {
         key[s] = occ[start + e];
         msk[s] = obs[start + e];
-        buf[e] = key[s];
         pend |= 1u << s;
       }
     }
-    for (uint32_t i = threadIdx.x; i <= tmask; i += RK_WG) tab[i] = 0;
+    for (uint32_t i = threadIdx.x; i <= tmask; i += WG) tab[i] = 0;
     __syncthreads();
-    uint32_t hs[KPT];
+    uint32_t hs[KPT], ent[KPT];  // probe slot; this occurrence's entry
 #pragma unroll
-    for (int s = 0; s < KPT; s++) hs[s] = ((pend >> s) & 1) ? tab_slot(key[s], sb, tb) : 0u;
+    for (int s = 0; s < KPT; s++) {
+      hs[s] = ent[s] = 0;
+      if ((pend >> s) & 1) {
+        uint32_t fp;
+        tab_slot(key[s], sb, tb, hs[s], fp);
+        ent[s] = (uint32_t)(s * WG + threadIdx.x + 1) | (msk[s] << 12) | (fp << 22);
+      }
+    }
     while (pend) {
       uint32_t old[KPT];
 #pragma unroll
       for (int s = 0; s < KPT; s++)
-        if ((pend >> s) & 1) old[s] = atomicCAS(&tab[hs[s]], 0u, (uint32_t)(s * RK_WG + threadIdx.x + 1) | (msk[s] << 12));
+        if ((pend >> s) & 1) old[s] = atomicCAS(&tab[hs[s]], 0u, ent[s]);
 #pragma unroll
       for (int s = 0; s < KPT; s++) {
         if ((pend >> s) & 1) {
           if (old[s] == 0) {
             alive |= 1u << s;
             pend &= ~(1u << s);
-          } else if (key_eq(buf[(old[s] & 0xFFFu) - 1], key[s])) {
+          } else if ((old[s] >> 22) == (ent[s] >> 22) && key_eq(occ[start + (old[s] & 0xFFFu) - 1], key[s])) {
             atomicOr(&tab[hs[s]], msk[s] << 12);
             pend &= ~(1u << s);
           } else {
@@ -890,18 +901,18 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     __syncthreads();  // every mask is joined
 #pragma unroll
     for (int s = 0; s < KPT; s++)
-      if ((alive >> s) & 1) msk[s] = tab[hs[s]] >> 12;
+      if ((alive >> s) & 1) msk[s] = (tab[hs[s]] >> 12) & 0x3FFu;
     uint64_t tot;
-    block_excl_scan_u64((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab and buf are free after it
+    block_excl_scan<WG>((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab is free after it
     u = tot;
     if (threadIdx.x == 0) ucount[seg] = u;
   }
   // ---- BBHash of the segment, into its padded block slot.
-  uint64_t* region = (uint64_t*)buf;
+  uint64_t* region = (uint64_t*)aux_b;
   uint32_t* ppos = tab;
   uint32_t* wrank = tab + CAP;
-  for (int i = threadIdx.x; i < REG_W; i += RK_WG) region[i] = 0;
-  for (int i = threadIdx.x; i < SEEN_W; i += RK_WG) {
+  for (int i = threadIdx.x; i < REG_W; i += WG) region[i] = 0;
+  for (int i = threadIdx.x; i < SEEN_W; i += WG) {
     seenA[i] = 0;
     colA[i] = 0;
   }
@@ -912,8 +923,11 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   tl.h = region + REG_W;
   tl.d = tl.h + TAIL_KEYS;
   tl.e = (uint16_t*)(tl.d + TAIL_KEYS);
-  const uint64_t bits = bbh_levels<KeyT, KPT>(key, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
-  const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < RK_WG
+  uint64_t h1[KPT];
+#pragma unroll
+  for (int s = 0; s < KPT; s++) h1[s] = ((alive >> s) & 1) ? bbh_h1(key[s]) : 0;
+  const uint64_t bits = bbh_levels<KPT, WG>(h1, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
+  const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < WG
   __syncthreads();
   // blocks {rank before, 7 words}; word ranks for the local ranks below
   const uint64_t j = threadIdx.x;
@@ -926,7 +940,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     pc += (uint64_t)__popcll(w[q]);
   }
   uint64_t tot;
-  const uint64_t rk = block_excl_scan_u64(j < nb ? pc : 0, sh8, &tot);
+  const uint64_t rk = block_excl_scan<WG>(j < nb ? pc : 0, sh8, &tot);
   if (j < nb) {
     uint64_t* B = blocks_pad + (pad_slot_off(start, seg) + j) * 8;
     B[0] = rk;
@@ -946,7 +960,7 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   for (int s = 0; s < KPT; s++) {
     rr[s] = ~0u;
     if ((alive >> s) & 1) {
-      const uint32_t p = ppos[s * RK_WG + threadIdx.x];
+      const uint32_t p = ppos[s * WG + threadIdx.x];
       const uint32_t wi = p >> 6;
       const uint32_t r = p < bits ? wrank[wi] + (uint32_t)__popcll(region[wi] & ((1ULL << (p & 63)) - 1)) : ~0u;
       if (r < u) rr[s] = r;
@@ -954,17 +968,17 @@ __global__ void __launch_bounds__(RK_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
   }
   __syncthreads();
-  KeyT* stage_k = buf;
-  uint16_t* stage_m = (uint16_t*)tab;
+  KeyT* stage_k = (KeyT*)tab_b;
+  uint16_t* stage_m = (uint16_t*)aux_b;
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     if (rr[s] != ~0u) {
-      stage_k[rr[s]] = key[s];
+      stage_k[rr[s]] = occ[start + (uint64_t)s * WG + threadIdx.x];  // the key of this slot (occ is unchanged)
       stage_m[rr[s]] = (uint16_t)msk[s];
     }
   }
   __syncthreads();
-  for (uint64_t i = threadIdx.x; i < u; i += RK_WG) {
+  for (uint64_t i = threadIdx.x; i < u; i += WG) {
     kout[start + i] = stage_k[i];
     vpad[start + i] = stage_m[i];
   }
@@ -1318,10 +1332,10 @@ void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const 
                      uint64_t* blocks_pad, void* kout, uint16_t* vpad) {
   if (n_segs == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (uint64_t*)occ, obs, seg_off, n_segs,
+    CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (uint64_t*)occ, obs, seg_off, n_segs,
                 2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, (uint64_t*)kout, vpad);
   else
-    CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (Key128*)occ, obs, seg_off, n_segs,
+    CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (Key128*)occ, obs, seg_off, n_segs,
                 2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, (Key128*)kout, vpad);
 }
 void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void* aux, uint16_t* aux_obs, const uint64_t* seg_off,
