@@ -771,17 +771,27 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   constexpr int SEEN_W = (2 * CAP) / 64 + 1;
   constexpr int REG_W = (4 * CAP + 8192) / 64;
   constexpr int TAB_BYTES = (4 << TAB_BITS) > (int)sizeof(KeyT) * CAP ? (4 << TAB_BITS) : (int)sizeof(KeyT) * CAP;
-  constexpr int AUX_BYTES = REG_W * 8 + TAIL_KEYS * 18;
+  // tab, byte offsets: hash table (or direct-address mask table) over all of
+  // it; then ppos [0, 4 CAP), word ranks, seen / collision bitmaps and the
+  // tail list; then the staged keys (all of it)
+  constexpr int T_WRANK = 4 * CAP, T_SEEN = T_WRANK + 4 * REG_W, T_LIST = T_SEEN + 4 * 8 * SEEN_W;
+  constexpr int T_END = T_LIST + TAIL_KEYS * 18;
+  // aux: region (level words), the block-scan and level-count words behind
+  // it; then the staged masks (all of it)
+  constexpr int A_SH = REG_W * 8, A_BSH = A_SH + 8 * (WG / 64);
+  constexpr int A_END = A_BSH + (int)sizeof(BbhShared<WG>);
+  constexpr int AUX_BYTES = A_END > CAP * 2 ? A_END : CAP * 2;
   static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
-  static_assert((CAP + REG_W) * 4 <= TAB_BYTES && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
-  static_assert(REG_W / 7 + 2 <= SEEN_W && CAP * 2 <= AUX_BYTES && CAP <= 65536, "LDS reuse fits");
-  // tab: hash table (or direct-address mask table) | ppos + word ranks | staged keys
-  // aux: region (level words) + tail list | staged masks
+  static_assert(T_END <= TAB_BYTES && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
+  static_assert(REG_W / 7 + 2 <= SEEN_W && CAP <= 65536 && T_SEEN % 8 == 0 && T_LIST % 8 == 0, "LDS reuse fits");
   __shared__ __align__(16) uint8_t tab_b[TAB_BYTES];
   __shared__ __align__(16) uint8_t aux_b[AUX_BYTES];
-  __shared__ uint64_t seenA[SEEN_W], colA[SEEN_W], seenB[SEEN_W], colB[SEEN_W];
-  __shared__ BbhShared<WG> bsh;
-  __shared__ uint64_t sh8[WG / 64];
+  uint64_t* seenA = (uint64_t*)(tab_b + T_SEEN);
+  uint64_t* colA = seenA + SEEN_W;
+  uint64_t* seenB = colA + SEEN_W;
+  uint64_t* colB = seenB + SEEN_W;
+  uint64_t* sh8 = (uint64_t*)(aux_b + A_SH);
+  BbhShared<WG>& bsh = *(BbhShared<WG>*)(aux_b + A_BSH);
   uint32_t* tab = (uint32_t*)tab_b;
   unsigned int* n_over = flags;
   unsigned int* n_big = flags + 1;
@@ -860,23 +870,21 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
-      if (e < n) {
-        key[s] = occ[start + e];
-        msk[s] = obs[start + e];
-        pend |= 1u << s;
-      }
+      const bool v = e < n;
+      key[s] = v ? occ[start + e] : KeyT{};
+      msk[s] = v ? (uint32_t)obs[start + e] : 0u;
+      pend |= (uint32_t)v << s;
     }
     for (uint32_t i = threadIdx.x; i <= tmask; i += WG) tab[i] = 0;
     __syncthreads();
     uint32_t hs[KPT], ent[KPT];  // probe slot; this occurrence's entry
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
-      hs[s] = ent[s] = 0;
-      if ((pend >> s) & 1) {
-        uint32_t fp;
-        tab_slot(key[s], sb, tb, hs[s], fp);
-        ent[s] = (uint32_t)(s * WG + threadIdx.x + 1) | (msk[s] << 12) | (fp << 22);
-      }
+      uint32_t slot = 0, fp = 0;
+      tab_slot(key[s], sb, tb, slot, fp);
+      const bool v = (pend >> s) & 1;
+      hs[s] = v ? slot : 0u;
+      ent[s] = v ? ((uint32_t)(s * WG + threadIdx.x + 1) | (msk[s] << 12) | (fp << 22)) : 0u;
     }
     while (pend) {
       uint32_t old[KPT];
@@ -910,7 +918,7 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   // ---- BBHash of the segment, into its padded block slot.
   uint64_t* region = (uint64_t*)aux_b;
   uint32_t* ppos = tab;
-  uint32_t* wrank = tab + CAP;
+  uint32_t* wrank = (uint32_t*)(tab_b + T_WRANK);
   for (int i = threadIdx.x; i < REG_W; i += WG) region[i] = 0;
   for (int i = threadIdx.x; i < SEEN_W; i += WG) {
     seenA[i] = 0;
@@ -920,7 +928,7 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   const uint64_t slot = pad_slot_blocks(n);
   const uint64_t max_bits = slot * BBH_BLOCK_BITS < (uint64_t)REG_W * 64 ? slot * BBH_BLOCK_BITS : (uint64_t)REG_W * 64;
   BbhList tl;
-  tl.h = region + REG_W;
+  tl.h = (uint64_t*)(tab_b + T_LIST);
   tl.d = tl.h + TAIL_KEYS;
   tl.e = (uint16_t*)(tl.d + TAIL_KEYS);
   uint64_t h1[KPT];
