@@ -850,7 +850,6 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < u) {
         key[s] = occ[start + e];
-        msk[s] = obs[start + e];
         alive |= 1u << s;
       }
     }
@@ -907,9 +906,10 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       }
     }
     __syncthreads();  // every mask is joined
+    // joined masks wait in obs[start + element] (read already) until staging
 #pragma unroll
     for (int s = 0; s < KPT; s++)
-      if ((alive >> s) & 1) msk[s] = (tab[hs[s]] >> 12) & 0x3FFu;
+      if ((alive >> s) & 1) obs[start + (uint64_t)s * WG + threadIdx.x] = (uint16_t)((tab[hs[s]] >> 12) & 0x3FFu);
     uint64_t tot;
     block_excl_scan<WG>((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab is free after it
     u = tot;
@@ -982,7 +982,7 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   for (int s = 0; s < KPT; s++) {
     if (rr[s] != ~0u) {
       stage_k[rr[s]] = occ[start + (uint64_t)s * WG + threadIdx.x];  // the key of this slot (occ is unchanged)
-      stage_m[rr[s]] = (uint16_t)msk[s];
+      stage_m[rr[s]] = obs[start + (uint64_t)s * WG + threadIdx.x];
     }
   }
   __syncthreads();
