@@ -153,7 +153,7 @@ __device__ __forceinline__ uint64_t sk_scan_excl(uint64_t v, uint64_t* sh4, uint
 }
 __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-__global__ void __launch_bounds__(SK_WG) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+__global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                    uint64_t n_pos, uint64_t n_tiles, int K, int m,
                                                    uint64_t* __restrict__ entries, uint32_t* __restrict__ tile_cnt,
                                                    uint32_t* __restrict__ tile_w) {
@@ -191,44 +191,38 @@ __global__ void __launch_bounds__(SK_WG) k_p1_scan(const uint64_t* __restrict__ 
       const uint64_t g = B + (uint64_t)j0;
       const uint64_t n0 = stream_bits64(nmask, g), n1 = stream_bits64(nmask, g + 64);
       auto key = [&](int u) -> uint64_t { return ((uint64_t)S.ord[PJ(j0 + u)] << 16) | (uint64_t)(j0 + u); };
-      uint64_t win[SK_Q];
+      // slot q's outputs from its window minimum
+      auto emit = [&](int q, uint64_t win) {
+        const int j = j0 + q;
+        if (j < np) {
+          // bases [g+q, g+q+K) of the N mask: bits q .. q+K-1 of n0:n1
+          const uint64_t hi = q ? ((n0 << q) | (n1 >> (64 - q))) : n0;
+          S.pidx[PJ(j)] = (uint16_t)(win & 0xFFFF);
+          S.valid[PJ(j)] = (hi >> (64 - K)) == 0 ? 1 : 0;
+        }
+      };
       if (w >= SK_Q) {
         uint64_t core = ~0ULL;
 #pragma unroll 4
         for (int u = SK_Q - 1; u < w; u++) core = min64(core, key(u));
         uint64_t suf = ~0ULL, pre = ~0ULL;
-        uint64_t L[SK_Q], R[SK_Q];
+        uint64_t L[SK_Q];  // L[q] = min(core, min over [q, 7))
 #pragma unroll
         for (int q = SK_Q - 1; q >= 0; q--) {
-          L[q] = suf;  // min over [q, 7) after the update below
-          if (q < SK_Q - 1) {
-            suf = min64(suf, key(q));
-            L[q] = suf;
-          }
+          if (q < SK_Q - 1) suf = min64(suf, key(q));
+          L[q] = min64(suf, core);
         }
 #pragma unroll
         for (int q = 0; q < SK_Q; q++) {
-          R[q] = pre;  // min over [w, w+q)
+          emit(q, min64(L[q], pre));  // pre = min over [w, w+q)
           pre = min64(pre, key(w + q));
         }
-#pragma unroll
-        for (int q = 0; q < SK_Q; q++) win[q] = min64(min64(L[q], core), R[q]);
       } else {
 #pragma unroll
         for (int q = 0; q < SK_Q; q++) {
           uint64_t b = ~0ULL;
           for (int u = 0; u < w; u++) b = min64(b, key(q + u));
-          win[q] = b;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < SK_Q; q++) {
-        const int j = j0 + q;
-        if (j < np) {
-          // bases [g+q, g+q+K) of the N mask: bits q .. q+K-1 of n0:n1
-          const uint64_t hi = q ? ((n0 << q) | (n1 >> (64 - q))) : n0;
-          S.pidx[PJ(j)] = (uint16_t)(win[q] & 0xFFFF);
-          S.valid[PJ(j)] = (hi >> (64 - K)) == 0 ? 1 : 0;
+          emit(q, b);
         }
       }
     }
