@@ -285,20 +285,22 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
 // W and S to sums[2*slice ...]. k_p1_reduce adds the partials up.
 constexpr int HB_RANGE = 16384;
 constexpr int HB_SLICES = 512;
-__global__ void __launch_bounds__(256) k_p1_hist(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
-                                                 const uint32_t* __restrict__ tile_w, uint64_t n_tiles, uint64_t tiles_per_slice,
-                                                 uint32_t* __restrict__ part, uint64_t* __restrict__ sums) {
+constexpr int HB_WG = 1024;  // 16 waves share one 64 KiB LDS range
+__global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
+                                                   const uint32_t* __restrict__ tile_w, uint64_t n_tiles, uint64_t tiles_per_slice,
+                                                   uint32_t* __restrict__ part, uint64_t* __restrict__ sums) {
+  constexpr int NW = HB_WG / 64;
   __shared__ uint32_t h[HB_RANGE];
-  __shared__ uint64_t red[8];
+  __shared__ uint64_t red[2 * NW];
   const uint32_t r = blockIdx.y;
-  for (int i = threadIdx.x; i < HB_RANGE; i += 256) h[i] = 0;
+  for (int i = threadIdx.x; i < HB_RANGE; i += HB_WG) h[i] = 0;
   __syncthreads();
   const uint64_t ta = (uint64_t)blockIdx.x * tiles_per_slice;
   uint64_t tb = ta + tiles_per_slice;
   if (tb > n_tiles) tb = n_tiles;
   const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
   uint64_t s = 0, wsum = 0;
-  for (uint64_t t = ta + wv; t < tb; t += 4) {
+  for (uint64_t t = ta + wv; t < tb; t += NW) {
     const uint32_t c = tile_cnt[t];
     const uint64_t* e = entries + t * SK_T;
     for (uint32_t i = lane; i < c; i += 64) {
@@ -310,28 +312,44 @@ __global__ void __launch_bounds__(256) k_p1_hist(const uint64_t* __restrict__ en
   }
   if (lane == 0) {
     red[wv] = s;
-    red[4 + wv] = wsum;
+    red[NW + wv] = wsum;
   }
   __syncthreads();
   uint32_t* dst = part + ((uint64_t)blockIdx.x * (65536 / HB_RANGE) + r) * HB_RANGE;
-  for (int i = threadIdx.x; i < HB_RANGE; i += 256) dst[i] = h[i];
+  for (int i = threadIdx.x; i < HB_RANGE; i += HB_WG) dst[i] = h[i];
   if (r == 0 && threadIdx.x == 0) {
-    sums[2 * blockIdx.x] = red[4] + red[5] + red[6] + red[7];
-    sums[2 * blockIdx.x + 1] = red[0] + red[1] + red[2] + red[3];
+    uint64_t ws = 0, ss = 0;
+    for (int q = 0; q < NW; q++) {
+      ss += red[q];
+      ws += red[NW + q];
+    }
+    sums[2 * blockIdx.x] = ws;
+    sums[2 * blockIdx.x + 1] = ss;
   }
 }
-// hist[b] = sum over slices of the partial counts; totals = {W, S}.
+// hist[b] = sum over slices of the partial counts; totals = {W, S}. Four
+// threads per bucket take every fourth slice; LDS adds their sums.
 __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ part, const uint64_t* __restrict__ sums,
                                                    uint32_t n_slices, unsigned long long* __restrict__ hist,
                                                    unsigned long long* __restrict__ totals) {
-  const uint32_t b = blockIdx.x * 256 + threadIdx.x;  // grid covers 65536 buckets
+  __shared__ uint64_t acc4[4][64];
+  const uint32_t g = threadIdx.x >> 6, bl = __lane_id();
+  const uint32_t b = blockIdx.x * 64 + bl;  // grid covers 65536 buckets, 64 per work-group
   uint64_t acc = 0;
-  for (uint32_t sl = 0; sl < n_slices; sl++) acc += part[(uint64_t)sl * 65536 + b];
-  hist[b] = acc;
-  if (b < 2) {
+  uint32_t sl = g;
+  for (; sl + 12 < n_slices; sl += 16) {
+    const uint32_t v0 = part[(uint64_t)sl * 65536 + b], v1 = part[(uint64_t)(sl + 4) * 65536 + b];
+    const uint32_t v2 = part[(uint64_t)(sl + 8) * 65536 + b], v3 = part[(uint64_t)(sl + 12) * 65536 + b];
+    acc += (uint64_t)v0 + v1 + v2 + v3;
+  }
+  for (; sl < n_slices; sl += 4) acc += part[(uint64_t)sl * 65536 + b];
+  acc4[g][bl] = acc;
+  __syncthreads();
+  if (g == 0) hist[b] = acc4[0][bl] + acc4[1][bl] + acc4[2][bl] + acc4[3][bl];
+  if (blockIdx.x == 0 && threadIdx.x < 2) {
     uint64_t t = 0;
-    for (uint32_t sl = 0; sl < n_slices; sl++) t += sums[2 * sl + b];
-    totals[b] = t;
+    for (uint32_t q = 0; q < n_slices; q++) t += sums[2 * q + threadIdx.x];
+    totals[threadIdx.x] = t;
   }
 }
 
@@ -596,9 +614,9 @@ void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint6
   const uint64_t slices = p1_slices(n_pos);
   const uint64_t per = (nt + slices - 1) / slices;
   const uint64_t used = (nt + per - 1) / per;
-  CF1G_LAUNCH("p1_hist", k_p1_hist, dim3((unsigned)used, 65536 / HB_RANGE), dim3(256), 0, st, entries, tile_cnt, tile_w, nt, per,
+  CF1G_LAUNCH("p1_hist", k_p1_hist, dim3((unsigned)used, 65536 / HB_RANGE), dim3(HB_WG), 0, st, entries, tile_cnt, tile_w, nt, per,
               part, sums);
-  CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 256), dim3(256), 0, st, part, sums, (uint32_t)used, hist, totals);
+  CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 64), dim3(256), 0, st, part, sums, (uint32_t)used, hist, totals);
 }
 // Route chunking: chunk_tiles tiles per chunk, keeping the count matrix at
 // <= 2^22 entries.
