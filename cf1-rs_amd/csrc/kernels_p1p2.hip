@@ -389,8 +389,10 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
                                                   const uint16_t* __restrict__ b2b, int num_bins,
                                                   const uint64_t* __restrict__ off, const uint64_t* __restrict__ bin_start,
                                                   uint64_t* __restrict__ rec, unsigned int* __restrict__ overflow) {
-  __shared__ uint32_t cnt[MAX_BINS];
-  __shared__ uint64_t base[MAX_BINS];
+  // per-bin bases and cursors of this chunk, num_bins of each (dynamic LDS)
+  extern __shared__ __align__(8) uint8_t route_lds[];
+  uint64_t* base = (uint64_t*)route_lds;
+  uint32_t* cnt = (uint32_t*)(base + num_bins);
   const uint64_t chunk = blockIdx.x;
   for (int b = threadIdx.x; b < num_bins; b += 256) {
     cnt[b] = 0;
@@ -445,6 +447,15 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
 // minimizer.rs:156-172). One work-group of 1024 threads, 64 buckets each.
 constexpr int PT_WG = 1024;
 constexpr int PT_PER = 65536 / PT_WG;
+static_assert(MAX_BINS % PT_WG == 0, "bins split evenly over the threads");
+// floor(p / d) for d > 0 from a double estimate p * (1/d), corrected exactly
+// (the estimate is within one of the quotient for p, d < 2^52).
+__device__ __forceinline__ uint64_t div_floor(uint64_t p, uint64_t d, double inv_d) {
+  uint64_t q = (uint64_t)((double)p * inv_d);
+  while (q > 0 && q * d > p) q--;
+  while ((q + 1) * d <= p) q++;
+  return q;
+}
 
 __device__ __forceinline__ uint64_t blk_scan_excl_u64(uint64_t v, uint64_t* sh, uint64_t* total) {
   // sh: PT_WG/64 slots. Inclusive wave scan then block offsets.
@@ -520,6 +531,7 @@ __global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* _
   uint64_t total;
   uint64_t pre = blk_scan_excl_u64(loc, shu, &total);
   const uint64_t t = total / (uint64_t)num_bins_req;
+  const double inv_t = t ? 1.0 / (double)t : 0.0;
   const int64_t cap = (int64_t)num_bins_req - 1;
   // running min of g(j) = f(j) - j over this thread's buckets (inclusive, local)
   int64_t gmin_loc = INT64_MAX;
@@ -527,7 +539,7 @@ __global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* _
     uint64_t P = pre;
     for (int q = 0; q < PT_PER; q++) {
       P += hist[b0 + q];
-      int64_t g = (int64_t)(P / t) - (int64_t)(b0 + q);
+      int64_t g = (int64_t)div_floor(P, t, inv_t) - (int64_t)(b0 + q);
       gmin_loc = g < gmin_loc ? g : gmin_loc;
     }
   }
@@ -555,7 +567,7 @@ __global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* _
       if (t == 0) {
         nb = (int64_t)b + 1 < cap ? (int64_t)b + 1 : cap;
       } else {
-        int64_t g = (int64_t)(P / t) - (int64_t)b;
+        int64_t g = (int64_t)div_floor(P, t, inv_t) - (int64_t)b;
         run = g < run ? g : run;
         int64_t mm = run < 1 ? run : 1;
         int64_t v = (int64_t)b + mm;
@@ -570,23 +582,39 @@ __global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* _
     if (b0 + PT_PER == 65536) s_nbins = (uint32_t)(prev_nb + 1);
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t nb = s_nbins;
-    info[0] = nb;
-    uint64_t run = 0, crun = 0;
-    for (uint32_t b = 0; b < nb; b++) {
+  // bin starts and chunk offsets: block scans over MAX_BINS / PT_WG bins per thread
+  constexpr int BPT = MAX_BINS / PT_WG;
+  const uint32_t nb = s_nbins;
+  const double inv_c = 1.0 / (double)chunk_records;
+  uint64_t rl = 0, cl = 0;
+  for (int q = 0; q < BPT; q++) {
+    const uint32_t b = (uint32_t)tid * BPT + q;
+    if (b < nb) {
+      rl += binrec[b];
+      cl += div_floor(binrec[b] + chunk_records - 1, chunk_records, inv_c);
+    }
+  }
+  uint64_t rtot, ctot;
+  uint64_t run = blk_scan_excl_u64(rl, shu, &rtot);
+  uint64_t crun = blk_scan_excl_u64(cl, shu, &ctot);
+  for (int q = 0; q < BPT; q++) {
+    const uint32_t b = (uint32_t)tid * BPT + q;
+    if (b < nb) {
+      const uint64_t r = binrec[b];
       bin_start[b] = run;
       chunk_off[b] = crun;
-      uint64_t r = binrec[b];
       bin_records[b] = r;
       bin_cursor[b] = 0;
       run += r;
-      crun += (r + chunk_records - 1) / chunk_records;
+      crun += div_floor(r + chunk_records - 1, chunk_records, inv_c);
     }
-    bin_start[nb] = run;
-    chunk_off[nb] = crun;
-    info[1] = run;
-    info[2] = crun;
+  }
+  if (tid == 0) {
+    bin_start[nb] = rtot;
+    chunk_off[nb] = ctot;
+    info[0] = nb;
+    info[1] = rtot;
+    info[2] = ctot;
   }
 }
 
@@ -638,11 +666,12 @@ void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask,
                      const uint64_t* off, const uint64_t* bin_start, uint64_t* rec, int rw, unsigned int* overflow) {
   if (n_chunks == 0) return;
   const uint64_t nt = p1_tiles(n_pos);
+  const size_t lds = (size_t)num_bins * 12;
   if (rw == 2)
-    CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), 0, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
+    CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
                 n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
   else
-    CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), 0, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
+    CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
                 n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
 }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,
