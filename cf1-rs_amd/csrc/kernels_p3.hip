@@ -209,10 +209,10 @@ __device__ __forceinline__ uint64_t rec2_occ(uint64_t w0, uint64_t w1, int j, in
   }
   return c;
 }
-// Round-local k-mer -> record map for RECS_PER_ROUND records of at most
-// OWN_KMERS k-mers each (P2 emits super k-mers of at most w <= 17 k-mers for
-// K <= 31; a round with longer records takes the per-record loop).
-constexpr int OWN_KMERS = 18;
+// Round-local k-mer -> record map for P3_WG records of at most OWN_KMERS
+// k-mers each (P2 emits super k-mers of at most w <= 17 k-mers for K <= 31;
+// a round with longer records takes the per-record loop).
+constexpr int OWN_KMERS = 17;
 
 
 // Work-group -> (bin, record range) for the per-bin passes.
@@ -302,17 +302,21 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
 
 // Tiles staged in LDS and counting-sorted by destination before the stores,
 // so that consecutive lanes write consecutive addresses of one run.
+// (the scatter tile keeps its work-group's LDS under 32 KiB, five per CU;
+// the split tile is a multiple of P3_WG)
 template <typename KeyT>
 struct StageCfg {
-  static constexpr int TILE = sizeof(KeyT) == 8 ? 2048 : 1024;
+  static constexpr int TILE = sizeof(KeyT) == 8 ? 1920 : 1024;        // k_p3_scatter
+  static constexpr int SPLIT_TILE = sizeof(KeyT) == 8 ? 2048 : 1024;  // k_p3_split
 };
 
 // Within a staged tile of tn elements with destination ids dst_id[i] < nd:
-// rk[i] = rank among equal ids, cnt / start = per-id counts / tile offsets,
-// inv[start[id] + rk] = i. Barriers inside; cnt must be zero on entry.
-__device__ __forceinline__ void stage_sort(uint32_t tn, const uint8_t* dst_id, uint16_t* rk, uint16_t* inv, uint32_t* cnt,
-                                           uint32_t* start, int nd) {
-  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) rk[i] = (uint16_t)atomicAdd(&cnt[dst_id[i]], 1u);
+// cnt = per-id counts, inv lists the elements grouped by id (id order; the
+// order inside a group is arbitrary), and end[id] = the end of id's group in
+// inv (its start is end[id] - cnt[id]). Barriers inside; cnt must be zero on
+// entry.
+__device__ __forceinline__ void stage_sort(uint32_t tn, const uint8_t* dst_id, uint16_t* inv, uint32_t* cnt, uint32_t* end, int nd) {
+  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) atomicAdd(&cnt[dst_id[i]], 1u);
   __syncthreads();
   if (threadIdx.x < 64) {  // exclusive scan of <= 64 counts in one wave
     const uint32_t v = (int)threadIdx.x < nd ? cnt[threadIdx.x] : 0u;
@@ -321,10 +325,10 @@ __device__ __forceinline__ void stage_sort(uint32_t tn, const uint8_t* dst_id, u
       const uint32_t y = __shfl_up(x, d, 64);
       if ((int)threadIdx.x >= d) x += y;
     }
-    if ((int)threadIdx.x < nd) start[threadIdx.x] = x - v;
+    if ((int)threadIdx.x < nd) end[threadIdx.x] = x - v;  // group start, advanced to its end below
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) inv[start[dst_id[i]] + rk[i]] = (uint16_t)i;
+  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) inv[atomicAdd(&end[dst_id[i]], 1u)] = (uint16_t)i;
   __syncthreads();
 }
 
@@ -341,9 +345,8 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   __shared__ KeyT tk[TILE];
   __shared__ uint16_t tm[TILE];
   __shared__ uint8_t tp[TILE];
-  __shared__ uint16_t rk[TILE];
   __shared__ uint16_t inv[TILE];
-  __shared__ uint32_t cnt[1 << P3_PART_BITS], start[1 << P3_PART_BITS];
+  __shared__ uint32_t cnt[1 << P3_PART_BITS], end[1 << P3_PART_BITS];
   __shared__ uint64_t base[1 << P3_PART_BITS];
   __shared__ uint64_t sh4[4];
   __shared__ uint8_t own[RW == 2 ? P3_WG * OWN_KMERS : 1];
@@ -407,10 +410,10 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
         });
       }
       __syncthreads();
-      stage_sort(tn, tp, rk, inv, cnt, start, np);
+      stage_sort(tn, tp, inv, cnt, end, np);
       for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
         const uint32_t e = inv[i], p = tp[e];
-        const uint64_t at = base[p] + (i - start[p]);
+        const uint64_t at = base[p] + (i - (end[p] - cnt[p]));
         occ[at] = tk[e];
         occ_obs[at] = tm[e];
       }
@@ -431,14 +434,13 @@ template <typename KeyT>
 __global__ void __launch_bounds__(P3_WG) k_p3_split(const KeyT* __restrict__ src, const uint16_t* __restrict__ src_obs,
                                                     const uint64_t* __restrict__ seg_off, int sb, int pb, KeyT* __restrict__ dst,
                                                     uint16_t* __restrict__ dst_obs) {
-  constexpr int TILE = StageCfg<KeyT>::TILE;
+  constexpr int TILE = StageCfg<KeyT>::SPLIT_TILE;
   constexpr int MAXS = 1 << (P3_MAX_SB - P3_PART_BITS);
   __shared__ KeyT tk[TILE];
   __shared__ uint16_t tm[TILE];
   __shared__ uint8_t tp[TILE];
-  __shared__ uint16_t rk[TILE];
   __shared__ uint16_t inv[TILE];
-  __shared__ uint32_t cnt[MAXS], start[MAXS];
+  __shared__ uint32_t cnt[MAXS], end[MAXS];
   __shared__ uint64_t base[MAXS];
   const int spp = 1 << (sb - pb);  // <= 64: the split runs only when sb > pb = P3_PART_BITS
   const uint64_t s0 = (uint64_t)blockIdx.x * spp;  // first segment of the part (bin-major)
@@ -478,10 +480,10 @@ __global__ void __launch_bounds__(P3_WG) k_p3_split(const KeyT* __restrict__ src
     }
     if (t0 + TILE < b) fetch(t0 + TILE);
     __syncthreads();
-    stage_sort(tn, tp, rk, inv, cnt, start, spp);
+    stage_sort(tn, tp, inv, cnt, end, spp);
     for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
       const uint32_t e = inv[i], t = tp[e];
-      const uint64_t at = base[t] + (i - start[t]);
+      const uint64_t at = base[t] + (i - (end[t] - cnt[t]));
       dst[at] = tk[e];
       dst_obs[at] = tm[e];
     }
