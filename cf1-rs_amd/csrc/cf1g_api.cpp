@@ -314,10 +314,11 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->stage = 3;
 }
 
-// P3: count, scan, scatter (keys + observation masks), per-segment sort /
-// dedup / mask join / BBHash into padded block slots, big-segment sorts and
-// BBHash, scans of the distinct counts (MPHF key offsets) and block counts,
-// then one compaction of keys, masks (to MPHF-value order) and blocks.
+// P3: count, scan, two-level scatter (keys + observation masks into parts,
+// then segments), per-segment dedup / mask join / BBHash into padded block
+// slots (keys and masks in local-rank order), big-segment sorts and BBHash,
+// scans of the distinct counts (MPHF key offsets) and block counts, then one
+// compaction of the masks (MPHF-value order) and blocks; keys on first use.
 void do_p3(cf1g_ctx* c) {
   hipStream_t st = c->st;
   const size_t kb = c->key_bytes;

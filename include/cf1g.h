@@ -103,8 +103,9 @@ int cf1g_run_p1_p4(cf1g_ctx* ctx, uint64_t* out_D);
 
 /* AtomicStateVector words (12 x 5-bit states per u64, src/state_vector.rs:7-42). */
 int cf1g_states_export(cf1g_ctx* ctx, uint64_t* words, uint64_t n_words);
-/* The distinct canonical keys in bin order, each bin sorted (the order of
- * cf1_dedup_all.bin, src/mphf.rs:120-132): D u64, or D (lo, hi) pairs. */
+/* The distinct canonical keys (the key set of cf1_dedup_all.bin,
+ * src/mphf.rs:120-132) in MPHF value order: key v has value v. D u64, or D
+ * (lo, hi) pairs. */
 int cf1g_keys_export(cf1g_ctx* ctx, uint64_t* out, uint64_t n_words);
 /* Serialised MPHF (minimizer-partitioned BBHash): 16 header words
  * (magic, k, m, segment bits, bins, segments, D, blocks, wide), the
