@@ -47,8 +47,10 @@ void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, co
                        unsigned int*, uint32_t*);
 void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
                     uint32_t, uint64_t*, uint64_t*, uint64_t*, uint64_t*, void*, unsigned int*);
-void launch_p3_compact(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*,
-                       const uint64_t*, const uint64_t*, uint64_t, int, int, void*, uint16_t*, uint64_t*);
+void launch_p3_compact(hipStream_t, bool, const uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
+                       const uint64_t*, uint64_t, uint16_t*, uint64_t*);
+void launch_p3_keys(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
+                    uint64_t, void*, unsigned int*);
 void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
 uint64_t sort_tiles(uint64_t);
 void launch_sort_hist(hipStream_t, const uint64_t*, uint64_t, int, uint64_t*);
@@ -409,7 +411,7 @@ void do_p3(cf1g_ctx* c) {
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
   // masks and blocks now; the dense key array (which the reference's MPHF
   // does not keep) is compacted from kout on first use, dense_keys()
-  launch_p3_compact(st, c->wide, kout, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, K, sb, nullptr, vmask, blocks);
+  launch_p3_compact(st, c->wide, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, vmask, blocks);
   c->keys_ready = false;
   c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
   c->d2h(hf, flags, 12);
@@ -419,16 +421,22 @@ void do_p3(cf1g_ctx* c) {
   c->stage = 4;
 }
 
-// The distinct keys in MPHF value order (keys[v] has value v), compacted
-// from the per-segment rank-order runs P3 left in occ_aux on first use.
+// The distinct keys in MPHF value order (keys[v] has value v), built on
+// first use from the per-segment key runs P3 left in occ_aux (each key placed
+// by its BBHash value).
 void* dense_keys(cf1g_ctx* c) {
   const uint64_t kb = c->key_bytes;
   void* keys = c->get("keys", c->D * kb + 16);
   if (!c->keys_ready) {
     const uint64_t ns = c->n_segs;
-    launch_p3_compact(c->st, c->wide, c->get("occ_aux", c->W * kb + 16), nullptr, c->getT<uint64_t>("seg_off", ns + 1),
-                      c->getT<uint64_t>("seg_ucount", ns + 1), c->getT<uint64_t>("seg_out_off", ns + 1), nullptr, nullptr, ns, c->k,
-                      c->sb, keys, nullptr, nullptr);
+    unsigned int* err = c->getT<unsigned int>("keys_err", 2);
+    c->zero(err, 4);
+    launch_p3_keys(c->st, c->wide, c->get("occ_aux", c->W * kb + 16), c->getT<uint64_t>("seg_off", ns + 1),
+                   c->getT<uint64_t>("seg_ucount", ns + 1), c->getT<uint64_t>("seg_out_off", ns + 1),
+                   c->getT<uint64_t>("bbh_blk_off", ns + 1), c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), ns, keys, err);
+    unsigned int he = 0;
+    c->d2h(&he, err, 4);
+    if (he) fail(CF1G_E_INTERNAL, "a P3 key is missing from its MPHF segment");
     c->keys_ready = true;
   }
   return keys;

@@ -751,8 +751,10 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
 // joined mask << 12 | 10-bit key fingerprint << 22; keys are compared, from
 // occ, only when the fingerprints match); which occurrence represents a key
 // depends on timing, but the set of keys, their joined masks and the BBHash
-// built over them do not. Output: the distinct keys in MPHF value order at
-// kout[start + local rank], their masks at vpad[start + local rank], their
+// built over them do not. Output: kout[start + e] = the key of occurrence e
+// if it represents its key, else the pad (all ones; the dense key array is
+// built from these on request, k_p3_keys); the joined masks in MPHF value
+// order at vpad[start + local rank], their
 // count in ucount, the BBHash blocks in the segment's padded slot of
 // blocks_pad and their count in nblk. Segments with more occurrences than
 // DD_CAP go to over_list (sorted by k_p3_bigsort); segments whose distinct
@@ -772,10 +774,10 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
   constexpr int TAB_BITS = 12;
   constexpr int SEEN_W = (2 * CAP) / 64 + 1;
   constexpr int REG_W = (4 * CAP + 8192) / 64;
-  constexpr int TAB_BYTES = (4 << TAB_BITS) > (int)sizeof(KeyT) * CAP ? (4 << TAB_BITS) : (int)sizeof(KeyT) * CAP;
+  constexpr int TAB_BYTES = 4 << TAB_BITS;
   // tab, byte offsets: hash table (or direct-address mask table) over all of
   // it; then ppos [0, 4 CAP), word ranks, seen / collision bitmaps and the
-  // tail list; then the staged keys (all of it)
+  // tail list
   constexpr int T_WRANK = 4 * CAP, T_SEEN = T_WRANK + 4 * REG_W, T_LIST = T_SEEN + 4 * 8 * SEEN_W;
   constexpr int T_END = T_LIST + TAIL_KEYS * 18;
   // aux: region (level words), the block-scan and level-count words behind
@@ -852,9 +854,11 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < u) {
         key[s] = occ[start + e];
+        kout[start + e] = key[s];
         alive |= 1u << s;
       }
     }
+    for (uint64_t e = u + threadIdx.x; e < n; e += WG) kout[start + e] = key_pad<KeyT>();
   } else if (n > (uint64_t)CAP) {
     if (threadIdx.x == 0) {
       over_list[atomicAdd(n_over, 1u)] = (uint32_t)seg;
@@ -908,10 +912,14 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
       }
     }
     __syncthreads();  // every mask is joined
-    // joined masks wait in obs[start + element] (read already) until staging
+    // joined masks wait in obs[start + element] (read already) until staging;
+    // each occurrence's key, or the pad for a duplicate, goes to kout now
 #pragma unroll
-    for (int s = 0; s < KPT; s++)
-      if ((alive >> s) & 1) obs[start + (uint64_t)s * WG + threadIdx.x] = (uint16_t)((tab[hs[s]] >> 12) & 0x3FFu);
+    for (int s = 0; s < KPT; s++) {
+      const uint64_t e = (uint64_t)s * WG + threadIdx.x;
+      if ((alive >> s) & 1) obs[start + e] = (uint16_t)((tab[hs[s]] >> 12) & 0x3FFu);
+      if (e < n) kout[start + e] = ((alive >> s) & 1) ? key[s] : key_pad<KeyT>();
+    }
     uint64_t tot;
     block_excl_scan<WG>((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab is free after it
     u = tot;
@@ -978,20 +986,12 @@ __global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint
     }
   }
   __syncthreads();
-  KeyT* stage_k = (KeyT*)tab_b;
   uint16_t* stage_m = (uint16_t*)aux_b;
 #pragma unroll
-  for (int s = 0; s < KPT; s++) {
-    if (rr[s] != ~0u) {
-      stage_k[rr[s]] = occ[start + (uint64_t)s * WG + threadIdx.x];  // the key of this slot (occ is unchanged)
-      stage_m[rr[s]] = obs[start + (uint64_t)s * WG + threadIdx.x];
-    }
-  }
+  for (int s = 0; s < KPT; s++)
+    if (rr[s] != ~0u) stage_m[rr[s]] = obs[start + (uint64_t)s * WG + threadIdx.x];
   __syncthreads();
-  for (uint64_t i = threadIdx.x; i < u; i += WG) {
-    kout[start + i] = stage_k[i];
-    vpad[start + i] = stage_m[i];
-  }
+  for (uint64_t i = threadIdx.x; i < u; i += WG) vpad[start + i] = stage_m[i];
 }
 
 // Oversized segments: one work-group per segment runs a stable LSD radix
@@ -1134,27 +1134,41 @@ __global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, ui
 // ------------------------------------------------------------------ compaction
 // One work-group per segment copies the segment's masks (local-rank order,
 // vpad) to their MPHF values in vmask and its BBHash blocks from the padded
-// slot to the dense block order (blk_off); or, with keys != nullptr, only its
-// distinct keys (local-rank order, kout) to their MPHF values in keys.
+// slot to the dense block order (blk_off).
 template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ kout, const uint16_t* __restrict__ vpad,
-                                                      const uint64_t* __restrict__ seg_off,
+__global__ void __launch_bounds__(RK_WG) k_p3_compact(const uint16_t* __restrict__ vpad, const uint64_t* __restrict__ seg_off,
                                                       const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
                                                       const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks_pad,
-                                                      uint64_t n_segs, int K2, int sb, KeyT* __restrict__ keys,
-                                                      uint16_t* __restrict__ vmask, uint64_t* __restrict__ blocks) {
+                                                      uint64_t n_segs, uint16_t* __restrict__ vmask, uint64_t* __restrict__ blocks) {
   const uint64_t seg = blockIdx.x;
   if (seg >= n_segs) return;
   const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
   if (u == 0) return;
-  if (keys) {
-    for (uint64_t i = threadIdx.x; i < u; i += RK_WG) keys[ko + i] = kout[s0 + i];
-    return;
-  }
   for (uint64_t i = threadIdx.x; i < u; i += RK_WG) vmask[ko + i] = vpad[s0 + i];
   const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
   const uint64_t* src = blocks_pad + pad_slot_off(s0, seg) * 8;
   for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) blocks[b0 * 8 + i] = src[i];
+}
+
+// The dense key array on request: every key P3 left in its segment's run of
+// kout (pads skipped) goes to keys[its MPHF value], the local rank coming
+// from the segment's (dense) BBHash blocks.
+template <typename KeyT>
+__global__ void __launch_bounds__(RK_WG) k_p3_keys(const KeyT* __restrict__ kout, const uint64_t* __restrict__ seg_off,
+                                                   const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
+                                                   const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks,
+                                                   uint64_t n_segs, KeyT* __restrict__ keys, unsigned int* __restrict__ err) {
+  const uint64_t seg = blockIdx.x;
+  if (seg >= n_segs) return;
+  const uint64_t s0 = seg_off[seg], n = seg_off[seg + 1] - s0, u = ucount[seg], ko = key_off[seg];
+  const uint64_t* base = blocks + blk_off[seg] * 8;
+  for (uint64_t e = threadIdx.x; e < n; e += RK_WG) {
+    const KeyT k = kout[s0 + e];
+    if (key_eq(k, key_pad<KeyT>())) continue;
+    const uint64_t r = mphf_local_rank<KeyT>(base, u, k);
+    if (r < u) keys[ko + r] = k;
+    else atomicOr(err, 1u);
+  }
 }
 
 // ------------------------------------------------------------------ BBHash of big segments
@@ -1163,8 +1177,9 @@ __global__ void __launch_bounds__(RK_WG) k_p3_compact(const KeyT* __restrict__ k
 // the seen / collision / alive bitmaps sit in LDS (segments of <= MED_CAP
 // distinct keys); otherwise in global scratch (seen / collision words of the
 // segment at 4*floor(seg_off/64) + 8*seg, alive bits per occurrence position)
-// with device-scope atomics. Blocks go to the padded slot; each key and its
-// mask are written to kout / vpad[seg_off + local rank].
+// with device-scope atomics. Blocks go to the padded slot; each key's mask
+// is written to vpad[seg_off + local rank], the keys (then pads up to the
+// segment's occurrence count) to kout[seg_off ...].
 constexpr uint64_t MED_CAP = 1ULL << 17;
 constexpr int MED_W = (int)(MED_CAP / 64);
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
@@ -1284,13 +1299,11 @@ __global__ void __launch_bounds__(RK_WG) k_bbh_big(const KeyT* __restrict__ occ,
   for (uint64_t e = threadIdx.x; e < D; e += RK_WG) {
     const KeyT kv = keys[e];
     uint64_t rk = mphf_local_rank<KeyT>(region, D, kv);
-    if (rk >= D) {
-      atomicOr(err, 4u);
-    } else {
-      kout[so + rk] = kv;
-      vpad[so + rk] = obs[so + e];
-    }
+    kout[so + e] = kv;
+    if (rk >= D) atomicOr(err, 4u);
+    else vpad[so + rk] = obs[so + e];
   }
+  for (uint64_t e = D + threadIdx.x; e < n_occ; e += RK_WG) kout[so + e] = key_pad<KeyT>();
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1379,16 +1392,27 @@ void launch_bbh_big(hipStream_t st, bool wide, const void* occ, const uint16_t* 
                   nblk, blocks_pad, scratch, alive_g, (Key128*)kout, err);
   }
 }
-void launch_p3_compact(hipStream_t st, bool wide, const void* kout, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount,
-                       const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad, uint64_t n_segs, int K, int sb,
-                       void* keys, uint16_t* vmask, uint64_t* blocks) {
+void launch_p3_compact(hipStream_t st, bool wide, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount,
+                       const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad, uint64_t n_segs, uint16_t* vmask,
+                       uint64_t* blocks) {
   if (n_segs == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)kout, vpad, seg_off,
-                ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (uint64_t*)keys, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, vpad, seg_off, ucount, key_off,
+                blk_off, blocks_pad, n_segs, vmask, blocks);
   else
-    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)kout, vpad, seg_off,
-                ucount, key_off, blk_off, blocks_pad, n_segs, 2 * K, sb, (Key128*)keys, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, vpad, seg_off, ucount, key_off,
+                blk_off, blocks_pad, n_segs, vmask, blocks);
+}
+void launch_p3_keys(hipStream_t st, bool wide, const void* kout, const uint64_t* seg_off, const uint64_t* ucount,
+                    const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks, uint64_t n_segs, void* keys,
+                    unsigned int* err) {
+  if (n_segs == 0) return;
+  if (!wide)
+    CF1G_LAUNCH("p3_keys", k_p3_keys<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const uint64_t*)kout, seg_off, ucount,
+                key_off, blk_off, blocks, n_segs, (uint64_t*)keys, err);
+  else
+    CF1G_LAUNCH("p3_keys", k_p3_keys<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, (const Key128*)kout, seg_off, ucount,
+                key_off, blk_off, blocks, n_segs, (Key128*)keys, err);
 }
 
 }  // namespace cf1g
