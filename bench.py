@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lookups", type=int, default=1 << 28, help="lookups per batch (0: skip the lookup benchmark)")
     ap.add_argument("--lookup-batches", type=int, default=4)
+    ap.add_argument("--p4-cas-runs", type=int, default=3, help="timed runs of the per-occurrence CAS classification (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (MiB) to scale HBM byte counters")
     args = ap.parse_args()
@@ -235,6 +236,10 @@ def main():
         phases.setdefault(PHASE_OF.get(name, name), 0.0)
         phases[PHASE_OF.get(name, name)] += v["ms_per_step"]
 
+    # P4 in the reference's form (per-occurrence lookup + CAS join) beside
+    # the mask-join P4 the step uses; both must give the same state words
+    p4cas = p4_cas_bench(g, info, e_probe) if args.p4_cas_runs > 0 else None
+
     # learned index vs BBHash lookups over device-resident present keys
     lookup = None
     if args.lookups > 0 and D > 0 and kb == 8:
@@ -271,10 +276,35 @@ def main():
         "phase_wall_ms": {k2: round(v * 1000, 3) for k2, v in ph.items()},
         "phase_kernel_ms": {k2: round(v, 3) for k2, v in phases.items()},
         "kernels": {k2: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k2, v in kern.items()},
+        "p4_cas": p4cas,
         "lookup": lookup,
     }
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def p4_cas_bench(g, info, e_probe, runs=3):
+    """cf1g_p4_classify_cas on the same records: one BBHash lookup and one
+    64-bit CAS join per k-mer occurrence (persistent work-stealing launch).
+    Algorithmic bytes: the packed stream once plus one 64-byte granule per
+    dependent random access of each occurrence (the BBHash levels probed,
+    the segment tables, the state word)."""
+    import numpy as np
+
+    want = g.states()
+    g.classify_vertices_cas()  # warm-up
+    st = [g.classify_vertices_cas() for _ in range(runs)]
+    same = bool(np.array_equal(g.states(), want))
+    ms = sorted(x["kernel_ms"] for x in st)[len(st) // 2]
+    W, NG = info["n_kmers"], info["stream_len"]
+    ab = NG / 4 + NG / 8 + 64.0 * W * (e_probe + 2)
+    s0 = st[0]
+    print(f"[bench] CAS classification: {ms:.3f} ms, states equal the mask join: {same}", file=sys.stderr, flush=True)
+    return {"kernel_ms": round(ms, 4), "occurrences": s0["occurrences"], "occurrences_per_s": W / (ms * 1e-3),
+            "cas_issued": s0["cas_issued"], "cas_failed": s0["cas_failed"], "cas_per_s": s0["cas_issued"] / (ms * 1e-3),
+            "chunk": s0["chunk"], "wgs_per_cu": s0["wgs_per_cu"], "grid": s0["grid"], "n_chunks": s0["n_chunks"],
+            "algorithmic_bytes": ab, "achieved_gbs": round(ab / (ms * 1e-3) / 1e9, 1),
+            "states_equal_mask_join": same}
 
 
 def copy_calibration(nbytes, reps=3):

@@ -30,7 +30,7 @@ NUM_BUCKETS = 65536
 EXPORTS = [
     "cf1g_create", "cf1g_destroy", "cf1g_last_error", "cf1g_get_info", "cf1g_load_records",
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
-    "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
+    "cf1g_p4_classify_cas", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
@@ -59,6 +59,15 @@ class CfInfo(C.Structure):
                 ("n_superkmers", C.c_uint64), ("n_distinct", C.c_uint64), ("n_state_words", C.c_uint64),
                 ("mphf_words", C.c_uint64), ("num_bins", C.c_uint32), ("seg_bits", C.c_uint32),
                 ("mphf_big_segments", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32)]
+
+
+class CasStats(C.Structure):
+    _fields_ = [("occurrences", C.c_uint64), ("cas_issued", C.c_uint64), ("cas_failed", C.c_uint64),
+                ("n_chunks", C.c_uint64), ("chunk", C.c_uint32), ("wgs_per_cu", C.c_uint32), ("grid", C.c_uint32),
+                ("reserved", C.c_uint32), ("kernel_ms", C.c_double)]
+
+
+CAS_FORCE_RETRY = 1
 
 
 class LidxParams(C.Structure):
@@ -95,6 +104,7 @@ def lib():
         L.cf1g_p2_partition_route.argtypes = [C.c_void_p, U64P, C.c_uint32, U16P, U32P]
         L.cf1g_p3_build.argtypes = [C.c_void_p, U64P]
         L.cf1g_p4_classify.argtypes = [C.c_void_p]
+        L.cf1g_p4_classify_cas.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(CasStats)]
         L.cf1g_run_p1_p4.argtypes = [C.c_void_p, U64P]
         L.cf1g_states_export.argtypes = [C.c_void_p, U64P, C.c_uint64]
         L.cf1g_keys_export.argtypes = [C.c_void_p, U64P, C.c_uint64]
@@ -215,6 +225,13 @@ class Cf1g:
 
     def classify_vertices(self):
         self._check(self.L.cf1g_p4_classify(self.h))
+
+    def classify_vertices_cas(self, force_retry=False):
+        """P4 in the reference's form: per-occurrence MPHF lookup and CAS
+        join into a zeroed state vector; returns the kernel's statistics."""
+        o = CasStats()
+        self._check(self.L.cf1g_p4_classify_cas(self.h, CAS_FORCE_RETRY if force_retry else 0, C.byref(o)))
+        return {f: getattr(o, f) for f, _ in CasStats._fields_ if f != "reserved"}
 
     def run_p1_p4(self):
         d = C.c_uint64(0)

@@ -52,6 +52,8 @@ void launch_p3_compact(hipStream_t, bool, const uint16_t*, const uint64_t*, cons
 void launch_p3_keys(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
                     uint64_t, void*, unsigned int*);
 void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
+void launch_p4_cas(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, uint64_t, uint64_t*, uint64_t*,
+                   uint32_t, uint32_t, int);
 uint64_t sort_tiles(uint64_t);
 void launch_sort_hist(hipStream_t, const uint64_t*, uint64_t, int, uint64_t*);
 void launch_sort_scatter(hipStream_t, const uint64_t*, uint64_t*, uint64_t, int, const uint64_t*);
@@ -84,7 +86,7 @@ std::mutex g_prof_mu;
 bool g_prof_on = false;
 std::vector<ProfRec> g_prof;
 std::vector<hipEvent_t> g_ev_pool;
-hipEvent_t ev_get() {
+hipEvent_t ev_get_locked() {
   if (!g_ev_pool.empty()) {
     hipEvent_t e = g_ev_pool.back();
     g_ev_pool.pop_back();
@@ -97,17 +99,27 @@ hipEvent_t ev_get() {
 thread_local hipEvent_t t_pending = nullptr;
 }  // namespace
 
+void ev_put(hipEvent_t e) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_ev_pool.push_back(e);
+}
+
+hipEvent_t ev_get() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  return ev_get_locked();
+}
+
 void prof_begin(const char* name, hipStream_t st) {
   if (!g_prof_on) return;
   std::lock_guard<std::mutex> g(g_prof_mu);
-  t_pending = ev_get();
+  t_pending = ev_get_locked();
   hip_check(hipEventRecord(t_pending, st), "hipEventRecord");
   (void)name;
 }
 void prof_end(const char* name, hipStream_t st) {
   if (!g_prof_on || !t_pending) return;
   std::lock_guard<std::mutex> g(g_prof_mu);
-  hipEvent_t e = ev_get();
+  hipEvent_t e = ev_get_locked();
   hip_check(hipEventRecord(e, st), "hipEventRecord");
   g_prof.push_back({name, t_pending, e});
   t_pending = nullptr;
@@ -151,6 +163,9 @@ struct cf1g_ctx {
   uint32_t lidx_R = 0, lidx_L = 0;
   int lidx_s = 0;
   int use_index = 0;
+  // launch shapes the autotuner may change (cf1g_autotune); the params'
+  // non-zero fields take precedence
+  uint32_t tuned_p4_chunk = 8192, tuned_p4_wgs = 8;
 
   void* get(const std::string& name, size_t bytes) {
     DevBuf& b = bufs[name];
@@ -450,6 +465,50 @@ void do_p4(cf1g_ctx* c) {
   c->stage = 5;
 }
 
+// P4 in the reference's form: zeroed states, then one MPHF lookup and CAS
+// join per k-mer occurrence of the loaded records (k_p4_cas).
+void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
+  hipStream_t st = c->st;
+  const uint64_t nsw = (c->D + 11) / 12;
+  uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
+  uint64_t* cnt = c->getT<uint64_t>("p4_counters", 8);
+  c->zero(states, (nsw + 1) * 8);
+  c->zero(cnt, 64);
+  const uint32_t chunk = c->prm.p4_chunk ? c->prm.p4_chunk : c->tuned_p4_chunk;
+  const uint32_t wpc = c->prm.p4_wgs_per_cu ? c->prm.p4_wgs_per_cu : c->tuned_p4_wgs;
+  if (chunk < 256 || chunk > (1u << 24)) fail(CF1G_E_INVALID, "p4_chunk must be in [256, 2^24]");
+  if (wpc < 1 || wpc > 32) fail(CF1G_E_INVALID, "p4_wgs_per_cu must be in [1, 32]");
+  const uint64_t n_chunks = c->n_pos > 1 ? (c->n_pos - 1 + chunk - 1) / chunk : 0;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->cu_count * wpc, std::max<uint64_t>(n_chunks, 1));
+  hipEvent_t e0 = ev_get(), e1 = ev_get();
+  hip_check(hipEventRecord(e0, st), "hipEventRecord");
+  launch_p4_cas(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), c->n_pos,
+                c->k, c->view(), c->D, states, cnt, chunk, grid, (flags & CF1G_CAS_FORCE_RETRY) ? 1 : 0);
+  hip_check(hipEventRecord(e1, st), "hipEventRecord");
+  uint64_t h[5] = {0, 0, 0, 0, 0};
+  c->d2h(h, cnt, 40);
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  ev_put(e0);
+  ev_put(e1);
+  if (h[3]) fail(CF1G_E_INTERNAL, std::to_string(h[3]) + " k-mer occurrences are missing from the MPHF");
+  if (h[0] != c->W) fail(CF1G_E_INTERNAL, "P4 classified a different number of occurrences than P1 counted");
+  if (out) {
+    cf1g_cas_stats o{};
+    o.occurrences = h[0];
+    o.cas_issued = h[1];
+    o.cas_failed = h[2];
+    o.chunk = chunk;
+    o.wgs_per_cu = wpc;
+    o.grid = grid;
+    o.n_chunks = n_chunks;
+    o.kernel_ms = ms;
+    *out = o;
+  }
+  c->use_index = 0;
+  c->stage = 5;
+}
+
 // ---------------------------------------------------------------- learned index
 LidxTrainBufs lidx_bufs(cf1g_ctx* c, uint64_t n_segs) {
   LidxTrainBufs b;
@@ -736,6 +795,13 @@ int cf1g_p4_classify(cf1g_ctx* c) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_p4_classify");
     do_p4(c);
+  });
+}
+
+int cf1g_p4_classify_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_p4_classify_cas");
+    do_p4_cas(c, flags, out);
   });
 }
 

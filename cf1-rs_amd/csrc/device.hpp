@@ -312,6 +312,23 @@ __host__ __device__ __forceinline__ uint32_t mask_state(uint32_t m) {
   if (si) return 8u | f;
   return 3u;
 }
+// The label sets a 5-bit state stands for, as a mask that joins like the
+// DFA: a single side keeps its label bit, a multi side is the "none" bit
+// (absorbing under OR). mask_state(state_mask(s) | m) is the state after
+// observation m (classify.rs:202-511 branch by branch); unvisited is 0.
+__host__ __device__ __forceinline__ uint32_t state_mask(uint32_t s) {
+  if (s == 0) return 0;
+  uint32_t I = 1u << LBL_NONE, O = 1u << LBL_NONE;  // MIMO (3); outputted states never reach P4
+  if (s >= 16) {
+    I = 1u << ((s >> 2) & 3u);
+    O = 1u << (s & 3u);
+  } else if (s >= 4 && s <= 7) {
+    O = 1u << (s & 3u);  // MISO: in multi, out single
+  } else if (s >= 8 && s <= 11) {
+    I = 1u << (s & 3u);  // SIMO: in single, out multi
+  }
+  return I | (O << 5);
+}
 
 // ------------------------------------------------------------ wave helpers (wave64)
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

@@ -11,6 +11,16 @@
 // order-free, so P4 here is a streaming pass that encodes each joined mask
 // as its 5-bit state (State::from_vertex, state.rs:92-113) and packs 12 per
 // word.
+//
+// k_p4_cas is the reference's own form of P4, for callers that hold an MPHF
+// and a zeroed state vector and classify the records against them: one lane
+// per k-mer occurrence of the flat stream derives the observation from the
+// two neighbouring bases, looks the canonical k-mer up in the BBHash, and
+// joins the observation into the 5-bit state by a 64-bit CAS loop on the
+// packed word (state_vector.rs:47-70). A persistent grid of work-groups
+// takes fixed-size chunks of stream positions from an atomic counter (work
+// stealing over records of any length; a chunk reads up to K bases past its
+// end, the k-1 overlap). Its states equal k_p4_pack's word for word.
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -59,6 +69,98 @@ __global__ void __launch_bounds__(256) k_p4_pack(const uint64_t* __restrict__ vm
   states[w] = out;
 }
 
+// Observation mask of the k-mer occurrence at stream position g >= 1, or 0
+// when no valid k-mer starts there. Neighbours are the bases at g-1 and g+K
+// (a placeholder or record end is "none"); the occurrence type follows
+// classify.rs:65-177: isolated -> MIMO, no right neighbour -> rightmost,
+// self loop (the next k-mer has the same canonical form, :143-145) -> MIMO
+// when leftmost or a palindromic step, else rightmost (:160-177); otherwise
+// leftmost / internal.
+template <typename KeyT>
+__device__ __forceinline__ uint32_t observe(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t g, int K,
+                                            KeyT& canon) {
+  if (!window_clear(nmask, g, K)) return 0;
+  const bool pv = !is_n(nmask, g - 1), nv = !is_n(nmask, g + K);
+  const KeyT x = KOps<KeyT>::at(seq, g, K);
+  const KeyT r = KOps<KeyT>::rc(x, K);
+  canon = kmer_min(x, r);
+  const uint32_t pb = base_at(seq, g - 1), nb = base_at(seq, g + K);
+  uint32_t type;
+  if (!pv && !nv) {
+    type = OBS_MIMO;
+  } else if (!nv) {
+    type = OBS_RIGHT;
+  } else {
+    KeyT x2 = x, r2 = r;
+    KOps<KeyT>::roll(x2, r2, nb, K);
+    if (key_eq(kmer_min(x2, r2), canon)) type = (!pv || key_eq(x, x2)) ? OBS_MIMO : OBS_RIGHT;
+    else type = pv ? OBS_INTERNAL : OBS_LEFT;
+  }
+  return obs_mask(type, key_eq(x, canon), pb, nb);
+}
+
+// Persistent work-stealing classification by CAS (see the file comment).
+// counters: [0] occurrences, [1] CAS issued, [2] CAS lost to a concurrent
+// update, [3] lookups outside [0, D) (a key missing from the MPHF: never
+// written), [4] the chunk ticket. force_retry makes every lane's first CAS
+// compare against a word with bit 63 set, which no state word has (bits
+// 60-63 are zero), so each occurrence's first attempt fails once and the
+// retry path runs.
+constexpr int P4_WG = 256;
+template <typename KeyT>
+__global__ void __launch_bounds__(P4_WG) k_p4_cas(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t n_pos,
+                                                  int K, const MphfView mv, uint64_t D, unsigned long long* __restrict__ states,
+                                                  unsigned long long* __restrict__ counters, uint32_t chunk, uint64_t n_chunks,
+                                                  int force_retry) {
+  __shared__ uint64_t s_c;
+  __shared__ unsigned long long s_sum[4];
+  if (threadIdx.x < 4) s_sum[threadIdx.x] = 0;
+  uint32_t n_occ = 0, n_cas = 0, n_fail = 0, n_miss = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_c = atomicAdd(&counters[4], 1ULL);
+    __syncthreads();
+    const uint64_t c = s_c;
+    __syncthreads();
+    if (c >= n_chunks) break;
+    const uint64_t g0 = 1 + c * chunk;
+    const uint64_t g1 = g0 + chunk < n_pos ? g0 + chunk : n_pos;
+    for (uint64_t g = g0 + threadIdx.x; g < g1; g += P4_WG) {
+      KeyT canon;
+      const uint32_t m = observe<KeyT>(seq, nmask, g, K, canon);
+      if (!m) continue;
+      n_occ++;
+      const uint64_t idx = mphf_lookup<KeyT>(mv, canon);
+      if (idx >= D) {
+        n_miss++;
+        continue;
+      }
+      unsigned long long* wp = states + idx / 12;
+      const uint32_t sh = (uint32_t)(idx % 12) * 5;
+      unsigned long long cur = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool inject = force_retry != 0;
+      for (;;) {
+        const uint32_t s = (uint32_t)(cur >> sh) & 31u;
+        const uint32_t ns = mask_state(state_mask(s) | m);
+        if (ns == s) break;
+        const unsigned long long want = (cur & ~(31ULL << sh)) | ((unsigned long long)ns << sh);
+        const unsigned long long expect = inject ? (cur | (1ULL << 63)) : cur;
+        inject = false;
+        n_cas++;
+        const unsigned long long seen = atomicCAS(wp, expect, want);
+        if (seen == expect) break;
+        n_fail++;
+        cur = seen;
+      }
+    }
+  }
+  atomicAdd(&s_sum[0], (unsigned long long)n_occ);
+  atomicAdd(&s_sum[1], (unsigned long long)n_cas);
+  atomicAdd(&s_sum[2], (unsigned long long)n_fail);
+  atomicAdd(&s_sum[3], (unsigned long long)n_miss);
+  __syncthreads();
+  if (threadIdx.x < 4) atomicAdd(&counters[threadIdx.x], s_sum[threadIdx.x]);
+}
+
 // Per stream position: bit0 valid k-mer, bit1 FWD, bits2-3 state class.
 template <typename KeyT>
 __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t n_pos,
@@ -90,6 +192,21 @@ void launch_p4_pack(hipStream_t st, const uint16_t* vmask, uint64_t D, uint64_t*
   const uint64_t nw = (D + 11) / 12;
   if (nw == 0) return;
   CF1G_LAUNCH("p4_classify", k_p4_pack, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (const uint64_t*)vmask, D, nw, states);
+}
+// counters: 5 zeroed words (see k_p4_cas). Chunks cover stream positions
+// [1, n_pos); grid = persistent work-groups.
+void launch_p4_cas(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, const MphfView& mv,
+                   uint64_t D, uint64_t* states, uint64_t* counters, uint32_t chunk, uint32_t grid, int force_retry) {
+  if (n_pos <= 1 || chunk == 0 || grid == 0) return;
+  const uint64_t n_chunks = (n_pos - 1 + chunk - 1) / chunk;
+  auto* sv = (unsigned long long*)states;
+  auto* cn = (unsigned long long*)counters;
+  if (!wide)
+    CF1G_LAUNCH("p4_cas", k_p4_cas<uint64_t>, dim3(grid), dim3(P4_WG), 0, st, seq, nmask, n_pos, K, mv, D, sv, cn, chunk, n_chunks,
+                force_retry);
+  else
+    CF1G_LAUNCH("p4_cas", k_p4_cas<Key128>, dim3(grid), dim3(P4_WG), 0, st, seq, nmask, n_pos, K, mv, D, sv, cn, chunk, n_chunks,
+                force_retry);
 }
 void launch_annotate(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
                      const MphfView& mv, const uint64_t* states, uint8_t* out) {

@@ -93,8 +93,35 @@ int cf1g_p2_partition_route(cf1g_ctx* ctx, const uint64_t* hist, uint32_t num_bi
  * out_D (may be NULL) = Mphf::total_kmers (src/mphf.rs:278-280). */
 int cf1g_p3_build(cf1g_ctx* ctx, uint64_t* out_D);
 
-/* P4 — classify_vertices::<K> (src/classify.rs:524-601). */
+/* P4 — classify_vertices::<K> (src/classify.rs:524-601). P3 joined each
+ * distinct k-mer's per-occurrence observations; this encodes the joined
+ * states into the state vector. */
 int cf1g_p4_classify(cf1g_ctx* ctx);
+
+/* P4 in the reference's form — classify_vertices::<K> (src/classify.rs:524-601)
+ * with Classifier::process_* (src/classify.rs:25-519) and
+ * AtomicStateVector::compare_and_swap (src/state_vector.rs:47-70): the state
+ * vector is zeroed, then every k-mer occurrence of the loaded records is
+ * looked up in the MPHF (Mphf::hash, src/mphf.rs:268-270) and joined into its
+ * 5-bit state by a 64-bit compare-and-swap loop. A persistent launch
+ * work-steals fixed-size chunks of positions (cf1g_params.p4_chunk,
+ * p4_wgs_per_cu; 0 = the tuned values). The states equal cf1g_p4_classify's
+ * word for word. flags: CF1G_CAS_FORCE_RETRY fails each occurrence's first
+ * compare-and-swap once (the retry path, as the reference's
+ * compare_exchange_weak may fail spuriously). out may be NULL. */
+#define CF1G_CAS_FORCE_RETRY 1u
+typedef struct cf1g_cas_stats {
+  uint64_t occurrences;  /* k-mer occurrences classified (= cf1g_info.n_kmers) */
+  uint64_t cas_issued;   /* 64-bit compare-and-swap operations issued */
+  uint64_t cas_failed;   /* of which failed (a concurrent update or a forced retry) */
+  uint64_t n_chunks;     /* work-stealing chunks */
+  uint32_t chunk;        /* stream positions per chunk */
+  uint32_t wgs_per_cu;   /* persistent work-groups per CU */
+  uint32_t grid;         /* work-groups launched */
+  uint32_t reserved;
+  double kernel_ms;      /* device time of the classification kernel */
+} cf1g_cas_stats;
+int cf1g_p4_classify_cas(cf1g_ctx* ctx, uint32_t flags, cf1g_cas_stats* out);
 
 /* P1..P4 in order on the loaded records; out_D may be NULL. */
 int cf1g_run_p1_p4(cf1g_ctx* ctx, uint64_t* out_D);
