@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lookups", type=int, default=1 << 28, help="lookups per batch (0: skip the lookup benchmark)")
     ap.add_argument("--lookup-batches", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true", help="keep --num-bins instead of the autotuner's choice")
     ap.add_argument("--p4-cas-runs", type=int, default=3, help="timed runs of the per-occurrence CAS classification (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (MiB) to scale HBM byte counters")
@@ -187,6 +188,12 @@ def main():
     print(f"[bench] generated {len(ascii_) / 1e6:.0f} Mbp in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     g = cf1g.Cf1g(k=args.k, num_bins=args.num_bins)
     g.load_records(ascii=ascii_, rec_off=rec_off)
+    tune = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        tune = g.autotune(reps=2)
+        print(f"[bench] autotuner: {tune['num_bins']} bins, segment target {tune['p3_target_seg']}, P4 chunk "
+              f"{tune['p4_chunk']} x {tune['p4_wgs_per_cu']} WG/CU ({time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         g.run_p1_p4()
         print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -224,7 +231,7 @@ def main():
     dom = max((n for n in kern if n in ab), key=lambda n: kern[n]["ms_per_step"])
     dk = kern[dom]
     achieved = dk["gbs"]
-    tag = "genome300m_k%d" % args.k if args.scale == 1.0 and args.num_bins == 128 else None
+    tag = "genome300m_k%d_b%d" % (args.k, info["num_bins"]) if args.scale == 1.0 else None
     traffic, tsrc = profiled_traffic(dom, tag) if tag else (None, None)
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
@@ -264,7 +271,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"genome-like {len(ascii_) / 1e6:.0f} Mbp, 24 records, k={args.k}, {args.num_bins} bins, seed 3",
+        "config": {"workload": f"genome-like {len(ascii_) / 1e6:.0f} Mbp, 24 records, k={args.k}, "
+                               f"{info['num_bins']} bins{' (autotuned)' if tune else ''}, seed 3",
                    "k": args.k, "bases": int(len(ascii_)), "records": int(len(rec_off) - 1),
                    "stream_len": info["stream_len"], "mphf_words": info["mphf_words"],
                    "kmer_occurrences": info["n_kmers"], "superkmers": info["n_superkmers"], "distinct_kmers": D,
@@ -276,6 +284,7 @@ def main():
         "phase_wall_ms": {k2: round(v * 1000, 3) for k2, v in ph.items()},
         "phase_kernel_ms": {k2: round(v, 3) for k2, v in phases.items()},
         "kernels": {k2: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k2, v in kern.items()},
+        "autotune": tune,
         "p4_cas": p4cas,
         "lookup": lookup,
     }

@@ -30,7 +30,7 @@ NUM_BUCKETS = 65536
 EXPORTS = [
     "cf1g_create", "cf1g_destroy", "cf1g_last_error", "cf1g_get_info", "cf1g_load_records",
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
-    "cf1g_p4_classify_cas", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
+    "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
@@ -70,6 +70,16 @@ class CasStats(C.Structure):
 CAS_FORCE_RETRY = 1
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("a", C.c_uint32), ("b", C.c_uint32), ("c", C.c_uint32), ("ms", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("num_bins", C.c_uint32), ("p3_target_seg", C.c_uint32), ("p4_chunk", C.c_uint32),
+                ("p4_wgs_per_cu", C.c_uint32), ("step_ms", C.c_double), ("p4_cas_ms", C.c_double),
+                ("n_entries", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class LidxParams(C.Structure):
     _fields_ = [("leaf_bits", C.c_uint32), ("root_bits", C.c_uint32), ("epochs", C.c_uint32), ("lr", C.c_float),
                 ("max_window", C.c_uint32), ("untrained", C.c_uint32), ("reserved", C.c_uint32 * 2)]
@@ -105,6 +115,7 @@ def lib():
         L.cf1g_p3_build.argtypes = [C.c_void_p, U64P]
         L.cf1g_p4_classify.argtypes = [C.c_void_p]
         L.cf1g_p4_classify_cas.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(CasStats)]
+        L.cf1g_autotune.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(TuneEntry), C.c_uint32, C.POINTER(TuneResult)]
         L.cf1g_run_p1_p4.argtypes = [C.c_void_p, U64P]
         L.cf1g_states_export.argtypes = [C.c_void_p, U64P, C.c_uint64]
         L.cf1g_keys_export.argtypes = [C.c_void_p, U64P, C.c_uint64]
@@ -232,6 +243,22 @@ class Cf1g:
         o = CasStats()
         self._check(self.L.cf1g_p4_classify_cas(self.h, CAS_FORCE_RETRY if force_retry else 0, C.byref(o)))
         return {f: getattr(o, f) for f, _ in CasStats._fields_ if f != "reserved"}
+
+    def autotune(self, reps=2):
+        """Time the candidate launch configurations on the loaded records and
+        keep the fastest (cf1g_autotune); returns the choice and every
+        measurement."""
+        ents = (TuneEntry * 64)()
+        o = TuneResult()
+        self._check(self.L.cf1g_autotune(self.h, reps, ents, 64, C.byref(o)))
+        res = {f: getattr(o, f) for f, _ in TuneResult._fields_ if f != "reserved"}
+        kinds = {0: ("step", "num_bins", "p3_target_seg"), 1: ("p4_cas", "p4_chunk", "p4_wgs_per_cu")}
+        table = []
+        for e in ents[:o.n_entries]:
+            kind, an, bn = kinds[e.kind]
+            table.append({"kind": kind, an: e.a, bn: e.b, "ms": round(e.ms, 4)})
+        res["table"] = table
+        return res
 
     def run_p1_p4(self):
         d = C.c_uint64(0)

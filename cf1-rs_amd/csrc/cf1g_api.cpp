@@ -509,6 +509,91 @@ void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
   c->stage = 5;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Times candidate launch configurations on the loaded records and keeps the
+// fastest: bin count x P3 segment target over whole P1..P4 steps (host wall
+// clock around the synchronised phases, the minimum of `reps` runs after
+// one warm-up), then P4 chunk x work-groups per CU over the CAS
+// classification kernel (HIP events). The context keeps the chosen values
+// and ends with the products of the chosen configuration (stage 5).
+void do_autotune(cf1g_ctx* c, uint32_t reps, cf1g_tune_entry* ent, uint32_t cap, cf1g_tune_result* out) {
+  static const uint32_t kBins[] = {128, 256, 512};
+  static const uint32_t kTarget[] = {1024, 1536, 2048};
+  static const uint32_t kChunk[] = {2048, 8192, 32768};
+  static const uint32_t kWgs[] = {4, 8, 16};
+  if (reps < 1 || reps > 16) fail(CF1G_E_INVALID, "reps must be in [1, 16]");
+  std::vector<cf1g_tune_entry> all;
+  const cf1g_params saved = c->prm;
+  double best_step = 1e300, best_cas = 1e300;
+  uint32_t bb = saved.num_bins, bt = saved.p3_target_seg ? saved.p3_target_seg : 1536;
+  for (uint32_t nb : kBins) {
+    for (uint32_t tg : kTarget) {
+      c->prm.num_bins = nb;
+      c->prm.p3_target_seg = tg;
+      double best = 1e300;
+      for (uint32_t r = 0; r <= reps; r++) {
+        c->sync();
+        auto t0 = std::chrono::steady_clock::now();
+        do_p1(c);
+        do_p2(c, nullptr, nb);
+        do_p3(c);
+        do_p4(c);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (r > 0) best = std::min(best, ms);
+      }
+      all.push_back(cf1g_tune_entry{CF1G_TUNE_STEP, nb, tg, c->num_bins, best});
+      if (best < best_step) {
+        best_step = best;
+        bb = nb;
+        bt = tg;
+      }
+    }
+  }
+  c->prm.num_bins = bb;
+  c->prm.p3_target_seg = bt;
+  do_p1(c);
+  do_p2(c, nullptr, bb);
+  do_p3(c);
+  do_p4(c);
+  uint32_t bc = c->tuned_p4_chunk, bw = c->tuned_p4_wgs;
+  for (uint32_t ch : kChunk) {
+    for (uint32_t wg : kWgs) {
+      c->prm.p4_chunk = ch;
+      c->prm.p4_wgs_per_cu = wg;
+      double best = 1e300;
+      for (uint32_t r = 0; r <= reps; r++) {
+        cf1g_cas_stats cs{};
+        do_p4_cas(c, 0, &cs);
+        if (r > 0) best = std::min(best, cs.kernel_ms);
+      }
+      all.push_back(cf1g_tune_entry{CF1G_TUNE_P4_CAS, ch, wg, 0, best});
+      if (best < best_cas) {
+        best_cas = best;
+        bc = ch;
+        bw = wg;
+      }
+    }
+  }
+  c->prm.p4_chunk = saved.p4_chunk;
+  c->prm.p4_wgs_per_cu = saved.p4_wgs_per_cu;
+  c->tuned_p4_chunk = bc;
+  c->tuned_p4_wgs = bw;
+  do_p4(c);  // the state vector of the chosen step configuration
+  if (ent)
+    for (uint32_t i = 0; i < cap && i < all.size(); i++) ent[i] = all[i];
+  if (out) {
+    cf1g_tune_result o{};
+    o.num_bins = bb;
+    o.p3_target_seg = bt;
+    o.p4_chunk = bc;
+    o.p4_wgs_per_cu = bw;
+    o.step_ms = best_step;
+    o.p4_cas_ms = best_cas;
+    o.n_entries = (uint32_t)all.size();
+    *out = o;
+  }
+}
+
 // ---------------------------------------------------------------- learned index
 LidxTrainBufs lidx_bufs(cf1g_ctx* c, uint64_t n_segs) {
   LidxTrainBufs b;
@@ -802,6 +887,13 @@ int cf1g_p4_classify_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_p4_classify_cas");
     do_p4_cas(c, flags, out);
+  });
+}
+
+int cf1g_autotune(cf1g_ctx* c, uint32_t reps, cf1g_tune_entry* entries, uint32_t cap, cf1g_tune_result* out) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_autotune");
+    do_autotune(c, reps, entries, cap, out);
   });
 }
 

@@ -123,6 +123,31 @@ typedef struct cf1g_cas_stats {
 } cf1g_cas_stats;
 int cf1g_p4_classify_cas(cf1g_ctx* ctx, uint32_t flags, cf1g_cas_stats* out);
 
+/* Autotuner (no reference counterpart: the reference takes --num-bins from
+ * the command line, src/params.rs:58-60, and sizes its tasks by fixed
+ * constants, src/classify.rs:576-594). On the loaded records it times
+ * bin counts {128, 256, 512} x P3 segment targets {1024, 1536, 2048} over
+ * whole P1..P4 steps, and P4 chunks {2048, 8192, 32768} x work-groups per CU
+ * {4, 8, 16} over the CAS classification, `reps` timed runs each after a
+ * warm-up (minimum kept). The context then uses the fastest values (they
+ * replace num_bins and p3_target_seg of its params) and holds the products
+ * of the chosen configuration. entries (may be NULL) receives up to cap
+ * measurements. Bin count and segment target change the MPHF's partition
+ * (exported), never the graph: keys, states per k-mer and P5 output. */
+#define CF1G_TUNE_STEP 0u   /* a = bins, b = P3 segment target, c = bins after partitioning */
+#define CF1G_TUNE_P4_CAS 1u /* a = chunk, b = work-groups per CU */
+typedef struct cf1g_tune_entry {
+  uint32_t kind, a, b, c;
+  double ms;
+} cf1g_tune_entry;
+typedef struct cf1g_tune_result {
+  uint32_t num_bins, p3_target_seg, p4_chunk, p4_wgs_per_cu;
+  double step_ms;   /* best P1..P4 step */
+  double p4_cas_ms; /* best CAS classification kernel */
+  uint32_t n_entries, reserved;
+} cf1g_tune_result;
+int cf1g_autotune(cf1g_ctx* ctx, uint32_t reps, cf1g_tune_entry* entries, uint32_t cap, cf1g_tune_result* out);
+
 /* P1..P4 in order on the loaded records; out_D may be NULL. */
 int cf1g_run_p1_p4(cf1g_ctx* ctx, uint64_t* out_D);
 
