@@ -32,7 +32,7 @@ EXPORTS = [
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
-    "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_profiling", "cf1g_kernel_times",
+    "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_graphs", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
     "cf1g_lidx_gradients", "cf1g_classify_via", "cf1g_select_index", "cf1g_selected_index",
 ]
@@ -58,7 +58,8 @@ class CfInfo(C.Structure):
     _fields_ = [("n_records", C.c_uint64), ("stream_len", C.c_uint64), ("n_kmers", C.c_uint64),
                 ("n_superkmers", C.c_uint64), ("n_distinct", C.c_uint64), ("n_state_words", C.c_uint64),
                 ("mphf_words", C.c_uint64), ("num_bins", C.c_uint32), ("seg_bits", C.c_uint32),
-                ("mphf_big_segments", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32)]
+                ("mphf_big_segments", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32),
+                ("graph_captures", C.c_uint64), ("graph_replays", C.c_uint64)]
 
 
 class CasStats(C.Structure):
@@ -138,6 +139,7 @@ def lib():
         L.cf1g_p5_write.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64, C.c_char_p, U64P, C.c_char_p, C.c_char_p,
                                     C.c_int, C.c_int, C.c_int, U64P]
         L.cf1g_sync.argtypes = [C.c_void_p]
+        L.cf1g_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.cf1g_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.cf1g_kernel_times.restype = C.c_size_t
         L.cf1g_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]
@@ -347,6 +349,9 @@ class Cf1g:
     # ---- measurement
     def sync(self):
         self._check(self.L.cf1g_sync(self.h))
+
+    def set_graphs(self, on):
+        self._check(self.L.cf1g_set_graphs(self.h, 1 if on else 0))
 
     def set_profiling(self, on):
         self._check(self.L.cf1g_set_profiling(self.h, int(on)))

@@ -34,6 +34,7 @@ void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64
                      const uint16_t*, int, const uint64_t*, const uint64_t*, uint64_t*, int, unsigned int*);
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
+void launch_zero(hipStream_t, void*, uint64_t);
 void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*, uint64_t*);
 int p3_part_bits(int);
 void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int, uint64_t*,
@@ -134,6 +135,15 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// A captured launch sequence and the sizes / allocation generation it was
+// captured with.
+struct GraphSeg {
+  std::vector<uint64_t> key;
+  hipGraphExec_t exec = nullptr;
+  size_t nodes = 0;
+  bool valid = false;
+};
+
 struct cf1g_ctx {
   cf1g_params prm{};
   int k = 31, m = 15, w = 17;
@@ -166,6 +176,11 @@ struct cf1g_ctx {
   // launch shapes the autotuner may change (cf1g_autotune); the params'
   // non-zero fields take precedence
   uint32_t tuned_p4_chunk = 8192, tuned_p4_wgs = 8;
+  // hipGraph segments (cf1g_set_graphs); alloc_gen changes whenever a
+  // device buffer is (re)allocated, which invalidates every captured graph
+  bool graphs_on = true;
+  uint64_t alloc_gen = 0, graph_captures = 0, graph_replays = 0;
+  std::map<std::string, GraphSeg> graphs;
 
   void* get(const std::string& name, size_t bytes) {
     DevBuf& b = bufs[name];
@@ -177,6 +192,7 @@ struct cf1g_ctx {
       size_t want = bytes + bytes / 8;  // headroom for repeated runs
       hip_check(hipMalloc(&b.p, want), ("hipMalloc " + name).c_str());
       b.cap = want;
+      alloc_gen++;
     }
     return b.p;
   }
@@ -209,7 +225,49 @@ struct cf1g_ctx {
     return v;
   }
   void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
-  void zero(void* p, size_t bytes) { hip_check(hipMemsetAsync(p, 0, bytes, st), "hipMemsetAsync"); }
+  void zero(void* p, size_t bytes) { launch_zero(st, p, bytes); }
+  // Run `launches` (kernel launches on st only: no allocation, copy, memset
+  // or synchronisation) through a hipGraph captured under `name`, replayed
+  // while `key` and the allocation generation are unchanged. Eager when
+  // graphs are off or per-kernel timing is on.
+  template <typename F>
+  void seg(const char* name, std::vector<uint64_t> key, F&& launches) {
+    if (!graphs_on || g_prof_on) {
+      launches();
+      return;
+    }
+    key.push_back(alloc_gen);
+    key.push_back((uint64_t)wide);
+    GraphSeg& g = graphs[name];
+    if (!g.valid || g.key != key) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+      g.valid = false;
+      hipGraph_t graph = nullptr;
+      hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+      try {
+        launches();
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(st, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+      size_t n = 0;
+      hipError_t e = hipGraphGetNodes(graph, nullptr, &n);
+      if (e == hipSuccess && n) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      hip_check(e, "hipGraphInstantiate");
+      g.nodes = n;
+      g.key = key;
+      g.valid = true;
+      graph_captures++;
+    } else {
+      graph_replays++;
+    }
+    if (g.nodes) hip_check(hipGraphLaunch(g.exec, st), "hipGraphLaunch");
+  }
   void d2h(void* dst, const void* src, size_t bytes) {
     hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "hipMemcpy D2H");
     sync();
@@ -263,22 +321,32 @@ uint64_t level_bits(uint64_t remaining) {
 }
 
 // ---------------------------------------------------------------- phases
+// Each phase's launches between host synchronisations form one segment
+// (cf1g_ctx::seg): captured into a hipGraph the first time and replayed
+// while the sizes and buffers it was captured with are unchanged.
 void do_p1(cf1g_ctx* c) {
-  hipStream_t st = c->st;
+  const uint64_t nt = p1_tiles(c->n_pos), nsl = p1_slices(c->n_pos);
+  uint8_t* ascii = c->getT<uint8_t>("ascii", c->n_ascii + 8);
+  uint64_t* roff = c->getT<uint64_t>("rec_off", c->n_rec + 1);
+  uint64_t* gst = c->getT<uint64_t>("gstart", c->n_rec + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
-  launch_pack(st, c->getT<uint8_t>("ascii", c->n_ascii + 8), c->getT<uint64_t>("rec_off", c->n_rec + 1),
-              c->getT<uint64_t>("gstart", c->n_rec + 1), c->n_rec, c->n_words64, seq, nm);
   auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
   auto* tot = c->getT<unsigned long long>("totals", 4);
-  c->zero(hist, CF1G_NUM_BUCKETS * 8);
-  c->zero(tot, 32);
   // super k-mer entries: one slot of tile-size per scan tile; partial
   // histograms per slice of tiles
-  const uint64_t nt = p1_tiles(c->n_pos), nsl = p1_slices(c->n_pos);
-  launch_p1(st, seq, nm, c->n_pos, c->k, c->m, c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1),
-            c->getT<uint32_t>("sk_tile_cnt", nt + 1), c->getT<uint32_t>("sk_tile_w", nt + 1),
-            c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1), c->getT<uint64_t>("p1_sums", 2 * nsl + 2), hist, tot);
+  uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
+  uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
+  uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
+  uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
+  uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
+  c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m}, [&] {
+    hipStream_t st = c->st;
+    launch_pack(st, ascii, roff, gst, c->n_rec, c->n_words64, seq, nm);
+    c->zero(hist, CF1G_NUM_BUCKETS * 8);
+    c->zero(tot, 32);
+    launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, tcnt, tw, part, sums, hist, tot);
+  });
   c->stage = 2;
 }
 
@@ -297,35 +365,44 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   auto* bcur = c->getT<unsigned long long>("bin_cursor", CF1G_MAX_BINS + 1);
   uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
   uint64_t* info = c->getT<uint64_t>("info", 4);
-  launch_partition(st, hpart, hist, num_bins, CHUNK_RECORDS, b2b, brec, bstart, bcur, coff, info);
+  uint64_t* totals = c->getT<uint64_t>("totals", 4);
+  c->seg("p2a", {num_bins, hist_in ? 1u : 0u}, [&] {
+    launch_partition(c->st, hpart, hist, num_bins, CHUNK_RECORDS, b2b, brec, bstart, bcur, coff, info);
+  });
   uint64_t hinfo[4], htot[4];
   hip_check(hipMemcpyAsync(hinfo, info, 32, hipMemcpyDeviceToHost, st), "D2H info");
-  hip_check(hipMemcpyAsync(htot, c->getT<uint64_t>("totals", 4), 32, hipMemcpyDeviceToHost, st), "D2H totals");
+  hip_check(hipMemcpyAsync(htot, totals, 32, hipMemcpyDeviceToHost, st), "D2H totals");
   c->sync();
   c->num_bins = (uint32_t)hinfo[0];
   c->S = hinfo[1];
   c->n_chunks = hinfo[2];
   c->W = htot[0];
   if (c->S != htot[1]) fail(CF1G_E_INTERNAL, "bin record total differs from the P1 super k-mer count");
-  uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
-  unsigned int* ovf = c->getT<unsigned int>("route_overflow", 2);
-  c->zero(ovf, 4);
   // count records per (bin, chunk), scan bin-major, write
   const uint64_t nt = p1_tiles(c->n_pos);
   uint64_t chunk_tiles = 0, n_chunks = 0;
   p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
   const uint64_t nm_ = (uint64_t)c->num_bins * n_chunks;
+  uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
+  unsigned int* ovf = c->getT<unsigned int>("route_overflow", 2);
   uint64_t* mat = c->getT<uint64_t>("p2_mat", nm_ + 1);
   uint64_t* moff = c->getT<uint64_t>("p2_off", nm_ + 1);
+  uint64_t* stmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(nm_ + 1) + 16);
   const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
   const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
-  launch_p2_count(st, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, b2b, (int)c->num_bins, mat);
-  scan_excl_u64(st, mat, moff, nm_, moff + nm_, c->getT<uint64_t>("scan_tmp", scan_temp_words(nm_ + 1) + 16));
-  launch_p2_route(st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), ent, tcnt,
-                  c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, moff, bstart, rec, c->rw, ovf);
+  uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
+  uint64_t* nmask = c->getT<uint64_t>("nmask", c->n_words64 + 8);
+  c->seg("p2b", {c->n_pos, c->num_bins, c->S, chunk_tiles, n_chunks, (uint64_t)c->rw, (uint64_t)c->k}, [&] {
+    hipStream_t s2 = c->st;
+    c->zero(ovf, 8);
+    launch_p2_count(s2, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, b2b, (int)c->num_bins, mat);
+    scan_excl_u64(s2, mat, moff, nm_, moff + nm_, stmp);
+    launch_p2_route(s2, seq, nmask, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, moff, bstart, rec, c->rw,
+                    ovf);
+  });
   uint64_t routed = 0;
   unsigned int hov = 0;
-  if (nm_) c->d2h(&routed, moff + nm_, 8);
+  if (nm_) hip_check(hipMemcpyAsync(&routed, moff + nm_, 8, hipMemcpyDeviceToHost, st), "D2H routed");
   c->d2h(&hov, ovf, 4);
   if (hov || routed != c->S) fail(CF1G_E_INTERNAL, "P2 routing disagrees with the bin sizes from the histogram");
   c->stage = 3;
@@ -333,9 +410,11 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
 
 // P3: count, scan, two-level scatter (keys + observation masks into parts,
 // then segments), per-segment dedup / mask join / BBHash into padded block
-// slots (keys and masks in local-rank order), big-segment sorts and BBHash,
-// scans of the distinct counts (MPHF key offsets) and block counts, then one
-// compaction of the masks (MPHF-value order) and blocks; keys on first use.
+// slots (keys and masks in local-rank order), scans of the distinct counts
+// (MPHF key offsets) and block counts — one segment up to here — then, for
+// the rare segments above LDS capacity, the global-memory sort / BBHash and
+// the scans again; then one compaction of the masks (MPHF-value order) and
+// blocks. The dense key array is compacted on first use (dense_keys).
 void do_p3(cf1g_ctx* c) {
   hipStream_t st = c->st;
   const size_t kb = c->key_bytes;
@@ -361,44 +440,55 @@ void do_p3(cf1g_ctx* c) {
   const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   const uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
   const uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
-  c->zero(counts, (ns + 1) * 8);
-  c->zero(flags, 16);
   // per-(bin, part, chunk) totals -> scan -> each chunk's run in its part
   const int pb = p3_part_bits(sb);
   const uint64_t nm = ((uint64_t)1 << pb) * c->n_chunks;
   uint64_t* pmat = c->getT<uint64_t>("p3_pmat", nm + 1);
   uint64_t* poff = c->getT<uint64_t>("p3_poff", nm + 1);
-  launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts, pmat);
-  scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
   uint64_t* tmp2 = c->getT<uint64_t>("scan_tmp2", scan_temp_words(nm + 1) + 16);
-  scan_excl_u64(st, pmat, poff, nm, poff + nm, tmp2);
+  void* occ = c->get("occ", c->W * kb + 16);
+  uint16_t* occ_obs = c->getT<uint16_t>("occ_obs", c->W + 8);
+  // pass A target (parts) when it differs from the segments; it is free once
+  // pass B has run and then holds the distinct keys (kout); k_p3_bigsort
+  // uses it only for the oversized segments, which k_p3_dedup skips
+  void* aux = c->get("occ_aux", c->W * kb + 16);
+  uint16_t* aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
+  // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
+  uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
+  uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
+  auto scans = [&] {
+    scan_excl_u64(st, ucount, key_off, ns, key_off + ns, tmp);
+    scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
+  };
+  const std::vector<uint64_t> shape = {c->S, c->W, c->n_chunks, c->num_bins, (uint64_t)sb, (uint64_t)K, (uint64_t)c->rw};
+  c->seg("p3a", shape, [&] {
+    c->zero(counts, (ns + 1) * 8);
+    c->zero(flags, 16);
+    launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts, pmat);
+    scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
+    scan_excl_u64(st, pmat, poff, nm, poff + nm, tmp2);
+  });
+  // the occurrence buffers are sized by W: check before anything is scattered
   uint64_t wcount = 0;
   c->d2h(&wcount, seg_off + ns, 8);
   if (wcount != c->W) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
-  void* occ = c->get("occ", c->W * kb + 16);
-  uint16_t* occ_obs = c->getT<uint16_t>("occ_obs", c->W + 8);
-  if (pb == sb) {
-    launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, occ, occ_obs);
-  } else {
-    void* part = c->get("occ_aux", c->W * kb + 16);
-    uint16_t* part_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
-    launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, part, part_obs);
-    launch_p3_split(st, c->wide, part, part_obs, seg_off, c->num_bins, sb, occ, occ_obs);
-  }
-  // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
-  uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
-  // distinct keys and joined masks in local-rank order per segment (occ_aux is
-  // free once P3 pass B has run; k_p3_bigsort's use of it is limited to the
-  // oversized segments, which k_p3_dedup skips)
-  void* kout = c->get("occ_aux", c->W * kb + 16);
-  uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
-  launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, kout, vpad);
+  c->seg("p3b", shape, [&] {
+    if (pb == sb) {
+      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, occ, occ_obs);
+    } else {
+      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, aux, aux_obs);
+      launch_p3_split(st, c->wide, aux, aux_obs, seg_off, c->num_bins, sb, occ, occ_obs);
+    }
+    launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, aux, vpad);
+    scans();
+  });
   unsigned int hf[3];
-  c->d2h(hf, flags, 12);
+  uint64_t tot[3] = {0, 0, 0};
+  hip_check(hipMemcpyAsync(hf, flags, 12, hipMemcpyDeviceToHost, st), "D2H flags");
+  hip_check(hipMemcpyAsync(&tot[1], key_off + ns, 8, hipMemcpyDeviceToHost, st), "D2H D");
+  c->d2h(&tot[2], blk_off + ns, 8);
   c->n_over = hf[0];
   if (hf[0] > 0) {
-    void* aux = c->get("occ_aux", c->W * kb + 16);
-    uint16_t* aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
     launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over, hf[0], ucount, flags, big);
     c->d2h(hf, flags, 12);
   }
@@ -411,26 +501,29 @@ void do_p3(cf1g_ctx* c) {
     const uint64_t alive_words = c->W / 64 + 2;
     uint64_t* alive = c->getT<uint64_t>("bbh_alive", alive_words);
     hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
-    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, kout, flags + 2);
+    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, aux, flags + 2);
   }
-  scan_excl_u64(st, ucount, key_off, ns, key_off + ns, tmp);
-  scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
-  uint64_t tot[2] = {0, 0};
-  c->d2h(&tot[0], key_off + ns, 8);
-  c->d2h(&tot[1], blk_off + ns, 8);
-  const uint64_t D = tot[0];
+  if (hf[0] || c->n_big) {
+    scans();
+    hip_check(hipMemcpyAsync(hf, flags, 12, hipMemcpyDeviceToHost, st), "D2H flags");
+    hip_check(hipMemcpyAsync(&tot[1], key_off + ns, 8, hipMemcpyDeviceToHost, st), "D2H D");
+    c->d2h(&tot[2], blk_off + ns, 8);
+  }
+  if (hf[2]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit (flags " + std::to_string(hf[2]) + ")");
+  const uint64_t D = tot[1];
   c->D = D;
-  c->n_blocks = tot[1];
+  c->n_blocks = tot[2];
   const uint64_t nsw = (D + 11) / 12;
   uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
+  uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
   // masks and blocks now; the dense key array (which the reference's MPHF
-  // does not keep) is compacted from kout on first use, dense_keys()
-  launch_p3_compact(st, c->wide, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, vmask, blocks);
+  // does not keep) is compacted from occ_aux on first use, dense_keys()
+  c->seg("p3c", {ns, D, c->n_blocks}, [&] {
+    launch_p3_compact(st, c->wide, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, vmask, blocks);
+    c->zero(states, (nsw + 1) * 8);
+  });
   c->keys_ready = false;
-  c->zero(c->getT<uint64_t>("states", nsw + 1), (nsw + 1) * 8);
-  c->d2h(hf, flags, 12);
-  if (hf[2]) fail(CF1G_E_INTERNAL, "BBHash build exceeded its level or region limit (flags " + std::to_string(hf[2]) + ")");
   c->lidx_ready = c->lidx_states = false;
   c->use_index = 0;
   c->stage = 4;
@@ -459,8 +552,10 @@ void* dense_keys(cf1g_ctx* c) {
 
 // P4: the joined masks, already in MPHF-value order, become packed states.
 void do_p4(cf1g_ctx* c) {
-  uint64_t nsw = (c->D + 11) / 12;
-  launch_p4_pack(c->st, c->getT<uint16_t>("vmask", 12 * nsw + 16), c->D, c->getT<uint64_t>("states", nsw + 1));
+  const uint64_t nsw = (c->D + 11) / 12;
+  uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
+  uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
+  c->seg("p4", {c->D}, [&] { launch_p4_pack(c->st, vmask, c->D, states); });
   c->sync();
   c->stage = 5;
 }
@@ -790,6 +885,8 @@ int cf1g_create(const cf1g_params* params, cf1g_ctx** out) {
 void cf1g_destroy(cf1g_ctx* c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto& kv : c->graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -815,6 +912,8 @@ int cf1g_get_info(cf1g_ctx* c, cf1g_info* o) {
     o->n_oversized = c->n_over;
     o->k = (uint32_t)c->k;
     o->m = (uint32_t)c->m;
+    o->graph_captures = c->graph_captures;
+    o->graph_replays = c->graph_replays;
   });
 }
 
@@ -1016,6 +1115,10 @@ int cf1g_bins_export(cf1g_ctx* c, uint32_t* rec_words, uint64_t* bin_start, uint
       c->d2h(records, c->get("rec", nw * 8 + 8), nw * 8);
     }
   });
+}
+
+int cf1g_set_graphs(cf1g_ctx* c, int on) {
+  return guarded(c, [&]() { c->graphs_on = on != 0; });
 }
 
 int cf1g_sync(cf1g_ctx* c) {

@@ -13,6 +13,8 @@
 // segments above the LDS capacity take a global-memory radix sort
 // (k_p3_bigsort) and k_bbh_big. k_p3_compact then packs keys and masks (both
 // in value order) and the blocks densely.
+#include <algorithm>
+
 #include "block.hpp"
 #include "device.hpp"
 #include "launch.hpp"
@@ -53,6 +55,22 @@ __global__ void __launch_bounds__(SC_WG) k_scan_add(uint64_t* __restrict__ out, 
   if (total_out && blockIdx.x == 0 && threadIdx.x == 0) *total_out = *total_src;
 }
 
+// Zero `bytes` bytes at p (p 8-byte aligned) / copy one word: kernels, so
+// that a captured launch sequence holds no memset or copy nodes.
+__global__ void __launch_bounds__(256) k_zero(uint64_t* __restrict__ p, uint64_t words, uint8_t* __restrict__ tail, uint32_t tail_bytes) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < tail_bytes) tail[threadIdx.x] = 0;
+}
+__global__ void k_copy_word(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src) { *dst = *src; }
+void launch_zero(hipStream_t st, void* p, uint64_t bytes) {
+  if (bytes == 0) return;
+  const uint64_t words = bytes / 8;
+  const uint32_t tail = (uint32_t)(bytes % 8);
+  const uint64_t g = std::min<uint64_t>(std::max<uint64_t>((words + 255) / 256, 1), 4096);
+  CF1G_LAUNCH("zero", k_zero, dim3((unsigned)g), dim3(256), 0, st, (uint64_t*)p, words, (uint8_t*)p + 8 * words, tail);
+}
+
 uint64_t scan_temp_words(uint64_t n) {
   uint64_t t = 0;
   while (n > 1) {
@@ -65,14 +83,14 @@ uint64_t scan_temp_words(uint64_t n) {
 // grand total is written to *total (device). tmp: scan_temp_words(n) words.
 void scan_excl_u64(hipStream_t st, const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total, uint64_t* tmp) {
   if (n == 0) {
-    if (total) hip_check(hipMemsetAsync(total, 0, 8, st), "memset");
+    if (total) launch_zero(st, total, 8);
     return;
   }
   uint64_t nb = (n + SC_TILE - 1) / SC_TILE;
   uint64_t* sums = tmp;
   CF1G_LAUNCH("scan_tile", k_scan_tile, dim3((unsigned)nb), dim3(SC_WG), 0, st, in, out, n, sums);
   if (nb == 1) {
-    if (total) hip_check(hipMemcpyAsync(total, sums, 8, hipMemcpyDeviceToDevice, st), "copy");
+    if (total) CF1G_LAUNCH("scan_total", k_copy_word, dim3(1), dim3(1), 0, st, total, (const uint64_t*)sums);
     return;
   }
   uint64_t* sub_total = sums + nb;  // one word

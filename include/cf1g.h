@@ -60,6 +60,8 @@ typedef struct cf1g_info {
   uint32_t mphf_big_segments; /* MPHF segments built through global memory */
   uint32_t n_oversized;    /* P3 segments sorted through global memory */
   uint32_t k, m;
+  uint64_t graph_captures; /* phase launch sequences captured into hipGraphs */
+  uint64_t graph_replays;  /* ... and replayed */
 } cf1g_info;
 
 /* Create / destroy a context (device 0 of the current process). */
@@ -255,9 +257,16 @@ int cf1g_classify_via(cf1g_ctx* ctx, int index);
 int cf1g_select_index(cf1g_ctx* ctx, int index);
 int cf1g_selected_index(const cf1g_ctx* ctx);
 
-/* ---- measurement */
+/* ---- execution and measurement */
+/* Each phase's kernel launches between host synchronisations run as one
+ * hipGraph, captured on first use and replayed while the sizes and device
+ * buffers it was captured with are unchanged (on by default; 0 launches
+ * them one by one). No reference counterpart (the reference runs rayon
+ * tasks, src/pipeline.rs:66-118). */
+int cf1g_set_graphs(cf1g_ctx* ctx, int on);
 int cf1g_sync(cf1g_ctx* ctx);
-/* Enable per-kernel HIP-event timing on the context stream (0/1). */
+/* Enable per-kernel HIP-event timing on the context stream (0/1); phases
+ * then launch kernel by kernel (no graphs). */
 int cf1g_set_profiling(cf1g_ctx* ctx, int on);
 /* Kernel timing summary as text lines "name count total_ms" ('\n'-separated);
  * returns the needed buffer size (including the NUL). Resets when reset != 0. */
