@@ -31,6 +31,19 @@ sys.path.insert(0, os.path.join(ROOT, "cf1-rs_amd"))
 METRIC = "distinct k-mers/s phases 1-4 at k=31 vs CPU; learned-index vs BBHash lookups/s"
 HBM_PEAK_GBS = 8000.0
 
+# workload name -> (generator(synth, scale, k) -> (ascii, rec_off, names), description(bases, records))
+WORKLOADS = {
+    "genome300m": (lambda sy, sc, k: sy.genome_300m(scale=sc),
+                   lambda nb, nr: f"genome-like {nb / 1e6:.0f} Mbp, {nr} records (Zipf lengths, GC 41% with drift, "
+                                  f"5% repeat families, poly-N runs), seed 3"),
+    "genome5m": (lambda sy, sc, k: sy.genome_5m(scale=sc),
+                 lambda nb, nr: f"one {nb / 1e6:.0f} Mbp uniform record with 200 planted 2 kbp repeats (half reverse-"
+                                f"complemented), seed 1"),
+    "transcriptome": (lambda sy, sc, k: sy.transcriptome(scale=sc, k=k),
+                      lambda nb, nr: f"transcriptome-like {nr} records, {nb / 1e6:.0f} Mbp (lognormal lengths from "
+                                     f"400k shared 150 bp exons, 2% shorter than k), seed 2"),
+}
+
 PHASE_OF = {
     "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
     "p2_partition": "P2", "p2_count": "P2", "p2_route": "P2",
@@ -113,20 +126,25 @@ def algorithmic_bytes(info, kb, rw):
     }
 
 
-def profiled_traffic(kernel, workload_tag):
+def profiled_traffic(kernel, workload, k, bins):
     """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, each
     scaled by the counter's reading for a 1 GiB device copy) from the
-    committed counter summary of this workload, or None."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload_tag}.md")
-    if not os.path.exists(path):
-        return None, None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_dispatch:"):
-            for tok in line.split(":", 1)[1].split():
-                name, val = tok.split("=")
-                if name == kernel:
-                    return float(val), os.path.relpath(path, ROOT)
-    return None, os.path.relpath(path, ROOT)
+    committed counter summary of this workload, k and bin count; when the
+    autotuner chose a bin count without a summary, the summary of the same
+    workload and k at another bin count (named in the returned source), or
+    None."""
+    import glob
+
+    exact = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_k{k}_b{bins}.md")
+    paths = [exact] if os.path.exists(exact) else sorted(glob.glob(os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_k{k}_b*.md")))
+    for path in paths:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                for tok in line.split(":", 1)[1].split():
+                    name, val = tok.split("=")
+                    if name == kernel:
+                        return float(val), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(ascii_, rec_off, k, num_bins, sample_bases, threads):
@@ -158,8 +176,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="genome300m", choices=sorted(WORKLOADS),
+                    help="genome300m (the headline: 24 records, 300 Mbp), genome5m (one 5 Mbp record with 200 planted "
+                         "2 kbp repeats), transcriptome (100k records built from shared exons)")
     ap.add_argument("--k", type=int, default=31)
-    ap.add_argument("--num-bins", type=int, default=128)
+    ap.add_argument("--num-bins", type=int, default=128, help="bins when the autotuner is off")
+    ap.add_argument("--p3-target-seg", type=int, default=0, help="P3 segment target when the autotuner is off (0: 1536)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cpu-sample-mbp", type=float, default=60.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -184,9 +206,10 @@ def main():
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb << 20)
     t = time.perf_counter()
-    ascii_, rec_off, names = synth.genome_300m(scale=args.scale)
+    gen, wdesc = WORKLOADS[args.workload]
+    ascii_, rec_off, names = gen(synth, args.scale, args.k)
     print(f"[bench] generated {len(ascii_) / 1e6:.0f} Mbp in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    g = cf1g.Cf1g(k=args.k, num_bins=args.num_bins)
+    g = cf1g.Cf1g(k=args.k, num_bins=args.num_bins, p3_target_seg=args.p3_target_seg)
     g.load_records(ascii=ascii_, rec_off=rec_off)
     tune = None
     if not args.no_autotune:
@@ -245,8 +268,7 @@ def main():
     dom = max((n for n in kern if n in ab), key=lambda n: kern[n]["ms_per_step"])
     dk = kern[dom]
     achieved = dk["gbs"]
-    tag = "genome300m_k%d_b%d" % (args.k, info["num_bins"]) if args.scale == 1.0 else None
-    traffic, tsrc = profiled_traffic(dom, tag) if tag else (None, None)
+    traffic, tsrc = profiled_traffic(dom, args.workload, args.k, info["num_bins"]) if args.scale == 1.0 else (None, None)
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
             "bytes_per_launch": ab[dom] / dk["launches_per_step"], "mean_launch_ms": round(dk["mean_launch_ms"], 4)}
@@ -285,8 +307,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"genome-like {len(ascii_) / 1e6:.0f} Mbp, 24 records, k={args.k}, "
-                               f"{info['num_bins']} bins{' (autotuned)' if tune else ''}, seed 3",
+        "config": {"name": args.workload, "workload": f"{wdesc(len(ascii_), len(rec_off) - 1)}, k={args.k}, "
+                               f"{info['num_bins']} bins{' (autotuned)' if tune else ''}",
                    "k": args.k, "bases": int(len(ascii_)), "records": int(len(rec_off) - 1),
                    "stream_len": info["stream_len"], "mphf_words": info["mphf_words"],
                    "kmer_occurrences": info["n_kmers"], "superkmers": info["n_superkmers"], "distinct_kmers": D,
