@@ -167,7 +167,7 @@ def cpu_baseline(ascii_, rec_off, k, num_bins, sample_bases, threads):
     d = p.run_p1_p4()
     dt = time.perf_counter() - t
     return {"value": d / dt, "unit": "distinct k-mers/s", "cores": threads, "kind": "port",
-            "sample": f"first {tot / 1e6:.0f} Mbp of the same genome ({len(seqs)} records), D={d}, {dt:.2f} s for P1-P4",
+            "sample": f"first {tot / 1e6:.0f} Mbp of the same records ({len(seqs)} records), D={d}, {dt:.2f} s for P1-P4",
             "phases_s": {k2: round(v, 3) for k2, v in p.times().items() if k2 != "p5"}}
 
 

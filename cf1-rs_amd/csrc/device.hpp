@@ -132,14 +132,19 @@ __device__ __forceinline__ uint32_t seg_sub(const KeyT& k, int sb) {
   return sb ? (uint32_t)(seg_hash(k) >> (64 - sb)) : 0u;
 }
 // BBHash level hashes: one 64-bit state per key and level, g_l = h1 + l * d
-// (mod 2^64), h1 = (key ^ S1) * M1 (wide keys: (lo ^ hi * WIDE_MUL ^ S1) *
+// (mod 2^64), h1 = (key ^ S1) * M1 (wide keys: (lo ^ mix64(hi ^ S1)) *
 // M1), d = rotl(h1, 21) | 1. The position among n < 2^32 bits is
 // floor(x * n / 2^32) with x = fmix32(hi32(g_l) ^ lo32(g_l)); the fmix32 step
 // keeps the levels independent (top bits of g_l alone would keep two keys
 // with close h1 and close d on one bit at every level).
 constexpr uint64_t BBH_M1 = 0xD6E8FEB86659FD93ULL;
 __device__ __forceinline__ uint64_t bbh_h1(uint64_t k) { return (k ^ BBH_S1) * BBH_M1; }
-__device__ __forceinline__ uint64_t bbh_h1(const Key128& k) { return (k.lo ^ (k.hi * WIDE_MUL) ^ BBH_S1) * BBH_M1; }
+// For wide keys the high word goes through a full mixer before the fold: a
+// linear fold (lo ^ hi * C) gives one h1 to pairs of 63-mers that differ only
+// near the top of both words (bases 0 and 32, say: hi * C then changes in its
+// top bits alone, which a change of lo can cancel), and two keys with one h1
+// share every level position, so no level could separate them.
+__device__ __forceinline__ uint64_t bbh_h1(const Key128& k) { return (k.lo ^ mix64(k.hi ^ BBH_S1)) * BBH_M1; }
 __device__ __forceinline__ uint64_t bbh_d(uint64_t h1) { return ((h1 << 21) | (h1 >> 43)) | 1ULL; }
 __device__ __forceinline__ uint32_t bbh_pos(uint64_t g, uint32_t n) {
   const uint32_t x = fmix32((uint32_t)(g >> 32) ^ (uint32_t)g);

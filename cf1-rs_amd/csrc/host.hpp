@@ -26,7 +26,7 @@ inline uint64_t h_seg_hash(uint64_t lo, uint64_t hi, bool wide) {
 }
 inline uint64_t h_bbh_h1(uint64_t lo, uint64_t hi, bool wide) {
   const uint64_t s = h_seed(0), m1 = 0xD6E8FEB86659FD93ULL;
-  return wide ? (lo ^ (hi * 0xC2B2AE3D27D4EB4FULL) ^ s) * m1 : (lo ^ s) * m1;
+  return wide ? (lo ^ h_mix64(hi ^ s)) * m1 : (lo ^ s) * m1;
 }
 inline uint64_t h_bbh_d(uint64_t h1) { return ((h1 << 21) | (h1 >> 43)) | 1ULL; }
 

@@ -458,19 +458,22 @@ inline uint64_t mix64(uint64_t z) {
 inline uint64_t bbh_seed(int i) { return mix64(0x243F6A8885A308D3ULL + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL); }
 // Level hashes of a key come from one 64-bit state per key and level,
 // g_l = h1 + l * d (mod 2^64), with h1 = (lo ^ S) * M1 for keys of <= 64
-// bits, (lo ^ hi * 0xC2B2AE3D27D4EB4F ^ S) * M1 for wider ones (S =
+// bits, (lo ^ mix64(hi ^ S)) * M1 for wider ones (S =
 // bbh_seed(0), M1 = 0xD6E8FEB86659FD93), and d = rotl(h1, 21) | 1. The
 // level position among n bits (n < 2^32) is floor(x * n / 2^32) with x =
 // fmix32(hi32(g_l) ^ lo32(g_l)). The fmix32 step is what keeps levels
 // independent: reading the position straight from the top bits of g_l would
 // keep two keys whose h1 and d are both close together on one bit at every
 // level. (A rotation by 32 would make the fold symmetric in the two halves
-// and is avoided for the same reason.)
+// and is avoided for the same reason. A linear fold of the wide key,
+// lo ^ hi * C, is avoided too: it gives one h1 to pairs of 63-mers that
+// differ only near the top of both words, and such a pair can never be
+// separated.)
 constexpr uint64_t BBH_M1 = 0xD6E8FEB86659FD93ULL;
 inline uint64_t bbh_h1(uint64_t lo, uint64_t hi, bool wide) {
   uint64_t s = bbh_seed(0);
   if (!wide) return (lo ^ s) * BBH_M1;
-  return (lo ^ (hi * 0xC2B2AE3D27D4EB4FULL) ^ s) * BBH_M1;
+  return (lo ^ mix64(hi ^ s)) * BBH_M1;
 }
 inline uint64_t bbh_d(uint64_t h1) { return ((h1 << 21) | (h1 >> 43)) | 1ULL; }
 inline uint64_t bbh_pos_g(uint64_t g, uint64_t n) {

@@ -191,3 +191,11 @@ def test_phase_order_is_enforced():
         g.build_mphf()
     assert e.value.code == -2
     g.close()
+
+
+def test_wide_keys_with_equal_linear_fold():
+    # two 63-mers of the genome-like workload whose linear fold lo ^ hi * C is
+    # equal, in one segment: the BBHash must still place both
+    from test_oracle_mphf_wide import PAIR
+    run_pair(PAIR, 63, num_bins=1, target_seg=1 << 24, bins=False)
+    run_pair(PAIR + [PAIR[0] + b"ACGTTGCA" + PAIR[1]], 63, num_bins=2, bins=False)
