@@ -188,6 +188,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lookups", type=int, default=1 << 28, help="lookups per batch (0: skip the lookup benchmark)")
     ap.add_argument("--lookup-batches", type=int, default=4)
+    ap.add_argument("--end-to-end", type=int, default=1, help="also time FASTA file -> P1..P4 (0: skip)")
+    ap.add_argument("--ingest-threads", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true", help="keep --num-bins instead of the autotuner's choice")
     ap.add_argument("--p4-cas-runs", type=int, default=3, help="timed runs of the per-occurrence CAS classification (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -279,6 +281,10 @@ def main():
         phases.setdefault(PHASE_OF.get(name, name), 0.0)
         phases[PHASE_OF.get(name, name)] += v["ms_per_step"]
 
+    # end to end from a FASTA file: host parse + host packing + copy of the
+    # packed stream, then P1..P4 (the timed step starts from records in HBM)
+    e2e = end_to_end(g, ascii_, rec_off, names, args.ingest_threads) if args.end_to_end else None
+
     # P4 in the reference's form (per-occurrence lookup + CAS join) beside
     # the mask-join P4 the step uses; both must give the same state words
     p4cas = p4_cas_bench(g, info, e_probe) if args.p4_cas_runs > 0 else None
@@ -322,12 +328,49 @@ def main():
         "phase_wall_ms": {k2: round(v * 1000, 3) for k2, v in ph.items()},
         "phase_kernel_ms": {k2: round(v, 3) for k2, v in phases.items()},
         "kernels": {k2: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k2, v in kern.items()},
+        "end_to_end": e2e,
         "autotune": tune,
         "p4_cas": p4cas,
         "lookup": lookup,
     }
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def end_to_end(g, ascii_, rec_off, names, threads, runs=3):
+    """FASTA file on disk (page cache) -> cf1g_load_fasta (parse, host
+    packing, copy of the packed stream) -> P1..P4, and the same records
+    given as ASCII in host memory (cf1g_load_records: 300 MB copied, packed
+    on the device). Median of `runs`."""
+    import tempfile
+
+    from cf1g import synth
+
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "records.fa")
+        synth.write_fasta(path, ascii_, rec_off, names)
+        fbytes = os.path.getsize(path)
+        rows = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            st = g.load_fasta(path, threads=threads)
+            t1 = time.perf_counter()
+            g.run_p1_p4()
+            t2 = time.perf_counter()
+            g.load_records(ascii=ascii_, rec_off=rec_off)
+            t3 = time.perf_counter()
+            g.run_p1_p4()
+            t4 = time.perf_counter()
+            rows.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3, st))
+    rows.sort(key=lambda r: r[0] + r[1])
+    m = rows[len(rows) // 2]
+    st = m[4]
+    print(f"[bench] end to end: FASTA load {m[0] * 1e3:.1f} ms + P1-P4 {m[1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    return {"fasta_bytes": fbytes, "threads": threads,
+            "fasta_ms": round((m[0] + m[1]) * 1e3, 3), "fasta_load_ms": round(m[0] * 1e3, 3),
+            "parse_ms": round(st["parse_ms"], 3), "pack_ms": round(st["pack_ms"], 3), "copy_ms": round(st["h2d_ms"], 3),
+            "packed_bytes_copied": st["stream_bytes"], "p1_p4_ms": round(m[1] * 1e3, 3),
+            "ascii_in_host_memory_ms": round((m[2] + m[3]) * 1e3, 3), "ascii_copy_ms": round(m[2] * 1e3, 3)}
 
 
 def p4_cas_bench(g, info, e_probe, runs=3):

@@ -29,6 +29,7 @@ NUM_BUCKETS = 65536
 # Every symbol include/cf1g.h declares.
 EXPORTS = [
     "cf1g_create", "cf1g_destroy", "cf1g_last_error", "cf1g_get_info", "cf1g_load_records",
+    "cf1g_load_fasta", "cf1g_loaded_records", "cf1g_stream_export",
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
@@ -60,6 +61,11 @@ class CfInfo(C.Structure):
                 ("mphf_words", C.c_uint64), ("num_bins", C.c_uint32), ("seg_bits", C.c_uint32),
                 ("mphf_big_segments", C.c_uint32), ("n_oversized", C.c_uint32), ("k", C.c_uint32), ("m", C.c_uint32),
                 ("graph_captures", C.c_uint64), ("graph_replays", C.c_uint64)]
+
+
+class IngestStats(C.Structure):
+    _fields_ = [("n_records", C.c_uint64), ("bases", C.c_uint64), ("stream_bytes", C.c_uint64),
+                ("parse_ms", C.c_double), ("pack_ms", C.c_double), ("h2d_ms", C.c_double)]
 
 
 class CasStats(C.Structure):
@@ -111,6 +117,10 @@ def lib():
         L.cf1g_last_error.argtypes = [C.c_void_p]
         L.cf1g_get_info.argtypes = [C.c_void_p, C.POINTER(CfInfo)]
         L.cf1g_load_records.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64]
+        L.cf1g_load_fasta.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.POINTER(IngestStats)]
+        L.cf1g_loaded_records.argtypes = [C.c_void_p, C.POINTER(U8P), C.POINTER(U64P), U64P, C.POINTER(C.c_char_p),
+                                          C.POINTER(U64P)]
+        L.cf1g_stream_export.argtypes = [C.c_void_p, U64P, U64P, C.c_uint64]
         L.cf1g_p1_histogram.argtypes = [C.c_void_p, U64P]
         L.cf1g_p2_partition_route.argtypes = [C.c_void_p, U64P, C.c_uint32, U16P, U32P]
         L.cf1g_p3_build.argtypes = [C.c_void_p, U64P]
@@ -216,9 +226,44 @@ class Cf1g:
             ascii, rec_off = pack_records(seqs)
         self._ascii = np.ascontiguousarray(ascii, np.uint8)
         self._off = np.ascontiguousarray(rec_off, np.uint64)
+        self._from_fasta = False
         self._check(self.L.cf1g_load_records(self.h, _p(self._ascii, U8P), _p(self._off, U64P), len(self._off) - 1))
 
     # ---- phases (src/pipeline.rs:77-103)
+    def load_fasta(self, paths, threads=8):
+        """Parse FASTA files (plain or .gz) on the host, pack the 2-bit stream
+        there and copy it to HBM (cf1g_load_fasta); returns ingest stats."""
+        if isinstance(paths, (str, bytes, os.PathLike)):
+            paths = [paths]
+        arr = (C.c_char_p * max(len(paths), 1))(*[os.fsencode(p) for p in paths])
+        o = IngestStats()
+        self._from_fasta = False
+        self._check(self.L.cf1g_load_fasta(self.h, arr, len(paths), threads, C.byref(o)))
+        self._from_fasta = True
+        return {f: getattr(o, f) for f, _ in IngestStats._fields_}
+
+    def loaded_records(self):
+        """(ascii, rec_off, names) of the last load_fasta (copies)."""
+        a, ro, no = U8P(), U64P(), U64P()
+        n = C.c_uint64(0)
+        nm = C.c_char_p()
+        self._check(self.L.cf1g_loaded_records(self.h, C.byref(a), C.byref(ro), C.byref(n), C.byref(nm), C.byref(no)))
+        n = n.value
+        rec_off = np.ctypeslib.as_array(ro, shape=(n + 1,)).copy()
+        name_off = np.ctypeslib.as_array(no, shape=(n + 1,)).copy()
+        ascii_ = np.ctypeslib.as_array(a, shape=(max(int(rec_off[-1]), 1),))[:int(rec_off[-1])].copy()
+        blob = C.string_at(nm, int(name_off[-1])) if name_off[-1] else b""
+        names = [blob[int(name_off[i]):int(name_off[i + 1])].decode() for i in range(n)]
+        return ascii_, rec_off, names
+
+    def stream_export(self):
+        """(seq, nmask) words of the packed stream in HBM."""
+        nw = (self.info()["stream_len"] + 63) // 64
+        seq = np.zeros(2 * nw + 1, np.uint64)
+        nmask = np.zeros(nw + 1, np.uint64)
+        self._check(self.L.cf1g_stream_export(self.h, _p(seq, U64P), _p(nmask, U64P), nw))
+        return seq[:2 * nw], nmask[:nw]
+
     def count_minimizer_histogram(self, want=True):
         h = np.zeros(NUM_BUCKETS, np.uint64) if want else None
         self._check(self.L.cf1g_p1_histogram(self.h, _p(h, U64P) if want else None))
@@ -330,7 +375,11 @@ class Cf1g:
         return rw.value, bs, recs[: inf["n_superkmers"] * rw.value].reshape(-1, rw.value)
 
     def p5_write(self, prefix, names=None, input_desc="", poly_n_stretch=True, track_short=True, threads=4):
-        """Host P5: extract_and_output + write_json (src/output.rs:152-354, 785-834)."""
+        """Host P5: extract_and_output + write_json (src/output.rs:152-354, 785-834).
+        After load_fasta the context's records (and their ids) are used."""
+        if getattr(self, "_from_fasta", False):
+            self._ascii, self._off, fnames = self.loaded_records()
+            names = fnames if names is None else names
         if names is not None:
             enc = [n.encode() for n in names]
             blob = b"".join(enc)

@@ -15,6 +15,7 @@
 
 #include "../../include/cf1g.h"
 #include "device.hpp"
+#include "ingest.hpp"
 #include "launch.hpp"
 #include "lidx.hpp"
 
@@ -181,6 +182,12 @@ struct cf1g_ctx {
   bool graphs_on = true;
   uint64_t alloc_gen = 0, graph_captures = 0, graph_replays = 0;
   std::map<std::string, GraphSeg> graphs;
+  // records from cf1g_load_fasta, kept on the host for P5; the stream was
+  // packed on the host (P1 then starts from the packed stream)
+  HostRecords host;
+  bool host_records = false, host_packed = false;
+  void* pinned = nullptr;
+  size_t pinned_cap = 0;
 
   void* get(const std::string& name, size_t bytes) {
     DevBuf& b = bufs[name];
@@ -326,9 +333,9 @@ uint64_t level_bits(uint64_t remaining) {
 // while the sizes and buffers it was captured with are unchanged.
 void do_p1(cf1g_ctx* c) {
   const uint64_t nt = p1_tiles(c->n_pos), nsl = p1_slices(c->n_pos);
-  uint8_t* ascii = c->getT<uint8_t>("ascii", c->n_ascii + 8);
-  uint64_t* roff = c->getT<uint64_t>("rec_off", c->n_rec + 1);
-  uint64_t* gst = c->getT<uint64_t>("gstart", c->n_rec + 1);
+  uint8_t* ascii = c->host_packed ? nullptr : c->getT<uint8_t>("ascii", c->n_ascii + 8);
+  uint64_t* roff = c->host_packed ? nullptr : c->getT<uint64_t>("rec_off", c->n_rec + 1);
+  uint64_t* gst = c->host_packed ? nullptr : c->getT<uint64_t>("gstart", c->n_rec + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
   auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
@@ -340,9 +347,9 @@ void do_p1(cf1g_ctx* c) {
   uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
-  c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m}, [&] {
+  c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m, (uint64_t)c->host_packed}, [&] {
     hipStream_t st = c->st;
-    launch_pack(st, ascii, roff, gst, c->n_rec, c->n_words64, seq, nm);
+    if (!c->host_packed) launch_pack(st, ascii, roff, gst, c->n_rec, c->n_words64, seq, nm);
     c->zero(hist, CF1G_NUM_BUCKETS * 8);
     c->zero(tot, 32);
     launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, tcnt, tw, part, sums, hist, tot);
@@ -889,6 +896,7 @@ void cf1g_destroy(cf1g_ctx* c) {
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -935,6 +943,7 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
     }
     c->n_pos = c->n_ascii + n_rec + 1;
     c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
+    c->host_records = c->host_packed = false;
     uint8_t* da = c->getT<uint8_t>("ascii", c->n_ascii + 8);
     if (c->n_ascii) hip_check(hipMemcpyAsync(da, ascii + base, c->n_ascii, hipMemcpyHostToDevice, c->st), "H2D ascii");
     hip_check(hipMemcpyAsync(c->getT<uint64_t>("rec_off", n_rec + 1), off.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice, c->st),
@@ -945,6 +954,91 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
     c->sync();
     c->W = c->S = c->D = 0;
     c->stage = 1;
+  });
+}
+
+int cf1g_load_fasta(cf1g_ctx* c, const char* const* paths, uint32_t n_paths, uint32_t threads, cf1g_ingest_stats* out) {
+  return guarded(c, [&]() {
+    if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
+    if (!paths && n_paths) fail(CF1G_E_INVALID, "paths is NULL");
+    std::vector<std::string> ps;
+    for (uint32_t i = 0; i < n_paths; i++) {
+      if (!paths[i]) fail(CF1G_E_INVALID, "a path is NULL");
+      ps.emplace_back(paths[i]);
+    }
+    const int th = threads ? (int)threads : 1;
+    auto t0 = std::chrono::steady_clock::now();
+    c->host_records = c->host_packed = false;
+    try {
+      parse_fasta(ps, th, c->host);
+    } catch (const std::runtime_error& e) {
+      fail(CF1G_E_INVALID, e.what());
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    const HostRecords& R = c->host;
+    const uint64_t n_rec = R.off.size() - 1;
+    c->n_rec = n_rec;
+    c->n_ascii = R.off[n_rec];
+    c->gstart_h.assign(n_rec + 1, 0);
+    for (uint64_t r = 0; r <= n_rec; r++) c->gstart_h[r] = R.off[r] + r + 1;
+    c->n_pos = c->n_ascii + n_rec + 1;
+    c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
+    // pack into pinned memory, then one copy of the packed stream
+    const size_t bytes = 3 * c->n_words64 * 8;
+    if (c->pinned_cap < bytes) {
+      if (c->pinned) hip_check(hipHostFree(c->pinned), "hipHostFree");
+      c->pinned = nullptr;
+      c->pinned_cap = 0;
+      hip_check(hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
+      c->pinned_cap = bytes;
+    }
+    uint64_t* hseq = (uint64_t*)c->pinned;
+    uint64_t* hnm = hseq + 2 * c->n_words64;
+    pack_stream(R.ascii.data(), R.off.data(), n_rec, c->n_words64, hseq, hnm, th);
+    auto t2 = std::chrono::steady_clock::now();
+    uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
+    uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
+    hip_check(hipMemcpyAsync(seq, hseq, 2 * c->n_words64 * 8, hipMemcpyHostToDevice, c->st), "H2D stream");
+    hip_check(hipMemcpyAsync(nm, hnm, c->n_words64 * 8, hipMemcpyHostToDevice, c->st), "H2D placeholder bits");
+    c->sync();
+    auto t3 = std::chrono::steady_clock::now();
+    c->host_records = c->host_packed = true;
+    c->W = c->S = c->D = 0;
+    c->stage = 1;
+    if (out) {
+      auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+      cf1g_ingest_stats o{};
+      o.n_records = n_rec;
+      o.bases = c->n_ascii;
+      o.stream_bytes = bytes;
+      o.parse_ms = ms(t0, t1);
+      o.pack_ms = ms(t1, t2);
+      o.h2d_ms = ms(t2, t3);
+      *out = o;
+    }
+  });
+}
+
+int cf1g_loaded_records(cf1g_ctx* c, const uint8_t** ascii, const uint64_t** rec_off, uint64_t* n_rec, const char** names,
+                        const uint64_t** name_off) {
+  return guarded(c, [&]() {
+    if (!c->host_records) fail(CF1G_E_ORDER, "cf1g_loaded_records needs records from cf1g_load_fasta");
+    if (ascii) *ascii = c->host.ascii.data();
+    if (rec_off) *rec_off = c->host.off.data();
+    if (n_rec) *n_rec = c->host.off.size() - 1;
+    if (names) *names = c->host.names.data();
+    if (name_off) *name_off = c->host.name_off.data();
+  });
+}
+
+int cf1g_stream_export(cf1g_ctx* c, uint64_t* seq, uint64_t* nmask, uint64_t n_words) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_stream_export");
+    const uint64_t nw = (c->n_pos + 63) / 64;
+    if (n_words < nw) fail(CF1G_E_INVALID, "stream buffers too small");
+    if (!c->host_packed) need(c, 2, "cf1g_stream_export of records packed on the device (after P1)");
+    if (seq) c->d2h(seq, c->get("seq", (2 * c->n_words64 + 8) * 8), 2 * nw * 8);
+    if (nmask) c->d2h(nmask, c->get("nmask", (c->n_words64 + 8) * 8), nw * 8);
   });
 }
 

@@ -3,6 +3,7 @@
 // C ABI and P5 on the host, like run_pipeline (src/pipeline.rs:66-118).
 #include <dirent.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -14,39 +15,6 @@
 #include "../../include/cf1g.h"
 
 namespace {
-
-struct Records {
-  std::vector<uint8_t> ascii;
-  std::vector<uint64_t> off{0};
-  std::string names;
-  std::vector<uint64_t> name_off{0};
-};
-
-// Plain FASTA: '>' header (record id = up to the first whitespace,
-// src/classify.rs:552-557), sequence lines concatenated.
-bool read_fasta(const std::string& path, Records& R, std::string& err) {
-  std::ifstream in(path, std::ios::binary);
-  if (!in) {
-    err = "failed to open FASTA file " + path;
-    return false;
-  }
-  std::string line;
-  bool have = false;
-  while (std::getline(in, line)) {
-    if (!line.empty() && line.back() == '\r') line.pop_back();
-    if (!line.empty() && line[0] == '>') {
-      if (have) R.off.push_back(R.ascii.size());
-      size_t e = line.find_first_of(" \t", 1);
-      R.names += line.substr(1, e == std::string::npos ? std::string::npos : e - 1);
-      R.name_off.push_back(R.names.size());
-      have = true;
-    } else if (have) {
-      R.ascii.insert(R.ascii.end(), line.begin(), line.end());
-    }
-  }
-  if (have) R.off.push_back(R.ascii.size());
-  return true;
-}
 
 int usage() {
   std::fprintf(stderr,
@@ -107,16 +75,12 @@ int main(int argc, char** argv) {
     return 2;
   }
   if (out.empty()) return usage();
-  Records R;
-  std::string err, desc;
+  std::string desc;
+  std::vector<const char*> paths;
   for (size_t i = 0; i < inputs.size(); i++) {
-    if (!read_fasta(inputs[i], R, err)) {
-      std::fprintf(stderr, "%s\n", err.c_str());
-      return 1;
-    }
+    paths.push_back(inputs[i].c_str());
     desc += (i ? ", " : "") + inputs[i];
   }
-  uint64_t n_rec = R.off.size() - 1;
   cf1g_params p{};
   p.k = (uint32_t)k;
   p.num_bins = (uint32_t)num_bins;
@@ -129,19 +93,30 @@ int main(int argc, char** argv) {
   };
   if (rc) return die("cf1g_create");
   auto t0 = std::chrono::steady_clock::now();
-  if ((rc = cf1g_load_records(ctx, R.ascii.data(), R.off.data(), n_rec))) return die("load");
+  // FASTA parsed once on the host (at least 8 threads for the parse and the
+  // packing), the packed stream copied to HBM
+  cf1g_ingest_stats ist{};
+  if ((rc = cf1g_load_fasta(ctx, paths.data(), (uint32_t)paths.size(), (uint32_t)std::max(threads, 8), &ist)))
+    return die("load");
   auto t1 = std::chrono::steady_clock::now();
   uint64_t D = 0;
   if ((rc = cf1g_run_p1_p4(ctx, &D))) return die("P1..P4");
   auto t2 = std::chrono::steady_clock::now();
+  const uint8_t* ascii = nullptr;
+  const uint64_t *off = nullptr, *name_off = nullptr;
+  const char* names = nullptr;
+  uint64_t n_rec = 0;
+  if ((rc = cf1g_loaded_records(ctx, &ascii, &off, &n_rec, &names, &name_off))) return die("records");
   uint64_t meta[5];
-  if ((rc = cf1g_p5_write(ctx, R.ascii.data(), R.off.data(), n_rec, R.names.data(), R.name_off.data(), out.c_str(), desc.c_str(),
-                          poly_n, track_short, threads, meta)))
+  if ((rc = cf1g_p5_write(ctx, ascii, off, n_rec, names, name_off, out.c_str(), desc.c_str(), poly_n, track_short, threads, meta)))
     return die("P5");
   auto t3 = std::chrono::steady_clock::now();
   auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-  std::fprintf(stderr, "cf1g: D=%llu unitigs=%llu  load %.1f ms, P1-P4 %.1f ms, P5 %.1f ms\n", (unsigned long long)D,
-               (unsigned long long)meta[0], ms(t0, t1), ms(t1, t2), ms(t2, t3));
+  std::fprintf(stderr,
+               "cf1g: %llu records, %llu bases, D=%llu unitigs=%llu  load %.1f ms (parse %.1f, pack %.1f, copy %.1f), "
+               "P1-P4 %.1f ms, P5 %.1f ms\n",
+               (unsigned long long)ist.n_records, (unsigned long long)ist.bases, (unsigned long long)D, (unsigned long long)meta[0],
+               ms(t0, t1), ist.parse_ms, ist.pack_ms, ist.h2d_ms, ms(t1, t2), ms(t2, t3));
   cf1g_destroy(ctx);
   return 0;
 }

@@ -1,0 +1,247 @@
+// Host ingest: FASTA files parsed once, in parallel, and the flat 2-bit
+// stream packed on the host (the layout of device.hpp), so that only the
+// packed bases and the placeholder bits cross PCIe.
+//
+// The reference parses every input with needletail four times, once per
+// phase (src/minimizer.rs:44-89, src/superkmer.rs:44-91,
+// src/classify.rs:540-597, src/output.rs:166-206). A record is a '>' header
+// line (its id is the text up to the first whitespace,
+// src/classify.rs:552-557) followed by sequence lines; line breaks (LF or
+// CRLF) are dropped and every other byte is kept, so that non-ACGT bytes stay
+// placeholders (src/dna.rs:64-92). Files ending in .gz are inflated with zlib.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <zlib.h>
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ingest.hpp"
+
+namespace cf1g {
+namespace {
+
+void run_threads(int threads, const std::function<void(int)>& f) {
+  std::vector<std::thread> th;
+  for (int t = 1; t < threads; t++) th.emplace_back(f, t);
+  f(0);
+  for (auto& x : th) x.join();
+}
+
+// The bytes of one input file: mapped, or inflated for .gz.
+struct FileBytes {
+  const uint8_t* p = nullptr;
+  size_t n = 0;
+  void* map = nullptr;
+  std::vector<uint8_t> buf;
+  explicit FileBytes(const std::string& path) {
+    const bool gz = path.size() > 3 && path.compare(path.size() - 3, 3, ".gz") == 0;
+    if (gz) {
+      gzFile f = gzopen(path.c_str(), "rb");
+      if (!f) throw std::runtime_error("failed to open FASTA file " + path);
+      gzbuffer(f, 1 << 20);
+      size_t used = 0;
+      buf.resize(1 << 24);
+      for (;;) {
+        if (buf.size() - used < (1u << 22)) buf.resize(buf.size() * 2);
+        const int got = gzread(f, buf.data() + used, (unsigned)std::min<size_t>(buf.size() - used, 1u << 30));
+        if (got < 0) {
+          int e = 0;
+          std::string msg = gzerror(f, &e);
+          gzclose(f);
+          throw std::runtime_error("failed to inflate " + path + ": " + msg);
+        }
+        if (got == 0) break;
+        used += (size_t)got;
+      }
+      gzclose(f);
+      buf.resize(used);
+      p = buf.data();
+      n = used;
+      return;
+    }
+    const int fd = open(path.c_str(), O_RDONLY);
+    if (fd < 0) throw std::runtime_error("failed to open FASTA file " + path);
+    struct stat st;
+    if (fstat(fd, &st) != 0) {
+      close(fd);
+      throw std::runtime_error("failed to stat " + path);
+    }
+    n = (size_t)st.st_size;
+    if (n) {
+      map = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+      if (map == MAP_FAILED) {
+        close(fd);
+        throw std::runtime_error("failed to map " + path);
+      }
+      p = (const uint8_t*)map;
+    }
+    close(fd);
+  }
+  ~FileBytes() {
+    if (map) munmap(map, n);
+  }
+};
+
+struct Rec {
+  size_t name_b, name_e;  // id bytes in the file
+  size_t seq_b, seq_e;    // sequence lines (line breaks included)
+};
+
+// Header positions ('>' at a line start) of the file, found by `threads`
+// scanners over byte ranges.
+std::vector<size_t> headers(const FileBytes& F, int threads) {
+  const size_t n = F.n;
+  std::vector<std::vector<size_t>> part(threads);
+  run_threads(threads, [&](int t) {
+    const size_t a = n * (size_t)t / threads, b = n * (size_t)(t + 1) / threads;
+    for (size_t i = a; i < b;) {
+      const void* q = memchr(F.p + i, '>', b - i);
+      if (!q) break;
+      const size_t j = (const uint8_t*)q - F.p;
+      if (j == 0 || F.p[j - 1] == '\n') part[t].push_back(j);
+      i = j + 1;
+    }
+  });
+  std::vector<size_t> out;
+  for (auto& v : part) out.insert(out.end(), v.begin(), v.end());
+  return out;
+}
+
+// Sequence bytes of [b, e) without line breaks.
+size_t strip_count(const uint8_t* p, size_t b, size_t e) {
+  size_t c = 0;
+  for (size_t i = b; i < e; i++) c += (p[i] != '\n' && p[i] != '\r');
+  return c;
+}
+// Copies exactly the bytes strip_count counts: runs between LF / CR bytes.
+size_t strip_copy(const uint8_t* p, size_t b, size_t e, uint8_t* out) {
+  size_t c = 0;
+  for (size_t i = b; i < e;) {
+    const void* q = memchr(p + i, '\n', e - i);
+    const size_t j = q ? (size_t)((const uint8_t*)q - p) : e;
+    for (size_t s = i; s < j;) {  // [s, j) may hold CRs (normally one, last)
+      const void* cr = memchr(p + s, '\r', j - s);
+      const size_t t = cr ? (size_t)((const uint8_t*)cr - p) : j;
+      std::memcpy(out + c, p + s, t - s);
+      c += t - s;
+      s = t + 1;
+    }
+    i = j + 1;
+  }
+  return c;
+}
+
+}  // namespace
+
+void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords& R) {
+  threads = std::max(1, threads);
+  R.ascii.clear();
+  R.off.assign(1, 0);
+  R.names.clear();
+  R.name_off.assign(1, 0);
+  for (const std::string& path : paths) {
+    FileBytes F(path);
+    const uint8_t* p = F.p;
+    std::vector<size_t> hs = headers(F, threads);
+    std::vector<Rec> recs(hs.size());
+    for (size_t r = 0; r < hs.size(); r++) {
+      const size_t h = hs[r];
+      const void* nl = memchr(p + h, '\n', F.n - h);
+      const size_t le = nl ? (size_t)((const uint8_t*)nl - p) : F.n;
+      size_t ne = h + 1;
+      while (ne < le && p[ne] != ' ' && p[ne] != '\t' && p[ne] != '\r') ne++;
+      recs[r] = Rec{h + 1, ne, std::min(le + 1, F.n), r + 1 < hs.size() ? hs[r + 1] : F.n};
+    }
+    // sequence lengths per record (records split over threads by bytes)
+    std::vector<size_t> len(recs.size());
+    run_threads(threads, [&](int t) {
+      for (size_t r = t; r < recs.size(); r += threads) len[r] = strip_count(p, recs[r].seq_b, recs[r].seq_e);
+    });
+    const size_t base = R.ascii.size();
+    std::vector<size_t> at(recs.size() + 1, base);
+    for (size_t r = 0; r < recs.size(); r++) at[r + 1] = at[r] + len[r];
+    R.ascii.resize(at[recs.size()]);
+    // long records are copied in pieces so that one chromosome spreads over
+    // every thread: pieces of ~16 MB of input, each starting at a line start
+    struct Piece {
+      size_t b, e, out;
+    };
+    std::vector<Piece> pieces;
+    for (size_t r = 0; r < recs.size(); r++) {
+      size_t b = recs[r].seq_b, out = at[r];
+      const size_t e = recs[r].seq_e;
+      while (b < e) {
+        size_t m = std::min(e, b + (16u << 20));
+        if (m < e) {
+          const void* q = memchr(p + m, '\n', e - m);
+          m = q ? (size_t)((const uint8_t*)q - p) + 1 : e;
+        }
+        pieces.push_back({b, m, out});
+        out += strip_count(p, b, m);
+        b = m;
+      }
+    }
+    run_threads(threads, [&](int t) {
+      for (size_t i = t; i < pieces.size(); i += threads) strip_copy(p, pieces[i].b, pieces[i].e, R.ascii.data() + pieces[i].out);
+    });
+    for (size_t r = 0; r < recs.size(); r++) {
+      R.off.push_back(at[r + 1]);
+      R.names.append((const char*)p + recs[r].name_b, recs[r].name_e - recs[r].name_b);
+      R.name_off.push_back(R.names.size());
+    }
+  }
+}
+
+// 2-bit code of a byte: A/a 0, C/c 1, G/g 2, T/t 3, anything else 4.
+static uint8_t g_code[256];
+static bool g_code_init = [] {
+  for (int i = 0; i < 256; i++) g_code[i] = 4;
+  g_code['A'] = g_code['a'] = 0;
+  g_code['C'] = g_code['c'] = 1;
+  g_code['G'] = g_code['g'] = 2;
+  g_code['T'] = g_code['t'] = 3;
+  return true;
+}();
+
+void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint64_t n_words64, uint64_t* seq, uint64_t* nmask,
+                 int threads) {
+  (void)g_code_init;
+  threads = std::max(1, threads);
+  // gstart[r] = off[r] - off[0] + r + 1: stream position of record r's first
+  // base; position gstart[r+1] - 1 is the separator after record r
+  std::vector<uint64_t> gstart(n_rec + 1);
+  for (uint64_t r = 0; r <= n_rec; r++) gstart[r] = rec_off[r] - rec_off[0] + r + 1;
+  run_threads(threads, [&](int t) {
+    const uint64_t w0 = n_words64 * (uint64_t)t / threads, w1 = n_words64 * (uint64_t)(t + 1) / threads;
+    if (w0 >= w1) return;
+    uint64_t r = (uint64_t)(std::upper_bound(gstart.begin(), gstart.end(), w0 * 64) - gstart.begin());
+    r = r ? r - 1 : n_rec;  // record whose range [gstart[r], gstart[r+1]) holds the position, or none
+    if (w0 * 64 < gstart[0]) r = 0;
+    for (uint64_t w = w0; w < w1; w++) {
+      uint64_t hi_w = 0, lo_w = 0, nm = 0;
+      for (int i = 0; i < 64; i++) {
+        const uint64_t g = w * 64 + (uint64_t)i;
+        while (r < n_rec && g >= gstart[r + 1]) r++;
+        uint32_t code = 4;
+        if (r < n_rec && g >= gstart[r] && g + 1 < gstart[r + 1]) code = g_code[ascii[rec_off[r] - rec_off[0] + (g - gstart[r])]];
+        const uint64_t b = code & 3u;
+        if (i < 32) hi_w |= b << (62 - 2 * i);
+        else lo_w |= b << (62 - 2 * (i - 32));
+        nm |= (uint64_t)(code >> 2) << (63 - i);
+      }
+      seq[2 * w] = hi_w;
+      seq[2 * w + 1] = lo_w;
+      nmask[w] = nm;
+    }
+  });
+}
+
+}  // namespace cf1g

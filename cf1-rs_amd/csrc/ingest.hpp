@@ -1,0 +1,28 @@
+// Host ingest (ingest.cpp): FASTA parsing and host packing of the flat stream.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace cf1g {
+
+// Records as the C ABI takes them: ascii[off[r] .. off[r+1]) is record r,
+// names[name_off[r] .. name_off[r+1]) its id.
+struct HostRecords {
+  std::vector<uint8_t> ascii;
+  std::vector<uint64_t> off{0};
+  std::string names;
+  std::vector<uint64_t> name_off{0};
+};
+
+// Parse FASTA files (plain or .gz) in order into R; throws std::runtime_error.
+void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords& R);
+
+// The flat stream of device.hpp for records ascii/rec_off: n_words64 words of
+// placeholder bits and 2 * n_words64 words of bases, position 0 and each
+// record's end a separator, positions past the last record padding.
+void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint64_t n_words64, uint64_t* seq, uint64_t* nmask,
+                 int threads);
+
+}  // namespace cf1g

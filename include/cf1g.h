@@ -76,6 +76,35 @@ int cf1g_get_info(cf1g_ctx* ctx, cf1g_info* out);
  * (src/dna.rs:81-92). Host buffers are copied to HBM. */
 int cf1g_load_records(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec);
 
+/* Host ingest — the FASTA reading that the reference repeats in every phase
+ * (needletail::parse_fastx_file at src/minimizer.rs:44-50,
+ * src/superkmer.rs:44-50, src/classify.rs:540-546, src/output.rs:166-172),
+ * done once: the files (plain FASTA, or gzip when the name ends in .gz) are
+ * parsed in order by `threads` host threads (0 = 1), the 2-bit stream is
+ * packed on the host and only the packed stream (a quarter of the bases'
+ * bytes plus one bit per base) is copied to HBM. A record is a '>' header
+ * (id = the text up to the first whitespace, src/classify.rs:552-557) and
+ * its sequence lines without line breaks. The records stay in the context
+ * for P5 (cf1g_loaded_records). An unreadable file returns CF1G_E_INVALID.
+ * out may be NULL. */
+typedef struct cf1g_ingest_stats {
+  uint64_t n_records, bases;
+  uint64_t stream_bytes; /* packed stream copied to HBM */
+  double parse_ms, pack_ms, h2d_ms;
+} cf1g_ingest_stats;
+int cf1g_load_fasta(cf1g_ctx* ctx, const char* const* paths, uint32_t n_paths, uint32_t threads, cf1g_ingest_stats* out);
+/* The records of the last cf1g_load_fasta, owned by the context (valid until
+ * the next load or cf1g_destroy): the arguments cf1g_p5_write takes. Any
+ * output may be NULL. */
+int cf1g_loaded_records(cf1g_ctx* ctx, const uint8_t** ascii, const uint64_t** rec_off, uint64_t* n_rec, const char** names,
+                        const uint64_t** name_off);
+/* The packed stream in HBM (the bases P1..P5 read): ceil(stream_len / 64)
+ * words of placeholder bits (MSB-first, one bit per position) and twice as
+ * many words of 2-bit bases (32 per word, MSB-first; placeholders 0).
+ * Available after cf1g_load_fasta, or after P1 for cf1g_load_records. Either
+ * output may be NULL. */
+int cf1g_stream_export(cf1g_ctx* ctx, uint64_t* seq, uint64_t* nmask, uint64_t n_words);
+
 /* P1 — count_minimizer_histogram (src/minimizer.rs:27-98): one count per super
  * k-mer at bucket_hash(minimizer) (src/minimizer.rs:18-21). out_hist may be
  * NULL; else it receives CF1G_NUM_BUCKETS counts. */
