@@ -30,6 +30,7 @@ NUM_BUCKETS = 65536
 EXPORTS = [
     "cf1g_create", "cf1g_destroy", "cf1g_last_error", "cf1g_get_info", "cf1g_load_records",
     "cf1g_load_fasta", "cf1g_loaded_records", "cf1g_stream_export",
+    "cf1g_bins_write", "cf1g_dedup_write", "cf1g_save", "cf1g_restore",
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
@@ -121,6 +122,10 @@ def lib():
         L.cf1g_loaded_records.argtypes = [C.c_void_p, C.POINTER(U8P), C.POINTER(U64P), U64P, C.POINTER(C.c_char_p),
                                           C.POINTER(U64P)]
         L.cf1g_stream_export.argtypes = [C.c_void_p, U64P, U64P, C.c_uint64]
+        L.cf1g_bins_write.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32, U64P]
+        L.cf1g_dedup_write.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
+        L.cf1g_save.argtypes = [C.c_void_p, C.c_char_p]
+        L.cf1g_restore.argtypes = [C.c_void_p, C.c_char_p]
         L.cf1g_p1_histogram.argtypes = [C.c_void_p, U64P]
         L.cf1g_p2_partition_route.argtypes = [C.c_void_p, U64P, C.c_uint32, U16P, U32P]
         L.cf1g_p3_build.argtypes = [C.c_void_p, U64P]
@@ -263,6 +268,21 @@ class Cf1g:
         nmask = np.zeros(nw + 1, np.uint64)
         self._check(self.L.cf1g_stream_export(self.h, _p(seq, U64P), _p(nmask, U64P), nw))
         return seq[:2 * nw], nmask[:nw]
+
+    # ---- the reference's intermediate files, and resume
+    def bins_write(self, directory, threads=8):
+        n = C.c_uint64(0)
+        self._check(self.L.cf1g_bins_write(self.h, os.fsencode(directory), threads, C.byref(n)))
+        return n.value
+
+    def dedup_write(self, path, threads=8):
+        self._check(self.L.cf1g_dedup_write(self.h, os.fsencode(path), threads))
+
+    def save(self, path):
+        self._check(self.L.cf1g_save(self.h, os.fsencode(path)))
+
+    def restore(self, path):
+        self._check(self.L.cf1g_restore(self.h, os.fsencode(path)))
 
     def count_minimizer_histogram(self, want=True):
         h = np.zeros(NUM_BUCKETS, np.uint64) if want else None

@@ -15,6 +15,7 @@
 
 #include "../../include/cf1g.h"
 #include "device.hpp"
+#include "formats.hpp"
 #include "ingest.hpp"
 #include "launch.hpp"
 #include "lidx.hpp"
@@ -186,6 +187,8 @@ struct cf1g_ctx {
   // packed on the host (P1 then starts from the packed stream)
   HostRecords host;
   bool host_records = false, host_packed = false;
+  // MPHF and states restored from a cf1g_save file: no keys or joined masks
+  bool restored = false;
   void* pinned = nullptr;
   size_t pinned_cap = 0;
 
@@ -533,6 +536,7 @@ void do_p3(cf1g_ctx* c) {
   c->keys_ready = false;
   c->lidx_ready = c->lidx_states = false;
   c->use_index = 0;
+  c->restored = false;
   c->stage = 4;
 }
 
@@ -540,6 +544,7 @@ void do_p3(cf1g_ctx* c) {
 // first use from the per-segment key runs P3 left in occ_aux (each key placed
 // by its BBHash value).
 void* dense_keys(cf1g_ctx* c) {
+  if (c->restored) fail(CF1G_E_ORDER, "a restored MPHF keeps no keys (run P1..P3 for them)");
   const uint64_t kb = c->key_bytes;
   void* keys = c->get("keys", c->D * kb + 16);
   if (!c->keys_ready) {
@@ -559,6 +564,7 @@ void* dense_keys(cf1g_ctx* c) {
 
 // P4: the joined masks, already in MPHF-value order, become packed states.
 void do_p4(cf1g_ctx* c) {
+  if (c->restored) fail(CF1G_E_ORDER, "a restored MPHF keeps no joined observations: classify with cf1g_p4_classify_cas");
   const uint64_t nsw = (c->D + 11) / 12;
   uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
   uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
@@ -943,7 +949,7 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
     }
     c->n_pos = c->n_ascii + n_rec + 1;
     c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
-    c->host_records = c->host_packed = false;
+    c->host_records = c->host_packed = c->restored = false;
     uint8_t* da = c->getT<uint8_t>("ascii", c->n_ascii + 8);
     if (c->n_ascii) hip_check(hipMemcpyAsync(da, ascii + base, c->n_ascii, hipMemcpyHostToDevice, c->st), "H2D ascii");
     hip_check(hipMemcpyAsync(c->getT<uint64_t>("rec_off", n_rec + 1), off.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice, c->st),
@@ -968,7 +974,7 @@ int cf1g_load_fasta(cf1g_ctx* c, const char* const* paths, uint32_t n_paths, uin
     }
     const int th = threads ? (int)threads : 1;
     auto t0 = std::chrono::steady_clock::now();
-    c->host_records = c->host_packed = false;
+    c->host_records = c->host_packed = c->restored = false;
     try {
       parse_fasta(ps, th, c->host);
     } catch (const std::runtime_error& e) {
@@ -1149,6 +1155,124 @@ int cf1g_mphf_export(cf1g_ctx* c, uint64_t* buf, uint64_t* n_words) {
       hip_check(hipMemcpyAsync(buf + hdr + tab + 2 * (ns + 1), v.blocks, c->n_blocks * 64, hipMemcpyDeviceToHost, c->st), "D2H");
     c->sync();
     *n_words = total;
+  });
+}
+
+int cf1g_bins_write(cf1g_ctx* c, const char* dir, uint32_t threads, uint64_t* out_bytes) {
+  return guarded(c, [&]() {
+    need(c, 3, "cf1g_bins_write");
+    if (!dir) fail(CF1G_E_INVALID, "dir is NULL");
+    std::vector<uint64_t> bstart(c->num_bins + 1);
+    c->d2h(bstart.data(), c->get("bin_start", (CF1G_MAX_BINS + 1) * 8), (c->num_bins + 1) * 8);
+    std::vector<uint64_t> recs(c->S * (uint64_t)c->rw + 1);
+    if (c->S) c->d2h(recs.data(), c->get("rec", (c->S * (uint64_t)c->rw + 1) * 8), c->S * (uint64_t)c->rw * 8);
+    uint64_t bytes = 0;
+    try {
+      bytes = write_bin_files(dir, c->num_bins, c->rw, bstart.data(), recs.data(), threads ? (int)threads : 1);
+    } catch (const std::runtime_error& e) {
+      fail(CF1G_E_INVALID, e.what());
+    }
+    if (out_bytes) *out_bytes = bytes;
+  });
+}
+
+int cf1g_dedup_write(cf1g_ctx* c, const char* path, uint32_t threads) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_dedup_write");
+    if (!path) fail(CF1G_E_INVALID, "path is NULL");
+    const uint64_t kw = c->key_bytes / 8;
+    std::vector<uint64_t> keys(c->D * kw + 2);
+    if (c->D) c->d2h(keys.data(), dense_keys(c), c->D * c->key_bytes);
+    // bin b's keys are those of its segments [b << sb, (b + 1) << sb)
+    std::vector<uint64_t> key_off(c->n_segs + 1);
+    c->d2h(key_off.data(), c->get("seg_out_off", (c->n_segs + 1) * 8), (c->n_segs + 1) * 8);
+    std::vector<uint64_t> bko(c->num_bins + 1);
+    for (uint32_t b = 0; b <= c->num_bins; b++) bko[b] = key_off[(uint64_t)b << c->sb];
+    try {
+      write_dedup_file(path, keys.data(), c->D, c->wide, bko, threads ? (int)threads : 1);
+    } catch (const std::runtime_error& e) {
+      fail(CF1G_E_INVALID, e.what());
+    }
+  });
+}
+
+namespace {
+constexpr uint64_t SAVE_MAGIC = 0x3176617367316663ULL;  // "cf1gsav1"
+constexpr uint64_t SAVE_WORDS = 16;                      // header words
+}  // namespace
+
+int cf1g_save(cf1g_ctx* c, const char* path) {
+  return guarded(c, [&]() {
+    need(c, 5, "cf1g_save");
+    if (!path) fail(CF1G_E_INVALID, "path is NULL");
+    uint64_t mw = 0;
+    if (cf1g_mphf_export(c, nullptr, &mw) != CF1G_OK) fail(CF1G_E_INTERNAL, c->err);
+    const uint64_t nsw = (c->D + 11) / 12;
+    std::vector<uint64_t> buf(SAVE_WORDS + mw + nsw, 0);
+    buf[0] = SAVE_MAGIC;
+    buf[1] = 1;  // version
+    buf[2] = c->n_rec;
+    buf[3] = c->n_pos;
+    buf[4] = c->W;
+    buf[5] = c->S;
+    buf[6] = mw;
+    buf[7] = nsw;
+    if (cf1g_mphf_export(c, buf.data() + SAVE_WORDS, &mw) != CF1G_OK) fail(CF1G_E_INTERNAL, c->err);
+    if (nsw) c->d2h(buf.data() + SAVE_WORDS + mw, c->get("states", (nsw + 1) * 8), nsw * 8);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) fail(CF1G_E_INVALID, std::string("cannot create ") + path);
+    const size_t n = buf.size() * 8;
+    const bool ok = std::fwrite(buf.data(), 1, n, f) == n;
+    if (std::fclose(f) != 0 || !ok) fail(CF1G_E_INVALID, std::string("write failed: ") + path);
+  });
+}
+
+int cf1g_restore(cf1g_ctx* c, const char* path) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_restore");
+    if (!path) fail(CF1G_E_INVALID, "path is NULL");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) fail(CF1G_E_INVALID, std::string("cannot open ") + path);
+    std::vector<uint64_t> buf;
+    uint64_t hdr[SAVE_WORDS];
+    bool ok = std::fread(hdr, 8, SAVE_WORDS, f) == SAVE_WORDS && hdr[0] == SAVE_MAGIC && hdr[1] == 1;
+    if (ok) {
+      buf.resize(hdr[6] + hdr[7]);
+      ok = std::fread(buf.data(), 8, buf.size(), f) == buf.size();
+    }
+    std::fclose(f);
+    if (!ok) fail(CF1G_E_INVALID, std::string("not a complete cf1g save file: ") + path);
+    if (hdr[2] != c->n_rec || hdr[3] != c->n_pos) fail(CF1G_E_INVALID, "the save file belongs to other records (count or length differ)");
+    const uint64_t* m = buf.data();
+    const uint64_t mw = hdr[6], nsw = hdr[7];
+    const uint64_t ns = m[5], D = m[6], nblk = m[7], tab = CF1G_NUM_BUCKETS / 4;
+    if (mw < 16 || m[0] != 0x6366316D70686631ULL || m[1] != (uint64_t)c->k || m[2] != (uint64_t)c->m || m[8] != (c->wide ? 1u : 0u) ||
+        mw != 16 + tab + 2 * (ns + 1) + nblk * 8 || nsw != (D + 11) / 12 || (ns >> m[3]) != m[4])
+      fail(CF1G_E_INVALID, "the save file's MPHF does not match this context (k, m or layout)");
+    c->sb = (int)m[3];
+    c->num_bins = (uint32_t)m[4];
+    c->n_segs = ns;
+    c->D = D;
+    c->n_blocks = nblk;
+    c->W = hdr[4];
+    c->S = hdr[5];
+    MphfView v = c->view();
+    c->h2d((void*)v.b2b, m + 16, CF1G_NUM_BUCKETS * 2);
+    c->h2d((void*)v.key_off, m + 16 + tab, (ns + 1) * 8);
+    c->h2d((void*)v.blk_off, m + 16 + tab + ns + 1, (ns + 1) * 8);
+    if (nblk) c->h2d((void*)v.blocks, m + 16 + tab + 2 * (ns + 1), nblk * 64);
+    uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
+    if (nsw) c->h2d(states, m + mw, nsw * 8);
+    if (!c->host_packed) {  // ASCII records: the stream annotation and P5 read is packed in P1
+      launch_pack(c->st, c->getT<uint8_t>("ascii", c->n_ascii + 8), c->getT<uint64_t>("rec_off", c->n_rec + 1),
+                  c->getT<uint64_t>("gstart", c->n_rec + 1), c->n_rec, c->n_words64, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8),
+                  c->getT<uint64_t>("nmask", c->n_words64 + 8));
+      c->sync();
+    }
+    c->keys_ready = c->lidx_ready = c->lidx_states = false;
+    c->use_index = 0;
+    c->restored = true;
+    c->stage = 5;
   });
 }
 

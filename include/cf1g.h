@@ -215,6 +215,30 @@ int cf1g_record_starts(cf1g_ctx* ctx, uint64_t* out, uint64_t n);
  * may be NULL. */
 int cf1g_bins_export(cf1g_ctx* ctx, uint32_t* rec_words, uint64_t* bin_start, uint64_t* records, uint64_t n_words);
 
+/* ---- the reference's intermediate files, and resume */
+/* Routed super k-mers as the reference's bin files (P2 product): dir/cf1_bin_%05d.bin
+ * per bin (bin_file_path, src/superkmer.rs:224-226), records of a u16 LE base
+ * count and ceil(n/4) bytes with base j at bits 6-2(j%4) of byte j/4
+ * (pack_and_buffer, src/superkmer.rs:181-210; BinReader :259-289). Records
+ * of a bin are in device order (the reference's order depends on its thread
+ * schedule). threads (0 = 1) write bins in parallel; out_bytes may be NULL. */
+int cf1g_bins_write(cf1g_ctx* ctx, const char* dir, uint32_t threads, uint64_t* out_bytes);
+/* The consolidated distinct keys as the reference writes them in P3
+ * (cf1_dedup_all.bin, src/mphf.rs:120-132 from dedup_bin :323-363): each
+ * bin's canonical keys sorted, bins in order, raw native-endian storage (u64,
+ * or u128 with the low word first for k > 32). */
+int cf1g_dedup_write(cf1g_ctx* ctx, const char* path, uint32_t threads);
+/* Resume (no reference counterpart; the reference's phase_bench reuses P1..P3
+ * intermediates ad hoc): cf1g_save writes the MPHF (the cf1g_mphf_export
+ * words) and the state words after P4; cf1g_restore, on a context with the
+ * same records loaded, reads them back and leaves the context after P4:
+ * lookups, annotation, cf1g_p4_classify_cas and P5 work, while products that
+ * need the keys or the joined observations (keys, dedup file, learned index,
+ * cf1g_p4_classify) return CF1G_E_ORDER until P1..P3 run again. A file of
+ * other records, k or layout returns CF1G_E_INVALID. */
+int cf1g_save(cf1g_ctx* ctx, const char* path);
+int cf1g_restore(cf1g_ctx* ctx, const char* path);
+
 /* ---- host P5 (stays on the host, per the reference's phase split)
  * extract_and_output + write_json (src/output.rs:152-354, 785-834): writes
  * prefix.cf_seg / .cf_seq / .json for the records given to cf1g_load_records
