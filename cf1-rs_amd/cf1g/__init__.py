@@ -33,7 +33,7 @@ EXPORTS = [
     "cf1g_bins_write", "cf1g_dedup_write", "cf1g_save", "cf1g_restore",
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
-    "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_record_starts",
+    "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_unitig_spans", "cf1g_record_starts",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_graphs", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
     "cf1g_lidx_gradients", "cf1g_classify_via", "cf1g_select_index", "cf1g_selected_index",
@@ -139,6 +139,7 @@ def lib():
         L.cf1g_mphf_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
         L.cf1g_mphf_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
         L.cf1g_annotate.argtypes = [C.c_void_p, U8P, C.c_uint64]
+        L.cf1g_unitig_spans.argtypes = [C.c_void_p, U64P, U64P, U64P, U64P, U8P, C.c_uint64]
         L.cf1g_lidx_train.argtypes = [C.c_void_p, C.POINTER(LidxParams), C.POINTER(LidxInfo)]
         L.cf1g_lidx_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
         L.cf1g_lidx_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
@@ -378,6 +379,17 @@ class Cf1g:
         out = np.zeros(max(n, 1), np.uint8)
         self._check(self.L.cf1g_annotate(self.h, _p(out, U8P), out.size))
         return out[:n]
+
+    def unitig_spans(self):
+        """(start, end, id, flags) per maximal unitig in stream order
+        (cf1g_unitig_spans)."""
+        n = C.c_uint64(0)
+        self._check(self.L.cf1g_unitig_spans(self.h, C.byref(n), None, None, None, None, 0))
+        u = n.value
+        s, e, i = (np.zeros(u + 1, np.uint64) for _ in range(3))
+        f = np.zeros(u + 1, np.uint8)
+        self._check(self.L.cf1g_unitig_spans(self.h, C.byref(n), _p(s, U64P), _p(e, U64P), _p(i, U64P), _p(f, U8P), u + 1))
+        return s[:u], e[:u], i[:u], f[:u]
 
     def record_starts(self):
         n = self.info()["n_records"] + 1

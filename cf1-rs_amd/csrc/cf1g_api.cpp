@@ -78,6 +78,11 @@ void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_
                           const uint64_t*, uint8_t*);
 void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint8_t*);
 void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*);
+uint64_t span_tiles(uint64_t);
+void launch_span_count(hipStream_t, const uint8_t*, uint64_t, uint64_t*, uint64_t*);
+void launch_span_emit(hipStream_t, const uint8_t*, uint64_t, const uint64_t*, const uint64_t*, uint64_t*, uint64_t*);
+void launch_span_keys(hipStream_t, bool, const uint64_t*, int, const uint64_t*, const uint64_t*, uint64_t, void*, uint8_t*);
+void launch_span_claim(hipStream_t, const uint64_t*, uint64_t, uint64_t, uint64_t*, uint8_t*, unsigned int*);
 
 // ------------------------------------------------------------------ profiling
 namespace {
@@ -852,6 +857,67 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   }
 }
 
+// Per stream position (cf1g_annotate's layout) through the selected index,
+// into the device buffer "annot".
+uint8_t* annotate_device(cf1g_ctx* c) {
+  uint8_t* d = c->getT<uint8_t>("annot", c->n_pos + 8);
+  const uint64_t nsw = (c->D + 11) / 12;
+  uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
+  uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
+  if (c->use_index == 1)
+    launch_annotate_lidx(c->st, seq, nm, c->n_pos, c->k, c->lview(), c->view(), c->getT<uint64_t>("states_lidx", nsw + 1), d);
+  else
+    launch_annotate(c->st, c->wide, seq, nm, c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), d);
+  return d;
+}
+
+// Unitig spans (k_span_* in kernels_p4.hip) into "span_s" / "span_e" (stream
+// positions of the first and last k-mer), "span_id" (MPHF value, or rank
+// through the learned index when selected) and "span_fl" (bit0 forward,
+// bit1 this unitig claims its id), in stream order. Returns the count.
+uint64_t build_spans(cf1g_ctx* c) {
+  hipStream_t st = c->st;
+  uint8_t* ann = annotate_device(c);
+  const uint64_t nt = span_tiles(c->n_pos);
+  uint64_t* cs = c->getT<uint64_t>("span_cnt_s", nt + 1);
+  uint64_t* ce = c->getT<uint64_t>("span_cnt_e", nt + 1);
+  uint64_t* os = c->getT<uint64_t>("span_off_s", nt + 1);
+  uint64_t* oe = c->getT<uint64_t>("span_off_e", nt + 1);
+  uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(nt + 1) + 16);
+  launch_span_count(st, ann, c->n_pos, cs, ce);
+  scan_excl_u64(st, cs, os, nt, os + nt, tmp);
+  scan_excl_u64(st, ce, oe, nt, oe + nt, tmp);
+  uint64_t tot[2] = {0, 0};
+  if (nt) {
+    hip_check(hipMemcpyAsync(&tot[0], os + nt, 8, hipMemcpyDeviceToHost, st), "D2H spans");
+    c->d2h(&tot[1], oe + nt, 8);
+  }
+  if (tot[0] != tot[1]) fail(CF1G_E_INTERNAL, "unitig starts and ends do not pair up");
+  const uint64_t U = tot[0];
+  uint64_t* S = c->getT<uint64_t>("span_s", U + 1);
+  uint64_t* E = c->getT<uint64_t>("span_e", U + 1);
+  uint64_t* ids = c->getT<uint64_t>("span_id", U + 1);
+  uint8_t* fl = c->getT<uint8_t>("span_fl", U + 8);
+  void* keys = c->get("span_keys", U * c->key_bytes + 16);
+  const uint64_t nsw = (c->D + 11) / 12;
+  uint64_t* claim = c->getT<uint64_t>("p5_states", nsw + 1);
+  unsigned int* err = c->getT<unsigned int>("span_err", 2);
+  launch_span_emit(st, ann, c->n_pos, os, oe, S, E);
+  launch_span_keys(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->k, S, E, U, keys, fl);
+  if (c->use_index == 1) launch_lidx_lookup(st, (const uint64_t*)keys, U, c->lview(), c->view(), ids);
+  else launch_lookup(st, c->wide, keys, U, c->view(), ids);
+  if (nsw)
+    hip_check(hipMemcpyAsync(claim, c->get(c->use_index == 1 ? "states_lidx" : "states", (nsw + 1) * 8), nsw * 8,
+                             hipMemcpyDeviceToDevice, st),
+              "D2D states");
+  c->zero(err, 8);
+  launch_span_claim(st, ids, U, c->D, claim, fl, err);
+  unsigned int he = 0;
+  c->d2h(&he, err, 4);
+  if (he) fail(CF1G_E_INTERNAL, "a unitig's k-mer is missing from the index");
+  return U;
+}
+
 void check_k(const cf1g_params* p) {
   if (!p) fail(CF1G_E_INVALID, "params is NULL");
   if (p->k < 1 || p->k > 63 || p->k % 2 == 0) fail(CF1G_E_INVALID, "k must be odd and in [1, 63], got " + std::to_string(p->k));
@@ -1302,15 +1368,23 @@ int cf1g_annotate(cf1g_ctx* c, uint8_t* out, uint64_t n) {
   return guarded(c, [&]() {
     need(c, 5, "cf1g_annotate");
     if (!out || n < c->n_pos) fail(CF1G_E_INVALID, "annotation buffer too small");
-    uint8_t* d = c->getT<uint8_t>("annot", c->n_pos + 8);
-    uint64_t nsw = (c->D + 11) / 12;
-    if (c->use_index == 1)
-      launch_annotate_lidx(c->st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), c->n_pos,
-                           c->k, c->lview(), c->view(), c->getT<uint64_t>("states_lidx", nsw + 1), d);
-    else
-      launch_annotate(c->st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
-                      c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), d);
-    c->d2h(out, d, c->n_pos);
+    c->d2h(out, annotate_device(c), c->n_pos);
+  });
+}
+
+int cf1g_unitig_spans(cf1g_ctx* c, uint64_t* n_spans, uint64_t* start, uint64_t* end, uint64_t* id, uint8_t* flags, uint64_t cap) {
+  return guarded(c, [&]() {
+    need(c, 5, "cf1g_unitig_spans");
+    if (!n_spans) fail(CF1G_E_INVALID, "n_spans is NULL");
+    const uint64_t U = build_spans(c);
+    *n_spans = U;
+    if (!start && !end && !id && !flags) return;
+    if (cap < U) fail(CF1G_E_INVALID, "span buffers too small");
+    if (!U) return;
+    if (start) c->d2h(start, c->get("span_s", (U + 1) * 8), U * 8);
+    if (end) c->d2h(end, c->get("span_e", (U + 1) * 8), U * 8);
+    if (id) c->d2h(id, c->get("span_id", (U + 1) * 8), U * 8);
+    if (flags) c->d2h(flags, c->get("span_fl", U + 8), U);
   });
 }
 

@@ -1,12 +1,15 @@
-// Host P5 of the cf1g product: unitig extraction and GFA-reduced output
-// (src/output.rs:152-834) over the products of the GPU phases.
+// Host P5 of the cf1g product: GFA-reduced output (src/output.rs:152-834)
+// from the unitigs the GPU marked.
 //
-// Where the reference re-looks-up every k-mer's MPHF slot and state on the
-// host (AnnotatedKmer, src/directed_kmer.rs:81-109), this P5 reads one byte
-// per stream position computed on the GPU (valid, direction, state class)
-// and only looks up the two flanking k-mers of each unitig in the exported
-// BBHash (src/output.rs:728-731). Unitig claiming uses the exported state
-// words with the same outputted-state CAS (src/output.rs:747-751).
+// The reference walks every record on the host, re-looking-up each k-mer's
+// MPHF slot and state (AnnotatedKmer, src/directed_kmer.rs:81-109) to find
+// unitig boundaries (output_unitigs_contiguous, src/output.rs:506-709), looks
+// up each unitig's id and claims it with a CAS on the state vector
+// (output_gfa_unitig, :714-782). Here all of that runs on the GPU
+// (cf1g_unitig_spans: per-unitig first / last k-mer position, id,
+// orientation and claim), and the host only formats the lines: cf_seg for
+// the unitigs that claimed their id, cf_seq tilings per record
+// (write_tiling, :208-259) and the JSON summary (write_json, :785-834).
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -21,14 +24,6 @@
 using namespace cf1g;
 
 namespace {
-
-inline uint8_t outputted_state(uint8_t s) {  // OUTPUTTED_TABLE, src/state.rs:272-298
-  if (s == 3 || s == 15) return 15;
-  if ((s >= 4 && s <= 7) || s == 13) return 13;
-  if ((s >= 8 && s <= 11) || s == 14) return 14;
-  if (s == 12 || s >= 16) return 12;
-  return 0;
-}
 
 struct Meta {  // UnipathsMeta, src/output.rs:34-75
   uint64_t count = 0, kmers = 0, maxl = 0, minl = UINT64_MAX, suml = 0;
@@ -46,172 +41,6 @@ struct Meta {  // UnipathsMeta, src/output.rs:34-75
     if (o.maxl > maxl) maxl = o.maxl;
     if (o.minl < minl) minl = o.minl;
     suml += o.suml;
-  }
-};
-
-struct OU {
-  uint64_t id;
-  bool fwd;
-  uint64_t s, e;
-};
-
-enum : uint8_t { C_SISO = 0, C_MISO = 1, C_SIMO = 2, C_MIMO = 3 };
-
-struct AK {  // an annotated k-mer position
-  uint64_t idx;
-  uint8_t cls;
-  bool fwd;
-};
-
-bool unipath_start(const AK& c, const AK& p) {  // src/output.rs:79-110
-  if (c.cls == C_MIMO) return true;
-  if (c.fwd ? c.cls == C_MISO : c.cls == C_SIMO) return true;
-  if (p.cls == C_MIMO) return true;
-  if (p.fwd ? p.cls == C_SIMO : p.cls == C_MISO) return true;
-  return false;
-}
-bool unipath_end(const AK& c, const AK& n) {  // src/output.rs:114-145
-  if (c.cls == C_MIMO) return true;
-  if (c.fwd ? c.cls == C_SIMO : c.cls == C_MISO) return true;
-  if (n.cls == C_MIMO) return true;
-  if (n.fwd ? n.cls == C_MISO : n.cls == C_SIMO) return true;
-  return false;
-}
-inline bool placeholder(uint8_t c) { return h_code(c) > 3; }
-
-struct P5 {
-  int K;
-  const uint8_t* annot;  // per stream position
-  HostMphf mphf;
-  const HostLidx* lidx = nullptr;  // lookups through the learned index when set
-  std::atomic<uint64_t>* states;
-
-  AK at(uint64_t g0, uint64_t idx) const {
-    uint8_t a = annot[g0 + idx];
-    return AK{idx, (uint8_t)((a >> 2) & 3), (a & 2) != 0};
-  }
-  uint8_t get_state(uint64_t i) const { return (uint8_t)((states[i / 12].load(std::memory_order_acquire) >> ((i % 12) * 5)) & 31); }
-  bool cas_state(uint64_t i, uint8_t expected, uint8_t nv) {  // src/state_vector.rs:47-70
-    uint64_t sh = (i % 12) * 5;
-    std::atomic<uint64_t>& w = states[i / 12];
-    for (;;) {
-      uint64_t cur = w.load(std::memory_order_acquire);
-      if (((cur >> sh) & 31) != expected) return false;
-      uint64_t nw = (cur & ~(31ULL << sh)) | ((uint64_t)nv << sh);
-      if (w.compare_exchange_weak(cur, nw, std::memory_order_acq_rel, std::memory_order_relaxed)) return true;
-    }
-  }
-
-  // output_gfa_unitig, src/output.rs:714-782
-  void unitig(const uint8_t* seq, const AK& st, const AK& en, std::string& seg, std::vector<OU>& us, Meta& mt) {
-    u128 xs = h_encode(seq + st.idx, K), xe = h_encode(seq + en.idx, K);
-    u128 rs = h_rc(xs, K), re = h_rc(xe, K);
-    u128 cs = xs < rs ? xs : rs, ce = xe < re ? xe : re;
-    u128 mf = cs < ce ? cs : ce;
-    uint64_t id = lidx ? lidx->lookup((uint64_t)mf) : mphf.lookup(mf);
-    bool fwd = xs < re;
-    us.push_back({id, fwd, st.idx, en.idx});
-    uint8_t s = get_state(id);
-    if (!(s >= 12 && s <= 15)) {
-      if (cas_state(id, s, outputted_state(s))) {
-        uint64_t L = en.idx - st.idx + (uint64_t)K;
-        char num[32];
-        int nn = std::snprintf(num, sizeof num, "%llu\t", (unsigned long long)id);
-        seg.append(num, (size_t)nn);
-        if (fwd) {
-          for (uint64_t t = 0; t < L; t++) {
-            uint8_t c = seq[st.idx + t];
-            seg.push_back((char)(c <= 'T' ? c : c - ('a' - 'A')));
-          }
-        } else {
-          for (uint64_t t = 0; t < L; t++) {
-            uint8_t c = seq[en.idx + (uint64_t)K - 1 - t];
-            seg.push_back("TGCA"[h_code(c) & 3]);
-          }
-        }
-        seg.push_back('\n');
-        mt.add(en.idx - st.idx + 1, K);
-      }
-    }
-  }
-
-  uint64_t search_valid(const uint8_t* seq, uint64_t start, uint64_t right_end) const {  // src/output.rs:484-501
-    while (start <= right_end) {
-      bool ok = true;
-      for (int j = 0; j < K; j++)
-        if (placeholder(seq[start + j])) {
-          start = start + j + 1;
-          ok = false;
-          break;
-        }
-      if (ok) return start;
-    }
-    return right_end + 1;
-  }
-
-  // output_unitigs_contiguous, src/output.rs:506-709
-  uint64_t contiguous(const uint8_t* seq, uint64_t g0, uint64_t len, uint64_t right_end, uint64_t start_idx, std::string& seg,
-                      std::vector<OU>& us, Meta& mt) {
-    uint64_t i = start_idx;
-    AK curr = at(g0, i);
-    bool no_left = i == 0 || placeholder(seq[i - 1]);
-    bool no_right = i + K == len || placeholder(seq[i + K]);
-    if (no_left && no_right) {
-      unitig(seq, curr, curr, seg, us, mt);
-      return i + K;
-    }
-    if (no_right) {
-      AK prev = at(g0, i - 1);
-      if (unipath_start(curr, prev)) unitig(seq, curr, curr, seg, us, mt);
-      return i + K;
-    }
-    AK next = at(g0, i + 1);
-    bool on = false;
-    AK ustart = curr;
-    if (no_left) {
-      on = true;
-    } else {
-      AK prev = at(g0, i - 1);
-      if (unipath_start(curr, prev)) on = true;
-    }
-    if (on && unipath_end(curr, next)) {
-      unitig(seq, ustart, curr, seg, us, mt);
-      on = false;
-    }
-    i += 1;
-    for (;;) {
-      if (!on && i > right_end) break;
-      AK prev = curr;
-      curr = next;
-      if (unipath_start(curr, prev)) {
-        on = true;
-        ustart = curr;
-      }
-      if (i + K == len || placeholder(seq[i + K])) {
-        if (on) unitig(seq, ustart, curr, seg, us, mt);
-        return i + K;
-      }
-      next = at(g0, i + 1);
-      if (on && unipath_end(curr, next)) {
-        unitig(seq, ustart, curr, seg, us, mt);
-        on = false;
-      }
-      i += 1;
-    }
-    return i + K;
-  }
-
-  // extract_unitigs_from_chunk, src/output.rs:439-482
-  void chunk(const uint8_t* seq, uint64_t g0, uint64_t len, uint64_t cs, uint64_t cre, std::string& seg, std::vector<OU>& us,
-             Meta& mt) {
-    uint64_t abs_re = len - K;
-    uint64_t i = cs;
-    while (i <= abs_re) {
-      i = search_valid(seq, i, cre);
-      if (i > cre) break;
-      i = contiguous(seq, g0, len, cre, i, seg, us, mt);
-      if (i > cre) break;
-    }
   }
 };
 
@@ -261,119 +90,78 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
   int rc = cf1g_get_info(ctx, &info);
   if (rc) return rc;
   if (info.n_records != n_rec) return CF1G_E_INVALID;
-  const int K = (int)info.k;
-  std::vector<uint8_t> annot(info.stream_len);
-  if ((rc = cf1g_annotate(ctx, annot.data(), annot.size()))) return rc;
-  std::vector<uint64_t> words(info.n_state_words + 1);
-  if ((rc = cf1g_states_export(ctx, words.data(), words.size()))) return rc;
-  uint64_t mw = 0;
-  if ((rc = cf1g_mphf_export(ctx, nullptr, &mw))) return rc;
-  std::vector<uint64_t> mbuf(mw);
-  if ((rc = cf1g_mphf_export(ctx, mbuf.data(), &mw))) return rc;
+  const uint64_t K = info.k;
+  uint64_t U = 0;
+  if ((rc = cf1g_unitig_spans(ctx, &U, nullptr, nullptr, nullptr, nullptr, 0))) return rc;
+  std::vector<uint64_t> S(U + 1), E(U + 1), ID(U + 1);
+  std::vector<uint8_t> FL(U + 1);
+  if (U && (rc = cf1g_unitig_spans(ctx, &U, S.data(), E.data(), ID.data(), FL.data(), U + 1))) return rc;
   std::vector<uint64_t> gstart(n_rec + 1);
   if ((rc = cf1g_record_starts(ctx, gstart.data(), gstart.size()))) return rc;
-  // learned index selected: unitig ids are ranks in the sorted key order
-  HostLidx hl;
-  std::vector<uint64_t> lkeys;
-  std::vector<RootPiece> lroot;
-  std::vector<LeafRec> lleaf;
-  const bool learned = cf1g_selected_index(ctx) == 1;
-  if (learned) {
-    cf1g_lidx_shape sh;
-    if ((rc = cf1g_lidx_export(ctx, &sh, nullptr, nullptr, nullptr))) return rc;
-    lkeys.resize(sh.n_keys + 1);
-    lroot.resize(sh.n_pieces);
-    lleaf.resize(sh.n_leaves);
-    if ((rc = cf1g_lidx_export(ctx, nullptr, lkeys.data(), lroot.data(), lleaf.data()))) return rc;
-    hl.keys = lkeys.data();
-    hl.D = sh.n_keys;
-    hl.root = lroot.data();
-    hl.leaf = lleaf.data();
-    hl.L = sh.n_leaves;
-    hl.R = sh.n_pieces;
-    hl.s = sh.root_shift;
-  }
-
-  std::vector<std::atomic<uint64_t>> st(words.size());
-  for (size_t i = 0; i < words.size(); i++) st[i].store(words[i], std::memory_order_relaxed);
-  P5 p5;
-  p5.K = K;
-  p5.annot = annot.data();
-  p5.mphf.attach(mbuf.data());
-  if (learned) p5.lidx = &hl;
-  p5.states = st.data();
-
   const uint64_t base = rec_off[0];
   auto name_of = [&](uint64_t r) -> std::string {
     if (!names_blob || !name_off) return std::string();
     return std::string(names_blob + name_off[r], names_blob + name_off[r + 1]);
   };
-  std::vector<std::string> segs(n_rec), tiles(n_rec);
-  std::vector<Meta> metas(n_rec);
-  std::vector<char> present(n_rec, 0);
-  const int nth = threads < 1 ? 1 : threads;
-  auto tiling = [&](uint64_t r, const std::vector<OU>& us) {  // write_tiling, src/output.rs:208-259
-    std::string t = "Reference:1_Sequence:" + name_of(r) + "\t";
-    if (!us.empty()) {
-      if (poly_n_stretch && us[0].s > 0) t += "N" + std::to_string(us[0].s) + " ";
-      t += std::to_string(us[0].id);
-      t.push_back(us[0].fwd ? '+' : '-');
-      for (size_t i = 1; i < us.size(); i++) {
-        const OU& l = us[i - 1];
-        const OU& q = us[i];
-        if (poly_n_stretch && l.e + 1 != q.s) {
-          int64_t gap = (int64_t)q.s - (int64_t)l.e;
-          if (gap >= (int64_t)K + 1) t += " N" + std::to_string((uint64_t)gap - (uint64_t)K);
-        }
-        t.push_back(' ');
-        t += std::to_string(q.id);
-        t.push_back(q.fwd ? '+' : '-');
-      }
+  // the spans of record r are [first[r], first[r + 1]) (stream order)
+  std::vector<uint64_t> first(n_rec + 1, U);
+  {
+    uint64_t u = 0;
+    for (uint64_t r = 0; r < n_rec; r++) {
+      while (u < U && S[u] < gstart[r]) u++;
+      first[r] = u;
     }
-    t.push_back('\n');
-    return t;
+  }
+  // pieces of at most 65536 spans inside one record, formatted in parallel
+  struct Piece {
+    uint64_t r, u0, u1;
+    std::string seg, tile;
+    Meta mt;
   };
-  constexpr uint64_t LARGE = 4000000;  // LARGE_SEQ_THRESHOLD, src/output.rs:205
-  std::vector<uint64_t> small, large;
-  for (uint64_t r = 0; r < n_rec; r++) {
-    uint64_t len = rec_off[r + 1] - rec_off[r];
-    if (len < (uint64_t)K) continue;
-    present[r] = 1;
-    (len < LARGE ? small : large).push_back(r);
-  }
-  try {
-    par_for(small.size(), nth, [&](size_t i) {
-      uint64_t r = small[i];
-      const uint8_t* seq = ascii + (rec_off[r] - base);
-      uint64_t len = rec_off[r + 1] - rec_off[r];
-      std::vector<OU> us;
-      p5.chunk(seq, gstart[r], len, 0, len - K, segs[r], us, metas[r]);
-      tiles[r] = tiling(r, us);
-    });
-    for (uint64_t r : large) {
-      const uint8_t* seq = ascii + (rec_off[r] - base);
-      uint64_t len = rec_off[r + 1] - rec_off[r];
-      uint64_t total = len - K + 1;  // compute_chunk_boundaries, src/output.rs:406-423
-      std::vector<std::pair<uint64_t, uint64_t>> ch;
-      for (uint64_t i = 0; i < (uint64_t)nth; i++) {
-        uint64_t s = i * total / nth, e = (i + 1) * total / nth;
-        if (s < e) ch.push_back({s, e - 1});
+  std::vector<Piece> pieces;
+  for (uint64_t r = 0; r < n_rec; r++)
+    for (uint64_t u = first[r]; u < first[r + 1]; u += 65536) pieces.push_back({r, u, std::min(first[r + 1], u + 65536), {}, {}, {}});
+  const int nth = threads < 1 ? 1 : threads;
+  par_for(pieces.size(), nth, [&](size_t i) {
+    Piece& P = pieces[i];
+    const uint64_t r = P.r, g0 = gstart[r];
+    const uint8_t* seq = ascii + (rec_off[r] - base);
+    char num[48];
+    for (uint64_t u = P.u0; u < P.u1; u++) {
+      const uint64_t s = S[u] - g0, e = E[u] - g0;
+      const bool fwd = FL[u] & 1;
+      // tiling token (write_tiling): N-stretch before the first unitig or
+      // between unitigs, then id and orientation
+      if (u == first[r]) {
+        if (poly_n_stretch && s > 0) P.tile += "N" + std::to_string(s) + " ";
+      } else {
+        const uint64_t pe = E[u - 1] - g0;
+        if (poly_n_stretch && pe + 1 != s) {
+          const int64_t gap = (int64_t)s - (int64_t)pe;
+          if (gap >= (int64_t)K + 1) P.tile += " N" + std::to_string((uint64_t)gap - K);
+        }
+        P.tile.push_back(' ');
       }
-      std::vector<std::vector<OU>> cu(ch.size());
-      std::vector<std::string> cs(ch.size());
-      std::vector<Meta> cm(ch.size());
-      par_for(ch.size(), nth, [&](size_t c) { p5.chunk(seq, gstart[r], len, ch[c].first, ch[c].second, cs[c], cu[c], cm[c]); });
-      std::vector<OU> all;
-      for (size_t c = 0; c < ch.size(); c++) {
-        all.insert(all.end(), cu[c].begin(), cu[c].end());
-        segs[r] += cs[c];
-        metas[r].agg(cm[c]);
+      int nn = std::snprintf(num, sizeof num, "%llu%c", (unsigned long long)ID[u], fwd ? '+' : '-');
+      P.tile.append(num, (size_t)nn);
+      if (!(FL[u] & 2)) continue;  // another occurrence of this unitig writes its line
+      // cf_seg line (output_gfa_unitig): id, then the unitig's bases in its
+      // orientation, upper-case
+      const uint64_t L = e - s + K;
+      nn = std::snprintf(num, sizeof num, "%llu\t", (unsigned long long)ID[u]);
+      P.seg.append(num, (size_t)nn);
+      if (fwd) {
+        for (uint64_t t = 0; t < L; t++) {
+          const uint8_t c = seq[s + t];
+          P.seg.push_back((char)(c <= 'T' ? c : c - ('a' - 'A')));
+        }
+      } else {
+        for (uint64_t t = 0; t < L; t++) P.seg.push_back("TGCA"[h_code(seq[e + K - 1 - t]) & 3]);
       }
-      tiles[r] = tiling(r, all);
+      P.seg.push_back('\n');
+      P.mt.add(e - s + 1, (int)K);
     }
-  } catch (const std::exception&) {
-    return CF1G_E_INTERNAL;
-  }
+  });
   Meta meta;
   std::string pre(out_prefix);
   FILE* fs = std::fopen((pre + ".cf_seg").c_str(), "wb");
@@ -383,11 +171,19 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
     if (fq) std::fclose(fq);
     return CF1G_E_INVALID;
   }
+  size_t pi = 0;
   for (uint64_t r = 0; r < n_rec; r++) {
-    if (!present[r]) continue;
-    std::fwrite(segs[r].data(), 1, segs[r].size(), fs);
-    std::fwrite(tiles[r].data(), 1, tiles[r].size(), fq);
-    meta.agg(metas[r]);
+    // records shorter than k have no tiling line (src/output.rs:166-206)
+    const bool present = rec_off[r + 1] - rec_off[r] >= K;
+    std::string head = "Reference:1_Sequence:" + name_of(r) + "\t";
+    if (present) std::fwrite(head.data(), 1, head.size(), fq);
+    for (; pi < pieces.size() && pieces[pi].r == r; pi++) {
+      Piece& P = pieces[pi];
+      std::fwrite(P.seg.data(), 1, P.seg.size(), fs);
+      std::fwrite(P.tile.data(), 1, P.tile.size(), fq);
+      meta.agg(P.mt);
+    }
+    if (present) std::fputc('\n', fq);
   }
   std::fclose(fs);
   std::fclose(fq);
@@ -409,7 +205,7 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
   std::vector<uint64_t> shorts;
   if (track_short)
     for (uint64_t r = 0; r < n_rec; r++)
-      if (rec_off[r + 1] - rec_off[r] < (uint64_t)K) shorts.push_back(r);
+      if (rec_off[r + 1] - rec_off[r] < K) shorts.push_back(r);
   if (!shorts.empty()) {
     j += ",\n  \"short seqs\": [\n";
     for (size_t i = 0; i < shorts.size(); i++) {

@@ -180,6 +180,153 @@ __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ s
   out[g] = a;
 }
 
+// ---------------------------------------------------------------- unitig spans (P5)
+// A unitig of output_unitigs_contiguous (src/output.rs:506-709) is a maximal
+// run of valid k-mers cut wherever is_unipath_start(curr, prev) holds
+// (src/output.rs:79-110); is_unipath_end(prev, curr) (:114-145) is the same
+// predicate with the roles swapped, so the unitigs partition every run of
+// valid k-mers at exactly those cuts. From the per-position annotation
+// (bit0 valid, bit1 forward, bits 2-3 class): a unitig starts at a valid g
+// when g-1 holds no valid k-mer or the cut holds between g-1 and g, and ends
+// at g when g+1 holds no valid k-mer or the cut holds between g and g+1.
+// Work-groups take tiles of SP_T positions; starts and ends are compacted
+// separately in stream order, so the k-th start and the k-th end bound the
+// k-th unitig.
+constexpr int SP_WG = 256, SP_T = 4096;
+__device__ __forceinline__ bool unipath_cut(uint32_t c, uint32_t p) {  // is_unipath_start(c, p)
+  const uint32_t cc = (c >> 2) & 3u, pc = (p >> 2) & 3u;
+  const bool cf = (c >> 1) & 1u, pf = (p >> 1) & 1u;
+  return cc == CLS_MIMO || cc == (cf ? CLS_MISO : CLS_SIMO) || pc == CLS_MIMO || pc == (pf ? CLS_SIMO : CLS_MISO);
+}
+__device__ __forceinline__ void span_flags(const uint8_t* __restrict__ ann, uint64_t g, uint64_t n_pos, bool& st, bool& en) {
+  st = en = false;
+  if (g == 0 || g >= n_pos) return;
+  const uint32_t a = ann[g];
+  if (!(a & 1u)) return;
+  // a valid k-mer ends before the closing separator, so g + 1 < n_pos
+  const uint32_t p = ann[g - 1], n = ann[g + 1];
+  st = !(p & 1u) || unipath_cut(a, p);
+  en = !(n & 1u) || unipath_cut(n, a);
+}
+__global__ void __launch_bounds__(SP_WG) k_span_count(const uint8_t* __restrict__ ann, uint64_t n_pos, uint64_t* __restrict__ cnt_s,
+                                                      uint64_t* __restrict__ cnt_e) {
+  __shared__ uint32_t sh[2][SP_WG / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * SP_T;
+  uint32_t ns = 0, ne = 0;
+  for (int q = 0; q < SP_T / SP_WG; q++) {
+    bool st, en;
+    span_flags(ann, base + (uint64_t)q * SP_WG + threadIdx.x, n_pos, st, en);
+    ns += st;
+    ne += en;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    ns += __shfl_down(ns, d, 64);
+    ne += __shfl_down(ne, d, 64);
+  }
+  if (__lane_id() == 0) {
+    sh[0][threadIdx.x >> 6] = ns;
+    sh[1][threadIdx.x >> 6] = ne;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = 0, b = 0;
+    for (int w = 0; w < SP_WG / 64; w++) {
+      a += sh[0][w];
+      b += sh[1][w];
+    }
+    cnt_s[blockIdx.x] = a;
+    cnt_e[blockIdx.x] = b;
+  }
+}
+__global__ void __launch_bounds__(SP_WG) k_span_emit(const uint8_t* __restrict__ ann, uint64_t n_pos, const uint64_t* __restrict__ off_s,
+                                                     const uint64_t* __restrict__ off_e, uint64_t* __restrict__ S,
+                                                     uint64_t* __restrict__ E) {
+  __shared__ uint32_t sh[2][SP_WG / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * SP_T;
+  const uint32_t wv = threadIdx.x >> 6;
+  uint64_t os = off_s[blockIdx.x], oe = off_e[blockIdx.x];
+  for (int q = 0; q < SP_T / SP_WG; q++) {
+    const uint64_t g = base + (uint64_t)q * SP_WG + threadIdx.x;
+    bool st, en;
+    span_flags(ann, g, n_pos, st, en);
+    const uint64_t bs = __ballot(st), be = __ballot(en);
+    if (__lane_id() == 0) {
+      sh[0][wv] = (uint32_t)__popcll(bs);
+      sh[1][wv] = (uint32_t)__popcll(be);
+    }
+    __syncthreads();
+    uint32_t ws = 0, we = 0, ts = 0, te = 0;
+    for (uint32_t w = 0; w < SP_WG / 64; w++) {
+      if (w < wv) {
+        ws += sh[0][w];
+        we += sh[1][w];
+      }
+      ts += sh[0][w];
+      te += sh[1][w];
+    }
+    const uint64_t lt = lanemask_lt();
+    if (st) S[os + ws + (uint32_t)__popcll(bs & lt)] = g;
+    if (en) E[oe + we + (uint32_t)__popcll(be & lt)] = g;
+    os += ts;
+    oe += te;
+    __syncthreads();
+  }
+}
+// Per unitig: the key its id is looked up by, min(canonical(first),
+// canonical(last)), and its orientation, forward iff first < rc(last)
+// (output_gfa_unitig, src/output.rs:728-736).
+template <typename KeyT>
+__global__ void __launch_bounds__(256) k_span_keys(const uint64_t* __restrict__ seq, int K, const uint64_t* __restrict__ S,
+                                                   const uint64_t* __restrict__ E, uint64_t n, KeyT* __restrict__ keys,
+                                                   uint8_t* __restrict__ fl) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const KeyT xs = KOps<KeyT>::at(seq, S[u], K), xe = KOps<KeyT>::at(seq, E[u], K);
+  const KeyT rs = KOps<KeyT>::rc(xs, K), re = KOps<KeyT>::rc(xe, K);
+  const KeyT cs = kmer_min(xs, rs), ce = kmer_min(xe, re);
+  keys[u] = kmer_min(cs, ce);
+  fl[u] = key_lt(xs, re) ? 1u : 0u;
+}
+// The outputted claim of output_gfa_unitig (src/output.rs:747-751): a unitig
+// whose id is not yet in an outputted state moves it there by CAS
+// (OUTPUTTED_TABLE, src/state.rs:272-298); the one that succeeds writes the
+// unitig (flag bit 1). states is a copy, so the P4 state vector stays as it
+// was. err: ids outside [0, D).
+__device__ __forceinline__ uint32_t outputted_state(uint32_t s) {
+  if (s == 3 || s == 15) return 15;
+  if ((s >= 4 && s <= 7) || s == 13) return 13;
+  if ((s >= 8 && s <= 11) || s == 14) return 14;
+  if (s == 12 || s >= 16) return 12;
+  return 0;
+}
+__global__ void __launch_bounds__(256) k_span_claim(const uint64_t* __restrict__ ids, uint64_t n, uint64_t D,
+                                                    unsigned long long* __restrict__ states, uint8_t* __restrict__ fl,
+                                                    unsigned int* __restrict__ err) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const uint64_t id = ids[u];
+  if (id >= D) {
+    atomicOr(err, 1u);
+    return;
+  }
+  unsigned long long* wp = states + id / 12;
+  const uint32_t sh = (uint32_t)(id % 12) * 5;
+  unsigned long long cur = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  bool won = false;
+  for (;;) {
+    const uint32_t s = (uint32_t)(cur >> sh) & 31u;
+    if (s >= 12 && s <= 15) break;
+    const unsigned long long want = (cur & ~(31ULL << sh)) | ((unsigned long long)outputted_state(s) << sh);
+    const unsigned long long seen = atomicCAS(wp, cur, want);
+    if (seen == cur) {
+      won = true;
+      break;
+    }
+    cur = seen;
+  }
+  if (won) fl[u] |= 2u;
+}
+
 template <typename KeyT>
 __global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, uint64_t n, const MphfView mv,
                                                 uint64_t* __restrict__ out) {
@@ -207,6 +354,32 @@ void launch_p4_cas(hipStream_t st, bool wide, const uint64_t* seq, const uint64_
   else
     CF1G_LAUNCH("p4_cas", k_p4_cas<Key128>, dim3(grid), dim3(P4_WG), 0, st, seq, nmask, n_pos, K, mv, D, sv, cn, chunk, n_chunks,
                 force_retry);
+}
+uint64_t span_tiles(uint64_t n_pos) { return (n_pos + SP_T - 1) / SP_T; }
+void launch_span_count(hipStream_t st, const uint8_t* ann, uint64_t n_pos, uint64_t* cnt_s, uint64_t* cnt_e) {
+  const uint64_t nt = span_tiles(n_pos);
+  if (!nt) return;
+  CF1G_LAUNCH("p5_span_count", k_span_count, dim3((unsigned)nt), dim3(SP_WG), 0, st, ann, n_pos, cnt_s, cnt_e);
+}
+void launch_span_emit(hipStream_t st, const uint8_t* ann, uint64_t n_pos, const uint64_t* off_s, const uint64_t* off_e, uint64_t* S,
+                      uint64_t* E) {
+  const uint64_t nt = span_tiles(n_pos);
+  if (!nt) return;
+  CF1G_LAUNCH("p5_span_emit", k_span_emit, dim3((unsigned)nt), dim3(SP_WG), 0, st, ann, n_pos, off_s, off_e, S, E);
+}
+void launch_span_keys(hipStream_t st, bool wide, const uint64_t* seq, int K, const uint64_t* S, const uint64_t* E, uint64_t n, void* keys,
+                      uint8_t* fl) {
+  if (!n) return;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (!wide)
+    CF1G_LAUNCH("p5_span_keys", k_span_keys<uint64_t>, grid, dim3(256), 0, st, seq, K, S, E, n, (uint64_t*)keys, fl);
+  else
+    CF1G_LAUNCH("p5_span_keys", k_span_keys<Key128>, grid, dim3(256), 0, st, seq, K, S, E, n, (Key128*)keys, fl);
+}
+void launch_span_claim(hipStream_t st, const uint64_t* ids, uint64_t n, uint64_t D, uint64_t* states, uint8_t* fl, unsigned int* err) {
+  if (!n) return;
+  CF1G_LAUNCH("p5_span_claim", k_span_claim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ids, n, D,
+              (unsigned long long*)states, fl, err);
 }
 void launch_annotate(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
                      const MphfView& mv, const uint64_t* states, uint8_t* out) {

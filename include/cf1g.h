@@ -206,6 +206,18 @@ int cf1g_mphf_lookup_device(cf1g_ctx* ctx, const void* d_keys, uint64_t n, uint6
  * class (StateClass, src/state.rs:6-11) — what AnnotatedKmer::new computes per
  * k-mer for P5 (src/directed_kmer.rs:81-96). */
 int cf1g_annotate(cf1g_ctx* ctx, uint8_t* out, uint64_t n);
+/* Maximal unitigs, marked on the GPU (is_unipath_start / is_unipath_end of
+ * src/output.rs:79-145 over every position, the unitig id of
+ * output_gfa_unitig :714-736 through the selected index, and its outputted
+ * claim :747-751 by compare-and-swap on a copy of the state vector). In
+ * stream order: start / end = stream positions of the first and last k-mer
+ * (record-local index = position - cf1g_record_starts()[r]); id = the
+ * unitig's id; flags bit0 = forward (first k-mer < reverse complement of the
+ * last), bit1 = this unitig claimed its id (the one whose line cf_seg holds).
+ * *n_spans receives the count; with all outputs NULL nothing else is
+ * written; otherwise cap must be >= the count. The state vector is not
+ * changed. */
+int cf1g_unitig_spans(cf1g_ctx* ctx, uint64_t* n_spans, uint64_t* start, uint64_t* end, uint64_t* id, uint8_t* flags, uint64_t cap);
 /* Stream position of record r's first base (n_rec + 1 entries; the last is the
  * stream length). */
 int cf1g_record_starts(cf1g_ctx* ctx, uint64_t* out, uint64_t n);

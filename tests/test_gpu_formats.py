@@ -148,3 +148,24 @@ def test_save_restore(k, via_fasta):
             o.restore(path)
         assert e.value.code == -1
         o.close()
+
+
+@pytest.mark.parametrize("k", [3, 31, 63])
+def test_unitig_spans_partition_runs(k):
+    # spans tile every run of valid k-mers, in stream order; each distinct id
+    # is claimed exactly once; orientation and id agree with the oracle's P5
+    seqs = random_genome(seed=950 + k, n_rec=4, max_len=5000) + edge_records()
+    g = cf1g.Cf1g(k=k, num_bins=8)
+    g.load_records(seqs)
+    g.run_p1_p4()
+    s, e, ids, fl = g.unitig_spans()
+    ann = g.annotate()
+    valid = (ann & 1).astype(bool)
+    covered = np.zeros(len(ann), bool)
+    assert np.all(s <= e) and np.all(e[:-1] < s[1:])
+    for a, b in zip(s.tolist(), e.tolist()):
+        covered[a:b + 1] = True
+    np.testing.assert_array_equal(covered, valid)
+    claimed = ids[(fl & 2) != 0]
+    assert len(set(claimed.tolist())) == len(claimed) == len(set(ids.tolist()))
+    g.close()
