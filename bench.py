@@ -190,6 +190,7 @@ def main():
     ap.add_argument("--lookup-batches", type=int, default=4)
     ap.add_argument("--end-to-end", type=int, default=1, help="also time FASTA file -> P1..P4 (0: skip)")
     ap.add_argument("--ingest-threads", type=int, default=16)
+    ap.add_argument("--p5", type=int, default=1, help="also time P5 output (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep --num-bins instead of the autotuner's choice")
     ap.add_argument("--p4-cas-runs", type=int, default=3, help="timed runs of the per-occurrence CAS classification (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -285,6 +286,9 @@ def main():
     # packed stream, then P1..P4 (the timed step starts from records in HBM)
     e2e = end_to_end(g, ascii_, rec_off, names, args.ingest_threads) if args.end_to_end else None
 
+    # P5 (host, per the reference's phase split) over the GPU-marked unitigs
+    p5 = p5_bench(g, names, args.ingest_threads) if args.p5 else None
+
     # P4 in the reference's form (per-occurrence lookup + CAS join) beside
     # the mask-join P4 the step uses; both must give the same state words
     p4cas = p4_cas_bench(g, info, e_probe) if args.p4_cas_runs > 0 else None
@@ -329,6 +333,7 @@ def main():
         "phase_kernel_ms": {k2: round(v, 3) for k2, v in phases.items()},
         "kernels": {k2: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k2, v in kern.items()},
         "end_to_end": e2e,
+        "p5": p5,
         "autotune": tune,
         "p4_cas": p4cas,
         "lookup": lookup,
@@ -371,6 +376,28 @@ def end_to_end(g, ascii_, rec_off, names, threads, runs=3):
             "parse_ms": round(st["parse_ms"], 3), "pack_ms": round(st["pack_ms"], 3), "copy_ms": round(st["h2d_ms"], 3),
             "packed_bytes_copied": st["stream_bytes"], "p1_p4_ms": round(m[1] * 1e3, 3),
             "ascii_in_host_memory_ms": round((m[2] + m[3]) * 1e3, 3), "ascii_copy_ms": round(m[2] * 1e3, 3)}
+
+
+def p5_bench(g, names, threads):
+    """cf1g_p5_write into a temporary directory: unitig spans on the GPU
+    (annotation, cuts, ids, claims), then the host formats and writes
+    cf_seg / cf_seq / JSON."""
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "out")
+        g.set_profiling(True)
+        g.kernel_times(reset=True)
+        t = time.perf_counter()
+        meta = g.p5_write(pre, names=list(names), input_desc="records.fa", threads=threads)
+        dt = time.perf_counter() - t
+        kt = g.kernel_times(reset=True)
+        g.set_profiling(False)
+        sizes = {ext: os.path.getsize(pre + ext) for ext in (".cf_seg", ".cf_seq")}
+    gpu = {k2: round(v[1], 3) for k2, v in kt.items() if k2.startswith("p5_") or k2 in ("bbh_lookup", "lidx_lookup")}
+    print(f"[bench] P5: {dt * 1e3:.1f} ms, {meta['unitigs']} unitigs", file=sys.stderr, flush=True)
+    return {"ms": round(dt * 1e3, 3), "threads": threads, "unitigs": meta["unitigs"], "vertices": meta["vertices"],
+            "gpu_kernel_ms": gpu, "cf_seg_bytes": sizes[".cf_seg"], "cf_seq_bytes": sizes[".cf_seq"]}
 
 
 def p4_cas_bench(g, info, e_probe, runs=3):

@@ -160,23 +160,15 @@ void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords
       while (ne < le && p[ne] != ' ' && p[ne] != '\t' && p[ne] != '\r') ne++;
       recs[r] = Rec{h + 1, ne, std::min(le + 1, F.n), r + 1 < hs.size() ? hs[r + 1] : F.n};
     }
-    // sequence lengths per record (records split over threads by bytes)
-    std::vector<size_t> len(recs.size());
-    run_threads(threads, [&](int t) {
-      for (size_t r = t; r < recs.size(); r += threads) len[r] = strip_count(p, recs[r].seq_b, recs[r].seq_e);
-    });
-    const size_t base = R.ascii.size();
-    std::vector<size_t> at(recs.size() + 1, base);
-    for (size_t r = 0; r < recs.size(); r++) at[r + 1] = at[r] + len[r];
-    R.ascii.resize(at[recs.size()]);
-    // long records are copied in pieces so that one chromosome spreads over
-    // every thread: pieces of ~16 MB of input, each starting at a line start
+    // pieces of ~16 MB of input, each starting at a line start and inside
+    // one record, so that one chromosome spreads over every thread: bytes
+    // counted per piece in parallel, offsets by a prefix sum, then copied
     struct Piece {
-      size_t b, e, out;
+      size_t r, b, e, n, out;
     };
     std::vector<Piece> pieces;
     for (size_t r = 0; r < recs.size(); r++) {
-      size_t b = recs[r].seq_b, out = at[r];
+      size_t b = recs[r].seq_b;
       const size_t e = recs[r].seq_e;
       while (b < e) {
         size_t m = std::min(e, b + (16u << 20));
@@ -184,11 +176,27 @@ void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords
           const void* q = memchr(p + m, '\n', e - m);
           m = q ? (size_t)((const uint8_t*)q - p) + 1 : e;
         }
-        pieces.push_back({b, m, out});
-        out += strip_count(p, b, m);
+        pieces.push_back({r, b, m, 0, 0});
         b = m;
       }
     }
+    run_threads(threads, [&](int t) {
+      for (size_t i = t; i < pieces.size(); i += threads) pieces[i].n = strip_count(p, pieces[i].b, pieces[i].e);
+    });
+    const size_t base = R.ascii.size();
+    std::vector<size_t> at(recs.size() + 1, base);
+    {
+      size_t out = base, pi = 0;
+      for (size_t r = 0; r < recs.size(); r++) {
+        at[r] = out;
+        for (; pi < pieces.size() && pieces[pi].r == r; pi++) {
+          pieces[pi].out = out;
+          out += pieces[pi].n;
+        }
+      }
+      at[recs.size()] = out;
+    }
+    R.ascii.resize(at[recs.size()]);
     run_threads(threads, [&](int t) {
       for (size_t i = t; i < pieces.size(); i += threads) strip_copy(p, pieces[i].b, pieces[i].e, R.ascii.data() + pieces[i].out);
     });
@@ -227,6 +235,22 @@ void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, 
     if (w0 * 64 < gstart[0]) r = 0;
     for (uint64_t w = w0; w < w1; w++) {
       uint64_t hi_w = 0, lo_w = 0, nm = 0;
+      const uint64_t g0 = w * 64;
+      while (r < n_rec && g0 >= gstart[r + 1]) r++;
+      if (r < n_rec && g0 >= gstart[r] && g0 + 64 < gstart[r + 1]) {
+        // all 64 positions are bases of record r: straight from the bytes
+        const uint8_t* a = ascii + (rec_off[r] - rec_off[0]) + (g0 - gstart[r]);
+        for (int i = 0; i < 32; i++) {
+          const uint64_t c0 = g_code[a[i]], c1 = g_code[a[i + 32]];
+          hi_w |= (c0 & 3u) << (62 - 2 * i);
+          lo_w |= (c1 & 3u) << (62 - 2 * i);
+          nm |= ((c0 >> 2) << (63 - i)) | ((c1 >> 2) << (31 - i));
+        }
+        seq[2 * w] = hi_w;
+        seq[2 * w + 1] = lo_w;
+        nmask[w] = nm;
+        continue;
+      }
       for (int i = 0; i < 64; i++) {
         const uint64_t g = w * 64 + (uint64_t)i;
         while (r < n_rec && g >= gstart[r + 1]) r++;

@@ -2,15 +2,38 @@
 #pragma once
 #include <stdint.h>
 
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace cf1g {
 
+// Allocator that leaves new elements uninitialised (the parser writes every
+// byte), so that sizing the record buffer does not clear it first.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) {}
+  template <typename U>
+  void construct(U* p) {
+    ::new ((void*)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+
 // Records as the C ABI takes them: ascii[off[r] .. off[r+1]) is record r,
 // names[name_off[r] .. name_off[r+1]) its id.
 struct HostRecords {
-  std::vector<uint8_t> ascii;
+  std::vector<uint8_t, NoInitAlloc<uint8_t>> ascii;
   std::vector<uint64_t> off{0};
   std::string names;
   std::vector<uint64_t> name_off{0};
