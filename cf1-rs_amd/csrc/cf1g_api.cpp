@@ -938,6 +938,9 @@ int cf1g_create(const cf1g_params* params, cf1g_ctx** out) {
     c->k = (int)params->k;
     c->m = params->m ? (int)params->m : minimizer_len(c->k);
     if (c->m < 1 || c->m > 15 || c->m > c->k || c->m % 2 == 0) fail(CF1G_E_INVALID, "minimizer length must be odd, <= 15 and <= k");
+    if (2 * c->k - c->m > rec_cap(c->k > 32 ? 4 : 2))
+      fail(CF1G_E_INVALID, "minimizer length " + std::to_string(c->m) + " is too short for k = " + std::to_string(c->k) +
+                               ": super k-mers of up to 2k - m bases would not fit a routed record");
     c->w = c->k - c->m + 1;
     c->wide = c->k > 32;
     c->rw = c->wide ? 4 : 2;

@@ -335,6 +335,15 @@ __host__ __device__ __forceinline__ uint32_t state_mask(uint32_t s) {
   return I | (O << 5);
 }
 
+// ------------------------------------------------------------ P2 records
+// A routed super k-mer of L bases is a record of RW words holding positions
+// 0..L+1 (one flanking base each side) MSB-first, 32 per word, with L in bits
+// 0-7 of the last word and the flank flags in bits 8-9. Position
+// 32 * RW - 6 is the last one clear of those bits, so L <= 32 * RW - 7. A
+// super k-mer has at most w = K - m + 1 k-mers, so L <= 2K - m (the context
+// refuses a minimizer length that would exceed the record).
+__host__ __device__ constexpr int rec_cap(int rw) { return 32 * rw - 7; }
+
 // ------------------------------------------------------------ wave helpers (wave64)
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {

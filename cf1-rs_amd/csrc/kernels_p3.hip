@@ -159,7 +159,7 @@ __device__ __forceinline__ void for_each_rec_occ(const uint64_t* __restrict__ re
   for (int t = 0; t < RW; t++) w[t] = rec[idx * RW + t];
   const uint64_t tail = w[RW - 1];
   int L = (int)(tail & 255);
-  if (L > 32 * RW - 6) L = 32 * RW - 6;  // record capacity (bases above the flag bits)
+  if (L > rec_cap(RW)) L = rec_cap(RW);
   const bool lf = (tail >> 8) & 1, rf = (tail >> 9) & 1;
   const int n = L - K + 1;
   if (n <= 0) return;
@@ -199,7 +199,7 @@ __device__ __forceinline__ uint64_t rec2_bits(uint64_t w0, uint64_t w1, int sh) 
 }
 __device__ __forceinline__ int rec2_kmers(uint64_t w1, int K) {
   int L = (int)(w1 & 255);
-  if (L > 58) L = 58;
+  if (L > rec_cap(2)) L = rec_cap(2);
   return L >= K ? L - K + 1 : 0;
 }
 // Canonical k-mer j of a record of n k-mers and (OBS) its observation mask,
@@ -227,10 +227,81 @@ __device__ __forceinline__ uint64_t rec2_occ(uint64_t w0, uint64_t w1, int j, in
   }
   return c;
 }
-// Round-local k-mer -> record map for P3_WG records of at most OWN_KMERS
-// k-mers each (P2 emits super k-mers of at most w <= 17 k-mers for K <= 31;
-// a round with longer records takes the per-record loop).
+// ---- one lane per k-mer for wide records (RW = 4, 32 < K <= 63). The
+// record is the 256-bit value W = w0:w1:w2:w3, position t at bits
+// [254-2t, 255-2t]; k-mer j spans positions j+1..j+K.
+struct Rec4 {
+  uint64_t w0, w1, w2, w3;
+};
+// 64 bits of W from bit s up (bit 0 = least significant of w3), 0 <= s < 256.
+__device__ __forceinline__ uint64_t rec4_bits(const Rec4& R, int s) {
+  const int q = s >> 6, r = s & 63;
+  const uint64_t a = q == 0 ? R.w3 : q == 1 ? R.w2 : q == 2 ? R.w1 : R.w0;
+  const uint64_t b = q == 0 ? R.w2 : q == 1 ? R.w1 : q == 2 ? R.w0 : 0;
+  return r ? (a >> r) | (b << (64 - r)) : a;
+}
+__device__ __forceinline__ int rec4_kmers(uint64_t w3, int K) {
+  int L = (int)(w3 & 255);
+  if (L > rec_cap(4)) L = rec_cap(4);
+  return L >= K ? L - K + 1 : 0;
+}
+// Canonical k-mer j of a wide record of n k-mers and (OBS) its observation
+// mask, the same as the j-th call of for_each_rec_occ.
+template <bool OBS>
+__device__ __forceinline__ Key128 rec4_occ(const Rec4& R, int j, int n, int K, uint32_t& m) {
+  const uint64_t mh = (1ULL << (2 * K - 64)) - 1;
+  const int sh = 254 - 2 * (j + K);  // bit of k-mer j's last base
+  Key128 x;
+  x.lo = rec4_bits(R, sh);
+  x.hi = rec4_bits(R, sh + 64) & mh;
+  const Key128 r = rc128(x, K);
+  const Key128 c = kmer_min(x, r);
+  if (OBS) {
+    const bool lf = (R.w3 >> 8) & 1, rf = (R.w3 >> 9) & 1;
+    const bool pv = j > 0 || lf, nv = j < n - 1 || rf;
+    const uint32_t pb = (uint32_t)rec4_bits(R, 254 - 2 * j) & 3u;
+    const uint32_t nb = (uint32_t)rec4_bits(R, sh - 2) & 3u;
+    Key128 nx, nr;
+    nx.hi = ((x.hi << 2) | (x.lo >> 62)) & mh;
+    nx.lo = (x.lo << 2) | nb;
+    nr.lo = (r.lo >> 2) | (r.hi << 62);
+    nr.hi = (r.hi >> 2) | ((uint64_t)(3 - nb) << (2 * K - 66));
+    const Key128 nc = kmer_min(nx, nr);
+    uint32_t type;
+    if (!pv && !nv) type = OBS_MIMO;
+    else if (!nv) type = OBS_RIGHT;
+    else if (key_eq(nc, c)) type = (!pv || key_eq(x, nx)) ? OBS_MIMO : OBS_RIGHT;
+    else type = pv ? OBS_INTERNAL : OBS_LEFT;
+    m = obs_mask(type, key_eq(x, c), pb, nb);
+  }
+  return c;
+}
+
+// Round-local k-mer -> record map for P3_WG records of at most
+// OwnKmers<RW>::N k-mers each (P2 emits super k-mers of at most w k-mers:
+// w <= 17 for K <= 31 and w <= 49 for K <= 63 with the default minimizer
+// length; a round with longer records takes the per-record loop).
 constexpr int OWN_KMERS = 17;
+template <int RW>
+struct OwnKmers {
+  static constexpr int N = RW == 2 ? OWN_KMERS : 49;
+};
+// k-mer j of record r (n k-mers) for the one-lane-per-k-mer paths.
+template <typename KeyT, int RW, bool OBS>
+__device__ __forceinline__ KeyT rec_occ_at(const uint64_t* __restrict__ rec, uint64_t r, int j, int K, uint64_t kmask, uint32_t& m) {
+  if constexpr (RW == 2) {
+    const uint64_t x0 = rec[r * 2], x1 = rec[r * 2 + 1];
+    return (KeyT)rec2_occ<OBS>(x0, x1, j, rec2_kmers(x1, K), K, kmask, m);
+  } else {
+    const Rec4 R{rec[r * 4], rec[r * 4 + 1], rec[r * 4 + 2], rec[r * 4 + 3]};
+    return rec4_occ<OBS>(R, j, rec4_kmers(R.w3, K), K, m);
+  }
+}
+template <int RW>
+__device__ __forceinline__ int rec_kmers_at(const uint64_t* __restrict__ rec, uint64_t r, int K) {
+  if constexpr (RW == 2) return rec2_kmers(rec[r * 2 + 1], K);
+  else return rec4_kmers(rec[r * 4 + 3], K);
+}
 
 
 // Work-group -> (bin, record range) for the per-bin passes.
@@ -273,17 +344,18 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
   const int ns = 1 << sb;
   for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
   __syncthreads();
-  if constexpr (RW == 2) {
-    __shared__ uint8_t own[P3_WG * OWN_KMERS];
+  {
+    constexpr int OWN = OwnKmers<RW>::N;
+    __shared__ uint8_t own[P3_WG * OWN];
     __shared__ uint16_t oo[P3_WG];
     __shared__ uint64_t sh4[4];
-    const uint64_t mask = (1ULL << (2 * K)) - 1;
+    const uint64_t mask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
     for (uint64_t rr = r0; rr < r1; rr += P3_WG) {
       const uint64_t r = rr + threadIdx.x;
-      const int n = r < r1 ? rec2_kmers(rec[r * 2 + 1], K) : 0;
+      const int n = r < r1 ? rec_kmers_at<RW>(rec, r, K) : 0;
       uint64_t T;
       const uint32_t o = (uint32_t)block_excl_scan_u64((uint64_t)n, sh4, &T);
-      if (__syncthreads_or(n > OWN_KMERS)) {
+      if (__syncthreads_or(n > OWN)) {
         if (n) for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
         continue;
       }
@@ -292,17 +364,12 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
       __syncthreads();
       for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += P3_WG) {
         const uint32_t rl = own[g];
-        const uint64_t* wp = rec + (rr + rl) * 2;
-        const uint64_t w0 = wp[0], w1 = wp[1];
         uint32_t m;
-        const uint64_t c = rec2_occ<false>(w0, w1, (int)(g - oo[rl]), rec2_kmers(w1, K), K, mask, m);
+        const KeyT c = rec_occ_at<KeyT, RW, false>(rec, rr + rl, (int)(g - oo[rl]), K, mask, m);
         atomicAdd(&hist[seg_sub(c, sb)], 1u);
       }
       __syncthreads();
     }
-  } else {
-    for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG)
-      for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
   }
   __syncthreads();
   uint64_t* dst = counts + (uint64_t)bin * ns;
@@ -367,8 +434,9 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   __shared__ uint32_t cnt[1 << P3_PART_BITS], end[1 << P3_PART_BITS];
   __shared__ uint64_t base[1 << P3_PART_BITS];
   __shared__ uint64_t sh4[4];
-  __shared__ uint8_t own[RW == 2 ? P3_WG * OWN_KMERS : 1];
-  __shared__ uint16_t oo[RW == 2 ? P3_WG : 1];
+  constexpr int OWN = OwnKmers<RW>::N;
+  __shared__ uint8_t own[P3_WG * OWN];
+  __shared__ uint16_t oo[P3_WG];
   int bin;
   uint64_t r0, r1;
   if (!p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
@@ -385,34 +453,27 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
     uint32_t n = 0;
     if (r < r1) {
       int L = (int)(rec[r * RW + RW - 1] & 255);
-      if (L > 32 * RW - 6) L = 32 * RW - 6;
+      if (L > rec_cap(RW)) L = rec_cap(RW);
       n = L >= K ? (uint32_t)(L - K + 1) : 0u;
     }
     uint64_t T;
     const uint64_t o = block_excl_scan_u64(n, sh4, &T);
-    bool per_kmer = false;
-    if constexpr (RW == 2) {
-      per_kmer = !__syncthreads_or(n > OWN_KMERS);
-      if (per_kmer) {
-        oo[threadIdx.x] = (uint16_t)o;
-        for (uint32_t q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
-      }
+    const bool per_kmer = !__syncthreads_or(n > OWN);
+    if (per_kmer) {
+      oo[threadIdx.x] = (uint16_t)o;
+      for (uint32_t q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
     }
     for (uint64_t w0 = 0; w0 < T; w0 += TILE) {
       const uint32_t tn = (uint32_t)(T - w0 < (uint64_t)TILE ? T - w0 : (uint64_t)TILE);
-      if constexpr (RW == 2) {
-        if (per_kmer) {
-          __syncthreads();  // owner map complete
-          for (uint32_t t = threadIdx.x; t < tn; t += P3_WG) {
-            const uint32_t g = (uint32_t)w0 + t, rl = own[g];
-            const uint64_t* wp = rec + (rr + rl) * 2;
-            const uint64_t x0 = wp[0], x1 = wp[1];
-            uint32_t m;
-            const uint64_t key = rec2_occ<true>(x0, x1, (int)(g - oo[rl]), rec2_kmers(x1, K), K, kmask, m);
-            tk[t] = (KeyT)key;
-            tm[t] = (uint16_t)m;
-            tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
-          }
+      if (per_kmer) {
+        __syncthreads();  // owner map complete
+        for (uint32_t t = threadIdx.x; t < tn; t += P3_WG) {
+          const uint32_t g = (uint32_t)w0 + t, rl = own[g];
+          uint32_t m;
+          const KeyT key = rec_occ_at<KeyT, RW, true>(rec, rr + rl, (int)(g - oo[rl]), K, kmask, m);
+          tk[t] = key;
+          tm[t] = (uint16_t)m;
+          tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
         }
       }
       if (!per_kmer && n && o < w0 + tn && o + n > w0) {

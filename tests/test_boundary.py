@@ -55,3 +55,12 @@ def test_no_device_fails_loudly():
     with pytest.raises(cf1g.Cf1gError) as e:
         cf1g.Cf1g(k=31)
     assert e.value.code == -4
+
+
+def test_minimizer_length_must_fit_records():
+    # super k-mers of up to 2k - m bases must fit a routed record (57 bases
+    # for k <= 31, 121 for k > 32); refused before any device is touched
+    for k, m in ((31, 1), (31, 3), (63, 1), (63, 3), (9, 11)):
+        with pytest.raises(cf1g.Cf1gError) as e:
+            cf1g.Cf1g(k=k, m=m)
+        assert e.value.code == -1

@@ -199,3 +199,38 @@ def test_wide_keys_with_equal_linear_fold():
     from test_oracle_mphf_wide import PAIR
     run_pair(PAIR, 63, num_bins=1, target_seg=1 << 24, bins=False)
     run_pair(PAIR + [PAIR[0] + b"ACGTTGCA" + PAIR[1]], 63, num_bins=2, bins=False)
+
+
+def graph_by_kmer(g, k):
+    """States keyed by canonical k-mer and the canonicalised P5 output
+    (cf_seg strings; cf_seq tilings with ids replaced by those strings):
+    what does not depend on the minimizer scheme, bins or MPHF."""
+    lo, hi = g.keys()
+    words = g.states()
+    v = np.arange(len(lo), dtype=np.uint64)
+    st = (words[v // 12] >> ((v % 12) * 5).astype(np.uint64)) & np.uint64(31)
+    keys = list(zip(hi.tolist(), lo.tolist())) if k > 32 else lo.tolist()
+    states = dict(zip(keys, st.tolist()))
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "o")
+        g.p5_write(p)
+        seg = dict(line.split("\t") for line in open(p + ".cf_seg").read().splitlines())
+        tiles = []
+        for line in open(p + ".cf_seq").read().splitlines():
+            name, toks = line.split("\t")
+            tiles.append((name, [t if t.startswith("N") else (seg[t[:-1]], t[-1]) for t in toks.split()]))
+    return states, sorted(seg.values()), tiles
+
+
+@pytest.mark.parametrize("k,m", [(31, 5), (21, 3), (63, 5), (63, 9)])
+def test_minimizer_length_invariance(k, m):
+    # m = 5 at k = 31 gives super k-mers of up to 57 bases, the record capacity
+    seqs = random_genome(seed=60 + k + m, n_rec=4, max_len=6000)
+    out = []
+    for mm in (0, m):
+        g = cf1g.Cf1g(k=k, num_bins=8, m=mm)
+        g.load_records(seqs)
+        g.run_p1_p4()
+        out.append(graph_by_kmer(g, k))
+        g.close()
+    assert out[0] == out[1]
