@@ -1,23 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark: distinct k-mers/s over phases P1..P4 on one MI355X.
 
-Workload: the genome-like input (24 records, 300 Mbp, Zipf-like lengths, GC
-41% with drift, 5% interspersed repeat families, poly-N runs; seed 3), k=31.
-A step is one pass of P1..P4 over the records, which are resident in HBM as
-ASCII when the timed region starts (the host->device copy is not timed).
+Workload (--workload): genome300m by default (24 records, 300 Mbp,
+Zipf-like lengths, GC 41% with drift, 5% interspersed repeat families,
+poly-N runs; seed 3), k=31; genome5m and transcriptome are the other
+configurations. The autotuner picks the bin count and segment target first
+(untimed). A step is one pass of P1..P4 over records resident in HBM when
+the timed region starts, its per-phase launches replayed from hipGraphs.
 value = D / seconds-per-step (D = distinct canonical k-mers).
 
 Also reported on the same JSON line:
   roofline     : the dominant kernel's algorithmic bytes per launch over its
-                 mean launch time (HIP events on the context stream during the
-                 timed steps), against 8 TB/s HBM.
+                 mean launch time (HIP events on the context stream in a
+                 second, kernel-by-kernel pass of the same steps), against
+                 8 TB/s HBM, with its measured HBM traffic from profiles/.
   cpu_baseline : the multithreaded C++ CPU oracle (the restatement of the
                  reference's P1..P4) timed on a bounded sample of the same
-                 genome, on this host's cores.
-  lookup       : learned rank index vs BBHash lookups/s over 4 x 2^28 uniform
-                 lookups of present keys (every learned rank verified), the
-                 index's training summary, and the classification pass
-                 through each index.
+                 records, on this host's cores.
+  end_to_end   : FASTA file -> host parse and packing -> P1..P4, and the
+                 same records from host memory as ASCII.
+  p5           : unitig output (GPU-marked unitigs, host formatting).
+  p4_cas       : P4 in the reference's form (per-occurrence lookup + CAS).
+  autotune     : every configuration the autotuner measured.
+  lookup       : learned rank index vs BBHash lookups/s over 4 x 2^28
+                 uniform lookups of present keys (every learned rank
+                 verified), the index's training summary, and the
+                 classification pass through each index.
 """
 import argparse
 import json
