@@ -1,5 +1,5 @@
-// Learned rank index: model records and views shared by the kernels, the
-// C ABI and the host lookup (plain structs, identical layout on both sides).
+// Learned rank index: model records and views shared by the kernels and the
+// C ABI (plain structs; cf1g_lidx_export hands the records out as they are).
 #pragma once
 #include <stdint.h>
 

@@ -194,8 +194,9 @@ int cf1g_keys_export(cf1g_ctx* ctx, uint64_t* out, uint64_t n_words);
  * (magic, k, m, segment bits, bins, segments, D, blocks, wide), the
  * bucket->bin table (65536 u16), per-segment key offsets and block offsets
  * (segments + 1 words each), then 8-word blocks {rank, 7 bit words}. With
- * buf == NULL only *n_words is set. A host lookup over this buffer equals
- * Mphf::hash (src/mphf.rs:268-270) on the device. */
+ * buf == NULL only *n_words is set. The layout is complete: a lookup over
+ * it with the segment and level hashes the device uses gives exactly
+ * cf1g_mphf_lookup's values (Mphf::hash, src/mphf.rs:268-270). */
 int cf1g_mphf_export(cf1g_ctx* ctx, uint64_t* buf, uint64_t* n_words);
 /* Batch Mphf::hash (src/mphf.rs:268-270): keys in host memory (u64 or pairs). */
 int cf1g_mphf_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
