@@ -629,7 +629,17 @@ void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
 // one warm-up), then P4 chunk x work-groups per CU over the CAS
 // classification kernel (HIP events). The context keeps the chosen values
 // and ends with the products of the chosen configuration (stage 5).
+void do_autotune_run(cf1g_ctx* c, uint32_t reps, cf1g_tune_entry* ent, uint32_t cap, cf1g_tune_result* out);
 void do_autotune(cf1g_ctx* c, uint32_t reps, cf1g_tune_entry* ent, uint32_t cap, cf1g_tune_result* out) {
+  const cf1g_params saved = c->prm;
+  try {
+    do_autotune_run(c, reps, ent, cap, out);
+  } catch (...) {
+    c->prm = saved;  // a failed measurement leaves the caller's settings
+    throw;
+  }
+}
+void do_autotune_run(cf1g_ctx* c, uint32_t reps, cf1g_tune_entry* ent, uint32_t cap, cf1g_tune_result* out) {
   static const uint32_t kBins[] = {128, 256, 512};
   static const uint32_t kTarget[] = {1024, 1536, 2048};
   static const uint32_t kChunk[] = {2048, 8192, 32768};
@@ -1305,6 +1315,13 @@ int cf1g_restore(cf1g_ctx* c, const char* path) {
     std::vector<uint64_t> buf;
     uint64_t hdr[SAVE_WORDS];
     bool ok = std::fread(hdr, 8, SAVE_WORDS, f) == SAVE_WORDS && hdr[0] == SAVE_MAGIC && hdr[1] == 1;
+    if (ok) {  // the header's sizes must match the file before anything is allocated
+      const long here = std::ftell(f);
+      ok = std::fseek(f, 0, SEEK_END) == 0;
+      const long size = ok ? std::ftell(f) : -1;
+      ok = ok && here >= 0 && size >= here && std::fseek(f, here, SEEK_SET) == 0 && hdr[6] < (1ULL << 40) && hdr[7] < (1ULL << 40) &&
+           (uint64_t)(size - here) == 8 * (hdr[6] + hdr[7]);
+    }
     if (ok) {
       buf.resize(hdr[6] + hdr[7]);
       ok = std::fread(buf.data(), 8, buf.size(), f) == buf.size();

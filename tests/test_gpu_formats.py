@@ -169,3 +169,30 @@ def test_unitig_spans_partition_runs(k):
     claimed = ids[(fl & 2) != 0]
     assert len(set(claimed.tolist())) == len(claimed) == len(set(ids.tolist()))
     g.close()
+
+
+def test_restore_rejects_damaged_files():
+    seqs = random_genome(seed=990, n_rec=2, max_len=3000)
+    g = cf1g.Cf1g(k=31, num_bins=4)
+    g.load_records(seqs)
+    g.run_p1_p4()
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "run.cf1g")
+        g.save(path)
+        data = open(path, "rb").read()
+        bad = []
+        bad.append(data[: len(data) - 8])                       # truncated
+        hdr = bytearray(data)
+        hdr[6 * 8:7 * 8] = (1 << 50).to_bytes(8, "little")      # absurd size in the header
+        bad.append(bytes(hdr))
+        bad.append(b"not a save file" + data[15:])              # wrong magic
+        for i, b in enumerate(bad):
+            p = os.path.join(td, "bad%d" % i)
+            open(p, "wb").write(b)
+            h = cf1g.Cf1g(k=31, num_bins=4)
+            h.load_records(seqs)
+            with pytest.raises(cf1g.Cf1gError) as e:
+                h.restore(p)
+            assert e.value.code == -1
+            h.close()
+    g.close()
