@@ -388,11 +388,12 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
 // Tiles staged in LDS and counting-sorted by destination before the stores,
 // so that consecutive lanes write consecutive addresses of one run.
 // (the scatter tile keeps its work-group's LDS under 32 KiB, five per CU;
-// the split tile is a multiple of P3_WG)
+// the split tile is a multiple of P3_WG and keeps u64-key work-groups at
+// 24 KiB, six per CU)
 template <typename KeyT>
 struct StageCfg {
   static constexpr int TILE = sizeof(KeyT) == 8 ? 1920 : 1024;        // k_p3_scatter
-  static constexpr int SPLIT_TILE = sizeof(KeyT) == 8 ? 2048 : 1024;  // k_p3_split
+  static constexpr int SPLIT_TILE = sizeof(KeyT) == 8 ? 1792 : 1024;  // k_p3_split
 };
 
 // Within a staged tile of tn elements with destination ids dst_id[i] < nd:
@@ -822,6 +823,15 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
   fp = (uint32_t)((h << tb) >> 54);
 }
 
+// Work-groups per CU the dedup kernel is compiled for: 8 with u64 keys (64
+// VGPRs, a 16-byte spill, and 8 x 20 KiB of LDS fill a CU: 3.99 -> 3.60 ms
+// on the 300 Mbp genome against 7 per CU at 70 VGPRs); with 16-byte keys the
+// same bound spills 96 bytes per lane, so those keep the compiler's choice.
+template <typename KeyT>
+struct DdOccupancy {
+  static constexpr int WGS_PER_CU = sizeof(KeyT) == 8 ? 8 : 1;
+};
+
 // Dedup of one segment and its BBHash, DD_WG threads holding DD_KPT
 // occurrences each (element e = slot * DD_WG + thread). Each occurrence
 // carries its 10-bit observation mask; the masks of equal keys are OR-joined
@@ -841,7 +851,7 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
 // bits (k <= 6) use a direct-address mask table instead, which takes any
 // number of occurrences.
 template <typename KeyT>
-__global__ void __launch_bounds__(DD_WG) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+__global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
                                                     const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K2, int sb,
                                                     uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
                                                     uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
