@@ -825,11 +825,13 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
 
 // Work-groups per CU the dedup kernel is compiled for: 8 with u64 keys (64
 // VGPRs, a 16-byte spill, and 8 x 20 KiB of LDS fill a CU: 3.99 -> 3.60 ms
-// on the 300 Mbp genome against 7 per CU at 70 VGPRs); with 16-byte keys the
-// same bound spills 96 bytes per lane, so those keep the compiler's choice.
+// on the 300 Mbp genome against 7 per CU at 70 VGPRs); 6 with 16-byte keys
+// (80 VGPRs, a 16-byte spill: 5.22 -> 4.74 ms at k = 63 against the
+// compiler's 5 per CU at 92 VGPRs; 7 per CU spills 44 bytes and gains
+// nothing, 8 spills 96).
 template <typename KeyT>
 struct DdOccupancy {
-  static constexpr int WGS_PER_CU = sizeof(KeyT) == 8 ? 8 : 1;
+  static constexpr int WGS_PER_CU = sizeof(KeyT) == 8 ? 8 : 6;
 };
 
 // Dedup of one segment and its BBHash, DD_WG threads holding DD_KPT
