@@ -659,8 +659,8 @@ __device__ __forceinline__ void wave_lds_order() {
 }
 
 template <int KPT, int WG>
-__device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, uint64_t D, uint64_t* seenA, uint64_t* colA, uint64_t* seenB,
-                               uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhList L,
+__device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns, uint64_t D, uint64_t* seenA, uint64_t* colA,
+                               uint64_t* seenB, uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhList L,
                                BbhShared<WG>* sh, unsigned int* err) {
   // level l state of a slot: h1 + l * bbh_d(h1) (device.hpp), recomputed per
   // level so that only h1 stays in registers
@@ -681,6 +681,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, uint64
     uint32_t pp[KPT];
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
       if ((alive >> s) & 1) {
         const uint32_t pos = bbh_pos(h1[s] + (uint64_t)l * bbh_d(h1[s]), (uint32_t)n);
         pp[s] = pos;
@@ -699,6 +700,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, uint64
     }
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
       if ((alive >> s) & 1) {
         const uint32_t pos = pp[s];
         if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
@@ -737,6 +739,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, uint64
   base += wave_prefix4((uint32_t)__popc(alive));
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
+    if (s >= ns) break;
     if ((alive >> s) & 1) {
       const uint64_t d = bbh_d(h1[s]);
       L.h[base] = h1[s] + (uint64_t)l * d;
@@ -906,6 +909,9 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   uint32_t msk[KPT];
   uint32_t alive = 0;
   uint64_t u = 0;  // distinct keys
+  // slots that hold elements (uniform): the slot loops stop there, so a
+  // segment of n elements costs ceil(n / WG) slots of work, not KPT
+  int ns = KPT;
   if (sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS) {
     // ---- direct-address table of masks over all 2^K2 keys: any number of
     // occurrences; distinct keys come out sorted at occ / obs[start...].
@@ -940,8 +946,10 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     }
     __threadfence_block();
     __syncthreads();
+    ns = (int)((u + WG - 1) / WG);
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < u) {
         key[s] = occ[start + e];
@@ -963,8 +971,10 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     while ((1ull << tb) < 2 * n) tb++;
     const uint32_t tmask = (1u << tb) - 1;
     uint32_t pend = 0;
+    ns = (int)((n + WG - 1) / WG);
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       const bool v = e < n;
       key[s] = v ? occ[start + e] : KeyT{};
@@ -976,6 +986,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     uint32_t hs[KPT], ent[KPT];  // probe slot; this occurrence's entry
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
       uint32_t slot = 0, fp = 0;
       tab_slot(key[s], sb, tb, slot, fp);
       const bool v = (pend >> s) & 1;
@@ -985,10 +996,13 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     while (pend) {
       uint32_t old[KPT];
 #pragma unroll
-      for (int s = 0; s < KPT; s++)
+      for (int s = 0; s < KPT; s++) {
+        if (s >= ns) break;
         if ((pend >> s) & 1) old[s] = atomicCAS(&tab[hs[s]], 0u, ent[s]);
+      }
 #pragma unroll
       for (int s = 0; s < KPT; s++) {
+        if (s >= ns) break;
         if ((pend >> s) & 1) {
           if (old[s] == 0) {
             alive |= 1u << s;
@@ -1007,6 +1021,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     // each occurrence's key, or the pad for a duplicate, goes to kout now
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if ((alive >> s) & 1) obs[start + e] = (uint16_t)((tab[hs[s]] >> 12) & 0x3FFu);
       if (e < n) kout[start + e] = ((alive >> s) & 1) ? key[s] : key_pad<KeyT>();
@@ -1034,8 +1049,11 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   tl.e = (uint16_t*)(tl.d + TAIL_KEYS);
   uint64_t h1[KPT];
 #pragma unroll
-  for (int s = 0; s < KPT; s++) h1[s] = ((alive >> s) & 1) ? bbh_h1(key[s]) : 0;
-  const uint64_t bits = bbh_levels<KPT, WG>(h1, alive, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
+  for (int s = 0; s < KPT; s++) {
+    if (s >= ns) break;
+    h1[s] = ((alive >> s) & 1) ? bbh_h1(key[s]) : 0;
+  }
+  const uint64_t bits = bbh_levels<KPT, WG>(h1, alive, ns, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
   const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < WG
   __syncthreads();
   // blocks {rank before, 7 words}; word ranks for the local ranks below
@@ -1068,6 +1086,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     rr[s] = ~0u;
+    if (s >= ns) continue;
     if ((alive >> s) & 1) {
       const uint32_t p = ppos[s * WG + threadIdx.x];
       const uint32_t wi = p >> 6;
@@ -1079,8 +1098,10 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   __syncthreads();
   uint16_t* stage_m = (uint16_t*)aux_b;
 #pragma unroll
-  for (int s = 0; s < KPT; s++)
+  for (int s = 0; s < KPT; s++) {
+    if (s >= ns) break;
     if (rr[s] != ~0u) stage_m[rr[s]] = obs[start + (uint64_t)s * WG + threadIdx.x];
+  }
   __syncthreads();
   for (uint64_t i = threadIdx.x; i < u; i += WG) vpad[start + i] = stage_m[i];
 }
