@@ -679,17 +679,23 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
       return off;
     }
     uint32_t pp[KPT];
+    // mark: as a lambda, the register allocator keeps the kernel free of
+    // spills (the same loop written inline spills 32 bytes per lane at
+    // 64 VGPRs, and dedup takes 3.35 instead of 3.27 ms)
+    auto mark = [&]() {
 #pragma unroll
-    for (int s = 0; s < KPT; s++) {
-      if (s >= ns) break;
-      if ((alive >> s) & 1) {
-        const uint32_t pos = bbh_pos(h1[s] + (uint64_t)l * bbh_d(h1[s]), (uint32_t)n);
-        pp[s] = pos;
-        const unsigned long long bit = 1ULL << (pos & 63);
-        const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
-        if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+      for (int s = 0; s < KPT; s++) {
+        if (s >= ns) break;
+        if ((alive >> s) & 1) {
+          const uint32_t pos = bbh_pos(h1[s] + (uint64_t)l * bbh_d(h1[s]), (uint32_t)n);
+          pp[s] = pos;
+          const unsigned long long bit = 1ULL << (pos & 63);
+          const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
+          if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+        }
       }
-    }
+    };
+    mark();
     __syncthreads();
     const uint32_t nw = (uint32_t)(n >> 6);
     for (uint32_t i = threadIdx.x; i < nw; i += WG) region[(off >> 6) + i] = seen[i] & ~col[i];
@@ -827,11 +833,11 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
 }
 
 // Work-groups per CU the dedup kernel is compiled for: 8 with u64 keys (64
-// VGPRs, a 16-byte spill, and 8 x 20 KiB of LDS fill a CU: 3.99 -> 3.60 ms
-// on the 300 Mbp genome against 7 per CU at 70 VGPRs); 6 with 16-byte keys
-// (80 VGPRs, a 16-byte spill: 5.22 -> 4.74 ms at k = 63 against the
-// compiler's 5 per CU at 92 VGPRs; 7 per CU spills 44 bytes and gains
-// nothing, 8 spills 96).
+// VGPRs and 8 x 20 KiB of LDS fill a CU; 3.99 -> 3.60 ms on the 300 Mbp
+// genome against 7 per CU at 70 VGPRs); 6 with 16-byte keys (77 VGPRs;
+// 5.22 -> 4.74 ms at k = 63 against the compiler's 5 per CU, and 7 per CU
+// spills). Both compile without spills; a spill of 80 bytes per lane cost
+// 15% of the kernel's time.
 template <typename KeyT>
 struct DdOccupancy {
   static constexpr int WGS_PER_CU = sizeof(KeyT) == 8 ? 8 : 6;
@@ -981,7 +987,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
       msk[s] = v ? (uint32_t)obs[start + e] : 0u;
       pend |= (uint32_t)v << s;
     }
-    for (uint32_t i = threadIdx.x; i <= tmask; i += WG) tab[i] = 0;
+    for (uint32_t i = threadIdx.x; i <= tmask / 4; i += WG) ((uint4*)tab)[i] = make_uint4(0, 0, 0, 0);  // 2^tb >= 64 entries
     __syncthreads();
     uint32_t hs[KPT], ent[KPT];  // probe slot; this occurrence's entry
 #pragma unroll
