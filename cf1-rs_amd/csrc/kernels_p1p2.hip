@@ -124,6 +124,7 @@ struct SkTile {
   uint64_t brk[SK_BRK / 8];
   uint64_t sh[4];
 };
+static_assert(2 * SK_T <= 4 * SK_NJP, "the start list of a tile (u16 slots) fits in SkTile::ord");
 
 __device__ __forceinline__ uint64_t sk_entry(uint64_t pos, uint32_t n, uint32_t bucket) {
   return (pos << 24) | ((uint64_t)n << 16) | (uint64_t)bucket;
@@ -244,31 +245,41 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
     __syncthreads();
     // (4) super k-mer starts at positions t0 + [0, T) (slot j = 1 + li): a
     // valid slot with its break flag set. Its length runs to the next break.
-    uint32_t starts = 0, valid_cnt = 0;
+    // The start slots are listed in scan order (in S.ord, free after step
+    // 2) and emitted from the list by every lane: starts are about one slot
+    // in w/2, so emitting them slot by slot would leave most lanes idle.
+    uint32_t starts = 0, valid_cnt = 0, smask = 0;
     const int li0 = threadIdx.x * SK_Q;
 #pragma unroll
     for (int q = 0; q < SK_Q; q++) {
       const int j = li0 + q + 1;
       if (t0 + (uint64_t)(li0 + q) >= n_pos || !S.valid[PJ(j)]) continue;
       valid_cnt++;
-      if ((brk8[j >> 3] >> (j & 7)) & 1) starts++;
+      if ((brk8[j >> 3] >> (j & 7)) & 1) {
+        starts++;
+        smask |= 1u << q;
+      }
     }
     uint64_t wt;
     sk_scan_excl(valid_cnt, S.sh, &wt);
     uint64_t tot;
-    uint64_t o = sk_scan_excl(starts, S.sh, &tot);
+    uint32_t o = (uint32_t)sk_scan_excl(starts, S.sh, &tot);
+    uint16_t* list = (uint16_t*)S.ord;  // <= SK_T entries
+    while (smask) {
+      const int q = __builtin_ctz(smask);
+      smask &= smask - 1;
+      list[o++] = (uint16_t)(li0 + q + 1);
+    }
+    __syncthreads();
     uint64_t* dst = entries + tile * SK_T;
-#pragma unroll
-    for (int q = 0; q < SK_Q; q++) {
-      const int j = li0 + q + 1;
-      if (t0 + (uint64_t)(li0 + q) >= n_pos || !S.valid[PJ(j)]) continue;
-      if (!((brk8[j >> 3] >> (j & 7)) & 1)) continue;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += SK_WG) {
+      const int j = list[i];
       // next break after j: bits j+1 .. of the break flags (within w <= 49)
       const int wi = (j + 1) >> 6, sh = (j + 1) & 63;
       const uint64_t lo = S.brk[wi], hi = S.brk[wi + 1];
       const uint64_t after = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
       const uint32_t n = (uint32_t)__builtin_ctzll(after) + 1;
-      dst[o++] = sk_entry(t0 + (uint64_t)(li0 + q), n, bucket_hash((uint64_t)S.can[PJ((int)S.pidx[PJ(j)])]));
+      dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bucket_hash((uint64_t)S.can[PJ((int)S.pidx[PJ(j)])]));
     }
     if (threadIdx.x == 0) {
       tile_cnt[tile] = (uint32_t)tot;
