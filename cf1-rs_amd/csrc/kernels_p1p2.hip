@@ -183,47 +183,88 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
     }
     __syncthreads();
     // (2) minimizer slot and validity of the k-mers at slots [0, T + w + 1).
-    // Window minima of (order, slot): for 8 consecutive windows [q, q+w-1]
-    // (w >= 8), min(suffix min of [q, 7), min of the common core [7, w),
-    // prefix min of [w, q+w)); the slot in the low bits makes ties pick the
-    // leftmost m-mer.
+    // Window minima of (order, slot), leftmost on equal orders: for 8
+    // consecutive windows [q, q+w-1] (w >= 8), the least of the suffix
+    // [q, 7), the common core [7, w) and the prefix [w, q+w). Orders and
+    // slots are tracked as two 32-bit values: scans towards higher slots keep
+    // the incumbent on ties, scans towards lower slots take the newcomer, and
+    // a left part beats a right part on ties.
     const int np = SK_T + w + 1;
+    const int span = K + SK_Q - 1;  // bases under the 8 windows of a step
     for (int j0 = threadIdx.x * SK_Q; j0 < np; j0 += SK_WG * SK_Q) {
       const uint64_t g = B + (uint64_t)j0;
       const uint64_t n0 = stream_bits64(nmask, g), n1 = stream_bits64(nmask, g + 64);
-      auto key = [&](int u) -> uint64_t { return ((uint64_t)S.ord[PJ(j0 + u)] << 16) | (uint64_t)(j0 + u); };
-      // slot q's outputs from its window minimum
-      auto emit = [&](int q, uint64_t win) {
+      // validity of the 8 k-mers: bases [g+q, g+q+K) of the N mask are bits
+      // q .. q+K-1 of n0:n1; one test when no N lies under any of them
+      const bool clean = span <= 64 ? (n0 >> (64 - span)) == 0 : (n0 == 0 && (n1 >> (128 - span)) == 0);
+      uint32_t vbits = 0xFFu;
+      if (!clean) {
+        vbits = 0;
+#pragma unroll
+        for (int q = 0; q < SK_Q; q++) {
+          const uint64_t hi = q ? ((n0 << q) | (n1 >> (64 - q))) : n0;
+          vbits |= (uint32_t)((hi >> (64 - K)) == 0) << q;
+        }
+      }
+      auto emit = [&](int q, uint32_t slot) {
         const int j = j0 + q;
         if (j < np) {
-          // bases [g+q, g+q+K) of the N mask: bits q .. q+K-1 of n0:n1
-          const uint64_t hi = q ? ((n0 << q) | (n1 >> (64 - q))) : n0;
-          S.pidx[PJ(j)] = (uint16_t)(win & 0xFFFF);
-          S.valid[PJ(j)] = (hi >> (64 - K)) == 0 ? 1 : 0;
+          S.pidx[PJ(j)] = (uint16_t)slot;
+          S.valid[PJ(j)] = (uint8_t)((vbits >> q) & 1u);
         }
       };
       if (w >= SK_Q) {
-        uint64_t core = ~0ULL;
-#pragma unroll 4
-        for (int u = SK_Q - 1; u < w; u++) core = min64(core, key(u));
-        uint64_t suf = ~0ULL, pre = ~0ULL;
-        uint64_t L[SK_Q];  // L[q] = min(core, min over [q, 7))
+        // slot j0 + u is ord[PJ(j0) + u + (u >> 3)] (j0 is a multiple of 8),
+        // so the unrolled reads take constant offsets
+        const uint32_t* op = &S.ord[PJ(j0)];
+        uint32_t cb = op[SK_Q - 1], ci = SK_Q - 1;
 #pragma unroll
-        for (int q = SK_Q - 1; q >= 0; q--) {
-          if (q < SK_Q - 1) suf = min64(suf, key(q));
-          L[q] = min64(suf, core);
+        for (int u = SK_Q; u < SK_MAXW; u++) {
+          if (u >= w) break;
+          const uint32_t o = op[u + (u >> 3)];
+          if (o < cb) {
+            cb = o;
+            ci = (uint32_t)u;
+          }
         }
+        uint32_t Lb[SK_Q], Li[SK_Q];  // least of [q, w)
+        Lb[SK_Q - 1] = cb;
+        Li[SK_Q - 1] = ci;
+        uint32_t sb = 0, si = 0;
+#pragma unroll
+        for (int q = SK_Q - 2; q >= 0; q--) {
+          const uint32_t o = op[q];
+          if (q == SK_Q - 2 || o <= sb) {
+            sb = o;
+            si = (uint32_t)q;
+          }
+          const bool core = cb < sb;
+          Lb[q] = core ? cb : sb;
+          Li[q] = core ? ci : si;
+        }
+        // slot j0 + w + v is ord[PJ(j0 + w) + v + (((w & 7) + v) >> 3)]
+        const int w7 = w & 7;
+        const uint32_t* opw = &S.ord[PJ(j0 + w)];
+        uint32_t pb = 0, pi = 0;
 #pragma unroll
         for (int q = 0; q < SK_Q; q++) {
-          emit(q, min64(L[q], pre));  // pre = min over [w, w+q)
-          pre = min64(pre, key(w + q));
+          const uint32_t bi = (q > 0 && pb < Lb[q]) ? pi : Li[q];
+          emit(q, (uint32_t)j0 + bi);
+          if (q < SK_Q - 1) {
+            const uint32_t o = opw[q + ((w7 + q) >> 3)];
+            if (q == 0 || o < pb) {
+              pb = o;
+              pi = (uint32_t)(w + q);
+            }
+          }
         }
       } else {
+        auto key = [&](int u) -> uint64_t { return ((uint64_t)S.ord[PJ(j0 + u)] << 16) | (uint64_t)(j0 + u); };
 #pragma unroll
         for (int q = 0; q < SK_Q; q++) {
           uint64_t b = ~0ULL;
           for (int u = 0; u < w; u++) b = min64(b, key(q + u));
-          emit(q, b);
+          emit(q, (uint32_t)(b & 0xFFFF));
         }
       }
     }
