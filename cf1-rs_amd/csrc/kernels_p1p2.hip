@@ -21,71 +21,78 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
   const uint32_t ok = (uint32_t)(d < 32u) & (0x80045u >> (d & 31u));
   return (ok & 1u) ? ((c >> 1) ^ (c >> 2)) & 3u : 4u;
 }
-// bit i of x -> bit 2i
-__device__ __forceinline__ uint64_t spread32(uint32_t v) {
-  uint64_t x = v;
-  x = (x | (x << 16)) & 0x0000FFFF0000FFFFULL;
-  x = (x | (x << 8)) & 0x00FF00FF00FF00FFULL;
-  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0FULL;
-  x = (x | (x << 2)) & 0x3333333333333333ULL;
-  x = (x | (x << 1)) & 0x5555555555555555ULL;
-  return x;
+// Four bytes at once: the 2-bit codes (MSB-first, base 0 in bits 7..6) and
+// the placeholder bits (base 0 in bit 3). ((x >> 1) ^ (x >> 2)) & 3 is the
+// code of an ACGT byte of either case; a byte is ACGT iff its upper-cased
+// value equals the letter of that code (v_perm of the codes into "ACGT").
+// The multiplies gather one bit (two for codes) from each byte lane into the
+// top byte; the partial products below it never overlap.
+__device__ __forceinline__ void pack4(uint32_t x, uint32_t& cb, uint32_t& n4) {
+  const uint32_t u = x & 0xDFDFDFDFu;
+  uint32_t c = ((x >> 1) ^ (x >> 2)) & 0x03030303u;
+  const uint32_t bad = u ^ __builtin_amdgcn_perm(0u, 0x54474341u, c);  // bytes 'A' 'C' 'G' 'T' by code
+  const uint32_t nz = ((((bad & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | bad) & 0x80808080u) >> 7;  // 1 per non-ACGT byte
+  n4 = (nz * 0x08040201u) >> 24;
+  c &= ~(nz * 3u);
+  cb = (c * 0x40100401u) >> 24;
 }
-// One wave packs 64 consecutive 64-position words. Step q reads positions
-// 64q + lane (one coalesced byte per lane); the ballots of the three code
-// bits over the lanes are word q's N-mask and base bits, kept by lane q.
-// Each lane then interleaves its own word's bits.
+// One lane per 64-position word. A word whose positions are all bases of
+// one record reads its 64 bytes with 17 aligned dword loads (the device copy
+// of the records holds 8 bytes past the last one) and converts them 4 at a
+// time; a word that touches a separator or lies past the last record goes
+// position by position.
 __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii, const uint64_t* __restrict__ rec_off,
                                               const uint64_t* __restrict__ gstart, uint64_t n_rec, uint64_t n_words64,
                                               uint64_t* __restrict__ seq, uint64_t* __restrict__ nmask) {
-  const uint64_t w0 = (((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64;
-  if (w0 >= n_words64) return;  // whole waves
-  const uint32_t lane = __lane_id();
-  const uint32_t nq = n_words64 - w0 < 64 ? (uint32_t)(n_words64 - w0) : 64u;
-  const uint64_t gw = w0 * 64;  // first position of the wave
-  // largest r with gstart[r] <= gw (r = n_rec when past the last record)
+  const uint64_t wv0 = (((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64;  // first word of the wave
+  if (wv0 >= n_words64) return;
+  // largest r with gstart[r] <= the wave's first position (r = n_rec past the
+  // last record), then each lane moves on to its own word's record
+  const uint64_t gw = wv0 * 64;
   uint64_t lo = 0, hi = n_rec + 1;
   while (hi - lo > 1) {
     const uint64_t mid = (lo + hi) >> 1;
     if (gstart[mid] <= gw) lo = mid; else hi = mid;
   }
-  uint64_t wn = 0, wh = 0, wl = 0;  // ballots of word `lane`
-  auto emit = [&](uint32_t code, uint32_t q) {
-    const uint64_t bn = __ballot(code >> 2), bh = __ballot((code >> 1) & 1u), bl = __ballot(code & 1u);
-    const bool mine = lane == q;
-    wn = mine ? bn : wn;
-    wh = mine ? bh : wh;
-    wl = mine ? bl : wl;
-    __builtin_amdgcn_sched_barrier(0);  // consume each step's ballots before the next step's
-  };
+  const uint64_t w = wv0 + __lane_id();
+  if (w >= n_words64) return;
+  const uint64_t g0 = w * 64;
   uint64_t r = lo;
-  if (nq == 64 && r < n_rec && gw >= gstart[r] && gw + 64 * 64 < gstart[r + 1]) {
-    // the wave's positions are all bases of record r: contiguous bytes
-    // (gw < gstart[0] on the first wave: position 0 is a separator)
-    const uint8_t* p = ascii + rec_off[r] + (gw - gstart[r]) + lane;
-    for (uint32_t q0 = 0; q0 < 64; q0 += 16) {
-      uint8_t c[16];
+  while (r < n_rec && g0 >= gstart[r + 1]) r++;
+  uint64_t hw = 0, lw = 0, nm = 0;
+  if (r < n_rec && g0 >= gstart[r] && g0 + 64 < gstart[r + 1]) {
+    const uint64_t a = rec_off[r] + (g0 - gstart[r]);
+    const uint32_t* d = (const uint32_t*)(ascii + (a & ~3ULL));
+    const uint32_t sh = (uint32_t)(a & 3);
+    uint32_t v[17];
 #pragma unroll
-      for (int u = 0; u < 16; u++) c[u] = p[(q0 + u) * 64];
+    for (int i = 0; i < 17; i++) v[i] = d[i];
+    uint32_t cw[4] = {0, 0, 0, 0}, mw[2] = {0, 0};
 #pragma unroll
-      for (int u = 0; u < 16; u++) emit(base_code(c[u]), q0 + u);
+    for (int i = 0; i < 16; i++) {
+      uint32_t cb, n4;
+      pack4(__builtin_amdgcn_alignbyte(v[i + 1], v[i], sh), cb, n4);  // bytes a + 4i ..
+      cw[i >> 2] |= cb << (24 - 8 * (i & 3));
+      mw[i >> 3] |= n4 << (28 - 4 * (i & 7));
     }
+    hw = ((uint64_t)cw[0] << 32) | cw[1];
+    lw = ((uint64_t)cw[2] << 32) | cw[3];
+    nm = ((uint64_t)mw[0] << 32) | mw[1];
   } else {
-    uint64_t g = gw + lane;
-    for (uint32_t q = 0; q < nq; q++, g += 64) {
+    uint64_t g = g0;
+    for (int i = 0; i < 64; i++, g++) {
       while (r < n_rec && g >= gstart[r + 1]) r++;
       uint32_t code = 4;
       if (r < n_rec && g >= gstart[r] && g + 1 < gstart[r + 1]) code = base_code(ascii[rec_off[r] + (g - gstart[r])]);
-      emit(code, q);
+      const uint64_t b = code & 3u;
+      if (i < 32) hw |= b << (62 - 2 * i);
+      else lw |= b << (62 - 2 * (i - 32));
+      nm |= (uint64_t)(code >> 2) << (63 - i);
     }
   }
-  if (lane < nq) {
-    // base i of the word (lane i of its step) -> bits 63-2i (high), 62-2i (low)
-    seq[2 * (w0 + lane)] = (spread32(__builtin_bitreverse32((uint32_t)wh)) << 1) | spread32(__builtin_bitreverse32((uint32_t)wl));
-    seq[2 * (w0 + lane) + 1] =
-        (spread32(__builtin_bitreverse32((uint32_t)(wh >> 32))) << 1) | spread32(__builtin_bitreverse32((uint32_t)(wl >> 32)));
-    nmask[w0 + lane] = __builtin_bitreverse64(wn);
-  }
+  seq[2 * w] = hw;
+  seq[2 * w + 1] = lw;
+  nmask[w] = nm;
 }
 
 // ------------------------------------------------------------------ minimizer scan (P1)
