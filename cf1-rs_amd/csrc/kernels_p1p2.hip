@@ -413,6 +413,12 @@ __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------ route (P2)
+// Count and route take RT_U tiles of a chunk per step, one entry of each per
+// thread, and issue each dependent load (entry, bucket -> bin, stream words)
+// for all of them before the next, so that a thread has RT_U chains in
+// flight. Records of one (bin, chunk) run land in the order of the cursor
+// increments, which the schedule decides.
+constexpr int RT_U = 4;
 // Chunks of `chunk_tiles` tiles. k_p2_count writes each chunk's records per
 // bin into mat[bin * n_chunks + chunk]; an exclusive scan of mat (bin-major)
 // gives every (bin, chunk) run its first record; k_p2_route writes the
@@ -432,10 +438,21 @@ __global__ void __launch_bounds__(256) k_p2_count(const uint64_t* __restrict__ e
   const uint64_t ta = chunk * chunk_tiles;
   uint64_t tb = ta + chunk_tiles;
   if (tb > n_tiles) tb = n_tiles;
-  for (uint64_t t = ta; t < tb; t++) {
-    const uint32_t c = tile_cnt[t];
-    const uint64_t* e = entries + t * SK_T;
-    for (uint32_t i = threadIdx.x; i < c; i += 256) atomicAdd(&cnt[b2b[sk_bucket(e[i])]], 1u);
+  for (uint64_t t0 = ta; t0 < tb; t0 += RT_U) {
+    uint32_t c[RT_U], cmax = 0;
+#pragma unroll
+    for (int u = 0; u < RT_U; u++) {
+      c[u] = t0 + u < tb ? tile_cnt[t0 + u] : 0u;
+      cmax = c[u] > cmax ? c[u] : cmax;
+    }
+    for (uint32_t i = threadIdx.x; i < cmax; i += 256) {
+      uint32_t bk[RT_U];
+#pragma unroll
+      for (int u = 0; u < RT_U; u++) bk[u] = i < c[u] ? b2b[sk_bucket(entries[(t0 + u) * SK_T + i])] : ~0u;
+#pragma unroll
+      for (int u = 0; u < RT_U; u++)
+        if (bk[u] != ~0u) atomicAdd(&cnt[bk[u]], 1u);
+    }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < num_bins; b += 256) mat[(uint64_t)b * n_chunks + chunk] = cnt[b];
@@ -461,36 +478,58 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
   const uint64_t ta = chunk * chunk_tiles;
   uint64_t tb = ta + chunk_tiles;
   if (tb > n_tiles) tb = n_tiles;
-  for (uint64_t t = ta; t < tb; t++) {
-    const uint32_t c = tile_cnt[t];
-    const uint64_t* e = entries + t * SK_T;
-    for (uint32_t i = threadIdx.x; i < c; i += 256) {
-      const uint64_t en = e[i];
-      const uint32_t bin = b2b[sk_bucket(en)];
-      const uint64_t out = base[bin] + atomicAdd(&cnt[bin], 1u);
-      if (out >= bin_start[bin + 1] || out < bin_start[bin]) {  // never write outside the bin
-        atomicOr(overflow, 1u);
-        continue;
-      }
-      const uint64_t pos = sk_pos(en);
-      const uint32_t L = sk_len(en) + (uint32_t)K - 1;
-      // bases [pos-1, pos+L]: the super k-mer with one flanking base each side
-      const uint32_t LS = L + 2;
-      uint64_t wv[RW];
+  constexpr uint64_t NONE = ~0ULL;  // no entry (an entry's position is < 2^40)
+  for (uint64_t t0 = ta; t0 < tb; t0 += RT_U) {
+    uint32_t c[RT_U], cmax = 0;
 #pragma unroll
-      for (int q = 0; q < RW; q++) {
-        const int lo_bit = 64 * q;  // record bit offset of this word
-        uint64_t x = stream_bits64(seq, 2 * (pos - 1) + (uint64_t)lo_bit);
-        const int keep = (int)(2 * LS) - lo_bit;  // record bits living in this word
-        if (keep <= 0) x = 0;
-        else if (keep < 64) x &= ~0ULL << (64 - keep);
-        wv[q] = x;
-      }
-      const uint64_t lf = is_n(nmask, pos - 1) ? 0 : 1, rf = is_n(nmask, pos + L) ? 0 : 1;
-      wv[RW - 1] |= (uint64_t)L | (lf << 8) | (rf << 9);
-      uint64_t* dst = rec + out * RW;
+    for (int u = 0; u < RT_U; u++) {
+      c[u] = t0 + u < tb ? tile_cnt[t0 + u] : 0u;
+      cmax = c[u] > cmax ? c[u] : cmax;
+    }
+    for (uint32_t i = threadIdx.x; i < cmax; i += 256) {
+      uint64_t en[RT_U];
 #pragma unroll
-      for (int q = 0; q < RW; q++) dst[q] = wv[q];
+      for (int u = 0; u < RT_U; u++) en[u] = i < c[u] ? entries[(t0 + u) * SK_T + i] : NONE;
+      uint32_t bin[RT_U];
+#pragma unroll
+      for (int u = 0; u < RT_U; u++) bin[u] = en[u] != NONE ? b2b[sk_bucket(en[u])] : 0u;
+      uint64_t out[RT_U];
+#pragma unroll
+      for (int u = 0; u < RT_U; u++) {
+        if (en[u] == NONE) continue;
+        out[u] = base[bin[u]] + atomicAdd(&cnt[bin[u]], 1u);
+        if (out[u] >= bin_start[bin[u] + 1] || out[u] < bin_start[bin[u]]) {  // never write outside the bin
+          atomicOr(overflow, 1u);
+          en[u] = NONE;
+        }
+      }
+      uint64_t wv[RT_U][RW];
+#pragma unroll
+      for (int u = 0; u < RT_U; u++) {
+        if (en[u] == NONE) continue;
+        const uint64_t pos = sk_pos(en[u]);
+        const uint32_t L = sk_len(en[u]) + (uint32_t)K - 1;
+        // bases [pos-1, pos+L]: the super k-mer with one flanking base each side
+        const uint32_t LS = L + 2;
+#pragma unroll
+        for (int q = 0; q < RW; q++) {
+          const int lo_bit = 64 * q;  // record bit offset of this word
+          uint64_t x = stream_bits64(seq, 2 * (pos - 1) + (uint64_t)lo_bit);
+          const int keep = (int)(2 * LS) - lo_bit;  // record bits living in this word
+          if (keep <= 0) x = 0;
+          else if (keep < 64) x &= ~0ULL << (64 - keep);
+          wv[u][q] = x;
+        }
+        const uint64_t lf = is_n(nmask, pos - 1) ? 0 : 1, rf = is_n(nmask, pos + L) ? 0 : 1;
+        wv[u][RW - 1] |= (uint64_t)L | (lf << 8) | (rf << 9);
+      }
+#pragma unroll
+      for (int u = 0; u < RT_U; u++) {
+        if (en[u] == NONE) continue;
+        uint64_t* dst = rec + out[u] * RW;
+#pragma unroll
+        for (int q = 0; q < RW; q++) dst[q] = wv[u][q];
+      }
     }
   }
 }

@@ -626,7 +626,10 @@ __device__ __forceinline__ uint64_t pad_slot_blocks(uint64_t n_occ) { return n_o
 // state, step, element) that wave 0 walks 64 keys at a time, with no block
 // barriers (the LDS operations of one wave execute in order); each level
 // compacts the unplaced keys to the front of the list. seenA / colA must be
-// clear for level 0.
+// clear for level 0. A block-wide level costs every slot of every wave however
+// few keys are left, the list one pass per 64 keys but on one wave: on the
+// 300 Mbp genome (~1,100 keys per segment) 256 measured best (dedup 3.21 ms,
+// against 3.33 ms with 128 and 3.39 ms with 512).
 constexpr int TAIL_KEYS = 256;
 struct BbhList {
   uint64_t* h;  // level state
@@ -660,7 +663,7 @@ __device__ __forceinline__ void wave_lds_order() {
 
 template <int KPT, int WG>
 __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns, uint64_t D, uint64_t* seenA, uint64_t* colA,
-                               uint64_t* seenB, uint64_t* colB, uint64_t* region, uint64_t max_bits, uint32_t* ppos, BbhList L,
+                               uint64_t* seenB, uint64_t* colB, uint64_t* region, uint64_t max_bits, uint16_t* ppos, BbhList L,
                                BbhShared<WG>* sh, unsigned int* err) {
   // level l state of a slot: h1 + l * bbh_d(h1) (device.hpp), recomputed per
   // level so that only h1 stays in registers
@@ -711,7 +714,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
         const uint32_t pos = pp[s];
         if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
           alive &= ~(1u << s);
-          ppos[s * WG + threadIdx.x] = (uint32_t)(off + pos);
+          ppos[s * WG + threadIdx.x] = (uint16_t)(off + pos);
         }
       }
     }
@@ -791,7 +794,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
           pos = bbh_pos(hh, (uint32_t)n);
           placed = ((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1;
         }
-        if (placed) ppos[ee] = (uint32_t)(off + pos);
+        if (placed) ppos[ee] = (uint16_t)(off + pos);
         const bool keep = act && !placed;
         const uint64_t km = __ballot(keep);
         wave_lds_order();
@@ -872,13 +875,13 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   constexpr int KPT = DD_KPT;
   constexpr int CAP = DD_CAP;
   constexpr int TAB_BITS = 12;
-  constexpr int SEEN_W = (2 * CAP) / 64 + 1;
+  constexpr int SEEN_W = (2 * CAP) / 64;  // level 0 has roundup64(2u) <= 2 CAP bits
   constexpr int REG_W = (4 * CAP + 8192) / 64;
   constexpr int TAB_BYTES = 4 << TAB_BITS;
   // tab, byte offsets: hash table (or direct-address mask table) over all of
-  // it; then ppos [0, 4 CAP), word ranks, seen / collision bitmaps and the
-  // tail list
-  constexpr int T_WRANK = 4 * CAP, T_SEEN = T_WRANK + 4 * REG_W, T_LIST = T_SEEN + 4 * 8 * SEEN_W;
+  // it; then ppos (u16) [0, 2 CAP), word ranks, seen / collision bitmaps
+  // and the tail list
+  constexpr int T_WRANK = 2 * CAP, T_SEEN = T_WRANK + 4 * REG_W, T_LIST = T_SEEN + 4 * 8 * SEEN_W;
   constexpr int T_END = T_LIST + TAIL_KEYS * 18;
   // aux: region (level words), the block-scan and level-count words behind
   // it; then the staged masks (all of it)
@@ -887,7 +890,8 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   constexpr int AUX_BYTES = A_END > CAP * 2 ? A_END : CAP * 2;
   static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
   static_assert(T_END <= TAB_BYTES && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
-  static_assert(REG_W / 7 + 2 <= SEEN_W && CAP <= 65536 && T_SEEN % 8 == 0 && T_LIST % 8 == 0, "LDS reuse fits");
+  static_assert(REG_W / 7 + 2 <= SEEN_W && CAP <= 65536 && REG_W * 64 <= 65536 && T_SEEN % 8 == 0 && T_LIST % 8 == 0,
+                "LDS reuse fits");
   __shared__ __align__(16) uint8_t tab_b[TAB_BYTES];
   __shared__ __align__(16) uint8_t aux_b[AUX_BYTES];
   uint64_t* seenA = (uint64_t*)(tab_b + T_SEEN);
@@ -1039,7 +1043,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   }
   // ---- BBHash of the segment, into its padded block slot.
   uint64_t* region = (uint64_t*)aux_b;
-  uint32_t* ppos = tab;
+  uint16_t* ppos = (uint16_t*)tab_b;  // bit positions < REG_W * 64 <= 2^16
   uint32_t* wrank = (uint32_t*)(tab_b + T_WRANK);
   for (int i = threadIdx.x; i < REG_W; i += WG) region[i] = 0;
   for (int i = threadIdx.x; i < SEEN_W; i += WG) {
