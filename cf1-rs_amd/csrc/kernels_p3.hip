@@ -1257,19 +1257,36 @@ __global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, ui
 // One work-group per segment copies the segment's masks (local-rank order,
 // vpad) to their MPHF values in vmask and its BBHash blocks from the padded
 // slot to the dense block order (blk_off).
+// One wave per segment (CP_WG / 64 segments per work-group): the masks and
+// the blocks are copied 8 loads deep per lane, loads before stores, so that a
+// wave keeps several requests in flight (with one work-group per segment the
+// copies were bound by work-group turnover).
+constexpr int CP_WG = 256;
+constexpr int CP_DEPTH = 8;
+template <typename T>
+__device__ __forceinline__ void wave_copy(const T* __restrict__ src, T* __restrict__ dst, uint64_t n) {
+  uint64_t i = __lane_id();
+  for (; i + (CP_DEPTH - 1) * 64 < n; i += CP_DEPTH * 64) {
+    T v[CP_DEPTH];
+#pragma unroll
+    for (int q = 0; q < CP_DEPTH; q++) v[q] = src[i + q * 64];
+#pragma unroll
+    for (int q = 0; q < CP_DEPTH; q++) dst[i + q * 64] = v[q];
+  }
+  for (; i < n; i += 64) dst[i] = src[i];
+}
 template <typename KeyT>
-__global__ void __launch_bounds__(RK_WG) k_p3_compact(const uint16_t* __restrict__ vpad, const uint64_t* __restrict__ seg_off,
+__global__ void __launch_bounds__(CP_WG) k_p3_compact(const uint16_t* __restrict__ vpad, const uint64_t* __restrict__ seg_off,
                                                       const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
                                                       const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks_pad,
                                                       uint64_t n_segs, uint16_t* __restrict__ vmask, uint64_t* __restrict__ blocks) {
-  const uint64_t seg = blockIdx.x;
+  const uint64_t seg = (uint64_t)blockIdx.x * (CP_WG / 64) + (threadIdx.x >> 6);
   if (seg >= n_segs) return;
   const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
   if (u == 0) return;
-  for (uint64_t i = threadIdx.x; i < u; i += RK_WG) vmask[ko + i] = vpad[s0 + i];
+  wave_copy(vpad + s0, vmask + ko, u);
   const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
-  const uint64_t* src = blocks_pad + pad_slot_off(s0, seg) * 8;
-  for (uint64_t i = threadIdx.x; i < nw; i += RK_WG) blocks[b0 * 8 + i] = src[i];
+  wave_copy(blocks_pad + pad_slot_off(s0, seg) * 8, blocks + b0 * 8, nw);
 }
 
 // The dense key array on request: every key P3 left in its segment's run of
@@ -1518,12 +1535,13 @@ void launch_p3_compact(hipStream_t st, bool wide, const uint16_t* vpad, const ui
                        const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad, uint64_t n_segs, uint16_t* vmask,
                        uint64_t* blocks) {
   if (n_segs == 0) return;
+  const dim3 grid((unsigned)((n_segs + CP_WG / 64 - 1) / (CP_WG / 64)));
   if (!wide)
-    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, vpad, seg_off, ucount, key_off,
-                blk_off, blocks_pad, n_segs, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, grid, dim3(CP_WG), 0, st, vpad, seg_off, ucount, key_off, blk_off, blocks_pad,
+                n_segs, vmask, blocks);
   else
-    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, dim3((unsigned)n_segs), dim3(RK_WG), 0, st, vpad, seg_off, ucount, key_off,
-                blk_off, blocks_pad, n_segs, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, grid, dim3(CP_WG), 0, st, vpad, seg_off, ucount, key_off, blk_off, blocks_pad,
+                n_segs, vmask, blocks);
 }
 void launch_p3_keys(hipStream_t st, bool wide, const void* kout, const uint64_t* seg_off, const uint64_t* ucount,
                     const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks, uint64_t n_segs, void* keys,
