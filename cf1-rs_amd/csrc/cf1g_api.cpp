@@ -30,8 +30,9 @@ void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int
                unsigned long long*, unsigned long long*);
 void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
 void launch_p2_count(hipStream_t, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, const uint16_t*, int, uint64_t*);
+uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
-                      uint64_t*, unsigned long long*, uint64_t*, uint64_t*);
+                      uint64_t*, unsigned long long*, uint64_t*, uint64_t*, uint64_t*);
 void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, int,
                      const uint16_t*, int, const uint64_t*, const uint64_t*, uint64_t*, int, unsigned int*);
 // kernels_p3.hip
@@ -381,8 +382,9 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
   uint64_t* info = c->getT<uint64_t>("info", 4);
   uint64_t* totals = c->getT<uint64_t>("totals", 4);
+  uint64_t* pscr = c->getT<uint64_t>("partition_scratch", partition_scratch_words());
   c->seg("p2a", {num_bins, hist_in ? 1u : 0u}, [&] {
-    launch_partition(c->st, hpart, hist, num_bins, CHUNK_RECORDS, b2b, brec, bstart, bcur, coff, info);
+    launch_partition(c->st, hpart, hist, num_bins, CHUNK_RECORDS, b2b, brec, bstart, bcur, coff, info, pscr);
   });
   uint64_t hinfo[4], htot[4];
   hip_check(hipMemcpyAsync(hinfo, info, 32, hipMemcpyDeviceToHost, st), "D2H info");

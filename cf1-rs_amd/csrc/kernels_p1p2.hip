@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
 
 // P1 histogram from the entries, without device atomics: work-group (slice,
 // range r) counts the buckets in [r*16384, (r+1)*16384) of a slice of tiles
-// in LDS (each wave takes one tile at a time) and writes its partial counts
+// in LDS (each wave takes two tiles at a time) and writes its partial counts
 // to part[(slice*4 + r)*16384 ...]; the r = 0 work-groups write the slice's
 // W and S to sums[2*slice ...]. k_p1_reduce adds the partials up.
 constexpr int HB_RANGE = 16384;
@@ -359,15 +359,26 @@ __global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint64_t* __restrict__ 
   if (tb > n_tiles) tb = n_tiles;
   const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
   uint64_t s = 0, wsum = 0;
-  for (uint64_t t = ta + wv; t < tb; t += NW) {
-    const uint32_t c = tile_cnt[t];
-    const uint64_t* e = entries + t * SK_T;
-    for (uint32_t i = lane; i < c; i += 64) {
-      const uint32_t b = sk_bucket(e[i]);
-      if ((b / HB_RANGE) == r) atomicAdd(&h[b % HB_RANGE], 1u);
+  // two tiles per step and four entries of each per lane, loaded before the
+  // LDS updates, so that a lane has eight loads in flight
+  for (uint64_t t = ta + wv; t < tb; t += 2 * NW) {
+    const uint64_t t2 = t + NW;
+    const uint32_t c1 = tile_cnt[t], c2 = t2 < tb ? tile_cnt[t2] : 0u;
+    const uint32_t cm = c1 > c2 ? c1 : c2;
+    for (uint32_t i0 = 0; i0 < cm; i0 += 4 * 64) {
+      uint32_t bk[8];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t i = i0 + (uint32_t)q * 64 + lane;
+        bk[q] = i < c1 ? sk_bucket(entries[t * SK_T + i]) : ~0u;
+        bk[4 + q] = i < c2 ? sk_bucket(entries[t2 * SK_T + i]) : ~0u;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (bk[q] != ~0u && bk[q] / HB_RANGE == r) atomicAdd(&h[bk[q] % HB_RANGE], 1u);
     }
-    s += c;
-    wsum += tile_w[t];
+    s += c1 + c2;
+    wsum += tile_w[t] + (t2 < tb ? tile_w[t2] : 0u);
   }
   if (lane == 0) {
     red[wv] = s;
@@ -543,9 +554,8 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
 //   nb(b) = min(cap, b + 1)                                  (t == 0)
 // because the greedy loop places at most one boundary per bucket and one
 // whenever f(b) >= nb(b-1)+1. bucket_to_bin[b] = nb(b-1) (from_boundaries,
-// minimizer.rs:156-172). One work-group of 1024 threads, 64 buckets each.
+// minimizer.rs:156-172).
 constexpr int PT_WG = 1024;
-constexpr int PT_PER = 65536 / PT_WG;
 static_assert(MAX_BINS % PT_WG == 0, "bins split evenly over the threads");
 // floor(p / d) for d > 0 from a double estimate p * (1/d), corrected exactly
 // (the estimate is within one of the quotient for p, d < 2^52).
@@ -613,85 +623,108 @@ __device__ __forceinline__ int64_t blk_scan_excl_min(int64_t v, int64_t* sh, int
 // Outputs: b2b[65536]; bin_records[B] (super k-mers per bin), bin_start[B+1]
 // (exclusive scan), bin_cursor[B] = 0, chunk_off[B+1] (exclusive scan of
 // ceil(records/chunk)), info[0..3] = {num_bins, S, chunks, unused}.
-__global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* __restrict__ hist,
-                                                     const unsigned long long* __restrict__ hist_true, uint32_t num_bins_req,
-                                                     uint32_t chunk_records, uint16_t* __restrict__ b2b,
+// Four launches over 64 blocks of 1024 buckets (one per thread, coalesced):
+// block sums of hist; g(b) = f(b) - b per bucket with each block's minimum;
+// bucket_to_bin[b] = nb(b-1) from the running minimum (earlier blocks' minima
+// and the block's own exclusive prefix minimum), with each bucket's true
+// count added to its bin's record count; then one work-group scans the
+// bins into record starts and chunk offsets.
+constexpr int PT_NBLK = 65536 / PT_WG;
+// scratch words: block sums, block minima, g per bucket, the bin count
+constexpr uint64_t PT_SCRATCH_WORDS = 2 * PT_NBLK + 65536 + 1;
+__global__ void __launch_bounds__(PT_WG) k_part_sums(const unsigned long long* __restrict__ hist, uint64_t* __restrict__ scratch,
+                                                     uint64_t* __restrict__ bin_records) {
+  __shared__ uint64_t shu[PT_WG / 64 + 1];
+  uint64_t tot;
+  blk_scan_excl_u64(hist[blockIdx.x * PT_WG + threadIdx.x], shu, &tot);
+  if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
+  if (blockIdx.x == 0)
+    for (int b = threadIdx.x; b < MAX_BINS; b += PT_WG) bin_records[b] = 0;
+}
+// total, this block's offset and t = total / num_bins from the block sums
+__device__ __forceinline__ void part_totals(const uint64_t* __restrict__ scratch, uint32_t num_bins_req, uint64_t& pre, uint64_t& t) {
+  uint64_t total = 0;
+  pre = 0;
+  for (int q = 0; q < PT_NBLK; q++) {
+    const uint64_t v = scratch[q];
+    total += v;
+    if (q < (int)blockIdx.x) pre += v;
+  }
+  t = total / (uint64_t)num_bins_req;
+}
+__global__ void __launch_bounds__(PT_WG) k_part_g(const unsigned long long* __restrict__ hist, uint32_t num_bins_req,
+                                                  uint64_t* __restrict__ scratch) {
+  __shared__ uint64_t shu[PT_WG / 64 + 1];
+  __shared__ int64_t shm[PT_WG / 64 + 1];
+  uint64_t pre, t;
+  part_totals(scratch, num_bins_req, pre, t);
+  const int b = blockIdx.x * PT_WG + threadIdx.x;
+  const uint64_t h = hist[b];
+  const uint64_t P = pre + blk_scan_excl_u64(h, shu, nullptr) + h;  // inclusive prefix
+  const int64_t g = t > 0 ? (int64_t)div_floor(P, t, 1.0 / (double)t) - (int64_t)b : INT64_MAX;
+  int64_t* gs = (int64_t*)(scratch + 2 * PT_NBLK);
+  gs[b] = g;
+  const int64_t ex = blk_scan_excl_min(g, shm, INT64_MAX);
+  if (threadIdx.x == PT_WG - 1) ((int64_t*)scratch)[PT_NBLK + blockIdx.x] = g < ex ? g : ex;
+}
+__global__ void __launch_bounds__(PT_WG) k_part_assign(const unsigned long long* __restrict__ hist_true, uint32_t num_bins_req,
+                                                       uint64_t* __restrict__ scratch, uint16_t* __restrict__ b2b,
+                                                       uint64_t* __restrict__ bin_records) {
+  __shared__ int64_t shm[PT_WG / 64 + 1];
+  uint64_t pre, t;
+  part_totals(scratch, num_bins_req, pre, t);
+  const int64_t cap = (int64_t)num_bins_req - 1;
+  int64_t carry = INT64_MAX;  // min of g over the earlier blocks
+  for (int q = 0; q < (int)blockIdx.x; q++) {
+    const int64_t m = ((const int64_t*)scratch)[PT_NBLK + q];
+    carry = m < carry ? m : carry;
+  }
+  const int b = blockIdx.x * PT_WG + threadIdx.x;
+  const int64_t g = ((const int64_t*)(scratch + 2 * PT_NBLK))[b];
+  const int64_t ex = blk_scan_excl_min(g, shm, INT64_MAX);
+  const int64_t run = ex < carry ? ex : carry;  // min of g over j < b
+  // nb(b - 1): boundaries placed up to the previous bucket
+  int64_t prev_nb;
+  if (b == 0) {
+    prev_nb = 0;
+  } else if (t == 0) {
+    prev_nb = (int64_t)b < cap ? (int64_t)b : cap;
+  } else {
+    const int64_t v = (int64_t)(b - 1) + (run < 1 ? run : 1);
+    prev_nb = v < cap ? v : cap;
+  }
+  b2b[b] = (uint16_t)prev_nb;
+  const uint64_t ht = hist_true[b];
+  if (ht) atomicAdd((unsigned long long*)&bin_records[prev_nb], (unsigned long long)ht);
+  if (b == 65535) {  // nb(65535) + 1 bins
+    int64_t nb;
+    if (t == 0) {
+      nb = 65536 < cap ? 65536 : cap;
+    } else {
+      const int64_t r2 = g < run ? g : run;
+      const int64_t v = 65535 + (r2 < 1 ? r2 : 1);
+      nb = v < cap ? v : cap;
+    }
+    scratch[2 * PT_NBLK + 65536] = (uint64_t)(nb + 1);
+  }
+}
+__global__ void __launch_bounds__(PT_WG) k_part_bins(const uint64_t* __restrict__ scratch, uint32_t chunk_records,
                                                      uint64_t* __restrict__ bin_records, uint64_t* __restrict__ bin_start,
                                                      unsigned long long* __restrict__ bin_cursor, uint64_t* __restrict__ chunk_off,
                                                      uint64_t* __restrict__ info) {
   __shared__ uint64_t shu[PT_WG / 64 + 1];
-  __shared__ int64_t shm[PT_WG / 64 + 1];
-  __shared__ unsigned long long binrec[MAX_BINS];
-  __shared__ uint32_t s_nbins;
   const int tid = threadIdx.x;
-  const int b0 = tid * PT_PER;
-  uint64_t loc = 0;
-  for (int q = 0; q < PT_PER; q++) loc += hist[b0 + q];
-  uint64_t total;
-  uint64_t pre = blk_scan_excl_u64(loc, shu, &total);
-  const uint64_t t = total / (uint64_t)num_bins_req;
-  const double inv_t = t ? 1.0 / (double)t : 0.0;
-  const int64_t cap = (int64_t)num_bins_req - 1;
-  // running min of g(j) = f(j) - j over this thread's buckets (inclusive, local)
-  int64_t gmin_loc = INT64_MAX;
-  if (t > 0) {
-    uint64_t P = pre;
-    for (int q = 0; q < PT_PER; q++) {
-      P += hist[b0 + q];
-      int64_t g = (int64_t)div_floor(P, t, inv_t) - (int64_t)(b0 + q);
-      gmin_loc = g < gmin_loc ? g : gmin_loc;
-    }
-  }
-  int64_t gmin_pre = blk_scan_excl_min(gmin_loc, shm, INT64_MAX);  // min over j < b0
-  for (int b = tid; b < (int)num_bins_req && b < MAX_BINS; b += PT_WG) binrec[b] = 0;
-  __syncthreads();
-  {
-    uint64_t P = pre;
-    int64_t run = gmin_pre;
-    // nb(b0 - 1)
-    int64_t prev_nb;
-    if (b0 == 0) {
-      prev_nb = 0;
-    } else if (t == 0) {
-      prev_nb = (int64_t)b0 < cap ? (int64_t)b0 : cap;  // min(cap, (b0-1)+1)
-    } else {
-      int64_t mm = run < 1 ? run : 1;
-      int64_t v = (int64_t)(b0 - 1) + mm;
-      prev_nb = v < cap ? v : cap;
-    }
-    for (int q = 0; q < PT_PER; q++) {
-      int b = b0 + q;
-      P += hist[b];
-      int64_t nb;
-      if (t == 0) {
-        nb = (int64_t)b + 1 < cap ? (int64_t)b + 1 : cap;
-      } else {
-        int64_t g = (int64_t)div_floor(P, t, inv_t) - (int64_t)b;
-        run = g < run ? g : run;
-        int64_t mm = run < 1 ? run : 1;
-        int64_t v = (int64_t)b + mm;
-        nb = v < cap ? v : cap;
-      }
-      uint16_t bin = (uint16_t)prev_nb;
-      b2b[b] = bin;
-      uint64_t ht = hist_true[b];
-      if (ht) atomicAdd(&binrec[bin], (unsigned long long)ht);
-      prev_nb = nb;
-    }
-    if (b0 + PT_PER == 65536) s_nbins = (uint32_t)(prev_nb + 1);
-  }
-  __syncthreads();
   // bin starts and chunk offsets: block scans over MAX_BINS / PT_WG bins per thread
   constexpr int BPT = MAX_BINS / PT_WG;
-  const uint32_t nb = s_nbins;
+  const uint32_t nb = (uint32_t)scratch[2 * PT_NBLK + 65536];
   const double inv_c = 1.0 / (double)chunk_records;
   uint64_t rl = 0, cl = 0;
+  uint64_t rv[BPT];
   for (int q = 0; q < BPT; q++) {
     const uint32_t b = (uint32_t)tid * BPT + q;
-    if (b < nb) {
-      rl += binrec[b];
-      cl += div_floor(binrec[b] + chunk_records - 1, chunk_records, inv_c);
-    }
+    rv[q] = b < nb ? bin_records[b] : 0;
+    rl += rv[q];
+    if (b < nb) cl += div_floor(rv[q] + chunk_records - 1, chunk_records, inv_c);
   }
   uint64_t rtot, ctot;
   uint64_t run = blk_scan_excl_u64(rl, shu, &rtot);
@@ -699,10 +732,9 @@ __global__ void __launch_bounds__(PT_WG) k_partition(const unsigned long long* _
   for (int q = 0; q < BPT; q++) {
     const uint32_t b = (uint32_t)tid * BPT + q;
     if (b < nb) {
-      const uint64_t r = binrec[b];
+      const uint64_t r = rv[q];
       bin_start[b] = run;
       chunk_off[b] = crun;
-      bin_records[b] = r;
       bin_cursor[b] = 0;
       run += r;
       crun += div_floor(r + chunk_records - 1, chunk_records, inv_c);
@@ -773,10 +805,14 @@ void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask,
     CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
                 n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
 }
+uint64_t partition_scratch_words() { return PT_SCRATCH_WORDS; }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,
                       uint32_t chunk_records, uint16_t* b2b, uint64_t* bin_records, uint64_t* bin_start,
-                      unsigned long long* bin_cursor, uint64_t* chunk_off, uint64_t* info) {
-  CF1G_LAUNCH("p2_partition", k_partition, dim3(1), dim3(PT_WG), 0, st, hist, hist_true, num_bins, chunk_records, b2b,
-              bin_records, bin_start, bin_cursor, chunk_off, info);
+                      unsigned long long* bin_cursor, uint64_t* chunk_off, uint64_t* info, uint64_t* scratch) {
+  CF1G_LAUNCH("p2_partition", k_part_sums, dim3(PT_NBLK), dim3(PT_WG), 0, st, hist, scratch, bin_records);
+  CF1G_LAUNCH("p2_partition", k_part_g, dim3(PT_NBLK), dim3(PT_WG), 0, st, hist, num_bins, scratch);
+  CF1G_LAUNCH("p2_partition", k_part_assign, dim3(PT_NBLK), dim3(PT_WG), 0, st, hist_true, num_bins, scratch, b2b, bin_records);
+  CF1G_LAUNCH("p2_partition", k_part_bins, dim3(1), dim3(PT_WG), 0, st, scratch, chunk_records, bin_records, bin_start, bin_cursor,
+              chunk_off, info);
 }
 }  // namespace cf1g
