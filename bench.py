@@ -54,7 +54,7 @@ WORKLOADS = {
 
 PHASE_OF = {
     "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
-    "p2_partition": "P2", "p2_count": "P2", "p2_route": "P2",
+    "p2_partition": "P2", "p2_route": "P2",
     "p3_count": "P3", "p3_scatter": "P3", "p3_split": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
     "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",
@@ -102,8 +102,9 @@ def algorithmic_bytes(info, kb, rw):
       pack      : ASCII in, 2-bit bases + placeholder bits out
       p1_scan   : packed stream in, one 8-byte entry per super k-mer out
       p1_hist   : entries in (the 512 KiB histogram is noise)
-      p2_count  : entries in
-      p2_route  : entries in, the super k-mers' bases in, records out
+      p2_route  : entries in (the kernel reads them twice: it counts its
+                  records per bin before writing them), the super k-mers'
+                  bases in, records out
       p3_count  : records in
       p3_scatter: records in, W (key, mask) occurrences out (64 parts per bin)
       p3_split  : W occurrences in, the same out (segments)
@@ -123,7 +124,6 @@ def algorithmic_bytes(info, kb, rw):
         "pack": NG + seq,
         "p1_scan": seq + 8 * S,
         "p1_hist": 8 * S,
-        "p2_count": 8 * S,
         "p2_route": 8 * S + 2 * rec,
         "p3_count": rec,
         "p3_scatter": rec + W * ob,

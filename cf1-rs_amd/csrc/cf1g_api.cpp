@@ -29,12 +29,11 @@ uint64_t p1_slices(uint64_t);
 void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, uint32_t*, uint32_t*, uint64_t*,
                unsigned long long*, unsigned long long*);
 void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
-void launch_p2_count(hipStream_t, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, const uint16_t*, int, uint64_t*);
 uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
                       uint64_t*, unsigned long long*, uint64_t*, uint64_t*, uint64_t*);
 void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, int,
-                     const uint16_t*, int, const uint64_t*, const uint64_t*, uint64_t*, int, unsigned int*);
+                     const uint16_t*, int, const uint64_t*, unsigned long long*, uint64_t*, int, unsigned int*, unsigned long long*);
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
 void launch_zero(hipStream_t, void*, uint64_t);
@@ -395,16 +394,14 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->n_chunks = hinfo[2];
   c->W = htot[0];
   if (c->S != htot[1]) fail(CF1G_E_INTERNAL, "bin record total differs from the P1 super k-mer count");
-  // count records per (bin, chunk), scan bin-major, write
+  // route: each chunk counts its records per bin, reserves a run from each
+  // bin's cursor and writes them
   const uint64_t nt = p1_tiles(c->n_pos);
   uint64_t chunk_tiles = 0, n_chunks = 0;
   p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
-  const uint64_t nm_ = (uint64_t)c->num_bins * n_chunks;
   uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   unsigned int* ovf = c->getT<unsigned int>("route_overflow", 2);
-  uint64_t* mat = c->getT<uint64_t>("p2_mat", nm_ + 1);
-  uint64_t* moff = c->getT<uint64_t>("p2_off", nm_ + 1);
-  uint64_t* stmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(nm_ + 1) + 16);
+  auto* routed_d = c->getT<unsigned long long>("route_total", 1);
   const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
   const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
@@ -412,14 +409,14 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->seg("p2b", {c->n_pos, c->num_bins, c->S, chunk_tiles, n_chunks, (uint64_t)c->rw, (uint64_t)c->k}, [&] {
     hipStream_t s2 = c->st;
     c->zero(ovf, 8);
-    launch_p2_count(s2, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, b2b, (int)c->num_bins, mat);
-    scan_excl_u64(s2, mat, moff, nm_, moff + nm_, stmp);
-    launch_p2_route(s2, seq, nmask, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, moff, bstart, rec, c->rw,
-                    ovf);
+    c->zero(routed_d, 8);
+    c->zero(bcur, (uint64_t)c->num_bins * 8);
+    launch_p2_route(s2, seq, nmask, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw,
+                    ovf, routed_d);
   });
   uint64_t routed = 0;
   unsigned int hov = 0;
-  if (nm_) hip_check(hipMemcpyAsync(&routed, moff + nm_, 8, hipMemcpyDeviceToHost, st), "D2H routed");
+  c->d2h(&routed, routed_d, 8);
   c->d2h(&hov, ovf, 4);
   if (hov || routed != c->S) fail(CF1G_E_INTERNAL, "P2 routing disagrees with the bin sizes from the histogram");
   c->stage = 3;

@@ -308,10 +308,10 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
         smask |= 1u << q;
       }
     }
-    uint64_t wt;
-    sk_scan_excl(valid_cnt, S.sh, &wt);
-    uint64_t tot;
-    uint32_t o = (uint32_t)sk_scan_excl(starts, S.sh, &tot);
+    // one scan of both counts: valid k-mers in the high half, starts low
+    uint64_t both;
+    uint32_t o = (uint32_t)sk_scan_excl(((uint64_t)valid_cnt << 32) | starts, S.sh, &both);
+    const uint64_t wt = both >> 32, tot = both & 0xFFFFFFFFu;
     uint16_t* list = (uint16_t*)S.ord;  // <= SK_T entries
     while (smask) {
       const int q = __builtin_ctz(smask);
@@ -424,38 +424,48 @@ __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------ route (P2)
-// Count and route take RT_U tiles of a chunk per step, one entry of each per
+// One work-group per chunk of `chunk_tiles` tiles. It counts its records per
+// bin in LDS, reserves one run per bin from the bin's cursor (bins are sized
+// exactly by the partition, from the true histogram), then writes the
+// records. Both passes take RT_U tiles per step, one entry of each per
 // thread, and issue each dependent load (entry, bucket -> bin, stream words)
 // for all of them before the next, so that a thread has RT_U chains in
 // flight (route on the 300 Mbp genome: 1.03 ms with one, 0.86 with 4, 0.77
-// with 8, 0.75 with 16 at twice the registers). Records of one (bin, chunk)
-// run land in the order of the cursor increments, which the schedule decides.
-constexpr int RT_U = 8;
-// Chunks of `chunk_tiles` tiles. k_p2_count writes each chunk's records per
-// bin into mat[bin * n_chunks + chunk]; an exclusive scan of mat (bin-major)
-// gives every (bin, chunk) run its first record; k_p2_route writes the
-// records. Each super k-mer [i, i+n) becomes one fixed-size record in its bin
+// with 8, 0.75 with 16 at twice the registers). The order of records inside
+// a bin follows the schedule. Each super k-mer [i, i+n) becomes one fixed-size record in its bin
 // (the bin record of superkmer.rs:181-202 in fixed width): its L = n-1+K
 // bases plus one flanking base on each side, MSB-first in RW words; the last
 // word's low byte holds L and bits 8/9 say whether the left/right flank is a
 // real base (not a placeholder or record end). The flanks carry the
 // neighbours P4's observations need (classify.rs:76-77).
-__global__ void __launch_bounds__(256) k_p2_count(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
-                                                  uint64_t n_tiles, uint64_t chunk_tiles, uint64_t n_chunks,
-                                                  const uint16_t* __restrict__ b2b, int num_bins, uint64_t* __restrict__ mat) {
-  __shared__ uint32_t cnt[MAX_BINS];
+constexpr int RT_U = 8;
+template <int RW>
+__global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+                                                  const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
+                                                  uint64_t n_tiles, uint64_t chunk_tiles, int K, const uint16_t* __restrict__ b2b,
+                                                  int num_bins, const uint64_t* __restrict__ bin_start,
+                                                  unsigned long long* __restrict__ bin_cursor, uint64_t* __restrict__ rec,
+                                                  unsigned int* __restrict__ overflow, unsigned long long* __restrict__ routed) {
+  // per-bin bases and cursors of this chunk, num_bins of each (dynamic LDS)
+  extern __shared__ __align__(8) uint8_t route_lds[];
+  uint64_t* base = (uint64_t*)route_lds;
+  uint32_t* cnt = (uint32_t*)(base + num_bins);
   const uint64_t chunk = blockIdx.x;
   for (int b = threadIdx.x; b < num_bins; b += 256) cnt[b] = 0;
   __syncthreads();
   const uint64_t ta = chunk * chunk_tiles;
   uint64_t tb = ta + chunk_tiles;
   if (tb > n_tiles) tb = n_tiles;
+  // the chunk's records per bin, then one range per bin reserved from its
+  // cursor (the bin sizes are exact from the histogram)
+  uint64_t mine = 0;
   for (uint64_t t0 = ta; t0 < tb; t0 += RT_U) {
     uint32_t c[RT_U], cmax = 0;
 #pragma unroll
     for (int u = 0; u < RT_U; u++) {
       c[u] = t0 + u < tb ? tile_cnt[t0 + u] : 0u;
       cmax = c[u] > cmax ? c[u] : cmax;
+      mine += threadIdx.x == 0 ? c[u] : 0u;
     }
     for (uint32_t i = threadIdx.x; i < cmax; i += 256) {
       uint32_t bk[RT_U];
@@ -467,29 +477,16 @@ __global__ void __launch_bounds__(256) k_p2_count(const uint64_t* __restrict__ e
     }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < num_bins; b += 256) mat[(uint64_t)b * n_chunks + chunk] = cnt[b];
-}
-
-template <int RW>
-__global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                  const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
-                                                  uint64_t n_tiles, uint64_t chunk_tiles, uint64_t n_chunks, int K,
-                                                  const uint16_t* __restrict__ b2b, int num_bins,
-                                                  const uint64_t* __restrict__ off, const uint64_t* __restrict__ bin_start,
-                                                  uint64_t* __restrict__ rec, unsigned int* __restrict__ overflow) {
-  // per-bin bases and cursors of this chunk, num_bins of each (dynamic LDS)
-  extern __shared__ __align__(8) uint8_t route_lds[];
-  uint64_t* base = (uint64_t*)route_lds;
-  uint32_t* cnt = (uint32_t*)(base + num_bins);
-  const uint64_t chunk = blockIdx.x;
   for (int b = threadIdx.x; b < num_bins; b += 256) {
+    const uint32_t n = cnt[b];
+    if (n) {
+      base[b] = bin_start[b] + atomicAdd(&bin_cursor[b], (unsigned long long)n);
+      if (base[b] + n > bin_start[b + 1]) atomicOr(overflow, 1u);
+    }
     cnt[b] = 0;
-    base[b] = off[(uint64_t)b * n_chunks + chunk];
   }
+  if (threadIdx.x == 0 && mine) atomicAdd(routed, (unsigned long long)mine);
   __syncthreads();
-  const uint64_t ta = chunk * chunk_tiles;
-  uint64_t tb = ta + chunk_tiles;
-  if (tb > n_tiles) tb = n_tiles;
   constexpr uint64_t NONE = ~0ULL;  // no entry (an entry's position is < 2^40)
   for (uint64_t t0 = ta; t0 < tb; t0 += RT_U) {
     uint32_t c[RT_U], cmax = 0;
@@ -786,24 +783,19 @@ void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint6
   *chunk_tiles = ct;
   *n_chunks = (nt + ct - 1) / ct;
 }
-void launch_p2_count(hipStream_t st, const uint64_t* entries, const uint32_t* tile_cnt, uint64_t n_pos, uint64_t chunk_tiles,
-                     uint64_t n_chunks, const uint16_t* b2b, int num_bins, uint64_t* mat) {
-  if (n_chunks == 0) return;
-  CF1G_LAUNCH("p2_count", k_p2_count, dim3((unsigned)n_chunks), dim3(256), 0, st, entries, tile_cnt, p1_tiles(n_pos), chunk_tiles,
-              n_chunks, b2b, num_bins, mat);
-}
 void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint32_t* tile_cnt,
                      uint64_t n_pos, uint64_t chunk_tiles, uint64_t n_chunks, int K, const uint16_t* b2b, int num_bins,
-                     const uint64_t* off, const uint64_t* bin_start, uint64_t* rec, int rw, unsigned int* overflow) {
+                     const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, int rw, unsigned int* overflow,
+                     unsigned long long* routed) {
   if (n_chunks == 0) return;
   const uint64_t nt = p1_tiles(n_pos);
   const size_t lds = (size_t)num_bins * 12;
   if (rw == 2)
     CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
-                n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
+                K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
   else
     CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
-                n_chunks, K, b2b, num_bins, off, bin_start, rec, overflow);
+                K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
 }
 uint64_t partition_scratch_words() { return PT_SCRATCH_WORDS; }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,
