@@ -123,11 +123,14 @@ __device__ __forceinline__ int PJ(int j) { return j + (j >> 3); }
 constexpr int SK_NJP = SK_NJ + SK_NJ / 8 + 8;
 constexpr int SK_NPP = (SK_T + SK_MAXW + 8) * 9 / 8 + 8;
 constexpr int SK_BRK = (SK_T + 2 * SK_MAXW) / 8 + 16;  // bytes of break flags (bit j%8 of byte j/8)
+// pidx: the minimizer slot of the k-mer at each slot, bit 15 set when the
+// k-mer is not valid (a placeholder among its bases). The canonical m-mer
+// values are not kept: a super k-mer start reads its minimizer's m-mer from
+// the stream again, which leaves the tile at ~15 KiB (7 work-groups per CU).
+constexpr uint16_t SK_INVALID = 0x8000;
 struct SkTile {
   uint32_t ord[SK_NJP];
-  uint32_t can[SK_NJP];
   uint16_t pidx[SK_NPP];
-  uint8_t valid[SK_NPP];
   uint64_t brk[SK_BRK / 8];
   uint64_t sh[4];
 };
@@ -161,7 +164,7 @@ __device__ __forceinline__ uint64_t sk_scan_excl(uint64_t v, uint64_t* sh4, uint
 }
 __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-__global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+__global__ void __launch_bounds__(SK_WG, 7) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                    uint64_t n_pos, uint64_t n_tiles, int K, int m,
                                                    uint64_t* __restrict__ entries, uint32_t* __restrict__ tile_cnt,
                                                    uint32_t* __restrict__ tile_w) {
@@ -182,10 +185,7 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
         const uint32_t f = (uint32_t)(x >> (64 - 2 * m - 2 * q)) & M;
         const uint32_t r = (uint32_t)(rx >> (2 * q)) & M;
         const uint32_t c = f < r ? f : r;
-        if (j0 + q < nj) {
-          S.can[PJ(j0 + q)] = c;
-          S.ord[PJ(j0 + q)] = mmer_order(c);
-        }
+        if (j0 + q < nj) S.ord[PJ(j0 + q)] = mmer_order(c);
       }
     }
     __syncthreads();
@@ -215,10 +215,7 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
       }
       auto emit = [&](int q, uint32_t slot) {
         const int j = j0 + q;
-        if (j < np) {
-          S.pidx[PJ(j)] = (uint16_t)slot;
-          S.valid[PJ(j)] = (uint8_t)((vbits >> q) & 1u);
-        }
+        if (j < np) S.pidx[PJ(j)] = (uint16_t)(slot | ((vbits >> q) & 1u ? 0u : SK_INVALID));
       };
       if (w >= SK_Q) {
         // slot j0 + u is ord[PJ(j0) + u + (u >> 3)] (j0 is a multiple of 8)
@@ -283,8 +280,10 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
       for (int q = 0; q < SK_Q; q++) {
         const int j = j0 + q;
         bool br = true;
-        if (j >= 1 && j < np)
-          br = !(S.valid[PJ(j)] && S.valid[PJ(j - 1)] && S.pidx[PJ(j)] == S.pidx[PJ(j - 1)]);
+        if (j >= 1 && j < np) {
+          const uint32_t a = S.pidx[PJ(j)], b = S.pidx[PJ(j - 1)];
+          br = (a & SK_INVALID) || a != b;  // unequal when either is invalid and the other not
+        }
         bits |= (uint32_t)br << q;
       }
       brk8[j0 >> 3] = (uint8_t)bits;
@@ -300,7 +299,7 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
 #pragma unroll
     for (int q = 0; q < SK_Q; q++) {
       const int j = li0 + q + 1;
-      if (t0 + (uint64_t)(li0 + q) >= n_pos || !S.valid[PJ(j)]) continue;
+      if (t0 + (uint64_t)(li0 + q) >= n_pos || (S.pidx[PJ(j)] & SK_INVALID)) continue;
       valid_cnt++;
       if ((brk8[j >> 3] >> (j & 7)) & 1) {
         starts++;
@@ -326,7 +325,10 @@ __global__ void __launch_bounds__(SK_WG, 5) k_p1_scan(const uint64_t* __restrict
       const uint64_t lo = S.brk[wi], hi = S.brk[wi + 1];
       const uint64_t after = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
       const uint32_t n = (uint32_t)__builtin_ctzll(after) + 1;
-      dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bucket_hash((uint64_t)S.can[PJ((int)S.pidx[PJ(j)])]));
+      // the canonical m-mer at the minimizer slot, from the stream (as in step 1)
+      const uint64_t x = stream_bits64(seq, 2 * (B + (uint64_t)S.pidx[PJ(j)]));
+      const uint32_t f = (uint32_t)(x >> (64 - 2 * m)) & M, r = (uint32_t)rc64(x, 32) & M;
+      dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bucket_hash((uint64_t)(f < r ? f : r)));
     }
     if (threadIdx.x == 0) {
       tile_cnt[tile] = (uint32_t)tot;
