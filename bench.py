@@ -55,7 +55,7 @@ WORKLOADS = {
 PHASE_OF = {
     "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
     "p2_partition": "P2", "p2_route": "P2",
-    "p3_count": "P3", "p3_scatter": "P3", "p3_split": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
+    "p3_gate": "P3", "p3_count": "P3", "p3_scatter": "P3", "p3_split": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
     "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",
 }

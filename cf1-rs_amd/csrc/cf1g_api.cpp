@@ -42,8 +42,10 @@ int p3_part_bits(int);
 void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int, uint64_t*,
                      uint64_t*);
 void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
-                       const uint64_t*, void*, uint16_t*);
-void launch_p3_split(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, uint32_t, int, void*, uint16_t*);
+                       const uint64_t*, void*, uint16_t*, const unsigned int*);
+void launch_p3_split(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, uint32_t, int, void*, uint16_t*,
+                     const unsigned int*);
+void launch_p3_gate(hipStream_t, const uint64_t*, uint64_t, unsigned int*);
 void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint64_t*, uint32_t*,
                      unsigned int*, uint32_t*, uint64_t*, void*, uint16_t*);
 void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
@@ -400,25 +402,24 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   uint64_t chunk_tiles = 0, n_chunks = 0;
   p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
   uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
-  unsigned int* ovf = c->getT<unsigned int>("route_overflow", 2);
-  auto* routed_d = c->getT<unsigned long long>("route_total", 1);
+  // one read after the route: the routed-record total, then the overflow flag
+  auto* route_chk = c->getT<unsigned long long>("route_check", 2);
+  auto* routed_d = route_chk;
+  unsigned int* ovf = (unsigned int*)(route_chk + 1);
   const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
   const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nmask = c->getT<uint64_t>("nmask", c->n_words64 + 8);
   c->seg("p2b", {c->n_pos, c->num_bins, c->S, chunk_tiles, n_chunks, (uint64_t)c->rw, (uint64_t)c->k}, [&] {
     hipStream_t s2 = c->st;
-    c->zero(ovf, 8);
-    c->zero(routed_d, 8);
+    c->zero(route_chk, 16);
     c->zero(bcur, (uint64_t)c->num_bins * 8);
     launch_p2_route(s2, seq, nmask, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw,
                     ovf, routed_d);
   });
-  uint64_t routed = 0;
-  unsigned int hov = 0;
-  c->d2h(&routed, routed_d, 8);
-  c->d2h(&hov, ovf, 4);
-  if (hov || routed != c->S) fail(CF1G_E_INTERNAL, "P2 routing disagrees with the bin sizes from the histogram");
+  unsigned long long chk[2] = {0, 0};
+  c->d2h(chk, route_chk, 16);
+  if (chk[1] || chk[0] != c->S) fail(CF1G_E_INTERNAL, "P2 routing disagrees with the bin sizes from the histogram");
   c->stage = 3;
 }
 
@@ -449,7 +450,7 @@ void do_p3(cf1g_ctx* c) {
   uint64_t* blk_off = c->getT<uint64_t>("bbh_blk_off", ns + 1);
   uint32_t* over = c->getT<uint32_t>("seg_over", ns + 1);
   uint32_t* big = c->getT<uint32_t>("bbh_big", ns + 1);
-  unsigned int* flags = c->getT<unsigned int>("p3_flags", 4);  // [0] n_over, [1] n_big, [2] error bits
+  unsigned int* flags = c->getT<unsigned int>("p3_flags", 4);  // [0] n_over, [1] n_big, [2] error bits, [3] count gate
   uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
   const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   const uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
@@ -475,32 +476,33 @@ void do_p3(cf1g_ctx* c) {
     scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
   };
   const std::vector<uint64_t> shape = {c->S, c->W, c->n_chunks, c->num_bins, (uint64_t)sb, (uint64_t)K, (uint64_t)c->rw};
-  c->seg("p3a", shape, [&] {
+  // count and scans, then (unless the gate finds the counted occurrences
+  // differ from W) the two-level scatter and the dedup: one launch sequence,
+  // checked afterwards with the flags
+  c->seg("p3ab", shape, [&] {
     c->zero(counts, (ns + 1) * 8);
     c->zero(flags, 16);
     launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts, pmat);
     scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
     scan_excl_u64(st, pmat, poff, nm, poff + nm, tmp2);
-  });
-  // the occurrence buffers are sized by W: check before anything is scattered
-  uint64_t wcount = 0;
-  c->d2h(&wcount, seg_off + ns, 8);
-  if (wcount != c->W) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
-  c->seg("p3b", shape, [&] {
+    launch_p3_gate(st, seg_off + ns, c->W, flags);
     if (pb == sb) {
-      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, occ, occ_obs);
+      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, occ, occ_obs,
+                        flags + 3);
     } else {
-      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, aux, aux_obs);
-      launch_p3_split(st, c->wide, aux, aux_obs, seg_off, c->num_bins, sb, occ, occ_obs);
+      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, aux, aux_obs,
+                        flags + 3);
+      launch_p3_split(st, c->wide, aux, aux_obs, seg_off, c->num_bins, sb, occ, occ_obs, flags + 3);
     }
     launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, aux, vpad);
     scans();
   });
-  unsigned int hf[3];
+  unsigned int hf[4];
   uint64_t tot[3] = {0, 0, 0};
-  hip_check(hipMemcpyAsync(hf, flags, 12, hipMemcpyDeviceToHost, st), "D2H flags");
+  hip_check(hipMemcpyAsync(hf, flags, 16, hipMemcpyDeviceToHost, st), "D2H flags");
   hip_check(hipMemcpyAsync(&tot[1], key_off + ns, 8, hipMemcpyDeviceToHost, st), "D2H D");
   c->d2h(&tot[2], blk_off + ns, 8);
+  if (hf[3]) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
   c->n_over = hf[0];
   if (hf[0] > 0) {
     launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over, hf[0], ucount, flags, big);

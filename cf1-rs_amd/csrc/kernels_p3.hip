@@ -426,7 +426,7 @@ template <typename KeyT, int RW>
 __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
                                                       const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
                                                       int K, int sb, int pb, const uint64_t* __restrict__ moff, KeyT* __restrict__ occ,
-                                                      uint16_t* __restrict__ occ_obs) {
+                                                      uint16_t* __restrict__ occ_obs, const unsigned int* __restrict__ stop) {
   constexpr int TILE = StageCfg<KeyT>::TILE;
   __shared__ KeyT tk[TILE];
   __shared__ uint16_t tm[TILE];
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   __shared__ uint16_t oo[P3_WG];
   int bin;
   uint64_t r0, r1;
-  if (!p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
+  if (*stop || !p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
   const int np = 1 << pb, psh = sb - pb;
   const uint64_t kmask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
   const uint64_t c0 = chunk_off[bin], nch = chunk_off[bin + 1] - c0, c = (uint64_t)blockIdx.x - c0;
@@ -513,9 +513,10 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
 template <typename KeyT>
 __global__ void __launch_bounds__(P3_WG) k_p3_split(const KeyT* __restrict__ src, const uint16_t* __restrict__ src_obs,
                                                     const uint64_t* __restrict__ seg_off, int sb, int pb, KeyT* __restrict__ dst,
-                                                    uint16_t* __restrict__ dst_obs) {
+                                                    uint16_t* __restrict__ dst_obs, const unsigned int* __restrict__ stop) {
   constexpr int TILE = StageCfg<KeyT>::SPLIT_TILE;
   constexpr int MAXS = 1 << (P3_MAX_SB - P3_PART_BITS);
+  if (*stop) return;
   __shared__ KeyT tk[TILE];
   __shared__ uint16_t tm[TILE];
   __shared__ uint8_t tp[TILE];
@@ -905,7 +906,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   unsigned int* n_big = flags + 1;
   unsigned int* err = flags + 2;
   const uint64_t seg = blockIdx.x;
-  if (seg >= n_segs) return;
+  if (seg >= n_segs || flags[3]) return;  // flags[3]: the occurrence counts disagreed with W (k_p3_gate)
   const uint64_t start = seg_off[seg];
   const uint64_t n = seg_off[seg + 1] - start;
   if (n == 0) {
@@ -1467,27 +1468,36 @@ void launch_p3_count(hipStream_t st, bool wide, const uint64_t* rec, const uint6
 }
 void launch_p3_scatter(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
                        int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, const uint64_t* moff, void* occ,
-                       uint16_t* occ_obs) {
+                       uint16_t* occ_obs, const unsigned int* stop) {
   if (n_chunks == 0) return;
   const int pb = p3_part_bits(sb);
   if (!wide)
     CF1G_LAUNCH("p3_scatter", (k_p3_scatter<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (uint64_t*)occ, occ_obs);
+                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (uint64_t*)occ, occ_obs, stop);
   else
     CF1G_LAUNCH("p3_scatter", (k_p3_scatter<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (Key128*)occ, occ_obs);
+                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (Key128*)occ, occ_obs, stop);
 }
 void launch_p3_split(hipStream_t st, bool wide, const void* src, const uint16_t* src_obs, const uint64_t* seg_off, uint32_t num_bins,
-                     int sb, void* dst, uint16_t* dst_obs) {
+                     int sb, void* dst, uint16_t* dst_obs, const unsigned int* stop) {
   const int pb = p3_part_bits(sb);
   const uint64_t parts = (uint64_t)num_bins << pb;
   if (parts == 0) return;
   if (!wide)
     CF1G_LAUNCH("p3_split", k_p3_split<uint64_t>, dim3((unsigned)parts), dim3(P3_WG), 0, st, (const uint64_t*)src, src_obs, seg_off,
-                sb, pb, (uint64_t*)dst, dst_obs);
+                sb, pb, (uint64_t*)dst, dst_obs, stop);
   else
     CF1G_LAUNCH("p3_split", k_p3_split<Key128>, dim3((unsigned)parts), dim3(P3_WG), 0, st, (const Key128*)src, src_obs, seg_off, sb,
-                pb, (Key128*)dst, dst_obs);
+                pb, (Key128*)dst, dst_obs, stop);
+}
+// Compares the scanned occurrence total with W (the valid k-mers P1 counted)
+// before the occurrence pass: on a mismatch flags[3] is set, and scatter,
+// split and dedup return at once instead of writing past buffers sized by W.
+__global__ void k_p3_gate(const uint64_t* __restrict__ total, uint64_t expect, unsigned int* __restrict__ flags) {
+  flags[3] = *total == expect ? 0u : 1u;
+}
+void launch_p3_gate(hipStream_t st, const uint64_t* total, uint64_t expect, unsigned int* flags) {
+  CF1G_LAUNCH("p3_gate", k_p3_gate, dim3(1), dim3(1), 0, st, total, expect, flags);
 }
 void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
                      uint64_t* ucount, uint64_t* nblk, uint32_t* over_list, unsigned int* flags, uint32_t* big_list,
