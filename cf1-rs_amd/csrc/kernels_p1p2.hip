@@ -411,19 +411,29 @@ __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ 
   const uint32_t b = blockIdx.x * 64 + bl;  // grid covers 65536 buckets, 64 per work-group
   uint64_t acc = 0;
   uint32_t sl = g;
-  for (; sl + 12 < n_slices; sl += 16) {
-    const uint32_t v0 = part[(uint64_t)sl * 65536 + b], v1 = part[(uint64_t)(sl + 4) * 65536 + b];
-    const uint32_t v2 = part[(uint64_t)(sl + 8) * 65536 + b], v3 = part[(uint64_t)(sl + 12) * 65536 + b];
-    acc += (uint64_t)v0 + v1 + v2 + v3;
+  for (; sl + 60 < n_slices; sl += 64) {  // 16 loads in flight per thread
+    uint32_t v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = part[(uint64_t)(sl + 4 * q) * 65536 + b];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc += v[q];
   }
   for (; sl < n_slices; sl += 4) acc += part[(uint64_t)sl * 65536 + b];
   acc4[g][bl] = acc;
   __syncthreads();
   if (g == 0) hist[b] = acc4[0][bl] + acc4[1][bl] + acc4[2][bl] + acc4[3][bl];
-  if (blockIdx.x == 0 && threadIdx.x < 2) {
+  if (blockIdx.x == 0) {  // totals {W, S}: every thread of the block adds a stripe of the slices
+    __syncthreads();
     uint64_t t = 0;
-    for (uint32_t q = 0; q < n_slices; q++) t += sums[2 * q + threadIdx.x];
-    totals[threadIdx.x] = t;
+    for (uint32_t q = threadIdx.x >> 1; q < n_slices; q += 128) t += sums[2 * q + (threadIdx.x & 1)];
+    uint64_t* red = &acc4[0][0];  // 256 words
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int d = 128; d >= 2; d >>= 1) {
+      if ((int)threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+      __syncthreads();
+    }
+    if (threadIdx.x < 2) totals[threadIdx.x] = red[threadIdx.x];
   }
 }
 
