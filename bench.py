@@ -100,8 +100,9 @@ def algorithmic_bytes(info, kb, rw):
     occurrences / super k-mers / distinct k-mers, B = BBHash bytes, rec = S
     fixed-width records, ob = key + 2-byte mask.
       pack      : ASCII in, 2-bit bases + placeholder bits out
-      p1_scan   : packed stream in, one 8-byte entry per super k-mer out
-      p1_hist   : entries in (the 512 KiB histogram is noise)
+      p1_scan   : packed stream in, one 8-byte entry and its 2-byte bucket
+                  per super k-mer out
+      p1_hist   : buckets in (the 512 KiB histogram is noise)
       p2_route  : entries in (the kernel reads them twice: it counts its
                   records per bin before writing them), the super k-mers'
                   bases in, records out
@@ -122,8 +123,8 @@ def algorithmic_bytes(info, kb, rw):
     ob = kb + 2
     return {
         "pack": NG + seq,
-        "p1_scan": seq + 8 * S,
-        "p1_hist": 8 * S,
+        "p1_scan": seq + 10 * S,
+        "p1_hist": 2 * S,
         "p2_route": 8 * S + 2 * rec,
         "p3_count": rec,
         "p3_scatter": rec + W * ob,

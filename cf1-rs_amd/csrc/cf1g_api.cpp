@@ -26,8 +26,8 @@ void launch_pack(hipStream_t, const uint8_t*, const uint64_t*, const uint64_t*, 
 uint64_t p1_tiles(uint64_t);
 uint64_t p1_tile_positions();
 uint64_t p1_slices(uint64_t);
-void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, uint32_t*, uint32_t*, uint64_t*,
-               unsigned long long*, unsigned long long*);
+void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint16_t*, uint32_t*, uint32_t*, uint32_t*,
+               uint64_t*, unsigned long long*, unsigned long long*);
 void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
 uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
@@ -355,6 +355,7 @@ void do_p1(cf1g_ctx* c) {
   uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
   uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
+  uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
   c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m, (uint64_t)c->host_packed}, [&] {
@@ -362,7 +363,7 @@ void do_p1(cf1g_ctx* c) {
     if (!c->host_packed) launch_pack(st, ascii, roff, gst, c->n_rec, c->n_words64, seq, nm);
     c->zero(hist, CF1G_NUM_BUCKETS * 8);
     c->zero(tot, 32);
-    launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, tcnt, tw, part, sums, hist, tot);
+    launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, bkt, tcnt, tw, part, sums, hist, tot);
   });
   c->stage = 2;
 }

@@ -169,8 +169,8 @@ __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b
 
 __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                    uint64_t n_pos, uint64_t n_tiles, int K, int m,
-                                                   uint64_t* __restrict__ entries, uint32_t* __restrict__ tile_cnt,
-                                                   uint32_t* __restrict__ tile_w) {
+                                                   uint64_t* __restrict__ entries, uint16_t* __restrict__ buckets,
+                                                   uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ tile_w) {
   __shared__ SkTile S;
   const int w = K - m + 1;
   const uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
@@ -331,7 +331,9 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
       // the canonical m-mer at the minimizer slot, from the stream (as in step 1)
       const uint64_t x = stream_bits64(seq, 2 * (B + (uint64_t)S.pidx[PJ(j)]));
       const uint32_t f = (uint32_t)(x >> (64 - 2 * m)) & M, r = (uint32_t)rc64(x, 32) & M;
-      dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bucket_hash((uint64_t)(f < r ? f : r)));
+      const uint32_t bk = bucket_hash((uint64_t)(f < r ? f : r));
+      dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bk);
+      buckets[tile * SK_T + i] = (uint16_t)bk;  // for the histogram, which needs only this
     }
     if (threadIdx.x == 0) {
       tile_cnt[tile] = (uint32_t)tot;
@@ -341,7 +343,8 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
   }
 }
 
-// P1 histogram from the entries, without device atomics: work-group (slice,
+// P1 histogram from the entries' buckets (a u16 copy the scan writes beside
+// each entry), without device atomics: work-group (slice,
 // range r) counts the buckets in [r*16384, (r+1)*16384) of a slice of tiles
 // in LDS (each wave takes two tiles at a time) and writes its partial counts
 // to part[(slice*4 + r)*16384 ...]; the r = 0 work-groups write the slice's
@@ -349,7 +352,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
 constexpr int HB_RANGE = 16384;
 constexpr int HB_SLICES = 512;
 constexpr int HB_WG = 1024;  // 16 waves share one 64 KiB LDS range
-__global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
+__global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint16_t* __restrict__ buckets, const uint32_t* __restrict__ tile_cnt,
                                                    const uint32_t* __restrict__ tile_w, uint64_t n_tiles, uint64_t tiles_per_slice,
                                                    uint32_t* __restrict__ part, uint64_t* __restrict__ sums) {
   constexpr int NW = HB_WG / 64;
@@ -374,8 +377,8 @@ __global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint64_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const uint32_t i = i0 + (uint32_t)q * 64 + lane;
-        bk[q] = i < c1 ? sk_bucket(entries[t * SK_T + i]) : ~0u;
-        bk[4 + q] = i < c2 ? sk_bucket(entries[t2 * SK_T + i]) : ~0u;
+        bk[q] = i < c1 ? (uint32_t)buckets[t * SK_T + i] : ~0u;
+        bk[4 + q] = i < c2 ? (uint32_t)buckets[t2 * SK_T + i] : ~0u;
       }
 #pragma unroll
       for (int q = 0; q < 8; q++)
@@ -775,16 +778,17 @@ uint64_t p1_slices(uint64_t n_pos) {
   return nt < (uint64_t)HB_SLICES ? nt : (uint64_t)HB_SLICES;
 }
 void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m, uint64_t* entries,
-               uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
+               uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
                unsigned long long* totals) {
   const uint64_t nt = p1_tiles(n_pos);
   if (nt == 0) return;
   const uint64_t grid = nt < 4096 ? nt : 4096;
-  CF1G_LAUNCH("p1_scan", k_p1_scan, dim3((unsigned)grid), dim3(SK_WG), 0, st, seq, nmask, n_pos, nt, K, m, entries, tile_cnt, tile_w);
+  CF1G_LAUNCH("p1_scan", k_p1_scan, dim3((unsigned)grid), dim3(SK_WG), 0, st, seq, nmask, n_pos, nt, K, m, entries, buckets, tile_cnt,
+              tile_w);
   const uint64_t slices = p1_slices(n_pos);
   const uint64_t per = (nt + slices - 1) / slices;
   const uint64_t used = (nt + per - 1) / per;
-  CF1G_LAUNCH("p1_hist", k_p1_hist, dim3((unsigned)used, 65536 / HB_RANGE), dim3(HB_WG), 0, st, entries, tile_cnt, tile_w, nt, per,
+  CF1G_LAUNCH("p1_hist", k_p1_hist, dim3((unsigned)used, 65536 / HB_RANGE), dim3(HB_WG), 0, st, buckets, tile_cnt, tile_w, nt, per,
               part, sums);
   CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 64), dim3(256), 0, st, part, sums, (uint32_t)used, hist, totals);
 }
