@@ -32,7 +32,8 @@ void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
 uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
                       uint64_t*, unsigned long long*, uint64_t*, uint64_t*, uint64_t*);
-void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, uint64_t, uint64_t, uint64_t, int,
+void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint16_t*, const uint32_t*, uint64_t, uint64_t,
+                     uint64_t, int,
                      const uint16_t*, int, const uint64_t*, unsigned long long*, uint64_t*, int, unsigned int*, unsigned long long*);
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
@@ -408,6 +409,7 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   auto* routed_d = route_chk;
   unsigned int* ovf = (unsigned int*)(route_chk + 1);
   const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
+  const uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
   const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nmask = c->getT<uint64_t>("nmask", c->n_words64 + 8);
@@ -415,7 +417,7 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
     hipStream_t s2 = c->st;
     c->zero(route_chk, 16);
     c->zero(bcur, (uint64_t)c->num_bins * 8);
-    launch_p2_route(s2, seq, nmask, ent, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw,
+    launch_p2_route(s2, seq, nmask, ent, bkt, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw,
                     ovf, routed_d);
   });
   unsigned long long chk[2] = {0, 0};

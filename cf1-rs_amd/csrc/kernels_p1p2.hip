@@ -458,8 +458,9 @@ __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ 
 constexpr int RT_U = 8;
 template <int RW>
 __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                  const uint64_t* __restrict__ entries, const uint32_t* __restrict__ tile_cnt,
-                                                  uint64_t n_tiles, uint64_t chunk_tiles, int K, const uint16_t* __restrict__ b2b,
+                                                  const uint64_t* __restrict__ entries, const uint16_t* __restrict__ buckets,
+                                                  const uint32_t* __restrict__ tile_cnt, uint64_t n_tiles, uint64_t chunk_tiles, int K,
+                                                  const uint16_t* __restrict__ b2b,
                                                   int num_bins, const uint64_t* __restrict__ bin_start,
                                                   unsigned long long* __restrict__ bin_cursor, uint64_t* __restrict__ rec,
                                                   unsigned int* __restrict__ overflow, unsigned long long* __restrict__ routed) {
@@ -487,7 +488,7 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
     for (uint32_t i = threadIdx.x; i < cmax; i += 256) {
       uint32_t bk[RT_U];
 #pragma unroll
-      for (int u = 0; u < RT_U; u++) bk[u] = i < c[u] ? b2b[sk_bucket(entries[(t0 + u) * SK_T + i])] : ~0u;
+      for (int u = 0; u < RT_U; u++) bk[u] = i < c[u] ? b2b[buckets[(t0 + u) * SK_T + i]] : ~0u;
 #pragma unroll
       for (int u = 0; u < RT_U; u++)
         if (bk[u] != ~0u) atomicAdd(&cnt[bk[u]], 1u);
@@ -800,19 +801,19 @@ void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint6
   *chunk_tiles = ct;
   *n_chunks = (nt + ct - 1) / ct;
 }
-void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint32_t* tile_cnt,
-                     uint64_t n_pos, uint64_t chunk_tiles, uint64_t n_chunks, int K, const uint16_t* b2b, int num_bins,
-                     const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, int rw, unsigned int* overflow,
-                     unsigned long long* routed) {
+void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint16_t* buckets,
+                     const uint32_t* tile_cnt, uint64_t n_pos, uint64_t chunk_tiles, uint64_t n_chunks, int K, const uint16_t* b2b,
+                     int num_bins, const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, int rw,
+                     unsigned int* overflow, unsigned long long* routed) {
   if (n_chunks == 0) return;
   const uint64_t nt = p1_tiles(n_pos);
   const size_t lds = (size_t)num_bins * 12;
   if (rw == 2)
-    CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
-                K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
+    CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, buckets, tile_cnt, nt,
+                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
   else
-    CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, tile_cnt, nt, chunk_tiles,
-                K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
+    CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, buckets, tile_cnt, nt,
+                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
 }
 uint64_t partition_scratch_words() { return PT_SCRATCH_WORDS; }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,
