@@ -362,8 +362,10 @@ void do_p1(cf1g_ctx* c) {
   c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m, (uint64_t)c->host_packed}, [&] {
     hipStream_t st = c->st;
     if (!c->host_packed) launch_pack(st, ascii, roff, gst, c->n_rec, c->n_words64, seq, nm);
-    c->zero(hist, CF1G_NUM_BUCKETS * 8);
-    c->zero(tot, 32);
+    if (nt == 0) {  // otherwise k_p1_reduce writes every bucket and both totals
+      c->zero(hist, CF1G_NUM_BUCKETS * 8);
+      c->zero(tot, 32);
+    }
     launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, bkt, tcnt, tw, part, sums, hist, tot);
   });
   c->stage = 2;
