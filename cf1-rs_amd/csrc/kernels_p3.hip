@@ -337,7 +337,7 @@ template <typename KeyT, int RW>
 __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
                                                     const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
                                                     int K, int sb, int pb, uint64_t* __restrict__ counts, uint64_t* __restrict__ mat) {
-  __shared__ uint32_t hist[1 << P3_MAX_SB];
+  extern __shared__ __align__(16) uint32_t hist[];  // 2^sb counters (dynamic)
   int bin;
   uint64_t r0, r1;
   if (!p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
@@ -349,10 +349,19 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
     __shared__ uint8_t own[P3_WG * OWN];
     __shared__ uint16_t oo[P3_WG];
     __shared__ uint64_t sh4[4];
+    __shared__ uint64_t srec[P3_WG * RW];  // the round's records, read once; the k-mer lanes read them here
     const uint64_t mask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
     for (uint64_t rr = r0; rr < r1; rr += P3_WG) {
       const uint64_t r = rr + threadIdx.x;
-      const int n = r < r1 ? rec_kmers_at<RW>(rec, r, K) : 0;
+      int n = 0;
+      if (r < r1) {
+        uint64_t w[RW];
+#pragma unroll
+        for (int t = 0; t < RW; t++) w[t] = rec[r * RW + t];
+#pragma unroll
+        for (int t = 0; t < RW; t++) srec[threadIdx.x * RW + t] = w[t];
+        n = RW == 2 ? rec2_kmers(w[RW - 1], K) : rec4_kmers(w[RW - 1], K);
+      }
       uint64_t T;
       const uint32_t o = (uint32_t)block_excl_scan_u64((uint64_t)n, sh4, &T);
       if (__syncthreads_or(n > OWN)) {
@@ -364,8 +373,16 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
       __syncthreads();
       for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += P3_WG) {
         const uint32_t rl = own[g];
+        const int j = (int)(g - oo[rl]);
+        const uint64_t* w = srec + rl * RW;
         uint32_t m;
-        const KeyT c = rec_occ_at<KeyT, RW, false>(rec, rr + rl, (int)(g - oo[rl]), K, mask, m);
+        KeyT c;
+        if constexpr (RW == 2) {
+          c = (KeyT)rec2_occ<false>(w[0], w[1], j, rec2_kmers(w[1], K), K, mask, m);
+        } else {
+          const Rec4 R{w[0], w[1], w[2], w[3]};
+          c = rec4_occ<false>(R, j, rec4_kmers(R.w3, K), K, m);
+        }
         atomicAdd(&hist[seg_sub(c, sb)], 1u);
       }
       __syncthreads();
@@ -1459,11 +1476,12 @@ void launch_p3_count(hipStream_t st, bool wide, const uint64_t* rec, const uint6
                      int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* counts, uint64_t* mat) {
   if (n_chunks == 0) return;
   const int pb = p3_part_bits(sb);
+  const size_t lds = (size_t)4 << sb;  // the histogram of the bin's 2^sb segments
   if (!wide)
-    CF1G_LAUNCH("p3_count", (k_p3_count<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start, chunk_off,
+    CF1G_LAUNCH("p3_count", (k_p3_count<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), lds, st, rec, bin_start, chunk_off,
                 num_bins, chunk_records, K, sb, pb, counts, mat);
   else
-    CF1G_LAUNCH("p3_count", (k_p3_count<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start, chunk_off,
+    CF1G_LAUNCH("p3_count", (k_p3_count<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), lds, st, rec, bin_start, chunk_off,
                 num_bins, chunk_records, K, sb, pb, counts, mat);
 }
 void launch_p3_scatter(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
