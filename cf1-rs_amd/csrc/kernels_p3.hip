@@ -404,9 +404,12 @@ __global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__
 
 // Tiles staged in LDS and counting-sorted by destination before the stores,
 // so that consecutive lanes write consecutive addresses of one run.
-// (the scatter tile keeps its work-group's LDS under 32 KiB, five per CU;
-// the split tile is a multiple of P3_WG and keeps u64-key work-groups at
-// 24 KiB, six per CU)
+// (with its staged records a u64-key scatter work-group takes 35 KiB, four
+// per CU: 1.78 ms on the 300 Mbp genome, against 1.82 ms at five per CU
+// reading the records from global memory, 1.80 ms with a 1,408-element tile
+// at five per CU; the split tile is a multiple of P3_WG and keeps u64-key
+// work-groups at 24 KiB, six per CU, which measured best against 1,536 and
+// 1,280)
 template <typename KeyT>
 struct StageCfg {
   static constexpr int TILE = sizeof(KeyT) == 8 ? 1920 : 1024;        // k_p3_scatter
@@ -455,6 +458,10 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
   constexpr int OWN = OwnKmers<RW>::N;
   __shared__ uint8_t own[P3_WG * OWN];
   __shared__ uint16_t oo[P3_WG];
+  // the round's records, read once for the k-mer lanes (2-word records only:
+  // with 4-word ones the extra 8 KiB costs a work-group per CU, 3.15 -> 3.27 ms)
+  constexpr bool STAGE = RW == 2;
+  __shared__ uint64_t srec[STAGE ? P3_WG * RW : 1];
   int bin;
   uint64_t r0, r1;
   if (*stop || !p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
@@ -470,7 +477,14 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
     const uint64_t r = rr + threadIdx.x;
     uint32_t n = 0;
     if (r < r1) {
-      int L = (int)(rec[r * RW + RW - 1] & 255);
+      uint64_t w[RW];
+#pragma unroll
+      for (int t = 0; t < RW; t++) w[t] = rec[r * RW + t];
+      if constexpr (STAGE) {
+#pragma unroll
+        for (int t = 0; t < RW; t++) srec[threadIdx.x * RW + t] = w[t];
+      }
+      int L = (int)(w[RW - 1] & 255);
       if (L > rec_cap(RW)) L = rec_cap(RW);
       n = L >= K ? (uint32_t)(L - K + 1) : 0u;
     }
@@ -488,7 +502,14 @@ __global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict
         for (uint32_t t = threadIdx.x; t < tn; t += P3_WG) {
           const uint32_t g = (uint32_t)w0 + t, rl = own[g];
           uint32_t m;
-          const KeyT key = rec_occ_at<KeyT, RW, true>(rec, rr + rl, (int)(g - oo[rl]), K, kmask, m);
+          const int j = (int)(g - oo[rl]);
+          KeyT key;
+          if constexpr (STAGE) {
+            const uint64_t* w = srec + rl * RW;
+            key = (KeyT)rec2_occ<true>(w[0], w[1], j, rec2_kmers(w[1], K), K, kmask, m);
+          } else {
+            key = rec_occ_at<KeyT, RW, true>(rec, rr + rl, j, K, kmask, m);
+          }
           tk[t] = key;
           tm[t] = (uint16_t)m;
           tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
