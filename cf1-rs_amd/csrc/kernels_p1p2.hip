@@ -36,11 +36,57 @@ __device__ __forceinline__ void pack4(uint32_t x, uint32_t& cb, uint32_t& n4) {
   c &= ~(nz * 3u);
   cb = (c * 0x40100401u) >> 24;
 }
-// One lane per 64-position word. A word whose positions are all bases of
-// one record reads its 64 bytes with 17 aligned dword loads (the device copy
-// of the records holds 8 bytes past the last one) and converts them 4 at a
-// time; a word that touches a separator or lies past the last record goes
-// position by position.
+// The 64 bytes at ascii[a ..] as one packed word: 17 aligned dword loads
+// (bytes up to a + 67 are read), 4 bytes at a time.
+__device__ __forceinline__ void pack64(const uint8_t* __restrict__ ascii, uint64_t a, uint64_t& hw, uint64_t& lw, uint64_t& nm) {
+  const uint32_t* d = (const uint32_t*)(ascii + (a & ~3ULL));
+  const uint32_t sh = (uint32_t)(a & 3);
+  uint32_t v[17];
+#pragma unroll
+  for (int i = 0; i < 17; i++) v[i] = d[i];
+  uint32_t cw[4] = {0, 0, 0, 0}, mw[2] = {0, 0};
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    uint32_t cb, n4;
+    pack4(__builtin_amdgcn_alignbyte(v[i + 1], v[i], sh), cb, n4);  // bytes a + 4i ..
+    cw[i >> 2] |= cb << (24 - 8 * (i & 3));
+    mw[i >> 3] |= n4 << (28 - 4 * (i & 7));
+  }
+  hw = ((uint64_t)cw[0] << 32) | cw[1];
+  lw = ((uint64_t)cw[2] << 32) | cw[3];
+  nm = ((uint64_t)mw[0] << 32) | mw[1];
+}
+// Mask of the code bits of positions s .. 63 of a packed word (position i in
+// bits 127-2i .. 126-2i of hw:lw), as (high word, low word).
+__device__ __forceinline__ void tail_mask(int s, uint64_t& mh, uint64_t& ml) {
+  const int cb = 128 - 2 * s;
+  if (cb >= 64) {
+    ml = ~0ULL;
+    mh = cb == 128 ? ~0ULL : ((1ULL << (cb - 64)) - 1);
+  } else {
+    ml = (1ULL << cb) - 1;
+    mh = 0;
+  }
+}
+// A separator (placeholder) inserted at position s: positions s .. 62 move
+// up by one, the last one drops out.
+__device__ __forceinline__ void insert_sep(int s, uint64_t& hw, uint64_t& lw, uint64_t& nm) {
+  uint64_t mh, ml;
+  tail_mask(s, mh, ml);
+  const uint64_t ah = hw & mh, al = lw & ml;
+  hw = (hw & ~mh) | (ah >> 2);
+  lw = (lw & ~ml) | (al >> 2) | (ah << 62);
+  const uint64_t na = s == 0 ? ~0ULL : ((1ULL << (64 - s)) - 1);
+  nm = (nm & ~na) | ((nm & na) >> 1) | (1ULL << (63 - s));
+}
+// One lane per 64-position word. Records lie back to back in `ascii`, so
+// position p holds byte p - (separators before p): a word whose positions
+// are all bases of one record reads its 64 bytes and converts them 4 at a
+// time; a word that holds up to 4 separators converts the bytes from its
+// first position's byte on and inserts a placeholder at each separator, the
+// positions past the stream end becoming placeholders too. The first word,
+// words whose bytes would run past the buffer's last 64 bytes and words with
+// more separators go position by position.
 __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii, const uint64_t* __restrict__ rec_off,
                                               const uint64_t* __restrict__ gstart, uint64_t n_rec, uint64_t n_words64,
                                               uint64_t* __restrict__ seq, uint64_t* __restrict__ nmask) {
@@ -60,25 +106,37 @@ __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii,
   uint64_t r = lo;
   while (r < n_rec && g0 >= gstart[r + 1]) r++;
   uint64_t hw = 0, lw = 0, nm = 0;
+  bool done = false;
   if (r < n_rec && g0 >= gstart[r] && g0 + 64 < gstart[r + 1]) {
-    const uint64_t a = rec_off[r] + (g0 - gstart[r]);
-    const uint32_t* d = (const uint32_t*)(ascii + (a & ~3ULL));
-    const uint32_t sh = (uint32_t)(a & 3);
-    uint32_t v[17];
-#pragma unroll
-    for (int i = 0; i < 17; i++) v[i] = d[i];
-    uint32_t cw[4] = {0, 0, 0, 0}, mw[2] = {0, 0};
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      uint32_t cb, n4;
-      pack4(__builtin_amdgcn_alignbyte(v[i + 1], v[i], sh), cb, n4);  // bytes a + 4i ..
-      cw[i >> 2] |= cb << (24 - 8 * (i & 3));
-      mw[i >> 3] |= n4 << (28 - 4 * (i & 7));
+    pack64(ascii, rec_off[r] + (g0 - gstart[r]), hw, lw, nm);
+    done = true;
+  } else if (r < n_rec && g0 >= gstart[r]) {
+    const uint64_t a = rec_off[r] + (g0 - gstart[r]);  // byte of position g0 (or of the next base, at a separator)
+    int sep[4], k = 0;
+    bool few = true;
+    for (uint64_t q = r + 1; q <= n_rec && gstart[q] - 1 < g0 + 64; q++) {
+      if (k == 4) {
+        few = false;
+        break;
+      }
+      sep[k++] = (int)(gstart[q] - 1 - g0);
     }
-    hw = ((uint64_t)cw[0] << 32) | cw[1];
-    lw = ((uint64_t)cw[2] << 32) | cw[3];
-    nm = ((uint64_t)mw[0] << 32) | mw[1];
-  } else {
+    if (few && a + 64 <= rec_off[n_rec]) {
+      pack64(ascii, a, hw, lw, nm);
+      for (int t = 0; t < k; t++) insert_sep(sep[t], hw, lw, nm);
+      const uint64_t e = gstart[n_rec];  // positions from here on lie past the stream
+      if (g0 + 64 > e) {
+        const int er = (int)(e - g0);
+        uint64_t mh, ml;
+        tail_mask(er, mh, ml);
+        hw &= ~mh;
+        lw &= ~ml;
+        nm |= er == 0 ? ~0ULL : ((1ULL << (64 - er)) - 1);
+      }
+      done = true;
+    }
+  }
+  if (!done) {
     uint64_t g = g0;
     for (int i = 0; i < 64; i++, g++) {
       while (r < n_rec && g >= gstart[r + 1]) r++;
