@@ -731,9 +731,10 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
         if ((alive >> s) & 1) {
           const uint32_t pos = bbh_pos(h1[s] + (uint64_t)l * bbh_d(h1[s]), (uint32_t)n);
           pp[s] = pos;
-          const unsigned long long bit = 1ULL << (pos & 63);
-          const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
-          if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+          // 32-bit views of the bitmaps: bit pos is bit pos & 31 of word pos >> 5
+          const uint32_t bit = 1u << (pos & 31);
+          const uint32_t old = atomicOr((uint32_t*)seen + (pos >> 5), bit);
+          if (old & bit) atomicOr((uint32_t*)col + (pos >> 5), bit);
         }
       }
     };
@@ -751,7 +752,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
       if (s >= ns) break;
       if ((alive >> s) & 1) {
         const uint32_t pos = pp[s];
-        if (((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1) {
+        if (((((const uint32_t*)seen)[pos >> 5] & ~((const uint32_t*)col)[pos >> 5]) >> (pos & 31)) & 1) {
           alive &= ~(1u << s);
           ppos[s * WG + threadIdx.x] = (uint16_t)(off + pos);
         }
@@ -809,9 +810,9 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
       }
       for (uint32_t i = lane; i < cnt; i += 64) {
         const uint32_t pos = bbh_pos(L.h[i], (uint32_t)n);
-        const unsigned long long bit = 1ULL << (pos & 63);
-        const unsigned long long old = atomicOr((unsigned long long*)&seen[pos >> 6], bit);
-        if (old & bit) atomicOr((unsigned long long*)&col[pos >> 6], bit);
+        const uint32_t bit = 1u << (pos & 31);
+        const uint32_t old = atomicOr((uint32_t*)seen + (pos >> 5), bit);
+        if (old & bit) atomicOr((uint32_t*)col + (pos >> 5), bit);
       }
       wave_lds_order();
       const uint32_t nw = (uint32_t)(n >> 6);
@@ -831,7 +832,7 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
           dd = L.d[i];
           ee = L.e[i];
           pos = bbh_pos(hh, (uint32_t)n);
-          placed = ((seen[pos >> 6] & ~col[pos >> 6]) >> (pos & 63)) & 1;
+          placed = ((((const uint32_t*)seen)[pos >> 5] & ~((const uint32_t*)col)[pos >> 5]) >> (pos & 31)) & 1;
         }
         if (placed) ppos[ee] = (uint16_t)(off + pos);
         const bool keep = act && !placed;
