@@ -879,8 +879,9 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
 // VGPRs and 8 x 20 KiB of LDS fill a CU; 3.99 -> 3.60 ms on the 300 Mbp
 // genome against 7 per CU at 70 VGPRs); 6 with 16-byte keys (77 VGPRs;
 // 5.22 -> 4.74 ms at k = 63 against the compiler's 5 per CU, and 7 per CU
-// spills). Both compile without spills; a spill of 80 bytes per lane cost
-// 15% of the kernel's time.
+// spills). The u64 build spills 8 bytes per lane since the level bitmaps are
+// addressed as 32-bit words (which saved more than the spill costs); a
+// spill of 80 bytes per lane cost 15% of the kernel's time.
 template <typename KeyT>
 struct DdOccupancy {
   static constexpr int WGS_PER_CU = sizeof(KeyT) == 8 ? 8 : 6;
