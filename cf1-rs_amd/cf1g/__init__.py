@@ -34,6 +34,7 @@ EXPORTS = [
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_unitig_spans", "cf1g_record_starts",
+    "cf1g_set_record_files", "cf1g_record_files",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_graphs", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
     "cf1g_lidx_gradients", "cf1g_classify_via", "cf1g_select_index", "cf1g_selected_index",
@@ -151,6 +152,8 @@ def lib():
         L.cf1g_select_index.argtypes = [C.c_void_p, C.c_int]
         L.cf1g_selected_index.argtypes = [C.c_void_p]
         L.cf1g_record_starts.argtypes = [C.c_void_p, U64P, C.c_uint64]
+        L.cf1g_set_record_files.argtypes = [C.c_void_p, U32P, C.c_uint64]
+        L.cf1g_record_files.argtypes = [C.c_void_p, U32P, C.c_uint64]
         L.cf1g_bins_export.argtypes = [C.c_void_p, U32P, U64P, U64P, C.c_uint64]
         L.cf1g_p5_write.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64, C.c_char_p, U64P, C.c_char_p, C.c_char_p,
                                     C.c_int, C.c_int, C.c_int, U64P]
@@ -396,6 +399,17 @@ class Cf1g:
         out = np.zeros(n, np.uint64)
         self._check(self.L.cf1g_record_starts(self.h, _p(out, U64P), n))
         return out
+
+    def set_record_files(self, file_idx):
+        """Input file (0-based) of each loaded record: cf_seq's Reference:<file+1>."""
+        f = np.ascontiguousarray(file_idx, np.uint32)
+        self._check(self.L.cf1g_set_record_files(self.h, _p(f, U32P), len(f)))
+
+    def record_files(self):
+        n = self.info()["n_records"]
+        out = np.zeros(max(n, 1), np.uint32)
+        self._check(self.L.cf1g_record_files(self.h, _p(out, U32P), max(n, 1)))
+        return out[:n]
 
     def bins_export(self):
         inf = self.info()

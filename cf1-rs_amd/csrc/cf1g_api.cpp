@@ -27,7 +27,8 @@ uint64_t p1_tiles(uint64_t);
 uint64_t p1_tile_positions();
 uint64_t p1_slices(uint64_t);
 void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint16_t*, uint32_t*, uint32_t*, uint32_t*,
-               uint64_t*, unsigned long long*, unsigned long long*);
+               uint64_t*, unsigned long long*, unsigned long long*, unsigned long long*, const uint64_t*, uint32_t, uint32_t*,
+               uint16_t*, uint32_t, unsigned int*);
 void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
 uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
@@ -58,6 +59,7 @@ void launch_p3_compact(hipStream_t, bool, const uint16_t*, const uint64_t*, cons
 void launch_p3_keys(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
                     uint64_t, void*, unsigned int*);
 void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
+void launch_stream_digest(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, unsigned long long*);
 void launch_p4_cas(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, uint64_t, uint64_t*, uint64_t*,
                    uint32_t, uint32_t, int);
 uint64_t sort_tiles(uint64_t);
@@ -79,8 +81,9 @@ void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, co
 void launch_states_pack(hipStream_t, const uint8_t*, uint64_t, uint64_t*);
 void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&,
                           const uint64_t*, uint8_t*);
-void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint8_t*);
-void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*);
+void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint64_t,
+                     uint8_t*, unsigned int*);
+void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*, bool);
 uint64_t span_tiles(uint64_t);
 void launch_span_count(hipStream_t, const uint8_t*, uint64_t, uint64_t*, uint64_t*);
 void launch_span_emit(hipStream_t, const uint8_t*, uint64_t, const uint64_t*, const uint64_t*, uint64_t*, uint64_t*);
@@ -170,6 +173,11 @@ struct cf1g_ctx {
   // sizes
   uint64_t n_rec = 0, n_ascii = 0, n_pos = 0, n_words64 = 0;
   std::vector<uint64_t> gstart_h;
+  // chunk edges of records of CHUNK_BASES or more (P1 / P2 scan them in
+  // chunks, src/minimizer.rs:57-86, src/superkmer.rs:57-88)
+  std::vector<uint64_t> edges_h;
+  // input file of each record (cf_seq's Reference:<file+1>, src/output.rs:176-177)
+  std::vector<uint32_t> rec_file;
   uint64_t W = 0, S = 0, D = 0;
   uint32_t num_bins = 0;
   int sb = 0;
@@ -359,14 +367,23 @@ void do_p1(cf1g_ctx* c) {
   uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
-  c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m, (uint64_t)c->host_packed}, [&] {
+  auto* hist_all = c->getT<unsigned long long>("hist_all", CF1G_NUM_BUCKETS);
+  const uint32_t n_edges = (uint32_t)(c->edges_h.size() - 1);
+  const uint64_t* edges = c->getT<uint64_t>("edges", c->edges_h.size());
+  const uint32_t excl_cap = n_edges * 64 + 1;
+  uint32_t* excl_cnt = c->getT<uint32_t>("p1_excl_cnt", 2);  // [0] count, [1] overflow flag
+  uint16_t* excl = c->getT<uint16_t>("p1_excl", excl_cap);
+  c->seg("p1", {c->n_rec, c->n_pos, c->n_words64, (uint64_t)c->k, (uint64_t)c->m, (uint64_t)c->host_packed, n_edges}, [&] {
     hipStream_t st = c->st;
     if (!c->host_packed) launch_pack(st, ascii, roff, gst, c->n_rec, c->n_words64, seq, nm);
+    c->zero(excl_cnt, 8);
     if (nt == 0) {  // otherwise k_p1_reduce writes every bucket and both totals
       c->zero(hist, CF1G_NUM_BUCKETS * 8);
+      c->zero(hist_all, CF1G_NUM_BUCKETS * 8);
       c->zero(tot, 32);
     }
-    launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, bkt, tcnt, tw, part, sums, hist, tot);
+    launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, bkt, tcnt, tw, part, sums, hist, hist_all, tot, edges, n_edges, excl_cnt, excl,
+              excl_cap, excl_cnt + 1);
   });
   c->stage = 2;
 }
@@ -374,8 +391,8 @@ void do_p1(cf1g_ctx* c) {
 void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   hipStream_t st = c->st;
   if (num_bins < 1 || num_bins > CF1G_MAX_BINS) fail(CF1G_E_INVALID, "num_bins must be in [1, 4096]");
-  auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
-  auto* hpart = hist;
+  auto* hpart = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);     // P1's histogram: what the partition splits
+  auto* hist = c->getT<unsigned long long>("hist_all", CF1G_NUM_BUCKETS);  // every routed super k-mer: the bin sizes
   if (hist_in) {
     hpart = c->getT<unsigned long long>("hist_in", CF1G_NUM_BUCKETS);
     hip_check(hipMemcpyAsync(hpart, hist_in, CF1G_NUM_BUCKETS * 8, hipMemcpyHostToDevice, st), "H2D hist");
@@ -392,9 +409,12 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
     launch_partition(c->st, hpart, hist, num_bins, CHUNK_RECORDS, b2b, brec, bstart, bcur, coff, info, pscr);
   });
   uint64_t hinfo[4], htot[4];
+  uint32_t hexcl[2];
   hip_check(hipMemcpyAsync(hinfo, info, 32, hipMemcpyDeviceToHost, st), "D2H info");
   hip_check(hipMemcpyAsync(htot, totals, 32, hipMemcpyDeviceToHost, st), "D2H totals");
+  hip_check(hipMemcpyAsync(hexcl, c->getT<uint32_t>("p1_excl_cnt", 2), 8, hipMemcpyDeviceToHost, st), "D2H P1 edge list");
   c->sync();
+  if (hexcl[1]) fail(CF1G_E_INTERNAL, "P1 chunk-edge list overflowed");
   c->num_bins = (uint32_t)hinfo[0];
   c->S = hinfo[1];
   c->n_chunks = hinfo[2];
@@ -880,10 +900,16 @@ uint8_t* annotate_device(cf1g_ctx* c) {
   const uint64_t nsw = (c->D + 11) / 12;
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
-  if (c->use_index == 1)
+  if (c->use_index == 1) {
     launch_annotate_lidx(c->st, seq, nm, c->n_pos, c->k, c->lview(), c->view(), c->getT<uint64_t>("states_lidx", nsw + 1), d);
-  else
-    launch_annotate(c->st, c->wide, seq, nm, c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), d);
+  } else {
+    unsigned int* err = c->getT<unsigned int>("annot_err", 2);
+    c->zero(err, 8);
+    launch_annotate(c->st, c->wide, seq, nm, c->n_pos, c->k, c->view(), c->getT<uint64_t>("states", nsw + 1), c->D, d, err);
+    unsigned int he = 0;
+    c->d2h(&he, err, 4);
+    if (he) fail(CF1G_E_INTERNAL, "a k-mer of the loaded records is missing from the MPHF (a restored file of other records?)");
+  }
   return d;
 }
 
@@ -921,7 +947,7 @@ uint64_t build_spans(cf1g_ctx* c) {
   launch_span_emit(st, ann, c->n_pos, os, oe, S, E);
   launch_span_keys(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->k, S, E, U, keys, fl);
   if (c->use_index == 1) launch_lidx_lookup(st, (const uint64_t*)keys, U, c->lview(), c->view(), ids);
-  else launch_lookup(st, c->wide, keys, U, c->view(), ids);
+  else launch_lookup(st, c->wide, keys, U, c->view(), ids, true);
   if (nsw)
     hip_check(hipMemcpyAsync(claim, c->get(c->use_index == 1 ? "states_lidx" : "states", (nsw + 1) * 8), nsw * 8,
                              hipMemcpyDeviceToDevice, st),
@@ -1016,6 +1042,21 @@ int cf1g_get_info(cf1g_ctx* c, cf1g_info* o) {
   });
 }
 
+// Chunk edges (stream positions) of the loaded records, plus a UINT64_MAX
+// sentinel, into edges_h and the device buffer "edges".
+void set_chunk_edges(cf1g_ctx* c) {
+  constexpr uint64_t CHUNK_BASES = 4000000;  // src/minimizer.rs:13, src/superkmer.rs:16
+  c->edges_h.clear();
+  for (uint64_t r = 0; r < c->n_rec; r++) {
+    const uint64_t len = c->gstart_h[r + 1] - 1 - c->gstart_h[r];
+    for (uint64_t a = CHUNK_BASES; a < len; a += CHUNK_BASES) c->edges_h.push_back(c->gstart_h[r] + a);
+  }
+  c->edges_h.push_back(UINT64_MAX);
+  hip_check(hipMemcpyAsync(c->getT<uint64_t>("edges", c->edges_h.size()), c->edges_h.data(), c->edges_h.size() * 8,
+                           hipMemcpyHostToDevice, c->st),
+            "H2D chunk edges");
+}
+
 int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec) {
   return guarded(c, [&]() {
     if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
@@ -1035,6 +1076,7 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
     c->n_pos = c->n_ascii + n_rec + 1;
     c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
     c->host_records = c->host_packed = c->restored = false;
+    c->rec_file.assign(n_rec, 0);
     uint8_t* da = c->getT<uint8_t>("ascii", c->n_ascii + 8);
     if (c->n_ascii) hip_check(hipMemcpyAsync(da, ascii + base, c->n_ascii, hipMemcpyHostToDevice, c->st), "H2D ascii");
     hip_check(hipMemcpyAsync(c->getT<uint64_t>("rec_off", n_rec + 1), off.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice, c->st),
@@ -1042,6 +1084,7 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
     hip_check(hipMemcpyAsync(c->getT<uint64_t>("gstart", n_rec + 1), c->gstart_h.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice,
                              c->st),
               "H2D gstart");
+    set_chunk_edges(c);
     c->sync();
     c->W = c->S = c->D = 0;
     c->stage = 1;
@@ -1069,6 +1112,7 @@ int cf1g_load_fasta(cf1g_ctx* c, const char* const* paths, uint32_t n_paths, uin
     const HostRecords& R = c->host;
     const uint64_t n_rec = R.off.size() - 1;
     c->n_rec = n_rec;
+    c->rec_file = R.file;
     c->n_ascii = R.off[n_rec];
     c->gstart_h.assign(n_rec + 1, 0);
     for (uint64_t r = 0; r <= n_rec; r++) c->gstart_h[r] = R.off[r] + r + 1;
@@ -1091,6 +1135,7 @@ int cf1g_load_fasta(cf1g_ctx* c, const char* const* paths, uint32_t n_paths, uin
     uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
     hip_check(hipMemcpyAsync(seq, hseq, 2 * c->n_words64 * 8, hipMemcpyHostToDevice, c->st), "H2D stream");
     hip_check(hipMemcpyAsync(nm, hnm, c->n_words64 * 8, hipMemcpyHostToDevice, c->st), "H2D placeholder bits");
+    set_chunk_edges(c);
     c->sync();
     auto t3 = std::chrono::steady_clock::now();
     c->host_records = c->host_packed = true;
@@ -1107,6 +1152,22 @@ int cf1g_load_fasta(cf1g_ctx* c, const char* const* paths, uint32_t n_paths, uin
       o.h2d_ms = ms(t2, t3);
       *out = o;
     }
+  });
+}
+
+int cf1g_set_record_files(cf1g_ctx* c, const uint32_t* file_idx, uint64_t n_rec) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_set_record_files");
+    if (n_rec != c->n_rec || (n_rec && !file_idx)) fail(CF1G_E_INVALID, "one file index per loaded record");
+    c->rec_file.assign(file_idx, file_idx + n_rec);
+  });
+}
+
+int cf1g_record_files(cf1g_ctx* c, uint32_t* out, uint64_t n) {
+  return guarded(c, [&]() {
+    need(c, 1, "cf1g_record_files");
+    if (n < c->n_rec || (c->n_rec && !out)) fail(CF1G_E_INVALID, "output too small");
+    for (uint64_t r = 0; r < c->n_rec; r++) out[r] = r < c->rec_file.size() ? c->rec_file[r] : 0;
   });
 }
 
@@ -1284,6 +1345,46 @@ int cf1g_dedup_write(cf1g_ctx* c, const char* path, uint32_t threads) {
 namespace {
 constexpr uint64_t SAVE_MAGIC = 0x3176617367316663ULL;  // "cf1gsav1"
 constexpr uint64_t SAVE_WORDS = 16;                      // header words
+constexpr uint64_t SAVE_VERSION = 2;
+// Digest of a run of words (the save body): sum mod 2^64 of
+// mix64(word ^ index * golden), as k_stream_digest computes it.
+uint64_t words_digest(const uint64_t* w, uint64_t n) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; i++) acc += mix64(w[i] ^ (i * 0x9E3779B97F4A7C15ULL));
+  return acc;
+}
+// Digest of the context's packed stream (packed by P1 for ASCII records).
+uint64_t stream_digest(cf1g_ctx* c) {
+  auto* d = c->getT<unsigned long long>("stream_digest", 1);
+  c->zero(d, 8);
+  launch_stream_digest(c->st, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8),
+                       c->n_words64, d);
+  uint64_t h = 0;
+  c->d2h(&h, d, 8);
+  return h;
+}
+// The MPHF words of a save file, checked before anything reaches the device:
+// the layout sizes (bounded before any arithmetic), bins of the
+// bucket->bin table, and monotone key and block offsets ending at D and the
+// block count (a lookup reads only its segment's blocks).
+void check_mphf_words(const uint64_t* m, uint64_t mw, int k, int mm, bool wide) {
+  auto bad = [](const char* why) { fail(CF1G_E_INVALID, std::string("the save file's MPHF is not valid: ") + why); };
+  const uint64_t tab = CF1G_NUM_BUCKETS / 4;
+  if (mw < 16 + tab) bad("too short");
+  if (m[0] != 0x6366316D70686631ULL || m[1] != (uint64_t)k || m[2] != (uint64_t)mm || m[8] != (wide ? 1u : 0u))
+    fail(CF1G_E_INVALID, "the save file's MPHF does not match this context (k, m or layout)");
+  const uint64_t sb = m[3], bins = m[4], ns = m[5], D = m[6], nblk = m[7];
+  if (sb > 12 || bins < 1 || bins > CF1G_MAX_BINS || ns != (bins << sb) || D >= (1ULL << 40) || nblk >= (1ULL << 40)) bad("header");
+  if (mw != 16 + tab + 2 * (ns + 1) + nblk * 8) bad("size");
+  const uint16_t* b2b = (const uint16_t*)(m + 16);
+  for (uint64_t b = 0; b < CF1G_NUM_BUCKETS; b++)
+    if (b2b[b] >= bins) bad("bucket->bin table");
+  const uint64_t* ko = m + 16 + tab;
+  const uint64_t* bo = ko + ns + 1;
+  if (ko[0] != 0 || bo[0] != 0 || ko[ns] != D || bo[ns] != nblk) bad("offsets");
+  for (uint64_t s = 0; s < ns; s++)
+    if (ko[s + 1] < ko[s] || bo[s + 1] < bo[s]) bad("offsets");
+}
 }  // namespace
 
 int cf1g_save(cf1g_ctx* c, const char* path) {
@@ -1295,15 +1396,17 @@ int cf1g_save(cf1g_ctx* c, const char* path) {
     const uint64_t nsw = (c->D + 11) / 12;
     std::vector<uint64_t> buf(SAVE_WORDS + mw + nsw, 0);
     buf[0] = SAVE_MAGIC;
-    buf[1] = 1;  // version
+    buf[1] = SAVE_VERSION;
     buf[2] = c->n_rec;
     buf[3] = c->n_pos;
     buf[4] = c->W;
     buf[5] = c->S;
     buf[6] = mw;
     buf[7] = nsw;
+    buf[8] = stream_digest(c);
     if (cf1g_mphf_export(c, buf.data() + SAVE_WORDS, &mw) != CF1G_OK) fail(CF1G_E_INTERNAL, c->err);
     if (nsw) c->d2h(buf.data() + SAVE_WORDS + mw, c->get("states", (nsw + 1) * 8), nsw * 8);
+    buf[9] = words_digest(buf.data() + SAVE_WORDS, mw + nsw);
     FILE* f = std::fopen(path, "wb");
     if (!f) fail(CF1G_E_INVALID, std::string("cannot create ") + path);
     const size_t n = buf.size() * 8;
@@ -1320,7 +1423,7 @@ int cf1g_restore(cf1g_ctx* c, const char* path) {
     if (!f) fail(CF1G_E_INVALID, std::string("cannot open ") + path);
     std::vector<uint64_t> buf;
     uint64_t hdr[SAVE_WORDS];
-    bool ok = std::fread(hdr, 8, SAVE_WORDS, f) == SAVE_WORDS && hdr[0] == SAVE_MAGIC && hdr[1] == 1;
+    bool ok = std::fread(hdr, 8, SAVE_WORDS, f) == SAVE_WORDS && hdr[0] == SAVE_MAGIC && hdr[1] == SAVE_VERSION;
     if (ok) {  // the header's sizes must match the file before anything is allocated
       const long here = std::ftell(f);
       ok = std::fseek(f, 0, SEEK_END) == 0;
@@ -1334,13 +1437,19 @@ int cf1g_restore(cf1g_ctx* c, const char* path) {
     }
     std::fclose(f);
     if (!ok) fail(CF1G_E_INVALID, std::string("not a complete cf1g save file: ") + path);
+    if (words_digest(buf.data(), buf.size()) != hdr[9]) fail(CF1G_E_INVALID, "the save file is damaged (body checksum)");
     if (hdr[2] != c->n_rec || hdr[3] != c->n_pos) fail(CF1G_E_INVALID, "the save file belongs to other records (count or length differ)");
     const uint64_t* m = buf.data();
     const uint64_t mw = hdr[6], nsw = hdr[7];
+    check_mphf_words(m, mw, c->k, c->m, c->wide);
     const uint64_t ns = m[5], D = m[6], nblk = m[7], tab = CF1G_NUM_BUCKETS / 4;
-    if (mw < 16 || m[0] != 0x6366316D70686631ULL || m[1] != (uint64_t)c->k || m[2] != (uint64_t)c->m || m[8] != (c->wide ? 1u : 0u) ||
-        mw != 16 + tab + 2 * (ns + 1) + nblk * 8 || nsw != (D + 11) / 12 || (ns >> m[3]) != m[4])
-      fail(CF1G_E_INVALID, "the save file's MPHF does not match this context (k, m or layout)");
+    if (nsw != (D + 11) / 12) fail(CF1G_E_INVALID, "the save file's state vector does not match its MPHF");
+    if (!c->host_packed) {  // ASCII records: the stream annotation and P5 read is packed in P1
+      launch_pack(c->st, c->getT<uint8_t>("ascii", c->n_ascii + 8), c->getT<uint64_t>("rec_off", c->n_rec + 1),
+                  c->getT<uint64_t>("gstart", c->n_rec + 1), c->n_rec, c->n_words64, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8),
+                  c->getT<uint64_t>("nmask", c->n_words64 + 8));
+    }
+    if (stream_digest(c) != hdr[8]) fail(CF1G_E_INVALID, "the save file belongs to other records (stream digest differs)");
     c->sb = (int)m[3];
     c->num_bins = (uint32_t)m[4];
     c->n_segs = ns;
@@ -1355,12 +1464,6 @@ int cf1g_restore(cf1g_ctx* c, const char* path) {
     if (nblk) c->h2d((void*)v.blocks, m + 16 + tab + 2 * (ns + 1), nblk * 64);
     uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
     if (nsw) c->h2d(states, m + mw, nsw * 8);
-    if (!c->host_packed) {  // ASCII records: the stream annotation and P5 read is packed in P1
-      launch_pack(c->st, c->getT<uint8_t>("ascii", c->n_ascii + 8), c->getT<uint64_t>("rec_off", c->n_rec + 1),
-                  c->getT<uint64_t>("gstart", c->n_rec + 1), c->n_rec, c->n_words64, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8),
-                  c->getT<uint64_t>("nmask", c->n_words64 + 8));
-      c->sync();
-    }
     c->keys_ready = c->lidx_ready = c->lidx_states = false;
     c->use_index = 0;
     c->restored = true;
@@ -1377,7 +1480,7 @@ int cf1g_mphf_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* ou
     void* dk = c->get("lookup_keys", n * kb);
     uint64_t* dout = c->getT<uint64_t>("lookup_out", n);
     hip_check(hipMemcpyAsync(dk, keys, n * kb, hipMemcpyHostToDevice, c->st), "H2D keys");
-    launch_lookup(c->st, c->wide, dk, n, c->view(), dout);
+    launch_lookup(c->st, c->wide, dk, n, c->view(), dout, false);
     c->d2h(out, dout, n * 8);
   });
 }
@@ -1385,7 +1488,7 @@ int cf1g_mphf_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* ou
 int cf1g_mphf_lookup_device(cf1g_ctx* c, const void* d_keys, uint64_t n, uint64_t* d_out) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_mphf_lookup_device");
-    launch_lookup(c->st, c->wide, d_keys, n, c->view(), d_out);
+    launch_lookup(c->st, c->wide, d_keys, n, c->view(), d_out, false);
     c->sync();
   });
 }

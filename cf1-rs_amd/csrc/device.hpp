@@ -237,19 +237,26 @@ __device__ __forceinline__ uint64_t mphf_local_rank(const uint64_t* __restrict__
   }
   return ~0ULL;
 }
-// Exact value of a present key; for an absent key, some value (0 if it
-// misses every level — the reference maps absent keys to 0, mphf.rs:269).
+// Exact value of a present key; for an absent key, some value, or ~0 if it
+// misses every level (callers that must notice a missing key test >= D).
+// Levels are read only while they lie inside the segment's blocks: a segment
+// stores the levels its build needed (until every key was placed), and the
+// level sizes never reach 0, so an absent key would otherwise walk on into
+// the next segment's blocks or past the array.
 template <typename KeyT>
-__device__ __forceinline__ uint64_t mphf_lookup(const MphfView& v, const KeyT& c) {
+__device__ __forceinline__ uint64_t mphf_find(const MphfView& v, const KeyT& c) {
   const uint64_t seg = mphf_segment(v, c);
   const uint64_t ko = v.key_off[seg];
   uint64_t t = v.key_off[seg + 1] - ko;
-  const uint64_t* base = v.blocks + v.blk_off[seg] * 8;
+  const uint64_t b0 = v.blk_off[seg];
+  const uint64_t lim = (v.blk_off[seg + 1] - b0) * BBH_BLOCK_BITS;
+  const uint64_t* base = v.blocks + b0 * 8;
   uint64_t off = 0;
   uint64_t h = bbh_h1(c);
   const uint64_t d = bbh_d(h);
   for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++, h += d) {
     const uint64_t n = bbh_level_n(t);
+    if (off + n > lim) break;
     const uint64_t pos = off + bbh_pos(h, (uint32_t)n);
     const uint64_t q = pos / BBH_BLOCK_BITS;
     const uint32_t b = (uint32_t)(pos - q * BBH_BLOCK_BITS);
@@ -264,7 +271,14 @@ __device__ __forceinline__ uint64_t mphf_lookup(const MphfView& v, const KeyT& c
     off += n;
     t = bbh_next_t(t);
   }
-  return 0;
+  return ~0ULL;
+}
+// Mphf::hash (mphf.rs:268-270): get(&key).unwrap_or(0) — a key that misses
+// every level maps to 0.
+template <typename KeyT>
+__device__ __forceinline__ uint64_t mphf_lookup(const MphfView& v, const KeyT& c) {
+  const uint64_t r = mphf_find(v, c);
+  return r == ~0ULL ? 0 : r;
 }
 
 // ------------------------------------------------------------ DFA (state.rs / classify.rs)

@@ -98,6 +98,8 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
   if (U && (rc = cf1g_unitig_spans(ctx, &U, S.data(), E.data(), ID.data(), FL.data(), U + 1))) return rc;
   std::vector<uint64_t> gstart(n_rec + 1);
   if ((rc = cf1g_record_starts(ctx, gstart.data(), gstart.size()))) return rc;
+  std::vector<uint32_t> file(n_rec + 1, 0);
+  if ((rc = cf1g_record_files(ctx, file.data(), file.size()))) return rc;
   const uint64_t base = rec_off[0];
   auto name_of = [&](uint64_t r) -> std::string {
     if (!names_blob || !name_off) return std::string();
@@ -175,7 +177,8 @@ extern "C" int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t
   for (uint64_t r = 0; r < n_rec; r++) {
     // records shorter than k have no tiling line (src/output.rs:166-206)
     const bool present = rec_off[r + 1] - rec_off[r] >= K;
-    std::string head = "Reference:1_Sequence:" + name_of(r) + "\t";
+    // ref_id = file_idx + 1 (src/output.rs:176-177, 214-217)
+    std::string head = "Reference:" + std::to_string((uint64_t)file[r] + 1) + "_Sequence:" + name_of(r) + "\t";
     if (present) std::fwrite(head.data(), 1, head.size(), fq);
     for (; pi < pieces.size() && pieces[pi].r == r; pi++) {
       Piece& P = pieces[pi];

@@ -147,7 +147,9 @@ void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords
   R.off.assign(1, 0);
   R.names.clear();
   R.name_off.assign(1, 0);
-  for (const std::string& path : paths) {
+  R.file.clear();
+  for (size_t file_idx = 0; file_idx < paths.size(); file_idx++) {
+    const std::string& path = paths[file_idx];
     FileBytes F(path);
     const uint8_t* p = F.p;
     std::vector<size_t> hs = headers(F, threads);
@@ -204,6 +206,7 @@ void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords
       R.off.push_back(at[r + 1]);
       R.names.append((const char*)p + recs[r].name_b, recs[r].name_e - recs[r].name_b);
       R.name_off.push_back(R.names.size());
+      R.file.push_back((uint32_t)file_idx);
     }
   }
 }

@@ -37,6 +37,7 @@ struct HostRecords {
   std::vector<uint64_t> off{0};
   std::string names;
   std::vector<uint64_t> name_off{0};
+  std::vector<uint32_t> file;  // file[r] = index of the input file record r came from
 };
 
 // Parse FASTA files (plain or .gz) in order into R; throws std::runtime_error.

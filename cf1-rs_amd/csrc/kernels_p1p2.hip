@@ -168,6 +168,17 @@ __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii,
 // k-mer starting in the tile into the tile's slot of the entry array:
 //   entry = start position << 24 | length (k-mers) << 16 | bucket.
 // tile_cnt[tile] = entries of the tile, tile_w[tile] = its valid k-mers.
+//
+// Chunk edges: the reference scans a record of CHUNK_BASES (4,000,000) bases
+// or more in chunks that start every CHUNK_BASES bases (minimizer.rs:57-86,
+// superkmer.rs:57-88), each as a sequence of its own. P2's chunks overlap by
+// K-1 bases, so a super k-mer always starts at a chunk's first k-mer: the
+// scan breaks runs at each edge (record start + j * CHUNK_BASES, j >= 1).
+// P1's chunks do not overlap, so the k-mers across an edge are not counted:
+// a super k-mer that starts less than K positions before an edge (all its
+// k-mers span it) is routed but left out of the histogram; its bucket goes
+// to the `excl` list, which k_p1_excl takes off the counts. `edges` is
+// sorted and ends with a UINT64_MAX sentinel.
 constexpr int SK_WG = 256;
 constexpr int SK_T = 2048;  // k-mer positions per tile
 constexpr int SK_Q = 8;     // consecutive m-mers / positions per thread step
@@ -194,6 +205,7 @@ struct SkTile {
   uint16_t pidx[SK_NPP];
   uint64_t brk[SK_BRK / 8];
   uint64_t sh[4];
+  uint64_t edge;  // first chunk edge at or after the tile's first k-mer position
 };
 static_assert(2 * SK_T <= 4 * SK_NJP, "the start list of a tile (u16 slots) fits in SkTile::ord");
 
@@ -228,7 +240,9 @@ __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b
 __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                    uint64_t n_pos, uint64_t n_tiles, int K, int m,
                                                    uint64_t* __restrict__ entries, uint16_t* __restrict__ buckets,
-                                                   uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ tile_w) {
+                                                   uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ tile_w,
+                                                   const uint64_t* __restrict__ edges, uint32_t n_edges,
+                                                   uint32_t* __restrict__ excl_cnt, uint16_t* __restrict__ excl, uint32_t excl_cap) {
   __shared__ SkTile S;
   const int w = K - m + 1;
   const uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
@@ -236,6 +250,14 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const uint64_t t0 = 1 + tile * SK_T;  // first k-mer position of the tile
     const uint64_t B = t0 - 1;            // tile slot 0 <-> stream position B
+    if (threadIdx.x == 0) {  // first edge >= t0 (edges are >= 4M apart: at most one in the tile's reach)
+      uint32_t lo = 0, hi = n_edges;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (edges[mid] < t0) lo = mid + 1; else hi = mid;
+      }
+      S.edge = edges[lo];
+    }
     // (1) m-mer orders of slots [0, T + 2w)
     const int nj = SK_T + 2 * w;
     for (int j0 = threadIdx.x * SK_Q; j0 < nj; j0 += SK_WG * SK_Q) {
@@ -334,7 +356,9 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
     }
     __syncthreads();
     // (3) break flags of slots [1, T + w]: slot j breaks a run unless it and
-    // slot j-1 are valid with the same minimizer slot. Byte j/8, bit j%8.
+    // slot j-1 are valid with the same minimizer slot, or a chunk edge lies
+    // at slot j (slot jE). Byte j/8, bit j%8.
+    const uint64_t jE = S.edge - B;
     for (int j0 = threadIdx.x * SK_Q; j0 < np; j0 += SK_WG * SK_Q) {
       uint32_t bits = 0;
 #pragma unroll
@@ -343,7 +367,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
         bool br = true;
         if (j >= 1 && j < np) {
           const uint32_t a = S.pidx[PJ(j)], b = S.pidx[PJ(j - 1)];
-          br = (a & SK_INVALID) || a != b;  // unequal when either is invalid and the other not
+          br = (a & SK_INVALID) || a != b || (uint64_t)j == jE;  // unequal when either is invalid and the other not
         }
         bits |= (uint32_t)br << q;
       }
@@ -392,6 +416,10 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
       const uint32_t bk = bucket_hash((uint64_t)(f < r ? f : r));
       dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bk);
       buckets[tile * SK_T + i] = (uint16_t)bk;  // for the histogram, which needs only this
+      if ((uint64_t)j < jE && jE - (uint64_t)j < (uint64_t)K) {  // every k-mer spans the edge: not in P1's counts
+        const uint32_t x = atomicAdd(excl_cnt, 1u);
+        if (x < excl_cap) excl[x] = (uint16_t)bk;
+      }
     }
     if (threadIdx.x == 0) {
       tile_cnt[tile] = (uint32_t)tot;
@@ -466,6 +494,7 @@ __global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint16_t* __restrict__ 
 // threads per bucket take every fourth slice; LDS adds their sums.
 __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ part, const uint64_t* __restrict__ sums,
                                                    uint32_t n_slices, unsigned long long* __restrict__ hist,
+                                                   unsigned long long* __restrict__ hist_all,
                                                    unsigned long long* __restrict__ totals) {
   __shared__ uint64_t acc4[4][64];
   const uint32_t g = threadIdx.x >> 6, bl = __lane_id();
@@ -482,7 +511,7 @@ __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ 
   for (; sl < n_slices; sl += 4) acc += part[(uint64_t)sl * 65536 + b];
   acc4[g][bl] = acc;
   __syncthreads();
-  if (g == 0) hist[b] = acc4[0][bl] + acc4[1][bl] + acc4[2][bl] + acc4[3][bl];
+  if (g == 0) hist[b] = hist_all[b] = acc4[0][bl] + acc4[1][bl] + acc4[2][bl] + acc4[3][bl];
   if (blockIdx.x == 0) {  // totals {W, S}: every thread of the block adds a stripe of the slices
     __syncthreads();
     uint64_t t = 0;
@@ -496,6 +525,46 @@ __global__ void __launch_bounds__(256) k_p1_reduce(const uint32_t* __restrict__ 
     }
     if (threadIdx.x < 2) totals[threadIdx.x] = red[threadIdx.x];
   }
+}
+
+// P1's counts without the super k-mers that only span a chunk edge (the
+// histogram the partition uses; hist_all keeps every routed super k-mer and
+// sizes the bins). flag |= 1 when the list overflowed (cannot happen: at
+// most K-1 starts per edge).
+__global__ void __launch_bounds__(256) k_p1_excl(const uint16_t* __restrict__ excl, const uint32_t* __restrict__ excl_cnt,
+                                                 uint32_t excl_cap, unsigned long long* __restrict__ hist,
+                                                 unsigned int* __restrict__ flag) {
+  const uint32_t n = *excl_cnt;
+  if (n > excl_cap) {
+    if (threadIdx.x == 0) atomicOr(flag, 1u);
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < n; i += 256) atomicAdd(&hist[excl[i]], ~0ULL);
+}
+
+// 64-bit digest of the packed stream (a save file records the records it
+// belongs to): the sum mod 2^64 of mix64(word ^ index * golden) over the
+// bases words, then the placeholder words (indices continuing).
+__global__ void __launch_bounds__(256) k_stream_digest(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+                                                       uint64_t n_words64, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[4];
+  const uint64_t n = 3 * n_words64;
+  unsigned long long acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t w = i < 2 * n_words64 ? seq[i] : nmask[i - 2 * n_words64];
+    acc += mix64(w ^ (i * 0x9E3779B97F4A7C15ULL));
+  }
+  for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+}
+void launch_stream_digest(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_words64, unsigned long long* out) {
+  const uint64_t n = 3 * n_words64;
+  uint64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g == 0) return;
+  CF1G_LAUNCH("stream_digest", k_stream_digest, dim3((unsigned)g), dim3(256), 0, st, seq, nmask, n_words64, out);
 }
 
 // ------------------------------------------------------------------ route (P2)
@@ -838,18 +907,20 @@ uint64_t p1_slices(uint64_t n_pos) {
 }
 void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m, uint64_t* entries,
                uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
-               unsigned long long* totals) {
+               unsigned long long* hist_all, unsigned long long* totals, const uint64_t* edges, uint32_t n_edges,
+               uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap, unsigned int* flag) {
   const uint64_t nt = p1_tiles(n_pos);
   if (nt == 0) return;
   const uint64_t grid = nt < 4096 ? nt : 4096;
   CF1G_LAUNCH("p1_scan", k_p1_scan, dim3((unsigned)grid), dim3(SK_WG), 0, st, seq, nmask, n_pos, nt, K, m, entries, buckets, tile_cnt,
-              tile_w);
+              tile_w, edges, n_edges, excl_cnt, excl, excl_cap);
   const uint64_t slices = p1_slices(n_pos);
   const uint64_t per = (nt + slices - 1) / slices;
   const uint64_t used = (nt + per - 1) / per;
   CF1G_LAUNCH("p1_hist", k_p1_hist, dim3((unsigned)used, 65536 / HB_RANGE), dim3(HB_WG), 0, st, buckets, tile_cnt, tile_w, nt, per,
               part, sums);
-  CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 64), dim3(256), 0, st, part, sums, (uint32_t)used, hist, totals);
+  CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 64), dim3(256), 0, st, part, sums, (uint32_t)used, hist, hist_all, totals);
+  if (n_edges) CF1G_LAUNCH("p1_excl", k_p1_excl, dim3(1), dim3(256), 0, st, excl, excl_cnt, excl_cap, hist, flag);
 }
 // Route chunking: chunk_tiles tiles per chunk, one route work-group each.
 void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint64_t* n_chunks) {

@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(P4_WG) k_p4_cas(const uint64_t* __restrict__ s
       const uint32_t m = observe<KeyT>(seq, nmask, g, K, canon);
       if (!m) continue;
       n_occ++;
-      const uint64_t idx = mphf_lookup<KeyT>(mv, canon);
+      const uint64_t idx = mphf_find<KeyT>(mv, canon);
       if (idx >= D) {
         n_miss++;
         continue;
@@ -162,10 +162,11 @@ __global__ void __launch_bounds__(P4_WG) k_p4_cas(const uint64_t* __restrict__ s
 }
 
 // Per stream position: bit0 valid k-mer, bit1 FWD, bits2-3 state class.
+// err |= 1 when a k-mer misses the MPHF (the position is left invalid).
 template <typename KeyT>
 __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t n_pos,
-                                                  int K, const MphfView mv, const uint64_t* __restrict__ states,
-                                                  uint8_t* __restrict__ out) {
+                                                  int K, const MphfView mv, const uint64_t* __restrict__ states, uint64_t D,
+                                                  uint8_t* __restrict__ out, unsigned int* __restrict__ err) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_pos) return;
   uint8_t a = 0;
@@ -173,9 +174,13 @@ __global__ void __launch_bounds__(256) k_annotate(const uint64_t* __restrict__ s
     KeyT x = KOps<KeyT>::at(seq, g, K);
     KeyT r = KOps<KeyT>::rc(x, K);
     KeyT c = kmer_min(x, r);
-    uint64_t idx = mphf_lookup<KeyT>(mv, c);
-    uint32_t s = (uint32_t)(states[idx / 12] >> ((idx % 12) * 5)) & 31u;
-    a = (uint8_t)(1u | (key_eq(x, c) ? 2u : 0u) | (st_class(s) << 2));
+    const uint64_t idx = mphf_find<KeyT>(mv, c);
+    if (idx < D) {
+      const uint32_t s = (uint32_t)(states[idx / 12] >> ((idx % 12) * 5)) & 31u;
+      a = (uint8_t)(1u | (key_eq(x, c) ? 2u : 0u) | (st_class(s) << 2));
+    } else {
+      atomicOr(err, 1u);
+    }
   }
   out[g] = a;
 }
@@ -327,12 +332,14 @@ __global__ void __launch_bounds__(256) k_span_claim(const uint64_t* __restrict__
   if (won) fl[u] |= 2u;
 }
 
-template <typename KeyT>
+// Batch Mphf::hash; FIND: a key that misses every level gets ~0 instead of
+// the reference's 0 (for callers that check the ids).
+template <typename KeyT, bool FIND>
 __global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, uint64_t n, const MphfView mv,
                                                 uint64_t* __restrict__ out) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    out[i] = mphf_lookup<KeyT>(mv, keys[i]);
+    out[i] = FIND ? mphf_find<KeyT>(mv, keys[i]) : mphf_lookup<KeyT>(mv, keys[i]);
 }
 
 void launch_p4_pack(hipStream_t st, const uint16_t* vmask, uint64_t D, uint64_t* states) {
@@ -382,22 +389,26 @@ void launch_span_claim(hipStream_t st, const uint64_t* ids, uint64_t n, uint64_t
               (unsigned long long*)states, fl, err);
 }
 void launch_annotate(hipStream_t st, bool wide, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K,
-                     const MphfView& mv, const uint64_t* states, uint8_t* out) {
+                     const MphfView& mv, const uint64_t* states, uint64_t D, uint8_t* out, unsigned int* err) {
   if (n_pos == 0) return;
   dim3 grid((unsigned)((n_pos + 255) / 256));
   if (!wide)
-    CF1G_LAUNCH("p5_annotate", k_annotate<uint64_t>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, mv, states, out);
+    CF1G_LAUNCH("p5_annotate", k_annotate<uint64_t>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, mv, states, D, out, err);
   else
-    CF1G_LAUNCH("p5_annotate", k_annotate<Key128>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, mv, states, out);
+    CF1G_LAUNCH("p5_annotate", k_annotate<Key128>, grid, dim3(256), 0, st, seq, nmask, n_pos, K, mv, states, D, out, err);
 }
-void launch_lookup(hipStream_t st, bool wide, const void* keys, uint64_t n, const MphfView& mv, uint64_t* out) {
+void launch_lookup(hipStream_t st, bool wide, const void* keys, uint64_t n, const MphfView& mv, uint64_t* out, bool find) {
   if (n == 0) return;
   uint64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
-  if (!wide)
-    CF1G_LAUNCH("bbh_lookup", k_lookup<uint64_t>, dim3((unsigned)g), dim3(256), 0, st, (const uint64_t*)keys, n, mv, out);
-  else
-    CF1G_LAUNCH("bbh_lookup", k_lookup<Key128>, dim3((unsigned)g), dim3(256), 0, st, (const Key128*)keys, n, mv, out);
+  const dim3 gr((unsigned)g);
+  if (!wide) {
+    if (find) CF1G_LAUNCH("bbh_lookup", (k_lookup<uint64_t, true>), gr, dim3(256), 0, st, (const uint64_t*)keys, n, mv, out);
+    else CF1G_LAUNCH("bbh_lookup", (k_lookup<uint64_t, false>), gr, dim3(256), 0, st, (const uint64_t*)keys, n, mv, out);
+  } else {
+    if (find) CF1G_LAUNCH("bbh_lookup", (k_lookup<Key128, true>), gr, dim3(256), 0, st, (const Key128*)keys, n, mv, out);
+    else CF1G_LAUNCH("bbh_lookup", (k_lookup<Key128, false>), gr, dim3(256), 0, st, (const Key128*)keys, n, mv, out);
+  }
 }
 
 }  // namespace cf1g

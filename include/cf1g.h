@@ -219,6 +219,13 @@ int cf1g_annotate(cf1g_ctx* ctx, uint8_t* out, uint64_t n);
  * written; otherwise cap must be >= the count. The state vector is not
  * changed. */
 int cf1g_unitig_spans(cf1g_ctx* ctx, uint64_t* n_spans, uint64_t* start, uint64_t* end, uint64_t* id, uint8_t* flags, uint64_t cap);
+/* The input file of each loaded record (0-based), for cf_seq's
+ * "Reference:<file+1>" (ref_id = file_idx + 1 over params.input_files,
+ * src/output.rs:176-177). cf1g_load_fasta sets it from its paths;
+ * cf1g_load_records sets every record to file 0, and a caller that loaded
+ * records of several files gives them here (n_rec entries). */
+int cf1g_set_record_files(cf1g_ctx* ctx, const uint32_t* file_idx, uint64_t n_rec);
+int cf1g_record_files(cf1g_ctx* ctx, uint32_t* out, uint64_t n);
 /* Stream position of record r's first base (n_rec + 1 entries; the last is the
  * stream length). */
 int cf1g_record_starts(cf1g_ctx* ctx, uint64_t* out, uint64_t n);

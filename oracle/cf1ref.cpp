@@ -242,6 +242,7 @@ struct Records {
   std::vector<uint8_t> ascii;
   std::vector<uint64_t> off;  // n+1
   std::vector<std::string> names;
+  std::vector<uint32_t> file;  // input file index per record (output.rs:176 file_idx)
   size_t n() const { return off.empty() ? 0 : off.size() - 1; }
   const uint8_t* seq(size_t r) const { return ascii.data() + off[r]; }
   size_t len(size_t r) const { return (size_t)(off[r + 1] - off[r]); }
@@ -358,22 +359,47 @@ class Pipeline : public PipelineBase {
     return t;
   }
 
+  // The sequences P1 and P2 scan (minimizer.rs:57-86, superkmer.rs:57-88):
+  // a record shorter than K is skipped, one shorter than CHUNK_BASES is
+  // scanned whole, and a longer one in chunks starting every CHUNK_BASES
+  // bases. P1's chunks are CHUNK_BASES long (no overlap: k-mers across a
+  // chunk edge are not counted); P2's extend `overlap` = K-1 bases into the
+  // next chunk, so every k-mer is routed once and a super k-mer always
+  // starts at a chunk's first k-mer. Each piece is scanned as a sequence of
+  // its own.
+  struct Piece {
+    size_t r, a, b;  // bases [a, b) of record r
+  };
+  std::vector<Piece> scan_pieces(size_t overlap) const {
+    std::vector<Piece> out;
+    for (size_t r = 0; r < rec.n(); r++) {
+      const size_t len = rec.len(r);
+      if (len < (size_t)K) continue;
+      if (len < CHUNK_BASES) {
+        out.push_back({r, 0, len});
+        continue;
+      }
+      for (size_t a = 0; a < len; a += CHUNK_BASES) out.push_back({r, a, std::min(a + CHUNK_BASES + overlap, len)});
+    }
+    return out;
+  }
+  // cb(record-relative start, n_kmers, minimizer value) for every super
+  // k-mer of a piece.
   template <typename CB>
-  void superkmers_of_task(const Task& t, CB cb) const {
-    const uint8_t* s = rec.seq(t.r);
-    for_each_run(s, rec.len(t.r), K, [&](size_t a, size_t b) {
-      if (b - (size_t)K + 1 <= t.lo || a >= t.hi) return;
-      superkmers_in_run(s, a, b, K, m, t.lo, t.hi, cb);
+  void superkmers_of_piece(const Piece& p, CB cb) const {
+    const uint8_t* s = rec.seq(p.r) + p.a;
+    for_each_run(s, p.b - p.a, K, [&](size_t a, size_t b) {
+      superkmers_in_run(s, a, b, K, m, a, b, [&](size_t st, size_t n, uint64_t v) { cb(p.a + st, n, v); });
     });
   }
 
   // P1 — minimizer.rs:27-98: one count per super k-mer at bucket_hash(v).
   void p1() override {
-    auto tasks = make_tasks(CHUNK_BASES);
+    auto pieces = scan_pieces(0);
     std::vector<std::atomic<uint64_t>> h(NUM_HISTOGRAM_BUCKETS);
     for (auto& x : h) x.store(0);
-    parallel_for(tasks.size(), threads, [&](size_t ti) {
-      superkmers_of_task(tasks[ti], [&](size_t, size_t, uint64_t v) { h[bucket_hash(v)].fetch_add(1, std::memory_order_relaxed); });
+    parallel_for(pieces.size(), threads, [&](size_t pi) {
+      superkmers_of_piece(pieces[pi], [&](size_t, size_t, uint64_t v) { h[bucket_hash(v)].fetch_add(1, std::memory_order_relaxed); });
     });
     hist.resize(NUM_HISTOGRAM_BUCKETS);
     for (size_t i = 0; i < NUM_HISTOGRAM_BUCKETS; i++) hist[i] = h[i].load();
@@ -402,15 +428,15 @@ class Pipeline : public PipelineBase {
     if (len - seg_start >= (size_t)K) pack_and_buffer(sub + seg_start, len - seg_start, buf);
   }
 
-  // P2 — superkmer.rs:21-153: each super k-mer [s, s+n-1+K) to its bin.
+  // P2 — superkmer.rs:21-153: each super k-mer [s, s+n-1+K) of a piece to its bin.
   void p2_route() override {
-    auto tasks = make_tasks(CHUNK_BASES);
+    auto pieces = scan_pieces((size_t)K - 1);
     bins.assign((size_t)num_bins, {});
     std::vector<std::mutex> mu((size_t)num_bins);
-    parallel_for(tasks.size(), threads, [&](size_t ti) {
+    parallel_for(pieces.size(), threads, [&](size_t pi) {
       std::vector<std::vector<uint8_t>> local((size_t)num_bins);
-      const uint8_t* s = rec.seq(tasks[ti].r);
-      superkmers_of_task(tasks[ti], [&](size_t start, size_t n, uint64_t v) {
+      const uint8_t* s = rec.seq(pieces[pi].r);
+      superkmers_of_piece(pieces[pi], [&](size_t start, size_t n, uint64_t v) {
         int bin = bucket_to_bin[bucket_hash(v)];
         write_packed_segments(s + start, n - 1 + K, local[(size_t)bin]);
       });
@@ -841,7 +867,7 @@ class Pipeline : public PipelineBase {
     std::vector<P5Meta> metas(nrec);
     std::vector<char> present(nrec, 0);
     auto tiling_line = [&](size_t r, const std::vector<OU>& us) {  // output.rs:208-259
-      std::string t = "Reference:1_Sequence:";
+      std::string t = "Reference:" + std::to_string((uint64_t)(r < rec.file.size() ? rec.file[r] : 0) + 1) + "_Sequence:";  // output.rs:176-177
       t += rec.names.size() > r ? rec.names[r] : std::string();
       t.push_back('\t');
       if (!us.empty()) {
@@ -1012,6 +1038,13 @@ int ref_load(void* hp, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n
   for (auto& o : r.off) o -= base;
   r.ascii.assign(ascii + base, ascii + base + r.off.back());
   r.names.clear();
+  r.file.assign(n_rec, 0);
+  return 0;
+}
+int ref_set_files(void* hp, const uint32_t* file_idx, uint64_t n) {
+  Handle* h = (Handle*)hp;
+  if (n != h->p->rec.n()) return -1;
+  h->p->rec.file.assign(file_idx, file_idx + n);
   return 0;
 }
 int ref_set_names(void* hp, const char* blob, const uint64_t* name_off, uint64_t n) {
@@ -1046,6 +1079,37 @@ uint64_t ref_bin_bytes(void* hp, uint32_t b, uint8_t* out) {
   const auto& d = h->p->bins[b];
   if (out) std::memcpy(out, d.data(), d.size());
   return d.size();
+}
+// All routed records of every bin in a fixed-width form for bulk
+// comparison (test infrastructure): per record its bin, its base count L and
+// `rw` u64 words holding the bases MSB-first from bit 63 of word 0 (base t at
+// bits 63-2(t%32) .. 62-2(t%32) of word t/32), zero past base L. Parses the
+// reference byte records (BinReader, superkmer.rs:276-288). Returns the
+// record count; outputs may be NULL.
+uint64_t ref_bin_records(void* hp, uint32_t rw, uint64_t* out_bin, uint64_t* out_len, uint64_t* out_words) {
+  Handle* h = (Handle*)hp;
+  uint64_t n = 0;
+  for (size_t b = 0; b < h->p->bins.size(); b++) {
+    const std::vector<uint8_t>& d = h->p->bins[b];
+    size_t p = 0;
+    while (p + 2 <= d.size()) {
+      const size_t bc = (size_t)d[p] | ((size_t)d[p + 1] << 8);
+      p += 2;
+      if (out_bin) {
+        out_bin[n] = b;
+        out_len[n] = bc;
+        uint64_t* w = out_words + n * rw;
+        for (uint32_t q = 0; q < rw; q++) w[q] = 0;
+        for (size_t t = 0; t < bc && t < 32 * (size_t)rw; t++) {
+          const uint64_t code = (d[p + t / 4] >> (6 - 2 * (t % 4))) & 3u;
+          w[t / 32] |= code << (62 - 2 * (t % 32));
+        }
+      }
+      p += (bc + 3) / 4;
+      n++;
+    }
+  }
+  return n;
 }
 int ref_p3(void* hp, uint64_t* d_out) {
   Handle* h = (Handle*)hp;

@@ -541,9 +541,13 @@ struct Mphf {
     uint64_t seg = segment(c);
     uint64_t ko = key_off[seg], t = key_off[seg + 1] - ko;
     const uint64_t* base = blocks.data() + blk_off[seg] * 8;
+    // only the levels inside the segment's blocks (the build stores levels
+    // until every key is placed; the level sizes never reach 0)
+    const uint64_t lim = (blk_off[seg + 1] - blk_off[seg]) * BBH_BLOCK_BITS;
     uint64_t off = 0;
     for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++) {
       uint64_t n = bbh_level_bits(t);
+      if (off + n > lim) break;
       uint64_t pos = off + bbh_pos(lo, hi, wide, l, n);
       const uint64_t* B = base + (pos / BBH_BLOCK_BITS) * 8;
       uint64_t b = pos % BBH_BLOCK_BITS;

@@ -29,10 +29,13 @@ def lib():
         L.ref_destroy.argtypes = [C.c_void_p]
         L.ref_load.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64]
         L.ref_set_names.argtypes = [C.c_void_p, C.c_char_p, U64P, C.c_uint64]
+        L.ref_set_files.argtypes = [C.c_void_p, U32P, C.c_uint64]
         L.ref_p1.argtypes = [C.c_void_p, U64P]
         L.ref_p2.argtypes = [C.c_void_p, U16P, U32P]
         L.ref_bin_bytes.restype = C.c_uint64
         L.ref_bin_bytes.argtypes = [C.c_void_p, C.c_uint32, U8P]
+        L.ref_bin_records.restype = C.c_uint64
+        L.ref_bin_records.argtypes = [C.c_void_p, C.c_uint32, U64P, U64P, U64P]
         L.ref_p3.argtypes = [C.c_void_p, U64P]
         L.ref_keys.restype = C.c_uint64
         L.ref_keys.argtypes = [C.c_void_p, U64P, U64P]
@@ -112,13 +115,27 @@ class RefPipeline:
 
     def load(self, seqs, names=None):
         a, off = pack_records(seqs)
+        self.load_arrays(a, off, names)
+
+    def load_arrays(self, ascii_, off, names=None):
+        """Records as one byte array and n+1 offsets (ascii_[off[r]:off[r+1]])."""
+        a = np.ascontiguousarray(ascii_, np.uint8)
+        if a.size == 0:
+            a = np.zeros(1, np.uint8)
+        off = np.ascontiguousarray(off, np.uint64)
         self._ascii, self._off = a, off
-        self.L.ref_load(self.h, _p(a, U8P), _p(off, U64P), len(seqs))
+        self.L.ref_load(self.h, _p(a, U8P), _p(off, U64P), len(off) - 1)
         if names is not None:
             blob = "".join(names).encode()
             noff = np.zeros(len(names) + 1, np.uint64)
             noff[1:] = np.cumsum([len(n.encode()) for n in names])
             self.L.ref_set_names(self.h, blob, _p(noff, U64P), len(names))
+
+    def set_files(self, file_idx):
+        """Input file index of each record (cf_seq's Reference:<file+1>)."""
+        f = np.ascontiguousarray(file_idx, np.uint32)
+        if self.L.ref_set_files(self.h, _p(f, U32P), len(f)) != 0:
+            raise ValueError("one file index per record")
 
     def p1(self):
         h = np.zeros(65536, np.uint64)
@@ -137,6 +154,15 @@ class RefPipeline:
         out = np.zeros(max(n, 1), np.uint8)
         self.L.ref_bin_bytes(self.h, b, _p(out, U8P))
         return out[:n].tobytes()
+
+    def bin_records(self, rw):
+        """Every routed record as (bin, base count, rw words of bases MSB-first)."""
+        n = self.L.ref_bin_records(self.h, rw, None, None, None)
+        b = np.zeros(max(n, 1), np.uint64)
+        ln = np.zeros(max(n, 1), np.uint64)
+        w = np.zeros((max(n, 1), rw), np.uint64)
+        self.L.ref_bin_records(self.h, rw, _p(b, U64P), _p(ln, U64P), _p(w, U64P))
+        return b[:n], ln[:n], w[:n]
 
     def p3(self):
         d = C.c_uint64(0)

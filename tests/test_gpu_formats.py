@@ -171,6 +171,17 @@ def test_unitig_spans_partition_runs(k):
     g.close()
 
 
+def _digest(words):
+    """Save-body checksum: sum mod 2^64 of mix64(word ^ index * golden)."""
+    M = (1 << 64) - 1
+    z = (words ^ (np.arange(len(words), dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)))
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return int(z.sum(dtype=np.uint64)) & M
+
+
 def test_restore_rejects_damaged_files():
     seqs = random_genome(seed=990, n_rec=2, max_len=3000)
     g = cf1g.Cf1g(k=31, num_bins=4)
@@ -180,12 +191,38 @@ def test_restore_rejects_damaged_files():
         path = os.path.join(td, "run.cf1g")
         g.save(path)
         data = open(path, "rb").read()
+        w = np.frombuffer(data, np.uint64).copy()
+        mw = int(w[6])
+        ns = int(w[16 + 5])
+        tab = 65536 // 4
+        ko = 16 + 16 + tab          # first key offset word of the MPHF body
         bad = []
         bad.append(data[: len(data) - 8])                       # truncated
         hdr = bytearray(data)
         hdr[6 * 8:7 * 8] = (1 << 50).to_bytes(8, "little")      # absurd size in the header
         bad.append(bytes(hdr))
         bad.append(b"not a save file" + data[15:])              # wrong magic
+        for word in (16 + 20, ko + 1, ko + ns + 2, 16 + mw - 3, 16 + mw + 1):  # any body word flipped
+            x = w.copy()
+            x[word] ^= np.uint64(1 << 33)
+            bad.append(x.tobytes())
+
+        def resealed(x):  # a damaged body with a matching checksum: the structure checks
+            x[9] = np.uint64(_digest(x[16:]))
+            return x.tobytes()
+
+        x = w.copy()
+        x[16 + 16] = np.uint64(0xFFFF)                          # bucket->bin entries beyond the bins
+        bad.append(resealed(x))
+        x = w.copy()
+        x[ko + 1], x[ko + 2] = x[ko + 2], x[ko + 1]             # key offsets not monotone
+        bad.append(resealed(x))
+        x = w.copy()
+        x[ko + 2 * ns + 1] += np.uint64(1)                      # last block offset != block count
+        bad.append(resealed(x))
+        x = w.copy()
+        x[16 + 7] = np.uint64(1 << 62)                          # block count that would wrap the size arithmetic
+        bad.append(resealed(x))
         for i, b in enumerate(bad):
             p = os.path.join(td, "bad%d" % i)
             open(p, "wb").write(b)
@@ -193,6 +230,20 @@ def test_restore_rejects_damaged_files():
             h.load_records(seqs)
             with pytest.raises(cf1g.Cf1gError) as e:
                 h.restore(p)
-            assert e.value.code == -1
+            assert e.value.code == -1, (i, e.value)
             h.close()
+        # records of the same count and length, other bases: refused (stream digest)
+        other = [bytes(b"ACGT"[(b"ACGT".index(c) + 1) % 4] if c in b"ACGT" else c for c in s) for s in seqs]
+        h = cf1g.Cf1g(k=31, num_bins=4)
+        h.load_records(other)
+        with pytest.raises(cf1g.Cf1gError) as e:
+            h.restore(path)
+        assert e.value.code == -1 and "digest" in str(e.value)
+        h.close()
+        # the untouched file still restores
+        h = cf1g.Cf1g(k=31, num_bins=4)
+        h.load_records(seqs)
+        h.restore(path)
+        np.testing.assert_array_equal(h.states(), g.states())
+        h.close()
     g.close()

@@ -172,12 +172,57 @@ def test_cli_build_matches_oracle():
         assert r.returncode == 0, r.stderr
         ref = RefPipeline(31, num_bins=128, threads=4)
         ref.load(recs + extra, list(names) + ["x0", "x1", "x2"])
+        ref.set_files([0] * len(recs) + [1] * len(extra))
         ref.run_p1_p4()
         rp = os.path.join(td, "ref")
         ref.p5(rp, input_desc=p1 + ", " + p2)
         for ext in (".cf_seg", ".cf_seq"):
             assert sorted(open(pre + ext).read().splitlines()) == sorted(open(rp + ext).read().splitlines()), ext
+        # ref_id = file_idx + 1 (src/output.rs:176-177); x1 is shorter than k
+        heads = sorted(line.split("\t")[0] for line in open(pre + ".cf_seq").read().splitlines())
+        assert heads == sorted(["Reference:1_Sequence:" + n for n in names] + ["Reference:2_Sequence:x0", "Reference:2_Sequence:x2"])
         jg, jr = json.load(open(pre + ".json")), json.load(open(rp + ".json"))
         jg["parameters info"].pop("output prefix")
         jr["parameters info"].pop("output prefix")
         assert jg == jr
+
+
+def test_two_files_reference_ids():
+    # records of the second input file are Reference:2 in cf_seq, through
+    # cf1g_load_fasta and through cf1g_load_records + cf1g_set_record_files
+    from oracle import RefPipeline
+
+    r = random.Random(4)
+    f1 = [rand_seq(r, 400).encode() for _ in range(3)]
+    f2 = [rand_seq(r, 300).encode() for _ in range(2)]
+    names = ["a0", "a1", "a2", "b0", "b1"]
+    with tempfile.TemporaryDirectory() as td:
+        paths = [os.path.join(td, "1.fa"), os.path.join(td, "2.fa")]
+        for p, recs, nm in ((paths[0], f1, names[:3]), (paths[1], f2, names[3:])):
+            with open(p, "wb") as f:
+                for n, s in zip(nm, recs):
+                    f.write(b">" + n.encode() + b"\n" + s + b"\n")
+        ref = RefPipeline(31, num_bins=8, threads=2)
+        ref.load(f1 + f2, names)
+        ref.set_files([0, 0, 0, 1, 1])
+        ref.run_p1_p4()
+        rp = os.path.join(td, "ref")
+        ref.p5(rp, input_desc="x")
+        want = sorted(open(rp + ".cf_seq").read().splitlines())
+        assert sum(w.startswith("Reference:2_Sequence:b") for w in want) == 2
+        for via in ("fasta", "records"):
+            g = cf1g.Cf1g(k=31, num_bins=8)
+            if via == "fasta":
+                g.load_fasta(paths, threads=2)
+            else:
+                g.load_records(f1 + f2)
+                assert g.record_files().tolist() == [0] * 5
+                g.set_record_files([0, 0, 0, 1, 1])
+            assert g.record_files().tolist() == [0, 0, 0, 1, 1]
+            g.run_p1_p4()
+            pre = os.path.join(td, via)
+            g.p5_write(pre, names=names, input_desc="x")
+            assert sorted(open(pre + ".cf_seq").read().splitlines()) == want, via
+            with pytest.raises(cf1g.Cf1gError):
+                g.set_record_files([0, 1])
+            g.close()

@@ -234,3 +234,26 @@ def test_minimizer_length_invariance(k, m):
         out.append(graph_by_kmer(g, k))
         g.close()
     assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("k", [15, 31, 63])
+def test_mphf_lookup_absent_keys(k):
+    # keys outside the set: the device's values equal the oracle's (0 when a
+    # key misses every level of its segment; levels are read only inside the
+    # segment's blocks), for many segments and for one large one
+    seqs = random_genome(seed=31 + k, n_rec=3, max_len=6000)
+    rng = np.random.default_rng(k)
+    for bins, tgt in ((8, 0), (1, 1 << 24)):
+        ref = RefPipeline(k, num_bins=bins, threads=2, target_seg=tgt)
+        ref.load(seqs)
+        ref.run_p1_p4()
+        g = cf1g.Cf1g(k=k, num_bins=bins, p3_target_seg=tgt)
+        g.load_records(seqs)
+        g.run_p1_p4()
+        mask = (1 << (2 * k)) - 1 if k <= 32 else (1 << 64) - 1
+        lo = rng.integers(0, 1 << 63, 20000, dtype=np.uint64) & np.uint64(mask)
+        hi = (rng.integers(0, 1 << 62, 20000, dtype=np.uint64) & np.uint64((1 << (2 * k - 64)) - 1)) if k > 32 else None
+        got = g.mphf_lookup(lo, hi)
+        np.testing.assert_array_equal(got, ref.mphf_lookup(lo, hi))
+        assert got.max() < g.info()["n_distinct"]
+        g.close()
