@@ -34,7 +34,7 @@ EXPORTS = [
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_unitig_spans", "cf1g_record_starts",
-    "cf1g_set_record_files", "cf1g_record_files",
+    "cf1g_set_record_files", "cf1g_record_files", "cf1g_mphf_lookup_noalloc",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_graphs", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
     "cf1g_lidx_gradients", "cf1g_classify_via", "cf1g_select_index", "cf1g_selected_index",
@@ -139,6 +139,7 @@ def lib():
         L.cf1g_mphf_export.argtypes = [C.c_void_p, U64P, U64P]
         L.cf1g_mphf_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
         L.cf1g_mphf_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.cf1g_mphf_lookup_noalloc.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
         L.cf1g_annotate.argtypes = [C.c_void_p, U8P, C.c_uint64]
         L.cf1g_unitig_spans.argtypes = [C.c_void_p, U64P, U64P, U64P, U64P, U8P, C.c_uint64]
         L.cf1g_lidx_train.argtypes = [C.c_void_p, C.POINTER(LidxParams), C.POINTER(LidxInfo)]

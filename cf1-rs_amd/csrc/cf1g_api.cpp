@@ -60,6 +60,8 @@ void launch_p3_keys(hipStream_t, bool, const void*, const uint64_t*, const uint6
                     uint64_t, void*, unsigned int*);
 void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
 void launch_stream_digest(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, unsigned long long*);
+void noalloc_setup(int device);
+void* noalloc_scratch(int device, size_t* bytes);
 void launch_p4_cas(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, uint64_t, uint64_t*, uint64_t*,
                    uint32_t, uint32_t, int);
 uint64_t sort_tiles(uint64_t);
@@ -996,6 +998,8 @@ int cf1g_create(const cf1g_params* params, cf1g_ctx** out) {
       fail(CF1G_E_NODEVICE, std::string("cf1g is built for gfx950, device is ") + prop.gcnArchName);
     c->cu_count = prop.multiProcessorCount;
     hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
+    c->device = 0;
+    noalloc_setup(c->device);  // the fixed scratch of cf1g_mphf_lookup_noalloc
   });
   if (rc != CF1G_OK) {
     // keep the context so the caller can read the error
@@ -1482,6 +1486,41 @@ int cf1g_mphf_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* ou
     hip_check(hipMemcpyAsync(dk, keys, n * kb, hipMemcpyHostToDevice, c->st), "H2D keys");
     launch_lookup(c->st, c->wide, dk, n, c->view(), dout, false);
     c->d2h(out, dout, n * 8);
+  });
+}
+
+int cf1g_mphf_lookup_noalloc(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* out) {
+  return guarded(c, [&]() {
+    need(c, 4, "cf1g_mphf_lookup_noalloc");
+    if (n == 0) return;
+    if (!keys || !out) fail(CF1G_E_INVALID, "NULL buffer");
+    size_t cap = 0;
+    uint8_t* s = (uint8_t*)noalloc_scratch(c->device, &cap);
+    if (!s) fail(CF1G_E_INTERNAL, "the fixed scratch of this device was not set up");
+    const uint64_t kb = c->key_bytes;
+    const uint64_t max_n = cap / (kb + 8) - 64;
+    if (n > max_n)
+      fail(CF1G_E_INVALID, "cf1g_mphf_lookup_noalloc takes at most " + std::to_string(max_n) + " keys per call, got " + std::to_string(n));
+    // the MPHF's buffers exist since P3 (or a restore); nothing is (re)allocated
+    auto have = [&](const char* name, uint64_t bytes) -> void* {
+      auto it = c->bufs.find(name);
+      if (it == c->bufs.end() || !it->second.p || it->second.cap < bytes) fail(CF1G_E_ORDER, std::string("no MPHF buffer ") + name);
+      return it->second.p;
+    };
+    MphfView v;
+    v.b2b = (const uint16_t*)have("b2b", CF1G_NUM_BUCKETS * 2);
+    v.key_off = (const uint64_t*)have("seg_out_off", (c->n_segs + 1) * 8);
+    v.blk_off = (const uint64_t*)have("bbh_blk_off", (c->n_segs + 1) * 8);
+    v.blocks = (const uint64_t*)have("blocks", (c->n_blocks * 8 + 8) * 8);
+    v.K = c->k;
+    v.m = c->m;
+    v.sb = c->sb;
+    uint8_t* dk = s;                                               // keys
+    uint64_t* dout = (uint64_t*)(s + ((n * kb + 255) & ~uint64_t(255)));  // values
+    hip_check(hipMemcpyAsync(dk, keys, n * kb, hipMemcpyHostToDevice, c->st), "H2D keys");
+    launch_lookup(c->st, c->wide, dk, n, v, dout, false);
+    hip_check(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, c->st), "D2H values");
+    c->sync();
   });
 }
 

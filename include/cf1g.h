@@ -200,6 +200,16 @@ int cf1g_keys_export(cf1g_ctx* ctx, uint64_t* out, uint64_t n_words);
 int cf1g_mphf_export(cf1g_ctx* ctx, uint64_t* buf, uint64_t* n_words);
 /* Batch Mphf::hash (src/mphf.rs:268-270): keys in host memory (u64 or pairs). */
 int cf1g_mphf_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
+/* No counterpart in the reference: cf1g_mphf_lookup for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same keys in host memory, same values,
+ * but the keys and values are staged through a fixed 32 MiB device array
+ * that each device holds for the life of the process, so the call allocates
+ * nothing; at most 32 MiB / (key bytes + 8) - 64 keys per call (2,097,088 for
+ * k <= 32, 1,398,037 for k > 32), more returns CF1G_E_INVALID. Calls on one
+ * device share that array: calls that overlap on one device (from several
+ * threads or contexts) race. */
+int cf1g_mphf_lookup_noalloc(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
 /* Same on device-resident keys and output (n keys). */
 int cf1g_mphf_lookup_device(cf1g_ctx* ctx, const void* d_keys, uint64_t n, uint64_t* d_out);
 /* Per stream position (cf1g_info.stream_len bytes): bit0 valid k-mer,
