@@ -34,7 +34,8 @@ EXPORTS = [
     "cf1g_p1_histogram", "cf1g_p2_partition_route", "cf1g_p3_build", "cf1g_p4_classify",
     "cf1g_p4_classify_cas", "cf1g_autotune", "cf1g_run_p1_p4", "cf1g_states_export", "cf1g_keys_export", "cf1g_mphf_export",
     "cf1g_mphf_lookup", "cf1g_mphf_lookup_device", "cf1g_annotate", "cf1g_unitig_spans", "cf1g_record_starts",
-    "cf1g_set_record_files", "cf1g_record_files", "cf1g_mphf_lookup_noalloc",
+    "cf1g_set_record_files", "cf1g_record_files", "cf1g_mphf_lookup_noalloc", "cf1g_p1_stream",
+    "cf1g_run_p1_p4_stream",
     "cf1g_bins_export", "cf1g_p5_write", "cf1g_sync", "cf1g_set_graphs", "cf1g_set_profiling", "cf1g_kernel_times",
     "cf1g_device_keys", "cf1g_lidx_train", "cf1g_lidx_lookup", "cf1g_lidx_lookup_device", "cf1g_lidx_export",
     "cf1g_lidx_gradients", "cf1g_classify_via", "cf1g_select_index", "cf1g_selected_index",
@@ -128,6 +129,8 @@ def lib():
         L.cf1g_save.argtypes = [C.c_void_p, C.c_char_p]
         L.cf1g_restore.argtypes = [C.c_void_p, C.c_char_p]
         L.cf1g_p1_histogram.argtypes = [C.c_void_p, U64P]
+        L.cf1g_p1_stream.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64, C.c_uint32, U64P]
+        L.cf1g_run_p1_p4_stream.argtypes = [C.c_void_p, U8P, U64P, C.c_uint64, C.c_uint32, U64P]
         L.cf1g_p2_partition_route.argtypes = [C.c_void_p, U64P, C.c_uint32, U16P, U32P]
         L.cf1g_p3_build.argtypes = [C.c_void_p, U64P]
         L.cf1g_p4_classify.argtypes = [C.c_void_p]
@@ -288,6 +291,29 @@ class Cf1g:
 
     def restore(self, path):
         self._check(self.L.cf1g_restore(self.h, os.fsencode(path)))
+
+    def p1_stream(self, seqs=None, ascii=None, rec_off=None, threads=0, want=True):
+        """Load the records and run P1 while they stream to the device
+        (cf1g_p1_stream): the same products as load_records + P1."""
+        if seqs is not None:
+            ascii, rec_off = pack_records(seqs)
+        self._ascii = np.ascontiguousarray(ascii, np.uint8)
+        self._off = np.ascontiguousarray(rec_off, np.uint64)
+        self._from_fasta = False
+        h = np.zeros(NUM_BUCKETS, np.uint64) if want else None
+        self._check(self.L.cf1g_p1_stream(self.h, _p(self._ascii, U8P), _p(self._off, U64P), len(self._off) - 1, threads,
+                                          _p(h, U64P) if want else None))
+        return h
+
+    def run_p1_p4_stream(self, ascii, rec_off, threads=0):
+        """P1..P4 from records in host memory (cf1g_run_p1_p4_stream)."""
+        self._ascii = np.ascontiguousarray(ascii, np.uint8)
+        self._off = np.ascontiguousarray(rec_off, np.uint64)
+        self._from_fasta = False
+        d = C.c_uint64(0)
+        self._check(self.L.cf1g_run_p1_p4_stream(self.h, _p(self._ascii, U8P), _p(self._off, U64P), len(self._off) - 1, threads,
+                                                 C.byref(d)))
+        return d.value
 
     def count_minimizer_histogram(self, want=True):
         h = np.zeros(NUM_BUCKETS, np.uint64) if want else None

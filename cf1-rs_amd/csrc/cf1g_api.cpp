@@ -26,6 +26,11 @@ void launch_pack(hipStream_t, const uint8_t*, const uint64_t*, const uint64_t*, 
 uint64_t p1_tiles(uint64_t);
 uint64_t p1_tile_positions();
 uint64_t p1_slices(uint64_t);
+void launch_p1_scan(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t, int, int, uint64_t*, uint16_t*,
+                    uint32_t*, uint32_t*, const uint64_t*, uint32_t, uint32_t*, uint16_t*, uint32_t);
+uint64_t p1_tile_reach(uint64_t);
+void launch_p1_finish(hipStream_t, uint64_t, uint16_t*, uint32_t*, uint32_t*, uint32_t*, uint64_t*, unsigned long long*,
+                      unsigned long long*, unsigned long long*, uint32_t, uint32_t*, uint16_t*, uint32_t, unsigned int*);
 void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint16_t*, uint32_t*, uint32_t*, uint32_t*,
                uint64_t*, unsigned long long*, unsigned long long*, unsigned long long*, const uint64_t*, uint32_t, uint32_t*,
                uint16_t*, uint32_t, unsigned int*);
@@ -209,6 +214,14 @@ struct cf1g_ctx {
   bool restored = false;
   void* pinned = nullptr;
   size_t pinned_cap = 0;
+  // streaming P1 (cf1g_p1_stream): host packing threads, a copy stream, and
+  // pinned staging slots with the events that free them
+  std::unique_ptr<HostPool> pool;
+  hipStream_t st_copy = nullptr;
+  static constexpr int N_SLOTS = 3;
+  void* slot_mem = nullptr;
+  size_t slot_words = 0;  // packed words per slot
+  hipEvent_t slot_ev[N_SLOTS] = {};
 
   void* get(const std::string& name, size_t bytes) {
     DevBuf& b = bufs[name];
@@ -387,6 +400,116 @@ void do_p1(cf1g_ctx* c) {
     launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, bkt, tcnt, tw, part, sums, hist, hist_all, tot, edges, n_edges, excl_cnt, excl,
               excl_cap, excl_cnt + 1);
   });
+  c->stage = 2;
+}
+
+// Chunk edges (stream positions) of the loaded records, plus a UINT64_MAX
+// sentinel, into edges_h and the device buffer "edges".
+void set_chunk_edges(cf1g_ctx* c) {
+  constexpr uint64_t CHUNK_BASES = 4000000;  // src/minimizer.rs:13, src/superkmer.rs:16
+  c->edges_h.clear();
+  for (uint64_t r = 0; r < c->n_rec; r++) {
+    const uint64_t len = c->gstart_h[r + 1] - 1 - c->gstart_h[r];
+    for (uint64_t a = CHUNK_BASES; a < len; a += CHUNK_BASES) c->edges_h.push_back(c->gstart_h[r] + a);
+  }
+  c->edges_h.push_back(UINT64_MAX);
+  hip_check(hipMemcpyAsync(c->getT<uint64_t>("edges", c->edges_h.size()), c->edges_h.data(), c->edges_h.size() * 8,
+                           hipMemcpyHostToDevice, c->st),
+            "H2D chunk edges");
+}
+
+// Record sizes, record starts and chunk edges of host records (what
+// cf1g_load_records and cf1g_p1_stream share).
+void set_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec) {
+  if (!rec_off) fail(CF1G_E_INVALID, "rec_off is NULL");
+  for (uint64_t r = 0; r < n_rec; r++)
+    if (rec_off[r + 1] < rec_off[r]) fail(CF1G_E_INVALID, "rec_off must be non-decreasing");
+  const uint64_t base = rec_off[0];
+  if (rec_off[n_rec] - base > 0 && !ascii) fail(CF1G_E_INVALID, "ascii is NULL");
+  c->n_rec = n_rec;
+  c->n_ascii = rec_off[n_rec] - base;
+  c->gstart_h.assign(n_rec + 1, 0);
+  for (uint64_t r = 0; r <= n_rec; r++) c->gstart_h[r] = rec_off[r] - base + r + 1;
+  c->n_pos = c->n_ascii + n_rec + 1;
+  c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
+  c->host_records = c->host_packed = c->restored = false;
+  c->rec_file.assign(n_rec, 0);
+  c->W = c->S = c->D = 0;
+}
+
+// P1 over host records while they stream to the device: host threads pack
+// chunks of the 2-bit stream into pinned slots, each chunk's copy (on the
+// copy stream) overlaps the packing of the next, and the P1 scan runs over
+// the tiles whose stream words have landed; the histogram follows the last
+// chunk. The products equal cf1g_load_records + cf1g_p1_histogram's.
+void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint32_t threads) {
+  set_records(c, ascii, rec_off, n_rec);
+  c->stage = 0;
+  set_chunk_edges(c);
+  const int th = threads ? (int)threads : std::max(1, (int)std::thread::hardware_concurrency());
+  if (!c->pool || c->pool->threads() != th) c->pool.reset(new HostPool(th));
+  if (!c->st_copy) hip_check(hipStreamCreateWithFlags(&c->st_copy, hipStreamNonBlocking), "hipStreamCreate (copy)");
+  constexpr size_t CW = size_t(1) << 18;  // stream words per chunk (16.8 M positions, 6 MiB packed)
+  if (!c->slot_mem) {
+    hip_check(hipHostMalloc(&c->slot_mem, cf1g_ctx::N_SLOTS * 3 * CW * 8, hipHostMallocDefault), "hipHostMalloc (staging)");
+    c->slot_words = CW;
+    for (auto& e : c->slot_ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  }
+  const uint64_t nw = c->n_words64, nt = p1_tiles(c->n_pos), nsl = p1_slices(c->n_pos);
+  uint64_t* seq = c->getT<uint64_t>("seq", 2 * nw + 8);
+  uint64_t* nm = c->getT<uint64_t>("nmask", nw + 8);
+  auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
+  auto* hist_all = c->getT<unsigned long long>("hist_all", CF1G_NUM_BUCKETS);
+  auto* tot = c->getT<unsigned long long>("totals", 4);
+  uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
+  uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
+  uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
+  uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
+  uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
+  const uint32_t n_edges = (uint32_t)(c->edges_h.size() - 1);
+  const uint64_t* edges = c->getT<uint64_t>("edges", c->edges_h.size());
+  const uint32_t excl_cap = n_edges * 64 + 1;
+  uint32_t* excl_cnt = c->getT<uint32_t>("p1_excl_cnt", 2);
+  uint16_t* excl = c->getT<uint16_t>("p1_excl", excl_cap);
+  hipStream_t st = c->st;
+  c->zero(excl_cnt, 8);
+  if (nt == 0) {
+    c->zero(hist, CF1G_NUM_BUCKETS * 8);
+    c->zero(hist_all, CF1G_NUM_BUCKETS * 8);
+    c->zero(tot, 32);
+  }
+  // the first records' offset is subtracted by pack_words (rec_off[0] is the base)
+  const uint8_t* a = ascii ? ascii + rec_off[0] : (const uint8_t*)"";
+  uint64_t t_done = 0;
+  const uint64_t parts = (uint64_t)th * 2;
+  for (uint64_t w0 = 0, i = 0; w0 < nw; w0 += CW, i++) {
+    const uint64_t w1 = std::min<uint64_t>(nw, w0 + CW);
+    const int s = (int)(i % cf1g_ctx::N_SLOTS);
+    uint64_t* sseq = (uint64_t*)c->slot_mem + (size_t)s * 3 * CW;
+    uint64_t* snm = sseq + 2 * CW;
+    if (i >= (uint64_t)cf1g_ctx::N_SLOTS) hip_check(hipEventSynchronize(c->slot_ev[s]), "hipEventSynchronize (staging slot)");
+    c->pool->run(parts, [&](uint64_t p) {
+      const uint64_t a0 = w0 + (w1 - w0) * p / parts, a1 = w0 + (w1 - w0) * (p + 1) / parts;
+      pack_words(a, rec_off, c->gstart_h.data(), n_rec, a0, a1, sseq + 2 * (a0 - w0), snm + (a0 - w0));
+    });
+    hip_check(hipMemcpyAsync(seq + 2 * w0, sseq, 2 * (w1 - w0) * 8, hipMemcpyHostToDevice, c->st_copy), "H2D stream chunk");
+    hip_check(hipMemcpyAsync(nm + w0, snm, (w1 - w0) * 8, hipMemcpyHostToDevice, c->st_copy), "H2D placeholder chunk");
+    hip_check(hipEventRecord(c->slot_ev[s], c->st_copy), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(st, c->slot_ev[s], 0), "hipStreamWaitEvent");
+    // tiles whose reads lie inside the words that have landed
+    uint64_t t_ready = nt;
+    if (w1 < nw) {
+      const uint64_t have = w1 * 64;
+      t_ready = t_done;
+      while (t_ready < nt && p1_tile_reach(t_ready) <= have) t_ready++;
+    }
+    launch_p1_scan(st, seq, nm, c->n_pos, t_done, t_ready, c->k, c->m, ent, bkt, tcnt, tw, edges, n_edges, excl_cnt, excl, excl_cap);
+    t_done = t_ready;
+  }
+  launch_p1_finish(st, c->n_pos, bkt, tcnt, tw, part, sums, hist, hist_all, tot, n_edges, excl_cnt, excl, excl_cap, excl_cnt + 1);
+  c->sync();
+  c->host_packed = true;
   c->stage = 2;
 }
 
@@ -1018,6 +1141,11 @@ void cf1g_destroy(cf1g_ctx* c) {
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->st_copy) (void)hipStreamSynchronize(c->st_copy);
+  for (auto& e : c->slot_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->slot_mem) (void)hipHostFree(c->slot_mem);
+  if (c->st_copy) (void)hipStreamDestroy(c->st_copy);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1046,41 +1174,14 @@ int cf1g_get_info(cf1g_ctx* c, cf1g_info* o) {
   });
 }
 
-// Chunk edges (stream positions) of the loaded records, plus a UINT64_MAX
-// sentinel, into edges_h and the device buffer "edges".
-void set_chunk_edges(cf1g_ctx* c) {
-  constexpr uint64_t CHUNK_BASES = 4000000;  // src/minimizer.rs:13, src/superkmer.rs:16
-  c->edges_h.clear();
-  for (uint64_t r = 0; r < c->n_rec; r++) {
-    const uint64_t len = c->gstart_h[r + 1] - 1 - c->gstart_h[r];
-    for (uint64_t a = CHUNK_BASES; a < len; a += CHUNK_BASES) c->edges_h.push_back(c->gstart_h[r] + a);
-  }
-  c->edges_h.push_back(UINT64_MAX);
-  hip_check(hipMemcpyAsync(c->getT<uint64_t>("edges", c->edges_h.size()), c->edges_h.data(), c->edges_h.size() * 8,
-                           hipMemcpyHostToDevice, c->st),
-            "H2D chunk edges");
-}
-
 int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec) {
   return guarded(c, [&]() {
     if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
-    if (!rec_off) fail(CF1G_E_INVALID, "rec_off is NULL");
-    for (uint64_t r = 0; r < n_rec; r++)
-      if (rec_off[r + 1] < rec_off[r]) fail(CF1G_E_INVALID, "rec_off must be non-decreasing");
+    set_records(c, ascii, rec_off, n_rec);
+    c->stage = 0;
     const uint64_t base = rec_off[0];
-    c->n_rec = n_rec;
-    c->n_ascii = rec_off[n_rec] - base;
-    if (c->n_ascii > 0 && !ascii) fail(CF1G_E_INVALID, "ascii is NULL");
     std::vector<uint64_t> off(n_rec + 1);
-    c->gstart_h.assign(n_rec + 1, 0);
-    for (uint64_t r = 0; r <= n_rec; r++) {
-      off[r] = rec_off[r] - base;
-      c->gstart_h[r] = off[r] + r + 1;
-    }
-    c->n_pos = c->n_ascii + n_rec + 1;
-    c->n_words64 = (c->n_pos + STREAM_PAD + 63) / 64;
-    c->host_records = c->host_packed = c->restored = false;
-    c->rec_file.assign(n_rec, 0);
+    for (uint64_t r = 0; r <= n_rec; r++) off[r] = rec_off[r] - base;
     uint8_t* da = c->getT<uint8_t>("ascii", c->n_ascii + 8);
     if (c->n_ascii) hip_check(hipMemcpyAsync(da, ascii + base, c->n_ascii, hipMemcpyHostToDevice, c->st), "H2D ascii");
     hip_check(hipMemcpyAsync(c->getT<uint64_t>("rec_off", n_rec + 1), off.data(), (n_rec + 1) * 8, hipMemcpyHostToDevice, c->st),
@@ -1198,6 +1299,14 @@ int cf1g_stream_export(cf1g_ctx* c, uint64_t* seq, uint64_t* nmask, uint64_t n_w
   });
 }
 
+int cf1g_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint32_t threads, uint64_t* out_hist) {
+  return guarded(c, [&]() {
+    if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
+    do_p1_stream(c, ascii, rec_off, n_rec, threads);
+    if (out_hist) c->d2h(out_hist, c->get("hist", CF1G_NUM_BUCKETS * 8), CF1G_NUM_BUCKETS * 8);
+  });
+}
+
 int cf1g_p1_histogram(cf1g_ctx* c, uint64_t* out_hist) {
   return guarded(c, [&]() {
     need(c, 1, "cf1g_p1_histogram");
@@ -1250,6 +1359,18 @@ int cf1g_run_p1_p4(cf1g_ctx* c, uint64_t* out_D) {
   return guarded(c, [&]() {
     need(c, 1, "cf1g_run_p1_p4");
     do_p1(c);
+    do_p2(c, nullptr, c->prm.num_bins);
+    do_p3(c);
+    do_p4(c);
+    if (out_D) *out_D = c->D;
+  });
+}
+
+int cf1g_run_p1_p4_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint32_t threads,
+                          uint64_t* out_D) {
+  return guarded(c, [&]() {
+    if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
+    do_p1_stream(c, ascii, rec_off, n_rec, threads);
     do_p2(c, nullptr, c->prm.num_bins);
     do_p3(c);
     do_p4(c);

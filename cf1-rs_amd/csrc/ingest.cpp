@@ -15,6 +15,8 @@
 #include <unistd.h>
 #include <zlib.h>
 
+#include <immintrin.h>
+
 #include <algorithm>
 #include <cstring>
 #include <functional>
@@ -222,40 +224,71 @@ static bool g_code_init = [] {
   return true;
 }();
 
-void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint64_t n_words64, uint64_t* seq, uint64_t* nmask,
-                 int threads) {
+namespace {
+// 64 bases of one record (a[0..63]) -> the two sequence words (32 bases
+// each, base 0 in bits 63..62) and the placeholder word (base 0 in bit 63).
+void pack64_table(const uint8_t* a, uint64_t& hw, uint64_t& lw, uint64_t& nm) {
+  hw = lw = nm = 0;
+  for (int i = 0; i < 32; i++) {
+    const uint64_t c0 = g_code[a[i]], c1 = g_code[a[i + 32]];
+    hw |= (c0 & 3u) << (62 - 2 * i);
+    lw |= (c1 & 3u) << (62 - 2 * i);
+    nm |= ((c0 >> 2) << (63 - i)) | ((c1 >> 2) << (31 - i));
+  }
+}
+// The same 32 bases at a time with AVX2. ((x >> 1) ^ (x >> 2)) & 3 is the
+// code of an ACGT byte of either case; the byte is ACGT iff its upper-cased
+// value equals the letter of that code (a byte shuffle of "ACGT"). Codes of
+// other bytes are cleared; pairs of multiply-adds with weights 64, 16, 4, 1
+// put four codes in a byte, MSB-first, and one byte shuffle gathers them.
+__attribute__((target("avx2"))) void pack32_avx2(const uint8_t* a, uint64_t& w, uint32_t& nbits) {
+  const __m256i v = _mm256_loadu_si256((const __m256i*)a);
+  const __m256i m3 = _mm256_set1_epi8(3);
+  const __m256i c0 = _mm256_and_si256(_mm256_xor_si256(_mm256_srli_epi16(v, 1), _mm256_srli_epi16(v, 2)), m3);
+  const __m256i lut = _mm256_setr_epi8('A', 'C', 'G', 'T', 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 'A', 'C', 'G', 'T', 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                       0, 0, 0);
+  const __m256i ok = _mm256_cmpeq_epi8(_mm256_and_si256(v, _mm256_set1_epi8((char)0xDF)), _mm256_shuffle_epi8(lut, c0));
+  const __m256i c = _mm256_and_si256(c0, ok);
+  const __m256i p16 = _mm256_maddubs_epi16(c, _mm256_set1_epi32(0x01041040));  // bytes 64, 16, 4, 1
+  const __m256i p32 = _mm256_madd_epi16(p16, _mm256_set1_epi16(1));
+  const __m256i g = _mm256_shuffle_epi8(p32, _mm256_setr_epi8(0, 4, 8, 12, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 4, 8, 12,
+                                                              -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1));
+  const uint64_t bytes = (uint64_t)(uint32_t)_mm256_extract_epi32(g, 0) | ((uint64_t)(uint32_t)_mm256_extract_epi32(g, 4) << 32);
+  w = __builtin_bswap64(bytes);
+  // placeholder bits MSB-first: the flags in reverse byte order, then their signs
+  const __m256i rev = _mm256_setr_epi8(15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3,
+                                       2, 1, 0);
+  nbits = ~(uint32_t)_mm256_movemask_epi8(_mm256_permute4x64_epi64(_mm256_shuffle_epi8(ok, rev), 0x4E));
+}
+__attribute__((target("avx2"))) void pack64_avx2(const uint8_t* a, uint64_t& hw, uint64_t& lw, uint64_t& nm) {
+  uint32_t n0, n1;
+  pack32_avx2(a, hw, n0);
+  pack32_avx2(a + 32, lw, n1);
+  nm = ((uint64_t)n0 << 32) | n1;
+}
+const bool g_avx2 = __builtin_cpu_supports("avx2");
+}  // namespace
+
+void pack_words(const uint8_t* ascii, const uint64_t* rec_off, const uint64_t* gstart, uint64_t n_rec, uint64_t w0, uint64_t w1,
+                uint64_t* seq, uint64_t* nmask) {
   (void)g_code_init;
-  threads = std::max(1, threads);
-  // gstart[r] = off[r] - off[0] + r + 1: stream position of record r's first
-  // base; position gstart[r+1] - 1 is the separator after record r
-  std::vector<uint64_t> gstart(n_rec + 1);
-  for (uint64_t r = 0; r <= n_rec; r++) gstart[r] = rec_off[r] - rec_off[0] + r + 1;
-  run_threads(threads, [&](int t) {
-    const uint64_t w0 = n_words64 * (uint64_t)t / threads, w1 = n_words64 * (uint64_t)(t + 1) / threads;
-    if (w0 >= w1) return;
-    uint64_t r = (uint64_t)(std::upper_bound(gstart.begin(), gstart.end(), w0 * 64) - gstart.begin());
-    r = r ? r - 1 : n_rec;  // record whose range [gstart[r], gstart[r+1]) holds the position, or none
-    if (w0 * 64 < gstart[0]) r = 0;
-    for (uint64_t w = w0; w < w1; w++) {
-      uint64_t hi_w = 0, lo_w = 0, nm = 0;
-      const uint64_t g0 = w * 64;
-      while (r < n_rec && g0 >= gstart[r + 1]) r++;
-      if (r < n_rec && g0 >= gstart[r] && g0 + 64 < gstart[r + 1]) {
-        // all 64 positions are bases of record r: straight from the bytes
-        const uint8_t* a = ascii + (rec_off[r] - rec_off[0]) + (g0 - gstart[r]);
-        for (int i = 0; i < 32; i++) {
-          const uint64_t c0 = g_code[a[i]], c1 = g_code[a[i + 32]];
-          hi_w |= (c0 & 3u) << (62 - 2 * i);
-          lo_w |= (c1 & 3u) << (62 - 2 * i);
-          nm |= ((c0 >> 2) << (63 - i)) | ((c1 >> 2) << (31 - i));
-        }
-        seq[2 * w] = hi_w;
-        seq[2 * w + 1] = lo_w;
-        nmask[w] = nm;
-        continue;
-      }
+  if (w0 >= w1) return;
+  auto pack64 = g_avx2 ? pack64_avx2 : pack64_table;
+  // the record whose range [gstart[r], gstart[r+1]) holds the position, or none
+  uint64_t r = (uint64_t)(std::upper_bound(gstart, gstart + n_rec + 1, w0 * 64) - gstart);
+  r = r ? r - 1 : n_rec;
+  if (w0 * 64 < gstart[0]) r = 0;
+  for (uint64_t w = w0; w < w1; w++) {
+    uint64_t hi_w = 0, lo_w = 0, nm = 0;
+    const uint64_t g0 = w * 64;
+    while (r < n_rec && g0 >= gstart[r + 1]) r++;
+    uint64_t* sq = seq + 2 * (w - w0);
+    if (r < n_rec && g0 >= gstart[r] && g0 + 64 < gstart[r + 1]) {
+      // all 64 positions are bases of record r: straight from the bytes
+      pack64(ascii + (rec_off[r] - rec_off[0]) + (g0 - gstart[r]), hi_w, lo_w, nm);
+    } else {
       for (int i = 0; i < 64; i++) {
-        const uint64_t g = w * 64 + (uint64_t)i;
+        const uint64_t g = g0 + (uint64_t)i;
         while (r < n_rec && g >= gstart[r + 1]) r++;
         uint32_t code = 4;
         if (r < n_rec && g >= gstart[r] && g + 1 < gstart[r + 1]) code = g_code[ascii[rec_off[r] - rec_off[0] + (g - gstart[r])]];
@@ -264,11 +297,86 @@ void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, 
         else lo_w |= b << (62 - 2 * (i - 32));
         nm |= (uint64_t)(code >> 2) << (63 - i);
       }
-      seq[2 * w] = hi_w;
-      seq[2 * w + 1] = lo_w;
-      nmask[w] = nm;
     }
+    sq[0] = hi_w;
+    sq[1] = lo_w;
+    nmask[w - w0] = nm;
+  }
+}
+
+void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint64_t n_words64, uint64_t* seq, uint64_t* nmask,
+                 int threads) {
+  threads = std::max(1, threads);
+  // gstart[r] = off[r] - off[0] + r + 1: stream position of record r's first
+  // base; position gstart[r+1] - 1 is the separator after record r
+  std::vector<uint64_t> gstart(n_rec + 1);
+  for (uint64_t r = 0; r <= n_rec; r++) gstart[r] = rec_off[r] - rec_off[0] + r + 1;
+  run_threads(threads, [&](int t) {
+    const uint64_t w0 = n_words64 * (uint64_t)t / threads, w1 = n_words64 * (uint64_t)(t + 1) / threads;
+    pack_words(ascii, rec_off, gstart.data(), n_rec, w0, w1, seq + 2 * w0, nmask + w0);
   });
+}
+
+// ---------------------------------------------------------------- host pool
+// Each run() is a job of its own (shared with the workers that take part):
+// a worker still leaving an earlier job only sees that job's exhausted
+// counter.
+struct HostPool::Job {
+  const std::function<void(uint64_t)>* fn;
+  uint64_t n;
+  std::atomic<uint64_t> next{0}, done{0};
+};
+HostPool::HostPool(int threads) {
+  for (int t = 1; t < threads; t++) th_.emplace_back([this] { worker(); });
+}
+HostPool::~HostPool() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  for (auto& x : th_) x.join();
+}
+void HostPool::drain(Job& j) {
+  for (;;) {
+    const uint64_t i = j.next.fetch_add(1);
+    if (i >= j.n) break;
+    (*j.fn)(i);
+    if (j.done.fetch_add(1) + 1 == j.n) {
+      std::lock_guard<std::mutex> g(mu_);
+      cv_done_.notify_all();
+    }
+  }
+}
+void HostPool::worker() {
+  uint64_t seen = 0;
+  for (;;) {
+    std::shared_ptr<Job> j;
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      j = job_;
+    }
+    if (j) drain(*j);
+  }
+}
+void HostPool::run(uint64_t n, const std::function<void(uint64_t)>& fn) {
+  if (n == 0) return;
+  auto j = std::make_shared<Job>();
+  j->fn = &fn;
+  j->n = n;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    job_ = j;
+    gen_++;
+  }
+  cv_.notify_all();
+  drain(*j);
+  std::unique_lock<std::mutex> g(mu_);
+  cv_done_.wait(g, [&] { return j->done.load() == n; });
+  job_.reset();
 }
 
 }  // namespace cf1g

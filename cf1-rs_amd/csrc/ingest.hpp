@@ -2,7 +2,12 @@
 #pragma once
 #include <stdint.h>
 
+#include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <string>
 #include <utility>
 #include <vector>
@@ -48,5 +53,30 @@ void parse_fasta(const std::vector<std::string>& paths, int threads, HostRecords
 // record's end a separator, positions past the last record padding.
 void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint64_t n_words64, uint64_t* seq, uint64_t* nmask,
                  int threads);
+// Words [w0, w1) of that stream into seq[0 .. 2(w1-w0)) and nmask[0 .. w1-w0);
+// gstart[r] = stream position of record r's first base (n_rec + 1 entries).
+void pack_words(const uint8_t* ascii, const uint64_t* rec_off, const uint64_t* gstart, uint64_t n_rec, uint64_t w0, uint64_t w1,
+                uint64_t* seq, uint64_t* nmask);
+
+// A fixed set of host threads: run(n, fn) calls fn(0 .. n-1) on them and on
+// the calling thread, and returns when every call has returned.
+class HostPool {
+ public:
+  explicit HostPool(int threads);
+  ~HostPool();
+  void run(uint64_t n, const std::function<void(uint64_t)>& fn);
+  int threads() const { return (int)th_.size() + 1; }
+
+ private:
+  struct Job;
+  void worker();
+  void drain(Job& j);
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::shared_ptr<Job> job_;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
 
 }  // namespace cf1g

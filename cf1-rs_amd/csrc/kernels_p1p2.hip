@@ -238,7 +238,7 @@ __device__ __forceinline__ uint64_t sk_scan_excl(uint64_t v, uint64_t* sh4, uint
 __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
 __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                   uint64_t n_pos, uint64_t n_tiles, int K, int m,
+                                                   uint64_t n_pos, uint64_t tile_begin, uint64_t n_tiles, int K, int m,
                                                    uint64_t* __restrict__ entries, uint16_t* __restrict__ buckets,
                                                    uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ tile_w,
                                                    const uint64_t* __restrict__ edges, uint32_t n_edges,
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
   const int w = K - m + 1;
   const uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
   uint8_t* brk8 = (uint8_t*)S.brk;
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  for (uint64_t tile = tile_begin + blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const uint64_t t0 = 1 + tile * SK_T;  // first k-mer position of the tile
     const uint64_t B = t0 - 1;            // tile slot 0 <-> stream position B
     if (threadIdx.x == 0) {  // first edge >= t0 (edges are >= 4M apart: at most one in the tile's reach)
@@ -905,15 +905,29 @@ uint64_t p1_slices(uint64_t n_pos) {
   const uint64_t nt = p1_tiles(n_pos);
   return nt < (uint64_t)HB_SLICES ? nt : (uint64_t)HB_SLICES;
 }
-void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m, uint64_t* entries,
-               uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
-               unsigned long long* hist_all, unsigned long long* totals, const uint64_t* edges, uint32_t n_edges,
-               uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap, unsigned int* flag) {
+// P1 scan of tiles [t_begin, t_end) (a tile reads the stream up to
+// p1_tile_reach(t_end - 1) positions; the streaming load scans each range as
+// its words arrive).
+void launch_p1_scan(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, uint64_t t_begin, uint64_t t_end, int K,
+                    int m, uint64_t* entries, uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, const uint64_t* edges,
+                    uint32_t n_edges, uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap) {
+  if (t_end <= t_begin) return;
+  const uint64_t nt = t_end - t_begin;
+  const uint64_t grid = nt < 4096 ? nt : 4096;
+  CF1G_LAUNCH("p1_scan", k_p1_scan, dim3((unsigned)grid), dim3(SK_WG), 0, st, seq, nmask, n_pos, t_begin, t_end, K, m, entries, buckets,
+              tile_cnt, tile_w, edges, n_edges, excl_cnt, excl, excl_cap);
+}
+// First stream position past what the scan of tile t reads (orders over
+// [t0-1, t0+T+2w) from 64-position windows, validity from two placeholder
+// words per step, each start's minimizer m-mer).
+uint64_t p1_tile_reach(uint64_t t) { return t * SK_T + SK_T + 2 * SK_MAXW + 512; }
+// The histogram from every scanned tile: partial counts per slice, their sum,
+// then the chunk-edge exclusions.
+void launch_p1_finish(hipStream_t st, uint64_t n_pos, uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part,
+                      uint64_t* sums, unsigned long long* hist, unsigned long long* hist_all, unsigned long long* totals,
+                      uint32_t n_edges, uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap, unsigned int* flag) {
   const uint64_t nt = p1_tiles(n_pos);
   if (nt == 0) return;
-  const uint64_t grid = nt < 4096 ? nt : 4096;
-  CF1G_LAUNCH("p1_scan", k_p1_scan, dim3((unsigned)grid), dim3(SK_WG), 0, st, seq, nmask, n_pos, nt, K, m, entries, buckets, tile_cnt,
-              tile_w, edges, n_edges, excl_cnt, excl, excl_cap);
   const uint64_t slices = p1_slices(n_pos);
   const uint64_t per = (nt + slices - 1) / slices;
   const uint64_t used = (nt + per - 1) / per;
@@ -921,6 +935,15 @@ void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint6
               part, sums);
   CF1G_LAUNCH("p1_reduce", k_p1_reduce, dim3(65536 / 64), dim3(256), 0, st, part, sums, (uint32_t)used, hist, hist_all, totals);
   if (n_edges) CF1G_LAUNCH("p1_excl", k_p1_excl, dim3(1), dim3(256), 0, st, excl, excl_cnt, excl_cap, hist, flag);
+}
+void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m, uint64_t* entries,
+               uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
+               unsigned long long* hist_all, unsigned long long* totals, const uint64_t* edges, uint32_t n_edges,
+               uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap, unsigned int* flag) {
+  const uint64_t nt = p1_tiles(n_pos);
+  if (nt == 0) return;
+  launch_p1_scan(st, seq, nmask, n_pos, 0, nt, K, m, entries, buckets, tile_cnt, tile_w, edges, n_edges, excl_cnt, excl, excl_cap);
+  launch_p1_finish(st, n_pos, buckets, tile_cnt, tile_w, part, sums, hist, hist_all, totals, n_edges, excl_cnt, excl, excl_cap, flag);
 }
 // Route chunking: chunk_tiles tiles per chunk, one route work-group each.
 void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint64_t* n_chunks) {

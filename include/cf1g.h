@@ -110,6 +110,16 @@ int cf1g_stream_export(cf1g_ctx* ctx, uint64_t* seq, uint64_t* nmask, uint64_t n
  * NULL; else it receives CF1G_NUM_BUCKETS counts. */
 int cf1g_p1_histogram(cf1g_ctx* ctx, uint64_t* out_hist);
 
+/* P1 straight from host records — cf1g_load_records + cf1g_p1_histogram in
+ * one call, with the records' transfer overlapped: `threads` host threads
+ * (0 = all) pack chunks of the 2-bit stream into pinned staging buffers, each
+ * chunk's copy to HBM overlaps the packing of the next, and the P1 scan runs
+ * on the tiles whose bases have landed. The same records (ascii, rec_off as
+ * for cf1g_load_records, read only during the call) give the same histogram
+ * and the same later products. What the reference's P1 reads from the FASTA
+ * records (src/minimizer.rs:27-98). out_hist may be NULL. */
+int cf1g_p1_stream(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint32_t threads, uint64_t* out_hist);
+
 /* P2 — partition_minimizers (src/minimizer.rs:116-144) + route_superkmers
  * (src/superkmer.rs:21-101). hist == NULL uses P1's device histogram;
  * otherwise the caller's CF1G_NUM_BUCKETS counts are partitioned. Optional
@@ -181,6 +191,11 @@ int cf1g_autotune(cf1g_ctx* ctx, uint32_t reps, cf1g_tune_entry* entries, uint32
 
 /* P1..P4 in order on the loaded records; out_D may be NULL. */
 int cf1g_run_p1_p4(cf1g_ctx* ctx, uint64_t* out_D);
+/* P1..P4 from host records: cf1g_p1_stream, then P2..P4 (run_pipeline's
+ * phases from records in host memory to the state vector,
+ * src/pipeline.rs:77-103). */
+int cf1g_run_p1_p4_stream(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint32_t threads,
+                          uint64_t* out_D);
 
 /* ---- products consumed by the host P5 */
 
