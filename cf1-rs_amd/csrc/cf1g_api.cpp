@@ -26,35 +26,34 @@ void launch_pack(hipStream_t, const uint8_t*, const uint64_t*, const uint64_t*, 
 uint64_t p1_tiles(uint64_t);
 uint64_t p1_tile_positions();
 uint64_t p1_slices(uint64_t);
-void launch_p1_scan(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t, int, int, uint64_t*, uint16_t*,
+void launch_p1_scan(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t, int, int, uint64_t*, uint32_t*,
                     uint32_t*, uint32_t*, const uint64_t*, uint32_t, uint32_t*, uint16_t*, uint32_t);
 uint64_t p1_tile_reach(uint64_t);
-void launch_p1_finish(hipStream_t, uint64_t, uint16_t*, uint32_t*, uint32_t*, uint32_t*, uint64_t*, unsigned long long*,
+void launch_p1_finish(hipStream_t, uint64_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint64_t*, unsigned long long*,
                       unsigned long long*, unsigned long long*, uint32_t, uint32_t*, uint16_t*, uint32_t, unsigned int*);
-void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint16_t*, uint32_t*, uint32_t*, uint32_t*,
+void launch_p1(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*,
                uint64_t*, unsigned long long*, unsigned long long*, unsigned long long*, const uint64_t*, uint32_t, uint32_t*,
                uint16_t*, uint32_t, unsigned int*);
 void p2_chunking(uint64_t, uint32_t, uint64_t*, uint64_t*);
 uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
                       uint64_t*, unsigned long long*, uint64_t*, uint64_t*, uint64_t*);
-void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint16_t*, const uint32_t*, uint64_t, uint64_t,
-                     uint64_t, int,
-                     const uint16_t*, int, const uint64_t*, unsigned long long*, uint64_t*, int, unsigned int*, unsigned long long*);
+void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, const uint32_t*, uint64_t, uint64_t,
+                     uint64_t, int, const uint16_t*, int, const uint64_t*, unsigned long long*, uint64_t*, uint16_t*, int,
+                     unsigned int*, unsigned long long*);
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
 void launch_zero(hipStream_t, void*, uint64_t);
 void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*, uint64_t*);
-int p3_part_bits(int);
-void launch_p3_count(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int, uint64_t*,
-                     uint64_t*);
-void launch_p3_scatter(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int, int,
-                       const uint64_t*, void*, uint16_t*, const unsigned int*);
-void launch_p3_split(hipStream_t, bool, const void*, const uint16_t*, const uint64_t*, uint32_t, int, void*, uint16_t*,
-                     const unsigned int*);
+void launch_p3_rcount(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int,
+                      int, unsigned long long*, unsigned long long*);
+void launch_p3_rscatter(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t,
+                        int, const uint64_t*, unsigned int*, uint64_t*, const unsigned int*);
+void launch_p3_expand(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, uint64_t, int, void*,
+                      uint16_t*, const unsigned int*);
 void launch_p3_gate(hipStream_t, const uint64_t*, uint64_t, unsigned int*);
-void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, uint64_t, int, int, uint64_t*, uint64_t*, uint32_t*,
-                     unsigned int*, uint32_t*, uint64_t*, void*, uint16_t*);
+void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*,
+                     uint64_t*, uint32_t*, unsigned int*, uint32_t*, uint64_t*, uint16_t*);
 void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
                        unsigned int*, uint32_t*);
 void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
@@ -379,7 +378,7 @@ void do_p1(cf1g_ctx* c) {
   uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
   uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
-  uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_positions() + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
   auto* hist_all = c->getT<unsigned long long>("hist_all", CF1G_NUM_BUCKETS);
@@ -464,7 +463,7 @@ void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, ui
   uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
   uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
-  uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_positions() + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
   const uint32_t n_edges = (uint32_t)(c->edges_h.size() - 1);
@@ -551,12 +550,13 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   uint64_t chunk_tiles = 0, n_chunks = 0;
   p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
   uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
+  uint16_t* rec_sub = c->getT<uint16_t>("rec_sub", c->S + 8);
   // one read after the route: the routed-record total, then the overflow flag
   auto* route_chk = c->getT<unsigned long long>("route_check", 2);
   auto* routed_d = route_chk;
   unsigned int* ovf = (unsigned int*)(route_chk + 1);
   const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
-  const uint16_t* bkt = c->getT<uint16_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  const uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_positions() + 1);
   const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nmask = c->getT<uint64_t>("nmask", c->n_words64 + 8);
@@ -564,8 +564,8 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
     hipStream_t s2 = c->st;
     c->zero(route_chk, 16);
     c->zero(bcur, (uint64_t)c->num_bins * 8);
-    launch_p2_route(s2, seq, nmask, ent, bkt, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec, c->rw,
-                    ovf, routed_d);
+    launch_p2_route(s2, seq, nmask, ent, bkt, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec,
+                    rec_sub, c->rw, ovf, routed_d);
   });
   unsigned long long chk[2] = {0, 0};
   c->d2h(chk, route_chk, 16);
@@ -573,13 +573,17 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->stage = 3;
 }
 
-// P3: count, scan, two-level scatter (keys + observation masks into parts,
-// then segments), per-segment dedup / mask join / BBHash into padded block
-// slots (keys and masks in local-rank order), scans of the distinct counts
-// (MPHF key offsets) and block counts — one segment up to here — then, for
-// the rare segments above LDS capacity, the global-memory sort / BBHash and
-// the scans again; then one compaction of the masks (MPHF-value order) and
-// blocks. The dense key array is compacted on first use (dense_keys).
+// P3: records and k-mers per MPHF segment (a segment is fixed by a super
+// k-mer's minimizer, so P2's records are grouped, not their k-mers), scans,
+// the records into their segments' runs, then per segment: its k-mers
+// decoded from the records, dedup / mask join / BBHash into padded block
+// slots (masks in local-rank order), scans of the distinct counts (MPHF key
+// offsets) and block counts — one segment up to here — then, for the rare
+// segments above LDS capacity, their occurrences expanded into global
+// memory, the global-memory sort / BBHash and the scans again; then one
+// compaction of the masks (MPHF-value order) and blocks. The dense key array
+// is built on first use (dense_keys) from the keys the dedup left per
+// occurrence.
 void do_p3(cf1g_ctx* c) {
   hipStream_t st = c->st;
   const size_t kb = c->key_bytes;
@@ -592,8 +596,11 @@ void do_p3(cf1g_ctx* c) {
   c->sb = sb;
   c->n_segs = (uint64_t)c->num_bins << sb;
   const uint64_t ns = c->n_segs;
-  uint64_t* counts = c->getT<uint64_t>("seg_counts", ns + 1);
+  auto* kcnt = c->getT<unsigned long long>("seg_counts", ns + 1);
+  auto* rcnt = c->getT<unsigned long long>("seg_rcounts", ns + 1);
   uint64_t* seg_off = c->getT<uint64_t>("seg_off", ns + 1);
+  uint64_t* roff = c->getT<uint64_t>("seg_roff", ns + 1);
+  unsigned int* rcur = c->getT<unsigned int>("seg_rcur", ns + 1);
   uint64_t* ucount = c->getT<uint64_t>("seg_ucount", ns + 1);
   uint64_t* key_off = c->getT<uint64_t>("seg_out_off", ns + 1);
   uint64_t* nblk = c->getT<uint64_t>("bbh_seg_nblk", ns + 1);
@@ -603,21 +610,15 @@ void do_p3(cf1g_ctx* c) {
   unsigned int* flags = c->getT<unsigned int>("p3_flags", 4);  // [0] n_over, [1] n_big, [2] error bits, [3] count gate
   uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
   const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
+  const uint16_t* rec_sub = c->getT<uint16_t>("rec_sub", c->S + 8);
+  uint64_t* srec = c->getT<uint64_t>("rec_seg", c->S * (uint64_t)c->rw + 1);
   const uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
   const uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
-  // per-(bin, part, chunk) totals -> scan -> each chunk's run in its part
-  const int pb = p3_part_bits(sb);
-  const uint64_t nm = ((uint64_t)1 << pb) * c->n_chunks;
-  uint64_t* pmat = c->getT<uint64_t>("p3_pmat", nm + 1);
-  uint64_t* poff = c->getT<uint64_t>("p3_poff", nm + 1);
-  uint64_t* tmp2 = c->getT<uint64_t>("scan_tmp2", scan_temp_words(nm + 1) + 16);
+  // occ: every occurrence's key (the dedup's output; the expanded
+  // occurrences of oversized segments); obs: masks of expanded occurrences;
+  // aux: the global-memory sort's other buffer
   void* occ = c->get("occ", c->W * kb + 16);
   uint16_t* occ_obs = c->getT<uint16_t>("occ_obs", c->W + 8);
-  // pass A target (parts) when it differs from the segments; it is free once
-  // pass B has run and then holds the distinct keys (kout); k_p3_bigsort
-  // uses it only for the oversized segments, which k_p3_dedup skips
-  void* aux = c->get("occ_aux", c->W * kb + 16);
-  uint16_t* aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
   // padded BBHash block slots: floor(seg_off/64) + 16*seg, floor(n/64) + 16 blocks each
   uint64_t* blocks_pad = c->getT<uint64_t>("blocks_pad", 8 * (c->W / 64 + 16 * (ns + 1)) + 8);
   uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
@@ -625,26 +626,24 @@ void do_p3(cf1g_ctx* c) {
     scan_excl_u64(st, ucount, key_off, ns, key_off + ns, tmp);
     scan_excl_u64(st, nblk, blk_off, ns, blk_off + ns, tmp);
   };
+  const bool direct = !c->wide && 2 * K <= 12;  // k <= 6: the dedup's direct-address table reads expanded occurrences
   const std::vector<uint64_t> shape = {c->S, c->W, c->n_chunks, c->num_bins, (uint64_t)sb, (uint64_t)K, (uint64_t)c->rw};
-  // count and scans, then (unless the gate finds the counted occurrences
-  // differ from W) the two-level scatter and the dedup: one launch sequence,
-  // checked afterwards with the flags
+  // counts and scans, then (unless the gate finds the counted occurrences
+  // differ from W) the records into segments and the dedup: one launch
+  // sequence, checked afterwards with the flags
   c->seg("p3ab", shape, [&] {
-    c->zero(counts, (ns + 1) * 8);
+    c->zero(kcnt, (ns + 1) * 8);
+    c->zero(rcnt, (ns + 1) * 8);
+    c->zero(rcur, (ns + 1) * 4);
     c->zero(flags, 16);
-    launch_p3_count(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, counts, pmat);
-    scan_excl_u64(st, counts, seg_off, ns, seg_off + ns, tmp);
-    scan_excl_u64(st, pmat, poff, nm, poff + nm, tmp2);
+    launch_p3_rcount(st, c->wide, rec, rec_sub, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, rcnt, kcnt);
+    scan_excl_u64(st, (const uint64_t*)kcnt, seg_off, ns, seg_off + ns, tmp);
+    scan_excl_u64(st, (const uint64_t*)rcnt, roff, ns, roff + ns, tmp);
     launch_p3_gate(st, seg_off + ns, c->W, flags);
-    if (pb == sb) {
-      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, occ, occ_obs,
-                        flags + 3);
-    } else {
-      launch_p3_scatter(st, c->wide, rec, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, poff, aux, aux_obs,
-                        flags + 3);
-      launch_p3_split(st, c->wide, aux, aux_obs, seg_off, c->num_bins, sb, occ, occ_obs, flags + 3);
-    }
-    launch_p3_dedup(st, c->wide, occ, occ_obs, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, aux, vpad);
+    launch_p3_rscatter(st, c->wide, rec, rec_sub, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, sb, roff, rcur, srec,
+                       flags + 3);
+    if (direct) launch_p3_expand(st, c->wide, srec, roff, seg_off, nullptr, ns, K, occ, occ_obs, flags + 3);
+    launch_p3_dedup(st, c->wide, occ, occ_obs, srec, roff, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, vpad);
     scans();
   });
   unsigned int hf[4];
@@ -655,6 +654,9 @@ void do_p3(cf1g_ctx* c) {
   if (hf[3]) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
   c->n_over = hf[0];
   if (hf[0] > 0) {
+    void* aux = c->get("occ_aux", c->W * kb + 16);
+    uint16_t* aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
+    launch_p3_expand(st, c->wide, srec, roff, seg_off, over, hf[0], K, occ, occ_obs, flags + 3);
     launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over, hf[0], ucount, flags, big);
     c->d2h(hf, flags, 12);
   }
@@ -667,7 +669,7 @@ void do_p3(cf1g_ctx* c) {
     const uint64_t alive_words = c->W / 64 + 2;
     uint64_t* alive = c->getT<uint64_t>("bbh_alive", alive_words);
     hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
-    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, aux, flags + 2);
+    launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, occ, flags + 2);
   }
   if (hf[0] || c->n_big) {
     scans();
@@ -684,7 +686,7 @@ void do_p3(cf1g_ctx* c) {
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
   uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
   // masks and blocks now; the dense key array (which the reference's MPHF
-  // does not keep) is compacted from occ_aux on first use, dense_keys()
+  // does not keep) is built from occ on first use, dense_keys()
   c->seg("p3c", {ns, D, c->n_blocks}, [&] {
     launch_p3_compact(st, c->wide, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, vmask, blocks);
     c->zero(states, (nsw + 1) * 8);
@@ -697,8 +699,8 @@ void do_p3(cf1g_ctx* c) {
 }
 
 // The distinct keys in MPHF value order (keys[v] has value v), built on
-// first use from the per-segment key runs P3 left in occ_aux (each key placed
-// by its BBHash value).
+// first use from the per-segment key runs P3 left in occ (each key placed by
+// its BBHash value).
 void* dense_keys(cf1g_ctx* c) {
   if (c->restored) fail(CF1G_E_ORDER, "a restored MPHF keeps no keys (run P1..P3 for them)");
   const uint64_t kb = c->key_bytes;
@@ -707,7 +709,7 @@ void* dense_keys(cf1g_ctx* c) {
     const uint64_t ns = c->n_segs;
     unsigned int* err = c->getT<unsigned int>("keys_err", 2);
     c->zero(err, 4);
-    launch_p3_keys(c->st, c->wide, c->get("occ_aux", c->W * kb + 16), c->getT<uint64_t>("seg_off", ns + 1),
+    launch_p3_keys(c->st, c->wide, c->get("occ", c->W * kb + 16), c->getT<uint64_t>("seg_off", ns + 1),
                    c->getT<uint64_t>("seg_ucount", ns + 1), c->getT<uint64_t>("seg_out_off", ns + 1),
                    c->getT<uint64_t>("bbh_blk_off", ns + 1), c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8), ns, keys, err);
     unsigned int he = 0;

@@ -155,8 +155,10 @@ constexpr int BBH_MAX_LEVELS = 64;
 constexpr uint64_t BBH_BLOCK_BITS = 448;  // 7 words of bits + 1 rank word = one 64-byte line
 
 // ------------------------------------------------------------ MPHF (minimizer-partitioned BBHash)
-// The distinct keys are partitioned exactly as P3a left them: segment
-// (bin, seg_sub(key)), bin = bucket_to_bin[bucket_hash(minimizer value)].
+// The distinct keys are partitioned exactly as P3a left them: with h =
+// xxh3_64(minimizer value), segment (bin, s), bin = bucket_to_bin[h >> 48]
+// (its bucket) and s = the next sb bits of h, so every k-mer of a super
+// k-mer lies in one segment.
 // Each segment holds its own BBHash: level l has n_l = roundup64(max(2 t_l, 1))
 // bits with t_0 = D_s (the segment's key count) and t_{l+1} = ceil(9 t_l / 20);
 // a key lands at pos = bbh_pos(g_l(key), n_l) and is placed at the first
@@ -208,10 +210,14 @@ __device__ __forceinline__ uint32_t minimizer_value(const KeyT& c, int K, int m)
   }
   return bestv;
 }
+// Segment bits of a minimizer value: the sb bits of xxh3_64(v) just below
+// its 16 bucket bits.
+__device__ __forceinline__ uint32_t seg_sub_min(uint64_t h, int sb) { return sb ? (uint32_t)(h >> (48 - sb)) & ((1u << sb) - 1) : 0u; }
 template <typename KeyT>
 __device__ __forceinline__ uint64_t mphf_segment(const MphfView& v, const KeyT& c) {
-  uint32_t bin = v.b2b[bucket_hash((uint64_t)minimizer_value(c, v.K, v.m))];
-  return ((uint64_t)bin << v.sb) | seg_sub(c, v.sb);
+  const uint64_t h = xxh3_u64le((uint64_t)minimizer_value(c, v.K, v.m));
+  const uint32_t bin = v.b2b[h >> 48];
+  return ((uint64_t)bin << v.sb) | seg_sub_min(h, v.sb);
 }
 // Rank of a present key inside its segment (segment already known).
 template <typename KeyT>

@@ -239,7 +239,7 @@ __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b
 
 __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                    uint64_t n_pos, uint64_t tile_begin, uint64_t n_tiles, int K, int m,
-                                                   uint64_t* __restrict__ entries, uint16_t* __restrict__ buckets,
+                                                   uint64_t* __restrict__ entries, uint32_t* __restrict__ buckets,
                                                    uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ tile_w,
                                                    const uint64_t* __restrict__ edges, uint32_t n_edges,
                                                    uint32_t* __restrict__ excl_cnt, uint16_t* __restrict__ excl, uint32_t excl_cap) {
@@ -413,9 +413,12 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
       // the canonical m-mer at the minimizer slot, from the stream (as in step 1)
       const uint64_t x = stream_bits64(seq, 2 * (B + (uint64_t)S.pidx[PJ(j)]));
       const uint32_t f = (uint32_t)(x >> (64 - 2 * m)) & M, r = (uint32_t)rc64(x, 32) & M;
-      const uint32_t bk = bucket_hash((uint64_t)(f < r ? f : r));
+      const uint64_t h = xxh3_u64le((uint64_t)(f < r ? f : r));
+      const uint32_t bk = (uint32_t)(h >> 48);  // bucket_hash
       dst[i] = sk_entry(t0 + (uint64_t)(j - 1), n, bk);
-      buckets[tile * SK_T + i] = (uint16_t)bk;  // for the histogram, which needs only this
+      // the top 32 hash bits: the bucket (for the histogram and the route)
+      // and the 16 bits below it, which pick the MPHF segment in P3
+      buckets[tile * SK_T + i] = (uint32_t)(h >> 32);
       if ((uint64_t)j < jE && jE - (uint64_t)j < (uint64_t)K) {  // every k-mer spans the edge: not in P1's counts
         const uint32_t x = atomicAdd(excl_cnt, 1u);
         if (x < excl_cap) excl[x] = (uint16_t)bk;
@@ -429,8 +432,8 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
   }
 }
 
-// P1 histogram from the entries' buckets (a u16 copy the scan writes beside
-// each entry), without device atomics: work-group (slice,
+// P1 histogram from the entries' buckets (the top 16 of the 32 hash bits the
+// scan writes beside each entry), without device atomics: work-group (slice,
 // range r) counts the buckets in [r*16384, (r+1)*16384) of a slice of tiles
 // in LDS (each wave takes two tiles at a time) and writes its partial counts
 // to part[(slice*4 + r)*16384 ...]; the r = 0 work-groups write the slice's
@@ -438,7 +441,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
 constexpr int HB_RANGE = 16384;
 constexpr int HB_SLICES = 512;
 constexpr int HB_WG = 1024;  // 16 waves share one 64 KiB LDS range
-__global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint16_t* __restrict__ buckets, const uint32_t* __restrict__ tile_cnt,
+__global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint32_t* __restrict__ buckets, const uint32_t* __restrict__ tile_cnt,
                                                    const uint32_t* __restrict__ tile_w, uint64_t n_tiles, uint64_t tiles_per_slice,
                                                    uint32_t* __restrict__ part, uint64_t* __restrict__ sums) {
   constexpr int NW = HB_WG / 64;
@@ -463,8 +466,8 @@ __global__ void __launch_bounds__(HB_WG) k_p1_hist(const uint16_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const uint32_t i = i0 + (uint32_t)q * 64 + lane;
-        bk[q] = i < c1 ? (uint32_t)buckets[t * SK_T + i] : ~0u;
-        bk[4 + q] = i < c2 ? (uint32_t)buckets[t2 * SK_T + i] : ~0u;
+        bk[q] = i < c1 ? buckets[t * SK_T + i] >> 16 : ~0u;
+        bk[4 + q] = i < c2 ? buckets[t2 * SK_T + i] >> 16 : ~0u;
       }
 #pragma unroll
       for (int q = 0; q < 8; q++)
@@ -585,12 +588,13 @@ void launch_stream_digest(hipStream_t st, const uint64_t* seq, const uint64_t* n
 constexpr int RT_U = 8;
 template <int RW>
 __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
-                                                  const uint64_t* __restrict__ entries, const uint16_t* __restrict__ buckets,
+                                                  const uint64_t* __restrict__ entries, const uint32_t* __restrict__ buckets,
                                                   const uint32_t* __restrict__ tile_cnt, uint64_t n_tiles, uint64_t chunk_tiles, int K,
                                                   const uint16_t* __restrict__ b2b,
                                                   int num_bins, const uint64_t* __restrict__ bin_start,
                                                   unsigned long long* __restrict__ bin_cursor, uint64_t* __restrict__ rec,
-                                                  unsigned int* __restrict__ overflow, unsigned long long* __restrict__ routed) {
+                                                  uint16_t* __restrict__ rec_sub, unsigned int* __restrict__ overflow,
+                                                  unsigned long long* __restrict__ routed) {
   // per-bin bases and cursors of this chunk, num_bins of each (dynamic LDS)
   extern __shared__ __align__(8) uint8_t route_lds[];
   uint64_t* base = (uint64_t*)route_lds;
@@ -615,7 +619,7 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
     for (uint32_t i = threadIdx.x; i < cmax; i += 256) {
       uint32_t bk[RT_U];
 #pragma unroll
-      for (int u = 0; u < RT_U; u++) bk[u] = i < c[u] ? b2b[buckets[(t0 + u) * SK_T + i]] : ~0u;
+      for (int u = 0; u < RT_U; u++) bk[u] = i < c[u] ? b2b[buckets[(t0 + u) * SK_T + i] >> 16] : ~0u;
 #pragma unroll
       for (int u = 0; u < RT_U; u++)
         if (bk[u] != ~0u) atomicAdd(&cnt[bk[u]], 1u);
@@ -677,12 +681,16 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
         const uint64_t lf = is_n(nmask, pos - 1) ? 0 : 1, rf = is_n(nmask, pos + L) ? 0 : 1;
         wv[u][RW - 1] |= (uint64_t)L | (lf << 8) | (rf << 9);
       }
+      uint32_t hb[RT_U];
+#pragma unroll
+      for (int u = 0; u < RT_U; u++) hb[u] = en[u] != NONE ? buckets[(t0 + u) * SK_T + i] : 0u;
 #pragma unroll
       for (int u = 0; u < RT_U; u++) {
         if (en[u] == NONE) continue;
         uint64_t* dst = rec + out[u] * RW;
 #pragma unroll
         for (int q = 0; q < RW; q++) dst[q] = wv[u][q];
+        rec_sub[out[u]] = (uint16_t)hb[u];  // the segment bits below the bucket
       }
     }
   }
@@ -909,7 +917,7 @@ uint64_t p1_slices(uint64_t n_pos) {
 // p1_tile_reach(t_end - 1) positions; the streaming load scans each range as
 // its words arrive).
 void launch_p1_scan(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, uint64_t t_begin, uint64_t t_end, int K,
-                    int m, uint64_t* entries, uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, const uint64_t* edges,
+                    int m, uint64_t* entries, uint32_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, const uint64_t* edges,
                     uint32_t n_edges, uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap) {
   if (t_end <= t_begin) return;
   const uint64_t nt = t_end - t_begin;
@@ -923,7 +931,7 @@ void launch_p1_scan(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, 
 uint64_t p1_tile_reach(uint64_t t) { return t * SK_T + SK_T + 2 * SK_MAXW + 512; }
 // The histogram from every scanned tile: partial counts per slice, their sum,
 // then the chunk-edge exclusions.
-void launch_p1_finish(hipStream_t st, uint64_t n_pos, uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part,
+void launch_p1_finish(hipStream_t st, uint64_t n_pos, uint32_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part,
                       uint64_t* sums, unsigned long long* hist, unsigned long long* hist_all, unsigned long long* totals,
                       uint32_t n_edges, uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap, unsigned int* flag) {
   const uint64_t nt = p1_tiles(n_pos);
@@ -937,7 +945,7 @@ void launch_p1_finish(hipStream_t st, uint64_t n_pos, uint16_t* buckets, uint32_
   if (n_edges) CF1G_LAUNCH("p1_excl", k_p1_excl, dim3(1), dim3(256), 0, st, excl, excl_cnt, excl_cap, hist, flag);
 }
 void launch_p1(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, int m, uint64_t* entries,
-               uint16_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
+               uint32_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part, uint64_t* sums, unsigned long long* hist,
                unsigned long long* hist_all, unsigned long long* totals, const uint64_t* edges, uint32_t n_edges,
                uint32_t* excl_cnt, uint16_t* excl, uint32_t excl_cap, unsigned int* flag) {
   const uint64_t nt = p1_tiles(n_pos);
@@ -953,19 +961,19 @@ void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint6
   *chunk_tiles = ct;
   *n_chunks = (nt + ct - 1) / ct;
 }
-void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint16_t* buckets,
+void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint32_t* buckets,
                      const uint32_t* tile_cnt, uint64_t n_pos, uint64_t chunk_tiles, uint64_t n_chunks, int K, const uint16_t* b2b,
-                     int num_bins, const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, int rw,
+                     int num_bins, const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, uint16_t* rec_sub, int rw,
                      unsigned int* overflow, unsigned long long* routed) {
   if (n_chunks == 0) return;
   const uint64_t nt = p1_tiles(n_pos);
   const size_t lds = (size_t)num_bins * 12;
   if (rw == 2)
     CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, buckets, tile_cnt, nt,
-                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
+                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, rec_sub, overflow, routed);
   else
     CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, buckets, tile_cnt, nt,
-                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, overflow, routed);
+                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, rec_sub, overflow, routed);
 }
 uint64_t partition_scratch_words() { return PT_SCRATCH_WORDS; }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,

@@ -324,294 +324,161 @@ __device__ __forceinline__ bool p3_chunk(const uint64_t* __restrict__ chunk_off,
   return true;
 }
 
-// Two-level scatter of the occurrences. The 2^sb segments of a bin form
-// 2^pb parts of 2^(sb-pb) consecutive segments (pb = min(sb, P3_PART_BITS)).
-// k_p3_count counts k-mers per (bin, segment) and per (bin, part, chunk);
-// k_p3_scatter writes each chunk's occurrences into its (part, chunk) run
-// (runs of ~hundreds of keys, so the stores fill whole lines); k_p3_split
-// then moves each part's keys into its segments with one work-group per
-// part. A chunk is one work-group's record range of one bin.
-constexpr int P3_PART_BITS = 6;
+// ---- records grouped by MPHF segment. A record's segment inside its bin is
+// fixed by its minimizer: the sb bits just below the bucket in the
+// minimizer's hash (rec_sub holds the 16 hash bits below the bucket), so
+// every k-mer of a super k-mer lands in the same segment and P3 moves
+// records (16 or 32 bytes per super k-mer) instead of occurrences.
+__device__ __forceinline__ uint32_t rec_seg(uint32_t sub16, int sb) { return sb ? sub16 >> (16 - sb) : 0u; }
 
-template <typename KeyT, int RW>
-__global__ void __launch_bounds__(P3_WG) k_p3_count(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
-                                                    const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
-                                                    int K, int sb, int pb, uint64_t* __restrict__ counts, uint64_t* __restrict__ mat) {
-  extern __shared__ __align__(16) uint32_t hist[];  // 2^sb counters (dynamic)
+// Records and k-mers per (bin, segment), counted per chunk of a bin's records
+// in LDS (2^sb counters each, dynamic) and added to rcnt / kcnt.
+template <int RW>
+__global__ void __launch_bounds__(P3_WG) k_p3_rcount(const uint64_t* __restrict__ rec, const uint16_t* __restrict__ rec_sub,
+                                                     const uint64_t* __restrict__ bin_start, const uint64_t* __restrict__ chunk_off,
+                                                     int num_bins, uint32_t chunk_records, int K, int sb,
+                                                     unsigned long long* __restrict__ rcnt, unsigned long long* __restrict__ kcnt) {
+  extern __shared__ __align__(16) uint32_t rk_lds[];  // records, then k-mers, 2^sb each
   int bin;
   uint64_t r0, r1;
   if (!p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
   const int ns = 1 << sb;
-  for (int s = threadIdx.x; s < ns; s += P3_WG) hist[s] = 0;
+  uint32_t* rc = rk_lds;
+  uint32_t* kc = rk_lds + ns;
+  for (int i = threadIdx.x; i < 2 * ns; i += P3_WG) rk_lds[i] = 0;
   __syncthreads();
-  {
-    constexpr int OWN = OwnKmers<RW>::N;
-    __shared__ uint8_t own[P3_WG * OWN];
-    __shared__ uint16_t oo[P3_WG];
-    __shared__ uint64_t sh4[4];
-    __shared__ uint64_t srec[P3_WG * RW];  // the round's records, read once; the k-mer lanes read them here
-    const uint64_t mask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
-    for (uint64_t rr = r0; rr < r1; rr += P3_WG) {
-      const uint64_t r = rr + threadIdx.x;
-      int n = 0;
-      if (r < r1) {
-        uint64_t w[RW];
-#pragma unroll
-        for (int t = 0; t < RW; t++) w[t] = rec[r * RW + t];
-#pragma unroll
-        for (int t = 0; t < RW; t++) srec[threadIdx.x * RW + t] = w[t];
-        n = RW == 2 ? rec2_kmers(w[RW - 1], K) : rec4_kmers(w[RW - 1], K);
-      }
-      uint64_t T;
-      const uint32_t o = (uint32_t)block_excl_scan_u64((uint64_t)n, sh4, &T);
-      if (__syncthreads_or(n > OWN)) {
-        if (n) for_each_rec_occ<KeyT, RW, false>(rec, r, K, [&](const KeyT& c, uint32_t) { atomicAdd(&hist[seg_sub(c, sb)], 1u); });
-        continue;
-      }
-      oo[threadIdx.x] = (uint16_t)o;
-      for (int q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
-      __syncthreads();
-      for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += P3_WG) {
-        const uint32_t rl = own[g];
-        const int j = (int)(g - oo[rl]);
-        const uint64_t* w = srec + rl * RW;
-        uint32_t m;
-        KeyT c;
-        if constexpr (RW == 2) {
-          c = (KeyT)rec2_occ<false>(w[0], w[1], j, rec2_kmers(w[1], K), K, mask, m);
-        } else {
-          const Rec4 R{w[0], w[1], w[2], w[3]};
-          c = rec4_occ<false>(R, j, rec4_kmers(R.w3, K), K, m);
-        }
-        atomicAdd(&hist[seg_sub(c, sb)], 1u);
-      }
-      __syncthreads();
-    }
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += P3_WG) {
+    const uint32_t sg = rec_seg(rec_sub[r], sb);
+    const int n = rec_kmers_at<RW>(rec, r, K);
+    atomicAdd(&rc[sg], 1u);
+    atomicAdd(&kc[sg], (uint32_t)n);
   }
   __syncthreads();
-  uint64_t* dst = counts + (uint64_t)bin * ns;
-  for (int s = threadIdx.x; s < ns; s += P3_WG)
-    if (hist[s]) atomicAdd((unsigned long long*)&dst[s], (unsigned long long)hist[s]);
-  // per-part totals of this chunk at mat[P * chunk_off[bin] + part * chunks_of_bin + chunk]
-  const int np = 1 << pb, spp = 1 << (sb - pb);
-  const uint64_t c0 = chunk_off[bin], nch = chunk_off[bin + 1] - c0, c = (uint64_t)blockIdx.x - c0;
-  for (int p = threadIdx.x; p < np; p += P3_WG) {
-    uint64_t t = 0;
-    for (int q = 0; q < spp; q++) t += hist[p * spp + q];
-    mat[(uint64_t)np * c0 + (uint64_t)p * nch + c] = t;
+  const uint64_t s0 = (uint64_t)bin << sb;
+  for (int sg = threadIdx.x; sg < ns; sg += P3_WG) {
+    if (rc[sg]) {
+      atomicAdd(&rcnt[s0 + sg], (unsigned long long)rc[sg]);
+      atomicAdd(&kcnt[s0 + sg], (unsigned long long)kc[sg]);
+    }
   }
 }
 
-// Tiles staged in LDS and counting-sorted by destination before the stores,
-// so that consecutive lanes write consecutive addresses of one run.
-// (with its staged records a u64-key scatter work-group takes 35 KiB, four
-// per CU: 1.78 ms on the 300 Mbp genome, against 1.82 ms at five per CU
-// reading the records from global memory, 1.80 ms with a 1,408-element tile
-// at five per CU; the split tile is a multiple of P3_WG and keeps u64-key
-// work-groups at 24 KiB, six per CU, which measured best against 1,536 and
-// 1,280)
-template <typename KeyT>
-struct StageCfg {
-  static constexpr int TILE = sizeof(KeyT) == 8 ? 1920 : 1024;        // k_p3_scatter
-  static constexpr int SPLIT_TILE = sizeof(KeyT) == 8 ? 1792 : 1024;  // k_p3_split
+// Records of each chunk into their segments' runs: a tile of RS_TILE records
+// is ranked per segment in LDS, each (tile, segment) run is reserved from
+// the segment's cursor, and the tile is staged in segment order so that
+// consecutive lanes store consecutive records of one run. The order of
+// records inside a segment depends on the schedule; nothing downstream
+// depends on it (the dedup joins and ranks by key).
+template <int RW>
+struct RsCfg {
+  static constexpr int TILE = RW == 2 ? 1024 : 512;  // records per staged tile
 };
-
-// Within a staged tile of tn elements with destination ids dst_id[i] < nd:
-// cnt = per-id counts, inv lists the elements grouped by id (id order; the
-// order inside a group is arbitrary), and end[id] = the end of id's group in
-// inv (its start is end[id] - cnt[id]). Barriers inside; cnt must be zero on
-// entry.
-__device__ __forceinline__ void stage_sort(uint32_t tn, const uint8_t* dst_id, uint16_t* inv, uint32_t* cnt, uint32_t* end, int nd) {
-  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) atomicAdd(&cnt[dst_id[i]], 1u);
-  __syncthreads();
-  if (threadIdx.x < 64) {  // exclusive scan of <= 64 counts in one wave
-    const uint32_t v = (int)threadIdx.x < nd ? cnt[threadIdx.x] : 0u;
-    uint32_t x = v;
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(x, d, 64);
-      if ((int)threadIdx.x >= d) x += y;
-    }
-    if ((int)threadIdx.x < nd) end[threadIdx.x] = x - v;  // group start, advanced to its end below
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) inv[atomicAdd(&end[dst_id[i]], 1u)] = (uint16_t)i;
-  __syncthreads();
+template <int RW>
+size_t rscatter_lds(int sb) {
+  return (size_t)RsCfg<RW>::TILE * RW * 8 + (size_t)RsCfg<RW>::TILE * 2 + ((size_t)12 << sb) + 64;
 }
-
-// Pass A: each chunk's occurrences go to its (part, chunk) runs. Records are
-// taken 256 at a time (one per thread); their k-mers fill the LDS tile at the
-// scanned k-mer offsets (a round with more k-mers than a tile is generated
-// in tile windows), then the tile is sorted by part and stored.
-template <typename KeyT, int RW>
-__global__ void __launch_bounds__(P3_WG) k_p3_scatter(const uint64_t* __restrict__ rec, const uint64_t* __restrict__ bin_start,
-                                                      const uint64_t* __restrict__ chunk_off, int num_bins, uint32_t chunk_records,
-                                                      int K, int sb, int pb, const uint64_t* __restrict__ moff, KeyT* __restrict__ occ,
-                                                      uint16_t* __restrict__ occ_obs, const unsigned int* __restrict__ stop) {
-  constexpr int TILE = StageCfg<KeyT>::TILE;
-  __shared__ KeyT tk[TILE];
-  __shared__ uint16_t tm[TILE];
-  __shared__ uint8_t tp[TILE];
-  __shared__ uint16_t inv[TILE];
-  __shared__ uint32_t cnt[1 << P3_PART_BITS], end[1 << P3_PART_BITS];
-  __shared__ uint64_t base[1 << P3_PART_BITS];
+template <int RW>
+__global__ void __launch_bounds__(P3_WG) k_p3_rscatter(const uint64_t* __restrict__ rec, const uint16_t* __restrict__ rec_sub,
+                                                       const uint64_t* __restrict__ bin_start, const uint64_t* __restrict__ chunk_off,
+                                                       int num_bins, uint32_t chunk_records, int sb, const uint64_t* __restrict__ roff,
+                                                       unsigned int* __restrict__ rcur, uint64_t* __restrict__ srec,
+                                                       const unsigned int* __restrict__ stop) {
+  constexpr int TILE = RsCfg<RW>::TILE;
+  constexpr int PT = TILE / P3_WG;
+  extern __shared__ __align__(16) uint64_t rs_lds[];
   __shared__ uint64_t sh4[4];
-  constexpr int OWN = OwnKmers<RW>::N;
-  __shared__ uint8_t own[P3_WG * OWN];
-  __shared__ uint16_t oo[P3_WG];
-  // the round's records, read once for the k-mer lanes (2-word records only:
-  // with 4-word ones the extra 8 KiB costs a work-group per CU, 3.15 -> 3.27 ms)
-  constexpr bool STAGE = RW == 2;
-  __shared__ uint64_t srec[STAGE ? P3_WG * RW : 1];
   int bin;
   uint64_t r0, r1;
   if (*stop || !p3_chunk(chunk_off, bin_start, num_bins, chunk_records, bin, r0, r1)) return;
-  const int np = 1 << pb, psh = sb - pb;
-  const uint64_t kmask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
-  const uint64_t c0 = chunk_off[bin], nch = chunk_off[bin + 1] - c0, c = (uint64_t)blockIdx.x - c0;
-  for (int p = threadIdx.x; p < np; p += P3_WG) {
-    cnt[p] = 0;
-    base[p] = moff[(uint64_t)np * c0 + (uint64_t)p * nch + c];
-  }
-  __syncthreads();
-  for (uint64_t rr = r0; rr < r1; rr += P3_WG) {
-    const uint64_t r = rr + threadIdx.x;
-    uint32_t n = 0;
-    if (r < r1) {
-      uint64_t w[RW];
-#pragma unroll
-      for (int t = 0; t < RW; t++) w[t] = rec[r * RW + t];
-      if constexpr (STAGE) {
-#pragma unroll
-        for (int t = 0; t < RW; t++) srec[threadIdx.x * RW + t] = w[t];
-      }
-      int L = (int)(w[RW - 1] & 255);
-      if (L > rec_cap(RW)) L = rec_cap(RW);
-      n = L >= K ? (uint32_t)(L - K + 1) : 0u;
-    }
-    uint64_t T;
-    const uint64_t o = block_excl_scan_u64(n, sh4, &T);
-    const bool per_kmer = !__syncthreads_or(n > OWN);
-    if (per_kmer) {
-      oo[threadIdx.x] = (uint16_t)o;
-      for (uint32_t q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
-    }
-    for (uint64_t w0 = 0; w0 < T; w0 += TILE) {
-      const uint32_t tn = (uint32_t)(T - w0 < (uint64_t)TILE ? T - w0 : (uint64_t)TILE);
-      if (per_kmer) {
-        __syncthreads();  // owner map complete
-        for (uint32_t t = threadIdx.x; t < tn; t += P3_WG) {
-          const uint32_t g = (uint32_t)w0 + t, rl = own[g];
-          uint32_t m;
-          const int j = (int)(g - oo[rl]);
-          KeyT key;
-          if constexpr (STAGE) {
-            const uint64_t* w = srec + rl * RW;
-            key = (KeyT)rec2_occ<true>(w[0], w[1], j, rec2_kmers(w[1], K), K, kmask, m);
-          } else {
-            key = rec_occ_at<KeyT, RW, true>(rec, rr + rl, j, K, kmask, m);
-          }
-          tk[t] = key;
-          tm[t] = (uint16_t)m;
-          tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
-        }
-      }
-      if (!per_kmer && n && o < w0 + tn && o + n > w0) {
-        uint64_t j = o;
-        for_each_rec_occ<KeyT, RW, true>(rec, r, K, [&](const KeyT& key, uint32_t m) {
-          if (j >= w0 && j < w0 + tn) {
-            const uint32_t t = (uint32_t)(j - w0);
-            tk[t] = key;
-            tm[t] = (uint16_t)m;
-            tp[t] = (uint8_t)(seg_sub(key, sb) >> psh);
-          }
-          j++;
-        });
-      }
-      __syncthreads();
-      stage_sort(tn, tp, inv, cnt, end, np);
-      for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
-        const uint32_t e = inv[i], p = tp[e];
-        const uint64_t at = base[p] + (i - (end[p] - cnt[p]));
-        occ[at] = tk[e];
-        occ_obs[at] = tm[e];
-      }
-      __syncthreads();
-      for (int p = threadIdx.x; p < np; p += P3_WG) {
-        base[p] += cnt[p];
-        cnt[p] = 0;
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Pass B: one work-group per (bin, part) moves the part's keys [seg_off[s0],
-// seg_off[s0 + spp]) from src to their segments in dst, a staged tile at a
-// time.
-template <typename KeyT>
-__global__ void __launch_bounds__(P3_WG) k_p3_split(const KeyT* __restrict__ src, const uint16_t* __restrict__ src_obs,
-                                                    const uint64_t* __restrict__ seg_off, int sb, int pb, KeyT* __restrict__ dst,
-                                                    uint16_t* __restrict__ dst_obs, const unsigned int* __restrict__ stop) {
-  constexpr int TILE = StageCfg<KeyT>::SPLIT_TILE;
-  constexpr int MAXS = 1 << (P3_MAX_SB - P3_PART_BITS);
-  if (*stop) return;
-  __shared__ KeyT tk[TILE];
-  __shared__ uint16_t tm[TILE];
-  __shared__ uint8_t tp[TILE];
-  __shared__ uint16_t inv[TILE];
-  __shared__ uint32_t cnt[MAXS], end[MAXS];
-  __shared__ uint64_t base[MAXS];
-  const int spp = 1 << (sb - pb);  // <= 64: the split runs only when sb > pb = P3_PART_BITS
-  const uint64_t s0 = (uint64_t)blockIdx.x * spp;  // first segment of the part (bin-major)
-  for (int t = threadIdx.x; t < spp; t += P3_WG) {
-    cnt[t] = 0;
-    base[t] = seg_off[s0 + t];
-  }
-  __syncthreads();
-  const uint64_t a = seg_off[s0], b = seg_off[s0 + spp];
-  const uint32_t mask = (uint32_t)spp - 1;
-  // the next tile is loaded into registers while the current one is sorted
-  // and stored
-  constexpr int PT = TILE / P3_WG;
-  KeyT nk[PT];
-  uint16_t nm[PT];
-  auto fetch = [&](uint64_t t0) {
+  const int ns = 1 << sb;
+  uint64_t* tr = rs_lds;                          // staged records, TILE * RW words
+  uint16_t* ts = (uint16_t*)(tr + TILE * RW);     // their segments
+  uint32_t* cnt = (uint32_t*)(ts + TILE);         // per segment: records in the tile
+  uint32_t* beg = cnt + ns;                       //   their first staged index
+  uint32_t* gb = beg + ns;                        //   their run's offset in the segment
+  const uint64_t s0 = (uint64_t)bin << sb;
+  const int per = ns > P3_WG ? ns / P3_WG : 1;    // counters per thread in the scan
+  for (uint64_t t0 = r0; t0 < r1; t0 += TILE) {
+    const uint32_t tn = (uint32_t)(r1 - t0 < (uint64_t)TILE ? r1 - t0 : (uint64_t)TILE);
+    for (int i = threadIdx.x; i < ns; i += P3_WG) cnt[i] = 0;
+    __syncthreads();
+    uint64_t w[PT][RW];
+    uint32_t sg[PT], rk[PT];
 #pragma unroll
     for (int q = 0; q < PT; q++) {
-      const uint64_t i = t0 + (uint64_t)q * P3_WG + threadIdx.x;
-      if (i < b) {
-        nk[q] = src[i];
-        nm[q] = src_obs[i];
+      const uint32_t i = (uint32_t)q * P3_WG + threadIdx.x;
+      sg[q] = 0;
+      rk[q] = 0;
+      if (i < tn) {
+#pragma unroll
+        for (int t = 0; t < RW; t++) w[q][t] = rec[(t0 + i) * RW + t];
+        sg[q] = rec_seg(rec_sub[t0 + i], sb);
+        rk[q] = atomicAdd(&cnt[sg[q]], 1u);
       }
     }
-  };
-  if (a < b) fetch(a);
-  for (uint64_t t0 = a; t0 < b; t0 += TILE) {
-    const uint32_t tn = (uint32_t)(b - t0 < (uint64_t)TILE ? b - t0 : (uint64_t)TILE);
+    __syncthreads();
+    // exclusive scan of the counts (per consecutive counters per thread) and
+    // one reservation per non-empty (tile, segment) run
+    uint32_t loc = 0;
+    const int a = (int)threadIdx.x * per;
+    if (a < ns)
+      for (int q = 0; q < per; q++) loc += cnt[a + q];
+    uint32_t run = (uint32_t)block_excl_scan_u64(loc, sh4, nullptr);
+    if (a < ns)
+      for (int q = 0; q < per; q++) {
+        const uint32_t c = cnt[a + q];
+        beg[a + q] = run;
+        gb[a + q] = c ? atomicAdd(&rcur[s0 + a + q], c) : 0u;
+        run += c;
+      }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < PT; q++) {
       const uint32_t i = (uint32_t)q * P3_WG + threadIdx.x;
       if (i < tn) {
-        tk[i] = nk[q];
-        tm[i] = nm[q];
-        tp[i] = (uint8_t)(seg_sub(nk[q], sb) & mask);
+        const uint32_t j = beg[sg[q]] + rk[q];
+#pragma unroll
+        for (int t = 0; t < RW; t++) tr[j * RW + t] = w[q][t];
+        ts[j] = (uint16_t)sg[q];
       }
     }
-    if (t0 + TILE < b) fetch(t0 + TILE);
     __syncthreads();
-    stage_sort(tn, tp, inv, cnt, end, spp);
     for (uint32_t i = threadIdx.x; i < tn; i += P3_WG) {
-      const uint32_t e = inv[i], t = tp[e];
-      const uint64_t at = base[t] + (i - (end[t] - cnt[t]));
-      dst[at] = tk[e];
-      dst_obs[at] = tm[e];
+      const uint32_t g = ts[i];
+      const uint64_t out = roff[s0 + g] + gb[g] + (i - beg[g]);
+#pragma unroll
+      for (int t = 0; t < RW; t++) srec[out * RW + t] = tr[i * RW + t];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < spp; t += P3_WG) {
-      base[t] += cnt[t];
-      cnt[t] = 0;
-    }
-    __syncthreads();
+  }
+}
+
+// Occurrences (canonical key, observation mask) of the listed segments'
+// records, in record order, at occ / obs[seg_off[seg] ...] (all segments
+// when list is null): the input of the global-memory sort and of the
+// direct-address dedup. One work-group per segment, 256 records at a time.
+template <typename KeyT, int RW>
+__global__ void __launch_bounds__(P3_WG) k_p3_expand(const uint64_t* __restrict__ srec, const uint64_t* __restrict__ roff,
+                                                     const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ list,
+                                                     int K, KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+                                                     const unsigned int* __restrict__ stop) {
+  __shared__ uint64_t sh4[4];
+  if (*stop) return;
+  const uint64_t seg = list ? (uint64_t)list[blockIdx.x] : (uint64_t)blockIdx.x;
+  const uint64_t ra = roff[seg], rb = roff[seg + 1];
+  uint64_t at = seg_off[seg];
+  for (uint64_t rr = ra; rr < rb; rr += P3_WG) {
+    const uint64_t r = rr + threadIdx.x;
+    const int n = r < rb ? rec_kmers_at<RW>(srec, r, K) : 0;
+    uint64_t T;
+    uint64_t o = at + block_excl_scan_u64((uint64_t)n, sh4, &T);
+    if (n)
+      for_each_rec_occ<KeyT, RW, true>(srec, r, K, [&](const KeyT& c, uint32_t m) {
+        occ[o] = c;
+        obs[o] = (uint16_t)m;
+        o++;
+      });
+    at += T;
   }
 }
 
@@ -907,11 +774,12 @@ struct DdOccupancy {
 // number of occurrences.
 template <typename KeyT>
 __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
-                                                    const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K2, int sb,
+                                                    const uint64_t* __restrict__ srec, const uint64_t* __restrict__ roff,
+                                                    const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K, int sb,
                                                     uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
                                                     uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
                                                     uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad,
-                                                    KeyT* __restrict__ kout, uint16_t* __restrict__ vpad) {
+                                                    uint16_t* __restrict__ vpad) {
   constexpr int WG = DD_WG;
   constexpr int KPT = DD_KPT;
   constexpr int CAP = DD_CAP;
@@ -924,13 +792,16 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   // and the tail list
   constexpr int T_WRANK = 2 * CAP, T_SEEN = T_WRANK + 4 * REG_W, T_LIST = T_SEEN + 4 * 8 * SEEN_W;
   constexpr int T_END = T_LIST + TAIL_KEYS * 18;
+  // the joined masks by element, from the join until they are staged in rank
+  // order (the level structures end below them)
+  constexpr int T_MJ = TAB_BYTES - 2 * CAP;
   // aux: region (level words), the block-scan and level-count words behind
   // it; then the staged masks (all of it)
   constexpr int A_SH = REG_W * 8, A_BSH = A_SH + 8 * (WG / 64);
   constexpr int A_END = A_BSH + (int)sizeof(BbhShared<WG>);
   constexpr int AUX_BYTES = A_END > CAP * 2 ? A_END : CAP * 2;
   static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
-  static_assert(T_END <= TAB_BYTES && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
+  static_assert(T_END <= T_MJ && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
   static_assert(REG_W / 7 + 2 <= SEEN_W && CAP <= 65536 && REG_W * 64 <= 65536 && T_SEEN % 8 == 0 && T_LIST % 8 == 0,
                 "LDS reuse fits");
   __shared__ __align__(16) uint8_t tab_b[TAB_BYTES];
@@ -942,6 +813,9 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   uint64_t* sh8 = (uint64_t*)(aux_b + A_SH);
   BbhShared<WG>& bsh = *(BbhShared<WG>*)(aux_b + A_BSH);
   uint32_t* tab = (uint32_t*)tab_b;
+  uint16_t* mj = (uint16_t*)(tab_b + T_MJ);
+  constexpr int RW = sizeof(KeyT) == 8 ? 2 : 4;
+  const int K2 = 2 * K;
   unsigned int* n_over = flags;
   unsigned int* n_big = flags + 1;
   unsigned int* err = flags + 2;
@@ -1004,11 +878,11 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < u) {
         key[s] = occ[start + e];
-        kout[start + e] = key[s];
+        mj[e] = obs[start + e];
         alive |= 1u << s;
       }
     }
-    for (uint64_t e = u + threadIdx.x; e < n; e += WG) kout[start + e] = key_pad<KeyT>();
+    for (uint64_t e = u + threadIdx.x; e < n; e += WG) occ[start + e] = key_pad<KeyT>();
   } else if (n > (uint64_t)CAP) {
     if (threadIdx.x == 0) {
       over_list[atomicAdd(n_over, 1u)] = (uint32_t)seg;
@@ -1023,14 +897,77 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     const uint32_t tmask = (1u << tb) - 1;
     uint32_t pend = 0;
     ns = (int)((n + WG - 1) / WG);
+    // the segment's k-mers and observation masks straight from its records:
+    // rounds of RR records, one lane per k-mer through an LDS owner map
+    // (built in the table's space before the table is cleared); element e
+    // is k-mer e of the segment in record order. Every key goes to occ (the
+    // table reads a representative's key there when fingerprints match, and
+    // the dense key array is built from it on request), every mask to mj;
+    // the slots then load their elements back (a decode per slot, unrolled,
+    // would hold every slot's record words at once: 86 VGPRs instead of 64).
+    {
+      constexpr int RR = RW == 2 ? 256 : 128;
+      constexpr int OWN = OwnKmers<RW>::N;
+      static_assert(RR * RW * 8 + RR * 2 + RR * OWN <= T_MJ, "decode scratch fits below the joined masks");
+      uint64_t* srl = (uint64_t*)tab_b;
+      uint16_t* oo = (uint16_t*)(tab_b + RR * RW * 8);
+      uint8_t* own = (uint8_t*)(oo + RR);
+      const uint64_t kmask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
+      const uint64_t ra = roff[seg], rb = roff[seg + 1];
+      uint64_t done = 0;
+      for (uint64_t rr = ra; rr < rb; rr += RR) {
+        const uint64_t r = rr + threadIdx.x;
+        int nk = 0;
+        if ((int)threadIdx.x < RR && r < rb) {
 #pragma unroll
-    for (int s = 0; s < KPT; s++) {
-      if (s >= ns) break;
-      const uint64_t e = (uint64_t)s * WG + threadIdx.x;
-      const bool v = e < n;
-      key[s] = v ? occ[start + e] : KeyT{};
-      msk[s] = v ? (uint32_t)obs[start + e] : 0u;
-      pend |= (uint32_t)v << s;
+          for (int t = 0; t < RW; t++) srl[threadIdx.x * RW + t] = srec[r * RW + t];
+          nk = rec_kmers_at<RW>(srec, r, K);
+        }
+        // one scan for the offsets and for records longer than the owner map
+        // (counted in the high half; such a segment takes the global-memory path)
+        uint64_t T;
+        const uint32_t o = (uint32_t)block_excl_scan<WG>((uint64_t)nk | ((uint64_t)(nk > OWN) << 32), sh8, &T);
+        if (T >> 32) {
+          if (threadIdx.x == 0) {
+            over_list[atomicAdd(n_over, 1u)] = (uint32_t)seg;
+            ucount[seg] = 0;
+            nblk[seg] = 0;
+          }
+          return;
+        }
+        if (nk) {
+          oo[threadIdx.x] = (uint16_t)o;
+          for (int q = 0; q < nk; q++) own[o + q] = (uint8_t)threadIdx.x;
+        }
+        __syncthreads();
+        for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += WG) {
+          const uint32_t rl = own[g];
+          const int j = (int)(g - oo[rl]);
+          const uint64_t* w = srl + rl * RW;
+          uint32_t m;
+          KeyT kk;
+          if constexpr (RW == 2) {
+            kk = (KeyT)rec2_occ<true>(w[0], w[1], j, rec2_kmers(w[1], K), K, kmask, m);
+          } else {
+            const Rec4 R{w[0], w[1], w[2], w[3]};
+            kk = rec4_occ<true>(R, j, rec4_kmers(R.w3, K), K, m);
+          }
+          occ[start + done + g] = kk;
+          mj[done + g] = (uint16_t)m;
+        }
+        __syncthreads();
+        done += T;
+      }
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if (s >= ns) break;
+        const uint64_t e = (uint64_t)s * WG + threadIdx.x;
+        const bool v = e < n;
+        key[s] = v ? occ[start + e] : KeyT{};
+        msk[s] = v ? (uint32_t)mj[e] : 0u;
+        pend |= (uint32_t)v << s;
+      }
+      __syncthreads();  // every element is loaded before the table (over mj) is cleared
     }
     for (uint32_t i = threadIdx.x; i <= tmask / 4; i += WG) ((uint4*)tab)[i] = make_uint4(0, 0, 0, 0);  // 2^tb >= 64 entries
     __syncthreads();
@@ -1068,19 +1005,22 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
       }
     }
     __syncthreads();  // every mask is joined
-    // joined masks wait in obs[start + element] (read already) until staging;
-    // each occurrence's key, or the pad for a duplicate, goes to kout now
+    // each representative's joined mask: read from the table, then (the
+    // table being free after the scan's barriers) kept by element in mj
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if (s >= ns) break;
-      const uint64_t e = (uint64_t)s * WG + threadIdx.x;
-      if ((alive >> s) & 1) obs[start + e] = (uint16_t)((tab[hs[s]] >> 12) & 0x3FFu);
-      if (e < n) kout[start + e] = ((alive >> s) & 1) ? key[s] : key_pad<KeyT>();
+      if ((alive >> s) & 1) msk[s] = (tab[hs[s]] >> 12) & 0x3FFu;
     }
     uint64_t tot;
     block_excl_scan<WG>((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab is free after it
     u = tot;
     if (threadIdx.x == 0) ucount[seg] = u;
+#pragma unroll
+    for (int s = 0; s < KPT; s++) {
+      if (s >= ns) break;
+      if ((alive >> s) & 1) mj[s * WG + threadIdx.x] = (uint16_t)msk[s];
+    }
   }
   // ---- BBHash of the segment, into its padded block slot.
   uint64_t* region = (uint64_t*)aux_b;
@@ -1151,7 +1091,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     if (s >= ns) break;
-    if (rr[s] != ~0u) stage_m[rr[s]] = obs[start + (uint64_t)s * WG + threadIdx.x];
+    if (rr[s] != ~0u) stage_m[rr[s]] = mj[s * WG + threadIdx.x];
   }
   __syncthreads();
   for (uint64_t i = threadIdx.x; i < u; i += WG) vpad[start + i] = stage_m[i];
@@ -1494,42 +1434,38 @@ static inline unsigned grid_for(uint64_t n, unsigned wg, unsigned cap) {
   return (unsigned)g;
 }
 
-int p3_part_bits(int sb) { return sb < P3_PART_BITS ? sb : P3_PART_BITS; }
-void launch_p3_count(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
-                     int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, uint64_t* counts, uint64_t* mat) {
+void launch_p3_rcount(hipStream_t st, bool wide, const uint64_t* rec, const uint16_t* rec_sub, const uint64_t* bin_start,
+                      const uint64_t* chunk_off, int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb,
+                      unsigned long long* rcnt, unsigned long long* kcnt) {
   if (n_chunks == 0) return;
-  const int pb = p3_part_bits(sb);
-  const size_t lds = (size_t)4 << sb;  // the histogram of the bin's 2^sb segments
+  const size_t lds = (size_t)8 << sb;
   if (!wide)
-    CF1G_LAUNCH("p3_count", (k_p3_count<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), lds, st, rec, bin_start, chunk_off,
-                num_bins, chunk_records, K, sb, pb, counts, mat);
+    CF1G_LAUNCH("p3_rcount", k_p3_rcount<2>, dim3((unsigned)n_chunks), dim3(P3_WG), lds, st, rec, rec_sub, bin_start, chunk_off,
+                num_bins, chunk_records, K, sb, rcnt, kcnt);
   else
-    CF1G_LAUNCH("p3_count", (k_p3_count<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), lds, st, rec, bin_start, chunk_off,
-                num_bins, chunk_records, K, sb, pb, counts, mat);
+    CF1G_LAUNCH("p3_rcount", k_p3_rcount<4>, dim3((unsigned)n_chunks), dim3(P3_WG), lds, st, rec, rec_sub, bin_start, chunk_off,
+                num_bins, chunk_records, K, sb, rcnt, kcnt);
 }
-void launch_p3_scatter(hipStream_t st, bool wide, const uint64_t* rec, const uint64_t* bin_start, const uint64_t* chunk_off,
-                       int num_bins, uint32_t chunk_records, uint64_t n_chunks, int K, int sb, const uint64_t* moff, void* occ,
-                       uint16_t* occ_obs, const unsigned int* stop) {
+void launch_p3_rscatter(hipStream_t st, bool wide, const uint64_t* rec, const uint16_t* rec_sub, const uint64_t* bin_start,
+                        const uint64_t* chunk_off, int num_bins, uint32_t chunk_records, uint64_t n_chunks, int sb, const uint64_t* roff,
+                        unsigned int* rcur, uint64_t* srec, const unsigned int* stop) {
   if (n_chunks == 0) return;
-  const int pb = p3_part_bits(sb);
   if (!wide)
-    CF1G_LAUNCH("p3_scatter", (k_p3_scatter<uint64_t, 2>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (uint64_t*)occ, occ_obs, stop);
+    CF1G_LAUNCH("p3_rscatter", k_p3_rscatter<2>, dim3((unsigned)n_chunks), dim3(P3_WG), rscatter_lds<2>(sb), st, rec, rec_sub,
+                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, stop);
   else
-    CF1G_LAUNCH("p3_scatter", (k_p3_scatter<Key128, 4>), dim3((unsigned)n_chunks), dim3(P3_WG), 0, st, rec, bin_start,
-                chunk_off, num_bins, chunk_records, K, sb, pb, moff, (Key128*)occ, occ_obs, stop);
+    CF1G_LAUNCH("p3_rscatter", k_p3_rscatter<4>, dim3((unsigned)n_chunks), dim3(P3_WG), rscatter_lds<4>(sb), st, rec, rec_sub,
+                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, stop);
 }
-void launch_p3_split(hipStream_t st, bool wide, const void* src, const uint16_t* src_obs, const uint64_t* seg_off, uint32_t num_bins,
-                     int sb, void* dst, uint16_t* dst_obs, const unsigned int* stop) {
-  const int pb = p3_part_bits(sb);
-  const uint64_t parts = (uint64_t)num_bins << pb;
-  if (parts == 0) return;
+void launch_p3_expand(hipStream_t st, bool wide, const uint64_t* srec, const uint64_t* roff, const uint64_t* seg_off,
+                      const uint32_t* list, uint64_t n, int K, void* occ, uint16_t* obs, const unsigned int* stop) {
+  if (n == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_split", k_p3_split<uint64_t>, dim3((unsigned)parts), dim3(P3_WG), 0, st, (const uint64_t*)src, src_obs, seg_off,
-                sb, pb, (uint64_t*)dst, dst_obs, stop);
+    CF1G_LAUNCH("p3_expand", (k_p3_expand<uint64_t, 2>), dim3((unsigned)n), dim3(P3_WG), 0, st, srec, roff, seg_off, list, K,
+                (uint64_t*)occ, obs, stop);
   else
-    CF1G_LAUNCH("p3_split", k_p3_split<Key128>, dim3((unsigned)parts), dim3(P3_WG), 0, st, (const Key128*)src, src_obs, seg_off, sb,
-                pb, (Key128*)dst, dst_obs, stop);
+    CF1G_LAUNCH("p3_expand", (k_p3_expand<Key128, 4>), dim3((unsigned)n), dim3(P3_WG), 0, st, srec, roff, seg_off, list, K,
+                (Key128*)occ, obs, stop);
 }
 // Compares the scanned occurrence total with W (the valid k-mers P1 counted)
 // before the occurrence pass: on a mismatch flags[3] is set, and scatter,
@@ -1540,16 +1476,16 @@ __global__ void k_p3_gate(const uint64_t* __restrict__ total, uint64_t expect, u
 void launch_p3_gate(hipStream_t st, const uint64_t* total, uint64_t expect, unsigned int* flags) {
   CF1G_LAUNCH("p3_gate", k_p3_gate, dim3(1), dim3(1), 0, st, total, expect, flags);
 }
-void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* seg_off, uint64_t n_segs, int K, int sb,
-                     uint64_t* ucount, uint64_t* nblk, uint32_t* over_list, unsigned int* flags, uint32_t* big_list,
-                     uint64_t* blocks_pad, void* kout, uint16_t* vpad) {
+void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* srec, const uint64_t* roff,
+                     const uint64_t* seg_off, uint64_t n_segs, int K, int sb, uint64_t* ucount, uint64_t* nblk, uint32_t* over_list,
+                     unsigned int* flags, uint32_t* big_list, uint64_t* blocks_pad, uint16_t* vpad) {
   if (n_segs == 0) return;
   if (!wide)
-    CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (uint64_t*)occ, obs, seg_off, n_segs,
-                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, (uint64_t*)kout, vpad);
+    CF1G_LAUNCH("p3_dedup", k_p3_dedup<uint64_t>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (uint64_t*)occ, obs, srec, roff, seg_off,
+                n_segs, K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
   else
-    CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (Key128*)occ, obs, seg_off, n_segs,
-                2 * K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, (Key128*)kout, vpad);
+    CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (Key128*)occ, obs, srec, roff, seg_off,
+                n_segs, K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
 }
 void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void* aux, uint16_t* aux_obs, const uint64_t* seg_off,
                        const uint32_t* over_list, uint32_t n_over, uint64_t* ucount, unsigned int* flags, uint32_t* big_list) {

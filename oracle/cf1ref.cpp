@@ -101,13 +101,15 @@ void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi,
   if (bin_start[bins] != lo.size()) throw std::runtime_error("MPHF: per-bin counts do not cover the keys");
   std::vector<uint64_t> nlo(lo.size()), nhi(hi.size());
   std::vector<uint64_t> cnt(ns + 1, 0);
+  std::vector<uint16_t> sub(lo.size());  // segment bits of each key's minimizer
   parallel_for(bins, threads, [&](size_t b) {
     const uint64_t a = bin_start[b], e = bin_start[b + 1];
     uint64_t* c = cnt.data() + (b << sb);
     for (uint64_t i = a; i < e; i++) {
       if (i > a && !(((u128)hi[i - 1] << 64 | lo[i - 1]) < ((u128)hi[i] << 64 | lo[i])))
         throw std::runtime_error("MPHF: keys of a bin are not sorted and distinct");
-      c[seg_sub(lo[i], hi[i], out.wide, sb)]++;
+      sub[i] = (uint16_t)(sb ? seg_sub_min(minimizer_value(((u128)hi[i] << 64) | lo[i], K, m), sb) : 0);
+      c[sub[i]]++;
     }
     std::vector<uint64_t> at(1ULL << sb);
     uint64_t run = a;
@@ -116,7 +118,7 @@ void mphf_build(Mphf& out, std::vector<uint64_t>& lo, std::vector<uint64_t>& hi,
       run += c[s];
     }
     for (uint64_t i = a; i < e; i++) {
-      uint64_t d = at[seg_sub(lo[i], hi[i], out.wide, sb)]++;
+      uint64_t d = at[sub[i]]++;
       nlo[d] = lo[i];
       nhi[d] = hi[i];
     }

@@ -434,10 +434,12 @@ inline uint32_t mmer_order(uint32_t canon) { return fmix32(canon + 0x9E3779B9u);
 // CPU and GPU; exact values parity-unpinned against ph; graph output is
 // invariant to the MPHF once unitig ids are canonicalised) is a
 // minimizer-partitioned BBHash:
-//   segment of key c = (bin, s): bin = bucket_to_bin[bucket_hash(v(c))] with
-//   v(c) the minimizer value of c; s = top sb bits of seg_hash(c) =
-//   (lo ^ hi * 0xC2B2AE3D27D4EB4F) * 0x9E3779B97F4A7C15 (a Fibonacci hash),
-//   which spreads every bin evenly over its 2^sb segments. Segments are
+//   segment of key c = (bin, s): with h = xxh3_64(v(c)) (v(c) the minimizer
+//   value of c, the hash whose top 16 bits are its bucket), bin =
+//   bucket_to_bin[h >> 48] and s = the sb bits of h just below the bucket,
+//   (h >> (48 - sb)) & (2^sb - 1): every k-mer of a super k-mer shares its
+//   segment, and a bin's minimizers spread evenly over its 2^sb segments.
+//   Segments are
 //   numbered (bin << sb) | s and hold consecutive value ranges key_off[seg]..;
 //   the distinct keys are stored in value order (keys[v] has value v).
 //   sb = fewest bits (<= 12) with W <= bins * 2^sb * target
@@ -523,7 +525,8 @@ inline int segment_bits(uint64_t W, uint64_t bins, uint64_t target) {
   while (sb < 12 && W > bins * (1ULL << sb) * target) sb++;
   return sb;
 }
-inline uint64_t seg_sub(uint64_t lo, uint64_t hi, bool wide, int sb) { return sb ? seg_hash(lo, hi, wide) >> (64 - sb) : 0; }
+// The segment bits of a minimizer value (below its bucket bits).
+inline uint64_t seg_sub_min(uint32_t v, int sb) { return sb ? (xxh3_64_u64le(v) >> (48 - sb)) & ((1ULL << sb) - 1) : 0; }
 
 struct Mphf {
   int K = 0, m = 0, sb = 0;
@@ -534,7 +537,7 @@ struct Mphf {
   uint64_t segment(u128 c) const {
     uint32_t v = minimizer_value(c, K, m);
     uint64_t bin = b2b[bucket_hash(v)];
-    return (bin << sb) | seg_sub((uint64_t)c, (uint64_t)(c >> 64), wide, sb);
+    return (bin << sb) | seg_sub_min(v, sb);
   }
   uint64_t lookup(uint64_t lo, uint64_t hi) const {
     u128 c = ((u128)hi << 64) | lo;
