@@ -55,7 +55,8 @@ class Cf1gError(RuntimeError):
 
 class CfParams(C.Structure):
     _fields_ = [("k", C.c_uint32), ("num_bins", C.c_uint32), ("m", C.c_uint32), ("p4_chunk", C.c_uint32),
-                ("p4_wgs_per_cu", C.c_uint32), ("p3_target_seg", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("p4_wgs_per_cu", C.c_uint32), ("p3_target_seg", C.c_uint32), ("stream_chunk_log2", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class CfInfo(C.Structure):
@@ -198,11 +199,11 @@ def minimizer_len(k):
 class Cf1g:
     """One device context: the P1..P4 phases on the loaded records."""
 
-    def __init__(self, k=31, num_bins=128, m=0, p4_chunk=0, p4_wgs_per_cu=0, p3_target_seg=0):
+    def __init__(self, k=31, num_bins=128, m=0, p4_chunk=0, p4_wgs_per_cu=0, p3_target_seg=0, stream_chunk_log2=0):
         self.L = lib()
         self.k = k
         p = CfParams(k=k, num_bins=num_bins, m=m, p4_chunk=p4_chunk, p4_wgs_per_cu=p4_wgs_per_cu,
-                     p3_target_seg=p3_target_seg)
+                     p3_target_seg=p3_target_seg, stream_chunk_log2=stream_chunk_log2)
         h = C.c_void_p()
         rc = self.L.cf1g_create(C.byref(p), C.byref(h))
         self.h = h

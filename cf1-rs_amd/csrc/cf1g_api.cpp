@@ -2,12 +2,15 @@
 // Mirrors the phase sequence of run_pipeline (src/pipeline.rs:66-118); all
 // intermediates (packed stream, bins, keys, MPHF, states) stay in HBM.
 #include <hip/hip_runtime.h>
+#include <immintrin.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -445,15 +448,23 @@ void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, ui
   set_records(c, ascii, rec_off, n_rec);
   c->stage = 0;
   set_chunk_edges(c);
-  const int th = threads ? (int)threads : std::max(1, (int)std::thread::hardware_concurrency());
+  // the calling thread issues the copies and launches; th - 1 pool threads pack
+  const int th = std::max(2, threads ? (int)threads : (int)std::thread::hardware_concurrency());
   if (!c->pool || c->pool->threads() != th) c->pool.reset(new HostPool(th));
   if (!c->st_copy) hip_check(hipStreamCreateWithFlags(&c->st_copy, hipStreamNonBlocking), "hipStreamCreate (copy)");
-  constexpr size_t CW = size_t(1) << 18;  // stream words per chunk (16.8 M positions, 6 MiB packed)
-  if (!c->slot_mem) {
+  // stream words per chunk (default 2^18: 16.8 M positions, 6 MiB packed)
+  const uint32_t cl = c->prm.stream_chunk_log2 ? c->prm.stream_chunk_log2 : 18;
+  if (cl < 10 || cl > 24) fail(CF1G_E_INVALID, "stream_chunk_log2 must be in [10, 24]");
+  const size_t CW = size_t(1) << cl;
+  if (c->slot_words != CW) {
+    if (c->slot_mem) hip_check(hipHostFree(c->slot_mem), "hipHostFree (staging)");
+    c->slot_mem = nullptr;
+    c->slot_words = 0;
     hip_check(hipHostMalloc(&c->slot_mem, cf1g_ctx::N_SLOTS * 3 * CW * 8, hipHostMallocDefault), "hipHostMalloc (staging)");
     c->slot_words = CW;
-    for (auto& e : c->slot_ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
   }
+  if (!c->slot_ev[0])
+    for (auto& e : c->slot_ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
   const uint64_t nw = c->n_words64, nt = p1_tiles(c->n_pos), nsl = p1_slices(c->n_pos);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * nw + 8);
   uint64_t* nm = c->getT<uint64_t>("nmask", nw + 8);
@@ -481,17 +492,47 @@ void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, ui
   // the first records' offset is subtracted by pack_words (rec_off[0] is the base)
   const uint8_t* a = ascii ? ascii + rec_off[0] : (const uint8_t*)"";
   uint64_t t_done = 0;
-  const uint64_t parts = (uint64_t)th * 2;
-  for (uint64_t w0 = 0, i = 0; w0 < nw; w0 += CW, i++) {
-    const uint64_t w1 = std::min<uint64_t>(nw, w0 + CW);
-    const int s = (int)(i % cf1g_ctx::N_SLOTS);
+  // The pool packs every chunk's parts in order, each chunk into staging slot
+  // chunk % N_SLOTS once that slot's previous copy has completed
+  // (slot_free[s] = the first chunk that may use slot s); the calling thread
+  // waits for each chunk's parts, issues its copies and the scan of its
+  // tiles, and frees the slot when the copy is done.
+  constexpr int NS = cf1g_ctx::N_SLOTS;
+  const uint64_t n_chunks = (nw + CW - 1) / CW;
+  const uint64_t parts = (uint64_t)(th - 1) * 2;
+  std::unique_ptr<std::atomic<uint64_t>[]> left(new std::atomic<uint64_t>[n_chunks + 1]);
+  for (uint64_t i = 0; i < n_chunks; i++) left[i].store(parts);
+  std::atomic<uint64_t> slot_free[NS];
+  for (int s = 0; s < NS; s++) slot_free[s].store((uint64_t)s);
+  std::atomic<bool> abandon{false};
+  const std::function<void(uint64_t)> pack_part = [&](uint64_t task) {
+    const uint64_t ci = task / parts, p = task % parts;
+    const int s = (int)(ci % NS);
+    while (slot_free[s].load(std::memory_order_acquire) < ci)
+      if (abandon.load(std::memory_order_relaxed)) return;
+    const uint64_t w0 = ci * CW, w1 = std::min<uint64_t>(nw, w0 + CW);
     uint64_t* sseq = (uint64_t*)c->slot_mem + (size_t)s * 3 * CW;
     uint64_t* snm = sseq + 2 * CW;
-    if (i >= (uint64_t)cf1g_ctx::N_SLOTS) hip_check(hipEventSynchronize(c->slot_ev[s]), "hipEventSynchronize (staging slot)");
-    c->pool->run(parts, [&](uint64_t p) {
-      const uint64_t a0 = w0 + (w1 - w0) * p / parts, a1 = w0 + (w1 - w0) * (p + 1) / parts;
-      pack_words(a, rec_off, c->gstart_h.data(), n_rec, a0, a1, sseq + 2 * (a0 - w0), snm + (a0 - w0));
-    });
+    const uint64_t a0 = w0 + (w1 - w0) * p / parts, a1 = w0 + (w1 - w0) * (p + 1) / parts;
+    pack_words(a, rec_off, c->gstart_h.data(), n_rec, a0, a1, sseq + 2 * (a0 - w0), snm + (a0 - w0));
+    left[ci].fetch_sub(1, std::memory_order_release);
+  };
+  c->pool->start(n_chunks * parts, pack_part);
+  // a HIP error below must not leave the pool waiting on a slot forever
+  struct Join {
+    HostPool* pool;
+    std::atomic<bool>* abandon;
+    ~Join() {
+      abandon->store(true);
+      pool->wait();
+    }
+  } join{c->pool.get(), &abandon};
+  for (uint64_t i = 0; i < n_chunks; i++) {
+    const uint64_t w0 = i * CW, w1 = std::min<uint64_t>(nw, w0 + CW);
+    const int s = (int)(i % NS);
+    uint64_t* sseq = (uint64_t*)c->slot_mem + (size_t)s * 3 * CW;
+    uint64_t* snm = sseq + 2 * CW;
+    while (left[i].load(std::memory_order_acquire) != 0) _mm_pause();
     hip_check(hipMemcpyAsync(seq + 2 * w0, sseq, 2 * (w1 - w0) * 8, hipMemcpyHostToDevice, c->st_copy), "H2D stream chunk");
     hip_check(hipMemcpyAsync(nm + w0, snm, (w1 - w0) * 8, hipMemcpyHostToDevice, c->st_copy), "H2D placeholder chunk");
     hip_check(hipEventRecord(c->slot_ev[s], c->st_copy), "hipEventRecord");
@@ -505,6 +546,8 @@ void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, ui
     }
     launch_p1_scan(st, seq, nm, c->n_pos, t_done, t_ready, c->k, c->m, ent, bkt, tcnt, tw, edges, n_edges, excl_cnt, excl, excl_cap);
     t_done = t_ready;
+    hip_check(hipEventSynchronize(c->slot_ev[s]), "hipEventSynchronize (staging slot)");
+    slot_free[s].store(i + NS, std::memory_order_release);
   }
   launch_p1_finish(st, c->n_pos, bkt, tcnt, tw, part, sums, hist, hist_all, tot, n_edges, excl_cnt, excl, excl_cap, excl_cnt + 1);
   c->sync();

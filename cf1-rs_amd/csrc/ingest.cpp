@@ -348,22 +348,25 @@ void HostPool::drain(Job& j) {
     }
   }
 }
+// Workers spin briefly on the job counter before sleeping: a pipeline that
+// hands out a job every ~0.1 ms (the streaming load's chunks) would
+// otherwise pay a condition-variable wake-up per job and thread.
 void HostPool::worker() {
   uint64_t seen = 0;
   for (;;) {
+    for (int spin = 0; spin < 200000 && gen_.load(std::memory_order_acquire) == seen; spin++) _mm_pause();
     std::shared_ptr<Job> j;
     {
       std::unique_lock<std::mutex> g(mu_);
-      cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+      cv_.wait(g, [&] { return stop_ || gen_.load() != seen; });
       if (stop_) return;
-      seen = gen_;
+      seen = gen_.load();
       j = job_;
     }
     if (j) drain(*j);
   }
 }
-void HostPool::run(uint64_t n, const std::function<void(uint64_t)>& fn) {
-  if (n == 0) return;
+void HostPool::start(uint64_t n, const std::function<void(uint64_t)>& fn) {
   auto j = std::make_shared<Job>();
   j->fn = &fn;
   j->n = n;
@@ -373,10 +376,24 @@ void HostPool::run(uint64_t n, const std::function<void(uint64_t)>& fn) {
     gen_++;
   }
   cv_.notify_all();
-  drain(*j);
+}
+void HostPool::wait() {
   std::unique_lock<std::mutex> g(mu_);
-  cv_done_.wait(g, [&] { return j->done.load() == n; });
+  if (!job_) return;
+  std::shared_ptr<Job> j = job_;
+  cv_done_.wait(g, [&] { return j->done.load() == j->n; });
   job_.reset();
+}
+void HostPool::run(uint64_t n, const std::function<void(uint64_t)>& fn) {
+  if (n == 0) return;
+  start(n, fn);
+  std::shared_ptr<Job> j;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    j = job_;
+  }
+  drain(*j);
+  wait();
 }
 
 }  // namespace cf1g

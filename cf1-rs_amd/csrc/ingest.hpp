@@ -59,12 +59,16 @@ void pack_words(const uint8_t* ascii, const uint64_t* rec_off, const uint64_t* g
                 uint64_t* seq, uint64_t* nmask);
 
 // A fixed set of host threads: run(n, fn) calls fn(0 .. n-1) on them and on
-// the calling thread, and returns when every call has returned.
+// the calling thread, and returns when every call has returned; start(n, fn)
+// hands the calls to the pool's threads alone and returns at once (fn must
+// stay alive until wait() returns). Calls are taken in index order.
 class HostPool {
  public:
   explicit HostPool(int threads);
   ~HostPool();
   void run(uint64_t n, const std::function<void(uint64_t)>& fn);
+  void start(uint64_t n, const std::function<void(uint64_t)>& fn);
+  void wait();
   int threads() const { return (int)th_.size() + 1; }
 
  private:
@@ -75,7 +79,7 @@ class HostPool {
   std::mutex mu_;
   std::condition_variable cv_, cv_done_;
   std::shared_ptr<Job> job_;
-  uint64_t gen_ = 0;
+  std::atomic<uint64_t> gen_{0};
   bool stop_ = false;
 };
 

@@ -44,7 +44,8 @@ typedef struct cf1g_params {
   uint32_t p4_chunk;       /* P4 work-stealing chunk (k-mer positions); 0 = tuned */
   uint32_t p4_wgs_per_cu;  /* P4 persistent work-groups per CU; 0 = tuned */
   uint32_t p3_target_seg;  /* P3 target k-mers per sort segment; 0 = tuned */
-  uint32_t reserved[2];
+  uint32_t stream_chunk_log2; /* cf1g_p1_stream: log2 of the stream words (64 positions) per copied chunk; 0 = 18 */
+  uint32_t reserved;
 } cf1g_params;
 
 typedef struct cf1g_info {
