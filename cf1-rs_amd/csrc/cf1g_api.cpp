@@ -42,7 +42,7 @@ uint64_t partition_scratch_words();
 void launch_partition(hipStream_t, const unsigned long long*, const unsigned long long*, uint32_t, uint32_t, uint16_t*, uint64_t*,
                       uint64_t*, unsigned long long*, uint64_t*, uint64_t*, uint64_t*);
 void launch_p2_route(hipStream_t, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, const uint32_t*, uint64_t, uint64_t,
-                     uint64_t, int, const uint16_t*, int, const uint64_t*, unsigned long long*, uint64_t*, uint16_t*, int,
+                     uint64_t, int, const uint16_t*, int, const uint64_t*, unsigned long long*, uint64_t*, uint16_t*, uint32_t*, int,
                      unsigned int*, unsigned long long*);
 // kernels_p3.hip
 uint64_t scan_temp_words(uint64_t n);
@@ -51,7 +51,11 @@ void scan_excl_u64(hipStream_t, const uint64_t*, uint64_t*, uint64_t, uint64_t*,
 void launch_p3_rcount(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t, int,
                       int, unsigned long long*, unsigned long long*);
 void launch_p3_rscatter(hipStream_t, bool, const uint64_t*, const uint16_t*, const uint64_t*, const uint64_t*, int, uint32_t, uint64_t,
-                        int, const uint64_t*, unsigned int*, uint64_t*, const unsigned int*);
+                        int, const uint64_t*, unsigned int*, uint64_t*, uint32_t*, const unsigned int*);
+void launch_p5_annot(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
+                     const uint64_t*, const uint64_t*, int, uint64_t, uint8_t*, unsigned int*);
+void launch_p5_list(hipStream_t, const uint64_t*, const uint32_t*, const uint32_t*, uint64_t, uint32_t, const uint32_t*, uint64_t,
+                    uint64_t*, unsigned int*);
 void launch_p3_expand(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, const uint32_t*, uint64_t, int, void*,
                       uint16_t*, const unsigned int*);
 void launch_p3_gate(hipStream_t, const uint64_t*, uint64_t, unsigned int*);
@@ -93,6 +97,7 @@ void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_
 void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint64_t,
                      uint8_t*, unsigned int*);
 void launch_lookup(hipStream_t, bool, const void*, uint64_t, const MphfView&, uint64_t*, bool);
+
 uint64_t span_tiles(uint64_t);
 void launch_span_count(hipStream_t, const uint8_t*, uint64_t, uint64_t*, uint64_t*);
 void launch_span_emit(hipStream_t, const uint8_t*, uint64_t, const uint64_t*, const uint64_t*, uint64_t*, uint64_t*);
@@ -587,6 +592,7 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   c->n_chunks = hinfo[2];
   c->W = htot[0];
   if (c->S != htot[1]) fail(CF1G_E_INTERNAL, "bin record total differs from the P1 super k-mer count");
+  if (c->S >= (1ULL << 32)) fail(CF1G_E_INVALID, "more than 2^32 super k-mers (record indices are 32-bit)");
   // route: each chunk counts its records per bin, reserves a run from each
   // bin's cursor and writes them
   const uint64_t nt = p1_tiles(c->n_pos);
@@ -594,6 +600,8 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
   uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   uint16_t* rec_sub = c->getT<uint16_t>("rec_sub", c->S + 8);
+  const uint64_t nslots = nt * p1_tile_positions();
+  uint32_t* ent_dest = c->getT<uint32_t>("ent_dest", nslots + 1);
   // one read after the route: the routed-record total, then the overflow flag
   auto* route_chk = c->getT<unsigned long long>("route_check", 2);
   auto* routed_d = route_chk;
@@ -608,7 +616,7 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
     c->zero(route_chk, 16);
     c->zero(bcur, (uint64_t)c->num_bins * 8);
     launch_p2_route(s2, seq, nmask, ent, bkt, tcnt, c->n_pos, chunk_tiles, n_chunks, c->k, b2b, (int)c->num_bins, bstart, bcur, rec,
-                    rec_sub, c->rw, ovf, routed_d);
+                    rec_sub, ent_dest, c->rw, ovf, routed_d);
   });
   unsigned long long chk[2] = {0, 0};
   c->d2h(chk, route_chk, 16);
@@ -655,6 +663,7 @@ void do_p3(cf1g_ctx* c) {
   const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   const uint16_t* rec_sub = c->getT<uint16_t>("rec_sub", c->S + 8);
   uint64_t* srec = c->getT<uint64_t>("rec_seg", c->S * (uint64_t)c->rw + 1);
+  uint32_t* rec_dest = c->getT<uint32_t>("rec_dest", c->S + 1);
   const uint64_t* bstart = c->getT<uint64_t>("bin_start", CF1G_MAX_BINS + 1);
   const uint64_t* coff = c->getT<uint64_t>("chunk_off", CF1G_MAX_BINS + 1);
   // occ: every occurrence's key (the dedup's output; the expanded
@@ -684,7 +693,7 @@ void do_p3(cf1g_ctx* c) {
     scan_excl_u64(st, (const uint64_t*)rcnt, roff, ns, roff + ns, tmp);
     launch_p3_gate(st, seg_off + ns, c->W, flags);
     launch_p3_rscatter(st, c->wide, rec, rec_sub, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, sb, roff, rcur, srec,
-                       flags + 3);
+                       rec_dest, flags + 3);
     if (direct) launch_p3_expand(st, c->wide, srec, roff, seg_off, nullptr, ns, K, occ, occ_obs, flags + 3);
     launch_p3_dedup(st, c->wide, occ, occ_obs, srec, roff, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, vpad);
     scans();
@@ -1072,6 +1081,25 @@ uint8_t* annotate_device(cf1g_ctx* c) {
   uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
   if (c->use_index == 1) {
     launch_annotate_lidx(c->st, seq, nm, c->n_pos, c->k, c->lview(), c->view(), c->getT<uint64_t>("states_lidx", nsw + 1), d);
+  } else if (!c->restored) {
+    // by segment: each segment's records (P3 left them grouped, with their
+    // stream positions) against the segment's own blocks (no lookup per
+    // position); positions without a valid k-mer stay 0
+    const uint64_t ns = c->n_segs, nt = p1_tiles(c->n_pos);
+    unsigned int* err = c->getT<unsigned int>("annot_err", 2);
+    uint64_t* list = c->getT<uint64_t>("p5_list", c->S + 1);
+    c->zero(err, 8);
+    c->zero(d, c->n_pos);
+    launch_p5_list(c->st, c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1),
+                   c->getT<uint32_t>("ent_dest", nt * p1_tile_positions() + 1), c->getT<uint32_t>("sk_tile_cnt", nt + 1), nt,
+                   (uint32_t)p1_tile_positions(), c->getT<uint32_t>("rec_dest", c->S + 1), c->S, list, err);
+    launch_p5_annot(c->st, c->wide, c->getT<uint64_t>("rec_seg", c->S * (uint64_t)c->rw + 1), list,
+                    c->getT<uint64_t>("seg_roff", ns + 1), c->getT<uint64_t>("seg_out_off", ns + 1),
+                    c->getT<uint64_t>("bbh_blk_off", ns + 1), c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8),
+                    c->getT<uint64_t>("states", nsw + 1), c->k, ns, d, err);
+    unsigned int he = 0;
+    c->d2h(&he, err, 4);
+    if (he) fail(CF1G_E_INTERNAL, "P5 annotation: a super k-mer or k-mer missing from its MPHF segment (flags " + std::to_string(he) + ")");
   } else {
     unsigned int* err = c->getT<unsigned int>("annot_err", 2);
     c->zero(err, 8);

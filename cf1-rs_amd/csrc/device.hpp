@@ -210,6 +210,9 @@ __device__ __forceinline__ uint32_t minimizer_value(const KeyT& c, int K, int m)
   }
   return bestv;
 }
+// A super k-mer's segment inside its bin from the 16 hash bits below its
+// bucket (bits 47..32 of xxh3_64(minimizer)): their top sb bits.
+__device__ __forceinline__ uint32_t rec_seg(uint32_t sub16, int sb) { return sb ? sub16 >> (16 - sb) : 0u; }
 // Segment bits of a minimizer value: the sb bits of xxh3_64(v) just below
 // its 16 bucket bits.
 __device__ __forceinline__ uint32_t seg_sub_min(uint64_t h, int sb) { return sb ? (uint32_t)(h >> (48 - sb)) & ((1u << sb) - 1) : 0u; }

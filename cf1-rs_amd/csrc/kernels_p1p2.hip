@@ -593,7 +593,8 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
                                                   const uint16_t* __restrict__ b2b,
                                                   int num_bins, const uint64_t* __restrict__ bin_start,
                                                   unsigned long long* __restrict__ bin_cursor, uint64_t* __restrict__ rec,
-                                                  uint16_t* __restrict__ rec_sub, unsigned int* __restrict__ overflow,
+                                                  uint16_t* __restrict__ rec_sub, uint32_t* __restrict__ ent_dest,
+                                                  unsigned int* __restrict__ overflow,
                                                   unsigned long long* __restrict__ routed) {
   // per-bin bases and cursors of this chunk, num_bins of each (dynamic LDS)
   extern __shared__ __align__(8) uint8_t route_lds[];
@@ -691,6 +692,7 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
 #pragma unroll
         for (int q = 0; q < RW; q++) dst[q] = wv[u][q];
         rec_sub[out[u]] = (uint16_t)hb[u];  // the segment bits below the bucket
+        ent_dest[(t0 + u) * SK_T + i] = (uint32_t)out[u];  // where the entry's record went (P5 finds it again)
       }
     }
   }
@@ -963,17 +965,18 @@ void p2_chunking(uint64_t n_pos, uint32_t num_bins, uint64_t* chunk_tiles, uint6
 }
 void launch_p2_route(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, const uint64_t* entries, const uint32_t* buckets,
                      const uint32_t* tile_cnt, uint64_t n_pos, uint64_t chunk_tiles, uint64_t n_chunks, int K, const uint16_t* b2b,
-                     int num_bins, const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, uint16_t* rec_sub, int rw,
+                     int num_bins, const uint64_t* bin_start, unsigned long long* bin_cursor, uint64_t* rec, uint16_t* rec_sub,
+                     uint32_t* ent_dest, int rw,
                      unsigned int* overflow, unsigned long long* routed) {
   if (n_chunks == 0) return;
   const uint64_t nt = p1_tiles(n_pos);
   const size_t lds = (size_t)num_bins * 12;
   if (rw == 2)
     CF1G_LAUNCH("p2_route", k_p2_route<2>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, buckets, tile_cnt, nt,
-                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, rec_sub, overflow, routed);
+                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, rec_sub, ent_dest, overflow, routed);
   else
     CF1G_LAUNCH("p2_route", k_p2_route<4>, dim3((unsigned)n_chunks), dim3(256), lds, st, seq, nmask, entries, buckets, tile_cnt, nt,
-                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, rec_sub, overflow, routed);
+                chunk_tiles, K, b2b, num_bins, bin_start, bin_cursor, rec, rec_sub, ent_dest, overflow, routed);
 }
 uint64_t partition_scratch_words() { return PT_SCRATCH_WORDS; }
 void launch_partition(hipStream_t st, const unsigned long long* hist, const unsigned long long* hist_true, uint32_t num_bins,

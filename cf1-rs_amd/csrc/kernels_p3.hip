@@ -329,7 +329,6 @@ __device__ __forceinline__ bool p3_chunk(const uint64_t* __restrict__ chunk_off,
 // minimizer's hash (rec_sub holds the 16 hash bits below the bucket), so
 // every k-mer of a super k-mer lands in the same segment and P3 moves
 // records (16 or 32 bytes per super k-mer) instead of occurrences.
-__device__ __forceinline__ uint32_t rec_seg(uint32_t sub16, int sb) { return sb ? sub16 >> (16 - sb) : 0u; }
 
 // Records and k-mers per (bin, segment), counted per chunk of a bin's records
 // in LDS (2^sb counters each, dynamic) and added to rcnt / kcnt.
@@ -382,7 +381,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_rscatter(const uint64_t* __restric
                                                        const uint64_t* __restrict__ bin_start, const uint64_t* __restrict__ chunk_off,
                                                        int num_bins, uint32_t chunk_records, int sb, const uint64_t* __restrict__ roff,
                                                        unsigned int* __restrict__ rcur, uint64_t* __restrict__ srec,
-                                                       const unsigned int* __restrict__ stop) {
+                                                       uint32_t* __restrict__ rec_dest, const unsigned int* __restrict__ stop) {
   constexpr int TILE = RsCfg<RW>::TILE;
   constexpr int PT = TILE / P3_WG;
   extern __shared__ __align__(16) uint64_t rs_lds[];
@@ -440,6 +439,7 @@ __global__ void __launch_bounds__(P3_WG) k_p3_rscatter(const uint64_t* __restric
 #pragma unroll
         for (int t = 0; t < RW; t++) tr[j * RW + t] = w[q][t];
         ts[j] = (uint16_t)sg[q];
+        rec_dest[t0 + i] = (uint32_t)(roff[s0 + sg[q]] + gb[sg[q]] + rk[q]);  // bin order -> segment order (for P5)
       }
     }
     __syncthreads();
@@ -479,6 +479,147 @@ __global__ void __launch_bounds__(P3_WG) k_p3_expand(const uint64_t* __restrict_
         o++;
       });
     at += T;
+  }
+}
+
+// ------------------------------------------------------------------ P5 annotation
+// What AnnotatedKmer::new computes per k-mer for P5 (src/directed_kmer.rs:81-96)
+// — valid bit, forward bit, class of the k-mer's state — without a lookup per
+// stream position: one work-group per MPHF segment decodes the k-mers of the
+// segment's records (P3 left them grouped by segment, each with its P1 entry:
+// stream position and length), finds each key's value through that
+// segment's BBHash blocks staged in LDS, and writes the annotation at the
+// k-mer's stream position.
+// The P1 entry of each record in segment order: the route kept where each
+// entry's record went (ent_dest, in entry order) and the record sort where
+// each record went (rec_dest), so list[rec_dest[ent_dest[slot]]] = entry.
+__global__ void __launch_bounds__(256) k_p5_list(const uint64_t* __restrict__ entries, const uint32_t* __restrict__ ent_dest,
+                                                 const uint32_t* __restrict__ tile_cnt, uint64_t n_tiles, uint32_t tile_slots,
+                                                 const uint32_t* __restrict__ rec_dest, uint64_t S, uint64_t* __restrict__ list,
+                                                 unsigned int* __restrict__ err) {
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint32_t c = tile_cnt[t];
+    for (uint32_t i = threadIdx.x; i < c; i += 256) {
+      const uint64_t slot = t * tile_slots + i;
+      const uint32_t rb = ent_dest[slot];
+      const uint32_t rs = rb < S ? rec_dest[rb] : ~0u;
+      if (rs < S) list[rs] = entries[slot];
+      else atomicOr(err, 1u);
+    }
+  }
+}
+template <int RW>
+struct P5Own {
+  static constexpr int N = RW == 2 ? 17 : 49;  // k-mers per record in the owner map (w with the default m)
+};
+constexpr int P5_LB = 64;  // BBHash blocks staged in LDS (a segment of up to ~3,000 keys)
+template <typename KeyT, int RW>
+__global__ void __launch_bounds__(P3_WG) k_p5_annot(const uint64_t* __restrict__ srec, const uint64_t* __restrict__ srec_ent,
+                                                    const uint64_t* __restrict__ roff, const uint64_t* __restrict__ key_off,
+                                                    const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks,
+                                                    const uint64_t* __restrict__ states, int K, uint64_t n_segs,
+                                                    uint8_t* __restrict__ annot, unsigned int* __restrict__ err) {
+  constexpr int OWN = P5Own<RW>::N;
+  constexpr int LW = P5_LB * 7;
+  __shared__ uint64_t lw[LW];  // the segment's level bits (7 words per block), if it has at most P5_LB blocks
+  __shared__ uint32_t lr[LW];  // rank before each word
+  __shared__ uint64_t srl[P3_WG * RW];
+  __shared__ uint64_t sent[P3_WG];
+  __shared__ uint16_t oo[P3_WG];
+  __shared__ uint8_t own[P3_WG * OWN];
+  __shared__ uint64_t sh4[4];
+  const uint64_t seg = blockIdx.x;
+  if (seg >= n_segs) return;
+  const uint64_t ko = key_off[seg], u = key_off[seg + 1] - ko;
+  const uint64_t b0 = blk_off[seg], nb = blk_off[seg + 1] - b0;
+  const uint64_t* base = blocks + b0 * 8;
+  const bool staged = nb <= (uint64_t)P5_LB;
+  if (staged) {
+    // block j word q (1..7) is level word 7j + q - 1 (448 = 7 * 64 bits per
+    // block); its rank = the block's rank word + popcounts of the words before
+    for (uint64_t j = threadIdx.x; j < nb; j += P3_WG) {
+      uint32_t rk = (uint32_t)base[j * 8];
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        const uint64_t x = base[j * 8 + 1 + q];
+        lw[j * 7 + q] = x;
+        lr[j * 7 + q] = rk;
+        rk += (uint32_t)__popcll(x);
+      }
+    }
+  }
+  __syncthreads();
+  const uint64_t lim = nb * BBH_BLOCK_BITS;
+  // local rank of a present key through the staged level words (the levels
+  // of mphf_local_rank, read from LDS)
+  auto lds_rank = [&](const KeyT& c) -> uint64_t {
+    uint64_t t = u, off = 0, h = bbh_h1(c);
+    const uint64_t d = bbh_d(h);
+    for (int l = 0; l < BBH_MAX_LEVELS && t > 0; l++, h += d) {
+      const uint64_t n = bbh_level_n(t);
+      if (off + n > lim) break;
+      const uint32_t p = (uint32_t)(off + bbh_pos(h, (uint32_t)n));
+      const uint64_t x = lw[p >> 6];
+      if ((x >> (p & 63)) & 1) return (uint64_t)lr[p >> 6] + (uint64_t)__popcll(x & ((1ULL << (p & 63)) - 1));
+      off += n;
+      t = bbh_next_t(t);
+    }
+    return ~0ULL;
+  };
+  const uint64_t kmask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
+  // k-mer j of the record at w (n k-mers), at stream position p
+  auto one = [&](const uint64_t* w, int j, int n, uint64_t p) {
+    KeyT x, c;
+    if constexpr (RW == 2) {
+      x = rec2_bits(w[0], w[1], 126 - 2 * (j + K)) & kmask;
+      c = kmer_min(x, rc64(x, K));
+    } else {
+      const Rec4 R{w[0], w[1], w[2], w[3]};
+      const int sh = 254 - 2 * (j + K);
+      x.lo = rec4_bits(R, sh);
+      x.hi = rec4_bits(R, sh + 64) & ((1ULL << (2 * K - 64)) - 1);
+      c = kmer_min(x, rc128(x, K));
+    }
+    (void)n;
+    const uint64_t rk = staged ? lds_rank(c) : mphf_local_rank<KeyT>(base, u, c);
+    uint8_t a = 0;
+    if (rk < u) {
+      const uint64_t v = ko + rk;
+      const uint32_t s = (uint32_t)(states[v / 12] >> ((v % 12) * 5)) & 31u;
+      a = (uint8_t)(1u | (key_eq(x, c) ? 2u : 0u) | (st_class(s) << 2));
+    } else {
+      atomicOr(err, 2u);
+    }
+    annot[p] = a;
+  };
+  const uint64_t ra = roff[seg], rb = roff[seg + 1];
+  for (uint64_t rr = ra; rr < rb; rr += P3_WG) {
+    const uint64_t r = rr + threadIdx.x;
+    int n = 0;
+    if (r < rb) {
+#pragma unroll
+      for (int t = 0; t < RW; t++) srl[threadIdx.x * RW + t] = srec[r * RW + t];
+      sent[threadIdx.x] = srec_ent[r];
+      n = rec_kmers_at<RW>(srec, r, K);
+    }
+    uint64_t T;
+    const uint32_t o = (uint32_t)block_excl_scan_u64((uint64_t)n | ((uint64_t)(n > OWN) << 32), sh4, &T);
+    if (T >> 32) {  // a record longer than the owner map: each lane walks its own
+      for (int j = 0; j < n; j++) one(srl + threadIdx.x * RW, j, n, (sent[threadIdx.x] >> 24) + (uint64_t)j);
+      __syncthreads();
+      continue;
+    }
+    if (n) {
+      oo[threadIdx.x] = (uint16_t)o;
+      for (int q = 0; q < n; q++) own[o + q] = (uint8_t)threadIdx.x;
+    }
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += P3_WG) {
+      const uint32_t rl = own[g];
+      const int j = (int)(g - oo[rl]);
+      one(srl + rl * RW, j, 0, (sent[rl] >> 24) + (uint64_t)j);
+    }
+    __syncthreads();
   }
 }
 
@@ -1448,14 +1589,14 @@ void launch_p3_rcount(hipStream_t st, bool wide, const uint64_t* rec, const uint
 }
 void launch_p3_rscatter(hipStream_t st, bool wide, const uint64_t* rec, const uint16_t* rec_sub, const uint64_t* bin_start,
                         const uint64_t* chunk_off, int num_bins, uint32_t chunk_records, uint64_t n_chunks, int sb, const uint64_t* roff,
-                        unsigned int* rcur, uint64_t* srec, const unsigned int* stop) {
+                        unsigned int* rcur, uint64_t* srec, uint32_t* rec_dest, const unsigned int* stop) {
   if (n_chunks == 0) return;
   if (!wide)
     CF1G_LAUNCH("p3_rscatter", k_p3_rscatter<2>, dim3((unsigned)n_chunks), dim3(P3_WG), rscatter_lds<2>(sb), st, rec, rec_sub,
-                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, stop);
+                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
   else
     CF1G_LAUNCH("p3_rscatter", k_p3_rscatter<4>, dim3((unsigned)n_chunks), dim3(P3_WG), rscatter_lds<4>(sb), st, rec, rec_sub,
-                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, stop);
+                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
 }
 void launch_p3_expand(hipStream_t st, bool wide, const uint64_t* srec, const uint64_t* roff, const uint64_t* seg_off,
                       const uint32_t* list, uint64_t n, int K, void* occ, uint16_t* obs, const unsigned int* stop) {
@@ -1466,6 +1607,24 @@ void launch_p3_expand(hipStream_t st, bool wide, const uint64_t* srec, const uin
   else
     CF1G_LAUNCH("p3_expand", (k_p3_expand<Key128, 4>), dim3((unsigned)n), dim3(P3_WG), 0, st, srec, roff, seg_off, list, K,
                 (Key128*)occ, obs, stop);
+}
+void launch_p5_list(hipStream_t st, const uint64_t* entries, const uint32_t* ent_dest, const uint32_t* tile_cnt, uint64_t n_tiles,
+                    uint32_t tile_slots, const uint32_t* rec_dest, uint64_t S, uint64_t* list, unsigned int* err) {
+  if (n_tiles == 0) return;
+  const uint64_t g = n_tiles < 8192 ? n_tiles : 8192;
+  CF1G_LAUNCH("p5_list", k_p5_list, dim3((unsigned)g), dim3(256), 0, st, entries, ent_dest, tile_cnt, n_tiles, tile_slots, rec_dest, S,
+              list, err);
+}
+void launch_p5_annot(hipStream_t st, bool wide, const uint64_t* srec, const uint64_t* srec_ent, const uint64_t* roff,
+                     const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks, const uint64_t* states, int K,
+                     uint64_t n_segs, uint8_t* annot, unsigned int* err) {
+  if (n_segs == 0) return;
+  if (!wide)
+    CF1G_LAUNCH("p5_annot", (k_p5_annot<uint64_t, 2>), dim3((unsigned)n_segs), dim3(P3_WG), 0, st, srec, srec_ent, roff, key_off,
+                blk_off, blocks, states, K, n_segs, annot, err);
+  else
+    CF1G_LAUNCH("p5_annot", (k_p5_annot<Key128, 4>), dim3((unsigned)n_segs), dim3(P3_WG), 0, st, srec, srec_ent, roff, key_off,
+                blk_off, blocks, states, K, n_segs, annot, err);
 }
 // Compares the scanned occurrence total with W (the valid k-mers P1 counted)
 // before the occurrence pass: on a mismatch flags[3] is set, and scatter,
