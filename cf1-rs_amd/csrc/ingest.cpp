@@ -322,7 +322,7 @@ void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, 
 // a worker still leaving an earlier job only sees that job's exhausted
 // counter.
 struct HostPool::Job {
-  const std::function<void(uint64_t)>* fn;
+  std::function<void(uint64_t)> fn;  // a copy: the caller's may be a temporary
   uint64_t n;
   std::atomic<uint64_t> next{0}, done{0};
 };
@@ -341,7 +341,7 @@ void HostPool::drain(Job& j) {
   for (;;) {
     const uint64_t i = j.next.fetch_add(1);
     if (i >= j.n) break;
-    (*j.fn)(i);
+    j.fn(i);
     if (j.done.fetch_add(1) + 1 == j.n) {
       std::lock_guard<std::mutex> g(mu_);
       cv_done_.notify_all();
@@ -368,7 +368,7 @@ void HostPool::worker() {
 }
 void HostPool::start(uint64_t n, const std::function<void(uint64_t)>& fn) {
   auto j = std::make_shared<Job>();
-  j->fn = &fn;
+  j->fn = fn;
   j->n = n;
   {
     std::lock_guard<std::mutex> g(mu_);

@@ -60,8 +60,9 @@ void pack_words(const uint8_t* ascii, const uint64_t* rec_off, const uint64_t* g
 
 // A fixed set of host threads: run(n, fn) calls fn(0 .. n-1) on them and on
 // the calling thread, and returns when every call has returned; start(n, fn)
-// hands the calls to the pool's threads alone and returns at once (fn must
-// stay alive until wait() returns). Calls are taken in index order.
+// hands the calls to the pool's threads alone (on a copy of fn) and returns
+// at once; wait() returns when they have all returned. Calls are taken in
+// index order.
 class HostPool {
  public:
   explicit HostPool(int threads);
