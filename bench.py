@@ -59,7 +59,8 @@ WORKLOADS = {
 PHASE_OF = {
     "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
     "p2_partition": "P2", "p2_route": "P2",
-    "p3_gate": "P3", "p3_count": "P3", "p3_scatter": "P3", "p3_split": "P3", "p3_dedup": "P3", "p3_bigsort": "P3", "p3_compact": "P3",
+    "p3_gate": "P3", "p3_rcount": "P3", "p3_rscatter": "P3", "p3_expand": "P3", "p3_dedup": "P3", "p3_bigsort": "P3",
+    "p3_compact": "P3", "p1_excl": "P1", "scan_total": "scan", "zero": "clear",
     "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",
 }
@@ -102,38 +103,37 @@ def algorithmic_bytes(info, kb, rw):
     function must read and write at least once (inputs, outputs, no
     intermediate re-reads). NG = stream positions, W / S / D = k-mer
     occurrences / super k-mers / distinct k-mers, B = BBHash bytes, rec = S
-    fixed-width records, ob = key + 2-byte mask.
-      pack      : ASCII in, 2-bit bases + placeholder bits out
-      p1_scan   : packed stream in, one 8-byte entry and its 2-byte bucket
-                  per super k-mer out
-      p1_hist   : buckets in (the 512 KiB histogram is noise)
-      p2_route  : entries in (the kernel reads them twice: it counts its
-                  records per bin before writing them), the super k-mers'
-                  bases in, records out
-      p3_count  : records in
-      p3_scatter: records in, W (key, mask) occurrences out (64 parts per bin)
-      p3_split  : W occurrences in, the same out (segments)
-      p3_dedup  : W occurrences in; D distinct keys and D masks (both in
-                  local-rank order) and the BBHash blocks out
-      p3_compact: D masks, B in; the same out (dense; the dense key array
-                  is compacted only when a caller asks for the keys)
+    fixed-width records (16 or 32 bytes).
+      pack       : ASCII in, 2-bit bases + placeholder bits out (the resident
+                   step; the streaming step packs on the host)
+      p1_scan    : packed stream in, one 8-byte entry and its 4-byte
+                   minimizer hash per super k-mer out
+      p1_hist    : the hashes in (the 512 KiB histogram is noise)
+      p2_route   : entries and hashes in, the super k-mers' bases in (one
+                   record's worth each), records out with a 2-byte segment
+                   and a 4-byte destination per super k-mer
+      p3_rcount  : the segments and one word per record in
+      p3_rscatter: records and segments in, records (segment order) and a
+                   4-byte destination per record out
+      p3_dedup   : records in; D distinct keys and D masks (local-rank
+                   order) and the BBHash blocks out
+      p3_compact : D masks, B in; the same out (dense; the dense key array is
+                   compacted only when a caller asks for the keys)
       p4_classify: D masks in, ceil(D/12) state words out
     """
     NG = info["stream_len"]
-    W, S, D = info["n_kmers"], info["n_superkmers"], info["n_distinct"]
+    S, D = info["n_superkmers"], info["n_distinct"]
     B = info["mphf_words"] * 8
     seq = NG / 4 + NG / 8
     rec = S * rw * 8
-    ob = kb + 2
     return {
         "pack": NG + seq,
-        "p1_scan": seq + 10 * S,
-        "p1_hist": 2 * S,
-        "p2_route": 8 * S + 2 * rec,
-        "p3_count": rec,
-        "p3_scatter": rec + W * ob,
-        "p3_split": 2 * W * ob,
-        "p3_dedup": W * ob + D * ob + B,
+        "p1_scan": seq + 12 * S,
+        "p1_hist": 4 * S,
+        "p2_route": 12 * S + 2 * rec + 6 * S,
+        "p3_rcount": 10 * S,
+        "p3_rscatter": 2 * rec + 6 * S,
+        "p3_dedup": rec + D * (kb + 2) + B,
         "p3_compact": 2 * (2 * D + B),
         "p4_classify": 2 * D + 8 * ((D + 11) // 12),
     }

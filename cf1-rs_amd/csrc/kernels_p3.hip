@@ -277,6 +277,75 @@ __device__ __forceinline__ Key128 rec4_occ(const Rec4& R, int j, int n, int K, u
   return c;
 }
 
+// k-mers j0 .. j0+cnt-1 (cnt <= PK) of a record of n k-mers with their
+// observation masks, the same as rec2_occ / rec4_occ for each j: the first
+// k-mer and its reverse complement are cut out of the record, the rest are
+// rolled one base at a time (kmer.rs:240-262), and the base before k-mer j
+// (j > j0) is the top base of k-mer j-1.
+template <int PK, typename FK>
+__device__ __forceinline__ void rec2_piece(uint64_t w0, uint64_t w1, int j0, int cnt, int n, int K, uint64_t mask, FK out) {
+  uint64_t x = rec2_bits(w0, w1, 126 - 2 * (j0 + K)) & mask;
+  uint64_t r = rc64(x, K);
+  const bool lf = (w1 >> 8) & 1, rf = (w1 >> 9) & 1;
+  uint32_t pb = (uint32_t)rec2_bits(w0, w1, 126 - 2 * j0) & 3u;
+#pragma unroll
+  for (int i = 0; i < PK; i++) {
+    if (i < cnt) {
+      const int j = j0 + i;
+      const uint64_t c = x < r ? x : r;
+      const bool pv = j > 0 || lf, nv = j < n - 1 || rf;
+      const uint32_t nb = (uint32_t)rec2_bits(w0, w1, 124 - 2 * (j + K)) & 3u;
+      const uint64_t nx = ((x << 2) | nb) & mask;
+      const uint64_t nr = (r >> 2) | ((uint64_t)(3 - nb) << (2 * K - 2));
+      const uint64_t nc = nx < nr ? nx : nr;
+      uint32_t type;
+      if (!pv && !nv) type = OBS_MIMO;
+      else if (!nv) type = OBS_RIGHT;
+      else if (nc == c) type = (!pv || x == nx) ? OBS_MIMO : OBS_RIGHT;
+      else type = pv ? OBS_INTERNAL : OBS_LEFT;
+      out(i, c, obs_mask(type, x == c, pb, nb));
+      pb = (uint32_t)(x >> (2 * K - 2)) & 3u;
+      x = nx;
+      r = nr;
+    }
+  }
+}
+template <int PK, typename FK>
+__device__ __forceinline__ void rec4_piece(const Rec4& R, int j0, int cnt, int n, int K, FK out) {
+  const uint64_t mh = (1ULL << (2 * K - 64)) - 1;
+  const int sh = 254 - 2 * (j0 + K);
+  Key128 x;
+  x.lo = rec4_bits(R, sh);
+  x.hi = rec4_bits(R, sh + 64) & mh;
+  Key128 r = rc128(x, K);
+  const bool lf = (R.w3 >> 8) & 1, rf = (R.w3 >> 9) & 1;
+  uint32_t pb = (uint32_t)rec4_bits(R, 254 - 2 * j0) & 3u;
+#pragma unroll
+  for (int i = 0; i < PK; i++) {
+    if (i < cnt) {
+      const int j = j0 + i;
+      const Key128 c = kmer_min(x, r);
+      const bool pv = j > 0 || lf, nv = j < n - 1 || rf;
+      const uint32_t nb = (uint32_t)rec4_bits(R, sh - 2 - 2 * i) & 3u;
+      Key128 nx, nr;
+      nx.hi = ((x.hi << 2) | (x.lo >> 62)) & mh;
+      nx.lo = (x.lo << 2) | nb;
+      nr.lo = (r.lo >> 2) | (r.hi << 62);
+      nr.hi = (r.hi >> 2) | ((uint64_t)(3 - nb) << (2 * K - 66));
+      const Key128 nc = kmer_min(nx, nr);
+      uint32_t type;
+      if (!pv && !nv) type = OBS_MIMO;
+      else if (!nv) type = OBS_RIGHT;
+      else if (key_eq(nc, c)) type = (!pv || key_eq(x, nx)) ? OBS_MIMO : OBS_RIGHT;
+      else type = pv ? OBS_INTERNAL : OBS_LEFT;
+      out(i, c, obs_mask(type, key_eq(x, c), pb, nb));
+      pb = (uint32_t)(x.hi >> (2 * K - 66)) & 3u;
+      x = nx;
+      r = nr;
+    }
+  }
+}
+
 // Round-local k-mer -> record map for P3_WG records of at most
 // OwnKmers<RW>::N k-mers each (P2 emits super k-mers of at most w k-mers:
 // w <= 17 for K <= 31 and w <= 49 for K <= 63 with the default minimizer
@@ -642,6 +711,7 @@ constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12
 constexpr int DD_WG = 256;
 constexpr int DD_KPT = 8;
 constexpr int DD_CAP = DD_WG * DD_KPT;
+constexpr int DEC_PK = 4;  // k-mers per lane in the dedup's record decode
 
 // A segment's MPHF is built by k_bbh_big when its distinct keys exceed the
 // register capacity, or when it was sorted by k_p3_bigsort (more
@@ -883,6 +953,15 @@ __device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t
   fp = (uint32_t)((h << tb) >> 54);
 }
 
+// Position of a key in the dedup's occurrence filter of FILT_BITS bits: 16
+// bits of seg_hash (shifted as for tab_slot) below those the table slot and
+// fingerprint use.
+constexpr int FILT_BITS = 1 << 16;
+template <typename KeyT>
+__device__ __forceinline__ uint32_t filt_pos(const KeyT& k, int sb) {
+  return (uint32_t)((seg_hash(k) << sb) >> 26) & (FILT_BITS - 1);
+}
+
 // Work-groups per CU the dedup kernel is compiled for: 8 with u64 keys (64
 // VGPRs and 8 x 20 KiB of LDS fill a CU; 3.99 -> 3.60 ms on the 300 Mbp
 // genome against 7 per CU at 70 VGPRs); 6 with 16-byte keys (77 VGPRs;
@@ -898,10 +977,13 @@ struct DdOccupancy {
 // Dedup of one segment and its BBHash, DD_WG threads holding DD_KPT
 // occurrences each (element e = slot * DD_WG + thread). Each occurrence
 // carries its 10-bit observation mask; the masks of equal keys are OR-joined
-// (the DFA join of classify.rs:147-518). The distinct keys are found with an
-// LDS hash table (linear probing, entries = representative occurrence + 1 |
-// joined mask << 12 | 10-bit key fingerprint << 22; keys are compared, from
-// occ, only when the fingerprints match); which occurrence represents a key
+// (the DFA join of classify.rs:147-518). An occurrence filter (2^16
+// positions, seen / shared bits in LDS) settles every occurrence alone at its
+// position: its key occurs once. The rest (repeated keys and ~n / 2^16
+// false sharers) are deduplicated with an LDS hash table (linear probing,
+// entries = representative occurrence + 1 | joined mask << 12 | 10-bit key
+// fingerprint << 22; keys are compared, from occ, only when the fingerprints
+// match); which occurrence represents a key
 // depends on timing, but the set of keys, their joined masks and the BBHash
 // built over them do not. Output: kout[start + e] = the key of occurrence e
 // if it represents its key, else the pad (all ones; the dense key array is
@@ -942,6 +1024,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   constexpr int A_END = A_BSH + (int)sizeof(BbhShared<WG>);
   constexpr int AUX_BYTES = A_END > CAP * 2 ? A_END : CAP * 2;
   static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
+  static_assert(2 * (FILT_BITS / 8) <= TAB_BYTES, "the occurrence filter fits the table space");
   static_assert(T_END <= T_MJ && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
   static_assert(REG_W / 7 + 2 <= SEEN_W && CAP <= 65536 && REG_W * 64 <= 65536 && T_SEEN % 8 == 0 && T_LIST % 8 == 0,
                 "LDS reuse fits");
@@ -1039,35 +1122,42 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     uint32_t pend = 0;
     ns = (int)((n + WG - 1) / WG);
     // the segment's k-mers and observation masks straight from its records:
-    // rounds of RR records, one lane per k-mer through an LDS owner map
-    // (built in the table's space before the table is cleared); element e
-    // is k-mer e of the segment in record order. Every key goes to occ (the
-    // table reads a representative's key there when fingerprints match, and
-    // the dense key array is built from it on request), every mask to mj;
-    // the slots then load their elements back (a decode per slot, unrolled,
-    // would hold every slot's record words at once: 86 VGPRs instead of 64).
+    // rounds of RR records staged in LDS (in the table's space, before the
+    // table is cleared), each record cut into pieces of up to DEC_PK k-mers,
+    // one lane per piece through an LDS owner map: the piece's first k-mer is
+    // cut out of the record, the others rolled. Element e is k-mer e of the
+    // segment in record order. Every key goes to occ (the table reads a
+    // representative's key there when fingerprints match, and the dense key
+    // array is built from it on request), every mask to mj; the slots then
+    // load their elements back (a decode per slot, unrolled, would hold every
+    // slot's record words at once: 86 VGPRs instead of 64).
     {
       constexpr int RR = RW == 2 ? 256 : 128;
       constexpr int OWN = OwnKmers<RW>::N;
-      static_assert(RR * RW * 8 + RR * 2 + RR * OWN <= T_MJ, "decode scratch fits below the joined masks");
+      constexpr int PK = DEC_PK;
+      constexpr int OWNP = (OWN + PK - 1) / PK;  // pieces per record at most
+      static_assert(RR * RW * 8 + RR * 4 + RR * OWNP <= T_MJ, "decode scratch fits below the joined masks");
       uint64_t* srl = (uint64_t*)tab_b;
-      uint16_t* oo = (uint16_t*)(tab_b + RR * RW * 8);
-      uint8_t* own = (uint8_t*)(oo + RR);
+      uint16_t* kofs = (uint16_t*)(tab_b + RR * RW * 8);  // per staged record: first k-mer in the round
+      uint16_t* pofs = kofs + RR;                          //   first piece in the round
+      uint8_t* ownp = (uint8_t*)(pofs + RR);               // per piece: its record
       const uint64_t kmask = K < 32 ? (1ULL << (2 * K)) - 1 : ~0ULL;
       const uint64_t ra = roff[seg], rb = roff[seg + 1];
       uint64_t done = 0;
       for (uint64_t rr = ra; rr < rb; rr += RR) {
         const uint64_t r = rr + threadIdx.x;
-        int nk = 0;
+        uint32_t nk = 0;
         if ((int)threadIdx.x < RR && r < rb) {
 #pragma unroll
           for (int t = 0; t < RW; t++) srl[threadIdx.x * RW + t] = srec[r * RW + t];
-          nk = rec_kmers_at<RW>(srec, r, K);
+          nk = (uint32_t)rec_kmers_at<RW>(srec, r, K);
         }
-        // one scan for the offsets and for records longer than the owner map
-        // (counted in the high half; such a segment takes the global-memory path)
+        const uint32_t np = (nk + PK - 1) / PK;
+        // one scan for the k-mer offsets (bits 0-15), the piece offsets (bits
+        // 16-31) and the records longer than the owner map (counted in the
+        // high half; such a segment takes the global-memory path)
         uint64_t T;
-        const uint32_t o = (uint32_t)block_excl_scan<WG>((uint64_t)nk | ((uint64_t)(nk > OWN) << 32), sh8, &T);
+        const uint64_t o = block_excl_scan<WG>((uint64_t)nk | ((uint64_t)np << 16) | ((uint64_t)(nk > (uint32_t)OWN) << 32), sh8, &T);
         if (T >> 32) {
           if (threadIdx.x == 0) {
             over_list[atomicAdd(n_over, 1u)] = (uint32_t)seg;
@@ -1077,27 +1167,34 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
           return;
         }
         if (nk) {
-          oo[threadIdx.x] = (uint16_t)o;
-          for (int q = 0; q < nk; q++) own[o + q] = (uint8_t)threadIdx.x;
+          kofs[threadIdx.x] = (uint16_t)o;
+          pofs[threadIdx.x] = (uint16_t)(o >> 16);
+          for (uint32_t q = 0; q < np; q++) ownp[(uint32_t)(o >> 16) + q] = (uint8_t)threadIdx.x;
         }
         __syncthreads();
-        for (uint32_t g = threadIdx.x; g < (uint32_t)T; g += WG) {
-          const uint32_t rl = own[g];
-          const int j = (int)(g - oo[rl]);
+        const uint32_t TP = (uint32_t)(T >> 16) & 0xFFFFu;
+        for (uint32_t p = threadIdx.x; p < TP; p += WG) {
+          const uint32_t rl = ownp[p];
+          const int j0 = (int)(p - pofs[rl]) * PK;
           const uint64_t* w = srl + rl * RW;
-          uint32_t m;
-          KeyT kk;
+          const uint64_t e0 = done + kofs[rl] + (uint32_t)j0;
+          KeyT* ko = occ + start + e0;
+          uint16_t* mo = mj + e0;
+          auto out = [&](int i, const KeyT& c, uint32_t m) {
+            ko[i] = c;
+            mo[i] = (uint16_t)m;
+          };
           if constexpr (RW == 2) {
-            kk = (KeyT)rec2_occ<true>(w[0], w[1], j, rec2_kmers(w[1], K), K, kmask, m);
+            const int nr = rec2_kmers(w[1], K);
+            rec2_piece<PK>(w[0], w[1], j0, nr - j0 < PK ? nr - j0 : PK, nr, K, kmask, out);
           } else {
             const Rec4 R{w[0], w[1], w[2], w[3]};
-            kk = rec4_occ<true>(R, j, rec4_kmers(R.w3, K), K, m);
+            const int nr = rec4_kmers(R.w3, K);
+            rec4_piece<PK>(R, j0, nr - j0 < PK ? nr - j0 : PK, nr, K, out);
           }
-          occ[start + done + g] = kk;
-          mj[done + g] = (uint16_t)m;
         }
         __syncthreads();
-        done += T;
+        done += T & 0xFFFFu;
       }
 #pragma unroll
       for (int s = 0; s < KPT; s++) {
@@ -1110,6 +1207,39 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
       }
       __syncthreads();  // every element is loaded before the table (over mj) is cleared
     }
+    // filter: an occurrence whose filter position no other occurrence of the
+    // segment shares has a key no other occurrence has (equal keys share
+    // their position), so it represents its key with its own mask; only the
+    // others (~n / 2^16 of them, and every repeated key) go through the table
+    {
+      uint32_t* seenF = (uint32_t*)tab_b;
+      uint32_t* colF = seenF + (FILT_BITS / 32);
+      for (uint32_t i = threadIdx.x; i < FILT_BITS / 64; i += WG) ((uint4*)tab_b)[i] = make_uint4(0, 0, 0, 0);
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if (s >= ns) break;
+        if ((pend >> s) & 1) {
+          const uint32_t f = filt_pos(key[s], sb);
+          const uint32_t bit = 1u << (f & 31);
+          if (atomicOr(&seenF[f >> 5], bit) & bit) atomicOr(&colF[f >> 5], bit);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < KPT; s++) {
+        if (s >= ns) break;
+        if ((pend >> s) & 1) {
+          const uint32_t f = filt_pos(key[s], sb);
+          if (!((colF[f >> 5] >> (f & 31)) & 1)) {
+            alive |= 1u << s;
+            pend &= ~(1u << s);
+          }
+        }
+      }
+      __syncthreads();  // the filter is read before the table is cleared over it
+    }
+    const uint32_t cand = pend;  // slots that go through the table
     for (uint32_t i = threadIdx.x; i <= tmask / 4; i += WG) ((uint4*)tab)[i] = make_uint4(0, 0, 0, 0);  // 2^tb >= 64 entries
     __syncthreads();
     uint32_t hs[KPT], ent[KPT];  // probe slot; this occurrence's entry
@@ -1117,8 +1247,8 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     for (int s = 0; s < KPT; s++) {
       if (s >= ns) break;
       uint32_t slot = 0, fp = 0;
-      tab_slot(key[s], sb, tb, slot, fp);
       const bool v = (pend >> s) & 1;
+      if (v) tab_slot(key[s], sb, tb, slot, fp);
       hs[s] = v ? slot : 0u;
       ent[s] = v ? ((uint32_t)(s * WG + threadIdx.x + 1) | (msk[s] << 12) | (fp << 22)) : 0u;
     }
@@ -1151,7 +1281,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if (s >= ns) break;
-      if ((alive >> s) & 1) msk[s] = (tab[hs[s]] >> 12) & 0x3FFu;
+      if ((alive & cand) >> s & 1) msk[s] = (tab[hs[s]] >> 12) & 0x3FFu;
     }
     uint64_t tot;
     block_excl_scan<WG>((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab is free after it
@@ -1188,27 +1318,30 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   const uint64_t bits = bbh_levels<KPT, WG>(h1, alive, ns, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
   const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < WG
   __syncthreads();
-  // blocks {rank before, 7 words}; word ranks for the local ranks below
-  const uint64_t j = threadIdx.x;
-  uint64_t w[7];
-  uint64_t pc = 0;
-#pragma unroll
-  for (int q = 0; q < 7; q++) {
-    const uint64_t x = j * 7 + q;
-    w[q] = (j < nb && x < (uint64_t)REG_W) ? region[x] : 0;
-    pc += (uint64_t)__popcll(w[q]);
-  }
-  uint64_t tot;
-  const uint64_t rk = block_excl_scan<WG>(j < nb ? pc : 0, sh8, &tot);
-  if (j < nb) {
-    uint64_t* B = blocks_pad + (pad_slot_off(start, seg) + j) * 8;
-    B[0] = rk;
-    uint64_t run = rk;
+  // blocks {rank before, 7 words} and the word ranks for the local ranks
+  // below, by wave 0 (nb <= ceil(REG_W * 64 / 448) <= 64 blocks)
+  static_assert((REG_W * 64 + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS <= 64, "a segment's blocks fit one wave");
+  if (threadIdx.x < 64) {
+    const uint64_t j = threadIdx.x;
+    uint64_t w[7];
+    uint64_t pc = 0;
 #pragma unroll
     for (int q = 0; q < 7; q++) {
-      B[1 + q] = w[q];
-      if (j * 7 + q < (uint64_t)REG_W) wrank[j * 7 + q] = (uint32_t)run;
-      run += (uint64_t)__popcll(w[q]);
+      const uint64_t x = j * 7 + q;
+      w[q] = (j < nb && x < (uint64_t)REG_W) ? region[x] : 0;
+      pc += (uint64_t)__popcll(w[q]);
+    }
+    const uint64_t rk = wave_incl_scan_u64(pc) - pc;
+    if (j < nb) {
+      uint64_t* B = blocks_pad + (pad_slot_off(start, seg) + j) * 8;
+      B[0] = rk;
+      uint64_t run = rk;
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        B[1 + q] = w[q];
+        if (j * 7 + q < (uint64_t)REG_W) wrank[j * 7 + q] = (uint32_t)run;
+        run += (uint64_t)__popcll(w[q]);
+      }
     }
   }
   if (threadIdx.x == 0) nblk[seg] = nb;
