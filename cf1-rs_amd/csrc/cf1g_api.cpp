@@ -182,6 +182,14 @@ struct cf1g_ctx {
   int cu_count = 256;
   std::string err;
   int stage = 0;  // 1 loaded, 2 P1, 3 P2, 4 P3, 5 P4
+  // state_gen changes whenever records, phase products, states or the
+  // selected index change; the unitig spans are kept until it does (a count
+  // query and the fill that follows it mark the spans once)
+  uint64_t state_gen = 1, spans_gen = 0, spans_U = 0;
+  void set_stage(int s) {
+    stage = s;
+    state_gen++;
+  }
   std::map<std::string, DevBuf> bufs;
 
   // sizes
@@ -407,7 +415,7 @@ void do_p1(cf1g_ctx* c) {
     launch_p1(st, seq, nm, c->n_pos, c->k, c->m, ent, bkt, tcnt, tw, part, sums, hist, hist_all, tot, edges, n_edges, excl_cnt, excl,
               excl_cap, excl_cnt + 1);
   });
-  c->stage = 2;
+  c->set_stage(2);
 }
 
 // Chunk edges (stream positions) of the loaded records, plus a UINT64_MAX
@@ -451,7 +459,7 @@ void set_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uin
 // chunk. The products equal cf1g_load_records + cf1g_p1_histogram's.
 void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint32_t threads) {
   set_records(c, ascii, rec_off, n_rec);
-  c->stage = 0;
+  c->set_stage(0);
   set_chunk_edges(c);
   // the calling thread issues the copies and launches; th - 1 pool threads pack
   const int th = std::max(2, threads ? (int)threads : (int)std::thread::hardware_concurrency());
@@ -557,7 +565,7 @@ void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, ui
   launch_p1_finish(st, c->n_pos, bkt, tcnt, tw, part, sums, hist, hist_all, tot, n_edges, excl_cnt, excl, excl_cap, excl_cnt + 1);
   c->sync();
   c->host_packed = true;
-  c->stage = 2;
+  c->set_stage(2);
 }
 
 void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
@@ -621,7 +629,7 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   unsigned long long chk[2] = {0, 0};
   c->d2h(chk, route_chk, 16);
   if (chk[1] || chk[0] != c->S) fail(CF1G_E_INTERNAL, "P2 routing disagrees with the bin sizes from the histogram");
-  c->stage = 3;
+  c->set_stage(3);
 }
 
 // P3: records and k-mers per MPHF segment (a segment is fixed by a super
@@ -746,8 +754,9 @@ void do_p3(cf1g_ctx* c) {
   c->keys_ready = false;
   c->lidx_ready = c->lidx_states = false;
   c->use_index = 0;
+  c->state_gen++;
   c->restored = false;
-  c->stage = 4;
+  c->set_stage(4);
 }
 
 // The distinct keys in MPHF value order (keys[v] has value v), built on
@@ -780,7 +789,7 @@ void do_p4(cf1g_ctx* c) {
   uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
   c->seg("p4", {c->D}, [&] { launch_p4_pack(c->st, vmask, c->D, states); });
   c->sync();
-  c->stage = 5;
+  c->set_stage(5);
 }
 
 // P4 in the reference's form: zeroed states, then one MPHF lookup and CAS
@@ -824,7 +833,8 @@ void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
     *out = o;
   }
   c->use_index = 0;
-  c->stage = 5;
+  c->state_gen++;
+  c->set_stage(5);
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -1116,6 +1126,7 @@ uint8_t* annotate_device(cf1g_ctx* c) {
 // through the learned index when selected) and "span_fl" (bit0 forward,
 // bit1 this unitig claims its id), in stream order. Returns the count.
 uint64_t build_spans(cf1g_ctx* c) {
+  if (c->spans_gen == c->state_gen) return c->spans_U;
   hipStream_t st = c->st;
   uint8_t* ann = annotate_device(c);
   const uint64_t nt = span_tiles(c->n_pos);
@@ -1155,6 +1166,8 @@ uint64_t build_spans(cf1g_ctx* c) {
   unsigned int he = 0;
   c->d2h(&he, err, 4);
   if (he) fail(CF1G_E_INTERNAL, "a unitig's k-mer is missing from the index");
+  c->spans_U = U;
+  c->spans_gen = c->state_gen;
   return U;
 }
 
@@ -1251,7 +1264,7 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
   return guarded(c, [&]() {
     if (!c->st) fail(CF1G_E_NODEVICE, "context has no device");
     set_records(c, ascii, rec_off, n_rec);
-    c->stage = 0;
+    c->set_stage(0);
     const uint64_t base = rec_off[0];
     std::vector<uint64_t> off(n_rec + 1);
     for (uint64_t r = 0; r <= n_rec; r++) off[r] = rec_off[r] - base;
@@ -1265,7 +1278,7 @@ int cf1g_load_records(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off
     set_chunk_edges(c);
     c->sync();
     c->W = c->S = c->D = 0;
-    c->stage = 1;
+    c->set_stage(1);
   });
 }
 
@@ -1318,7 +1331,7 @@ int cf1g_load_fasta(cf1g_ctx* c, const char* const* paths, uint32_t n_paths, uin
     auto t3 = std::chrono::steady_clock::now();
     c->host_records = c->host_packed = true;
     c->W = c->S = c->D = 0;
-    c->stage = 1;
+    c->set_stage(1);
     if (out) {
       auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
       cf1g_ingest_stats o{};
@@ -1418,6 +1431,7 @@ int cf1g_p4_classify_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_p4_classify_cas");
     do_p4_cas(c, flags, out);
+    c->state_gen++;
   });
 }
 
@@ -1664,8 +1678,9 @@ int cf1g_restore(cf1g_ctx* c, const char* path) {
     if (nsw) c->h2d(states, m + mw, nsw * 8);
     c->keys_ready = c->lidx_ready = c->lidx_states = false;
     c->use_index = 0;
+    c->state_gen++;
     c->restored = true;
-    c->stage = 5;
+    c->set_stage(5);
   });
 }
 
@@ -1907,6 +1922,7 @@ int cf1g_lidx_gradients(cf1g_ctx* c, int level, const float* w, const float* cc,
 int cf1g_classify_via(cf1g_ctx* c, int index) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_classify_via");
+    c->state_gen++;
     if (c->wide) fail(CF1G_E_INVALID, "classification through an index takes u64 keys (k <= 31)");
     if (index == 1 && !c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_classify_via(1) called before cf1g_lidx_train");
     if (index != 0 && index != 1) fail(CF1G_E_INVALID, "index is 0 (BBHash) or 1 (learned)");
@@ -1932,11 +1948,13 @@ int cf1g_select_index(cf1g_ctx* c, int index) {
   return guarded(c, [&]() {
     if (index == 0) {
       c->use_index = 0;
+      c->state_gen++;
       return;
     }
     if (index != 1) fail(CF1G_E_INVALID, "index is 0 (BBHash) or 1 (learned)");
     if (!c->lidx_states) fail(CF1G_E_ORDER, "cf1g_select_index(1) needs cf1g_classify_via(ctx, 1)");
     c->use_index = 1;
+    c->state_gen++;
   });
 }
 int cf1g_selected_index(const cf1g_ctx* c) { return c ? c->use_index : 0; }

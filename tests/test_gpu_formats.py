@@ -171,6 +171,41 @@ def test_unitig_spans_partition_runs(k):
     g.close()
 
 
+def test_unitig_spans_follow_new_records():
+    # the spans a count query marks are kept for the fill that follows it;
+    # new records, a new P4 or another index must mark them again
+    def ref_lines(seqs, k):
+        ref = RefPipeline(k, num_bins=8, threads=4)
+        ref.load(seqs)
+        for ph in (ref.p1, ref.p2, ref.p3, ref.p4):
+            ph()
+        with tempfile.TemporaryDirectory() as td:
+            ref.p5(os.path.join(td, "r"), input_desc="in.fa")
+            return sorted(open(os.path.join(td, "r.cf_seg")).read().splitlines())
+
+    def gpu_lines(g):
+        with tempfile.TemporaryDirectory() as td:
+            g.p5_write(os.path.join(td, "g"), input_desc="in.fa")
+            return sorted(open(os.path.join(td, "g.cf_seg")).read().splitlines())
+
+    a = random_genome(seed=961, n_rec=3, max_len=4000)
+    b = random_genome(seed=962, n_rec=5, max_len=2500) + edge_records()
+    g = cf1g.Cf1g(k=31, num_bins=8)
+    g.load_records(a)
+    g.run_p1_p4()
+    n_a = len(g.unitig_spans()[0])
+    assert gpu_lines(g) == ref_lines(a, 31)
+    g.load_records(b)
+    g.run_p1_p4()
+    assert gpu_lines(g) == ref_lines(b, 31)
+    g.classify_vertices_cas()
+    assert gpu_lines(g) == ref_lines(b, 31)
+    g.load_records(a)
+    g.run_p1_p4()
+    assert len(g.unitig_spans()[0]) == n_a
+    g.close()
+
+
 def _digest(words):
     """Save-body checksum: sum mod 2^64 of mix64(word ^ index * golden)."""
     M = (1 << 64) - 1
