@@ -16,7 +16,9 @@ Also reported on the same JSON line:
   roofline     : the dominant kernel's algorithmic bytes per launch over its
                  mean launch time (HIP events on the context stream in a
                  second, kernel-by-kernel pass of the same steps), against
-                 8 TB/s HBM, with its measured HBM traffic from profiles/.
+                 8 TB/s HBM, with its measured HBM traffic from profiles/;
+                 "issue": the same kernel's VALU instructions (profiles/)
+                 per second against the 614.4 G/s issue ceiling.
   cpu_baseline : the multithreaded C++ CPU oracle (the restatement of the
                  reference's P1..P4) on the same records from host memory,
                  on every core this process may use: per-phase medians of 3
@@ -42,6 +44,7 @@ sys.path.insert(0, os.path.join(ROOT, "cf1-rs_amd"))
 
 METRIC = "distinct k-mers/s phases 1-4 at k=31 vs CPU; learned-index vs BBHash lookups/s"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_G = 614.4  # wave64 VALU instructions per second (1,024 SIMDs x 2.4 GHz / 4 cycles), in G
 
 # workload name -> (generator(synth, scale, k) -> (ascii, rec_off, names), description(bases, records))
 WORKLOADS = {
@@ -153,6 +156,24 @@ def profiled_traffic(kernel, workload, k, bins):
     for path in paths:
         for line in open(path):
             if line.startswith("traffic_bytes_per_dispatch:"):
+                for tok in line.split(":", 1)[1].split():
+                    name, val = tok.split("=")
+                    if name == kernel:
+                        return float(val), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def profiled_valu(kernel, workload, k, bins):
+    """VALU wave-instructions per dispatch of `kernel` from the committed
+    instruction-counter summary of this workload and k (the bin count's own
+    summary when there is one), or None."""
+    import glob
+
+    exact = os.path.join(ROOT, "profiles", f"sq_counters_{workload}_k{k}_b{bins}.md")
+    paths = [exact] if os.path.exists(exact) else sorted(glob.glob(os.path.join(ROOT, "profiles", f"sq_counters_{workload}_k{k}_b*.md")))
+    for path in paths:
+        for line in open(path):
+            if line.startswith("valu_insts_per_dispatch:"):
                 for tok in line.split(":", 1)[1].split():
                     name, val = tok.split("=")
                     if name == kernel:
@@ -334,6 +355,13 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
             "bytes_per_launch": ab[dom] / dk["launches_per_step"], "mean_launch_ms": round(dk["mean_launch_ms"], 4)}
+    # the same kernel against the instruction-issue ceiling: 1,024 SIMDs at
+    # 2.4 GHz issue one wave64 VALU instruction per 4 cycles each
+    valu, vsrc = profiled_valu(dom, args.workload, args.k, info["num_bins"]) if args.scale == 1.0 else (None, None)
+    if valu:
+        ach = valu / (dk["mean_launch_ms"] * 1e-3) / 1e9
+        roof["issue"] = {"bound": "valu", "achieved": round(ach, 1), "peak": VALU_PEAK_G, "unit": "G wave-instructions/s",
+                         "frac": round(ach / VALU_PEAK_G, 4), "valu_per_launch": valu, "source": vsrc}
     copy_gbs = measured_copy_peak()
     roof["measured_copy_gbs"] = round(copy_gbs, 1)
     roof["frac_of_measured_copy"] = round(achieved / copy_gbs, 4)
@@ -358,7 +386,8 @@ def main():
     # learned index vs BBHash lookups over device-resident present keys
     lookup = None
     if args.lookups > 0 and D > 0 and kb == 8:
-        lookup = lookup_bench(g, D, args.lookups, args.lookup_batches)
+        lookup = lookup_bench(g, D, args.lookups, args.lookup_batches,
+                              far=profiled_far_requests(args.workload, args.k, info["num_bins"]) if args.scale == 1.0 else None)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -540,7 +569,7 @@ def measured_copy_peak(nbytes=2 << 30, reps=5):
     return gbs
 
 
-def lookup_bench(g, D, batch, n_batches, seed=5):
+def lookup_bench(g, D, batch, n_batches, far=None, seed=5):
     """The learned index against the BBHash on the same n_batches x batch
     lookups of present keys (uniform over the distinct keys, seeded), keys
     and results resident in HBM. Lookups/s from the lookup kernels' HIP-event
@@ -607,7 +636,39 @@ def lookup_bench(g, D, batch, n_batches, seed=5):
         "classify_via_ms": {k2: round(v[1], 3) for k2, v in kc.items()},
         "learned_index": {k2: (round(info[k2], 4) if isinstance(info[k2], float) else info[k2]) for k2 in keep},
         "train_wall_s": round(train_wall, 3),
+        "roofline": lookup_roofline(far, {"bbh_lookup": brate, "lidx_lookup": lrate}),
     }
+
+
+def profiled_far_requests(workload, k, bins):
+    """L2 read requests to the fabric per lookup (TCC_EA0_RDREQ / lookups) of
+    each lookup kernel, from the committed lookup counter summary, and its
+    path."""
+    import glob
+
+    exact = os.path.join(ROOT, "profiles", f"lookup_counters_{workload}_k{k}_b{bins}.md")
+    paths = [exact] if os.path.exists(exact) else sorted(glob.glob(os.path.join(ROOT, "profiles", f"lookup_counters_{workload}_k{k}_b*.md")))
+    for path in paths:
+        for line in open(path):
+            if line.startswith("far_requests_per_lookup:"):
+                return {t.split("=")[0]: float(t.split("=")[1]) for t in line.split(":", 1)[1].split()}, os.path.relpath(path, ROOT)
+    return None, None
+
+
+def lookup_roofline(far, rates):
+    """Per lookup kernel: far requests per lookup (profiled), bytes per
+    lookup at 64 bytes per request, and the rate those bytes move at against
+    8 TB/s."""
+    if not far or not far[0]:
+        return None
+    req, src = far
+    out = {"source": src, "bytes_per_request": 64}
+    for name, rate in rates.items():
+        if name in req and rate:
+            b = req[name] * 64
+            out[name] = {"far_requests_per_lookup": req[name], "bytes_per_lookup": round(b, 1),
+                         "achieved_gbs": round(b * rate / 1e9, 1), "frac_of_hbm_peak": round(b * rate / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
 
 
 if __name__ == "__main__":
