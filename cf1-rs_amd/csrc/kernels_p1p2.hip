@@ -577,15 +577,21 @@ void launch_stream_digest(hipStream_t st, const uint64_t* seq, const uint64_t* n
 // records. Both passes take RT_U tiles per step, one entry of each per
 // thread, and issue each dependent load (entry, bucket -> bin, stream words)
 // for all of them before the next, so that a thread has RT_U chains in
-// flight (route on the 300 Mbp genome: 1.03 ms with one, 0.86 with 4, 0.77
-// with 8, 0.75 with 16 at twice the registers). The order of records inside
+// flight. With 2-word records RT_U = 16 (157 VGPRs, 3 waves per SIMD): on
+// the 300 Mbp genome at 256 bins the route takes 1.15 ms, against 1.28 ms
+// with 8 (87 VGPRs, 5 waves) and 1.52 ms with 4 (55 VGPRs, 8 waves): the
+// chains in flight per thread matter more than waves per SIMD. 4-word records
+// keep 8 (16 would take 223 VGPRs). The order of records inside
 // a bin follows the schedule. Each super k-mer [i, i+n) becomes one fixed-size record in its bin
 // (the bin record of superkmer.rs:181-202 in fixed width): its L = n-1+K
 // bases plus one flanking base on each side, MSB-first in RW words; the last
 // word's low byte holds L and bits 8/9 say whether the left/right flank is a
 // real base (not a placeholder or record end). The flanks carry the
 // neighbours P4's observations need (classify.rs:76-77).
-constexpr int RT_U = 8;
+template <int RW>
+struct RouteTiles {
+  static constexpr int N = RW == 2 ? 16 : 8;
+};
 template <int RW>
 __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
                                                   const uint64_t* __restrict__ entries, const uint32_t* __restrict__ buckets,
@@ -596,6 +602,7 @@ __global__ void __launch_bounds__(256) k_p2_route(const uint64_t* __restrict__ s
                                                   uint16_t* __restrict__ rec_sub, uint32_t* __restrict__ ent_dest,
                                                   unsigned int* __restrict__ overflow,
                                                   unsigned long long* __restrict__ routed) {
+  constexpr int RT_U = RouteTiles<RW>::N;
   // per-bin bases and cursors of this chunk, num_bins of each (dynamic LDS)
   extern __shared__ __align__(8) uint8_t route_lds[];
   uint64_t* base = (uint64_t*)route_lds;

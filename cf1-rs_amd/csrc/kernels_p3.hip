@@ -437,21 +437,22 @@ __global__ void __launch_bounds__(P3_WG) k_p3_rcount(const uint64_t* __restrict_
 // consecutive lanes store consecutive records of one run. The order of
 // records inside a segment depends on the schedule; nothing downstream
 // depends on it (the dedup joins and ranks by key).
-template <int RW>
-struct RsCfg {
-  static constexpr int TILE = RW == 2 ? 1024 : 512;  // records per staged tile
-};
-template <int RW>
+// Records per staged tile: 2,048 two-word records when the tile and the
+// bin's segment counters fit 64 KiB of LDS (sb <= 10; on the 300 Mbp genome
+// at 256 bins this pass takes ~0.44 against 0.57 ms with 1,024, each
+// (tile, segment) run holding twice the records), else 1,024; 512 four-word
+// records.
+template <int RW, int TILE>
 size_t rscatter_lds(int sb) {
-  return (size_t)RsCfg<RW>::TILE * RW * 8 + (size_t)RsCfg<RW>::TILE * 2 + ((size_t)12 << sb) + 64;
+  return (size_t)TILE * RW * 8 + (size_t)TILE * 2 + ((size_t)12 << sb) + 64;
 }
-template <int RW>
+constexpr size_t RS_LDS_MAX = 65536;
+template <int RW, int TILE>
 __global__ void __launch_bounds__(P3_WG) k_p3_rscatter(const uint64_t* __restrict__ rec, const uint16_t* __restrict__ rec_sub,
                                                        const uint64_t* __restrict__ bin_start, const uint64_t* __restrict__ chunk_off,
                                                        int num_bins, uint32_t chunk_records, int sb, const uint64_t* __restrict__ roff,
                                                        unsigned int* __restrict__ rcur, uint64_t* __restrict__ srec,
                                                        uint32_t* __restrict__ rec_dest, const unsigned int* __restrict__ stop) {
-  constexpr int TILE = RsCfg<RW>::TILE;
   constexpr int PT = TILE / P3_WG;
   extern __shared__ __align__(16) uint64_t rs_lds[];
   __shared__ uint64_t sh4[4];
@@ -1724,12 +1725,16 @@ void launch_p3_rscatter(hipStream_t st, bool wide, const uint64_t* rec, const ui
                         const uint64_t* chunk_off, int num_bins, uint32_t chunk_records, uint64_t n_chunks, int sb, const uint64_t* roff,
                         unsigned int* rcur, uint64_t* srec, uint32_t* rec_dest, const unsigned int* stop) {
   if (n_chunks == 0) return;
-  if (!wide)
-    CF1G_LAUNCH("p3_rscatter", k_p3_rscatter<2>, dim3((unsigned)n_chunks), dim3(P3_WG), rscatter_lds<2>(sb), st, rec, rec_sub,
-                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
+  const size_t l2048 = rscatter_lds<2, 2048>(sb), l1024 = rscatter_lds<2, 1024>(sb), l512 = rscatter_lds<4, 512>(sb);
+  if (!wide && l2048 <= RS_LDS_MAX)
+    CF1G_LAUNCH("p3_rscatter", (k_p3_rscatter<2, 2048>), dim3((unsigned)n_chunks), dim3(P3_WG), l2048, st, rec, rec_sub, bin_start,
+                chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
+  else if (!wide)
+    CF1G_LAUNCH("p3_rscatter", (k_p3_rscatter<2, 1024>), dim3((unsigned)n_chunks), dim3(P3_WG), l1024, st, rec, rec_sub, bin_start,
+                chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
   else
-    CF1G_LAUNCH("p3_rscatter", k_p3_rscatter<4>, dim3((unsigned)n_chunks), dim3(P3_WG), rscatter_lds<4>(sb), st, rec, rec_sub,
-                bin_start, chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
+    CF1G_LAUNCH("p3_rscatter", (k_p3_rscatter<4, 512>), dim3((unsigned)n_chunks), dim3(P3_WG), l512, st, rec, rec_sub, bin_start,
+                chunk_off, num_bins, chunk_records, sb, roff, rcur, srec, rec_dest, stop);
 }
 void launch_p3_expand(hipStream_t st, bool wide, const uint64_t* srec, const uint64_t* roff, const uint64_t* seg_off,
                       const uint32_t* list, uint64_t n, int K, void* occ, uint16_t* obs, const unsigned int* stop) {
