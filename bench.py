@@ -257,6 +257,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lookups", type=int, default=1 << 28, help="lookups per batch (0: skip the lookup benchmark)")
     ap.add_argument("--lookup-batches", type=int, default=4)
+    ap.add_argument("--lidx-leaf-bits", type=int, default=0, help="learned index: log2 leaves (0: the library default, 18)")
+    ap.add_argument("--lidx-line-bytes", type=int, default=0, help="learned index: 64 or 128 (0: 128)")
+    ap.add_argument("--lidx-line-keys", type=int, default=0, help="learned index: keys per line (0: the default for the size)")
     ap.add_argument("--end-to-end", type=int, default=1, help="also time FASTA file -> P1..P4 (0: skip)")
     ap.add_argument("--ingest-threads", type=int, default=16)
     ap.add_argument("--p5", type=int, default=1, help="also time P5 output (0: skip)")
@@ -386,8 +389,10 @@ def main():
     # learned index vs BBHash lookups over device-resident present keys
     lookup = None
     if args.lookups > 0 and D > 0 and kb == 8:
+        lidx_args = {"leaf_bits": args.lidx_leaf_bits, "line_bytes": args.lidx_line_bytes, "line_keys": args.lidx_line_keys}
         lookup = lookup_bench(g, D, args.lookups, args.lookup_batches,
-                              far=profiled_far_requests(args.workload, args.k, info["num_bins"]) if args.scale == 1.0 else None)
+                              far=profiled_far_requests(args.workload, args.k, info["num_bins"]) if args.scale == 1.0 else None,
+                              lidx_args=lidx_args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -569,7 +574,7 @@ def measured_copy_peak(nbytes=2 << 30, reps=5):
     return gbs
 
 
-def lookup_bench(g, D, batch, n_batches, far=None, seed=5):
+def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
     """The learned index against the BBHash on the same n_batches x batch
     lookups of present keys (uniform over the distinct keys, seeded), keys
     and results resident in HBM. Lookups/s from the lookup kernels' HIP-event
@@ -581,7 +586,7 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5):
     import numpy as np
 
     t = time.perf_counter()
-    info = g.lidx_train()
+    info = g.lidx_train(**(lidx_args or {}))
     train_wall = time.perf_counter() - t
     print(f"[bench] learned index trained in {train_wall:.2f} s", file=sys.stderr, flush=True)
     keys = g.lidx_export()["keys"]
@@ -625,8 +630,9 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5):
     bms = kt.get("bbh_lookup", (0, 0.0))[1]
     lrate = n / (lms * 1e-3) if lms else None
     brate = n / (bms * 1e-3) if bms else None
-    keep = ("n_leaves", "n_pieces", "fallback_leaves", "fallback_keys", "tie_keys", "max_err_lo", "max_err_hi",
-            "mean_window", "mean_abs_err", "root_loss", "leaf_loss", "sort_ms", "train_ms")
+    keep = ("n_leaves", "n_pieces", "fallback_leaves", "fallback_keys", "n_lines", "line_bytes", "line_keys",
+            "overflow_lines", "search_keys", "max_err_lo", "max_err_hi", "mean_abs_err", "line_bits_per_key", "bytes_total",
+            "root_loss", "leaf_loss", "sort_ms", "train_ms")
     return {
         "n": n, "batch": batch, "keys": "uniform over the distinct keys, seed %d" % seed,
         "bbhash_lookups_per_s": brate, "learned_lookups_per_s": lrate,

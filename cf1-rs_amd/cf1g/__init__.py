@@ -42,9 +42,8 @@ EXPORTS = [
 ]
 
 # Learned-index records (the layouts of include/cf1g.h).
-ROOT_DTYPE = np.dtype([("A", "<f4"), ("B", "<f4"), ("lo", "<u4"), ("hi", "<u4")])
-LEAF_DTYPE = np.dtype([("base", "<u8"), ("q", "<u4"), ("n", "<u4"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"),
-                       ("ehi", "<u2"), ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
+ROOT_DTYPE = np.dtype([("A", "<f4"), ("B", "<f4"), ("leaf0", "<u4"), ("p", "<u4")])
+LEAF_DTYPE = np.dtype([("S", "<f4"), ("I", "<f4")])
 
 
 class Cf1gError(RuntimeError):
@@ -93,20 +92,23 @@ class TuneResult(C.Structure):
 
 class LidxParams(C.Structure):
     _fields_ = [("leaf_bits", C.c_uint32), ("root_bits", C.c_uint32), ("epochs", C.c_uint32), ("lr", C.c_float),
-                ("max_window", C.c_uint32), ("untrained", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("max_window", C.c_uint32), ("untrained", C.c_uint32), ("line_bytes", C.c_uint32), ("line_keys", C.c_uint32)]
 
 
 class LidxInfo(C.Structure):
     _fields_ = [("n_keys", C.c_uint64), ("n_leaves", C.c_uint64), ("n_pieces", C.c_uint64),
-                ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("tie_keys", C.c_uint64),
-                ("max_err_lo", C.c_uint32), ("max_err_hi", C.c_uint32), ("mean_window", C.c_double),
-                ("mean_abs_err", C.c_double), ("root_loss", C.c_double * 2), ("leaf_loss", C.c_double * 2),
-                ("sort_ms", C.c_double), ("train_ms", C.c_double)]
+                ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("n_lines", C.c_uint64),
+                ("overflow_lines", C.c_uint64), ("search_keys", C.c_uint64), ("line_bytes", C.c_uint32),
+                ("line_keys", C.c_uint32), ("max_err_lo", C.c_uint32), ("max_err_hi", C.c_uint32),
+                ("mean_abs_err", C.c_double), ("line_bits_per_key", C.c_double), ("bytes_total", C.c_uint64),
+                ("root_loss", C.c_double * 2), ("leaf_loss", C.c_double * 2), ("sort_ms", C.c_double),
+                ("train_ms", C.c_double)]
 
 
 class LidxShape(C.Structure):
     _fields_ = [("n_keys", C.c_uint64), ("n_pieces", C.c_uint32), ("n_leaves", C.c_uint32), ("root_shift", C.c_int32),
-                ("reserved", C.c_uint32)]
+                ("line_bytes", C.c_uint32), ("line_keys", C.c_uint32), ("n_lines", C.c_uint32), ("key_bits", C.c_uint32),
+                ("warp_groups", C.c_uint32)]
 
 
 def lib():
@@ -149,7 +151,7 @@ def lib():
         L.cf1g_lidx_train.argtypes = [C.c_void_p, C.POINTER(LidxParams), C.POINTER(LidxInfo)]
         L.cf1g_lidx_lookup.argtypes = [C.c_void_p, U64P, C.c_uint64, U64P]
         L.cf1g_lidx_lookup_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
-        L.cf1g_lidx_export.argtypes = [C.c_void_p, C.POINTER(LidxShape), U64P, C.c_void_p, C.c_void_p]
+        L.cf1g_lidx_export.argtypes = [C.c_void_p, C.POINTER(LidxShape), U64P, U32P, C.c_void_p, C.c_void_p, U32P, C.c_void_p]
         F32P = C.POINTER(C.c_float)
         F64P = C.POINTER(C.c_double)
         L.cf1g_lidx_gradients.argtypes = [C.c_void_p, C.c_int, F32P, F32P, C.c_uint64, F64P, F64P, F64P, U64P, F32P, F32P]
@@ -493,8 +495,9 @@ class Cf1g:
 def _lidx_methods():
     """Learned rank index methods of Cf1g (kept together here)."""
 
-    def lidx_train(self, leaf_bits=0, root_bits=0, epochs=0, lr=0.0, max_window=0, untrained=False):
-        p = LidxParams(leaf_bits, root_bits, epochs, lr, max_window, 1 if untrained else 0, (C.c_uint32 * 2)())
+    def lidx_train(self, leaf_bits=0, root_bits=0, epochs=0, lr=0.0, max_window=0, untrained=False, line_bytes=0,
+                   line_keys=0):
+        p = LidxParams(leaf_bits, root_bits, epochs, lr, max_window, 1 if untrained else 0, line_bytes, line_keys)
         o = LidxInfo()
         self._check(self.L.cf1g_lidx_train(self.h, C.byref(p), C.byref(o)))
         d = {f: getattr(o, f) for f, _ in LidxInfo._fields_}
@@ -510,13 +513,19 @@ def _lidx_methods():
 
     def lidx_export(self):
         sh = LidxShape()
-        self._check(self.L.cf1g_lidx_export(self.h, C.byref(sh), None, None, None))
+        self._check(self.L.cf1g_lidx_export(self.h, C.byref(sh), None, None, None, None, None, None))
         keys = np.zeros(max(sh.n_keys, 1), np.uint64)
-        root = np.zeros(sh.n_pieces, ROOT_DTYPE)
-        leaf = np.zeros(sh.n_leaves, LEAF_DTYPE)
-        self._check(self.L.cf1g_lidx_export(self.h, None, _p(keys, U64P), root.ctypes.data_as(C.c_void_p),
-                                            leaf.ctypes.data_as(C.c_void_p)))
-        return {"keys": keys[:sh.n_keys], "root": root, "leaf": leaf, "shift": sh.root_shift}
+        warp = np.zeros((max(sh.warp_groups, 1), 256), np.uint32)
+        root = np.zeros(sh.n_pieces + 1, ROOT_DTYPE)
+        leaf = np.zeros(max(sh.n_leaves, 1), LEAF_DTYPE)
+        err = np.zeros((max(sh.n_leaves, 1), 2), np.uint32)
+        lines = np.zeros((max(sh.n_lines, 1), sh.line_bytes), np.uint8)
+        self._check(self.L.cf1g_lidx_export(self.h, None, _p(keys, U64P), _p(warp, U32P), root.ctypes.data_as(C.c_void_p),
+                                            leaf.ctypes.data_as(C.c_void_p), _p(err, U32P),
+                                            lines.ctypes.data_as(C.c_void_p)))
+        return {"keys": keys[:sh.n_keys], "warp": warp[:sh.warp_groups], "key_bits": sh.key_bits, "root": root, "leaf": leaf[:sh.n_leaves], "err": err[:sh.n_leaves],
+                "lines": lines[:sh.n_lines], "shift": sh.root_shift, "line_bytes": sh.line_bytes,
+                "line_keys": sh.line_keys}
 
     def lidx_gradients(self, level, w, c):
         w = np.ascontiguousarray(w, np.float32)

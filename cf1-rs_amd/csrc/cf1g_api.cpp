@@ -82,12 +82,13 @@ void launch_lidx_stats_init(hipStream_t, int, const SegCtx&, uint64_t, const Lid
 void launch_lidx_fwd_bwd(hipStream_t, int, const SegCtx&, uint64_t, const LidxTrainBufs&);
 void launch_lidx_update(hipStream_t, uint64_t, float, const LidxTrainBufs&);
 void launch_lidx_pieces(hipStream_t, const uint64_t*, uint64_t, int, uint32_t, uint32_t, uint64_t*, uint32_t*);
-void launch_lidx_root_final(hipStream_t, uint32_t, uint64_t, uint32_t, const uint64_t*, const uint32_t*, const LidxTrainBufs&,
-                            RootPiece*);
+void launch_lidx_root_final(hipStream_t, uint32_t, uint64_t, const uint64_t*, const uint32_t*, const LidxTrainBufs&, RootPiece*);
 void launch_lidx_leaf_bounds(hipStream_t, const uint64_t*, uint64_t, int, uint32_t, const RootPiece*, uint64_t*);
-void launch_lidx_leaf_final(hipStream_t, const uint64_t*, uint64_t, uint32_t, const uint64_t*, const LidxTrainBufs&, uint32_t,
-                            LeafRec*, uint64_t*);
-void launch_lidx_fingerprints(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, uint16_t*, uint32_t*);
+void launch_lidx_leaf_final(hipStream_t, const uint64_t*, uint64_t, uint32_t, int, const uint64_t*, const RootPiece*,
+                            const LidxTrainBufs&, uint32_t, LeafRec*, uint32_t*, uint64_t*);
+void launch_lidx_lines(hipStream_t, int, const uint64_t*, uint64_t, uint32_t, uint64_t, uint64_t*, uint32_t*);
+void launch_lidx_warp_hist(hipStream_t, const uint64_t*, uint64_t, int, int, uint32_t*);
+void launch_lidx_warp_keys(hipStream_t, const uint64_t*, uint64_t, const uint32_t*, int, int, uint64_t*);
 void launch_lidx_bbh2rank(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, const MphfView&, uint32_t*);
 void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*);
 void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, const LidxView&, const MphfView&, uint8_t*, unsigned int*);
@@ -210,8 +211,10 @@ struct cf1g_ctx {
   // learned index
   bool lidx_ready = false, lidx_fallback = false, lidx_states = false;
   bool keys_ready = false;  // dense keys compacted since the last P3
-  uint32_t lidx_R = 0, lidx_L = 0;
+  uint32_t lidx_R = 0, lidx_L = 0, lidx_Lb = 0;  // root pieces; leaves in all (L + R); L
   int lidx_s = 0;
+  uint32_t lidx_C = 0, lidx_lines = 0, lidx_lw = 16, lidx_magic = 1, lidx_mshift = 0, lidx_steps = 1;
+  int lidx_groups = 0;
   int use_index = 0;
   // launch shapes the autotuner may change (cf1g_autotune); the params'
   // non-zero fields take precedence
@@ -272,12 +275,22 @@ struct cf1g_ctx {
     v.root = getT<RootPiece>("lidx_root", lidx_R + 1);
     v.leaf = getT<LeafRec>("lidx_leaf", lidx_L + 1);
     v.keys = getT<uint64_t>("lidx_keys", D + 1);
-    v.fp = getT<uint16_t>("lidx_fp", D + 128);
+    v.lines = getT<uint64_t>("lidx_lines", (uint64_t)lidx_lines * lidx_lw + 2);
     v.bbh2rank = lidx_fallback ? getT<uint32_t>("lidx_bbh2rank", D + 1) : nullptr;
+    v.warp = getT<uint32_t>("lidx_warp", WARP_MAX_GROUPS * 256);
     v.D = D;
-    v.L = lidx_L;
     v.R = lidx_R;
+    v.LT = lidx_L;
+    v.C = lidx_C;
+    v.n_lines = lidx_lines;
+    v.magic = lidx_magic;
+    v.mshift = lidx_mshift;
+    v.max_steps = lidx_steps;
     v.s = lidx_s;
+    v.line_words = (int)lidx_lw;
+    v.key_bits = 2 * k;
+    v.warp_groups = lidx_groups;
+    v.fallback = lidx_fallback;
     return v;
   }
   void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
@@ -973,10 +986,14 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   if (c->wide) fail(CF1G_E_INVALID, "the learned index takes u64 keys (k <= 31)");
   cf1g_lidx_params p{};
   if (prm) p = *prm;
-  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 20, root_bits = p.root_bits ? p.root_bits : 12;
-  const uint32_t epochs = p.epochs ? p.epochs : 4, max_window = p.max_window ? p.max_window : 256;
+  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 18, root_bits = p.root_bits ? p.root_bits : 10;
+  const uint32_t epochs = p.epochs ? p.epochs : 4, max_window = p.max_window ? p.max_window : 4096;
   const float lr = p.lr > 0 ? p.lr : 0.5f;
+  const uint32_t line_bytes = p.line_bytes ? p.line_bytes : 128;
   if (leaf_bits > 26 || root_bits > 20) fail(CF1G_E_INVALID, "leaf_bits <= 26 and root_bits <= 20");
+  if (line_bytes != 64 && line_bytes != 128) fail(CF1G_E_INVALID, "line_bytes is 64 or 128");
+  const uint32_t C = p.line_keys ? p.line_keys : (line_bytes == 128 ? 256u : 112u);
+  if (C < 2 || C > 4096) fail(CF1G_E_INVALID, "line_keys is in [2, 4096]");
   if (c->D >= (1ULL << 32)) fail(CF1G_E_INVALID, "the learned index holds fewer than 2^32 keys");
   hipStream_t st = c->st;
   const uint64_t D = c->D;
@@ -1001,77 +1018,124 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   if (src != ka) hip_check(hipMemcpyAsync(ka, src, D * 8, hipMemcpyDeviceToDevice, st), "D2D sorted");
   c->sync();
   auto t1 = std::chrono::steady_clock::now();
-  // ---- root pieces
-  const uint32_t rb = std::min<uint32_t>(root_bits, (uint32_t)K2);
+  // ---- key warp: byte counts per group -> cumulative tables -> warped keys
+  const int G = std::min(WARP_MAX_GROUPS, K2 / 8);
+  uint32_t* wcnt = c->getT<uint32_t>("lidx_warp_cnt", WARP_MAX_GROUPS * 256);
+  uint32_t* wtab = c->getT<uint32_t>("lidx_warp", WARP_MAX_GROUPS * 256);
+  c->zero(wcnt, WARP_MAX_GROUPS * 256 * 4);
+  launch_lidx_warp_hist(st, ka, D, K2, G, wcnt);
+  std::vector<uint32_t> hcnt(WARP_MAX_GROUPS * 256, 0), htab(WARP_MAX_GROUPS * 256, 0);
+  c->d2h(hcnt.data(), wcnt, hcnt.size() * 4);
+  for (int g = 0; g < G; g++) {
+    // cum_g[s] = floor(2^32 (sum_{t < s} (cnt_t + 1)) / (D + 256)); cum_g[256] = 2^32 is implied
+    const unsigned __int128 tot = (unsigned __int128)D + 256;
+    unsigned __int128 run = 0;
+    for (int x = 0; x < 256; x++) {
+      htab[g * 256 + x] = (uint32_t)((run << 32) / tot);
+      run += (unsigned __int128)hcnt[g * 256 + x] + 1;
+    }
+  }
+  c->h2d(wtab, htab.data(), htab.size() * 4);
+  uint64_t* uk = c->getT<uint64_t>("lidx_u", D + 1);
+  launch_lidx_warp_keys(st, ka, D, wtab, K2, G, uk);
+  // ---- root pieces by the top bits of the warped keys
+  const uint32_t rb = std::max<uint32_t>(1, std::min<uint32_t>(root_bits, 20));
   const uint32_t R = 1u << rb;
-  const int s = K2 - (int)rb;
+  const int s = 64 - (int)rb;
   uint32_t lb = 0;
   while (lb < leaf_bits && (2ULL << lb) <= std::max<uint64_t>(D, 1)) lb++;
-  const uint32_t L = 1u << lb;
+  const uint32_t L = 1u << lb, LT = L + R;
   uint64_t* pcs = c->getT<uint64_t>("lidx_p", R + 1);
   uint32_t* lo = c->getT<uint32_t>("lidx_lo", R + 1);
-  launch_lidx_pieces(st, ka, D, s, R, L, pcs, lo);
-  SegCtx sc0{ka, pcs, lo, D, L, s};
+  launch_lidx_pieces(st, uk, D, s, R, L, pcs, lo);
+  SegCtx sc0{uk, pcs, lo, D, L, s};
   double rl[2] = {0, 0}, ll[2] = {0, 0};
   lidx_train_level(c, 0, sc0, R, epochs, lr, !p.untrained, rl);
   RootPiece* root = c->getT<RootPiece>("lidx_root", R + 1);
-  launch_lidx_root_final(st, R, D, L, pcs, lo, lidx_bufs(c, R), root);
+  launch_lidx_root_final(st, R, D, pcs, lo, lidx_bufs(c, R), root);
   // ---- leaves
-  uint64_t* q = c->getT<uint64_t>("lidx_q", L + 1);
-  launch_lidx_leaf_bounds(st, ka, D, s, L, root, q);
-  SegCtx sc1{ka, q, nullptr, D, L, s};
-  lidx_train_level(c, 1, sc1, L, epochs, lr, !p.untrained, ll);
-  LeafRec* leaf = c->getT<LeafRec>("lidx_leaf", L + 1);
-  uint64_t* sabs = c->getT<uint64_t>("lidx_sabs", L + 1);
-  launch_lidx_leaf_final(st, ka, D, L, q, lidx_bufs(c, L), max_window, leaf, sabs);
-  uint16_t* fp = c->getT<uint16_t>("lidx_fp", D + 128);
-  uint32_t* ties = c->getT<uint32_t>("lidx_tiecnt", L + 1);
-  c->zero(fp, (D + 128) * 2);
-  launch_lidx_fingerprints(st, ka, L, q, leaf, fp, ties);
-  // ---- host-side summary of the leaves; fallback table when needed
-  std::vector<LeafRec> hl(L);
-  std::vector<uint64_t> hs(L);
-  std::vector<uint32_t> ht(L);
-  c->d2h(hl.data(), leaf, (size_t)L * sizeof(LeafRec));
-  c->d2h(hs.data(), sabs, (size_t)L * 8);
-  c->d2h(ht.data(), ties, (size_t)L * 4);
-  uint64_t fb_leaves = 0, fb_keys = 0, win_sum = 0, win_keys = 0, abs_sum = 0, n_ties = 0;
+  uint64_t* q = c->getT<uint64_t>("lidx_q", LT + 1);
+  launch_lidx_leaf_bounds(st, uk, D, s, LT, root, q);
+  SegCtx sc1{uk, q, nullptr, D, L, s};
+  lidx_train_level(c, 1, sc1, LT, epochs, lr, !p.untrained, ll);
+  LeafRec* leaf = c->getT<LeafRec>("lidx_leaf", LT + 1);
+  uint32_t* err = c->getT<uint32_t>("lidx_err", 2 * (uint64_t)LT + 2);
+  uint64_t* sabs = c->getT<uint64_t>("lidx_sabs", LT + 1);
+  launch_lidx_leaf_final(st, uk, D, LT, s, q, root, lidx_bufs(c, LT), max_window, leaf, err, sabs);
+  // ---- lines
+  const uint32_t lw = line_bytes / 8;
+  const uint64_t nl = (D + C - 1) / C;
+  uint64_t* lines = c->getT<uint64_t>("lidx_lines", nl * lw + 2);
+  uint32_t* lst = c->getT<uint32_t>("lidx_line_stats", 2 * nl + 2);
+  launch_lidx_lines(st, (int)lw, ka, D, C, nl, lines, lst);
+  // ---- host-side summary; fallback table when needed
+  std::vector<LeafRec> hl(LT);
+  std::vector<uint32_t> he(2 * (size_t)LT);
+  std::vector<uint64_t> hs(LT), hq(LT + 1);
+  std::vector<uint32_t> hst(2 * nl);
+  c->d2h(hl.data(), leaf, (size_t)LT * sizeof(LeafRec));
+  c->d2h(he.data(), err, (size_t)LT * 8);
+  c->d2h(hs.data(), sabs, (size_t)LT * 8);
+  c->d2h(hq.data(), q, ((size_t)LT + 1) * 8);
+  if (nl) c->d2h(hst.data(), lst, nl * 8);
+  uint64_t fb_leaves = 0, fb_keys = 0, abs_sum = 0, n_over = 0, n_search = 0, bits_sum = 0;
   uint32_t mlo = 0, mhi = 0;
-  for (uint32_t j = 0; j < L; j++) {
-    const uint64_t n = hl[j].n;
+  for (uint32_t j = 0; j < LT; j++) {
+    const uint64_t n = hq[j + 1] - hq[j];
     abs_sum += hs[j];
-    n_ties += ht[j];
     if (!n) continue;
-    if (hl[j].flags & 1) {
+    uint32_t sbits;
+    memcpy(&sbits, &hl[j].S, 4);
+    if (sbits >> 31) {
       fb_leaves++;
       fb_keys += n;
     } else {
-      win_sum += n * (uint64_t)(hl[j].elo + hl[j].ehi + 1);
-      win_keys += n;
-      mlo = std::max<uint32_t>(mlo, hl[j].elo);
-      mhi = std::max<uint32_t>(mhi, hl[j].ehi);
+      mlo = std::max(mlo, he[2 * j]);
+      mhi = std::max(mhi, he[2 * j + 1]);
     }
   }
+  for (uint64_t i = 0; i < nl; i++) {
+    if (hst[2 * i] == 0xFFFFFFFFu) n_over++;
+    else bits_sum += hst[2 * i];
+    n_search += hst[2 * i + 1];
+  }
+  // line = pred / C by multiply-high: l = ceil(log2 C), magic = floor(2^32 (2^l - C) / C) + 1
+  uint32_t l = 0;
+  while ((1ULL << l) < C) l++;
+  c->lidx_magic = (uint32_t)((((1ULL << l) - C) << 32) / C + 1);
+  c->lidx_mshift = l - 1;
   c->lidx_R = R;
-  c->lidx_L = L;
+  c->lidx_L = LT;
+  c->lidx_Lb = L;
+  c->lidx_groups = G;
   c->lidx_s = s;
+  c->lidx_C = C;
+  c->lidx_lines = (uint32_t)nl;
+  c->lidx_lw = lw;
+  c->lidx_steps = 3 + (std::max(mlo, mhi) + C - 1) / C;
   c->lidx_fallback = fb_leaves > 0;
-  if (c->lidx_fallback) launch_lidx_bbh2rank(st, ka, L, q, leaf, c->view(), c->getT<uint32_t>("lidx_bbh2rank", D + 1));
+  if (c->lidx_fallback) launch_lidx_bbh2rank(st, ka, LT, q, leaf, c->view(), c->getT<uint32_t>("lidx_bbh2rank", D + 1));
+  c->sync();
   auto t2 = std::chrono::steady_clock::now();
   c->lidx_ready = true;
   c->lidx_states = false;
   if (info) {
     cf1g_lidx_info o{};
     o.n_keys = D;
-    o.n_leaves = L;
+    o.n_leaves = LT;
     o.n_pieces = R;
     o.fallback_leaves = fb_leaves;
     o.fallback_keys = fb_keys;
-    o.tie_keys = n_ties;
+    o.n_lines = nl;
+    o.overflow_lines = n_over;
+    o.search_keys = n_search;
+    o.line_bytes = line_bytes;
+    o.line_keys = C;
     o.max_err_lo = mlo;
     o.max_err_hi = mhi;
-    o.mean_window = win_keys ? (double)win_sum / (double)win_keys : 0.0;
     o.mean_abs_err = D ? (double)abs_sum / (double)D : 0.0;
+    o.line_bits_per_key = D ? (double)bits_sum / (double)D : 0.0;
+    o.bytes_total = (uint64_t)(R + 1) * sizeof(RootPiece) + (uint64_t)LT * sizeof(LeafRec) + nl * line_bytes;
     o.root_loss[0] = rl[0];
     o.root_loss[1] = rl[1];
     o.leaf_loss[0] = ll[0];
@@ -1878,7 +1942,8 @@ int cf1g_lidx_lookup_device(cf1g_ctx* c, const void* d_keys, uint64_t n, uint64_
   });
 }
 
-int cf1g_lidx_export(cf1g_ctx* c, cf1g_lidx_shape* shape, uint64_t* sorted_keys, void* root, void* leaves) {
+int cf1g_lidx_export(cf1g_ctx* c, cf1g_lidx_shape* shape, uint64_t* sorted_keys, uint32_t* warp, void* root, void* leaves,
+                     uint32_t* leaf_err, void* lines) {
   return guarded(c, [&]() {
     if (!c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_lidx_export called before cf1g_lidx_train");
     if (shape) {
@@ -1887,11 +1952,19 @@ int cf1g_lidx_export(cf1g_ctx* c, cf1g_lidx_shape* shape, uint64_t* sorted_keys,
       shape->n_pieces = c->lidx_R;
       shape->n_leaves = c->lidx_L;
       shape->root_shift = c->lidx_s;
+      shape->line_bytes = c->lidx_lw * 8;
+      shape->line_keys = c->lidx_C;
+      shape->n_lines = c->lidx_lines;
+      shape->key_bits = 2 * c->k;
+      shape->warp_groups = (uint32_t)c->lidx_groups;
     }
     const LidxView v = c->lview();
     if (sorted_keys && c->D) c->d2h(sorted_keys, v.keys, c->D * 8);
-    if (root) c->d2h(root, v.root, (size_t)c->lidx_R * sizeof(RootPiece));
+    if (warp && c->lidx_groups) c->d2h(warp, v.warp, (size_t)c->lidx_groups * 256 * 4);
+    if (root) c->d2h(root, v.root, ((size_t)c->lidx_R + 1) * sizeof(RootPiece));
     if (leaves) c->d2h(leaves, v.leaf, (size_t)c->lidx_L * sizeof(LeafRec));
+    if (leaf_err) c->d2h(leaf_err, c->getT<uint32_t>("lidx_err", 2 * (uint64_t)c->lidx_L + 2), (size_t)c->lidx_L * 8);
+    if (lines && c->lidx_lines) c->d2h(lines, v.lines, (size_t)c->lidx_lines * c->lidx_lw * 8);
   });
 }
 
@@ -1902,9 +1975,9 @@ int cf1g_lidx_gradients(cf1g_ctx* c, int level, const float* w, const float* cc,
     if (level != 0 && level != 1) fail(CF1G_E_INVALID, "level is 0 (root) or 1 (leaves)");
     const uint64_t ns = level == 0 ? c->lidx_R : c->lidx_L;
     if (n_segs != ns || !w || !cc) fail(CF1G_E_INVALID, "one (w, c) pair per segment is required");
-    uint64_t* ka = c->getT<uint64_t>("lidx_keys", c->D + 1);
+    uint64_t* ka = c->getT<uint64_t>("lidx_u", c->D + 1);
     SegCtx sc{ka, level == 0 ? c->getT<uint64_t>("lidx_p", ns + 1) : c->getT<uint64_t>("lidx_q", ns + 1),
-              level == 0 ? c->getT<uint32_t>("lidx_lo", ns + 1) : nullptr, c->D, c->lidx_L, c->lidx_s};
+              level == 0 ? c->getT<uint32_t>("lidx_lo", ns + 1) : nullptr, c->D, c->lidx_Lb, c->lidx_s};
     const LidxTrainBufs b = lidx_bufs(c, ns);
     launch_lidx_stats_init(c->st, level, sc, ns, b);
     c->h2d(b.w, w, ns * 4);

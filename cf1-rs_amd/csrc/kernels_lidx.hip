@@ -1,22 +1,30 @@
 // Learned rank index over the globally sorted distinct k-mers (u64 keys,
 // k <= 31): a two-level piecewise-linear FP32 model, trained on the GPU by
-// gradient descent with the forward / backward kernels below, that predicts
-// a key's rank in the sorted array within a recorded per-leaf error bound; a
-// bounded search turns the prediction into the exact rank (a monotone minimal
+// gradient descent with the forward / backward kernels below, predicts a
+// key's rank within a recorded per-leaf error bound, and compact lines of the
+// sorted keys turn the prediction into the exact rank (a monotone minimal
 // perfect map). No reference counterpart: it sits beside the BBHash
 // (mphf.rs:268-276 is the lookup it can replace).
 //
 //  sort     : LSD radix sort of the keys, 8-bit digits (k_sort_hist / scan /
 //             k_sort_scatter), stable tile ranking in registers.
-//  root     : R = 2^rb pieces by the top rb key bits. Piece r (keys
-//             [p_r, p_{r+1})) maps x = key - (r << s) to a leaf:
-//             leaf = lo_r + clamp(floor(fma(A_r, x, B_r)), 0, hi_r - lo_r),
-//             trained towards y = rank * L / D - lo_r, with
-//             lo_r = floor(p_r L / D), hi_r = max(lo_r, ceil(p_{r+1} L / D) - 1).
-//  leaves   : L = 2^lb linear models over the keys the root routes to them
-//             (contiguous ranks [q_j, q_{j+1}) because routing is monotone):
-//             pred = q_j + floor(fma(S_j, (float)(key - base_j), I_j) + 0.5),
-//             trained towards y = rank - q_j.
+//  warp     : u = an order-0 arithmetic-coding CDF of the key's first four
+//             bytes (symbol counts of the keys themselves, histogrammed on
+//             the GPU), non-decreasing in the key and close to uniform, so
+//             the linear models below fit the keys' random spacing instead
+//             of the genome's base composition (lidx.hpp gives the exact
+//             fixed-point recurrence).
+//  root     : R = 2^rb pieces by the top rb bits of u. Piece r (keys
+//             [p_r, p_{r+1})) owns leaves [leaf0_r, leaf0_{r+1}), leaf0_r =
+//             floor(p_r L / D) + r, and maps x = u - (r << s) to
+//             leaf0_r + clamp(floor(fma(A_r, x, B_r)), 0, n_r - 1), trained
+//             towards y = rank * L / D - floor(p_r L / D).
+//  leaves   : L + R linear models of the rank over the keys the root routes
+//             to them, trained in leaf coordinates (x = u - first u,
+//             y = rank - first rank) and stored in the piece's:
+//             pred = p_r + floor(fma(S, (float)x, I) + 0.5), 8 bytes a leaf,
+//             so that the root and leaf tables (~2 MB at 2^18 leaves) stay in
+//             L2.
 //  training : per segment (piece or leaf), inputs standardised with the
 //             segment's mean / deviation: z = (x - mu) * isd, f = w z + c,
 //             MSE loss; forward writes residuals f - y, backward reduces the
@@ -24,16 +32,21 @@
 //             starting point (and the untrained method) is endpoint
 //             interpolation through the segment's first and last key.
 //  bounds   : elo_j = max(pred - rank), ehi_j = max(rank - pred) over the
-//             leaf's keys, evaluated with the exact lookup arithmetic, so a
-//             present key always lies in [pred - elo, pred + ehi].
-//  search   : over a 2-byte array aligned with the sorted keys: the 16-bit
-//             fingerprint (key - base_j) >> shift_j, non-decreasing inside a
-//             leaf. The walk starts at the 16-byte chunk holding the
-//             prediction and moves left / right by comparing fingerprints;
-//             a fingerprint shared with a neighbour resolves with the full
-//             keys, and leaves
-//             whose window exceeds the threshold resolve through the BBHash
-//             value mapped to its rank.
+//             leaf's keys, evaluated with the exact lookup arithmetic; a leaf
+//             whose window elo + ehi + 1 exceeds the threshold resolves
+//             through the BBHash value mapped to its rank.
+//  lines    : C consecutive keys by rank (and the next line's first key) in
+//             one 64- or 128-byte line: the first key, a cell shift t, the
+//             keys' cells (key - k0) >> t in unary, and for keys sharing a
+//             cell with their predecessor the highest bit where the two
+//             differ (3-bit code below t). ~3.2 bits a key.
+//  lookup   : warp (tables in LDS) -> root (L2) -> leaf (L2) -> predicted
+//             rank -> line pred / C (one
+//             far request) -> the key's cell in the unary, its run, and a
+//             blind descent over the run's branch bits; the walk steps to a
+//             neighbour line when the key lies before the line's first key
+//             or after its last cell, and searches the sorted keys for the
+//             rare runs the codes cannot resolve and for lines that overflow.
 #include "block.hpp"
 #include "device.hpp"
 #include "launch.hpp"
@@ -95,18 +108,64 @@ __global__ void __launch_bounds__(SRT_WG) k_sort_scatter(const uint64_t* __restr
   }
 }
 
-__device__ __forceinline__ uint32_t root_leaf(const RootPiece& P, uint64_t key, int s, uint32_t r) {
-  const float x = (float)(key - ((uint64_t)r << s));
-  const float f = floorf(__fmaf_rn(P.A, x, P.B));
-  const float span = (float)(P.hi - P.lo);
-  const float c = f < 0.f ? 0.f : (f > span ? span : f);
-  return P.lo + (uint32_t)c;
+// ------------------------------------------------------------------ warp
+// floor(w c / 2^32) for c < 2^32.
+__device__ __forceinline__ uint64_t warp_f(uint64_t w, uint32_t c) {
+  return (w >> 32) * (uint64_t)c + (uint64_t)__umulhi((uint32_t)w, c);
 }
-__device__ __forceinline__ int64_t leaf_pred(const LeafRec& R, uint64_t key) {
-  const float x = (float)(int64_t)(key - R.base);
-  float p = floorf(__fmaf_rn(R.S, x, R.I) + 0.5f);
+__device__ __forceinline__ uint64_t warp_key(uint64_t key, const uint32_t* __restrict__ cum, int K2, int G) {
+  uint64_t lo = 0, w = ~0ULL;
+#pragma unroll
+  for (int g = 0; g < WARP_MAX_GROUPS; g++) {
+    if (g < G) {  // uniform
+      const uint32_t sym = (uint32_t)(key >> (K2 - 8 * (g + 1))) & 255u;
+      const uint32_t c0 = cum[g * 256 + sym], c1 = cum[g * 256 + (sym < 255u ? sym + 1 : 255u)];
+      const uint64_t a = warp_f(w, c0);
+      const uint64_t b = sym == 255u ? w : warp_f(w, c1);
+      lo += a;
+      w = b - a;
+    }
+  }
+  const int rb = K2 - 8 * G;
+  if (rb > 0) lo += __umul64hi(w, (key & ((1ULL << rb) - 1)) << (64 - rb));
+  return lo;
+}
+
+// Byte counts per warp group over the sorted keys: cnt[g * 256 + s].
+__global__ void __launch_bounds__(256) k_lidx_warp_hist(const uint64_t* __restrict__ keys, uint64_t D, int K2, int G,
+                                                        uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[WARP_MAX_GROUPS * 256];
+  for (int i = threadIdx.x; i < WARP_MAX_GROUPS * 256; i += 256) h[i] = 0;
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < D; i += stride) {
+    const uint64_t k = keys[i];
+    for (int g = 0; g < G; g++) atomicAdd(&h[g * 256 + ((uint32_t)(k >> (K2 - 8 * (g + 1))) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < G * 256; i += 256)
+    if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+__global__ void __launch_bounds__(256) k_lidx_warp_keys(const uint64_t* __restrict__ keys, uint64_t D, const uint32_t* __restrict__ cum,
+                                                        int K2, int G, uint64_t* __restrict__ u) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < D) u[i] = warp_key(keys[i], cum, K2, G);
+}
+
+// Leaf of a key in piece r (P = root[r], leaf0n = root[r + 1].leaf0).
+__device__ __forceinline__ uint32_t root_leaf(const RootPiece& P, uint32_t leaf0n, float x) {
+  const float f = floorf(__fmaf_rn(P.A, x, P.B));
+  const float span = (float)(leaf0n - P.leaf0 - 1);
+  const float c = f < 0.f ? 0.f : (f > span ? span : f);
+  return P.leaf0 + (uint32_t)c;
+}
+__device__ __forceinline__ float piece_x(uint64_t key, int s, uint32_t r) { return (float)(key - ((uint64_t)r << s)); }
+__device__ __forceinline__ bool leaf_fallback(const LeafRec& Lf) { return (__float_as_uint(Lf.S) >> 31) != 0; }
+__device__ __forceinline__ int64_t leaf_pred(const RootPiece& P, const LeafRec& Lf, float x) {
+  const float S = __uint_as_float(__float_as_uint(Lf.S) & 0x7FFFFFFFu);
+  float p = floorf(__fmaf_rn(S, x, Lf.I) + 0.5f);
   p = p < -1.0e12f ? -1.0e12f : (p > 1.0e12f ? 1.0e12f : p);
-  return (int64_t)R.q + (int64_t)p;
+  return (int64_t)P.p + (int64_t)p;
 }
 
 // ------------------------------------------------------------------ training
@@ -272,8 +331,8 @@ __global__ void __launch_bounds__(256) k_lidx_tile_seg(const uint64_t* __restric
   seg_of_tile[t] = (uint32_t)lo;
 }
 
-// p[r] = first rank with key >= r << s, r in [0, R]; lo / hi leaf ranges.
-__global__ void __launch_bounds__(256) k_lidx_pieces(const uint64_t* __restrict__ keys, uint64_t D, int s, uint32_t R, uint32_t L,
+// p[r] = first rank with key >= r << s, r in [0, R] (p[R] = D).
+__global__ void __launch_bounds__(256) k_lidx_pieces(const uint64_t* __restrict__ keys, uint64_t D, int s, uint32_t R,
                                                      uint64_t* __restrict__ p) {
   const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (r > R) return;
@@ -289,230 +348,583 @@ __global__ void __launch_bounds__(256) k_lidx_pieces(const uint64_t* __restrict_
   }
   p[r] = lo;
 }
+// lo[r] = floor(p_r L / D), r in [0, R] (the root's training offset).
 __global__ void __launch_bounds__(256) k_lidx_root_ranges(const uint64_t* __restrict__ p, uint32_t R, uint64_t D, uint32_t L,
                                                           uint32_t* __restrict__ lo) {
   const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= R) return;
+  if (r > R) return;
   lo[r] = (uint32_t)(D ? (p[r] * (uint64_t)L / D) : 0);
-  if (lo[r] >= L) lo[r] = L - 1;
 }
 
-// Root records in lookup form: A = w isd, B = c - w mu isd (A >= 0), the leaf
-// range [lo_r, hi_r].
-__global__ void __launch_bounds__(256) k_lidx_root_final(uint32_t R, uint64_t D, uint32_t L, const uint64_t* __restrict__ p,
+// Root records in lookup form: A = w isd, B = c - w mu isd (A >= 0), the
+// piece's first leaf lo_r + r and first rank; record R closes the table.
+__global__ void __launch_bounds__(256) k_lidx_root_final(uint32_t R, uint64_t D, const uint64_t* __restrict__ p,
                                                          const uint32_t* __restrict__ lo, const float* __restrict__ mu,
                                                          const float* __restrict__ isd, const float* __restrict__ w,
                                                          const float* __restrict__ cc, RootPiece* __restrict__ out) {
   const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= R) return;
-  double A = (double)w[r] * (double)isd[r];
-  double B = (double)cc[r] - A * (double)mu[r];
-  if (!(A >= 0)) A = 0;  // monotone routing (also catches NaN)
-  if (!(B == B)) B = 0;
+  if (r > R) return;
   RootPiece P;
-  P.A = (float)A;
-  P.B = (float)B;
-  P.lo = lo[r];
-  uint64_t h = D ? (p[r + 1] * (uint64_t)L + D - 1) / D : 0;  // ceil(p_{r+1} L / D)
-  uint32_t hi = h > 0 ? (uint32_t)(h - 1) : 0;
-  if (hi < P.lo) hi = P.lo;
-  if (hi > L - 1) hi = L - 1;
-  P.hi = hi;
+  P.leaf0 = lo[r] + (uint32_t)r;
+  P.p = (uint32_t)p[r];
+  if (r == R) {
+    P.A = P.B = 0.f;
+  } else {
+    double A = (double)w[r] * (double)isd[r];
+    double B = (double)cc[r] - A * (double)mu[r];
+    if (!(A >= 0)) A = 0;  // monotone routing (also catches NaN)
+    if (!(B == B)) B = 0;
+    P.A = (float)A;
+    P.B = (float)B;
+  }
   out[r] = P;
 }
 
 // Leaf boundaries from the root routing of every key (non-decreasing in the
-// rank): q[j] = first rank routed to a leaf >= j, q[L] = D.
-__global__ void __launch_bounds__(256) k_lidx_leaf_bounds(const uint64_t* __restrict__ keys, uint64_t D, int s, uint32_t L,
+// rank): q[j] = first rank routed to a leaf >= j, q[LT] = D.
+__global__ void __launch_bounds__(256) k_lidx_leaf_bounds(const uint64_t* __restrict__ keys, uint64_t D, int s, uint32_t LT,
                                                           const RootPiece* __restrict__ root, uint64_t* __restrict__ q) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i > D) return;
-  int64_t prev = -1, cur = (int64_t)L;
+  int64_t prev = -1, cur = (int64_t)LT;
   if (i > 0) {
     const uint64_t k = keys[i - 1];
     const uint32_t r = (uint32_t)(k >> s);
-    prev = root_leaf(root[r], k, s, r);
+    prev = root_leaf(root[r], root[r + 1].leaf0, piece_x(k, s, r));
   }
   if (i < D) {
     const uint64_t k = keys[i];
     const uint32_t r = (uint32_t)(k >> s);
-    cur = root_leaf(root[r], k, s, r);
+    cur = root_leaf(root[r], root[r + 1].leaf0, piece_x(k, s, r));
   }
   for (int64_t j = prev + 1; j <= cur; j++) q[j] = i;
 }
 
-// Leaf records: S = w isd, I = c - w mu isd; fingerprint shift from the span;
-// error bounds over the leaf's keys (one wave per leaf); flags.
-__global__ void __launch_bounds__(256) k_lidx_leaf_final(const uint64_t* __restrict__ keys, uint64_t D, uint32_t L,
-                                                         const uint64_t* __restrict__ q, const float* __restrict__ mu,
-                                                         const float* __restrict__ isd, const float* __restrict__ w,
-                                                         const float* __restrict__ cc, uint32_t max_window,
-                                                         LeafRec* __restrict__ out, uint64_t* __restrict__ sabs) {
+// Leaf records in piece coordinates: S = w isd and, with I = c - S mu the
+// intercept in leaf coordinates, I' = I - S (first key - (r << s)) + (q_j -
+// p_r); error bounds over the leaf's keys with the lookup arithmetic (one
+// wave per leaf); the sign bit of S marks a window above max_window.
+__global__ void __launch_bounds__(256) k_lidx_leaf_final(const uint64_t* __restrict__ keys, uint64_t D, uint32_t LT, int s,
+                                                         const uint64_t* __restrict__ q, const RootPiece* __restrict__ root,
+                                                         const float* __restrict__ mu, const float* __restrict__ isd,
+                                                         const float* __restrict__ w, const float* __restrict__ cc,
+                                                         uint32_t max_window, LeafRec* __restrict__ out,
+                                                         uint32_t* __restrict__ err, uint64_t* __restrict__ sabs) {
   const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t lane = __lane_id();
-  if (j >= L) return;
+  if (j >= LT) return;
   const uint64_t a = q[j], b = q[j + 1];
-  LeafRec R;
-  R.base = a < D ? keys[a] : (D ? keys[D - 1] : 0);
-  R.q = (uint32_t)a;
-  R.n = (uint32_t)(b - a);
-  double S = (double)w[j] * (double)isd[j];
-  double I = (double)cc[j] - S * (double)mu[j];
-  if (!(S == S) || !(I == I)) S = I = 0;
-  R.S = (float)S;
-  R.I = (float)I;
-  int sh = 0;
-  if (b > a + 1) {
-    const uint64_t span = keys[b - 1] - keys[a];
-    const int bits = 64 - __clzll(span);
-    sh = bits > 16 ? bits - 16 : 0;
-  }
-  R.shift = (uint8_t)sh;
-  R.pad = 0;
+  LeafRec Lf{0.f, 0.f};
   int64_t elo = 0, ehi = 0;
   uint64_t sa = 0;
-  for (uint64_t i = a + lane; i < b; i += 64) {
-    const int64_t e = (int64_t)i - leaf_pred(R, keys[i]);
-    elo = -e > elo ? -e : elo;
-    ehi = e > ehi ? e : ehi;
-    sa += (uint64_t)(e < 0 ? -e : e);
+  if (b > a) {
+    const uint64_t k0 = keys[a];
+    const uint32_t r = (uint32_t)(k0 >> s);
+    const RootPiece P = root[r];
+    double S = (double)w[j] * (double)isd[j];
+    double I = (double)cc[j] - S * (double)mu[j];
+    I = I - S * (double)(k0 - ((uint64_t)r << s)) + (double)((int64_t)a - (int64_t)P.p);
+    if (!(S == S) || !(I == I) || S < 0) S = I = 0;
+    Lf.S = (float)S;
+    Lf.I = (float)I;
+    for (uint64_t i = a + lane; i < b; i += 64) {
+      const int64_t e = (int64_t)i - leaf_pred(P, Lf, piece_x(keys[i], s, r));
+      elo = -e > elo ? -e : elo;
+      ehi = e > ehi ? e : ehi;
+      sa += (uint64_t)(e < 0 ? -e : e);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int64_t o1 = __shfl_xor(elo, d, 64), o2 = __shfl_xor(ehi, d, 64);
+      elo = o1 > elo ? o1 : elo;
+      ehi = o2 > ehi ? o2 : ehi;
+      sa += __shfl_xor(sa, d, 64);
+    }
+    if ((uint64_t)(elo + ehi + 1) > max_window) Lf.S = __uint_as_float(__float_as_uint(Lf.S) | 0x80000000u);
   }
-  for (int d = 32; d >= 1; d >>= 1) {
-    const int64_t o1 = __shfl_xor(elo, d, 64), o2 = __shfl_xor(ehi, d, 64);
-    elo = o1 > elo ? o1 : elo;
-    ehi = o2 > ehi ? o2 : ehi;
-    sa += __shfl_xor(sa, d, 64);
-  }
-  const bool wide = elo > 65535 || ehi > 65535 || (uint64_t)(elo + ehi + 1) > max_window;
-  R.elo = (uint16_t)(elo > 65535 ? 65535 : elo);
-  R.ehi = (uint16_t)(ehi > 65535 ? 65535 : ehi);
-  R.flags = (b > a && wide) ? 1 : 0;
   if (lane == 0) {
-    out[j] = R;
+    out[j] = Lf;
+    err[2 * j] = (uint32_t)(elo > 0xFFFFFFFFLL ? 0xFFFFFFFFLL : elo);
+    err[2 * j + 1] = (uint32_t)(ehi > 0xFFFFFFFFLL ? 0xFFFFFFFFLL : ehi);
     sabs[j] = sa;
   }
 }
 
-// Fingerprints (key - base) >> shift, one wave per leaf; ties[j] = keys of
-// leaf j whose fingerprint equals a neighbour's.
-__device__ __forceinline__ uint32_t fp_of(const LeafRec& R, uint64_t key) { return (uint32_t)((key - R.base) >> R.shift) & 0xFFFFu; }
-__global__ void __launch_bounds__(256) k_lidx_fingerprints(const uint64_t* __restrict__ keys, uint32_t L,
-                                                           const uint64_t* __restrict__ q, const LeafRec* __restrict__ leaf,
-                                                           uint16_t* __restrict__ fp, uint32_t* __restrict__ ties) {
-  const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= L) return;
-  const LeafRec R = leaf[j];
-  const uint64_t a = q[j], b = q[j + 1];
-  uint32_t t = 0;
-  for (uint64_t i = a + __lane_id(); i < b; i += 64) {
-    const uint32_t f = fp_of(R, keys[i]);
-    bool tie = false;
-    if (i > a) tie |= fp_of(R, keys[i - 1]) == f;
-    if (i + 1 < b) tie |= fp_of(R, keys[i + 1]) == f;
-    fp[i] = (uint16_t)f;
-    t += tie ? 1u : 0u;
-  }
-  for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
-  if (__lane_id() == 0) ties[j] = t;
-}
-
 // Fallback leaves: bbh2rank[BBHash value of key i] = i, one wave per leaf.
-__global__ void __launch_bounds__(256) k_lidx_bbh2rank(const uint64_t* __restrict__ keys, uint32_t L, const uint64_t* __restrict__ q,
+__global__ void __launch_bounds__(256) k_lidx_bbh2rank(const uint64_t* __restrict__ keys, uint32_t LT, const uint64_t* __restrict__ q,
                                                        const LeafRec* __restrict__ leaf, const MphfView mv,
                                                        uint32_t* __restrict__ bbh2rank) {
   const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= L || !(leaf[j].flags & 1)) return;
-  const uint64_t D = q[L];
+  if (j >= LT || !leaf_fallback(leaf[j])) return;
+  const uint64_t D = q[LT];
   for (uint64_t i = q[j] + __lane_id(); i < q[j + 1]; i += 64) {
     const uint64_t v = mphf_lookup<uint64_t>(mv, keys[i]);
     if (v < D) bbh2rank[v] = (uint32_t)i;
   }
 }
 
+// ------------------------------------------------------------------ lines
+__device__ __forceinline__ int bitlen64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
+
+// One thread per line, built in its own LDS slot (padded against bank
+// conflicts) and stored as line_words u64 words. stats[2i] = bits used
+// (0xFFFFFFFF for an overflow line), stats[2i + 1] = keys of this line's runs
+// that resolve by search (a code of 7 or more than 21 codes).
+template <int NW>
+__global__ void __launch_bounds__(256) k_lidx_lines(const uint64_t* __restrict__ keys, uint64_t D, uint32_t C, uint64_t n_lines,
+                                                    uint64_t* __restrict__ lines, uint32_t* __restrict__ stats) {
+  constexpr int NU = 2 * NW, STRIDE = NU + 1;
+  __shared__ uint32_t sb[256 * STRIDE];
+  uint32_t* b = sb + threadIdx.x * STRIDE;
+#pragma unroll
+  for (int u = 0; u < NU; u++) b[u] = 0;
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_lines) return;
+  auto put = [&](uint32_t pos, uint32_t val, int width) {  // width <= 32
+    const uint32_t w = pos >> 5, o = pos & 31;
+    b[w] |= val << o;
+    if (o + width > 32) b[w + 1] |= val >> (32 - o);
+  };
+  const uint64_t a = i * C;
+  const uint32_t n = (uint32_t)((D - a) < C ? (D - a) : C);
+  const uint32_t m = n + ((a + n < D) ? 1u : 0u);
+  const uint64_t k0 = keys[a];
+  const uint64_t span = keys[a + m - 1] - k0;
+  const int tb = max(0, bitlen64(span) - bitlen64(m));
+  const int t_lo = max(0, tb - 2), t_hi = min(61, tb + 3);
+  uint32_t cnt[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t prev = 0;
+  for (uint32_t j = 1; j < m; j++) {
+    const uint64_t rel = keys[a + j] - k0;
+    const int bl = bitlen64(rel ^ prev);
+#pragma unroll
+    for (int u = 0; u < 6; u++) cnt[u] += (bl <= t_lo + u) ? 1u : 0u;
+    prev = rel;
+  }
+  int t = t_lo;
+  uint64_t best = ~0ULL;
+#pragma unroll
+  for (int u = 0; u < 6; u++) {
+    if (t_lo + u > t_hi) break;
+    const uint64_t bits = (uint64_t)m + (span >> (t_lo + u)) + 1 + (uint64_t)LINE_CODE_BITS * cnt[u];
+    if (bits < best) {
+      best = bits;
+      t = t_lo + u;
+    }
+  }
+  b[0] = (uint32_t)k0;
+  b[1] = (uint32_t)(k0 >> 32) & 0x3FFFFFFFu;
+  uint32_t n_search = 0;
+  if (best > (uint64_t)(NW * 64 - LINE_HDR_BITS)) {
+    b[1] |= 1u << 30;  // bit 62: overflow
+    stats[2 * i] = 0xFFFFFFFFu;
+    n_search = n;
+  } else {
+    const uint32_t cells = (uint32_t)(span >> t) + 1;
+    const uint32_t ulen = m + cells;
+    b[2] |= (uint32_t)t | (ulen << 6);
+    uint32_t pos = LINE_HDR_BITS, cpos = LINE_HDR_BITS + ulen, run = 1;
+    uint64_t prel = 0, pcell = 0;
+    bool bad = false;
+    for (uint32_t j = 0; j < m; j++) {
+      const uint64_t rel = keys[a + j] - k0;
+      const uint64_t c = rel >> t;
+      pos += (uint32_t)(c - pcell);
+      b[pos >> 5] |= 1u << (pos & 31);
+      pos++;
+      if (j > 0 && c == pcell) {
+        const int code = min(t - bitlen64(rel ^ prel), (int)LINE_CODE_SEARCH);
+        put(cpos, (uint32_t)code, LINE_CODE_BITS);
+        cpos += LINE_CODE_BITS;
+        run++;
+        bad |= code == (int)LINE_CODE_SEARCH;
+      } else if (j > 0) {
+        if (run > 1 && (bad || run - 1 > LINE_MAX_RUN_CODES)) n_search += run;
+        run = 1;
+        bad = false;
+      }
+      pcell = c;
+      prel = rel;
+    }
+    if (run > 1 && (bad || run - 1 > LINE_MAX_RUN_CODES)) n_search += run;
+    stats[2 * i] = (uint32_t)best;
+  }
+  stats[2 * i + 1] = n_search;
+  uint64_t* dst = lines + i * NW;
+#pragma unroll
+  for (int u = 0; u < NW; u++) dst[u] = (uint64_t)b[2 * u] | ((uint64_t)b[2 * u + 1] << 32);
+}
+
 // ------------------------------------------------------------------ lookup
-// Exact rank of a present key (absent keys give D, "not found"; callers map
-// it to 0 like mphf.rs:269). One lane per key: the walk reads the 16-byte
-// chunk of 8 fingerprints holding the prediction and steps one chunk left
-// or right while every fingerprint in it is larger / smaller than the key's
-// (adjacent chunks usually share the cache line).
-__device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key) {
-  if ((key >> V.s) >= V.R) return V.D;  // wider than a 2k-bit key
-  const uint32_t r = (uint32_t)(key >> V.s);
+// Position of the r-th (0-based) set bit of x (x has more than r).
+__device__ __forceinline__ uint32_t select64(uint64_t x, uint32_t r) {
+  uint32_t pos = 0, c;
+  c = __popc((uint32_t)x);
+  if (r >= c) { r -= c; x >>= 32; pos += 32; }
+  c = __popc((uint32_t)x & 0xFFFFu);
+  if (r >= c) { r -= c; x >>= 16; pos += 16; }
+  c = __popc((uint32_t)x & 0xFFu);
+  if (r >= c) { r -= c; x >>= 8; pos += 8; }
+  c = __popc((uint32_t)x & 0xFu);
+  if (r >= c) { r -= c; x >>= 4; pos += 4; }
+  c = __popc((uint32_t)x & 0x3u);
+  if (r >= c) { r -= c; x >>= 2; pos += 2; }
+  c = (uint32_t)x & 1u;
+  if (r >= c) pos += 1;
+  return pos;
+}
+enum LineStep : uint32_t { LS_FOUND, LS_ABSENT, LS_SEARCH };
+struct LineResult {
+  uint32_t kind;
+  uint32_t a, b;   // FOUND: index a; SEARCH: indices [a, b] of the line (b = ~0: to the last key)
+  bool right_ok;   // SEARCH: a key above them all lies further right
+};
+
+// The run of key q in the line whose header admitted it (k0 <= q, q's cell
+// below the line's cell count). Register chains pick words by index: a lane
+// holds its own line, so no word index is uniform across the wave.
+template <int NW>
+__device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint64_t q, bool has_sent, uint32_t C) {
+  LineResult R{LS_ABSENT, 0, 0, false};
+  const uint64_t k0 = W[0] & ((1ULL << 62) - 1);
+  if ((W[0] >> 62) & 1) {
+    R.kind = LS_SEARCH;
+    R.a = 0;
+    R.b = has_sent ? C : 0xFFFFFFFFu;
+    R.right_ok = has_sent;
+    return R;
+  }
+  const uint32_t t = (uint32_t)W[1] & 63u, ulen = (uint32_t)(W[1] >> 6) & 1023u;
+  const uint64_t rel = q - k0;
+  const uint32_t cq = (uint32_t)(rel >> t);
+  // Zero #(cq - 1) closes the cell before q's: the run starts right after it
+  // (or at bit 80 for cq = 0). Zeros are counted from bit 80 on (the header
+  // bits of word 1 read as ones); words past the unary hold codes, but the
+  // zero sought lies inside the unary, so their counts never decide.
+  uint32_t start = LINE_HDR_BITS;
+  if (cq) {
+    const uint32_t k = cq - 1;
+    uint32_t cz = 0, base = 0, nbefore = 0;
+    uint64_t zsel = 0;
+#pragma unroll
+    for (int w = 1; w < NW; w++) {
+      const uint64_t Z = ~(w == 1 ? (W[1] | 0xFFFFULL) : W[w]);
+      const bool in = cz <= k;  // zero #k lies in this word or a later one
+      zsel = in ? Z : zsel;
+      base = in ? cz : base;
+      nbefore += in ? 1u : 0u;
+      cz += __popcll(Z);
+    }
+    start = 64u * nbefore + select64(zsel, k - base) + 1u;
+  }
+  // the run of ones from `start`, and the collisions before it (a 1 right
+  // after a 1 is a key in its predecessor's cell: one code each)
+  const uint32_t ws = start >> 6, os = start & 63u;
+  uint64_t w0 = 0, w1 = 0, csel = 0;
+  uint32_t ccum = 0, cbase = 0;
+#pragma unroll
+  for (int w = 1; w < NW; w++) {
+    uint64_t Cw = W[w] & ((W[w] << 1) | (W[w - 1] >> 63));
+    if (w == 1) Cw &= ~0x1FFFFULL;  // bits 64..80: header, and the first key
+    const bool at = (uint32_t)w == ws;
+    w0 = at ? W[w] : w0;
+    csel = at ? Cw : csel;
+    cbase = at ? ccum : cbase;
+    w1 = ((uint32_t)w == ws + 1) ? W[w] : w1;
+    ccum += __popcll(Cw);
+  }
+  const uint64_t win = os ? ((w0 >> os) | (w1 << (64 - os))) : w0;
+  const uint32_t s = (uint32_t)__ffsll((long long)~win) - 1u;  // ones from `start` (64: a longer run)
+  const uint32_t j0 = start - LINE_HDR_BITS - cq;
+  if (s == 0) return R;  // q's cell is empty: absent
+  const uint32_t last = j0 + s - 1;
+  if (~win == 0ULL || (has_sent && last >= C) || s - 1 > LINE_MAX_RUN_CODES) {
+    // a run too long for one code window, or one holding the next line's
+    // first key: search its keys (to the line's end for a long run)
+    R.kind = LS_SEARCH;
+    R.a = j0;
+    R.b = (~win == 0ULL) ? (has_sent ? C : 0xFFFFFFFFu) : last;
+    R.right_ok = has_sent && (~win == 0ULL || last >= C);
+    return R;
+  }
+  if (s == 1) {
+    R.kind = LS_FOUND;
+    R.a = j0;
+    return R;
+  }
+  const uint32_t m0 = cbase + __popcll(csel & ((1ULL << os) - 1));
+  const uint32_t cpos = LINE_HDR_BITS + ulen + LINE_CODE_BITS * m0;
+  const uint32_t cw = cpos >> 6, co = cpos & 63u;
+  uint64_t c0 = 0, c1 = 0;
+#pragma unroll
+  for (int w = 1; w < NW; w++) {
+    c0 = ((uint32_t)w == cw) ? W[w] : c0;
+    c1 = ((uint32_t)w == cw + 1) ? W[w] : c1;
+  }
+  const uint64_t codes = co ? ((c0 >> co) | (c1 << (64 - co))) : c0;
+  // a code of 7 anywhere in the run: search it
+  uint64_t sev = codes & (codes >> 1) & (codes >> 2) & 0x1249249249249249ULL;  // bit 3x set when code x is 7
+  sev &= (1ULL << (LINE_CODE_BITS * (s - 1))) - 1;
+  if (sev) {
+    R.kind = LS_SEARCH;
+    R.a = j0;
+    R.b = last;
+    return R;
+  }
+  // blind descent: the boundary of least code (highest differing bit) splits
+  uint32_t lo = 0, hb = s - 1;
+  while (lo < hb) {
+    uint32_t bc = 8, bp = lo + 1;
+    for (uint32_t x = lo + 1; x <= hb; x++) {
+      const uint32_t c = (uint32_t)(codes >> (LINE_CODE_BITS * (x - 1))) & 7u;
+      if (c < bc) {
+        bc = c;
+        bp = x;
+      }
+    }
+    if ((rel >> (t - 1 - bc)) & 1) lo = bp; else hb = bp - 1;
+  }
+  R.kind = LS_FOUND;
+  R.a = j0 + lo;
+  return R;
+}
+
+template <int NW>
+__device__ __forceinline__ void load_line(const uint64_t* __restrict__ lines, uint32_t i, uint64_t (&W)[NW]) {
+  const uint4* lp = (const uint4*)(lines + (uint64_t)i * NW);
+#pragma unroll
+  for (int u = 0; u < NW / 2; u++) {
+    const uint4 v = lp[u];
+    W[2 * u] = (uint64_t)v.x | ((uint64_t)v.y << 32);
+    W[2 * u + 1] = (uint64_t)v.z | ((uint64_t)v.w << 32);
+  }
+}
+
+// The line a key's predicted rank falls in (leaves without a BBHash mark;
+// no branch, so that two keys' loads can be in flight together). ok = false
+// for a key wider than 2k bits or an empty index.
+__device__ __forceinline__ uint32_t lidx_pred_line(const LidxView& V, const uint32_t* __restrict__ warp, uint64_t key, bool& ok,
+                                                   LeafRec& Lf) {
+  ok = !((V.key_bits < 64 && (key >> V.key_bits) != 0) || V.n_lines == 0);
+  const uint64_t u = warp_key(key, warp, V.key_bits, V.warp_groups);
+  const uint32_t r = V.s >= 64 ? 0u : (uint32_t)(u >> V.s);
   const RootPiece P = V.root[r];
-  const uint32_t j = root_leaf(P, key, V.s, r);
-  const LeafRec R = V.leaf[j];
-  if (R.n == 0) return V.D;
-  if (R.flags & 1) {
+  const uint32_t leaf0n = V.root[r + 1].leaf0;
+  const float x = (float)(V.s >= 64 ? u : u - ((uint64_t)r << V.s));
+  Lf = V.leaf[root_leaf(P, leaf0n, x)];
+  int64_t pred = leaf_pred(P, Lf, x);
+  pred = pred < 0 ? 0 : (pred >= (int64_t)V.D ? (int64_t)V.D - 1 : pred);
+  const uint32_t pu = (uint32_t)pred, th = __umulhi(pu, V.magic);
+  const uint32_t i = (th + ((pu - th) >> 1)) >> V.mshift;
+  return i < V.n_lines ? i : (V.n_lines ? V.n_lines - 1 : 0);
+}
+
+// lidx_pred_line for NK keys in stages (warp + root records, then leaves,
+// then lines), each stage's loads issued for every key before any is used.
+template <int NK>
+__device__ __forceinline__ void lidx_pred_lines(const LidxView& V, const uint32_t* __restrict__ warp, const RootPiece* __restrict__ root,
+                                                const uint64_t (&key)[NK], bool (&ok)[NK], uint32_t (&line)[NK]) {
+  uint32_t r[NK], leaf0[NK], leaf0n[NK], p[NK];
+  float A[NK], B[NK], x[NK];
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    ok[k] = !((V.key_bits < 64 && (key[k] >> V.key_bits) != 0) || V.n_lines == 0);
+    const uint64_t u = warp_key(key[k], warp, V.key_bits, V.warp_groups);
+    r[k] = V.s >= 64 ? 0u : (uint32_t)(u >> V.s);
+    x[k] = (float)(V.s >= 64 ? u : u - ((uint64_t)r[k] << V.s));
+  }
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    const RootPiece P = root[r[k]];
+    A[k] = P.A;
+    B[k] = P.B;
+    leaf0[k] = P.leaf0;
+    p[k] = P.p;
+    leaf0n[k] = root[r[k] + 1].leaf0;
+  }
+  LeafRec Lf[NK];
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    RootPiece P;
+    P.A = A[k];
+    P.B = B[k];
+    P.leaf0 = leaf0[k];
+    P.p = p[k];
+    Lf[k] = V.leaf[root_leaf(P, leaf0n[k], x[k])];
+  }
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    RootPiece P;
+    P.p = p[k];
+    int64_t pred = leaf_pred(P, Lf[k], x[k]);
+    pred = pred < 0 ? 0 : (pred >= (int64_t)V.D ? (int64_t)V.D - 1 : pred);
+    const uint32_t pu = (uint32_t)pred, th = __umulhi(pu, V.magic);
+    const uint32_t i = (th + ((pu - th) >> 1)) >> V.mshift;
+    line[k] = i < V.n_lines ? i : (V.n_lines ? V.n_lines - 1 : 0);
+  }
+}
+
+// Waits for a line's words where it stands, so that the compiler does not
+// move the loads of the words the walk does not read past the walk (which
+// would cost a second round trip to L2 per lookup).
+template <int NW>
+__device__ __forceinline__ void hold_line(const uint64_t (&W)[NW]) {
+#pragma unroll
+  for (int w = 0; w < NW; w++) asm volatile("" ::"v"(W[w]));
+}
+
+// From line i (words W already loaded) to the key's rank, or D when absent.
+// The lines' headers alone decide where the key lies (before the first key:
+// left; at or past the cell count: right), so the walk touches no code and
+// the run is decoded once; a search that finds the key above every key it
+// covers (a run holding the next line's first key, or an overflow line) sends
+// the walk on to the next line. Every step visits one line, and the walk from
+// the predicted line to the key's own takes at most max_steps of them.
+template <int NW>
+__device__ __forceinline__ uint64_t lidx_finish(const LidxView& V, uint64_t key, uint32_t i, uint64_t (&W)[NW]) {
+  uint32_t steps = 1;
+  while (true) {
+    bool here = false;
+    while (true) {
+      const uint64_t k0 = W[0] & ((1ULL << 62) - 1);
+      bool move = false;
+      if (key < k0) {
+        if (i == 0) return V.D;
+        i--;
+        move = true;
+      } else if (!((W[0] >> 62) & 1)) {
+        const uint32_t t = (uint32_t)W[1] & 63u, ulen = (uint32_t)(W[1] >> 6) & 1023u;
+        const uint64_t base = (uint64_t)i * V.C;
+        const uint32_t m = i + 1 < V.n_lines ? V.C + 1 : (uint32_t)(V.D - base);
+        if (((key - k0) >> t) >= (uint64_t)(ulen - m)) {  // past the last cell
+          if (i + 1 >= V.n_lines) return V.D;
+          i++;
+          move = true;
+        }
+      }
+      if (!move) {
+        here = true;
+        break;
+      }
+      if (steps >= V.max_steps) break;
+      load_line<NW>(V.lines, i, W);
+      hold_line<NW>(W);
+      steps++;
+    }
+    if (!here) return V.D;
+    const bool has_sent = i + 1 < V.n_lines;
+    const LineResult res = line_decode<NW>(W, key, has_sent, V.C);
+    const uint64_t base = (uint64_t)i * V.C;
+    if (res.kind == LS_FOUND) return base + res.a;
+    if (res.kind == LS_ABSENT) return V.D;
+    // search ranks [base + a, base + b] of the sorted keys: up to 4 keys
+    // in one round of loads, a binary search beyond
+    const uint64_t a = base + res.a;
+    const uint64_t b = res.b == 0xFFFFFFFFu ? V.D - 1 : base + res.b;
+    uint64_t lo = a, hi = b + 1;
+    if (b - a < 4) {
+      uint64_t kk[4];
+#pragma unroll
+      for (int x = 0; x < 4; x++) kk[x] = a + x <= b ? V.keys[a + x] : ~0ULL;
+      uint32_t below = 0;
+      uint64_t hit = ~0ULL;
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        below += (a + x <= b && kk[x] < key) ? 1u : 0u;
+        hit = (a + x <= b && kk[x] == key) ? a + x : hit;
+      }
+      lo = a + below;
+      if (hit != ~0ULL) return hit;
+    } else {
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (V.keys[mid] < key) lo = mid + 1; else hi = mid;
+      }
+      if (lo <= b && V.keys[lo] == key) return lo;
+    }
+    if (!(res.right_ok && lo == b + 1) || steps >= V.max_steps) return V.D;
+    i++;
+    load_line<NW>(V.lines, i, W);
+    hold_line<NW>(W);
+    steps++;
+  }
+}
+
+// Rank of key q, or D when absent, one key at a time (FB: leaves may resolve
+// through the BBHash).
+template <int NW, bool FB>
+__device__ __forceinline__ uint64_t lidx_rank_t(const LidxView& V, const MphfView& mv, const uint32_t* __restrict__ warp,
+                                                uint64_t key) {
+  bool ok;
+  LeafRec Lf;
+  const uint32_t i = lidx_pred_line(V, warp, key, ok, Lf);
+  if (!ok) return V.D;
+  if (FB && leaf_fallback(Lf)) {
     const uint64_t v = mphf_lookup<uint64_t>(mv, key);
     const uint64_t rk = v < V.D ? V.bbh2rank[v] : V.D;
     return (rk < V.D && V.keys[rk] == key) ? rk : V.D;
   }
-  const int64_t qa = (int64_t)R.q, qb = qa + (int64_t)R.n;
-  const int64_t pr = leaf_pred(R, key);
-  int64_t lo = pr - (int64_t)R.elo, hi = pr + (int64_t)R.ehi;
-  if (lo < qa) lo = qa;
-  if (hi > qb - 1) hi = qb - 1;
-  if (lo > hi) return V.D;
-  const uint32_t f = fp_of(R, key);
-  int64_t c = (pr < lo ? lo : (pr > hi ? hi : pr)) & ~7LL;
-  bool resolve = false;
-  int64_t found = -1;
-  for (int it = 0; it < 16384; it++) {
-    const uint4 v = *(const uint4*)(V.fp + c);
-    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-    uint32_t n_eq = 0, lt = 0, gt = 0;
-    int64_t first_eq = -1;
-#pragma unroll
-    for (int u = 7; u >= 0; u--) {
-      const int64_t pos = c + u;
-      const uint32_t e = (wd[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-      const bool in = pos >= lo && pos <= hi;
-      const bool eq = in && e == f;
-      n_eq += eq ? 1u : 0u;
-      first_eq = eq ? pos : first_eq;
-      lt |= (in && e < f) ? 1u : 0u;
-      gt |= (in && e > f) ? 1u : 0u;
-    }
-    if (n_eq == 1) {
-      // a neighbour just outside the chunk could share the fingerprint
-      bool tie = false;
-      if (first_eq == c && c > lo) tie = V.fp[c - 1] == f;
-      if (first_eq == c + 7 && c + 7 < hi) tie |= V.fp[c + 8] == f;
-      if (tie) resolve = true;
-      else found = first_eq;
-      break;
-    }
-    if (n_eq > 1) {
-      resolve = true;
-      break;
-    }
-    if (!gt && c + 8 <= hi) {  // every fingerprint here is smaller: go right
-      c += 8;
-      continue;
-    }
-    if (!lt && c > lo) {  // every fingerprint here is larger: go left
-      c -= 8;
-      continue;
-    }
-    break;  // between two fingerprints: absent
-  }
-  if (resolve) {  // exact search over the window's full keys
-    int64_t a = lo, b = hi + 1;
-    while (a < b) {
-      const int64_t mid = (a + b) >> 1;
-      if (V.keys[mid] < key) a = mid + 1; else b = mid;
-    }
-    return (a <= hi && V.keys[a] == key) ? (uint64_t)a : V.D;
-  }
-  return found >= 0 ? (uint64_t)found : V.D;
+  uint64_t W[NW];
+  load_line<NW>(V.lines, i, W);
+  hold_line<NW>(W);
+  return lidx_finish<NW>(V, key, i, W);
+}
+__device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key) {
+  return V.line_words == 8 ? lidx_rank_t<8, true>(V, mv, V.warp, key) : lidx_rank_t<16, true>(V, mv, V.warp, key);
 }
 
+// The warp tables (and, with ROOT_LDS, the root records) go to LDS once per
+// work-group. Without BBHash leaves a lane carries two keys (i and i +
+// stride) through the front of the lookup together: both predictions, then
+// both first lines in flight at once; the next two keys are loaded while the
+// current ones are resolved.
+constexpr uint32_t LIDX_LDS_ROOT_MAX = 1024;  // root pieces held in LDS (16 KiB)
+template <int NW, bool FB, bool ROOT_LDS>
 __global__ void __launch_bounds__(256) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
                                                      uint64_t* __restrict__ out) {
+  __shared__ uint32_t wt[WARP_MAX_GROUPS * 256];
+  __shared__ RootPiece rt[ROOT_LDS ? LIDX_LDS_ROOT_MAX + 1 : 1];
+  for (int j = threadIdx.x; j < V.warp_groups * 256; j += 256) wt[j] = V.warp[j];
+  if (ROOT_LDS)
+    for (uint32_t j = threadIdx.x; j <= V.R; j += 256) rt[j] = V.root[j];
+  __syncthreads();
+  const RootPiece* root = ROOT_LDS ? rt : V.root;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t r = lidx_rank(V, mv, keys[i]);
-    out[i] = r < V.D ? r : 0;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (FB) {
+    for (; i < n; i += stride) {
+      const uint64_t r = lidx_rank_t<NW, true>(V, mv, wt, keys[i]);
+      out[i] = r < V.D ? r : 0;
+    }
+    return;
+  }
+  uint64_t next[2] = {0, 0};
+  if (i < n) {
+    next[0] = keys[i];
+    next[1] = keys[i + stride < n ? i + stride : i];
+  }
+  for (; i < n; i += 2 * stride) {
+    const uint64_t j = i + stride;
+    const bool has_b = j < n;
+    const uint64_t key[2] = {next[0], next[1]};
+    const uint64_t i2 = i + 2 * stride;
+    if (i2 < n) {
+      next[0] = keys[i2];
+      next[1] = keys[i2 + stride < n ? i2 + stride : i2];
+    }
+    bool ok[2];
+    uint32_t line[2];
+    lidx_pred_lines<2>(V, wt, root, key, ok, line);
+    uint64_t WA[NW], WB[NW];
+    load_line<NW>(V.lines, line[0], WA);
+    load_line<NW>(V.lines, line[1], WB);
+    hold_line<NW>(WA);
+    hold_line<NW>(WB);
+    const uint64_t ra = ok[0] ? lidx_finish<NW>(V, key[0], line[0], WA) : V.D;
+    out[i] = ra < V.D ? ra : 0;
+    if (has_b) {
+      const uint64_t rb = ok[1] ? lidx_finish<NW>(V, key[1], line[1], WB) : V.D;
+      out[j] = rb < V.D ? rb : 0;
+    }
   }
 }
 
@@ -626,35 +1038,66 @@ void launch_lidx_update(hipStream_t st, uint64_t n_segs, float lr, const LidxTra
   CF1G_LAUNCH("lidx_update", k_lidx_update, dim3(blocks_for(n_segs, 256)), dim3(256), 0, st, n_segs, lr, b.gw, b.gc, b.w, b.c);
 }
 void launch_lidx_pieces(hipStream_t st, const uint64_t* keys, uint64_t D, int s, uint32_t R, uint32_t L, uint64_t* p, uint32_t* lo) {
-  CF1G_LAUNCH("lidx_pieces", k_lidx_pieces, dim3(blocks_for((uint64_t)R + 1, 256)), dim3(256), 0, st, keys, D, s, R, L, p);
-  CF1G_LAUNCH("lidx_root_ranges", k_lidx_root_ranges, dim3(blocks_for(R, 256)), dim3(256), 0, st, p, R, D, L, lo);
+  CF1G_LAUNCH("lidx_pieces", k_lidx_pieces, dim3(blocks_for((uint64_t)R + 1, 256)), dim3(256), 0, st, keys, D, s, R, p);
+  CF1G_LAUNCH("lidx_root_ranges", k_lidx_root_ranges, dim3(blocks_for((uint64_t)R + 1, 256)), dim3(256), 0, st, p, R, D, L, lo);
 }
-void launch_lidx_root_final(hipStream_t st, uint32_t R, uint64_t D, uint32_t L, const uint64_t* p, const uint32_t* lo,
-                            const LidxTrainBufs& b, RootPiece* out) {
-  CF1G_LAUNCH("lidx_root_final", k_lidx_root_final, dim3(blocks_for(R, 256)), dim3(256), 0, st, R, D, L, p, lo, b.mu, b.isd, b.w, b.c,
-              out);
+void launch_lidx_root_final(hipStream_t st, uint32_t R, uint64_t D, const uint64_t* p, const uint32_t* lo, const LidxTrainBufs& b,
+                            RootPiece* out) {
+  CF1G_LAUNCH("lidx_root_final", k_lidx_root_final, dim3(blocks_for((uint64_t)R + 1, 256)), dim3(256), 0, st, R, D, p, lo, b.mu, b.isd,
+              b.w, b.c, out);
 }
-void launch_lidx_leaf_bounds(hipStream_t st, const uint64_t* keys, uint64_t D, int s, uint32_t L, const RootPiece* root, uint64_t* q) {
-  CF1G_LAUNCH("lidx_leaf_bounds", k_lidx_leaf_bounds, dim3(blocks_for(D + 1, 256)), dim3(256), 0, st, keys, D, s, L, root, q);
+void launch_lidx_leaf_bounds(hipStream_t st, const uint64_t* keys, uint64_t D, int s, uint32_t LT, const RootPiece* root, uint64_t* q) {
+  CF1G_LAUNCH("lidx_leaf_bounds", k_lidx_leaf_bounds, dim3(blocks_for(D + 1, 256)), dim3(256), 0, st, keys, D, s, LT, root, q);
 }
-void launch_lidx_leaf_final(hipStream_t st, const uint64_t* keys, uint64_t D, uint32_t L, const uint64_t* q, const LidxTrainBufs& b,
-                            uint32_t max_window, LeafRec* out, uint64_t* sabs) {
-  CF1G_LAUNCH("lidx_leaf_final", k_lidx_leaf_final, dim3(blocks_for(L, 4)), dim3(256), 0, st, keys, D, L, q, b.mu, b.isd, b.w, b.c,
-              max_window, out, sabs);
+void launch_lidx_leaf_final(hipStream_t st, const uint64_t* keys, uint64_t D, uint32_t LT, int s, const uint64_t* q,
+                            const RootPiece* root, const LidxTrainBufs& b, uint32_t max_window, LeafRec* out, uint32_t* err,
+                            uint64_t* sabs) {
+  CF1G_LAUNCH("lidx_leaf_final", k_lidx_leaf_final, dim3(blocks_for(LT, 4)), dim3(256), 0, st, keys, D, LT, s, q, root, b.mu, b.isd, b.w,
+              b.c, max_window, out, err, sabs);
 }
-void launch_lidx_fingerprints(hipStream_t st, const uint64_t* keys, uint32_t L, const uint64_t* q, const LeafRec* leaf, uint16_t* fp,
-                              uint32_t* ties) {
-  CF1G_LAUNCH("lidx_fingerprints", k_lidx_fingerprints, dim3(blocks_for(L, 4)), dim3(256), 0, st, keys, L, q, leaf, fp, ties);
+void launch_lidx_lines(hipStream_t st, int line_words, const uint64_t* keys, uint64_t D, uint32_t C, uint64_t n_lines, uint64_t* lines,
+                       uint32_t* stats) {
+  if (!n_lines) return;
+  if (line_words == 8)
+    CF1G_LAUNCH("lidx_lines", k_lidx_lines<8>, dim3(blocks_for(n_lines, 256)), dim3(256), 0, st, keys, D, C, n_lines, lines, stats);
+  else
+    CF1G_LAUNCH("lidx_lines", k_lidx_lines<16>, dim3(blocks_for(n_lines, 256)), dim3(256), 0, st, keys, D, C, n_lines, lines, stats);
 }
-void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t L, const uint64_t* q, const LeafRec* leaf, const MphfView& mv,
+void launch_lidx_warp_hist(hipStream_t st, const uint64_t* keys, uint64_t D, int K2, int G, uint32_t* cnt) {
+  if (!D || !G) return;
+  uint64_t g = (D + 255) / 256;
+  if (g > 4096) g = 4096;
+  CF1G_LAUNCH("lidx_warp_hist", k_lidx_warp_hist, dim3((unsigned)g), dim3(256), 0, st, keys, D, K2, G, cnt);
+}
+void launch_lidx_warp_keys(hipStream_t st, const uint64_t* keys, uint64_t D, const uint32_t* cum, int K2, int G, uint64_t* u) {
+  if (!D) return;
+  CF1G_LAUNCH("lidx_warp_keys", k_lidx_warp_keys, dim3(blocks_for(D, 256)), dim3(256), 0, st, keys, D, cum, K2, G, u);
+}
+void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t LT, const uint64_t* q, const LeafRec* leaf, const MphfView& mv,
                           uint32_t* bbh2rank) {
-  CF1G_LAUNCH("lidx_bbh2rank", k_lidx_bbh2rank, dim3(blocks_for(L, 4)), dim3(256), 0, st, keys, L, q, leaf, mv, bbh2rank);
+  CF1G_LAUNCH("lidx_bbh2rank", k_lidx_bbh2rank, dim3(blocks_for(LT, 4)), dim3(256), 0, st, keys, LT, q, leaf, mv, bbh2rank);
 }
 void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const LidxView& V, const MphfView& mv, uint64_t* out) {
   if (!n) return;
-  uint64_t g = (n + 255) / 256;
-  if (g > 65536) g = 65536;
-  CF1G_LAUNCH("lidx_lookup", k_lidx_lookup, dim3((unsigned)g), dim3(256), 0, st, keys, n, V, mv, out);
+  uint64_t g = (n + 511) / 512;  // two keys per lane a round
+  if (g > 8192) g = 8192;         // each work-group fills its LDS tables once
+  const bool rl = V.R <= LIDX_LDS_ROOT_MAX;
+#define CF1G_LIDX_LOOKUP(NW_, FB_, RL_) \
+  CF1G_LAUNCH("lidx_lookup", (k_lidx_lookup<NW_, FB_, RL_>), dim3((unsigned)g), dim3(256), 0, st, keys, n, V, mv, out)
+  if (V.line_words == 8) {
+    if (V.fallback) {
+      if (rl) CF1G_LIDX_LOOKUP(8, true, true); else CF1G_LIDX_LOOKUP(8, true, false);
+    } else {
+      if (rl) CF1G_LIDX_LOOKUP(8, false, true); else CF1G_LIDX_LOOKUP(8, false, false);
+    }
+  } else {
+    if (V.fallback) {
+      if (rl) CF1G_LIDX_LOOKUP(16, true, true); else CF1G_LIDX_LOOKUP(16, true, false);
+    } else {
+      if (rl) CF1G_LIDX_LOOKUP(16, false, true); else CF1G_LIDX_LOOKUP(16, false, false);
+    }
+  }
+#undef CF1G_LIDX_LOOKUP
 }
 void launch_count_mismatch(hipStream_t st, const uint64_t* a, const uint64_t* b, uint64_t n, unsigned long long* bad) {
   if (!n) return;

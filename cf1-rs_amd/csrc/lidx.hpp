@@ -5,40 +5,76 @@
 
 namespace cf1g {
 
+// ------------------------------------------------------------------ key warp
+// Before the linear models a key is mapped to u in [0, 2^64) by a learned
+// order-0 model of its bytes (four bases each): interval (lo, w) starts at
+// (0, 2^64 - 1); group g (bits [K2 - 8(g+1), K2 - 8g), up to 4 groups) with
+// symbol s moves lo by f(w, cum_g[s]) and sets w = f(w, cum_g[s+1]) -
+// f(w, cum_g[s]), f(w, c) = floor(w c / 2^32), cum_g[256] = 2^32; the bits
+// below the groups add mulhi(w, rest << (64 - rest_bits)). The intervals
+// nest, so u is non-decreasing in the key, and with cum_g from the keys' own
+// byte counts (+1 each) u is close to uniform: the leaves then see the
+// keys' random spacing rather than the genome's base composition.
+constexpr int WARP_MAX_GROUPS = 4;
+
 // ------------------------------------------------------------------ model records
+// Root piece r (warped keys with u >> s == r): leaf = leaf0 + clamp(floor(fma(A,
+// x, B)), 0, n_r - 1), x = key - (r << s), n_r = leaf0 of piece r + 1 minus
+// leaf0 (at least 1; no leaf holds keys of two pieces); p = rank of the
+// piece's first key. R + 1 records, the last one {0, 0, L + R, D}.
 struct RootPiece {
   float A, B;
-  uint32_t lo, hi;
+  uint32_t leaf0;
+  uint32_t p;
 };
+// Leaf: predicted rank = p_r + floor(fma(|S|, x, I) + 0.5) in the piece's
+// coordinates (x = u - (r << s)). The sign bit of S marks a leaf whose recorded error window
+// exceeds the threshold: its keys resolve through the BBHash.
 struct LeafRec {
-  uint64_t base;  // first key of the leaf
-  uint32_t q;     // rank of the first key
-  uint32_t n;     // keys in the leaf (ranks [q, q + n))
-  float S, I;     // pred = q + floor(fma(S, (float)(key - base), I) + 0.5)
-  uint16_t elo, ehi;
-  uint8_t shift;  // fingerprint = (key - base) >> shift (< 2^16 inside the leaf)
-  uint8_t flags;  // bit 0: window above threshold, resolve through the BBHash
-  uint16_t pad;
+  float S, I;
 };
-static_assert(sizeof(LeafRec) == 32, "leaf record is 32 bytes");
+static_assert(sizeof(LeafRec) == 8, "leaf record is 8 bytes");
+
+// Lines: line i holds ranks [iC, iC + n_i) and the sentinel (rank iC + C)
+// when there is one, as a little-endian bit stream of u64 words:
+//   bits 0..61 k0 (key of rank iC), bit 62 overflow (keys searched instead)
+//   bits 64..69 t (cell shift), bits 70..79 ulen
+//   unary [80, 80 + ulen): per cell c = 0 .. (k_last - k0) >> t, a 1 per key
+//   with (key - k0) >> t == c, then a 0
+//   3-bit codes after the unary, one per key that shares the cell of the key
+//   before it: min(t - bitlen(rel_j ^ rel_{j-1}), 7); the highest bit where the
+//   two differ is t - 1 - code, and 7 means "search the run's keys".
+constexpr int LINE_HDR_BITS = 80;
+constexpr int LINE_CODE_BITS = 3;
+constexpr uint32_t LINE_CODE_SEARCH = 7;
+constexpr uint32_t LINE_MAX_RUN_CODES = 21;  // codes of one run read as one 64-bit window
 
 struct LidxView {
-  const RootPiece* root;
-  const LeafRec* leaf;
-  const uint64_t* keys;      // sorted
-  const uint16_t* fp;        // fingerprint of each sorted key (monotone inside a leaf)
+  const RootPiece* root;     // R + 1
+  const LeafRec* leaf;       // LT = L + R
+  const uint64_t* keys;      // sorted (D)
+  const uint64_t* lines;     // n_lines x line_words
   const uint32_t* bbh2rank;  // rank of the key whose BBHash value is v (fallback leaves only)
+  const uint32_t* warp;      // warp_groups x 256 cumulative byte frequencies (x 2^-32)
   uint64_t D;
-  uint32_t L, R;
-  int s;  // root shift: piece = key >> s (< R for keys of 2k bits)
+  uint32_t R, LT;
+  uint32_t C, n_lines;
+  uint32_t magic, mshift;  // line = (t + ((pred - t) >> 1)) >> mshift, t = umulhi(pred, magic)
+  uint32_t max_steps;      // lines one lookup may visit
+  int s;                   // root shift: piece = u >> s
+  int line_words;          // 8 (64-byte lines) or 16 (128-byte lines)
+  int key_bits;            // 2k: wider keys are absent
+  int warp_groups;
+  bool fallback;           // some leaf resolves through the BBHash
 };
 
-// Segment views. LEVEL 0 = root pieces (x = key - (r << s), y = rank*L/D -
-// lo_r); LEVEL 1 = leaves (x = key - base_j, y = rank - q_j).
+// Segment views over the warped keys. LEVEL 0 = root pieces (x = u - (r << s),
+// y = rank*L/D - lo_r); LEVEL 1 = leaves (x = u - first u of the leaf, y =
+// rank - q_j).
 struct SegCtx {
-  const uint64_t* keys;
+  const uint64_t* keys;  // warped, non-decreasing
   const uint64_t* seg;  // segment starts (n_segs + 1)
-  const uint32_t* lo;   // root: lo_r
+  const uint32_t* lo;   // root: lo_r = floor(p_r L / D)
   uint64_t D;
   uint32_t L;
   int s;

@@ -300,46 +300,54 @@ int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, 
  * the globally sorted distinct k-mers — a monotone minimal perfect map).
  * u64 keys only (k <= 31). Trained on the GPU after P3. */
 typedef struct cf1g_lidx_params {
-  uint32_t leaf_bits;  /* log2 leaves; 0 = 20 (capped at floor(log2 D)) */
-  uint32_t root_bits;  /* log2 root pieces; 0 = 12 (capped at 2k) */
+  uint32_t leaf_bits;  /* log2 L; 0 = 18 (capped at floor(log2 D)); L + R leaves in all */
+  uint32_t root_bits;  /* log2 root pieces R; 0 = 10 (at most 20; up to 10 the lookup holds them in LDS) */
   uint32_t epochs;     /* gradient-descent epochs per level; 0 = 4 */
   float lr;            /* step size on standardised inputs; 0 = 0.5 */
-  uint32_t max_window; /* leaves with a wider search window resolve through the BBHash; 0 = 256 */
+  uint32_t max_window; /* leaves with a wider error window resolve through the BBHash; 0 = 4096 */
   uint32_t untrained;  /* 1: keep the endpoint interpolation (the method without training) */
-  uint32_t reserved[2];
+  uint32_t line_bytes; /* 64 or 128; 0 = 128 */
+  uint32_t line_keys;  /* keys per line C; 0 = 256 (128-byte lines) or 112 (64-byte lines) */
 } cf1g_lidx_params;
 
 typedef struct cf1g_lidx_info {
   uint64_t n_keys, n_leaves, n_pieces;
   uint64_t fallback_leaves, fallback_keys; /* leaves (and their keys) resolved through the BBHash */
-  uint64_t tie_keys;                       /* keys whose 16-bit fingerprint ties a neighbour's */
+  uint64_t n_lines, overflow_lines;        /* lines; lines too full for their codes (searched) */
+  uint64_t search_keys;                    /* keys in runs the codes leave to a search of the keys */
+  uint32_t line_bytes, line_keys;
   uint32_t max_err_lo, max_err_hi;         /* recorded error bounds (non-fallback leaves) */
-  double mean_window;                      /* mean search window over keys of non-fallback leaves */
   double mean_abs_err;                     /* mean |rank - prediction| over all keys */
+  double line_bits_per_key;                /* bits the lines' codes use, per key (overflow lines left out) */
+  uint64_t bytes_total;                    /* root + leaf records + lines */
   double root_loss[2], leaf_loss[2];       /* MSE before the first and after the last epoch */
-  double sort_ms, train_ms;                /* sort of the keys; whole training (sort included) */
+  double sort_ms, train_ms;                /* sort of the keys; whole build (sort included) */
 } cf1g_lidx_info;
 
 /* Sort the distinct keys, train root and leaves (forward / backward / update
- * kernels), record error bounds, build the search arrays. Needs P3. */
+ * kernels), record error bounds, build the lines. Needs P3. */
 int cf1g_lidx_train(cf1g_ctx* ctx, const cf1g_lidx_params* params, cf1g_lidx_info* out);
 /* Exact rank of each present key; an absent key gets 0 or some rank, as with
  * any minimal perfect map (src/mphf.rs:268-270). Host buffers /
  * device-resident buffers. */
 int cf1g_lidx_lookup(cf1g_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
 int cf1g_lidx_lookup_device(cf1g_ctx* ctx, const void* d_keys, uint64_t n, uint64_t* d_out);
-/* The trained model: its shape, the sorted keys (D), root pieces (16 bytes
- * each: f32 A, f32 B, u32 lo, u32 hi; piece = key >> root_shift) and leaf
- * records (32 bytes each: u64 base, u32 first rank, u32 key count, f32 slope,
- * f32 intercept, u16 err_lo, u16 err_hi, u8 shift, u8 flags, u16). Any output
- * may be NULL. */
+/* The trained index: its shape, the sorted keys (D), the key warp's tables
+ * (warp_groups x 256 u32 cumulative byte frequencies x 2^-32), root pieces
+ * (R + 1 records of 16 bytes: f32 A, f32 B, u32 first leaf, u32 first rank;
+ * piece = warped key >> root_shift), leaf records (n_leaves x 8 bytes: f32 slope whose sign
+ * bit marks a BBHash leaf, f32 intercept), the leaves' error bounds
+ * (n_leaves x {u32 below, u32 above}) and the lines (n_lines x line_bytes).
+ * Any output may be NULL. */
 typedef struct cf1g_lidx_shape {
   uint64_t n_keys;
   uint32_t n_pieces, n_leaves;
   int32_t root_shift;
-  uint32_t reserved;
+  uint32_t line_bytes, line_keys, n_lines;
+  uint32_t key_bits, warp_groups;
 } cf1g_lidx_shape;
-int cf1g_lidx_export(cf1g_ctx* ctx, cf1g_lidx_shape* shape, uint64_t* sorted_keys, void* root, void* leaves);
+int cf1g_lidx_export(cf1g_ctx* ctx, cf1g_lidx_shape* shape, uint64_t* sorted_keys, uint32_t* warp, void* root, void* leaves,
+                     uint32_t* leaf_err, void* lines);
 /* Gradient check: with the caller's parameters (w, c in standardised
  * coordinates, one pair per segment; level 0 = root pieces, 1 = leaves) run
  * the forward and backward kernels and return per-segment gradients and

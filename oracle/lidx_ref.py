@@ -13,11 +13,18 @@ What this file restates, for checking the GPU kernels:
     residual f - y (FP32) and the MSE gradient (2/n) sum r (z, 1) accumulated
     in double — the forward / backward kernels' arithmetic, key by key;
   * least_squares: the closed-form optimum the training descends towards;
+  * warp_tables / warp: the learned byte-frequency CDF the models read keys
+    through (fixed-point arithmetic coding, exact integer restatement);
   * route / leaf_pred / error_bounds: the lookup arithmetic (root routing,
-    leaf prediction) and the recorded per-leaf error bounds.
+    leaf prediction) and the recorded per-leaf error bounds;
+  * encode_lines / decode_line / line_lookup: the line format and the walk
+    that turns a predicted rank into the exact one.
 Pinning: fma32 against exact rational arithmetic, the gradient against
 finite differences of the loss and against the closed-form optimum (zero
-gradient there), least_squares against numpy.linalg.lstsq (tests/test_lidx_oracle.py).
+gradient there), least_squares against numpy.linalg.lstsq, the warp against
+a Fraction evaluation of its interval recurrence and its monotonicity, the
+lines by a hand-built line and by finding every key from wrong starting lines
+(tests/test_lidx_oracle.py).
 """
 import numpy as np
 
@@ -43,14 +50,77 @@ def fma32(a, b, c):
     return np.where(is_mid, np.where(e > 0, up, dn), r).astype(np.float32)
 
 
-# ---------------------------------------------------------------- segments
-def root_pieces(keys, k, root_bits):
-    """Piece starts p[0..R] (first rank with key >= r << s) and the leaf
-    ranges lo_r / hi_r; also returns s, R, L."""
+# ---------------------------------------------------------------- key warp
+M32 = np.uint64(0xFFFFFFFF)
+WARP_MAX_GROUPS = 4
+
+
+def warp_groups(key_bits):
+    return min(WARP_MAX_GROUPS, key_bits // 8)
+
+
+def warp_tables(keys, key_bits):
+    """cum[g, s] = floor(2^32 sum_{t < s} (count_g(t) + 1) / (D + 256)) over
+    the byte symbols of group g (bits [K2 - 8(g+1), K2 - 8g))."""
+    G = warp_groups(key_bits)
     D = len(keys)
-    rb = min(root_bits, 2 * k)
+    cum = np.zeros((WARP_MAX_GROUPS, 256), np.uint64)
+    for g in range(G):
+        sym = ((keys >> np.uint64(key_bits - 8 * (g + 1))) & np.uint64(255)).astype(np.int64)
+        cnt = np.bincount(sym, minlength=256).astype(object) + 1
+        run = 0
+        for x in range(256):
+            cum[g, x] = (run << 32) // (D + 256)
+            run += int(cnt[x])
+    return cum[:G].astype(np.uint32)
+
+
+def _f(w, c):
+    """floor(w c / 2^32), w u64, c u32 (numpy uint64 arithmetic, exact)."""
+    c = c.astype(np.uint64)
+    return (w >> np.uint64(32)) * c + (((w & M32) * c) >> np.uint64(32))
+
+
+def _mulhi64(a, b):
+    a_lo, a_hi = a & M32, a >> np.uint64(32)
+    b_lo, b_hi = b & M32, b >> np.uint64(32)
+    ll = a_lo * b_lo
+    hl = a_hi * b_lo
+    lh = a_lo * b_hi
+    hh = a_hi * b_hi
+    cross = (ll >> np.uint64(32)) + (hl & M32) + lh
+    return hh + (hl >> np.uint64(32)) + (cross >> np.uint64(32))
+
+
+def warp(keys, cum, key_bits):
+    """u for every key: nested intervals (lo, w) from (0, 2^64 - 1), one
+    byte group at a time, then the remaining bits scaled into the last
+    interval."""
+    keys = np.asarray(keys, np.uint64)
+    G = len(cum)
+    lo = np.zeros(len(keys), np.uint64)
+    w = np.full(len(keys), np.uint64(0xFFFFFFFFFFFFFFFF))
+    for g in range(G):
+        sym = ((keys >> np.uint64(key_bits - 8 * (g + 1))) & np.uint64(255)).astype(np.int64)
+        a = _f(w, cum[g][sym])
+        b = np.where(sym == 255, w, _f(w, cum[g][np.minimum(sym + 1, 255)]))
+        lo = lo + a
+        w = b - a
+    rb = key_bits - 8 * G
+    if rb > 0:
+        rest = (keys & np.uint64((1 << rb) - 1)) << np.uint64(64 - rb)
+        lo = lo + _mulhi64(w, rest)
+    return lo
+
+
+# ---------------------------------------------------------------- segments
+def root_pieces(keys, root_bits):
+    """Piece starts p[0..R] (first rank with warped key >= r << s, s = 64 -
+    root_bits) over the warped keys; also returns s, R."""
+    D = len(keys)
+    rb = max(1, min(root_bits, 20))
     R = 1 << rb
-    s = 2 * k - rb
+    s = 64 - rb
     targets = np.arange(R + 1, dtype=np.uint64) << np.uint64(s)
     p = np.searchsorted(keys, targets[:R], side="left").astype(np.uint64)
     p = np.concatenate([p, np.array([D], np.uint64)])
@@ -65,13 +135,14 @@ def leaf_count(D, leaf_bits):
 
 
 def root_ranges(p, L, D):
+    """lo_r = floor(p_r L / D) (the root's training offset, r < R) and the
+    first leaf of every piece, leaf0_r = lo_r + r (r <= R): piece r owns
+    leaves [leaf0_r, leaf0_{r+1}), at least one, and no leaf holds keys of two
+    pieces. L + R leaves in all."""
     R = len(p) - 1
-    lo = np.minimum((p[:R] * np.uint64(L)) // np.uint64(max(D, 1)), np.uint64(L - 1)).astype(np.uint64)
-    h = (p[1:] * np.uint64(L) + np.uint64(D - 1)) // np.uint64(max(D, 1)) if D else np.zeros(R, np.uint64)
-    hi = np.where(h > 0, h - np.uint64(1), np.uint64(0))
-    hi = np.maximum(hi, lo)
-    hi = np.minimum(hi, np.uint64(L - 1))
-    return lo, hi
+    lo = (p.astype(np.uint64) * np.uint64(L)) // np.uint64(max(D, 1)) if D else np.zeros(R + 1, np.uint64)
+    leaf0 = lo + np.arange(R + 1, dtype=np.uint64)
+    return lo[:R], leaf0
 
 
 def seg_of_keys(seg):
@@ -159,37 +230,261 @@ def least_squares(x, y, sg, n_segs, mu, isd):
 
 
 # ---------------------------------------------------------------- lookup arithmetic
+ROOT_DTYPE = np.dtype([("A", "<f4"), ("B", "<f4"), ("leaf0", "<u4"), ("p", "<u4")])
+LEAF_DTYPE = np.dtype([("S", "<f4"), ("I", "<f4")])
+
+
 def route(keys, root, s):
-    """Leaf of every key: lo_r + clamp(floor(fma(A_r, x, B_r)), 0, hi_r - lo_r)."""
+    """Leaf of every key: leaf0_r + clamp(floor(fma(A_r, x, B_r)), 0, n_r - 1)
+    with x = key - (r << s) and n_r = leaf0_{r+1} - leaf0_r."""
     r = (keys >> np.uint64(s)).astype(np.int64)
     P = root[r]
+    n = root["leaf0"][r + 1].astype(np.int64) - P["leaf0"].astype(np.int64)
     x = (keys - (r.astype(np.uint64) << np.uint64(s))).astype(np.float32)
-    f = np.floor(fma32(P["A"], x, P["B"])).astype(np.float32)
-    span = (P["hi"] - P["lo"]).astype(np.float32)
-    cl = np.where(f < 0, np.float32(0), np.where(f > span, span, f))
-    return P["lo"].astype(np.int64) + cl.astype(np.int64)
+    f = np.floor(fma32(P["A"], x, P["B"])).astype(np.float64)
+    cl = np.clip(f, 0, np.maximum(n - 1, 0)).astype(np.int64)
+    return P["leaf0"].astype(np.int64) + cl
 
 
-def leaf_pred(keys, leaf, j):
-    """q_j + floor(fma(S_j, (float)(key - base_j), I_j) + 0.5), clamped."""
-    R = leaf[j]
-    x = (keys - R["base"]).astype(np.int64).astype(np.float32)
-    p = np.floor((fma32(R["S"], x, R["I"]) + np.float32(0.5)).astype(np.float32))
-    p = np.clip(p, np.float32(-1.0e12), np.float32(1.0e12))
-    return R["q"].astype(np.int64) + p.astype(np.int64)
+def is_fallback(leaf):
+    """Leaves whose slope carries the sign bit resolve through the BBHash."""
+    return (leaf["S"].view(np.uint32) >> np.uint32(31)).astype(bool)
 
 
-def error_bounds(keys, leaf):
-    """(elo, ehi, sum |e|) per leaf over the keys in their leaves (leaf j
-    holds ranks [q_j, q_{j+1}))."""
+def leaf_pred(keys, root, leaf, s, j):
+    """p_r + floor(fma(|S_j|, (float)(key - (r << s)), I_j) + 0.5), clamped
+    to +-1e12: the rank the leaf predicts (piece coordinates)."""
+    r = (keys >> np.uint64(s)).astype(np.int64)
+    x = (keys - (r.astype(np.uint64) << np.uint64(s))).astype(np.float32)
+    S = np.abs(leaf["S"][j]).astype(np.float32)
+    f = (fma32(S, x, leaf["I"][j]) + np.float32(0.5)).astype(np.float32)
+    f = np.clip(np.floor(f), np.float32(-1.0e12), np.float32(1.0e12))
+    return root["p"][r].astype(np.int64) + f.astype(np.int64)
+
+
+def leaf_starts(keys, root, s, n_leaves):
+    """q[j] = first rank routed to a leaf >= j, j in [0, n_leaves]."""
+    j = route(keys, root, s) if len(keys) else np.zeros(0, np.int64)
+    return np.searchsorted(j, np.arange(n_leaves + 1), side="left").astype(np.int64)
+
+
+def error_bounds(keys, root, leaf, s):
+    """(elo, ehi, sum |e|) per leaf over the keys routed to it, with the
+    lookup's own arithmetic: rank - pred in [-elo, ehi]."""
     D = len(keys)
     L = len(leaf)
-    q = np.concatenate([leaf["q"].astype(np.int64), [D]])
-    j = np.repeat(np.arange(L), np.diff(q))
-    e = np.arange(D, dtype=np.int64) - leaf_pred(keys, leaf, j)
+    j = route(keys, root, s) if D else np.zeros(0, np.int64)
+    e = np.arange(D, dtype=np.int64) - leaf_pred(keys, root, leaf, s, j)
     elo = np.zeros(L, np.int64)
     ehi = np.zeros(L, np.int64)
     np.maximum.at(elo, j, -e)
     np.maximum.at(ehi, j, e)
     sabs = np.bincount(j, weights=np.abs(e).astype(np.float64), minlength=L)
     return elo, ehi, sabs
+
+
+# ---------------------------------------------------------------- lines
+# A line holds C consecutive keys by rank (line i: ranks [iC, iC + n_i)) plus
+# the next line's first key (the sentinel, rank iC + C) when there is one, in
+# line_bytes bytes read as a little-endian bit stream of u64 words:
+#   bits  0..61  k0, the key of rank iC;  bit 62: overflow (no payload)
+#   bits 64..69  t, the cell shift;        bits 70..79: ulen
+#   unary [80, 80 + ulen): for cells c = 0 .. (k_last - k0) >> t, one 1 per
+#         key with (key - k0) >> t == c, then a 0
+#   codes after the unary: for every key j >= 1 in the cell of key j-1, in
+#         order, 3 bits min(t - bitlen((k_j - k0) ^ (k_{j-1} - k0)), 7): the
+#         highest bit where the two differ is t - 1 - code
+# t minimises m + cells + 3 collisions over [tb - 2, tb + 3] (first minimum),
+# tb = max(0, bitlen(span) - bitlen(m)); a line whose minimum exceeds
+# 8 line_bytes - 80 bits is an overflow line.
+HDR_BITS = 80
+CODE_BITS = 3
+CODE_SEARCH = 7
+
+
+def default_line_keys(line_bytes):
+    return {64: 104, 128: 256}[line_bytes]
+
+
+def n_lines(D, C):
+    return (D + C - 1) // C
+
+
+def _line_keys(keys, i, C):
+    D = len(keys)
+    a = i * C
+    n = min(C, D - a)
+    m = n + (1 if a + n < D else 0)
+    return [int(x) for x in keys[a:a + m]], n, m
+
+
+def line_choice(ks):
+    """(t, bits, collisions) chosen for the keys of one line."""
+    m = len(ks)
+    k0 = ks[0]
+    rel = [x - k0 for x in ks]
+    span = rel[-1]
+    bl = [(rel[j] ^ rel[j - 1]).bit_length() for j in range(1, m)]
+    tb = max(0, span.bit_length() - m.bit_length())
+    best = None
+    for t in range(max(0, tb - 2), min(61, tb + 3) + 1):
+        coll = sum(1 for b in bl if b <= t)
+        bits = m + (span >> t) + 1 + CODE_BITS * coll
+        if best is None or bits < best[1]:
+            best = (t, bits, coll)
+    return best
+
+
+def encode_line(keys, i, C, line_bytes):
+    """The line_bytes bytes of line i and (bits used, overflow, keys of runs
+    that resolve by search)."""
+    ks, n, m = _line_keys(keys, i, C)
+    nw = line_bytes // 8
+    w = [0] * nw
+
+    def put(pos, val, width):
+        for b in range(width):
+            if (val >> b) & 1:
+                w[(pos + b) >> 6] |= 1 << ((pos + b) & 63)
+
+    k0 = ks[0]
+    put(0, k0, 62)
+    t, bits, coll = line_choice(ks)
+    if bits > 8 * line_bytes - HDR_BITS:
+        put(62, 1, 1)
+        return np.array(w, np.uint64).view(np.uint8), (0, True, n)
+    rel = [x - k0 for x in ks]
+    cells = (rel[-1] >> t) + 1
+    ulen = m + cells
+    put(64, t, 6)
+    put(70, ulen, 10)
+    pos, prev = HDR_BITS, 0
+    for j in range(m):
+        c = rel[j] >> t
+        pos += c - prev
+        prev = c
+        put(pos, 1, 1)
+        pos += 1
+    pos += 1
+    assert pos == HDR_BITS + ulen
+    # codes, and the keys whose run holds a search code
+    run_search, run_len, run_bad, n_search = False, 1, False, 0
+    for j in range(1, m + 1):
+        same = j < m and (rel[j] >> t) == (rel[j - 1] >> t)
+        if same:
+            code = min(t - (rel[j] ^ rel[j - 1]).bit_length(), CODE_SEARCH)
+            put(pos, code, CODE_BITS)
+            pos += CODE_BITS
+            run_len += 1
+            run_bad |= code == CODE_SEARCH
+        else:
+            if run_len > 1 and (run_bad or run_len - 1 > 21):
+                n_search += run_len
+            run_len, run_bad = 1, False
+    return np.array(w, np.uint64).view(np.uint8), (bits, False, n_search)
+
+
+def encode_lines(keys, C, line_bytes):
+    """All lines (n_lines x line_bytes bytes) and per-line statistics."""
+    nl = n_lines(len(keys), C)
+    out = np.zeros((nl, line_bytes), np.uint8)
+    st = []
+    for i in range(nl):
+        out[i], s = encode_line(keys, i, C, line_bytes)
+        st.append(s)
+    return out, st
+
+
+def decode_line(line, q, i, C, n_total_lines):
+    """One step of the walk for query q at line i: ("found", rank),
+    ("left",), ("right",), ("absent",) or ("search", a, b, right_ok): binary
+    search ranks [a, b] of the sorted keys; right_ok: a q above them all goes
+    right."""
+    w = [int(x) for x in np.asarray(line).view(np.uint64)]
+
+    def bit(b):
+        return (w[b >> 6] >> (b & 63)) & 1
+
+    def get(pos, width):
+        return sum(bit(pos + b) << b for b in range(width))
+
+    has_sent = i + 1 < n_total_lines
+    k0 = w[0] & ((1 << 62) - 1)
+    if (w[0] >> 62) & 1:
+        if q < k0:
+            return ("left",)
+        return ("search", i * C, i * C + C if has_sent else None, has_sent)
+    if q < k0:
+        return ("left",)
+    t = get(64, 6)
+    ulen = get(70, 10)
+    rel = q - k0
+    cq = rel >> t
+    zeros = [b for b in range(HDR_BITS, HDR_BITS + ulen) if not bit(b)]
+    if cq >= len(zeros):
+        return ("right",)
+    e_b = zeros[cq]
+    e_a = zeros[cq - 1] if cq > 0 else HDR_BITS - 1
+    s = e_b - e_a - 1
+    j0 = (e_a + 1 - HDR_BITS) - cq
+    if s == 0:
+        return ("absent",)
+    last = j0 + s - 1
+    if has_sent and last == C:
+        if s == 1:
+            return ("right",)
+        return ("search", i * C + j0, i * C + last, True)
+    if s == 1:
+        return ("found", i * C + j0)
+    if s - 1 > 21:
+        return ("search", i * C + j0, i * C + last, False)
+    ne = sum(1 for b in range(HDR_BITS + 1, e_a + 1) if not bit(b) and bit(b - 1))
+    m0 = j0 - ne
+    cpos = HDR_BITS + ulen + CODE_BITS * m0
+    codes = [get(cpos + CODE_BITS * x, CODE_BITS) for x in range(s - 1)]
+    if CODE_SEARCH in codes:
+        return ("search", i * C + j0, i * C + last, False)
+    lo, hi = 0, s - 1
+    while lo < hi:
+        b = min(range(lo + 1, hi + 1), key=lambda x: codes[x - 1])
+        d = t - 1 - codes[b - 1]
+        if (rel >> d) & 1:
+            lo = b
+        else:
+            hi = b - 1
+    return ("found", i * C + j0 + lo)
+
+
+def line_lookup(keys, lines, C, q, start, max_steps=64):
+    """Rank of q by the walk from line `start` (None when absent)."""
+    nl = len(lines)
+    if nl == 0:
+        return None
+    i = min(max(start, 0), nl - 1)
+    for _ in range(max_steps):
+        r = decode_line(lines[i], q, i, C, nl)
+        if r[0] == "found":
+            return r[1]
+        if r[0] == "absent":
+            return None
+        if r[0] == "left":
+            if i == 0:
+                return None
+            i -= 1
+            continue
+        if r[0] == "right":
+            if i == nl - 1:
+                return None
+            i += 1
+            continue
+        a, b, right_ok = r[1], r[2], r[3]
+        if b is None:
+            b = len(keys) - 1
+        x = int(np.searchsorted(keys[a:b + 1], np.uint64(q)))
+        if x <= b - a and int(keys[a + x]) == q:
+            return a + x
+        if right_ok and x == b - a + 1 and i < nl - 1:
+            i += 1
+            continue
+        return None
+    return None

@@ -1,11 +1,13 @@
 """GPU learned rank index, checked against the CPU oracle (oracle/lidx_ref.py).
 
-Exact ranks for every key (the sorted order is the map), the routing and the
-recorded error bounds recomputed with the device arithmetic, forward /
-backward gradients against double-precision recomputation, convergence to
-the closed-form optimum, the gain of training over endpoint interpolation,
-the BBHash fallback, and classification / P5 output through the learned
-index equal (canonicalised) to the BBHash path.
+Exact ranks for every key (the sorted order is the map), the key warp's
+tables and warped keys, the root records, routing and recorded error bounds
+recomputed with the device arithmetic, the
+lines byte for byte (both line sizes, overflowing lines, clustered keys whose
+runs are searched), forward / backward gradients against double-precision
+recomputation, convergence to the closed-form optimum, the gain of training
+over endpoint interpolation, the BBHash fallback, and classification / P5
+output through the learned index equal (canonicalised) to the BBHash path.
 """
 import os
 import tempfile
@@ -45,7 +47,8 @@ def test_sorted_keys_and_exact_ranks(built):
     m = g.lidx_export()
     keys = sorted_keys(g)
     np.testing.assert_array_equal(m["keys"], keys)
-    assert info["n_keys"] == D and info["n_leaves"] == 1 << LB and info["n_pieces"] == 1 << RB
+    assert info["n_keys"] == D and info["n_leaves"] == (1 << LB) + (1 << RB) and info["n_pieces"] == 1 << RB
+    assert info["n_lines"] == -(-D // info["line_keys"]) and info["line_bytes"] == 128 and info["line_keys"] == 256
     rng = np.random.default_rng(0)
     perm = rng.permutation(D)
     out = g.lidx_lookup(keys[perm])
@@ -59,35 +62,106 @@ def test_absent_and_repeated_queries(built):
     rng = np.random.default_rng(3)
     absent = rng.integers(0, 1 << 62, 20000, dtype=np.uint64)
     absent = absent[~np.isin(absent, keys)]
-    out = g.lidx_lookup(absent)
-    assert np.all(out < D)  # 0 or some rank, never out of range
+    near = keys[rng.integers(0, D, 20000)] + np.uint64(1)  # next to present keys: same cells
+    near = near[~np.isin(near, keys)]
+    for q in (absent, near):
+        out = g.lidx_lookup(q)
+        assert np.all(out < D)  # 0 or some rank, never out of range
     wide = np.array([np.uint64(1) << np.uint64(63), ~np.uint64(0)], np.uint64)  # wider than a 62-bit key
     np.testing.assert_array_equal(g.lidx_lookup(wide), [0, 0])
     rep = np.repeat(keys[:: max(1, D // 50)], 7)
     np.testing.assert_array_equal(g.lidx_lookup(rep), np.searchsorted(keys, rep).astype(np.uint64))
 
 
-def test_model_matches_oracle(built):
+@pytest.mark.parametrize("line_bytes,line_keys", [(128, 0), (64, 0), (128, 64), (64, 400)])
+def test_model_and_lines_match_oracle(built, line_bytes, line_keys):
+    """Root records, routing, leaf error bounds and every line byte equal the
+    oracle's (lines from the sorted keys); 64 keys in 128 bytes leaves slack,
+    400 keys in 64 bytes overflow every full line (searched)."""
     g, _, D = built
-    g.lidx_train(leaf_bits=LB, root_bits=RB)
+    info = g.lidx_train(leaf_bits=LB, root_bits=RB, line_bytes=line_bytes, line_keys=line_keys)
     m = g.lidx_export()
     keys, root, leaf, s = m["keys"], m["root"], m["leaf"], m["shift"]
-    # root ranges
-    p, s_o, R = LR.root_pieces(keys, K, RB)
-    assert s == s_o and len(root) == R
-    lo, hi = LR.root_ranges(p, 1 << LB, D)
-    np.testing.assert_array_equal(root["lo"], lo)
-    np.testing.assert_array_equal(root["hi"], hi)
+    C = m["line_keys"]
+    cum = LR.warp_tables(keys, 2 * K)
+    np.testing.assert_array_equal(m["warp"], cum)
+    u = LR.warp(keys, cum, 2 * K)
+    p, s_o, R = LR.root_pieces(u, RB)
+    assert s == s_o and len(root) == R + 1
+    lo, leaf0 = LR.root_ranges(p, 1 << LB, D)
+    np.testing.assert_array_equal(root["leaf0"], leaf0)
+    np.testing.assert_array_equal(root["p"], p)
     assert np.all(root["A"] >= 0)
-    # routing: leaf boundaries are where the oracle routes the sorted keys
-    j = LR.route(keys, root, s)
+    j = LR.route(u, root, s)
     assert np.all(np.diff(j) >= 0)
-    q = np.searchsorted(j, np.arange(1 << LB), side="left")
-    np.testing.assert_array_equal(leaf["q"].astype(np.int64), q)
-    # recorded error bounds are the exact extremes of the lookup arithmetic
-    elo, ehi, _ = LR.error_bounds(keys, leaf)
-    np.testing.assert_array_equal(leaf["elo"].astype(np.int64), np.minimum(elo, 65535))
-    np.testing.assert_array_equal(leaf["ehi"].astype(np.int64), np.minimum(ehi, 65535))
+    elo, ehi, _ = LR.error_bounds(u, root, leaf, s)
+    np.testing.assert_array_equal(m["err"][:, 0].astype(np.int64), elo)
+    np.testing.assert_array_equal(m["err"][:, 1].astype(np.int64), ehi)
+    assert info["max_err_lo"] == elo.max() and info["max_err_hi"] == ehi.max()
+    want, st = LR.encode_lines(keys, C, m["line_bytes"])
+    np.testing.assert_array_equal(m["lines"], want)
+    assert info["overflow_lines"] == sum(1 for _, o, _ in st if o)
+    assert info["search_keys"] == sum(n for _, _, n in st)
+    if line_keys == 400:
+        assert info["overflow_lines"] >= len(want) - 1
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(D)
+    np.testing.assert_array_equal(g.lidx_lookup(keys[perm]), perm.astype(np.uint64))
+
+
+def _clustered_genome(rng):
+    """Copies of one motif with substitutions near their ends: k-mers that
+    share all but their last bases, in clusters the leaves cannot follow."""
+    motif = rng.integers(0, 4, 40)
+    seqs = ["".join("ACGT"[c] for c in rng.integers(0, 4, 30000)).encode()]
+    for i in range(400):
+        cp = motif.copy()
+        cp[rng.integers(28, 40)] = rng.integers(0, 4)
+        cp[rng.integers(30, 40)] = rng.integers(0, 4)
+        seqs.append("".join("ACGT"[c] for c in cp).encode())
+    return seqs
+
+
+def test_walk_across_overflow_lines():
+    """Lines of 150 keys do not fit 64 bytes (searched), and one epoch at a
+    step size of 10^4 spoils the leaves (errors of many lines): the walk
+    crosses runs of overflow lines, searching each one, until it reaches the
+    key's own."""
+    rng = np.random.default_rng(32)
+    g = cf1g.Cf1g(k=K, num_bins=8)
+    try:
+        g.load_records(_clustered_genome(rng))
+        D = g.run_p1_p4()
+        info = g.lidx_train(leaf_bits=2, root_bits=1, line_bytes=64, line_keys=150, epochs=1, lr=1.0e4,
+                            max_window=1 << 31)
+        assert info["fallback_leaves"] == 0
+        assert info["overflow_lines"] >= info["n_lines"] // 2
+        assert max(info["max_err_lo"], info["max_err_hi"]) > 3 * 150
+        keys = g.lidx_export()["keys"]
+        perm = rng.permutation(D)
+        np.testing.assert_array_equal(g.lidx_lookup(keys[perm]), perm.astype(np.uint64))
+    finally:
+        g.close()
+
+
+def test_clustered_keys_search_runs():
+    """Clustered k-mers: adjacent keys differ only in low bits, so runs carry
+    codes of 7 and resolve by a search of the keys."""
+    rng = np.random.default_rng(31)
+    g = cf1g.Cf1g(k=K, num_bins=8)
+    try:
+        g.load_records(_clustered_genome(rng))
+        D = g.run_p1_p4()
+        info = g.lidx_train(leaf_bits=8, root_bits=4)
+        m = g.lidx_export()
+        keys = m["keys"]
+        want, st = LR.encode_lines(keys, m["line_keys"], m["line_bytes"])
+        np.testing.assert_array_equal(m["lines"], want)
+        assert info["search_keys"] > 0
+        perm = rng.permutation(D)
+        np.testing.assert_array_equal(g.lidx_lookup(keys[perm]), perm.astype(np.uint64))
+    finally:
+        g.close()
 
 
 @pytest.mark.parametrize("level", [0, 1])
@@ -95,19 +169,20 @@ def test_gradients_match_oracle(built, level):
     g, _, D = built
     g.lidx_train(leaf_bits=LB, root_bits=RB)
     m = g.lidx_export()
-    keys = m["keys"]
-    ns = (1 << RB) if level == 0 else (1 << LB)
+    keys = LR.warp(m["keys"], m["warp"], 2 * K)
+    ns = (1 << RB) if level == 0 else (1 << LB) + (1 << RB)
     rng = np.random.default_rng(7 + level)
     w = rng.standard_normal(ns).astype(np.float32) * np.float32(30)
     c = rng.standard_normal(ns).astype(np.float32) * np.float32(30)
     r = g.lidx_gradients(level, w, c)
     seg = r["seg"]
     if level == 0:
-        p, s, _ = LR.root_pieces(keys, K, RB)
+        p, s, _ = LR.root_pieces(keys, RB)
         np.testing.assert_array_equal(seg, p)
         lo, _ = LR.root_ranges(p, 1 << LB, D)
         x, y, sg = LR.level_xy(0, keys, seg, s=s, lo=lo, L=1 << LB)
     else:
+        np.testing.assert_array_equal(seg, LR.leaf_starts(keys, m["root"], m["shift"], ns))
         x, y, sg = LR.level_xy(1, keys, seg)
     mu, isd = LR.stats(x, sg, ns)
     np.testing.assert_allclose(r["mu"], mu.astype(np.float32), rtol=1e-6, atol=1e-3)
@@ -122,10 +197,10 @@ def test_training_reaches_least_squares(built):
     g, _, D = built
     info = g.lidx_train(leaf_bits=LB, root_bits=RB, epochs=4)
     m = g.lidx_export()
-    keys = m["keys"]
-    seg = np.concatenate([m["leaf"]["q"].astype(np.uint64), [np.uint64(D)]])
+    keys = LR.warp(m["keys"], m["warp"], 2 * K)
+    ns = (1 << LB) + (1 << RB)
+    seg = LR.leaf_starts(keys, m["root"], m["shift"], ns).astype(np.uint64)
     x, y, sg = LR.level_xy(1, keys, seg)
-    ns = 1 << LB
     mu, isd = LR.stats(x, sg, ns)
     w, c = LR.least_squares(x, y, sg, ns, mu.astype(np.float32), isd.astype(np.float32))
     best = LR.loss_double(x, y, sg, ns, mu.astype(np.float32), isd.astype(np.float32), w, c).sum() / D
@@ -142,13 +217,15 @@ def test_training_beats_endpoint_interpolation(built):
     np.testing.assert_array_equal(g.lidx_lookup(keys), np.arange(D, dtype=np.uint64))
     assert tr["leaf_loss"][1] < 0.8 * un["leaf_loss"][1]
     assert tr["mean_abs_err"] < un["mean_abs_err"]
-    assert tr["mean_window"] < un["mean_window"]
+    assert max(tr["max_err_lo"], tr["max_err_hi"]) <= max(un["max_err_lo"], un["max_err_hi"])
 
 
 def test_fallback_leaves_stay_exact(built):
     g, _, D = built
     info = g.lidx_train(leaf_bits=LB, root_bits=RB, max_window=3)
     assert info["fallback_leaves"] > 0
+    m = g.lidx_export()
+    assert np.any(LR.is_fallback(m["leaf"]))
     keys = sorted_keys(g)
     np.testing.assert_array_equal(g.lidx_lookup(keys), np.arange(D, dtype=np.uint64))
 

@@ -101,43 +101,224 @@ def test_least_squares_and_zero_gradient():
     assert np.max(np.abs(g)) < 1e-8
 
 
+def _model(rng, u, rb, lb):
+    """Root and leaf records over warped keys u with random non-negative
+    slopes (routing and bounds must hold for any such model)."""
+    D = len(u)
+    p, s, R = LR.root_pieces(u, rb)
+    L = LR.leaf_count(D, lb)
+    _, leaf0 = LR.root_ranges(p, L, D)
+    root = np.zeros(R + 1, LR.ROOT_DTYPE)
+    root["A"][:R] = np.abs(rng.standard_normal(R)).astype(np.float32) * np.float32(2.0 ** -s * L / R * 1.5)
+    root["B"][:R] = rng.standard_normal(R).astype(np.float32)
+    root["leaf0"] = leaf0
+    root["p"] = p
+    LT = L + R
+    leaf = np.zeros(LT, LR.LEAF_DTYPE)
+    leaf["S"] = np.float32(D / 2.0 ** 64) * rng.uniform(0.8, 1.2, LT).astype(np.float32)
+    leaf["I"] = rng.uniform(-20, 20, LT).astype(np.float32)
+    return root, leaf, s, LT
+
+
+def _warped(rng, n, k=15):
+    keys = np.sort(rng.choice(np.uint64(1) << np.uint64(2 * k), n, replace=False).astype(np.uint64))
+    cum = LR.warp_tables(keys, 2 * k)
+    return keys, LR.warp(keys, cum, 2 * k)
+
+
+def test_route_is_monotone_and_pieces_own_their_leaves():
+    rng = np.random.default_rng(5)
+    keys, u = _warped(rng, 4000)
+    root, leaf, s, LT = _model(rng, u, 6, 8)
+    j = LR.route(u, root, s)
+    assert np.all(np.diff(j) >= 0)
+    assert j.min() >= 0 and j.max() < LT
+    r = (u >> np.uint64(s)).astype(np.int64)
+    assert np.all(j >= root["leaf0"][r]) and np.all(j < root["leaf0"][r + 1])
+    # a leaf never holds keys of two pieces
+    for t in np.unique(j):
+        assert len(np.unique(r[j == t])) == 1
+    # every piece owns at least one leaf, L + R in all
+    assert np.all(np.diff(root["leaf0"].astype(np.int64)) >= 1) and root["leaf0"][-1] == LT
+
+
 def test_error_bounds_definition():
     rng = np.random.default_rng(4)
-    keys = np.sort(rng.choice(np.uint64(1) << np.uint64(30), 500, replace=False).astype(np.uint64))
-    L = 8
-    q = np.linspace(0, 500, L + 1).astype(np.int64)
-    leaf = np.zeros(L, [("base", "<u8"), ("q", "<u4"), ("n", "<u4"), ("S", "<f4"), ("I", "<f4"), ("elo", "<u2"),
-                        ("ehi", "<u2"), ("shift", "u1"), ("flags", "u1"), ("pad", "<u2")])
-    for j in range(L):
-        a, b = q[j], q[j + 1]
-        leaf[j]["base"] = keys[a]
-        leaf[j]["q"] = a
-        leaf[j]["n"] = b - a
-        span = float(keys[b - 1] - keys[a]) or 1.0
-        leaf[j]["S"] = np.float32((b - a - 1) / span * rng.uniform(0.8, 1.2))
-        leaf[j]["I"] = np.float32(rng.uniform(-3, 3))
-    elo, ehi, _ = LR.error_bounds(keys, leaf)
-    j = np.repeat(np.arange(L), np.diff(q))
-    pred = LR.leaf_pred(keys, leaf, j)
-    rank = np.arange(500)
+    keys, u = _warped(rng, 3000)
+    root, leaf, s, LT = _model(rng, u, 5, 6)
+    elo, ehi, _ = LR.error_bounds(u, root, leaf, s)
+    j = LR.route(u, root, s)
+    pred = LR.leaf_pred(u, root, leaf, s, j)
+    rank = np.arange(len(u))
     assert np.all(rank >= pred - elo[j]) and np.all(rank <= pred + ehi[j])
-    for t in range(L):  # bounds are attained (tight) or zero
+    for t in range(LT):  # bounds are attained (tight) or zero
         sel = j == t
+        if not sel.any():
+            assert elo[t] == 0 and ehi[t] == 0
+            continue
         e = rank[sel] - pred[sel]
         assert elo[t] == max(0, int((-e).max())) and ehi[t] == max(0, int(e.max()))
+    # the sign bit of S only marks the leaf: the prediction uses |S|
+    leaf2 = leaf.copy()
+    leaf2["S"] = -leaf2["S"]
+    assert np.all(LR.is_fallback(leaf2)) and not np.any(LR.is_fallback(leaf))
+    np.testing.assert_array_equal(LR.leaf_pred(u, root, leaf2, s, j), pred)
 
 
-def test_route_is_monotone_for_nonnegative_slopes():
-    rng = np.random.default_rng(5)
-    k = 15
-    keys = np.sort(rng.choice(np.uint64(1) << np.uint64(2 * k), 4000, replace=False).astype(np.uint64))
-    p, s, R = LR.root_pieces(keys, k, 6)
-    L = LR.leaf_count(len(keys), 8)
-    lo, hi = LR.root_ranges(p, L, len(keys))
-    root = np.zeros(R, [("A", "<f4"), ("B", "<f4"), ("lo", "<u4"), ("hi", "<u4")])
-    root["A"] = np.abs(rng.standard_normal(R)).astype(np.float32) * np.float32(2.0 ** -s * 8)
-    root["B"] = rng.standard_normal(R).astype(np.float32)
-    root["lo"], root["hi"] = lo, hi
-    j = LR.route(keys, root, s)
-    assert np.all(np.diff(j) >= 0)
-    assert j.min() >= 0 and j.max() < L
+# ---------------------------------------------------------------- warp
+def _warp_exact(key, cum, K2):
+    """The interval recurrence in Python integers."""
+    lo, w = 0, (1 << 64) - 1
+    G = len(cum)
+    for g in range(G):
+        sym = (key >> (K2 - 8 * (g + 1))) & 255
+        a = (w * int(cum[g][sym])) >> 32
+        b = w if sym == 255 else (w * int(cum[g][sym + 1])) >> 32
+        lo += a
+        w = b - a
+    rb = K2 - 8 * G
+    if rb > 0:
+        lo += (w * ((key & ((1 << rb) - 1)) << (64 - rb))) >> 64
+    return lo
+
+
+@pytest.mark.parametrize("k", [31, 21, 15, 5, 3])
+def test_warp_integer_recurrence_and_monotone(k):
+    rng = np.random.default_rng(20 + k)
+    K2 = 2 * k
+    n = min(3000, 1 << K2)
+    keys = np.sort(rng.choice(np.uint64(1) << np.uint64(K2), n, replace=False).astype(np.uint64))
+    # skew the byte counts (GC-poor genome-like keys) through the tables
+    cum = LR.warp_tables(keys[: n // 3], K2)
+    edge = np.array([0, (1 << K2) - 1, (1 << (K2 - 1)), (1 << (K2 - 1)) - 1], np.uint64)
+    allk = np.unique(np.concatenate([keys, edge]))
+    u = LR.warp(allk, cum, K2)
+    for i in range(0, len(allk), 37):
+        assert int(u[i]) == _warp_exact(int(allk[i]), cum, K2)
+    for e in edge:
+        assert int(LR.warp(np.array([e], np.uint64), cum, K2)[0]) == _warp_exact(int(e), cum, K2)
+    assert np.all(np.diff(u.astype(object)) >= 0)  # non-decreasing in the key
+
+
+def test_warp_of_uniform_counts_is_the_scaled_key():
+    """Equal byte counts make every interval split evenly: u is the key
+    scaled to 64 bits, up to the floors (a few units per group)."""
+    K2 = 62
+    i = np.arange(1 << 16, dtype=np.uint64)
+    # every group's byte takes each value 256 times (odd multipliers permute bytes)
+    g0, g1, g2, g3 = i & np.uint64(255), i >> np.uint64(8), (i * np.uint64(37)) & np.uint64(255), (i * np.uint64(101)) & np.uint64(255)
+    rest = (i * np.uint64(2654435761)) & np.uint64((1 << 30) - 1)
+    keys = np.unique((g0 << np.uint64(54)) | (g1 << np.uint64(46)) | (g2 << np.uint64(38)) | (g3 << np.uint64(30)) | rest)
+    assert len(keys) == 1 << 16
+    cum = LR.warp_tables(keys, K2)
+    np.testing.assert_array_equal(cum, np.tile(np.arange(256, dtype=np.uint32) << np.uint32(24), (4, 1)))
+    u = LR.warp(keys, cum, K2).astype(object)
+    want = [int(x) << 2 for x in keys]
+    assert max(abs(a - b) for a, b in zip(u, want)) < (1 << 34)
+
+
+def test_warp_flattens_skewed_keys():
+    """Keys whose bases follow a 0.35/0.15/0.15/0.35 composition: raw keys
+    cluster, warped keys spread evenly (Kolmogorov distance to uniform)."""
+    rng = np.random.default_rng(21)
+    n = 200000
+    bases = rng.choice(4, size=(n, 31), p=[0.35, 0.15, 0.15, 0.35])
+    keys = np.zeros(n, np.uint64)
+    for b in range(31):
+        keys = (keys << np.uint64(2)) | bases[:, b].astype(np.uint64)
+    keys = np.unique(keys)
+    cum = LR.warp_tables(keys, 62)
+    u = np.sort(LR.warp(keys, cum, 62))
+    ecdf = np.arange(len(keys)) / len(keys)
+    d_raw = np.max(np.abs(keys.astype(np.float64) / 2.0 ** 62 - ecdf))
+    d_warp = np.max(np.abs(u.astype(np.float64) / 2.0 ** 64 - ecdf))
+    assert d_raw > 0.05 and d_warp < 0.01
+
+
+# ---------------------------------------------------------------- lines
+def test_line_format_by_hand():
+    """Keys 16, 17, 20, 40 in one line (C = 4): rel 0, 1, 4, 24; t = 4 gives
+    cells [0, 0, 0, 1] (12 bits: 4 + 2 cells + 2 codes of 3), so the unary is
+    1110 10, and the codes are t - bitlen(1 ^ 0) = 3 and t - bitlen(4 ^ 1) = 1."""
+    keys = np.array([16, 17, 20, 40], np.uint64)
+    assert LR.line_choice([16, 17, 20, 40]) == (4, 12, 2)
+    lines, st = LR.encode_lines(keys, 4, 64)
+    w = [int(x) for x in lines[0].view(np.uint64)]
+    assert w[0] == 16  # k0, no overflow
+    assert w[1] & 63 == 4 and (w[1] >> 6) & 1023 == 6  # t, ulen
+    unary = [(w[1] >> (16 + b)) & 1 for b in range(6)]
+    assert unary == [1, 1, 1, 0, 1, 0]
+    codes = (w[1] >> 22) & 0x3F
+    assert codes == 3 | (1 << 3)
+    assert st[0] == (12, False, 0)
+    for r, q in enumerate(keys):
+        assert LR.line_lookup(keys, lines, 4, int(q), 0) == r
+    # absent keys in an occupied cell resolve to some member, never out of range
+    for q in (18, 19, 21, 33, 41, 15):
+        got = LR.line_lookup(keys, lines, 4, q, 0)
+        assert got is None or 0 <= got < 4
+
+
+def _check_all(keys, C, line_bytes, starts=(0,)):
+    lines, st = LR.encode_lines(keys, C, line_bytes)
+    D = len(keys)
+    for r in range(D):
+        true = r // C
+        for d in starts:
+            got = LR.line_lookup(keys, lines, C, int(keys[r]), true + d)
+            assert got == r, (r, d, got)
+    return lines, st
+
+
+@pytest.mark.parametrize("line_bytes,C", [(64, 40), (128, 96), (128, 256)])
+def test_lines_uniform_keys_from_any_start(line_bytes, C):
+    rng = np.random.default_rng(11)
+    keys = np.sort(rng.choice(np.uint64(1) << np.uint64(62), 3 * C + 17, replace=False).astype(np.uint64))
+    lines, st = _check_all(keys, C, line_bytes, starts=(0, -2, -1, 1, 2))
+    assert not any(o for _, o, _ in st)
+
+
+def test_lines_clustered_keys_need_search():
+    """Clusters of keys that share all but their low bits: long runs and
+    codes of 7 (searched); sizes and tiny D."""
+    rng = np.random.default_rng(12)
+    base = rng.choice(np.uint64(1) << np.uint64(62), 60, replace=False).astype(np.uint64)
+    clus = []
+    for b in base:
+        bits = int(rng.integers(5, 12))
+        clus.append(b + rng.choice(np.uint64(1) << np.uint64(bits), int(rng.integers(2, 30)), replace=False).astype(np.uint64))
+    keys = np.unique(np.concatenate([base] + clus))
+    lines, st = _check_all(keys, 64, 128, starts=(0, 1, -1))
+    assert sum(n for _, _, n in st) > 0  # some runs resolve by search
+    for D in (1, 2, 63, 64, 65, 129):
+        _check_all(keys[:D], 64, 128, starts=(0, 1))
+
+
+def test_lines_overflow_search():
+    rng = np.random.default_rng(13)
+    keys = np.sort(rng.choice(np.uint64(1) << np.uint64(62), 700, replace=False).astype(np.uint64))
+    lines, st = _check_all(keys, 200, 64, starts=(0, -3, -2, -1, 1, 2, 3))  # 200 keys never fit 432 bits
+    assert all(o for _, o, _ in st[:-1])  # the last line holds 100 keys
+
+
+def test_lines_sentinel_cell_shared():
+    """The next line's first key shares a cell with this line's last keys:
+    keys of both lines resolve (the run with the sentinel is searched)."""
+    keys = np.array([1000, 1001, 1002, 1003, 1004, 1005, 1006, 5000, 5001], np.uint64)
+    _check_all(keys, 4, 64, starts=(0, 1, -1))
+    keys = np.array([1 << 40, (1 << 40) + 3, (1 << 40) + 5, (1 << 40) + 6, (1 << 40) + 7, (1 << 41)], np.uint64)
+    _check_all(keys, 3, 64, starts=(0, 1, -1))
+
+
+def test_magic_division():
+    """line = pred / C by multiply-high, as the device computes it."""
+    rng = np.random.default_rng(14)
+    for C in (2, 3, 7, 100, 104, 112, 255, 256, 257, 4096):
+        l = (C - 1).bit_length()
+        M = ((((1 << l) - C) << 32) // C + 1) & 0xFFFFFFFF
+        ns = np.concatenate([rng.integers(0, 1 << 32, 20000, dtype=np.uint64), np.arange(0, 3 * C, dtype=np.uint64),
+                             np.array([(1 << 32) - 1, (1 << 32) - 2], np.uint64)])
+        for n in ns:
+            n = int(n)
+            t = (n * M) >> 32
+            assert (t + ((n - t) >> 1)) >> (l - 1) == n // C
