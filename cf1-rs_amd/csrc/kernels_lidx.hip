@@ -492,33 +492,50 @@ __global__ void __launch_bounds__(256) k_lidx_lines(const uint64_t* __restrict__
   const int tb = max(0, bitlen64(span) - bitlen64(m));
   const int t_lo = max(0, tb - 2), t_hi = min(61, tb + 3);
   uint32_t cnt[6] = {0, 0, 0, 0, 0, 0};
+  int min_bl = 64;  // the closest pair: its code is the line's largest
   uint64_t prev = 0;
   for (uint32_t j = 1; j < m; j++) {
     const uint64_t rel = keys[a + j] - k0;
     const int bl = bitlen64(rel ^ prev);
 #pragma unroll
     for (int u = 0; u < 6; u++) cnt[u] += (bl <= t_lo + u) ? 1u : 0u;
+    min_bl = min(min_bl, bl);
     prev = rel;
   }
-  int t = t_lo;
-  uint64_t best = ~0ULL;
+  // the first minimum of the bit count over the candidate shifts, for
+  // 3-bit and for 4-bit codes
+  int t3 = t_lo, t4 = t_lo;
+  uint64_t best3 = ~0ULL, best4 = ~0ULL;
 #pragma unroll
   for (int u = 0; u < 6; u++) {
     if (t_lo + u > t_hi) break;
-    const uint64_t bits = (uint64_t)m + (span >> (t_lo + u)) + 1 + (uint64_t)LINE_CODE_BITS * cnt[u];
-    if (bits < best) {
-      best = bits;
-      t = t_lo + u;
+    const uint64_t cells = (uint64_t)m + (span >> (t_lo + u)) + 1;
+    if (cells + 3ULL * cnt[u] < best3) {
+      best3 = cells + 3ULL * cnt[u];
+      t3 = t_lo + u;
+    }
+    if (cells + 4ULL * cnt[u] < best4) {
+      best4 = cells + 4ULL * cnt[u];
+      t4 = t_lo + u;
     }
   }
+  // 4-bit codes where they fit and 3-bit codes would leave a code to search
+  // (or would not fit)
+  const uint64_t payload = (uint64_t)(NW * 64 - LINE_HDR_BITS);
+  const bool wide = best4 <= payload && (best3 > payload || (min_bl <= t3 && t3 - min_bl >= 7));
+  const int cw = wide ? 4 : 3, t = wide ? t4 : t3;
+  const uint64_t best = wide ? best4 : best3;
+  const int csearch = (1 << cw) - 1;
+  const uint32_t max_run_codes = 64u / (uint32_t)cw;
   b[0] = (uint32_t)k0;
   b[1] = (uint32_t)(k0 >> 32) & 0x3FFFFFFFu;
   uint32_t n_search = 0;
-  if (best > (uint64_t)(NW * 64 - LINE_HDR_BITS)) {
+  if (best > payload) {
     b[1] |= 1u << 30;  // bit 62: overflow
     stats[2 * i] = 0xFFFFFFFFu;
     n_search = n;
   } else {
+    if (wide) b[1] |= 1u << 31;  // bit 63: 4-bit codes
     const uint32_t cells = (uint32_t)(span >> t) + 1;
     const uint32_t ulen = m + cells;
     b[2] |= (uint32_t)t | (ulen << 6);
@@ -532,20 +549,20 @@ __global__ void __launch_bounds__(256) k_lidx_lines(const uint64_t* __restrict__
       b[pos >> 5] |= 1u << (pos & 31);
       pos++;
       if (j > 0 && c == pcell) {
-        const int code = min(t - bitlen64(rel ^ prel), (int)LINE_CODE_SEARCH);
-        put(cpos, (uint32_t)code, LINE_CODE_BITS);
-        cpos += LINE_CODE_BITS;
+        const int code = min(t - bitlen64(rel ^ prel), csearch);
+        put(cpos, (uint32_t)code, cw);
+        cpos += (uint32_t)cw;
         run++;
-        bad |= code == (int)LINE_CODE_SEARCH;
+        bad |= code == csearch;
       } else if (j > 0) {
-        if (run > 1 && (bad || run - 1 > LINE_MAX_RUN_CODES)) n_search += run;
+        if (run > 1 && (bad || run - 1 > max_run_codes)) n_search += run;
         run = 1;
         bad = false;
       }
       pcell = c;
       prel = rel;
     }
-    if (run > 1 && (bad || run - 1 > LINE_MAX_RUN_CODES)) n_search += run;
+    if (run > 1 && (bad || run - 1 > max_run_codes)) n_search += run;
     stats[2 * i] = (uint32_t)best;
   }
   stats[2 * i + 1] = n_search;
@@ -594,6 +611,7 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
     return R;
   }
   const uint32_t t = (uint32_t)W[1] & 63u, ulen = (uint32_t)(W[1] >> 6) & 1023u;
+  const uint32_t cw = 3u + (uint32_t)(W[0] >> 63), cmask = (1u << cw) - 1u;
   const uint64_t rel = q - k0;
   const uint32_t cq = (uint32_t)(rel >> t);
   // Zero #(cq - 1) closes the cell before q's: the run starts right after it
@@ -637,7 +655,7 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
   const uint32_t j0 = start - LINE_HDR_BITS - cq;
   if (s == 0) return R;  // q's cell is empty: absent
   const uint32_t last = j0 + s - 1;
-  if (~win == 0ULL || (has_sent && last >= C) || s - 1 > LINE_MAX_RUN_CODES) {
+  if (~win == 0ULL || (has_sent && last >= C) || s - 1 > 64u / cw) {
     // a run too long for one code window, or one holding the next line's
     // first key: search its keys (to the line's end for a long run)
     R.kind = LS_SEARCH;
@@ -652,18 +670,20 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
     return R;
   }
   const uint32_t m0 = cbase + __popcll(csel & ((1ULL << os) - 1));
-  const uint32_t cpos = LINE_HDR_BITS + ulen + LINE_CODE_BITS * m0;
-  const uint32_t cw = cpos >> 6, co = cpos & 63u;
+  const uint32_t cpos = LINE_HDR_BITS + ulen + cw * m0;
+  const uint32_t cwd = cpos >> 6, co = cpos & 63u;
   uint64_t c0 = 0, c1 = 0;
 #pragma unroll
   for (int w = 1; w < NW; w++) {
-    c0 = ((uint32_t)w == cw) ? W[w] : c0;
-    c1 = ((uint32_t)w == cw + 1) ? W[w] : c1;
+    c0 = ((uint32_t)w == cwd) ? W[w] : c0;
+    c1 = ((uint32_t)w == cwd + 1) ? W[w] : c1;
   }
   const uint64_t codes = co ? ((c0 >> co) | (c1 << (64 - co))) : c0;
-  // a code of 7 anywhere in the run: search it
-  uint64_t sev = codes & (codes >> 1) & (codes >> 2) & 0x1249249249249249ULL;  // bit 3x set when code x is 7
-  sev &= (1ULL << (LINE_CODE_BITS * (s - 1))) - 1;
+  // a search code (all ones) anywhere in the run: search it
+  uint64_t sev = cw == 3u ? (codes & (codes >> 1) & (codes >> 2) & 0x1249249249249249ULL)
+                          : (codes & (codes >> 1) & (codes >> 2) & (codes >> 3) & 0x1111111111111111ULL);
+  const uint32_t used = cw * (s - 1);
+  sev &= used >= 64u ? ~0ULL : ((1ULL << used) - 1);
   if (sev) {
     R.kind = LS_SEARCH;
     R.a = j0;
@@ -673,9 +693,9 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
   // blind descent: the boundary of least code (highest differing bit) splits
   uint32_t lo = 0, hb = s - 1;
   while (lo < hb) {
-    uint32_t bc = 8, bp = lo + 1;
+    uint32_t bc = 16, bp = lo + 1;
     for (uint32_t x = lo + 1; x <= hb; x++) {
-      const uint32_t c = (uint32_t)(codes >> (LINE_CODE_BITS * (x - 1))) & 7u;
+      const uint32_t c = (uint32_t)(codes >> (cw * (x - 1))) & cmask;
       if (c < bc) {
         bc = c;
         bp = x;
@@ -849,6 +869,112 @@ __device__ __forceinline__ uint64_t lidx_finish(const LidxView& V, uint64_t key,
   }
 }
 
+// lidx_finish for two keys in lockstep: the header walk of both, each
+// round's neighbour loads issued together; both runs decoded; both searches'
+// loads issued together. A search that sends a key on to the next line (rare)
+// finishes one key at a time.
+template <int NW>
+__device__ __forceinline__ void lidx_finish2(const LidxView& V, const uint64_t (&key)[2], const bool (&ok)[2], uint32_t (&i)[2],
+                                             uint64_t (&W)[2][NW], uint64_t (&res)[2]) {
+  bool open[2];  // still walking
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    res[k] = V.D;
+    open[k] = ok[k];
+  }
+  uint32_t steps = 1;
+  while (true) {
+    bool moved[2] = {false, false};
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      if (!open[k]) continue;
+      const uint64_t k0 = W[k][0] & ((1ULL << 62) - 1);
+      if (key[k] < k0) {
+        if (i[k] == 0) open[k] = false; else { i[k]--; moved[k] = true; }
+      } else if (!((W[k][0] >> 62) & 1)) {
+        const uint32_t t = (uint32_t)W[k][1] & 63u, ulen = (uint32_t)(W[k][1] >> 6) & 1023u;
+        const uint64_t base = (uint64_t)i[k] * V.C;
+        const uint32_t m = i[k] + 1 < V.n_lines ? V.C + 1 : (uint32_t)(V.D - base);
+        if (((key[k] - k0) >> t) >= (uint64_t)(ulen - m)) {
+          if (i[k] + 1 >= V.n_lines) open[k] = false; else { i[k]++; moved[k] = true; }
+        }
+      }
+    }
+    if (!moved[0] && !moved[1]) break;
+    if (steps >= V.max_steps) {
+#pragma unroll
+      for (int k = 0; k < 2; k++) open[k] = open[k] && !moved[k];
+      break;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+      if (moved[k]) load_line<NW>(V.lines, i[k], W[k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) hold_line<NW>(W[k]);
+    steps++;
+  }
+  LineResult r[2];
+  uint64_t a[2], b[2];
+  bool srch[2], small[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    srch[k] = false;
+    small[k] = false;
+    a[k] = b[k] = 0;
+    if (!open[k]) continue;
+    r[k] = line_decode<NW>(W[k], key[k], i[k] + 1 < V.n_lines, V.C);
+    const uint64_t base = (uint64_t)i[k] * V.C;
+    if (r[k].kind == LS_FOUND) res[k] = base + r[k].a;
+    if (r[k].kind == LS_SEARCH) {
+      srch[k] = true;
+      a[k] = base + r[k].a;
+      b[k] = r[k].b == 0xFFFFFFFFu ? V.D - 1 : base + r[k].b;
+      small[k] = b[k] - a[k] < 4;
+    }
+  }
+  // searches of up to 4 keys: every key's loads in one round
+  uint64_t kk[2][4];
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int x = 0; x < 4; x++) kk[k][x] = (srch[k] && small[k] && a[k] + x <= b[k]) ? V.keys[a[k] + x] : ~0ULL;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (!srch[k]) continue;
+    uint64_t lo = a[k];
+    bool found = false;
+    if (small[k]) {
+      uint32_t below = 0;
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        below += (a[k] + x <= b[k] && kk[k][x] < key[k]) ? 1u : 0u;
+        if (a[k] + x <= b[k] && kk[k][x] == key[k]) {
+          res[k] = a[k] + x;
+          found = true;
+        }
+      }
+      lo = a[k] + below;
+    } else {
+      uint64_t hi = b[k] + 1;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (V.keys[mid] < key[k]) lo = mid + 1; else hi = mid;
+      }
+      if (lo <= b[k] && V.keys[lo] == key[k]) {
+        res[k] = lo;
+        found = true;
+      }
+    }
+    if (!found && r[k].right_ok && lo == b[k] + 1 && steps < V.max_steps && i[k] + 1 < V.n_lines) {
+      // on to the next line, one key at a time (rare)
+      uint64_t Wn[NW];
+      load_line<NW>(V.lines, i[k] + 1, Wn);
+      hold_line<NW>(Wn);
+      res[k] = lidx_finish<NW>(V, key[k], i[k] + 1, Wn);
+    }
+  }
+}
+
 // Rank of key q, or D when absent, one key at a time (FB: leaves may resolve
 // through the BBHash).
 template <int NW, bool FB>
@@ -914,17 +1040,15 @@ __global__ void __launch_bounds__(256) k_lidx_lookup(const uint64_t* __restrict_
     bool ok[2];
     uint32_t line[2];
     lidx_pred_lines<2>(V, wt, root, key, ok, line);
-    uint64_t WA[NW], WB[NW];
-    load_line<NW>(V.lines, line[0], WA);
-    load_line<NW>(V.lines, line[1], WB);
-    hold_line<NW>(WA);
-    hold_line<NW>(WB);
-    const uint64_t ra = ok[0] ? lidx_finish<NW>(V, key[0], line[0], WA) : V.D;
-    out[i] = ra < V.D ? ra : 0;
-    if (has_b) {
-      const uint64_t rb = ok[1] ? lidx_finish<NW>(V, key[1], line[1], WB) : V.D;
-      out[j] = rb < V.D ? rb : 0;
-    }
+    uint64_t W[2][NW];
+    load_line<NW>(V.lines, line[0], W[0]);
+    load_line<NW>(V.lines, line[1], W[1]);
+    hold_line<NW>(W[0]);
+    hold_line<NW>(W[1]);
+    uint64_t res[2];
+    lidx_finish2<NW>(V, key, ok, line, W, res);
+    out[i] = res[0] < V.D ? res[0] : 0;
+    if (has_b) out[j] = res[1] < V.D ? res[1] : 0;
   }
 }
 

@@ -37,17 +37,17 @@ static_assert(sizeof(LeafRec) == 8, "leaf record is 8 bytes");
 
 // Lines: line i holds ranks [iC, iC + n_i) and the sentinel (rank iC + C)
 // when there is one, as a little-endian bit stream of u64 words:
-//   bits 0..61 k0 (key of rank iC), bit 62 overflow (keys searched instead)
+//   bits 0..61 k0 (key of rank iC), bit 62 overflow (keys searched instead),
+//   bit 63 code width w (0: 3 bits, 1: 4 bits)
 //   bits 64..69 t (cell shift), bits 70..79 ulen
 //   unary [80, 80 + ulen): per cell c = 0 .. (k_last - k0) >> t, a 1 per key
 //   with (key - k0) >> t == c, then a 0
-//   3-bit codes after the unary, one per key that shares the cell of the key
-//   before it: min(t - bitlen(rel_j ^ rel_{j-1}), 7); the highest bit where the
-//   two differ is t - 1 - code, and 7 means "search the run's keys".
+//   w-bit codes after the unary, one per key that shares the cell of the key
+//   before it: min(t - bitlen(rel_j ^ rel_{j-1}), 2^w - 1); the highest bit
+//   where the two differ is t - 1 - code, and 2^w - 1 means "search the
+//   run's keys". A line takes 4-bit codes when they fit and 3-bit codes would
+//   leave a code to search; a run of more than 64 / w codes is searched.
 constexpr int LINE_HDR_BITS = 80;
-constexpr int LINE_CODE_BITS = 3;
-constexpr uint32_t LINE_CODE_SEARCH = 7;
-constexpr uint32_t LINE_MAX_RUN_CODES = 21;  // codes of one run read as one 64-bit window
 
 struct LidxView {
   const RootPiece* root;     // R + 1

@@ -287,21 +287,20 @@ def error_bounds(keys, root, leaf, s):
 # A line holds C consecutive keys by rank (line i: ranks [iC, iC + n_i)) plus
 # the next line's first key (the sentinel, rank iC + C) when there is one, in
 # line_bytes bytes read as a little-endian bit stream of u64 words:
-#   bits  0..61  k0, the key of rank iC;  bit 62: overflow (no payload)
+#   bits  0..61  k0, the key of rank iC;  bit 62: overflow (no payload);
+#   bit 63: code width w (0: 3 bits, 1: 4 bits)
 #   bits 64..69  t, the cell shift;        bits 70..79: ulen
 #   unary [80, 80 + ulen): for cells c = 0 .. (k_last - k0) >> t, one 1 per
 #         key with (key - k0) >> t == c, then a 0
 #   codes after the unary: for every key j >= 1 in the cell of key j-1, in
-#         order, 3 bits min(t - bitlen((k_j - k0) ^ (k_{j-1} - k0)), 7): the
-#         highest bit where the two differ is t - 1 - code
-# t minimises m + cells + 3 collisions over [tb - 2, tb + 3] (first minimum),
-# tb = max(0, bitlen(span) - bitlen(m)); a line whose minimum exceeds
-# 8 line_bytes - 80 bits is an overflow line.
+#         order, w bits min(t - bitlen((k_j - k0) ^ (k_{j-1} - k0)), 2^w - 1):
+#         the highest bit where the two differ is t - 1 - code
+# For each w, t_w minimises m + cells + w collisions over [tb - 2, tb + 3]
+# (first minimum), tb = max(0, bitlen(span) - bitlen(m)). The line takes
+# 4-bit codes when they fit 8 line_bytes - 80 bits and 3-bit codes would not
+# fit or would hold a code of 7 (some pair closer than t_3 - 7 bits); else
+# 3-bit codes when they fit; else it is an overflow line.
 HDR_BITS = 80
-CODE_BITS = 3
-CODE_SEARCH = 7
-
-
 def default_line_keys(line_bytes):
     return {64: 104, 128: 256}[line_bytes]
 
@@ -318,21 +317,28 @@ def _line_keys(keys, i, C):
     return [int(x) for x in keys[a:a + m]], n, m
 
 
-def line_choice(ks):
-    """(t, bits, collisions) chosen for the keys of one line."""
+def line_choice(ks, line_bytes):
+    """(t, bits, collisions, code width) chosen for the keys of one line
+    (bits above the payload: an overflow line)."""
     m = len(ks)
     k0 = ks[0]
     rel = [x - k0 for x in ks]
     span = rel[-1]
     bl = [(rel[j] ^ rel[j - 1]).bit_length() for j in range(1, m)]
+    min_bl = min(bl) if bl else 64
     tb = max(0, span.bit_length() - m.bit_length())
-    best = None
+    best = {3: None, 4: None}
     for t in range(max(0, tb - 2), min(61, tb + 3) + 1):
         coll = sum(1 for b in bl if b <= t)
-        bits = m + (span >> t) + 1 + CODE_BITS * coll
-        if best is None or bits < best[1]:
-            best = (t, bits, coll)
-    return best
+        for w in (3, 4):
+            bits = m + (span >> t) + 1 + w * coll
+            if best[w] is None or bits < best[w][1]:
+                best[w] = (t, bits, coll)
+    payload = 8 * line_bytes - HDR_BITS
+    t3, b3, _ = best[3]
+    if best[4][1] <= payload and (b3 > payload or (min_bl <= t3 and t3 - min_bl >= 7)):
+        return best[4] + (4,)
+    return best[3] + (3,)
 
 
 def encode_line(keys, i, C, line_bytes):
@@ -349,10 +355,12 @@ def encode_line(keys, i, C, line_bytes):
 
     k0 = ks[0]
     put(0, k0, 62)
-    t, bits, coll = line_choice(ks)
+    t, bits, coll, cw = line_choice(ks, line_bytes)
     if bits > 8 * line_bytes - HDR_BITS:
         put(62, 1, 1)
         return np.array(w, np.uint64).view(np.uint8), (0, True, n)
+    if cw == 4:
+        put(63, 1, 1)
     rel = [x - k0 for x in ks]
     cells = (rel[-1] >> t) + 1
     ulen = m + cells
@@ -367,18 +375,19 @@ def encode_line(keys, i, C, line_bytes):
         pos += 1
     pos += 1
     assert pos == HDR_BITS + ulen
-    # codes, and the keys whose run holds a search code
-    run_search, run_len, run_bad, n_search = False, 1, False, 0
+    # codes, and the keys whose run holds a search code or too many codes
+    csearch = (1 << cw) - 1
+    run_len, run_bad, n_search = 1, False, 0
     for j in range(1, m + 1):
         same = j < m and (rel[j] >> t) == (rel[j - 1] >> t)
         if same:
-            code = min(t - (rel[j] ^ rel[j - 1]).bit_length(), CODE_SEARCH)
-            put(pos, code, CODE_BITS)
-            pos += CODE_BITS
+            code = min(t - (rel[j] ^ rel[j - 1]).bit_length(), csearch)
+            put(pos, code, cw)
+            pos += cw
             run_len += 1
-            run_bad |= code == CODE_SEARCH
+            run_bad |= code == csearch
         else:
-            if run_len > 1 and (run_bad or run_len - 1 > 21):
+            if run_len > 1 and (run_bad or run_len - 1 > 64 // cw):
                 n_search += run_len
             run_len, run_bad = 1, False
     return np.array(w, np.uint64).view(np.uint8), (bits, False, n_search)
@@ -418,6 +427,7 @@ def decode_line(line, q, i, C, n_total_lines):
         return ("left",)
     t = get(64, 6)
     ulen = get(70, 10)
+    cw = 4 if (w[0] >> 63) & 1 else 3
     rel = q - k0
     cq = rel >> t
     zeros = [b for b in range(HDR_BITS, HDR_BITS + ulen) if not bit(b)]
@@ -436,13 +446,13 @@ def decode_line(line, q, i, C, n_total_lines):
         return ("search", i * C + j0, i * C + last, True)
     if s == 1:
         return ("found", i * C + j0)
-    if s - 1 > 21:
+    if s - 1 > 64 // cw:
         return ("search", i * C + j0, i * C + last, False)
     ne = sum(1 for b in range(HDR_BITS + 1, e_a + 1) if not bit(b) and bit(b - 1))
     m0 = j0 - ne
-    cpos = HDR_BITS + ulen + CODE_BITS * m0
-    codes = [get(cpos + CODE_BITS * x, CODE_BITS) for x in range(s - 1)]
-    if CODE_SEARCH in codes:
+    cpos = HDR_BITS + ulen + cw * m0
+    codes = [get(cpos + cw * x, cw) for x in range(s - 1)]
+    if (1 << cw) - 1 in codes:
         return ("search", i * C + j0, i * C + last, False)
     lo, hi = 0, s - 1
     while lo < hi:

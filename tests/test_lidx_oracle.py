@@ -241,7 +241,7 @@ def test_line_format_by_hand():
     cells [0, 0, 0, 1] (12 bits: 4 + 2 cells + 2 codes of 3), so the unary is
     1110 10, and the codes are t - bitlen(1 ^ 0) = 3 and t - bitlen(4 ^ 1) = 1."""
     keys = np.array([16, 17, 20, 40], np.uint64)
-    assert LR.line_choice([16, 17, 20, 40]) == (4, 12, 2)
+    assert LR.line_choice([16, 17, 20, 40], 64) == (4, 12, 2, 3)
     lines, st = LR.encode_lines(keys, 4, 64)
     w = [int(x) for x in lines[0].view(np.uint64)]
     assert w[0] == 16  # k0, no overflow
@@ -257,6 +257,25 @@ def test_line_format_by_hand():
     for q in (18, 19, 21, 33, 41, 15):
         got = LR.line_lookup(keys, lines, 4, q, 0)
         assert got is None or 0 <= got < 4
+
+
+def test_line_takes_4bit_codes_for_a_close_pair():
+    """rel 0, 1, 2000, 4000: t = 10 is best for 3-bit codes (11 bits), but
+    the pair (0, 1) differs only in bit 0 (code 10 - 1 = 9 > 7, a search);
+    4-bit codes fit (12 bits), so the line takes them and the pair resolves
+    by its code."""
+    b = 1 << 20
+    ks = [b, b + 1, b + 2000, b + 4000]
+    assert LR.line_choice(ks, 64) == (10, 12, 1, 4)
+    keys = np.array(ks, np.uint64)
+    lines, st = LR.encode_lines(keys, 4, 64)
+    w = [int(x) for x in lines[0].view(np.uint64)]
+    assert (w[0] >> 63) & 1 == 1 and (w[0] >> 62) & 1 == 0
+    assert w[1] & 63 == 10 and (w[1] >> 6) & 1023 == 8
+    assert (w[1] >> (16 + 8)) & 15 == 9  # the one code, 4 bits
+    assert st[0] == (12, False, 0)
+    for r, q in enumerate(ks):
+        assert LR.decode_line(lines[0], q, 0, 4, 1) == ("found", r)
 
 
 def _check_all(keys, C, line_bytes, starts=(0,)):
