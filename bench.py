@@ -257,8 +257,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lookups", type=int, default=1 << 28, help="lookups per batch (0: skip the lookup benchmark)")
     ap.add_argument("--lookup-batches", type=int, default=4)
-    ap.add_argument("--lidx-leaf-bits", type=int, default=0, help="learned index: log2 leaves (0: the library default, 18)")
-    ap.add_argument("--lidx-line-bytes", type=int, default=0, help="learned index: 64 or 128 (0: 128)")
+    ap.add_argument("--lidx-leaf-bits", type=int, default=0, help="learned index: log2 leaves (0: the library default, 17)")
+    ap.add_argument("--lidx-line-bytes", type=int, default=0, help="learned index: 64 or 128 (0: 64)")
     ap.add_argument("--lidx-line-keys", type=int, default=0, help="learned index: keys per line (0: the default for the size)")
     ap.add_argument("--end-to-end", type=int, default=1, help="also time FASTA file -> P1..P4 (0: skip)")
     ap.add_argument("--ingest-threads", type=int, default=16)
@@ -577,10 +577,13 @@ def measured_copy_peak(nbytes=2 << 30, reps=5):
 def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
     """The learned index against the BBHash on the same n_batches x batch
     lookups of present keys (uniform over the distinct keys, seeded), keys
-    and results resident in HBM. Lookups/s from the lookup kernels' HIP-event
-    times. Every learned rank is checked against the drawn index (ranks are
-    positions in the sorted keys); the BBHash values are checked against the
-    BBHash of the sorted keys."""
+    and results resident in HBM. Each index gets one untimed batch of other
+    keys first (its structure into the caches), then its timed batches back
+    to back. Lookups/s from the lookup kernels' HIP-event times. Every
+    learned rank is checked against the drawn index (ranks are positions in
+    the sorted keys); the BBHash values are checked against the BBHash of the
+    sorted keys. Then P4 in the reference's form through each index, and the
+    same lookups through 2^20 leaf segments."""
     import ctypes as C
 
     import numpy as np
@@ -604,27 +607,28 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
     rng = np.random.default_rng(seed)
     out = np.empty(batch, np.uint64)
     errs = {"learned": 0, "bbhash": 0}
-    g.set_profiling(True)
-    g.kernel_times(reset=True)
-    for bi in range(n_batches):
-        idx = rng.integers(0, D, batch).astype(np.uint64)
-        q = keys[idx]
-        assert hip.hipMemcpy(dq, q.ctypes.data_as(C.c_void_p), C.c_size_t(batch * 8), 1) == 0
-        g._check(L.cf1g_lidx_lookup_device(g.h, dq, batch, dout))
-        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
-        errs["learned"] += int(np.count_nonzero(out != idx))
-        g._check(L.cf1g_mphf_lookup_device(g.h, dq, batch, dout))
-        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
-        errs["bbhash"] += int(np.count_nonzero(out != vall[idx]))
-        print(f"[bench] lookup batch {bi + 1}/{n_batches} done", file=sys.stderr, flush=True)
-    kt = g.kernel_times(reset=True)
-    # classification pass through each index (every distinct k-mer looked up)
-    g.classify_via(1)
-    g.classify_via(0)
-    kc = g.kernel_times(reset=True)
-    g.set_profiling(False)
-    for p in (dq, dout, dall):
-        hip.hipFree(p)
+    batches = [rng.integers(0, D, batch).astype(np.uint64) for _ in range(n_batches + 1)]  # the last: warm-up
+    kt = {}
+    # each index: one untimed batch (its structure into the caches), then
+    # the timed batches back to back; the other index's batches after
+    for name, fn, kname in (("learned", L.cf1g_lidx_lookup_device, "lidx_lookup"), ("bbhash", L.cf1g_mphf_lookup_device, "bbh_lookup")):
+        for bi in range(-1, n_batches):
+            idx = batches[bi]
+            assert hip.hipMemcpy(dq, keys[idx].ctypes.data_as(C.c_void_p), C.c_size_t(batch * 8), 1) == 0
+            if bi == 0:
+                g.set_profiling(True)
+                g.kernel_times(reset=True)
+            g._check(fn(g.h, dq, batch, dout))
+            if bi < 0:
+                continue
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
+            errs[name] += int(np.count_nonzero(out != (idx if name == "learned" else vall[idx])))
+            print(f"[bench] {name} lookup batch {bi + 1}/{n_batches} done", file=sys.stderr, flush=True)
+        kt[kname] = g.kernel_times(reset=True).get(kname, (0, 0.0))
+        g.set_profiling(False)
+    # P4 in the reference's form through each index: one lookup per k-mer
+    # occurrence in sequence order plus the CAS join
+    via = {"bbhash": g.classify_via(0), "learned": g.classify_via(1)}
     n = batch * n_batches
     lms = kt.get("lidx_lookup", (0, 0.0))[1]
     bms = kt.get("bbh_lookup", (0, 0.0))[1]
@@ -633,14 +637,41 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
     keep = ("n_leaves", "n_pieces", "fallback_leaves", "fallback_keys", "n_lines", "line_bytes", "line_keys",
             "overflow_lines", "search_keys", "max_err_lo", "max_err_hi", "mean_abs_err", "line_bits_per_key", "bytes_total",
             "root_loss", "leaf_loss", "sort_ms", "train_ms")
+    summary = {k2: (round(info[k2], 4) if isinstance(info[k2], float) else info[k2]) for k2 in keep}
+    # the same lookups through 2^20 leaf segments (the leaf table then leaves L2)
+    shape20 = None
+    if D >= (1 << 21):
+        info20 = g.lidx_train(leaf_bits=20, line_bytes=(lidx_args or {}).get("line_bytes", 0),
+                              line_keys=(lidx_args or {}).get("line_keys", 0))
+        assert hip.hipMemcpy(dq, keys[batches[-1]].ctypes.data_as(C.c_void_p), C.c_size_t(batch * 8), 1) == 0
+        g._check(L.cf1g_lidx_lookup_device(g.h, dq, batch, dout))  # warm-up
+        idx = batches[0]
+        assert hip.hipMemcpy(dq, keys[idx].ctypes.data_as(C.c_void_p), C.c_size_t(batch * 8), 1) == 0
+        g.set_profiling(True)
+        g.kernel_times(reset=True)
+        g._check(L.cf1g_lidx_lookup_device(g.h, dq, batch, dout))
+        k20 = g.kernel_times(reset=True)
+        g.set_profiling(False)
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
+        ms20 = k20.get("lidx_lookup", (0, 0.0))[1]
+        shape20 = {"n_leaves": info20["n_leaves"], "lookups": batch, "lookups_per_s": batch / (ms20 * 1e-3) if ms20 else None,
+                   "rank_errors": int(np.count_nonzero(out != idx)), "mean_abs_err": round(info20["mean_abs_err"], 3),
+                   "max_err": [info20["max_err_lo"], info20["max_err_hi"]]}
+        g.lidx_train(**(lidx_args or {}))  # back to the measured shape (P5 through the index uses it)
+        print(f"[bench] 2^20 leaves: {shape20['lookups_per_s'] / 1e9:.2f} G lookups/s", file=sys.stderr, flush=True)
+    for p in (dq, dout, dall):
+        hip.hipFree(p)
     return {
         "n": n, "batch": batch, "keys": "uniform over the distinct keys, seed %d" % seed,
         "bbhash_lookups_per_s": brate, "learned_lookups_per_s": lrate,
         "learned_vs_bbhash": (lrate / brate) if (lrate and brate) else None,
         "learned_rank_errors": errs["learned"], "bbhash_value_errors": errs["bbhash"],
         "lookup_kernel_ms": {"learned": round(lms, 3), "bbhash": round(bms, 3)},
-        "classify_via_ms": {k2: round(v[1], 3) for k2, v in kc.items()},
-        "learned_index": {k2: (round(info[k2], 4) if isinstance(info[k2], float) else info[k2]) for k2 in keep},
+        "p4_cas_via": {k2: {"kernel_ms": round(v["kernel_ms"], 3), "occurrences": v["occurrences"],
+                            "occurrences_per_s": v["occurrences"] / (v["kernel_ms"] * 1e-3) if v["kernel_ms"] else None,
+                            "cas_issued": v["cas_issued"], "cas_failed": v["cas_failed"]} for k2, v in via.items()},
+        "learned_index": summary,
+        "leaf_segments_2p20": shape20,
         "train_wall_s": round(train_wall, 3),
         "roofline": lookup_roofline(far, {"bbh_lookup": brate, "lidx_lookup": lrate}),
     }

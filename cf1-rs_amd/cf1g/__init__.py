@@ -155,7 +155,7 @@ def lib():
         F32P = C.POINTER(C.c_float)
         F64P = C.POINTER(C.c_double)
         L.cf1g_lidx_gradients.argtypes = [C.c_void_p, C.c_int, F32P, F32P, C.c_uint64, F64P, F64P, F64P, U64P, F32P, F32P]
-        L.cf1g_classify_via.argtypes = [C.c_void_p, C.c_int]
+        L.cf1g_classify_via.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.cf1g_select_index.argtypes = [C.c_void_p, C.c_int]
         L.cf1g_selected_index.argtypes = [C.c_void_p]
         L.cf1g_record_starts.argtypes = [C.c_void_p, U64P, C.c_uint64]
@@ -541,7 +541,10 @@ def _lidx_methods():
         return {"gw": gw, "gc": gc, "sq": sq, "seg": seg, "mu": mu, "isd": isd}
 
     def classify_via(self, index):
-        self._check(self.L.cf1g_classify_via(self.h, index))
+        """P4 per occurrence (lookup + CAS) through an index; its CAS stats."""
+        o = CasStats()
+        self._check(self.L.cf1g_classify_via(self.h, index, C.byref(o)))
+        return {f: getattr(o, f) for f, _ in CasStats._fields_}
 
     def select_index(self, index):
         self._check(self.L.cf1g_select_index(self.h, index))

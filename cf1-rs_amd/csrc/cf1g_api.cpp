@@ -91,8 +91,8 @@ void launch_lidx_warp_hist(hipStream_t, const uint64_t*, uint64_t, int, int, uin
 void launch_lidx_warp_keys(hipStream_t, const uint64_t*, uint64_t, const uint32_t*, int, int, uint64_t*);
 void launch_lidx_bbh2rank(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, const MphfView&, uint32_t*);
 void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*);
-void launch_classify_via(hipStream_t, bool, const uint64_t*, const uint16_t*, const LidxView&, const MphfView&, uint8_t*, unsigned int*);
-void launch_states_pack(hipStream_t, const uint8_t*, uint64_t, uint64_t*);
+void launch_p4_cas_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&, uint64_t*,
+                        uint64_t*, uint32_t, uint32_t, int);
 void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&,
                           const uint64_t*, uint8_t*);
 void launch_annotate(hipStream_t, bool, const uint64_t*, const uint64_t*, uint64_t, int, const MphfView&, const uint64_t*, uint64_t,
@@ -807,10 +807,13 @@ void do_p4(cf1g_ctx* c) {
 
 // P4 in the reference's form: zeroed states, then one MPHF lookup and CAS
 // join per k-mer occurrence of the loaded records (k_p4_cas).
-void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
+// P4 in the reference's form: one lookup per k-mer occurrence and a CAS join,
+// through the BBHash (index 0, into "states") or the learned index (1, into
+// "states_lidx", in rank order).
+void do_p4_cas_via(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out, int index) {
   hipStream_t st = c->st;
   const uint64_t nsw = (c->D + 11) / 12;
-  uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
+  uint64_t* states = c->getT<uint64_t>(index == 1 ? "states_lidx" : "states", nsw + 1);
   uint64_t* cnt = c->getT<uint64_t>("p4_counters", 8);
   c->zero(states, (nsw + 1) * 8);
   c->zero(cnt, 64);
@@ -822,8 +825,11 @@ void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
   const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->cu_count * wpc, std::max<uint64_t>(n_chunks, 1));
   hipEvent_t e0 = ev_get(), e1 = ev_get();
   hip_check(hipEventRecord(e0, st), "hipEventRecord");
-  launch_p4_cas(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->getT<uint64_t>("nmask", c->n_words64 + 8), c->n_pos,
-                c->k, c->view(), c->D, states, cnt, chunk, grid, (flags & CF1G_CAS_FORCE_RETRY) ? 1 : 0);
+  const uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
+  const uint64_t* nm = c->getT<uint64_t>("nmask", c->n_words64 + 8);
+  const int fr = (flags & CF1G_CAS_FORCE_RETRY) ? 1 : 0;
+  if (index == 1) launch_p4_cas_lidx(st, seq, nm, c->n_pos, c->k, c->lview(), c->view(), states, cnt, chunk, grid, fr);
+  else launch_p4_cas(st, c->wide, seq, nm, c->n_pos, c->k, c->view(), c->D, states, cnt, chunk, grid, fr);
   hip_check(hipEventRecord(e1, st), "hipEventRecord");
   uint64_t h[5] = {0, 0, 0, 0, 0};
   c->d2h(h, cnt, 40);
@@ -831,7 +837,7 @@ void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
   hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
   ev_put(e0);
   ev_put(e1);
-  if (h[3]) fail(CF1G_E_INTERNAL, std::to_string(h[3]) + " k-mer occurrences are missing from the MPHF");
+  if (h[3]) fail(CF1G_E_INTERNAL, std::to_string(h[3]) + " k-mer occurrences are missing from the index");
   if (h[0] != c->W) fail(CF1G_E_INTERNAL, "P4 classified a different number of occurrences than P1 counted");
   if (out) {
     cf1g_cas_stats o{};
@@ -845,8 +851,11 @@ void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
     o.kernel_ms = ms;
     *out = o;
   }
-  c->use_index = 0;
   c->state_gen++;
+}
+void do_p4_cas(cf1g_ctx* c, uint32_t flags, cf1g_cas_stats* out) {
+  do_p4_cas_via(c, flags, out, 0);
+  c->use_index = 0;
   c->set_stage(5);
 }
 
@@ -986,10 +995,10 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   if (c->wide) fail(CF1G_E_INVALID, "the learned index takes u64 keys (k <= 31)");
   cf1g_lidx_params p{};
   if (prm) p = *prm;
-  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 18, root_bits = p.root_bits ? p.root_bits : 10;
+  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 17, root_bits = p.root_bits ? p.root_bits : 10;
   const uint32_t epochs = p.epochs ? p.epochs : 4, max_window = p.max_window ? p.max_window : 4096;
   const float lr = p.lr > 0 ? p.lr : 0.5f;
-  const uint32_t line_bytes = p.line_bytes ? p.line_bytes : 128;
+  const uint32_t line_bytes = p.line_bytes ? p.line_bytes : 64;
   if (leaf_bits > 26 || root_bits > 20) fail(CF1G_E_INVALID, "leaf_bits <= 26 and root_bits <= 20");
   if (line_bytes != 64 && line_bytes != 128) fail(CF1G_E_INVALID, "line_bytes is 64 or 128");
   const uint32_t C = p.line_keys ? p.line_keys : (line_bytes == 128 ? 256u : 112u);
@@ -1992,27 +2001,13 @@ int cf1g_lidx_gradients(cf1g_ctx* c, int level, const float* w, const float* cc,
   });
 }
 
-int cf1g_classify_via(cf1g_ctx* c, int index) {
+int cf1g_classify_via(cf1g_ctx* c, int index, cf1g_cas_stats* out) {
   return guarded(c, [&]() {
     need(c, 4, "cf1g_classify_via");
-    c->state_gen++;
     if (c->wide) fail(CF1G_E_INVALID, "classification through an index takes u64 keys (k <= 31)");
     if (index == 1 && !c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_classify_via(1) called before cf1g_lidx_train");
     if (index != 0 && index != 1) fail(CF1G_E_INVALID, "index is 0 (BBHash) or 1 (learned)");
-    const uint64_t D = c->D, nsw = (D + 11) / 12;
-    uint8_t* st8 = c->getT<uint8_t>("st8", D + 8);
-    c->zero(st8, D + 8);
-    unsigned int* err = c->getT<unsigned int>("via_err", 2);
-    c->zero(err, 4);
-    LidxView v{};
-    if (index == 1) v = c->lview();
-    v.D = D;
-    launch_classify_via(c->st, index == 1, (const uint64_t*)dense_keys(c), c->getT<uint16_t>("vmask", 12 * nsw + 16), v, c->view(),
-                        st8, err);
-    launch_states_pack(c->st, st8, D, c->getT<uint64_t>(index == 1 ? "states_lidx" : "states", nsw + 1));
-    unsigned int he = 0;
-    c->d2h(&he, err, 4);
-    if (he) fail(CF1G_E_INTERNAL, "a distinct k-mer was not found through the index");
+    do_p4_cas_via(c, 0, out, index);
     if (index == 1) c->lidx_states = true;
   });
 }

@@ -8,7 +8,7 @@
 //
 //  sort     : LSD radix sort of the keys, 8-bit digits (k_sort_hist / scan /
 //             k_sort_scatter), stable tile ranking in registers.
-//  warp     : u = an order-0 arithmetic-coding CDF of the key's first four
+//  warp     : u = an order-0 arithmetic-coding CDF of the key's first three
 //             bytes (symbol counts of the keys themselves, histogrammed on
 //             the GPU), non-decreasing in the key and close to uniform, so
 //             the linear models below fit the keys' random spacing instead
@@ -48,6 +48,7 @@
 //             or after its last cell, and searches the sorted keys for the
 //             rare runs the codes cannot resolve and for lines that overflow.
 #include "block.hpp"
+#include "classify.hpp"
 #include "device.hpp"
 #include "launch.hpp"
 #include "lidx.hpp"
@@ -159,13 +160,22 @@ __device__ __forceinline__ uint32_t root_leaf(const RootPiece& P, uint32_t leaf0
   const float c = f < 0.f ? 0.f : (f > span ? span : f);
   return P.leaf0 + (uint32_t)c;
 }
-__device__ __forceinline__ float piece_x(uint64_t key, int s, uint32_t r) { return (float)(key - ((uint64_t)r << s)); }
+// x = u - (r << s) in FP32 from its top 32 bits (32 < s): (float)(u32)(u >>
+// (s - 32)) * 2^(s - 32). The bits dropped lie far below the keys' spacing.
+__device__ __forceinline__ float piece_x(uint64_t u, int s) {
+  return ldexpf((float)(uint32_t)(u >> (s - 32)), s - 32);
+}
 __device__ __forceinline__ bool leaf_fallback(const LeafRec& Lf) { return (__float_as_uint(Lf.S) >> 31) != 0; }
-__device__ __forceinline__ int64_t leaf_pred(const RootPiece& P, const LeafRec& Lf, float x) {
+// The leaf's rank relative to its piece: floor(fma(|S|, x, I) + 0.5), clamped
+// to +-2^30.
+__device__ __forceinline__ int32_t leaf_local(const LeafRec& Lf, float x) {
   const float S = __uint_as_float(__float_as_uint(Lf.S) & 0x7FFFFFFFu);
   float p = floorf(__fmaf_rn(S, x, Lf.I) + 0.5f);
-  p = p < -1.0e12f ? -1.0e12f : (p > 1.0e12f ? 1.0e12f : p);
-  return (int64_t)P.p + (int64_t)p;
+  p = p < -1073741824.0f ? -1073741824.0f : (p > 1073741824.0f ? 1073741824.0f : p);
+  return (int32_t)p;
+}
+__device__ __forceinline__ int64_t leaf_pred(const RootPiece& P, const LeafRec& Lf, float x) {
+  return (int64_t)P.p + (int64_t)leaf_local(Lf, x);
 }
 
 // ------------------------------------------------------------------ training
@@ -390,12 +400,12 @@ __global__ void __launch_bounds__(256) k_lidx_leaf_bounds(const uint64_t* __rest
   if (i > 0) {
     const uint64_t k = keys[i - 1];
     const uint32_t r = (uint32_t)(k >> s);
-    prev = root_leaf(root[r], root[r + 1].leaf0, piece_x(k, s, r));
+    prev = root_leaf(root[r], root[r + 1].leaf0, piece_x(k, s));
   }
   if (i < D) {
     const uint64_t k = keys[i];
     const uint32_t r = (uint32_t)(k >> s);
-    cur = root_leaf(root[r], root[r + 1].leaf0, piece_x(k, s, r));
+    cur = root_leaf(root[r], root[r + 1].leaf0, piece_x(k, s));
   }
   for (int64_t j = prev + 1; j <= cur; j++) q[j] = i;
 }
@@ -428,7 +438,7 @@ __global__ void __launch_bounds__(256) k_lidx_leaf_final(const uint64_t* __restr
     Lf.S = (float)S;
     Lf.I = (float)I;
     for (uint64_t i = a + lane; i < b; i += 64) {
-      const int64_t e = (int64_t)i - leaf_pred(P, Lf, piece_x(keys[i], s, r));
+      const int64_t e = (int64_t)i - leaf_pred(P, Lf, piece_x(keys[i], s));
       elo = -e > elo ? -e : elo;
       ehi = e > ehi ? e : ehi;
       sa += (uint64_t)(e < 0 ? -e : e);
@@ -722,14 +732,14 @@ __device__ __forceinline__ void load_line(const uint64_t* __restrict__ lines, ui
 // The line a key's predicted rank falls in (leaves without a BBHash mark;
 // no branch, so that two keys' loads can be in flight together). ok = false
 // for a key wider than 2k bits or an empty index.
-__device__ __forceinline__ uint32_t lidx_pred_line(const LidxView& V, const uint32_t* __restrict__ warp, uint64_t key, bool& ok,
-                                                   LeafRec& Lf) {
+__device__ __forceinline__ uint32_t lidx_pred_line(const LidxView& V, const uint32_t* __restrict__ warp,
+                                                   const RootPiece* __restrict__ root, uint64_t key, bool& ok, LeafRec& Lf) {
   ok = !((V.key_bits < 64 && (key >> V.key_bits) != 0) || V.n_lines == 0);
   const uint64_t u = warp_key(key, warp, V.key_bits, V.warp_groups);
-  const uint32_t r = V.s >= 64 ? 0u : (uint32_t)(u >> V.s);
-  const RootPiece P = V.root[r];
-  const uint32_t leaf0n = V.root[r + 1].leaf0;
-  const float x = (float)(V.s >= 64 ? u : u - ((uint64_t)r << V.s));
+  const uint32_t r = (uint32_t)(u >> V.s);
+  const RootPiece P = root[r];
+  const uint32_t leaf0n = root[r + 1].leaf0;
+  const float x = piece_x(u, V.s);
   Lf = V.leaf[root_leaf(P, leaf0n, x)];
   int64_t pred = leaf_pred(P, Lf, x);
   pred = pred < 0 ? 0 : (pred >= (int64_t)V.D ? (int64_t)V.D - 1 : pred);
@@ -749,8 +759,8 @@ __device__ __forceinline__ void lidx_pred_lines(const LidxView& V, const uint32_
   for (int k = 0; k < NK; k++) {
     ok[k] = !((V.key_bits < 64 && (key[k] >> V.key_bits) != 0) || V.n_lines == 0);
     const uint64_t u = warp_key(key[k], warp, V.key_bits, V.warp_groups);
-    r[k] = V.s >= 64 ? 0u : (uint32_t)(u >> V.s);
-    x[k] = (float)(V.s >= 64 ? u : u - ((uint64_t)r[k] << V.s));
+    r[k] = (uint32_t)(u >> V.s);
+    x[k] = piece_x(u, V.s);
   }
 #pragma unroll
   for (int k = 0; k < NK; k++) {
@@ -979,10 +989,10 @@ __device__ __forceinline__ void lidx_finish2(const LidxView& V, const uint64_t (
 // through the BBHash).
 template <int NW, bool FB>
 __device__ __forceinline__ uint64_t lidx_rank_t(const LidxView& V, const MphfView& mv, const uint32_t* __restrict__ warp,
-                                                uint64_t key) {
+                                                const RootPiece* __restrict__ root, uint64_t key) {
   bool ok;
   LeafRec Lf;
-  const uint32_t i = lidx_pred_line(V, warp, key, ok, Lf);
+  const uint32_t i = lidx_pred_line(V, warp, root, key, ok, Lf);
   if (!ok) return V.D;
   if (FB && leaf_fallback(Lf)) {
     const uint64_t v = mphf_lookup<uint64_t>(mv, key);
@@ -995,7 +1005,7 @@ __device__ __forceinline__ uint64_t lidx_rank_t(const LidxView& V, const MphfVie
   return lidx_finish<NW>(V, key, i, W);
 }
 __device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView& mv, uint64_t key) {
-  return V.line_words == 8 ? lidx_rank_t<8, true>(V, mv, V.warp, key) : lidx_rank_t<16, true>(V, mv, V.warp, key);
+  return V.line_words == 8 ? lidx_rank_t<8, true>(V, mv, V.warp, V.root, key) : lidx_rank_t<16, true>(V, mv, V.warp, V.root, key);
 }
 
 // The warp tables (and, with ROOT_LDS, the root records) go to LDS once per
@@ -1003,9 +1013,11 @@ __device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView&
 // stride) through the front of the lookup together: both predictions, then
 // both first lines in flight at once; the next two keys are loaded while the
 // current ones are resolved.
+// Compiled for 6 waves per SIMD (80 VGPRs without spills for 64-byte lines):
+// the kernel waits on dependent far loads, so waves in flight set its rate.
 constexpr uint32_t LIDX_LDS_ROOT_MAX = 1024;  // root pieces held in LDS (16 KiB)
 template <int NW, bool FB, bool ROOT_LDS>
-__global__ void __launch_bounds__(256) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
                                                      uint64_t* __restrict__ out) {
   __shared__ uint32_t wt[WARP_MAX_GROUPS * 256];
   __shared__ RootPiece rt[ROOT_LDS ? LIDX_LDS_ROOT_MAX + 1 : 1];
@@ -1018,7 +1030,7 @@ __global__ void __launch_bounds__(256) k_lidx_lookup(const uint64_t* __restrict_
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (FB) {
     for (; i < n; i += stride) {
-      const uint64_t r = lidx_rank_t<NW, true>(V, mv, wt, keys[i]);
+      const uint64_t r = lidx_rank_t<NW, true>(V, mv, wt, root, keys[i]);
       out[i] = r < V.D ? r : 0;
     }
     return;
@@ -1063,37 +1075,56 @@ __global__ void __launch_bounds__(256) k_count_mismatch(const uint64_t* __restri
 }
 
 // ------------------------------------------------------------------ classification through an index
-// Every distinct k-mer (P3 keys with their joined masks, both in MPHF value
-// order) is looked up through the index and its 5-bit state stored as a byte
-// at that slot; k_states_pack packs the bytes 12 per word.
-template <bool LEARNED>
-__global__ void __launch_bounds__(256) k_classify_via(const uint64_t* __restrict__ keys, const uint16_t* __restrict__ vmask,
-                                                      const LidxView V, const MphfView mv, uint8_t* __restrict__ st8,
-                                                      unsigned int* __restrict__ err) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V.D; v += stride) {
-    const uint64_t k = keys[v];
-    uint64_t idx;
-    if (LEARNED) idx = lidx_rank(V, mv, k);
-    else idx = mphf_lookup<uint64_t>(mv, k);
-    if (idx >= V.D) {
-      atomicOr(err, 1u);
-      continue;
+// P4 in the reference's form (classify.rs:524-601) through the learned
+// index: one lane per k-mer occurrence of the flat stream, in sequence order,
+// looks the canonical k-mer up (its rank) and joins the occurrence's
+// observation into the state at that rank by CAS; the persistent grid takes
+// chunks of stream positions from a counter, as k_p4_cas does for the
+// BBHash. counters: [0] occurrences, [1] CAS issued, [2] CAS lost, [3]
+// lookups that found no rank, [4] the chunk ticket.
+template <int NW, bool FB>
+__global__ void __launch_bounds__(256) k_p4_cas_lidx(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask,
+                                                     uint64_t n_pos, int K, const LidxView V, const MphfView mv,
+                                                     unsigned long long* __restrict__ states, unsigned long long* __restrict__ counters,
+                                                     uint32_t chunk, uint64_t n_chunks, int force_retry) {
+  __shared__ uint32_t wt[WARP_MAX_GROUPS * 256];
+  __shared__ RootPiece rt[LIDX_LDS_ROOT_MAX + 1];
+  __shared__ uint64_t s_c;
+  __shared__ unsigned long long s_sum[4];
+  for (int j = threadIdx.x; j < V.warp_groups * 256; j += 256) wt[j] = V.warp[j];
+  const bool rl = V.R <= LIDX_LDS_ROOT_MAX;
+  if (rl)
+    for (uint32_t j = threadIdx.x; j <= V.R; j += 256) rt[j] = V.root[j];
+  if (threadIdx.x < 4) s_sum[threadIdx.x] = 0;
+  const RootPiece* root = rl ? rt : V.root;
+  uint32_t n_occ = 0, n_cas = 0, n_fail = 0, n_miss = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_c = atomicAdd(&counters[4], 1ULL);
+    __syncthreads();
+    const uint64_t c = s_c;
+    __syncthreads();
+    if (c >= n_chunks) break;
+    const uint64_t g0 = 1 + c * chunk;
+    const uint64_t g1 = g0 + chunk < n_pos ? g0 + chunk : n_pos;
+    for (uint64_t g = g0 + threadIdx.x; g < g1; g += 256) {
+      uint64_t canon;
+      const uint32_t m = observe<uint64_t>(seq, nmask, g, K, canon);
+      if (!m) continue;
+      n_occ++;
+      const uint64_t idx = lidx_rank_t<NW, FB>(V, mv, wt, root, canon);
+      if (idx >= V.D) {
+        n_miss++;
+        continue;
+      }
+      cas_join(states, idx, m, force_retry != 0, n_cas, n_fail);
     }
-    st8[idx] = (uint8_t)mask_state(vmask[v]);
   }
-}
-__global__ void __launch_bounds__(256) k_states_pack(const uint8_t* __restrict__ st8, uint64_t D, uint64_t n_words,
-                                                     uint64_t* __restrict__ states) {
-  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= n_words) return;
-  uint64_t out = 0;
-#pragma unroll
-  for (int i = 0; i < 12; i++) {
-    const uint64_t x = 12 * w + (uint64_t)i;
-    if (x < D) out |= (uint64_t)(st8[x] & 31u) << (5 * i);
-  }
-  states[w] = out;
+  atomicAdd(&s_sum[0], (unsigned long long)n_occ);
+  atomicAdd(&s_sum[1], (unsigned long long)n_cas);
+  atomicAdd(&s_sum[2], (unsigned long long)n_fail);
+  atomicAdd(&s_sum[3], (unsigned long long)n_miss);
+  __syncthreads();
+  if (threadIdx.x < 4) atomicAdd(&counters[threadIdx.x], s_sum[threadIdx.x]);
 }
 
 // Per stream position through the learned index: bit0 valid k-mer, bit1 FWD,
@@ -1229,20 +1260,21 @@ void launch_count_mismatch(hipStream_t st, const uint64_t* a, const uint64_t* b,
   if (g > 16384) g = 16384;
   CF1G_LAUNCH("count_mismatch", k_count_mismatch, dim3((unsigned)g), dim3(256), 0, st, a, b, n, bad);
 }
-void launch_classify_via(hipStream_t st, bool learned, const uint64_t* keys, const uint16_t* vmask, const LidxView& V,
-                         const MphfView& mv, uint8_t* st8, unsigned int* err) {
-  if (!V.D) return;
-  uint64_t g = (V.D + 255) / 256;
-  if (g > 65536) g = 65536;
-  if (learned)
-    CF1G_LAUNCH("p4_classify_lidx", k_classify_via<true>, dim3((unsigned)g), dim3(256), 0, st, keys, vmask, V, mv, st8, err);
-  else
-    CF1G_LAUNCH("p4_classify_bbh", k_classify_via<false>, dim3((unsigned)g), dim3(256), 0, st, keys, vmask, V, mv, st8, err);
-}
-void launch_states_pack(hipStream_t st, const uint8_t* st8, uint64_t D, uint64_t* states) {
-  const uint64_t nw = (D + 11) / 12;
-  if (!nw) return;
-  CF1G_LAUNCH("p4_states_pack", k_states_pack, dim3(blocks_for(nw, 256)), dim3(256), 0, st, st8, D, nw, states);
+void launch_p4_cas_lidx(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, const LidxView& V,
+                        const MphfView& mv, uint64_t* states, uint64_t* counters, uint32_t chunk, uint32_t grid, int force_retry) {
+  if (n_pos <= 1 || chunk == 0 || grid == 0) return;
+  const uint64_t n_chunks = (n_pos - 1 + chunk - 1) / chunk;
+  auto* sv = (unsigned long long*)states;
+  auto* cn = (unsigned long long*)counters;
+#define CF1G_P4_LIDX(NW_, FB_)                                                                                                 \
+  CF1G_LAUNCH("p4_cas_lidx", (k_p4_cas_lidx<NW_, FB_>), dim3(grid), dim3(256), 0, st, seq, nmask, n_pos, K, V, mv, sv, cn, chunk, \
+              n_chunks, force_retry)
+  if (V.line_words == 8) {
+    if (V.fallback) CF1G_P4_LIDX(8, true); else CF1G_P4_LIDX(8, false);
+  } else {
+    if (V.fallback) CF1G_P4_LIDX(16, true); else CF1G_P4_LIDX(16, false);
+  }
+#undef CF1G_P4_LIDX
 }
 void launch_annotate_lidx(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, uint64_t n_pos, int K, const LidxView& V,
                           const MphfView& mv, const uint64_t* states, uint8_t* out) {

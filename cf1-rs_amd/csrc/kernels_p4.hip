@@ -21,35 +21,11 @@
 // takes fixed-size chunks of stream positions from an atomic counter (work
 // stealing over records of any length; a chunk reads up to K bases past its
 // end, the k-1 overlap). Its states equal k_p4_pack's word for word.
+#include "classify.hpp"
 #include "device.hpp"
 #include "launch.hpp"
 
 namespace cf1g {
-
-template <typename KeyT>
-struct KOps;
-template <>
-struct KOps<uint64_t> {
-  static __device__ __forceinline__ uint64_t at(const uint64_t* seq, uint64_t g, int K) { return kmer64_at(seq, g, K); }
-  static __device__ __forceinline__ uint64_t rc(uint64_t x, int K) { return rc64(x, K); }
-  static __device__ __forceinline__ void roll(uint64_t& x, uint64_t& r, uint32_t b, int K) {
-    uint64_t mask = (K == 32) ? ~0ULL : ((1ULL << (2 * K)) - 1);
-    x = ((x << 2) | b) & mask;
-    r = (r >> 2) | ((uint64_t)(3 - b) << (2 * K - 2));
-  }
-};
-template <>
-struct KOps<Key128> {
-  static __device__ __forceinline__ Key128 at(const uint64_t* seq, uint64_t g, int K) { return kmer128_at(seq, g, K); }
-  static __device__ __forceinline__ Key128 rc(const Key128& x, int K) { return rc128(x, K); }
-  static __device__ __forceinline__ void roll(Key128& x, Key128& r, uint32_t b, int K) {
-    uint64_t mh = (1ULL << (2 * K - 64)) - 1;
-    x.hi = ((x.hi << 2) | (x.lo >> 62)) & mh;
-    x.lo = (x.lo << 2) | b;
-    r.lo = (r.lo >> 2) | (r.hi << 62);
-    r.hi = (r.hi >> 2) | ((uint64_t)(3 - b) << (2 * K - 66));
-  }
-};
 
 // One thread per state word: the 12 joined masks of values [12w, 12w+12)
 // (three 8-byte loads; the mask array is value-ordered by P3) become 5-bit
@@ -67,36 +43,6 @@ __global__ void __launch_bounds__(256) k_p4_pack(const uint64_t* __restrict__ vm
     if ((uint64_t)i < valid) out |= (uint64_t)mask_state(mk) << (5 * i);
   }
   states[w] = out;
-}
-
-// Observation mask of the k-mer occurrence at stream position g >= 1, or 0
-// when no valid k-mer starts there. Neighbours are the bases at g-1 and g+K
-// (a placeholder or record end is "none"); the occurrence type follows
-// classify.rs:65-177: isolated -> MIMO, no right neighbour -> rightmost,
-// self loop (the next k-mer has the same canonical form, :143-145) -> MIMO
-// when leftmost or a palindromic step, else rightmost (:160-177); otherwise
-// leftmost / internal.
-template <typename KeyT>
-__device__ __forceinline__ uint32_t observe(const uint64_t* __restrict__ seq, const uint64_t* __restrict__ nmask, uint64_t g, int K,
-                                            KeyT& canon) {
-  if (!window_clear(nmask, g, K)) return 0;
-  const bool pv = !is_n(nmask, g - 1), nv = !is_n(nmask, g + K);
-  const KeyT x = KOps<KeyT>::at(seq, g, K);
-  const KeyT r = KOps<KeyT>::rc(x, K);
-  canon = kmer_min(x, r);
-  const uint32_t pb = base_at(seq, g - 1), nb = base_at(seq, g + K);
-  uint32_t type;
-  if (!pv && !nv) {
-    type = OBS_MIMO;
-  } else if (!nv) {
-    type = OBS_RIGHT;
-  } else {
-    KeyT x2 = x, r2 = r;
-    KOps<KeyT>::roll(x2, r2, nb, K);
-    if (key_eq(kmer_min(x2, r2), canon)) type = (!pv || key_eq(x, x2)) ? OBS_MIMO : OBS_RIGHT;
-    else type = pv ? OBS_INTERNAL : OBS_LEFT;
-  }
-  return obs_mask(type, key_eq(x, canon), pb, nb);
 }
 
 // Persistent work-stealing classification by CAS (see the file comment).
@@ -134,23 +80,7 @@ __global__ void __launch_bounds__(P4_WG) k_p4_cas(const uint64_t* __restrict__ s
         n_miss++;
         continue;
       }
-      unsigned long long* wp = states + idx / 12;
-      const uint32_t sh = (uint32_t)(idx % 12) * 5;
-      unsigned long long cur = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      bool inject = force_retry != 0;
-      for (;;) {
-        const uint32_t s = (uint32_t)(cur >> sh) & 31u;
-        const uint32_t ns = mask_state(state_mask(s) | m);
-        if (ns == s) break;
-        const unsigned long long want = (cur & ~(31ULL << sh)) | ((unsigned long long)ns << sh);
-        const unsigned long long expect = inject ? (cur | (1ULL << 63)) : cur;
-        inject = false;
-        n_cas++;
-        const unsigned long long seen = atomicCAS(wp, expect, want);
-        if (seen == expect) break;
-        n_fail++;
-        cur = seen;
-      }
+      cas_join(states, idx, m, force_retry != 0, n_cas, n_fail);
     }
   }
   atomicAdd(&s_sum[0], (unsigned long long)n_occ);

@@ -8,14 +8,14 @@ namespace cf1g {
 // ------------------------------------------------------------------ key warp
 // Before the linear models a key is mapped to u in [0, 2^64) by a learned
 // order-0 model of its bytes (four bases each): interval (lo, w) starts at
-// (0, 2^64 - 1); group g (bits [K2 - 8(g+1), K2 - 8g), up to 4 groups) with
+// (0, 2^64 - 1); group g (bits [K2 - 8(g+1), K2 - 8g), up to 3 groups) with
 // symbol s moves lo by f(w, cum_g[s]) and sets w = f(w, cum_g[s+1]) -
 // f(w, cum_g[s]), f(w, c) = floor(w c / 2^32), cum_g[256] = 2^32; the bits
 // below the groups add mulhi(w, rest << (64 - rest_bits)). The intervals
 // nest, so u is non-decreasing in the key, and with cum_g from the keys' own
 // byte counts (+1 each) u is close to uniform: the leaves then see the
 // keys' random spacing rather than the genome's base composition.
-constexpr int WARP_MAX_GROUPS = 4;
+constexpr int WARP_MAX_GROUPS = 3;
 
 // ------------------------------------------------------------------ model records
 // Root piece r (warped keys with u >> s == r): leaf = leaf0 + clamp(floor(fma(A,
@@ -28,7 +28,8 @@ struct RootPiece {
   uint32_t p;
 };
 // Leaf: predicted rank = p_r + floor(fma(|S|, x, I) + 0.5) in the piece's
-// coordinates (x = u - (r << s)). The sign bit of S marks a leaf whose recorded error window
+// coordinates (x = u - (r << s) in FP32 from its top 32 bits, the local rank
+// clamped to +-2^30). The sign bit of S marks a leaf whose recorded error window
 // exceeds the threshold: its keys resolve through the BBHash.
 struct LeafRec {
   float S, I;

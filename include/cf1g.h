@@ -300,13 +300,13 @@ int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, 
  * the globally sorted distinct k-mers — a monotone minimal perfect map).
  * u64 keys only (k <= 31). Trained on the GPU after P3. */
 typedef struct cf1g_lidx_params {
-  uint32_t leaf_bits;  /* log2 L; 0 = 18 (capped at floor(log2 D)); L + R leaves in all */
+  uint32_t leaf_bits;  /* log2 L; 0 = 17 (capped at floor(log2 D)); L + R leaves in all */
   uint32_t root_bits;  /* log2 root pieces R; 0 = 10 (at most 20; up to 10 the lookup holds them in LDS) */
   uint32_t epochs;     /* gradient-descent epochs per level; 0 = 4 */
   float lr;            /* step size on standardised inputs; 0 = 0.5 */
   uint32_t max_window; /* leaves with a wider error window resolve through the BBHash; 0 = 4096 */
   uint32_t untrained;  /* 1: keep the endpoint interpolation (the method without training) */
-  uint32_t line_bytes; /* 64 or 128; 0 = 128 */
+  uint32_t line_bytes; /* 64 or 128; 0 = 64 */
   uint32_t line_keys;  /* keys per line C; 0 = 256 (128-byte lines) or 112 (64-byte lines) */
 } cf1g_lidx_params;
 
@@ -355,10 +355,13 @@ int cf1g_lidx_export(cf1g_ctx* ctx, cf1g_lidx_shape* shape, uint64_t* sorted_key
  * standardisation (mu, 1/deviation) the kernels used. Outputs may be NULL. */
 int cf1g_lidx_gradients(cf1g_ctx* ctx, int level, const float* w, const float* c, uint64_t n_segs, double* out_gw, double* out_gc,
                         double* out_sq, uint64_t* out_seg, float* out_mu, float* out_isd);
-/* Classification through an index (0 = BBHash, 1 = learned): every distinct
- * k-mer is looked up and its joined state stored at that slot of a state
- * vector in the index's order. */
-int cf1g_classify_via(cf1g_ctx* ctx, int index);
+/* P4 in the reference's form through an index (0 = BBHash, 1 = learned):
+ * one lookup per k-mer occurrence of the records, in sequence order, and a
+ * CAS join of its observation into a state vector in the index's order
+ * (classify_vertices, src/classify.rs:524-601, with Mphf::hash replaced by
+ * the index). The BBHash form is cf1g_p4_classify_cas's; out (may be NULL)
+ * as there. */
+int cf1g_classify_via(cf1g_ctx* ctx, int index, cf1g_cas_stats* out);
 /* Which index cf1g_annotate, cf1g_states_export and cf1g_p5_write use
  * (0 = BBHash, the default; 1 = learned, after cf1g_classify_via(ctx, 1)). */
 int cf1g_select_index(cf1g_ctx* ctx, int index);

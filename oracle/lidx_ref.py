@@ -52,7 +52,7 @@ def fma32(a, b, c):
 
 # ---------------------------------------------------------------- key warp
 M32 = np.uint64(0xFFFFFFFF)
-WARP_MAX_GROUPS = 4
+WARP_MAX_GROUPS = 3
 
 
 def warp_groups(key_bits):
@@ -234,13 +234,20 @@ ROOT_DTYPE = np.dtype([("A", "<f4"), ("B", "<f4"), ("leaf0", "<u4"), ("p", "<u4"
 LEAF_DTYPE = np.dtype([("S", "<f4"), ("I", "<f4")])
 
 
+def piece_x(keys, s):
+    """x = key - (r << s) in FP32 from its top 32 bits (s > 32):
+    (float)(u32)(key >> (s - 32)) * 2^(s - 32)."""
+    top = ((keys >> np.uint64(s - 32)) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    return (top.astype(np.float32) * np.float32(2.0 ** (s - 32))).astype(np.float32)
+
+
 def route(keys, root, s):
     """Leaf of every key: leaf0_r + clamp(floor(fma(A_r, x, B_r)), 0, n_r - 1)
-    with x = key - (r << s) and n_r = leaf0_{r+1} - leaf0_r."""
+    with x = piece_x(key) and n_r = leaf0_{r+1} - leaf0_r."""
     r = (keys >> np.uint64(s)).astype(np.int64)
     P = root[r]
     n = root["leaf0"][r + 1].astype(np.int64) - P["leaf0"].astype(np.int64)
-    x = (keys - (r.astype(np.uint64) << np.uint64(s))).astype(np.float32)
+    x = piece_x(keys, s)
     f = np.floor(fma32(P["A"], x, P["B"])).astype(np.float64)
     cl = np.clip(f, 0, np.maximum(n - 1, 0)).astype(np.int64)
     return P["leaf0"].astype(np.int64) + cl
@@ -252,13 +259,13 @@ def is_fallback(leaf):
 
 
 def leaf_pred(keys, root, leaf, s, j):
-    """p_r + floor(fma(|S_j|, (float)(key - (r << s)), I_j) + 0.5), clamped
-    to +-1e12: the rank the leaf predicts (piece coordinates)."""
+    """p_r + floor(fma(|S_j|, piece_x(key), I_j) + 0.5), the local part
+    clamped to +-2^30: the rank the leaf predicts (piece coordinates)."""
     r = (keys >> np.uint64(s)).astype(np.int64)
-    x = (keys - (r.astype(np.uint64) << np.uint64(s))).astype(np.float32)
+    x = piece_x(keys, s)
     S = np.abs(leaf["S"][j]).astype(np.float32)
     f = (fma32(S, x, leaf["I"][j]) + np.float32(0.5)).astype(np.float32)
-    f = np.clip(np.floor(f), np.float32(-1.0e12), np.float32(1.0e12))
+    f = np.clip(np.floor(f), np.float32(-1073741824.0), np.float32(1073741824.0))
     return root["p"][r].astype(np.int64) + f.astype(np.int64)
 
 
@@ -302,7 +309,7 @@ def error_bounds(keys, root, leaf, s):
 # 3-bit codes when they fit; else it is an overflow line.
 HDR_BITS = 80
 def default_line_keys(line_bytes):
-    return {64: 104, 128: 256}[line_bytes]
+    return {64: 112, 128: 256}[line_bytes]
 
 
 def n_lines(D, C):

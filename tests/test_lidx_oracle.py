@@ -206,15 +206,31 @@ def test_warp_of_uniform_counts_is_the_scaled_key():
     K2 = 62
     i = np.arange(1 << 16, dtype=np.uint64)
     # every group's byte takes each value 256 times (odd multipliers permute bytes)
-    g0, g1, g2, g3 = i & np.uint64(255), i >> np.uint64(8), (i * np.uint64(37)) & np.uint64(255), (i * np.uint64(101)) & np.uint64(255)
-    rest = (i * np.uint64(2654435761)) & np.uint64((1 << 30) - 1)
-    keys = np.unique((g0 << np.uint64(54)) | (g1 << np.uint64(46)) | (g2 << np.uint64(38)) | (g3 << np.uint64(30)) | rest)
+    g0, g1, g2 = i & np.uint64(255), i >> np.uint64(8), (i * np.uint64(37)) & np.uint64(255)
+    rest = (i * np.uint64(2654435761)) & np.uint64((1 << 38) - 1)
+    keys = np.unique((g0 << np.uint64(54)) | (g1 << np.uint64(46)) | (g2 << np.uint64(38)) | rest)
     assert len(keys) == 1 << 16
     cum = LR.warp_tables(keys, K2)
-    np.testing.assert_array_equal(cum, np.tile(np.arange(256, dtype=np.uint32) << np.uint32(24), (4, 1)))
+    assert cum.shape == (LR.WARP_MAX_GROUPS, 256) == (3, 256)
+    np.testing.assert_array_equal(cum, np.tile(np.arange(256, dtype=np.uint32) << np.uint32(24), (3, 1)))
     u = LR.warp(keys, cum, K2).astype(object)
     want = [int(x) << 2 for x in keys]
     assert max(abs(a - b) for a, b in zip(u, want)) < (1 << 34)
+
+
+def test_piece_x_keeps_the_top_32_bits():
+    """piece_x equals the exact FP32 conversion of the top 32 bits of x, and
+    differs from the exact conversion of x by less than 2^(s-32) + an ulp."""
+    rng = np.random.default_rng(22)
+    s = 54
+    u = rng.integers(0, 1 << 63, 50000, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 50000, dtype=np.uint64)
+    x = LR.piece_x(u, s).astype(np.float64)
+    low = (u & np.uint64((1 << s) - 1)).astype(object)
+    for a, b in zip(x[:2000], low[:2000]):
+        top = b >> (s - 32)
+        assert a == float(np.float32(top)) * 2.0 ** (s - 32)
+        assert abs(a - b) <= 2.0 ** (s - 32) + abs(a) * 2.0 ** -24
+    assert np.all(LR.piece_x(np.array([0, 1 << s, (1 << s) + 5], np.uint64), s) == 0)
 
 
 def test_warp_flattens_skewed_keys():
