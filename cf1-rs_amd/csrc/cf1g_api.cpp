@@ -995,7 +995,7 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   if (c->wide) fail(CF1G_E_INVALID, "the learned index takes u64 keys (k <= 31)");
   cf1g_lidx_params p{};
   if (prm) p = *prm;
-  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 17, root_bits = p.root_bits ? p.root_bits : 10;
+  const uint32_t leaf_bits = p.leaf_bits ? p.leaf_bits : 17, root_bits = p.root_bits ? p.root_bits : 8;
   const uint32_t epochs = p.epochs ? p.epochs : 4, max_window = p.max_window ? p.max_window : 4096;
   const float lr = p.lr > 0 ? p.lr : 0.5f;
   const uint32_t line_bytes = p.line_bytes ? p.line_bytes : 64;

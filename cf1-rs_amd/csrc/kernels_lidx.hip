@@ -609,8 +609,12 @@ struct LineResult {
 // The run of key q in the line whose header admitted it (k0 <= q, q's cell
 // below the line's cell count). Register chains pick words by index: a lane
 // holds its own line, so no word index is uniform across the wave.
-template <int NW>
-__device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint64_t q, bool has_sent, uint32_t C) {
+// fetch(w) gives word w of the line for a word index that differs from lane
+// to lane (0 past the line): from registers by a select chain, or from the
+// lane's slot in LDS by one read. The prefix counts over all words come from
+// the register copy W.
+template <int NW, typename Fetch>
+__device__ __forceinline__ LineResult line_decode_t(const uint64_t (&W)[NW], uint64_t q, bool has_sent, uint32_t C, Fetch fetch) {
   LineResult R{LS_ABSENT, 0, 0, false};
   const uint64_t k0 = W[0] & ((1ULL << 62) - 1);
   if ((W[0] >> 62) & 1) {
@@ -627,39 +631,32 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
   // Zero #(cq - 1) closes the cell before q's: the run starts right after it
   // (or at bit 80 for cq = 0). Zeros are counted from bit 80 on (the header
   // bits of word 1 read as ones); words past the unary hold codes, but the
-  // zero sought lies inside the unary, so their counts never decide.
+  // zero sought lies inside the unary, so their counts never decide. A 1
+  // right after a 1 is a key in its predecessor's cell (one code each):
+  // their counts before each word give the codes before the run.
   uint32_t start = LINE_HDR_BITS;
-  if (cq) {
-    const uint32_t k = cq - 1;
-    uint32_t cz = 0, base = 0, nbefore = 0;
-    uint64_t zsel = 0;
-#pragma unroll
-    for (int w = 1; w < NW; w++) {
-      const uint64_t Z = ~(w == 1 ? (W[1] | 0xFFFFULL) : W[w]);
-      const bool in = cz <= k;  // zero #k lies in this word or a later one
-      zsel = in ? Z : zsel;
-      base = in ? cz : base;
-      nbefore += in ? 1u : 0u;
-      cz += __popcll(Z);
-    }
-    start = 64u * nbefore + select64(zsel, k - base) + 1u;
-  }
-  // the run of ones from `start`, and the collisions before it (a 1 right
-  // after a 1 is a key in its predecessor's cell: one code each)
-  const uint32_t ws = start >> 6, os = start & 63u;
-  uint64_t w0 = 0, w1 = 0, csel = 0;
-  uint32_t ccum = 0, cbase = 0;
+  uint32_t cz = 0, zbase = 0, wa = 0, ccum = 0;
+  uint32_t ccs[NW];
+  const uint32_t k = cq - 1;
 #pragma unroll
   for (int w = 1; w < NW; w++) {
+    const uint64_t Z = ~(w == 1 ? (W[1] | 0xFFFFULL) : W[w]);
+    const bool in = cq && cz <= k;  // zero #k lies in this word or a later one
+    zbase = in ? cz : zbase;
+    wa += in ? 1u : 0u;
+    cz += __popcll(Z);
     uint64_t Cw = W[w] & ((W[w] << 1) | (W[w - 1] >> 63));
     if (w == 1) Cw &= ~0x1FFFFULL;  // bits 64..80: header, and the first key
-    const bool at = (uint32_t)w == ws;
-    w0 = at ? W[w] : w0;
-    csel = at ? Cw : csel;
-    cbase = at ? ccum : cbase;
-    w1 = ((uint32_t)w == ws + 1) ? W[w] : w1;
+    ccs[w] = ccum;
     ccum += __popcll(Cw);
   }
+  if (cq) {
+    const uint64_t zsel = ~(wa == 1 ? (fetch(1) | 0xFFFFULL) : fetch(wa));
+    start = 64u * wa + select64(zsel, k - zbase) + 1u;
+  }
+  // the run of ones from `start`
+  const uint32_t ws = start >> 6, os = start & 63u;
+  const uint64_t w0 = fetch(ws), w1 = fetch(ws + 1), wp = fetch(ws - 1);
   const uint64_t win = os ? ((w0 >> os) | (w1 << (64 - os))) : w0;
   const uint32_t s = (uint32_t)__ffsll((long long)~win) - 1u;  // ones from `start` (64: a longer run)
   const uint32_t j0 = start - LINE_HDR_BITS - cq;
@@ -679,15 +676,15 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
     R.a = j0;
     return R;
   }
+  uint64_t csel = w0 & ((w0 << 1) | (wp >> 63));
+  if (ws == 1) csel &= ~0x1FFFFULL;
+  uint32_t cbase = 0;
+#pragma unroll
+  for (int w = 1; w < NW; w++) cbase = ((uint32_t)w == ws) ? ccs[w] : cbase;
   const uint32_t m0 = cbase + __popcll(csel & ((1ULL << os) - 1));
   const uint32_t cpos = LINE_HDR_BITS + ulen + cw * m0;
   const uint32_t cwd = cpos >> 6, co = cpos & 63u;
-  uint64_t c0 = 0, c1 = 0;
-#pragma unroll
-  for (int w = 1; w < NW; w++) {
-    c0 = ((uint32_t)w == cwd) ? W[w] : c0;
-    c1 = ((uint32_t)w == cwd + 1) ? W[w] : c1;
-  }
+  const uint64_t c0 = fetch(cwd), c1 = fetch(cwd + 1);
   const uint64_t codes = co ? ((c0 >> co) | (c1 << (64 - co))) : c0;
   // a search code (all ones) anywhere in the run: search it
   uint64_t sev = cw == 3u ? (codes & (codes >> 1) & (codes >> 2) & 0x1249249249249249ULL)
@@ -716,6 +713,22 @@ __device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint6
   R.kind = LS_FOUND;
   R.a = j0 + lo;
   return R;
+}
+template <int NW>
+__device__ __forceinline__ LineResult line_decode(const uint64_t (&W)[NW], uint64_t q, bool has_sent, uint32_t C) {
+  return line_decode_t<NW>(W, q, has_sent, C, [&](uint32_t w) {
+    uint64_t v = 0;
+#pragma unroll
+    for (int x = 1; x < NW; x++) v = ((uint32_t)x == w) ? W[x] : v;
+    return v;
+  });
+}
+// The same from the lane's LDS slot (the line's words written there first).
+template <int NW>
+__device__ __forceinline__ LineResult line_decode_lds(const uint64_t (&W)[NW], uint64_t* slot, uint64_t q, bool has_sent, uint32_t C) {
+#pragma unroll
+  for (int w = 0; w < NW; w++) slot[w] = W[w];
+  return line_decode_t<NW>(W, q, has_sent, C, [&](uint32_t w) { return w < (uint32_t)NW ? slot[w] : 0ULL; });
 }
 
 template <int NW>
@@ -879,24 +892,27 @@ __device__ __forceinline__ uint64_t lidx_finish(const LidxView& V, uint64_t key,
   }
 }
 
-// lidx_finish for two keys in lockstep: the header walk of both, each
+// lidx_finish for NK keys in lockstep: the header walk of both, each
 // round's neighbour loads issued together; both runs decoded; both searches'
 // loads issued together. A search that sends a key on to the next line (rare)
 // finishes one key at a time.
-template <int NW>
-__device__ __forceinline__ void lidx_finish2(const LidxView& V, const uint64_t (&key)[2], const bool (&ok)[2], uint32_t (&i)[2],
-                                             uint64_t (&W)[2][NW], uint64_t (&res)[2]) {
-  bool open[2];  // still walking
+template <int NW, int NK>
+__device__ __forceinline__ void lidx_finish_n(const LidxView& V, const uint64_t (&key)[NK], const bool (&ok)[NK], uint32_t (&i)[NK],
+                                              uint64_t (&W)[NK][NW], uint64_t* slot, uint64_t (&res)[NK]) {
+  bool open[NK];  // still walking
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NK; k++) {
     res[k] = V.D;
     open[k] = ok[k];
   }
   uint32_t steps = 1;
   while (true) {
-    bool moved[2] = {false, false};
+    bool moved[NK];
+    bool any = false;
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < NK; k++) moved[k] = false;
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
       if (!open[k]) continue;
       const uint64_t k0 = W[k][0] & ((1ULL << 62) - 1);
       if (key[k] < k0) {
@@ -910,29 +926,31 @@ __device__ __forceinline__ void lidx_finish2(const LidxView& V, const uint64_t (
         }
       }
     }
-    if (!moved[0] && !moved[1]) break;
+#pragma unroll
+    for (int k = 0; k < NK; k++) any |= moved[k];
+    if (!any) break;
     if (steps >= V.max_steps) {
 #pragma unroll
-      for (int k = 0; k < 2; k++) open[k] = open[k] && !moved[k];
+      for (int k = 0; k < NK; k++) open[k] = open[k] && !moved[k];
       break;
     }
 #pragma unroll
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < NK; k++)
       if (moved[k]) load_line<NW>(V.lines, i[k], W[k]);
 #pragma unroll
-    for (int k = 0; k < 2; k++) hold_line<NW>(W[k]);
+    for (int k = 0; k < NK; k++) hold_line<NW>(W[k]);
     steps++;
   }
-  LineResult r[2];
-  uint64_t a[2], b[2];
-  bool srch[2], small[2];
+  LineResult r[NK];
+  uint64_t a[NK], b[NK];
+  bool srch[NK], small[NK];
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NK; k++) {
     srch[k] = false;
     small[k] = false;
     a[k] = b[k] = 0;
     if (!open[k]) continue;
-    r[k] = line_decode<NW>(W[k], key[k], i[k] + 1 < V.n_lines, V.C);
+    r[k] = line_decode_lds<NW>(W[k], slot, key[k], i[k] + 1 < V.n_lines, V.C);
     const uint64_t base = (uint64_t)i[k] * V.C;
     if (r[k].kind == LS_FOUND) res[k] = base + r[k].a;
     if (r[k].kind == LS_SEARCH) {
@@ -943,13 +961,13 @@ __device__ __forceinline__ void lidx_finish2(const LidxView& V, const uint64_t (
     }
   }
   // searches of up to 4 keys: every key's loads in one round
-  uint64_t kk[2][4];
+  uint64_t kk[NK][4];
 #pragma unroll
-  for (int k = 0; k < 2; k++)
+  for (int k = 0; k < NK; k++)
 #pragma unroll
     for (int x = 0; x < 4; x++) kk[k][x] = (srch[k] && small[k] && a[k] + x <= b[k]) ? V.keys[a[k] + x] : ~0ULL;
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NK; k++) {
     if (!srch[k]) continue;
     uint64_t lo = a[k];
     bool found = false;
@@ -1015,12 +1033,14 @@ __device__ __forceinline__ uint64_t lidx_rank(const LidxView& V, const MphfView&
 // current ones are resolved.
 // Compiled for 6 waves per SIMD (80 VGPRs without spills for 64-byte lines):
 // the kernel waits on dependent far loads, so waves in flight set its rate.
-constexpr uint32_t LIDX_LDS_ROOT_MAX = 1024;  // root pieces held in LDS (16 KiB)
+constexpr uint32_t LIDX_LDS_ROOT_MAX = 256;  // root pieces held in LDS (4 KiB)
+constexpr int LIDX_NK = 2;                    // keys a lane carries through one round
 template <int NW, bool FB, bool ROOT_LDS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
                                                      uint64_t* __restrict__ out) {
   __shared__ uint32_t wt[WARP_MAX_GROUPS * 256];
   __shared__ RootPiece rt[ROOT_LDS ? LIDX_LDS_ROOT_MAX + 1 : 1];
+  __shared__ uint64_t lslot[FB ? 1 : 256 * (NW + 1)];
   for (int j = threadIdx.x; j < V.warp_groups * 256; j += 256) wt[j] = V.warp[j];
   if (ROOT_LDS)
     for (uint32_t j = threadIdx.x; j <= V.R; j += 256) rt[j] = V.root[j];
@@ -1035,32 +1055,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))
     }
     return;
   }
-  uint64_t next[2] = {0, 0};
-  if (i < n) {
-    next[0] = keys[i];
-    next[1] = keys[i + stride < n ? i + stride : i];
-  }
-  for (; i < n; i += 2 * stride) {
-    const uint64_t j = i + stride;
-    const bool has_b = j < n;
-    const uint64_t key[2] = {next[0], next[1]};
-    const uint64_t i2 = i + 2 * stride;
+  uint64_t next[LIDX_NK];
+#pragma unroll
+  for (int k = 0; k < LIDX_NK; k++) next[k] = i < n ? keys[i + k * stride < n ? i + k * stride : i] : 0;
+  for (; i < n; i += LIDX_NK * stride) {
+    uint64_t key[LIDX_NK];
+#pragma unroll
+    for (int k = 0; k < LIDX_NK; k++) key[k] = next[k];
+    const uint64_t i2 = i + LIDX_NK * stride;
     if (i2 < n) {
-      next[0] = keys[i2];
-      next[1] = keys[i2 + stride < n ? i2 + stride : i2];
+#pragma unroll
+      for (int k = 0; k < LIDX_NK; k++) next[k] = keys[i2 + k * stride < n ? i2 + k * stride : i2];
     }
-    bool ok[2];
-    uint32_t line[2];
-    lidx_pred_lines<2>(V, wt, root, key, ok, line);
-    uint64_t W[2][NW];
-    load_line<NW>(V.lines, line[0], W[0]);
-    load_line<NW>(V.lines, line[1], W[1]);
-    hold_line<NW>(W[0]);
-    hold_line<NW>(W[1]);
-    uint64_t res[2];
-    lidx_finish2<NW>(V, key, ok, line, W, res);
-    out[i] = res[0] < V.D ? res[0] : 0;
-    if (has_b) out[j] = res[1] < V.D ? res[1] : 0;
+    bool ok[LIDX_NK];
+    uint32_t line[LIDX_NK];
+    lidx_pred_lines<LIDX_NK>(V, wt, root, key, ok, line);
+    uint64_t W[LIDX_NK][NW];
+#pragma unroll
+    for (int k = 0; k < LIDX_NK; k++) load_line<NW>(V.lines, line[k], W[k]);
+#pragma unroll
+    for (int k = 0; k < LIDX_NK; k++) hold_line<NW>(W[k]);
+    uint64_t res[LIDX_NK];
+    lidx_finish_n<NW, LIDX_NK>(V, key, ok, line, W, lslot + threadIdx.x * (NW + 1), res);
+#pragma unroll
+    for (int k = 0; k < LIDX_NK; k++)
+      if (i + k * stride < n) out[i + k * stride] = res[k] < V.D ? res[k] : 0;
   }
 }
 
@@ -1234,7 +1253,7 @@ void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t LT, con
 }
 void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const LidxView& V, const MphfView& mv, uint64_t* out) {
   if (!n) return;
-  uint64_t g = (n + 511) / 512;  // two keys per lane a round
+  uint64_t g = (n + 256 * LIDX_NK - 1) / (256 * LIDX_NK);  // LIDX_NK keys per lane a round
   if (g > 8192) g = 8192;         // each work-group fills its LDS tables once
   const bool rl = V.R <= LIDX_LDS_ROOT_MAX;
 #define CF1G_LIDX_LOOKUP(NW_, FB_, RL_) \

@@ -301,7 +301,7 @@ int cf1g_p5_write(cf1g_ctx* ctx, const uint8_t* ascii, const uint64_t* rec_off, 
  * u64 keys only (k <= 31). Trained on the GPU after P3. */
 typedef struct cf1g_lidx_params {
   uint32_t leaf_bits;  /* log2 L; 0 = 17 (capped at floor(log2 D)); L + R leaves in all */
-  uint32_t root_bits;  /* log2 root pieces R; 0 = 10 (at most 20; up to 10 the lookup holds them in LDS) */
+  uint32_t root_bits;  /* log2 root pieces R; 0 = 8 (at most 20; up to 8 the lookup holds them in LDS) */
   uint32_t epochs;     /* gradient-descent epochs per level; 0 = 4 */
   float lr;            /* step size on standardised inputs; 0 = 0.5 */
   uint32_t max_window; /* leaves with a wider error window resolve through the BBHash; 0 = 4096 */
