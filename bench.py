@@ -624,7 +624,10 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
             assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
             errs[name] += int(np.count_nonzero(out != (idx if name == "learned" else vall[idx])))
             print(f"[bench] {name} lookup batch {bi + 1}/{n_batches} done", file=sys.stderr, flush=True)
-        kt[kname] = g.kernel_times(reset=True).get(kname, (0, 0.0))
+        kts = g.kernel_times(reset=True)
+        # the learned index's call is its lookup kernel and the kernel that
+        # resolves the searches the lookup deferred
+        kt[kname] = tuple(sum(kts.get(kn, (0, 0.0))[j] for kn in KERNELS_OF[kname]) for j in (0, 1))
         g.set_profiling(False)
     # P4 in the reference's form through each index: one lookup per k-mer
     # occurrence in sequence order plus the CAS join
@@ -653,7 +656,7 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
         k20 = g.kernel_times(reset=True)
         g.set_profiling(False)
         assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), dout, C.c_size_t(batch * 8), 2) == 0
-        ms20 = k20.get("lidx_lookup", (0, 0.0))[1]
+        ms20 = sum(k20.get(kn, (0, 0.0))[1] for kn in KERNELS_OF["lidx_lookup"])
         shape20 = {"n_leaves": info20["n_leaves"], "lookups": batch, "lookups_per_s": batch / (ms20 * 1e-3) if ms20 else None,
                    "rank_errors": int(np.count_nonzero(out != idx)), "mean_abs_err": round(info20["mean_abs_err"], 3),
                    "max_err": [info20["max_err_lo"], info20["max_err_hi"]]}
@@ -675,6 +678,10 @@ def lookup_bench(g, D, batch, n_batches, far=None, seed=5, lidx_args=None):
         "train_wall_s": round(train_wall, 3),
         "roofline": lookup_roofline(far, {"bbh_lookup": brate, "lidx_lookup": lrate}),
     }
+
+
+# kernels each index's lookup call launches
+KERNELS_OF = {"lidx_lookup": ("lidx_lookup", "lidx_search"), "bbh_lookup": ("bbh_lookup",)}
 
 
 def profiled_far_requests(workload, k, bins):

@@ -90,7 +90,8 @@ void launch_lidx_lines(hipStream_t, int, const uint64_t*, uint64_t, uint32_t, ui
 void launch_lidx_warp_hist(hipStream_t, const uint64_t*, uint64_t, int, int, uint32_t*);
 void launch_lidx_warp_keys(hipStream_t, const uint64_t*, uint64_t, const uint32_t*, int, int, uint64_t*);
 void launch_lidx_bbh2rank(hipStream_t, const uint64_t*, uint32_t, const uint64_t*, const LeafRec*, const MphfView&, uint32_t*);
-void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*);
+void launch_lidx_lookup(hipStream_t, const uint64_t*, uint64_t, const LidxView&, const MphfView&, uint64_t*, LidxQueue);
+void lidx_queue_shape(uint64_t, uint32_t&, uint32_t&);
 void launch_p4_cas_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&, uint64_t*,
                         uint64_t*, uint32_t, uint32_t, int);
 void launch_annotate_lidx(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, int, const LidxView&, const MphfView&,
@@ -292,6 +293,19 @@ struct cf1g_ctx {
     v.warp_groups = lidx_groups;
     v.fallback = lidx_fallback;
     return v;
+  }
+  // Deferred-search queue of a lookup of n keys through the learned index
+  // (none when leaves resolve through the BBHash).
+  LidxQueue lqueue(uint64_t n) {
+    LidxQueue q{nullptr, nullptr, 0};
+    if (lidx_fallback || n == 0) return q;
+    uint32_t waves = 0, cap = 0;
+    lidx_queue_shape(n, waves, cap);
+    if (!cap) return q;
+    q.ent = getT<uint64_t>("lidx_queue", (uint64_t)waves * cap * 3);
+    q.cnt = getT<uint32_t>("lidx_queue_cnt", waves);
+    q.cap = cap;
+    return q;
   }
   void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
   void zero(void* p, size_t bytes) { launch_zero(st, p, bytes); }
@@ -1001,7 +1015,7 @@ void do_lidx_train(cf1g_ctx* c, const cf1g_lidx_params* prm, cf1g_lidx_info* inf
   const uint32_t line_bytes = p.line_bytes ? p.line_bytes : 64;
   if (leaf_bits > 26 || root_bits > 20) fail(CF1G_E_INVALID, "leaf_bits <= 26 and root_bits <= 20");
   if (line_bytes != 64 && line_bytes != 128) fail(CF1G_E_INVALID, "line_bytes is 64 or 128");
-  const uint32_t C = p.line_keys ? p.line_keys : (line_bytes == 128 ? 256u : 112u);
+  const uint32_t C = p.line_keys ? p.line_keys : (line_bytes == 128 ? 256u : 108u);
   if (C < 2 || C > 4096) fail(CF1G_E_INVALID, "line_keys is in [2, 4096]");
   if (c->D >= (1ULL << 32)) fail(CF1G_E_INVALID, "the learned index holds fewer than 2^32 keys");
   hipStream_t st = c->st;
@@ -1228,7 +1242,7 @@ uint64_t build_spans(cf1g_ctx* c) {
   unsigned int* err = c->getT<unsigned int>("span_err", 2);
   launch_span_emit(st, ann, c->n_pos, os, oe, S, E);
   launch_span_keys(st, c->wide, c->getT<uint64_t>("seq", 2 * c->n_words64 + 8), c->k, S, E, U, keys, fl);
-  if (c->use_index == 1) launch_lidx_lookup(st, (const uint64_t*)keys, U, c->lview(), c->view(), ids);
+  if (c->use_index == 1) launch_lidx_lookup(st, (const uint64_t*)keys, U, c->lview(), c->view(), ids, c->lqueue(U));
   else launch_lookup(st, c->wide, keys, U, c->view(), ids, true);
   if (nsw)
     hip_check(hipMemcpyAsync(claim, c->get(c->use_index == 1 ? "states_lidx" : "states", (nsw + 1) * 8), nsw * 8,
@@ -1936,7 +1950,7 @@ int cf1g_lidx_lookup(cf1g_ctx* c, const uint64_t* keys, uint64_t n, uint64_t* ou
     uint64_t* dk = c->getT<uint64_t>("lookup_in", n + 1);
     uint64_t* dout = c->getT<uint64_t>("lookup_out", n + 1);
     c->h2d(dk, keys, n * 8);
-    launch_lidx_lookup(c->st, dk, n, c->lview(), c->view(), dout);
+    launch_lidx_lookup(c->st, dk, n, c->lview(), c->view(), dout, c->lqueue(n));
     c->d2h(out, dout, n * 8);
   });
 }
@@ -1946,7 +1960,7 @@ int cf1g_lidx_lookup_device(cf1g_ctx* c, const void* d_keys, uint64_t n, uint64_
     if (!c->lidx_ready) fail(CF1G_E_ORDER, "cf1g_lidx_lookup_device called before cf1g_lidx_train");
     if (n == 0) return;
     if (!d_keys || !d_out) fail(CF1G_E_INVALID, "NULL buffer");
-    launch_lidx_lookup(c->st, (const uint64_t*)d_keys, n, c->lview(), c->view(), d_out);
+    launch_lidx_lookup(c->st, (const uint64_t*)d_keys, n, c->lview(), c->view(), d_out, c->lqueue(n));
     c->sync();
   });
 }

@@ -479,6 +479,14 @@ __device__ __forceinline__ int bitlen64(uint64_t x) { return x ? 64 - __clzll(x)
 // conflicts) and stored as line_words u64 words. stats[2i] = bits used
 // (0xFFFFFFFF for an overflow line), stats[2i + 1] = keys of this line's runs
 // that resolve by search (a code of 7 or more than 21 codes).
+// Top LINE_SP_BITS bits of rel's offset in its cell of 2^t (t <= 61), read as
+// a t-bit number (exact, shifted up, when t < LINE_SP_BITS): of two values in
+// one cell, the one with the larger prefix is the larger.
+__device__ __forceinline__ uint32_t sentinel_prefix(uint64_t rel, uint32_t t) {
+  const uint64_t off = rel & ((1ULL << t) - 1);
+  return t >= (uint32_t)LINE_SP_BITS ? (uint32_t)(off >> (t - LINE_SP_BITS)) : (uint32_t)(off << (LINE_SP_BITS - t));
+}
+
 template <int NW>
 __global__ void __launch_bounds__(256) k_lidx_lines(const uint64_t* __restrict__ keys, uint64_t D, uint32_t C, uint64_t n_lines,
                                                     uint64_t* __restrict__ lines, uint32_t* __restrict__ stats) {
@@ -549,6 +557,7 @@ __global__ void __launch_bounds__(256) k_lidx_lines(const uint64_t* __restrict__
     const uint32_t cells = (uint32_t)(span >> t) + 1;
     const uint32_t ulen = m + cells;
     b[2] |= (uint32_t)t | (ulen << 6);
+    if (m > n) b[2] |= sentinel_prefix(span, t) << 16;  // bits 80..87
     uint32_t pos = LINE_HDR_BITS, cpos = LINE_HDR_BITS + ulen, run = 1;
     uint64_t prel = 0, pcell = 0;
     bool bad = false;
@@ -599,12 +608,12 @@ __device__ __forceinline__ uint32_t select64(uint64_t x, uint32_t r) {
   if (r >= c) pos += 1;
   return pos;
 }
-enum LineStep : uint32_t { LS_FOUND, LS_ABSENT, LS_SEARCH };
+enum LineStep : uint32_t { LS_FOUND, LS_ABSENT, LS_SEARCH, LS_RIGHT };
 struct LineResult {
   uint32_t kind;
   uint32_t a, b;   // FOUND: index a; SEARCH: indices [a, b] of the line (b = ~0: to the last key)
   bool right_ok;   // SEARCH: a key above them all lies further right
-};
+};                 // RIGHT: the key lies above the sentinel (the next line)
 
 // The run of key q in the line whose header admitted it (k0 <= q, q's cell
 // below the line's cell count). Register chains pick words by index: a lane
@@ -629,7 +638,7 @@ __device__ __forceinline__ LineResult line_decode_t(const uint64_t (&W)[NW], uin
   const uint64_t rel = q - k0;
   const uint32_t cq = (uint32_t)(rel >> t);
   // Zero #(cq - 1) closes the cell before q's: the run starts right after it
-  // (or at bit 80 for cq = 0). Zeros are counted from bit 80 on (the header
+  // (or at bit 88 for cq = 0). Zeros are counted from bit 88 on (the header
   // bits of word 1 read as ones); words past the unary hold codes, but the
   // zero sought lies inside the unary, so their counts never decide. A 1
   // right after a 1 is a key in its predecessor's cell (one code each):
@@ -640,18 +649,18 @@ __device__ __forceinline__ LineResult line_decode_t(const uint64_t (&W)[NW], uin
   const uint32_t k = cq - 1;
 #pragma unroll
   for (int w = 1; w < NW; w++) {
-    const uint64_t Z = ~(w == 1 ? (W[1] | 0xFFFFULL) : W[w]);
+    const uint64_t Z = ~(w == 1 ? (W[1] | 0xFFFFFFULL) : W[w]);
     const bool in = cq && cz <= k;  // zero #k lies in this word or a later one
     zbase = in ? cz : zbase;
     wa += in ? 1u : 0u;
     cz += __popcll(Z);
     uint64_t Cw = W[w] & ((W[w] << 1) | (W[w - 1] >> 63));
-    if (w == 1) Cw &= ~0x1FFFFULL;  // bits 64..80: header, and the first key
+    if (w == 1) Cw &= ~0x1FFFFFFULL;  // bits 64..88: header, and the first key
     ccs[w] = ccum;
     ccum += __popcll(Cw);
   }
   if (cq) {
-    const uint64_t zsel = ~(wa == 1 ? (fetch(1) | 0xFFFFULL) : fetch(wa));
+    const uint64_t zsel = ~(wa == 1 ? (fetch(1) | 0xFFFFFFULL) : fetch(wa));
     start = 64u * wa + select64(zsel, k - zbase) + 1u;
   }
   // the run of ones from `start`
@@ -660,24 +669,49 @@ __device__ __forceinline__ LineResult line_decode_t(const uint64_t (&W)[NW], uin
   const uint64_t win = os ? ((w0 >> os) | (w1 << (64 - os))) : w0;
   const uint32_t s = (uint32_t)__ffsll((long long)~win) - 1u;  // ones from `start` (64: a longer run)
   const uint32_t j0 = start - LINE_HDR_BITS - cq;
-  if (s == 0) return R;  // q's cell is empty: absent
-  const uint32_t last = j0 + s - 1;
-  if (~win == 0ULL || (has_sent && last >= C) || s - 1 > 64u / cw) {
-    // a run too long for one code window, or one holding the next line's
-    // first key: search its keys (to the line's end for a long run)
+  if (~win == 0ULL) {
+    // a run longer than the window: search its keys to the line's end
     R.kind = LS_SEARCH;
     R.a = j0;
-    R.b = (~win == 0ULL) ? (has_sent ? C : 0xFFFFFFFFu) : last;
-    R.right_ok = has_sent && (~win == 0ULL || last >= C);
+    R.b = has_sent ? C : 0xFFFFFFFFu;
+    R.right_ok = has_sent;
     return R;
   }
-  if (s == 1) {
+  if (s == 0) return R;  // q's cell is empty: absent
+  uint32_t n = s;        // keys of the run below the sentinel
+  if (has_sent && j0 + s - 1 >= C) {
+    // the run holds the sentinel (the next line's first key): the prefixes
+    // of their offsets in the cell decide, a tie is searched
+    const uint32_t qp = sentinel_prefix(rel, t), sp = (uint32_t)(W[1] >> 16) & 255u;
+    if (qp > sp) {
+      R.kind = LS_RIGHT;
+      return R;
+    }
+    if (qp == sp) {
+      R.kind = LS_SEARCH;
+      R.a = j0;
+      R.b = j0 + s - 1;
+      R.right_ok = true;
+      return R;
+    }
+    n = s - 1;
+    if (n == 0) return R;  // below the sentinel, alone in its cell: absent
+  }
+  const uint32_t last = j0 + n - 1;
+  if (n - 1 > 64u / cw) {
+    // more codes than one window holds: search the run's keys
+    R.kind = LS_SEARCH;
+    R.a = j0;
+    R.b = last;
+    return R;
+  }
+  if (n == 1) {
     R.kind = LS_FOUND;
     R.a = j0;
     return R;
   }
   uint64_t csel = w0 & ((w0 << 1) | (wp >> 63));
-  if (ws == 1) csel &= ~0x1FFFFULL;
+  if (ws == 1) csel &= ~0x1FFFFFFULL;
   uint32_t cbase = 0;
 #pragma unroll
   for (int w = 1; w < NW; w++) cbase = ((uint32_t)w == ws) ? ccs[w] : cbase;
@@ -689,7 +723,7 @@ __device__ __forceinline__ LineResult line_decode_t(const uint64_t (&W)[NW], uin
   // a search code (all ones) anywhere in the run: search it
   uint64_t sev = cw == 3u ? (codes & (codes >> 1) & (codes >> 2) & 0x1249249249249249ULL)
                           : (codes & (codes >> 1) & (codes >> 2) & (codes >> 3) & 0x1111111111111111ULL);
-  const uint32_t used = cw * (s - 1);
+  const uint32_t used = cw * (n - 1);
   sev &= used >= 64u ? ~0ULL : ((1ULL << used) - 1);
   if (sev) {
     R.kind = LS_SEARCH;
@@ -698,7 +732,7 @@ __device__ __forceinline__ LineResult line_decode_t(const uint64_t (&W)[NW], uin
     return R;
   }
   // blind descent: the boundary of least code (highest differing bit) splits
-  uint32_t lo = 0, hb = s - 1;
+  uint32_t lo = 0, hb = n - 1;
   while (lo < hb) {
     uint32_t bc = 16, bp = lo + 1;
     for (uint32_t x = lo + 1; x <= hb; x++) {
@@ -859,6 +893,14 @@ __device__ __forceinline__ uint64_t lidx_finish(const LidxView& V, uint64_t key,
     const uint64_t base = (uint64_t)i * V.C;
     if (res.kind == LS_FOUND) return base + res.a;
     if (res.kind == LS_ABSENT) return V.D;
+    if (res.kind == LS_RIGHT) {  // above the sentinel: on to the next line
+      if (steps >= V.max_steps || i + 1 >= V.n_lines) return V.D;
+      i++;
+      load_line<NW>(V.lines, i, W);
+      hold_line<NW>(W);
+      steps++;
+      continue;
+    }
     // search ranks [base + a, base + b] of the sorted keys: up to 4 keys
     // in one round of loads, a binary search beyond
     const uint64_t a = base + res.a;
@@ -892,13 +934,57 @@ __device__ __forceinline__ uint64_t lidx_finish(const LidxView& V, uint64_t key,
   }
 }
 
+// Run code of a deferred search: a (bits 0-12) and b (bits 13-25, LQ_TO_END:
+// to the last key) as indices of the line (C <= 4096), bit 26: a key above
+// every key searched goes on to the next line.
+constexpr uint32_t LQ_TO_END = 8191;
+
+// Search of ranks [a, b] of the sorted keys for key (up to 4 keys in one
+// round of loads, a binary search beyond); when the key lies above them all
+// and cont is set, the walk goes on from line i + 1 (a = b + 1: no search, the
+// key lies above line i's sentinel). Rank, or D when absent.
+template <int NW>
+__device__ __forceinline__ uint64_t lidx_search(const LidxView& V, uint64_t key, uint32_t i, uint64_t a, uint64_t b, bool cont) {
+  uint64_t lo = a;
+  if (a > b) {
+    // nothing to search
+  } else if (b - a < 4) {
+    uint64_t kk[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) kk[x] = a + x <= b ? V.keys[a + x] : ~0ULL;
+    uint32_t below = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      if (a + x <= b && kk[x] == key) return a + x;
+      below += (a + x <= b && kk[x] < key) ? 1u : 0u;
+    }
+    lo = a + below;
+  } else {
+    uint64_t hi = b + 1;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (V.keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    if (lo <= b && V.keys[lo] == key) return lo;
+  }
+  if (!(cont && lo == b + 1 && i + 1 < V.n_lines)) return V.D;
+  uint64_t Wn[NW];
+  load_line<NW>(V.lines, i + 1, Wn);
+  hold_line<NW>(Wn);
+  return lidx_finish<NW>(V, key, i + 1, Wn);
+}
+
 // lidx_finish for NK keys in lockstep: the header walk of both, each
-// round's neighbour loads issued together; both runs decoded; both searches'
-// loads issued together. A search that sends a key on to the next line (rare)
-// finishes one key at a time.
+// round's neighbour loads issued together; both runs decoded. A key whose
+// run must be searched goes to the wave's queue segment (qn entries so far,
+// the same count in every lane; deferred[k] set) while it has room: the
+// wave then waits for no search loads. The others are searched in place,
+// every key's loads issued together; a search that sends a key on to the
+// next line (rare) finishes one key at a time.
 template <int NW, int NK>
 __device__ __forceinline__ void lidx_finish_n(const LidxView& V, const uint64_t (&key)[NK], const bool (&ok)[NK], uint32_t (&i)[NK],
-                                              uint64_t (&W)[NK][NW], uint64_t* slot, uint64_t (&res)[NK]) {
+                                              uint64_t (&W)[NK][NW], uint64_t* slot, uint64_t (&res)[NK], const LidxQueue& Q,
+                                              uint64_t* qseg, uint32_t& qn, uint64_t q0, uint64_t qstride, bool (&deferred)[NK]) {
   bool open[NK];  // still walking
 #pragma unroll
   for (int k = 0; k < NK; k++) {
@@ -943,64 +1029,52 @@ __device__ __forceinline__ void lidx_finish_n(const LidxView& V, const uint64_t 
   }
   LineResult r[NK];
   uint64_t a[NK], b[NK];
-  bool srch[NK], small[NK];
+  bool srch[NK];
 #pragma unroll
   for (int k = 0; k < NK; k++) {
     srch[k] = false;
-    small[k] = false;
+    deferred[k] = false;
     a[k] = b[k] = 0;
     if (!open[k]) continue;
     r[k] = line_decode_lds<NW>(W[k], slot, key[k], i[k] + 1 < V.n_lines, V.C);
     const uint64_t base = (uint64_t)i[k] * V.C;
     if (r[k].kind == LS_FOUND) res[k] = base + r[k].a;
+    if (r[k].kind == LS_RIGHT) {  // above the sentinel: an empty search that goes on to the next line
+      r[k].kind = LS_SEARCH;
+      r[k].a = V.C + 1;
+      r[k].b = V.C;
+      r[k].right_ok = true;
+    }
     if (r[k].kind == LS_SEARCH) {
       srch[k] = true;
       a[k] = base + r[k].a;
       b[k] = r[k].b == 0xFFFFFFFFu ? V.D - 1 : base + r[k].b;
-      small[k] = b[k] - a[k] < 4;
     }
   }
-  // searches of up to 4 keys: every key's loads in one round
-  uint64_t kk[NK][4];
+  if (Q.cap) {
+    const uint64_t below = (1ULL << __lane_id()) - 1;
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      const uint64_t m = __ballot(srch[k]);
+      if (!m) continue;
+      const uint32_t pos = qn + (uint32_t)__popcll(m & below);
+      qn = __builtin_amdgcn_readfirstlane(qn + (uint32_t)__popcll(m));
+      if (srch[k] && pos < Q.cap) {
+        const uint32_t bc = r[k].b == 0xFFFFFFFFu ? LQ_TO_END : r[k].b;
+        const uint32_t code = r[k].a | (bc << 13) | ((r[k].right_ok && steps < V.max_steps) ? 1u << 26 : 0u);
+        uint64_t* e = qseg + 3 * (uint64_t)pos;
+        e[0] = key[k];
+        e[1] = q0 + k * qstride;
+        e[2] = (uint64_t)i[k] | ((uint64_t)code << 32);
+        srch[k] = false;
+        deferred[k] = true;
+      }
+    }
+  }
+  // searches that found no room in the queue, one key at a time
 #pragma unroll
   for (int k = 0; k < NK; k++)
-#pragma unroll
-    for (int x = 0; x < 4; x++) kk[k][x] = (srch[k] && small[k] && a[k] + x <= b[k]) ? V.keys[a[k] + x] : ~0ULL;
-#pragma unroll
-  for (int k = 0; k < NK; k++) {
-    if (!srch[k]) continue;
-    uint64_t lo = a[k];
-    bool found = false;
-    if (small[k]) {
-      uint32_t below = 0;
-#pragma unroll
-      for (int x = 0; x < 4; x++) {
-        below += (a[k] + x <= b[k] && kk[k][x] < key[k]) ? 1u : 0u;
-        if (a[k] + x <= b[k] && kk[k][x] == key[k]) {
-          res[k] = a[k] + x;
-          found = true;
-        }
-      }
-      lo = a[k] + below;
-    } else {
-      uint64_t hi = b[k] + 1;
-      while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (V.keys[mid] < key[k]) lo = mid + 1; else hi = mid;
-      }
-      if (lo <= b[k] && V.keys[lo] == key[k]) {
-        res[k] = lo;
-        found = true;
-      }
-    }
-    if (!found && r[k].right_ok && lo == b[k] + 1 && steps < V.max_steps && i[k] + 1 < V.n_lines) {
-      // on to the next line, one key at a time (rare)
-      uint64_t Wn[NW];
-      load_line<NW>(V.lines, i[k] + 1, Wn);
-      hold_line<NW>(Wn);
-      res[k] = lidx_finish<NW>(V, key[k], i[k] + 1, Wn);
-    }
-  }
+    if (srch[k]) res[k] = lidx_search<NW>(V, key[k], i[k], a[k], b[k], r[k].right_ok && steps < V.max_steps);
 }
 
 // Rank of key q, or D when absent, one key at a time (FB: leaves may resolve
@@ -1037,7 +1111,7 @@ constexpr uint32_t LIDX_LDS_ROOT_MAX = 256;  // root pieces held in LDS (4 KiB)
 constexpr int LIDX_NK = 2;                    // keys a lane carries through one round
 template <int NW, bool FB, bool ROOT_LDS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_lidx_lookup(const uint64_t* __restrict__ keys, uint64_t n, const LidxView V, const MphfView mv,
-                                                     uint64_t* __restrict__ out) {
+                                                     uint64_t* __restrict__ out, const LidxQueue Q) {
   __shared__ uint32_t wt[WARP_MAX_GROUPS * 256];
   __shared__ RootPiece rt[ROOT_LDS ? LIDX_LDS_ROOT_MAX + 1 : 1];
   __shared__ uint64_t lslot[FB ? 1 : 256 * (NW + 1)];
@@ -1055,9 +1129,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))
     }
     return;
   }
+  // the wave's queue segment; lane 0 holds the smallest index of its wave,
+  // so it runs every round that any lane of the wave runs
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint64_t* qseg = Q.ent + wave * Q.cap * 3;
+  uint32_t qn = 0;
+  // queries and results stream through once: non-temporal loads and stores
   uint64_t next[LIDX_NK];
 #pragma unroll
-  for (int k = 0; k < LIDX_NK; k++) next[k] = i < n ? keys[i + k * stride < n ? i + k * stride : i] : 0;
+  for (int k = 0; k < LIDX_NK; k++) next[k] = i < n ? __builtin_nontemporal_load(&keys[i + k * stride < n ? i + k * stride : i]) : 0;
   for (; i < n; i += LIDX_NK * stride) {
     uint64_t key[LIDX_NK];
 #pragma unroll
@@ -1065,21 +1145,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))
     const uint64_t i2 = i + LIDX_NK * stride;
     if (i2 < n) {
 #pragma unroll
-      for (int k = 0; k < LIDX_NK; k++) next[k] = keys[i2 + k * stride < n ? i2 + k * stride : i2];
+      for (int k = 0; k < LIDX_NK; k++) next[k] = __builtin_nontemporal_load(&keys[i2 + k * stride < n ? i2 + k * stride : i2]);
     }
     bool ok[LIDX_NK];
     uint32_t line[LIDX_NK];
     lidx_pred_lines<LIDX_NK>(V, wt, root, key, ok, line);
+#pragma unroll
+    for (int k = 0; k < LIDX_NK; k++) ok[k] = ok[k] && i + k * stride < n;  // a slot past the end resolves nothing
     uint64_t W[LIDX_NK][NW];
 #pragma unroll
     for (int k = 0; k < LIDX_NK; k++) load_line<NW>(V.lines, line[k], W[k]);
 #pragma unroll
     for (int k = 0; k < LIDX_NK; k++) hold_line<NW>(W[k]);
     uint64_t res[LIDX_NK];
-    lidx_finish_n<NW, LIDX_NK>(V, key, ok, line, W, lslot + threadIdx.x * (NW + 1), res);
+    bool deferred[LIDX_NK];
+    lidx_finish_n<NW, LIDX_NK>(V, key, ok, line, W, lslot + threadIdx.x * (NW + 1), res, Q, qseg, qn, i, stride, deferred);
 #pragma unroll
     for (int k = 0; k < LIDX_NK; k++)
-      if (i + k * stride < n) out[i + k * stride] = res[k] < V.D ? res[k] : 0;
+      if (i + k * stride < n && !deferred[k]) __builtin_nontemporal_store(res[k] < V.D ? res[k] : 0, &out[i + k * stride]);
+  }
+  if (Q.cap && __lane_id() == 0) Q.cnt[wave] = qn < Q.cap ? qn : Q.cap;
+}
+
+// The searches the lookup deferred: one wave per wave of the lookup's grid,
+// over its queue segment.
+template <int NW>
+__global__ void __launch_bounds__(64) k_lidx_search(const uint64_t* __restrict__ keys, const LidxView V, const LidxQueue Q,
+                                                    uint64_t* __restrict__ out) {
+  const uint64_t wave = blockIdx.x;
+  const uint32_t cnt = Q.cnt[wave];
+  const uint64_t* seg = Q.ent + wave * Q.cap * 3;
+  for (uint32_t e = threadIdx.x; e < cnt; e += 64) {
+    const uint64_t key = seg[3 * e], idx = seg[3 * e + 1], lc = seg[3 * e + 2];
+    const uint32_t line = (uint32_t)lc, code = (uint32_t)(lc >> 32);
+    const uint64_t base = (uint64_t)line * V.C;
+    const uint32_t ar = code & 8191u, bc = (code >> 13) & 8191u;
+    const uint64_t r = lidx_search<NW>(V, key, line, base + ar, bc == LQ_TO_END ? V.D - 1 : base + bc, (code >> 26) & 1u);
+    out[idx] = r < V.D ? r : 0;
   }
 }
 
@@ -1251,13 +1353,34 @@ void launch_lidx_bbh2rank(hipStream_t st, const uint64_t* keys, uint32_t LT, con
                           uint32_t* bbh2rank) {
   CF1G_LAUNCH("lidx_bbh2rank", k_lidx_bbh2rank, dim3(blocks_for(LT, 4)), dim3(256), 0, st, keys, LT, q, leaf, mv, bbh2rank);
 }
-void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const LidxView& V, const MphfView& mv, uint64_t* out) {
-  if (!n) return;
+static uint64_t lidx_lookup_grid(uint64_t n) {
   uint64_t g = (n + 256 * LIDX_NK - 1) / (256 * LIDX_NK);  // LIDX_NK keys per lane a round
-  if (g > 8192) g = 8192;         // each work-group fills its LDS tables once
+  return g > 8192 ? 8192 : g;  // each work-group fills its LDS tables once
+}
+// Deferred-search queue of a lookup of n keys: a segment per wave of the
+// lookup's grid with room for 1/32 of the wave's keys (a multiple of 64, at
+// least 64; searched keys are ~1.2% on the 300 Mbp genome). Segments that
+// fill up search in place.
+void lidx_queue_shape(uint64_t n, uint32_t& waves, uint32_t& cap) {
+  waves = (uint32_t)(lidx_lookup_grid(n) * 4);
+  if (!waves) {
+    cap = 0;
+    return;
+  }
+  const uint64_t per = (n + waves - 1) / waves;
+  uint64_t c = (per / 32 + 63) / 64 * 64;
+  cap = (uint32_t)(c < 64 ? 64 : c);
+}
+// Q: from lidx_queue_shape for this n (ent: waves x cap x 3 words, cnt:
+// waves words), or cap = 0 to search in place; unused for BBHash leaves.
+void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const LidxView& V, const MphfView& mv, uint64_t* out,
+                        LidxQueue Q) {
+  if (!n) return;
+  const uint64_t g = lidx_lookup_grid(n);
   const bool rl = V.R <= LIDX_LDS_ROOT_MAX;
+  if (V.fallback) Q.cap = 0;
 #define CF1G_LIDX_LOOKUP(NW_, FB_, RL_) \
-  CF1G_LAUNCH("lidx_lookup", (k_lidx_lookup<NW_, FB_, RL_>), dim3((unsigned)g), dim3(256), 0, st, keys, n, V, mv, out)
+  CF1G_LAUNCH("lidx_lookup", (k_lidx_lookup<NW_, FB_, RL_>), dim3((unsigned)g), dim3(256), 0, st, keys, n, V, mv, out, Q)
   if (V.line_words == 8) {
     if (V.fallback) {
       if (rl) CF1G_LIDX_LOOKUP(8, true, true); else CF1G_LIDX_LOOKUP(8, true, false);
@@ -1272,6 +1395,11 @@ void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const 
     }
   }
 #undef CF1G_LIDX_LOOKUP
+  if (Q.cap) {
+    const dim3 sg((unsigned)(g * 4));
+    if (V.line_words == 8) CF1G_LAUNCH("lidx_search", k_lidx_search<8>, sg, dim3(64), 0, st, keys, V, Q, out);
+    else CF1G_LAUNCH("lidx_search", k_lidx_search<16>, sg, dim3(64), 0, st, keys, V, Q, out);
+  }
 }
 void launch_count_mismatch(hipStream_t st, const uint64_t* a, const uint64_t* b, uint64_t n, unsigned long long* bad) {
   if (!n) return;

@@ -263,13 +263,18 @@ __global__ void __launch_bounds__(256) k_span_claim(const uint64_t* __restrict__
 }
 
 // Batch Mphf::hash; FIND: a key that misses every level gets ~0 instead of
-// the reference's 0 (for callers that check the ids).
+// the reference's 0 (for callers that check the ids). Queries and results
+// stream through once: non-temporal loads (u64 keys) and stores.
 template <typename KeyT, bool FIND>
 __global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, uint64_t n, const MphfView mv,
                                                 uint64_t* __restrict__ out) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    out[i] = FIND ? mphf_find<KeyT>(mv, keys[i]) : mphf_lookup<KeyT>(mv, keys[i]);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    KeyT q;
+    if constexpr (sizeof(KeyT) == 8) q = (KeyT)__builtin_nontemporal_load((const uint64_t*)&keys[i]);
+    else q = keys[i];
+    __builtin_nontemporal_store(FIND ? mphf_find<KeyT>(mv, q) : mphf_lookup<KeyT>(mv, q), &out[i]);
+  }
 }
 
 void launch_p4_pack(hipStream_t st, const uint16_t* vmask, uint64_t D, uint64_t* states) {

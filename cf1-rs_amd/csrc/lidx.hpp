@@ -41,14 +41,20 @@ static_assert(sizeof(LeafRec) == 8, "leaf record is 8 bytes");
 //   bits 0..61 k0 (key of rank iC), bit 62 overflow (keys searched instead),
 //   bit 63 code width w (0: 3 bits, 1: 4 bits)
 //   bits 64..69 t (cell shift), bits 70..79 ulen
-//   unary [80, 80 + ulen): per cell c = 0 .. (k_last - k0) >> t, a 1 per key
+//   bits 80..87 the sentinel's prefix: the top 8 bits of its offset in its
+//   cell ((k_sent - k0) mod 2^t, shifted up when t < 8; 0 without sentinel)
+//   unary [88, 88 + ulen): per cell c = 0 .. (k_last - k0) >> t, a 1 per key
 //   with (key - k0) >> t == c, then a 0
 //   w-bit codes after the unary, one per key that shares the cell of the key
 //   before it: min(t - bitlen(rel_j ^ rel_{j-1}), 2^w - 1); the highest bit
 //   where the two differ is t - 1 - code, and 2^w - 1 means "search the
 //   run's keys". A line takes 4-bit codes when they fit and 3-bit codes would
 //   leave a code to search; a run of more than 64 / w codes is searched.
-constexpr int LINE_HDR_BITS = 80;
+// In the run that holds the sentinel the prefixes decide: a query with a
+// larger one lies above the sentinel (the next line), a smaller one below it
+// (the run without the sentinel), an equal one searches the run.
+constexpr int LINE_HDR_BITS = 88;
+constexpr int LINE_SP_BITS = 8;
 
 struct LidxView {
   const RootPiece* root;     // R + 1
@@ -67,6 +73,15 @@ struct LidxView {
   int key_bits;            // 2k: wider keys are absent
   int warp_groups;
   bool fallback;           // some leaf resolves through the BBHash
+};
+
+// Searches deferred by the lookup kernel: each wave of its grid owns a
+// segment of cap entries (3 words: key, query index, line | run code << 32)
+// and writes its count to cnt[wave]; k_lidx_search resolves them afterwards.
+struct LidxQueue {
+  uint64_t* ent;  // waves x cap x 3 words
+  uint32_t* cnt;  // entries per wave
+  uint32_t cap;   // 0: every search in place
 };
 
 // Segment views over the warped keys. LEVEL 0 = root pieces (x = u - (r << s),

@@ -307,7 +307,7 @@ typedef struct cf1g_lidx_params {
   uint32_t max_window; /* leaves with a wider error window resolve through the BBHash; 0 = 4096 */
   uint32_t untrained;  /* 1: keep the endpoint interpolation (the method without training) */
   uint32_t line_bytes; /* 64 or 128; 0 = 64 */
-  uint32_t line_keys;  /* keys per line C; 0 = 256 (128-byte lines) or 112 (64-byte lines) */
+  uint32_t line_keys;  /* keys per line C; 0 = 256 (128-byte lines) or 108 (64-byte lines) */
 } cf1g_lidx_params;
 
 typedef struct cf1g_lidx_info {

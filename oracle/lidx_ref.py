@@ -297,19 +297,32 @@ def error_bounds(keys, root, leaf, s):
 #   bits  0..61  k0, the key of rank iC;  bit 62: overflow (no payload);
 #   bit 63: code width w (0: 3 bits, 1: 4 bits)
 #   bits 64..69  t, the cell shift;        bits 70..79: ulen
-#   unary [80, 80 + ulen): for cells c = 0 .. (k_last - k0) >> t, one 1 per
+#   bits 80..87  the sentinel's prefix: the top 8 bits of its offset in its
+#         cell (sentinel_prefix; 0 without a sentinel)
+#   unary [88, 88 + ulen): for cells c = 0 .. (k_last - k0) >> t, one 1 per
 #         key with (key - k0) >> t == c, then a 0
 #   codes after the unary: for every key j >= 1 in the cell of key j-1, in
 #         order, w bits min(t - bitlen((k_j - k0) ^ (k_{j-1} - k0)), 2^w - 1):
 #         the highest bit where the two differ is t - 1 - code
 # For each w, t_w minimises m + cells + w collisions over [tb - 2, tb + 3]
 # (first minimum), tb = max(0, bitlen(span) - bitlen(m)). The line takes
-# 4-bit codes when they fit 8 line_bytes - 80 bits and 3-bit codes would not
+# 4-bit codes when they fit 8 line_bytes - 88 bits and 3-bit codes would not
 # fit or would hold a code of 7 (some pair closer than t_3 - 7 bits); else
 # 3-bit codes when they fit; else it is an overflow line.
-HDR_BITS = 80
+HDR_BITS = 88
+SP_BITS = 8
+
+
+def sentinel_prefix(rel, t):
+    """Top SP_BITS bits of rel's offset in its cell (rel mod 2^t), read as a
+    t-bit number (exact, shifted up, when t < SP_BITS): of two values in one
+    cell, the one with the larger prefix is the larger."""
+    off = rel & ((1 << t) - 1)
+    return off >> (t - SP_BITS) if t >= SP_BITS else off << (SP_BITS - t)
+
+
 def default_line_keys(line_bytes):
-    return {64: 112, 128: 256}[line_bytes]
+    return {64: 108, 128: 256}[line_bytes]
 
 
 def n_lines(D, C):
@@ -373,6 +386,8 @@ def encode_line(keys, i, C, line_bytes):
     ulen = m + cells
     put(64, t, 6)
     put(70, ulen, 10)
+    if m > n:
+        put(80, sentinel_prefix(rel[-1], t), SP_BITS)
     pos, prev = HDR_BITS, 0
     for j in range(m):
         c = rel[j] >> t
@@ -415,7 +430,9 @@ def decode_line(line, q, i, C, n_total_lines):
     """One step of the walk for query q at line i: ("found", rank),
     ("left",), ("right",), ("absent",) or ("search", a, b, right_ok): binary
     search ranks [a, b] of the sorted keys; right_ok: a q above them all goes
-    right."""
+    right. In the run that holds the sentinel, the prefixes of q and the
+    sentinel decide: a larger one goes right, a smaller one leaves the run
+    without the sentinel, and equal ones search the run."""
     w = [int(x) for x in np.asarray(line).view(np.uint64)]
 
     def bit(b):
@@ -448,9 +465,14 @@ def decode_line(line, q, i, C, n_total_lines):
         return ("absent",)
     last = j0 + s - 1
     if has_sent and last == C:
-        if s == 1:
+        qp, sp = sentinel_prefix(rel, t), get(80, SP_BITS)
+        if qp > sp:
             return ("right",)
-        return ("search", i * C + j0, i * C + last, True)
+        if qp == sp:
+            return ("search", i * C + j0, i * C + last, True)
+        s, last = s - 1, last - 1
+        if s == 0:
+            return ("absent",)
     if s == 1:
         return ("found", i * C + j0)
     if s - 1 > 64 // cw:

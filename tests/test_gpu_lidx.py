@@ -48,7 +48,7 @@ def test_sorted_keys_and_exact_ranks(built):
     keys = sorted_keys(g)
     np.testing.assert_array_equal(m["keys"], keys)
     assert info["n_keys"] == D and info["n_leaves"] == (1 << LB) + (1 << RB) and info["n_pieces"] == 1 << RB
-    assert info["n_lines"] == -(-D // info["line_keys"]) and info["line_bytes"] == 64 and info["line_keys"] == 112
+    assert info["n_lines"] == -(-D // info["line_keys"]) and info["line_bytes"] == 64 and info["line_keys"] == 108
     rng = np.random.default_rng(0)
     perm = rng.permutation(D)
     out = g.lidx_lookup(keys[perm])

@@ -262,9 +262,10 @@ def test_line_format_by_hand():
     w = [int(x) for x in lines[0].view(np.uint64)]
     assert w[0] == 16  # k0, no overflow
     assert w[1] & 63 == 4 and (w[1] >> 6) & 1023 == 6  # t, ulen
-    unary = [(w[1] >> (16 + b)) & 1 for b in range(6)]
+    assert (w[1] >> 16) & 255 == 0  # no sentinel: prefix 0
+    unary = [(w[1] >> (24 + b)) & 1 for b in range(6)]
     assert unary == [1, 1, 1, 0, 1, 0]
-    codes = (w[1] >> 22) & 0x3F
+    codes = (w[1] >> 30) & 0x3F
     assert codes == 3 | (1 << 3)
     assert st[0] == (12, False, 0)
     for r, q in enumerate(keys):
@@ -288,7 +289,7 @@ def test_line_takes_4bit_codes_for_a_close_pair():
     w = [int(x) for x in lines[0].view(np.uint64)]
     assert (w[0] >> 63) & 1 == 1 and (w[0] >> 62) & 1 == 0
     assert w[1] & 63 == 10 and (w[1] >> 6) & 1023 == 8
-    assert (w[1] >> (16 + 8)) & 15 == 9  # the one code, 4 bits
+    assert (w[1] >> (24 + 8)) & 15 == 9  # the one code, 4 bits
     assert st[0] == (12, False, 0)
     for r, q in enumerate(ks):
         assert LR.decode_line(lines[0], q, 0, 4, 1) == ("found", r)
@@ -336,9 +337,44 @@ def test_lines_overflow_search():
     assert all(o for _, o, _ in st[:-1])  # the last line holds 100 keys
 
 
+def test_sentinel_prefix_by_hand():
+    """Line 0 of keys 64, 70, 77, 79 (C = 3) with sentinel 79: rel 0, 6, 13,
+    15 (pair bit lengths 3, 4, 2), tb = 1 and t = 2 needs 4 + 3 + 1 + 3 = 11
+    bits against 12, 12, 20 and 14 for t = 1, 3, 0 and 4, so cells are
+    [0, 1, 3, 3]; the sentinel's offset in its cell is 3, prefix 3 << 6 = 192
+    (t < 8: exact). q = 77 (offset 1, prefix 64 < 192) resolves in the run
+    without the sentinel (one key, rank 2); q = 79 ties (searched, found at
+    rank 3). A query above the sentinel in its cell goes right."""
+    keys = np.array([64, 70, 77, 79, 200], np.uint64)
+    ks = [64, 70, 77, 79]
+    t, bits, coll, cw = LR.line_choice(ks, 64)
+    assert (t, bits, cw) == (2, 11, 3)
+    lines, st = LR.encode_lines(keys, 3, 64)
+    w = [int(x) for x in lines[0].view(np.uint64)]
+    assert (w[1] >> 16) & 255 == LR.sentinel_prefix(15, 2) == 192
+    assert LR.decode_line(lines[0], 77, 0, 3, 2) == ("found", 2)
+    assert LR.decode_line(lines[0], 79, 0, 3, 2) == ("search", 2, 3, True)
+    assert LR.line_lookup(keys, lines, 3, 79, 0) == 3
+    # rel 0, 100, 519, 812 (sentinel), then 912 in line 1: t = 8 (11 bits
+    # against 14 and 12 for t = 7 and 9), cells 0, 0, 2, 3; 912 lies in the
+    # sentinel's cell 3 with offset 144 > 44 (exact prefixes at t = 8): right
+    assert LR.sentinel_prefix(300, 9) == 150 and LR.sentinel_prefix(812, 8) == 44
+    base = 1 << 30
+    keys = np.array([base, base + 100, base + 519, base + 812, base + 912], np.uint64)
+    t, bits, coll, cw = LR.line_choice([int(x) for x in keys[:4]], 64)
+    assert (t, bits) == (8, 11)
+    lines, st = LR.encode_lines(keys, 3, 64)
+    w = [int(x) for x in lines[0].view(np.uint64)]
+    assert (w[1] >> 16) & 255 == 44
+    assert LR.decode_line(lines[0], int(keys[4]), 0, 3, 2) == ("right",)
+    for r, q in enumerate(keys):
+        assert LR.line_lookup(keys, lines, 3, int(q), 0) == r
+
+
 def test_lines_sentinel_cell_shared():
     """The next line's first key shares a cell with this line's last keys:
-    keys of both lines resolve (the run with the sentinel is searched)."""
+    keys of both lines resolve (the run with the sentinel: prefixes decide,
+    ties are searched)."""
     keys = np.array([1000, 1001, 1002, 1003, 1004, 1005, 1006, 5000, 5001], np.uint64)
     _check_all(keys, 4, 64, starts=(0, 1, -1))
     keys = np.array([1 << 40, (1 << 40) + 3, (1 << 40) + 5, (1 << 40) + 6, (1 << 40) + 7, (1 << 41)], np.uint64)
