@@ -944,23 +944,35 @@ __device__ uint64_t bbh_levels(const uint64_t (&h1)[KPT], uint32_t alive, int ns
   return sh->off;
 }
 
-// Slot of a key in its segment's dedup hash table of 2^tb entries: the bits
-// of seg_hash just below the segment bits (uniform inside a segment); the
-// next 10 bits are the key's fingerprint in its table entry.
-template <typename KeyT>
-__device__ __forceinline__ void tab_slot(const KeyT& k, int sb, int tb, uint32_t& slot, uint32_t& fp) {
-  const uint64_t h = seg_hash(k) << sb;
+// The dedup holds each u64 key as bbh_h1(key), a bijection (an odd
+// multiplier after an xor): the occurrence filter, the hash table and the
+// BBHash levels then share one hash per key, and two held values are equal
+// exactly when their keys are. 16-byte keys are held as they are (their h1
+// is not one to one) and hashed by seg_hash where the filter and the table
+// need bits.
+__device__ __forceinline__ uint64_t dd_hold(uint64_t k) { return bbh_h1(k); }
+__device__ __forceinline__ Key128 dd_hold(const Key128& k) { return k; }
+__device__ __forceinline__ uint64_t dd_bits(uint64_t h, int) { return h; }
+__device__ __forceinline__ uint64_t dd_bits(const Key128& k, int sb) { return seg_hash(k) << sb; }
+__device__ __forceinline__ uint64_t dd_h1(uint64_t h) { return h; }
+__device__ __forceinline__ uint64_t dd_h1(const Key128& k) { return bbh_h1(k); }
+// held value against the key of an occurrence in occ
+__device__ __forceinline__ bool dd_same(uint64_t h, uint64_t k) { return bbh_h1(k) == h; }
+__device__ __forceinline__ bool dd_same(const Key128& h, const Key128& k) { return key_eq(h, k); }
+
+// Position of a held key in the dedup's occurrence filter of FILT_BITS bits
+// (the top 16 of its bits), its slot in the segment's hash table of 2^tb
+// entries (the next tb bits) and its 10-bit fingerprint (the 10 after).
+constexpr int FILT_BITS = 1 << 16;
+template <typename HeldT>
+__device__ __forceinline__ uint32_t filt_pos(const HeldT& k, int sb) {
+  return (uint32_t)(dd_bits(k, sb) >> 48);
+}
+template <typename HeldT>
+__device__ __forceinline__ void tab_slot(const HeldT& k, int sb, int tb, uint32_t& slot, uint32_t& fp) {
+  const uint64_t h = dd_bits(k, sb) << 16;
   slot = (uint32_t)(h >> (64 - tb));
   fp = (uint32_t)((h << tb) >> 54);
-}
-
-// Position of a key in the dedup's occurrence filter of FILT_BITS bits: 16
-// bits of seg_hash (shifted as for tab_slot) below those the table slot and
-// fingerprint use.
-constexpr int FILT_BITS = 1 << 16;
-template <typename KeyT>
-__device__ __forceinline__ uint32_t filt_pos(const KeyT& k, int sb) {
-  return (uint32_t)((seg_hash(k) << sb) >> 26) & (FILT_BITS - 1);
 }
 
 // Work-groups per CU the dedup kernel is compiled for: 8 with u64 keys (64
@@ -1055,7 +1067,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     }
     return;
   }
-  KeyT key[KPT];
+  KeyT key[KPT];  // held values (dd_hold)
   uint32_t msk[KPT];
   uint32_t alive = 0;
   uint64_t u = 0;  // distinct keys
@@ -1102,7 +1114,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
       if (s >= ns) break;
       const uint64_t e = (uint64_t)s * WG + threadIdx.x;
       if (e < u) {
-        key[s] = occ[start + e];
+        key[s] = dd_hold(occ[start + e]);
         mj[e] = obs[start + e];
         alive |= 1u << s;
       }
@@ -1202,7 +1214,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
         if (s >= ns) break;
         const uint64_t e = (uint64_t)s * WG + threadIdx.x;
         const bool v = e < n;
-        key[s] = v ? occ[start + e] : KeyT{};
+        key[s] = v ? dd_hold(occ[start + e]) : KeyT{};
         msk[s] = v ? (uint32_t)mj[e] : 0u;
         pend |= (uint32_t)v << s;
       }
@@ -1267,7 +1279,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
           if (old[s] == 0) {
             alive |= 1u << s;
             pend &= ~(1u << s);
-          } else if ((old[s] >> 22) == (ent[s] >> 22) && key_eq(occ[start + (old[s] & 0xFFFu) - 1], key[s])) {
+          } else if ((old[s] >> 22) == (ent[s] >> 22) && dd_same(key[s], occ[start + (old[s] & 0xFFFu) - 1])) {
             atomicOr(&tab[hs[s]], msk[s] << 12);
             pend &= ~(1u << s);
           } else {
@@ -1314,7 +1326,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
 #pragma unroll
   for (int s = 0; s < KPT; s++) {
     if (s >= ns) break;
-    h1[s] = ((alive >> s) & 1) ? bbh_h1(key[s]) : 0;
+    h1[s] = ((alive >> s) & 1) ? dd_h1(key[s]) : 0;
   }
   const uint64_t bits = bbh_levels<KPT, WG>(h1, alive, ns, u, seenA, colA, seenB, colB, region, max_bits, ppos, tl, &bsh, err);
   const uint64_t nb = (bits + BBH_BLOCK_BITS - 1) / BBH_BLOCK_BITS;  // <= ceil(REG_W*64/448) < WG
