@@ -27,7 +27,7 @@ namespace cf1g {
 // kernels_p1p2.hip
 void launch_pack(hipStream_t, const uint8_t*, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t*, uint64_t*);
 uint64_t p1_tiles(uint64_t);
-uint64_t p1_tile_positions();
+uint64_t p1_tile_slots();
 uint64_t p1_slices(uint64_t);
 void launch_p1_scan(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint64_t, int, int, uint64_t*, uint32_t*,
                     uint32_t*, uint32_t*, const uint64_t*, uint32_t, uint32_t*, uint16_t*, uint32_t);
@@ -418,10 +418,10 @@ void do_p1(cf1g_ctx* c) {
   auto* tot = c->getT<unsigned long long>("totals", 4);
   // super k-mer entries: one slot of tile-size per scan tile; partial
   // histograms per slice of tiles
-  uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
+  uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_slots() + 1);
   uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
-  uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_slots() + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
   auto* hist_all = c->getT<unsigned long long>("hist_all", CF1G_NUM_BUCKETS);
@@ -511,10 +511,10 @@ void do_p1_stream(cf1g_ctx* c, const uint8_t* ascii, const uint64_t* rec_off, ui
   auto* hist = c->getT<unsigned long long>("hist", CF1G_NUM_BUCKETS);
   auto* hist_all = c->getT<unsigned long long>("hist_all", CF1G_NUM_BUCKETS);
   auto* tot = c->getT<unsigned long long>("totals", 4);
-  uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
+  uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_slots() + 1);
   uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint32_t* tw = c->getT<uint32_t>("sk_tile_w", nt + 1);
-  uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_slots() + 1);
   uint32_t* part = c->getT<uint32_t>("p1_part", nsl * CF1G_NUM_BUCKETS + 1);
   uint64_t* sums = c->getT<uint64_t>("p1_sums", 2 * nsl + 2);
   const uint32_t n_edges = (uint32_t)(c->edges_h.size() - 1);
@@ -635,14 +635,14 @@ void do_p2(cf1g_ctx* c, const uint64_t* hist_in, uint32_t num_bins) {
   p2_chunking(c->n_pos, c->num_bins, &chunk_tiles, &n_chunks);
   uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   uint16_t* rec_sub = c->getT<uint16_t>("rec_sub", c->S + 8);
-  const uint64_t nslots = nt * p1_tile_positions();
+  const uint64_t nslots = nt * p1_tile_slots();
   uint32_t* ent_dest = c->getT<uint32_t>("ent_dest", nslots + 1);
   // one read after the route: the routed-record total, then the overflow flag
   auto* route_chk = c->getT<unsigned long long>("route_check", 2);
   auto* routed_d = route_chk;
   unsigned int* ovf = (unsigned int*)(route_chk + 1);
-  const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1);
-  const uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_positions() + 1);
+  const uint64_t* ent = c->getT<uint64_t>("sk_entries", nt * p1_tile_slots() + 1);
+  const uint32_t* bkt = c->getT<uint32_t>("sk_buckets", nt * p1_tile_slots() + 1);
   const uint32_t* tcnt = c->getT<uint32_t>("sk_tile_cnt", nt + 1);
   uint64_t* seq = c->getT<uint64_t>("seq", 2 * c->n_words64 + 8);
   uint64_t* nmask = c->getT<uint64_t>("nmask", c->n_words64 + 8);
@@ -1187,9 +1187,9 @@ uint8_t* annotate_device(cf1g_ctx* c) {
     uint64_t* list = c->getT<uint64_t>("p5_list", c->S + 1);
     c->zero(err, 8);
     c->zero(d, c->n_pos);
-    launch_p5_list(c->st, c->getT<uint64_t>("sk_entries", nt * p1_tile_positions() + 1),
-                   c->getT<uint32_t>("ent_dest", nt * p1_tile_positions() + 1), c->getT<uint32_t>("sk_tile_cnt", nt + 1), nt,
-                   (uint32_t)p1_tile_positions(), c->getT<uint32_t>("rec_dest", c->S + 1), c->S, list, err);
+    launch_p5_list(c->st, c->getT<uint64_t>("sk_entries", nt * p1_tile_slots() + 1),
+                   c->getT<uint32_t>("ent_dest", nt * p1_tile_slots() + 1), c->getT<uint32_t>("sk_tile_cnt", nt + 1), nt,
+                   (uint32_t)p1_tile_slots(), c->getT<uint32_t>("rec_dest", c->S + 1), c->S, list, err);
     launch_p5_annot(c->st, c->wide, c->getT<uint64_t>("rec_seg", c->S * (uint64_t)c->rw + 1), list,
                     c->getT<uint64_t>("seg_roff", ns + 1), c->getT<uint64_t>("seg_out_off", ns + 1),
                     c->getT<uint64_t>("bbh_blk_off", ns + 1), c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8),

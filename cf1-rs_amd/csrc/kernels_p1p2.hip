@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii,
 // k-mers sharing the minimizer position, at most w long; it routes by
 // bucket_hash(value).
 //
-// k_p1_scan walks tiles of SK_T k-mer positions (work-groups loop over
+// k_p1_scan walks tiles of SK_TP k-mer positions (work-groups loop over
 // tiles). Per tile it computes the m-mer orders of positions [t0-1, t0+T+2w)
 // into LDS (8 consecutive m-mers per thread from one 64-bit window and its
 // reverse complement), each position's minimizer index (sliding window) and
@@ -180,9 +180,14 @@ __global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii,
 // to the `excl` list, which k_p1_excl takes off the counts. `edges` is
 // sorted and ends with a UINT64_MAX sentinel.
 constexpr int SK_WG = 256;
-constexpr int SK_T = 2048;  // k-mer positions per tile
+constexpr int SK_T = 2048;  // slots per tile: entries and buckets arrays, LDS
 constexpr int SK_Q = 8;     // consecutive m-mers / positions per thread step
 constexpr int SK_MAXW = 64;  // w <= 49 for k <= 63
+// k-mer positions per tile: the tile's m-mer orders cover T + 2w slots,
+// which at T = SK_T - 2 SK_MAXW fit one step of 8 slots per thread (with
+// SK_T positions the first wave made a second pass over the last 2w slots
+// while the other three waited at the barrier)
+constexpr int SK_TP = SK_T - 2 * SK_MAXW;
 constexpr int SK_NJ = SK_T + 2 * SK_MAXW;
 constexpr int MAX_BINS = 4096;
 
@@ -248,7 +253,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
   const uint32_t M = (uint32_t)((1ULL << (2 * m)) - 1);
   uint8_t* brk8 = (uint8_t*)S.brk;
   for (uint64_t tile = tile_begin + blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const uint64_t t0 = 1 + tile * SK_T;  // first k-mer position of the tile
+    const uint64_t t0 = 1 + tile * SK_TP;  // first k-mer position of the tile
     const uint64_t B = t0 - 1;            // tile slot 0 <-> stream position B
     if (threadIdx.x == 0) {  // first edge >= t0 (edges are >= 4M apart: at most one in the tile's reach)
       uint32_t lo = 0, hi = n_edges;
@@ -259,7 +264,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
       S.edge = edges[lo];
     }
     // (1) m-mer orders of slots [0, T + 2w)
-    const int nj = SK_T + 2 * w;
+    const int nj = SK_TP + 2 * w;
     for (int j0 = threadIdx.x * SK_Q; j0 < nj; j0 += SK_WG * SK_Q) {
       const uint64_t x = stream_bits64(seq, 2 * (B + (uint64_t)j0));
       const uint64_t rx = rc64(x, 32);
@@ -279,7 +284,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
     // slots are tracked as two 32-bit values: scans towards higher slots keep
     // the incumbent on ties, scans towards lower slots take the newcomer, and
     // a left part beats a right part on ties.
-    const int np = SK_T + w + 1;
+    const int np = SK_TP + w + 1;
     const int span = K + SK_Q - 1;  // bases under the 8 windows of a step
     for (int j0 = threadIdx.x * SK_Q; j0 < np; j0 += SK_WG * SK_Q) {
       const uint64_t g = B + (uint64_t)j0;
@@ -384,7 +389,7 @@ __global__ void __launch_bounds__(SK_WG, 8) k_p1_scan(const uint64_t* __restrict
 #pragma unroll
     for (int q = 0; q < SK_Q; q++) {
       const int j = li0 + q + 1;
-      if (t0 + (uint64_t)(li0 + q) >= n_pos || (S.pidx[PJ(j)] & SK_INVALID)) continue;
+      if (li0 + q >= SK_TP || t0 + (uint64_t)(li0 + q) >= n_pos || (S.pidx[PJ(j)] & SK_INVALID)) continue;
       valid_cnt++;
       if ((brk8[j >> 3] >> (j & 7)) & 1) {
         starts++;
@@ -916,8 +921,10 @@ void launch_pack(hipStream_t st, const uint8_t* ascii, const uint64_t* rec_off, 
   dim3 grid((unsigned)((waves + 3) / 4));
   CF1G_LAUNCH("pack", k_pack, grid, dim3(256), 0, st, ascii, rec_off, gstart, n_rec, n_words64, seq, nmask);
 }
-uint64_t p1_tiles(uint64_t n_pos) { return n_pos > 1 ? (n_pos - 1 + SK_T - 1) / SK_T : 0; }
-uint64_t p1_tile_positions() { return SK_T; }
+uint64_t p1_tiles(uint64_t n_pos) { return n_pos > 1 ? (n_pos - 1 + SK_TP - 1) / SK_TP : 0; }
+// Slots per tile in the entry, bucket and destination arrays (a tile emits at
+// most one entry per position).
+uint64_t p1_tile_slots() { return SK_T; }
 uint64_t p1_slices(uint64_t n_pos) {
   const uint64_t nt = p1_tiles(n_pos);
   return nt < (uint64_t)HB_SLICES ? nt : (uint64_t)HB_SLICES;
@@ -937,7 +944,7 @@ void launch_p1_scan(hipStream_t st, const uint64_t* seq, const uint64_t* nmask, 
 // First stream position past what the scan of tile t reads (orders over
 // [t0-1, t0+T+2w) from 64-position windows, validity from two placeholder
 // words per step, each start's minimizer m-mer).
-uint64_t p1_tile_reach(uint64_t t) { return t * SK_T + SK_T + 2 * SK_MAXW + 512; }
+uint64_t p1_tile_reach(uint64_t t) { return t * SK_TP + SK_TP + 2 * SK_MAXW + 512; }
 // The histogram from every scanned tile: partial counts per slice, their sum,
 // then the chunk-edge exclusions.
 void launch_p1_finish(hipStream_t st, uint64_t n_pos, uint32_t* buckets, uint32_t* tile_cnt, uint32_t* tile_w, uint32_t* part,
