@@ -1210,6 +1210,7 @@ __global__ void __launch_bounds__(256) k_p4_cas_lidx(const uint64_t* __restrict_
                                                      uint32_t chunk, uint64_t n_chunks, int force_retry) {
   __shared__ uint32_t wt[WARP_MAX_GROUPS * 256];
   __shared__ RootPiece rt[LIDX_LDS_ROOT_MAX + 1];
+  __shared__ uint64_t lslot[FB ? 1 : 256 * (NW + 1)];
   __shared__ uint64_t s_c;
   __shared__ unsigned long long s_sum[4];
   for (int j = threadIdx.x; j < V.warp_groups * 256; j += 256) wt[j] = V.warp[j];
@@ -1219,6 +1220,8 @@ __global__ void __launch_bounds__(256) k_p4_cas_lidx(const uint64_t* __restrict_
   if (threadIdx.x < 4) s_sum[threadIdx.x] = 0;
   const RootPiece* root = rl ? rt : V.root;
   uint32_t n_occ = 0, n_cas = 0, n_fail = 0, n_miss = 0;
+  const LidxQueue noq{nullptr, nullptr, 0};  // searches in place
+  uint32_t qn = 0;
   for (;;) {
     if (threadIdx.x == 0) s_c = atomicAdd(&counters[4], 1ULL);
     __syncthreads();
@@ -1227,17 +1230,54 @@ __global__ void __launch_bounds__(256) k_p4_cas_lidx(const uint64_t* __restrict_
     if (c >= n_chunks) break;
     const uint64_t g0 = 1 + c * chunk;
     const uint64_t g1 = g0 + chunk < n_pos ? g0 + chunk : n_pos;
-    for (uint64_t g = g0 + threadIdx.x; g < g1; g += 256) {
-      uint64_t canon;
-      const uint32_t m = observe<uint64_t>(seq, nmask, g, K, canon);
-      if (!m) continue;
-      n_occ++;
-      const uint64_t idx = lidx_rank_t<NW, FB>(V, mv, wt, root, canon);
-      if (idx >= V.D) {
-        n_miss++;
-        continue;
+    if (FB) {
+      for (uint64_t g = g0 + threadIdx.x; g < g1; g += 256) {
+        uint64_t canon;
+        const uint32_t m = observe<uint64_t>(seq, nmask, g, K, canon);
+        if (!m) continue;
+        n_occ++;
+        const uint64_t idx = lidx_rank_t<NW, FB>(V, mv, wt, root, canon);
+        if (idx >= V.D) {
+          n_miss++;
+          continue;
+        }
+        cas_join(states, idx, m, force_retry != 0, n_cas, n_fail);
       }
-      cas_join(states, idx, m, force_retry != 0, n_cas, n_fail);
+      continue;
+    }
+    // LIDX_NK occurrences per lane (positions g + 256 k) through the
+    // lookup's lockstep front and walk, then their CAS joins
+    for (uint64_t g = g0 + threadIdx.x; g < g1; g += 256 * LIDX_NK) {
+      uint64_t key[LIDX_NK];
+      uint32_t m[LIDX_NK];
+#pragma unroll
+      for (int k = 0; k < LIDX_NK; k++) {
+        key[k] = 0;
+        m[k] = g + 256 * k < g1 ? observe<uint64_t>(seq, nmask, g + 256 * k, K, key[k]) : 0u;
+      }
+      bool ok[LIDX_NK];
+      uint32_t line[LIDX_NK];
+      lidx_pred_lines<LIDX_NK>(V, wt, root, key, ok, line);
+#pragma unroll
+      for (int k = 0; k < LIDX_NK; k++) ok[k] = ok[k] && m[k] != 0;
+      uint64_t W[LIDX_NK][NW];
+#pragma unroll
+      for (int k = 0; k < LIDX_NK; k++) load_line<NW>(V.lines, line[k], W[k]);
+#pragma unroll
+      for (int k = 0; k < LIDX_NK; k++) hold_line<NW>(W[k]);
+      uint64_t res[LIDX_NK];
+      bool deferred[LIDX_NK];
+      lidx_finish_n<NW, LIDX_NK>(V, key, ok, line, W, lslot + threadIdx.x * (NW + 1), res, noq, nullptr, qn, 0, 0, deferred);
+#pragma unroll
+      for (int k = 0; k < LIDX_NK; k++) {
+        if (!m[k]) continue;
+        n_occ++;
+        if (res[k] >= V.D) {
+          n_miss++;
+          continue;
+        }
+        cas_join(states, res[k], m[k], force_retry != 0, n_cas, n_fail);
+      }
     }
   }
   atomicAdd(&s_sum[0], (unsigned long long)n_occ);
