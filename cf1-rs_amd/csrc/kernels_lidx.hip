@@ -1399,8 +1399,9 @@ static uint64_t lidx_lookup_grid(uint64_t n) {
 }
 // Deferred-search queue of a lookup of n keys: a segment per wave of the
 // lookup's grid with room for 1/32 of the wave's keys (a multiple of 64, at
-// least 64; searched keys are ~1.2% on the 300 Mbp genome). Segments that
-// fill up search in place.
+// least 64; searched keys are ~1.1% on the 300 Mbp genome), and at most
+// 2^24 entries in all (402 MB) for very large calls. Segments that fill up
+// search in place.
 void lidx_queue_shape(uint64_t n, uint32_t& waves, uint32_t& cap) {
   waves = (uint32_t)(lidx_lookup_grid(n) * 4);
   if (!waves) {
@@ -1409,6 +1410,8 @@ void lidx_queue_shape(uint64_t n, uint32_t& waves, uint32_t& cap) {
   }
   const uint64_t per = (n + waves - 1) / waves;
   uint64_t c = (per / 32 + 63) / 64 * 64;
+  const uint64_t most = ((1ULL << 24) / waves) / 64 * 64;
+  if (c > most) c = most;
   cap = (uint32_t)(c < 64 ? 64 : c);
 }
 // Q: from lidx_queue_shape for this n (ent: waves x cap x 3 words, cnt:

@@ -195,7 +195,11 @@ constexpr int P3_MAX_SB = 12;
 // is the 128-bit value W = w0:w1, position t at bits [126-2t, 127-2t]; k-mer
 // j spans positions j+1..j+K.
 __device__ __forceinline__ uint64_t rec2_bits(uint64_t w0, uint64_t w1, int sh) {  // low 64 bits of W >> sh, 0 < sh < 128
-  return sh >= 64 ? (w0 >> (sh - 64)) : ((w0 << (64 - sh)) | (w1 >> sh));
+  // both halves of the funnel, then a select (lanes of a wave take either
+  // side; branches would run both with exec masks)
+  const uint64_t lo = (w1 >> (sh & 63)) | (w0 << ((64 - sh) & 63));  // 0 < sh < 64
+  const uint64_t hi = w0 >> (sh & 63);                               // 64 <= sh < 128
+  return sh >= 64 ? hi : lo;
 }
 __device__ __forceinline__ int rec2_kmers(uint64_t w1, int K) {
   int L = (int)(w1 & 255);
@@ -284,17 +288,22 @@ __device__ __forceinline__ Key128 rec4_occ(const Rec4& R, int j, int n, int K, u
 // (j > j0) is the top base of k-mer j-1.
 template <int PK, typename FK>
 __device__ __forceinline__ void rec2_piece(uint64_t w0, uint64_t w1, int j0, int cnt, int n, int K, uint64_t mask, FK out) {
+  static_assert(PK <= 4, "the next bases of a piece come from one byte of the record");
   uint64_t x = rec2_bits(w0, w1, 126 - 2 * (j0 + K)) & mask;
   uint64_t r = rc64(x, K);
   const bool lf = (w1 >> 8) & 1, rf = (w1 >> 9) & 1;
   uint32_t pb = (uint32_t)rec2_bits(w0, w1, 126 - 2 * j0) & 3u;
+  // positions j0 + K + 1 .. j0 + K + 4 (the base after each k-mer of the
+  // piece) as one byte, the first in its top bits; a record holds at most
+  // rec_cap(2) + 2 <= 64 positions, so the shift is >= 0 for j0 + K + 4 <= 63
+  const uint32_t nbw = (uint32_t)rec2_bits(w0, w1, 126 - 2 * (j0 + K + 4)) & 0xFFu;
 #pragma unroll
   for (int i = 0; i < PK; i++) {
     if (i < cnt) {
       const int j = j0 + i;
       const uint64_t c = x < r ? x : r;
-      const bool pv = j > 0 || lf, nv = j < n - 1 || rf;
-      const uint32_t nb = (uint32_t)rec2_bits(w0, w1, 124 - 2 * (j + K)) & 3u;
+      const bool pv = i > 0 || j > 0 || lf, nv = j < n - 1 || rf;
+      const uint32_t nb = (nbw >> (6 - 2 * i)) & 3u;
       const uint64_t nx = ((x << 2) | nb) & mask;
       const uint64_t nr = (r >> 2) | ((uint64_t)(3 - nb) << (2 * K - 2));
       const uint64_t nc = nx < nr ? nx : nr;
