@@ -298,10 +298,14 @@ void pack_words(const uint8_t* ascii, const uint64_t* rec_off, const uint64_t* g
         nm |= (uint64_t)(code >> 2) << (63 - i);
       }
     }
-    sq[0] = hi_w;
-    sq[1] = lo_w;
-    nmask[w - w0] = nm;
+    // the packed words are written once and read by the copy engine (or the
+    // next copy): non-temporal stores skip each line's read for ownership,
+    // a third of the packing's host memory traffic
+    _mm_stream_si64((long long*)&sq[0], (long long)hi_w);
+    _mm_stream_si64((long long*)&sq[1], (long long)lo_w);
+    _mm_stream_si64((long long*)&nmask[w - w0], (long long)nm);
   }
+  _mm_sfence();  // the stores are complete before the caller hands the words on
 }
 
 void pack_stream(const uint8_t* ascii, const uint64_t* rec_off, uint64_t n_rec, uint64_t n_words64, uint64_t* seq, uint64_t* nmask,
