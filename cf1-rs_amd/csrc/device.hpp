@@ -311,21 +311,13 @@ enum : uint32_t { OBS_MIMO = 0, OBS_LEFT = 1, OBS_RIGHT = 2, OBS_INTERNAL = 3 };
 // mask (bits 0-4 in-label A,C,G,T,none; bits 5-9 out-label) and the joined
 // state of a k-mer is the OR of its occurrences' masks.
 constexpr uint32_t LBL_NONE = 4;
+// The type's bits say which neighbours count: bit 1 (RIGHT, INTERNAL) the
+// previous base, bit 0 (LEFT, INTERNAL) the next one; in reverse orientation
+// they swap sides and complement. Selects only, no branches.
 __host__ __device__ __forceinline__ uint32_t obs_mask(uint32_t type, bool fwd, uint32_t pb, uint32_t nb) {
-  uint32_t a, b;
-  if (type == OBS_MIMO) {
-    a = LBL_NONE;
-    b = LBL_NONE;
-  } else if (type == OBS_LEFT) {
-    a = fwd ? LBL_NONE : 3 - nb;
-    b = fwd ? nb : LBL_NONE;
-  } else if (type == OBS_RIGHT) {
-    a = fwd ? pb : LBL_NONE;
-    b = fwd ? LBL_NONE : 3 - pb;
-  } else {
-    a = fwd ? pb : 3 - nb;
-    b = fwd ? nb : 3 - pb;
-  }
+  const bool hp = (type >> 1) & 1u, hn = type & 1u;
+  const uint32_t a = fwd ? (hp ? pb : LBL_NONE) : (hn ? 3 - nb : LBL_NONE);
+  const uint32_t b = fwd ? (hn ? nb : LBL_NONE) : (hp ? 3 - pb : LBL_NONE);
   return (1u << a) | (1u << (5 + b));
 }
 // State::from_vertex (state.rs:92-113) of the joined label sets.

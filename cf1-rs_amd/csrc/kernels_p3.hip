@@ -307,12 +307,12 @@ __device__ __forceinline__ void rec2_piece(uint64_t w0, uint64_t w1, int j0, int
       const uint64_t nx = ((x << 2) | nb) & mask;
       const uint64_t nr = (r >> 2) | ((uint64_t)(3 - nb) << (2 * K - 2));
       const uint64_t nc = nx < nr ? nx : nr;
-      uint32_t type;
-      if (!pv && !nv) type = OBS_MIMO;
-      else if (!nv) type = OBS_RIGHT;
-      else if (nc == c) type = (!pv || x == nx) ? OBS_MIMO : OBS_RIGHT;
-      else type = pv ? OBS_INTERNAL : OBS_LEFT;
-      out(i, c, obs_mask(type, x == c, pb, nb));
+      // the type of rec2_occ as (has a previous, has a next) neighbour, no
+      // branches: a self loop (next canonical equal) drops the next one, and
+      // the previous one too when the next k-mer is this one itself
+      const bool self = nv && nc == c;
+      const bool hn = nv && !self, hp = pv && !(self && x == nx);
+      out(i, c, obs_mask((hp ? OBS_RIGHT : 0u) | (hn ? OBS_LEFT : 0u), x == c, pb, nb));
       pb = (uint32_t)(x >> (2 * K - 2)) & 3u;
       x = nx;
       r = nr;
