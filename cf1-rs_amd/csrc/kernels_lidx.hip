@@ -978,9 +978,8 @@ __device__ __forceinline__ uint64_t lidx_search(const LidxView& V, uint64_t key,
 // round's neighbour loads issued together; both runs decoded. A key whose
 // run must be searched goes to the wave's queue segment (qn entries so far,
 // the same count in every lane; deferred[k] set) while it has room: the
-// wave then waits for no search loads. The others are searched in place,
-// every key's loads issued together; a search that sends a key on to the
-// next line (rare) finishes one key at a time.
+// wave then waits for no search loads. The others (a full segment, or no
+// queue) are searched in place, one key at a time.
 template <int NW, int NK>
 __device__ __forceinline__ void lidx_finish_n(const LidxView& V, const uint64_t (&key)[NK], const bool (&ok)[NK], uint32_t (&i)[NK],
                                               uint64_t (&W)[NK][NW], uint64_t* slot, uint64_t (&res)[NK], const LidxQueue& Q,
@@ -1170,8 +1169,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))
 // The searches the lookup deferred: one wave per wave of the lookup's grid,
 // over its queue segment.
 template <int NW>
-__global__ void __launch_bounds__(64) k_lidx_search(const uint64_t* __restrict__ keys, const LidxView V, const LidxQueue Q,
-                                                    uint64_t* __restrict__ out) {
+__global__ void __launch_bounds__(64) k_lidx_search(const LidxView V, const LidxQueue Q, uint64_t* __restrict__ out) {
   const uint64_t wave = blockIdx.x;
   const uint32_t cnt = Q.cnt[wave];
   const uint64_t* seg = Q.ent + wave * Q.cap * 3;
@@ -1440,8 +1438,8 @@ void launch_lidx_lookup(hipStream_t st, const uint64_t* keys, uint64_t n, const 
 #undef CF1G_LIDX_LOOKUP
   if (Q.cap) {
     const dim3 sg((unsigned)(g * 4));
-    if (V.line_words == 8) CF1G_LAUNCH("lidx_search", k_lidx_search<8>, sg, dim3(64), 0, st, keys, V, Q, out);
-    else CF1G_LAUNCH("lidx_search", k_lidx_search<16>, sg, dim3(64), 0, st, keys, V, Q, out);
+    if (V.line_words == 8) CF1G_LAUNCH("lidx_search", k_lidx_search<8>, sg, dim3(64), 0, st, V, Q, out);
+    else CF1G_LAUNCH("lidx_search", k_lidx_search<16>, sg, dim3(64), 0, st, V, Q, out);
   }
 }
 void launch_count_mismatch(hipStream_t st, const uint64_t* a, const uint64_t* b, uint64_t n, unsigned long long* bad) {
