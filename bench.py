@@ -62,7 +62,7 @@ WORKLOADS = {
 PHASE_OF = {
     "pack": "P1", "p1_scan": "P1", "p1_hist": "P1", "p1_reduce": "P1",
     "p2_partition": "P2", "p2_route": "P2",
-    "p3_gate": "P3", "p3_rcount": "P3", "p3_rscatter": "P3", "p3_expand": "P3", "p3_dedup": "P3", "p3_bigsort": "P3",
+    "p3_gate": "P3", "p3_rcount": "P3", "p3_rscatter": "P3", "p3_expand": "P3", "p3_dedup": "P3", "p3_dedup_list": "P3", "p3_bigsort": "P3",
     "p3_compact": "P3", "p1_excl": "P1", "scan_total": "scan", "zero": "clear",
     "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
     "p4_classify": "P4",

@@ -61,6 +61,8 @@ void launch_p3_expand(hipStream_t, bool, const uint64_t*, const uint64_t*, const
 void launch_p3_gate(hipStream_t, const uint64_t*, uint64_t, unsigned int*);
 void launch_p3_dedup(hipStream_t, bool, void*, uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, int, int, uint64_t*,
                      uint64_t*, uint32_t*, unsigned int*, uint32_t*, uint64_t*, uint16_t*);
+void launch_p3_dedup_list(hipStream_t, bool, void*, uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, int, int,
+                          uint64_t*, uint64_t*, const uint32_t*, uint32_t*, unsigned int*, uint64_t*, uint16_t*);
 void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, const uint64_t*, const uint32_t*, uint32_t, uint64_t*,
                        unsigned int*, uint32_t*);
 void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
@@ -692,8 +694,10 @@ void do_p3(cf1g_ctx* c) {
   uint64_t* nblk = c->getT<uint64_t>("bbh_seg_nblk", ns + 1);
   uint64_t* blk_off = c->getT<uint64_t>("bbh_blk_off", ns + 1);
   uint32_t* over = c->getT<uint32_t>("seg_over", ns + 1);
+  uint32_t* over2 = c->getT<uint32_t>("seg_over2", ns + 1);
   uint32_t* big = c->getT<uint32_t>("bbh_big", ns + 1);
-  unsigned int* flags = c->getT<unsigned int>("p3_flags", 4);  // [0] n_over, [1] n_big, [2] error bits, [3] count gate
+  // [0] n_over, [1] n_big, [2] error bits, [3] count gate, [4] n_over2
+  unsigned int* flags = c->getT<unsigned int>("p3_flags", 8);
   uint64_t* tmp = c->getT<uint64_t>("scan_tmp", scan_temp_words(ns + 1) + 16);
   const uint64_t* rec = c->getT<uint64_t>("rec", c->S * (uint64_t)c->rw + 1);
   const uint16_t* rec_sub = c->getT<uint16_t>("rec_sub", c->S + 8);
@@ -722,7 +726,7 @@ void do_p3(cf1g_ctx* c) {
     c->zero(kcnt, (ns + 1) * 8);
     c->zero(rcnt, (ns + 1) * 8);
     c->zero(rcur, (ns + 1) * 4);
-    c->zero(flags, 16);
+    c->zero(flags, 32);
     launch_p3_rcount(st, c->wide, rec, rec_sub, bstart, coff, (int)c->num_bins, CHUNK_RECORDS, c->n_chunks, K, sb, rcnt, kcnt);
     scan_excl_u64(st, (const uint64_t*)kcnt, seg_off, ns, seg_off + ns, tmp);
     scan_excl_u64(st, (const uint64_t*)rcnt, roff, ns, roff + ns, tmp);
@@ -731,20 +735,22 @@ void do_p3(cf1g_ctx* c) {
                        rec_dest, flags + 3);
     if (direct) launch_p3_expand(st, c->wide, srec, roff, seg_off, nullptr, ns, K, occ, occ_obs, flags + 3);
     launch_p3_dedup(st, c->wide, occ, occ_obs, srec, roff, seg_off, ns, K, sb, ucount, nblk, over, flags, big, blocks_pad, vpad);
+    launch_p3_dedup_list(st, c->wide, occ, occ_obs, srec, roff, seg_off, ns, K, sb, ucount, nblk, over, over2, flags, blocks_pad, vpad);
     scans();
   });
-  unsigned int hf[4];
+  unsigned int hf[8];
   uint64_t tot[3] = {0, 0, 0};
-  hip_check(hipMemcpyAsync(hf, flags, 16, hipMemcpyDeviceToHost, st), "D2H flags");
+  hip_check(hipMemcpyAsync(hf, flags, 32, hipMemcpyDeviceToHost, st), "D2H flags");
   hip_check(hipMemcpyAsync(&tot[1], key_off + ns, 8, hipMemcpyDeviceToHost, st), "D2H D");
   c->d2h(&tot[2], blk_off + ns, 8);
   if (hf[3]) fail(CF1G_E_INTERNAL, "k-mers in bins differ from the valid k-mer count");
-  c->n_over = hf[0];
-  if (hf[0] > 0) {
+  // segments the work-group dedups could not hold: sorted through global memory
+  c->n_over = hf[4];
+  if (c->n_over > 0) {
     void* aux = c->get("occ_aux", c->W * kb + 16);
     uint16_t* aux_obs = c->getT<uint16_t>("occ_aux_obs", c->W + 8);
-    launch_p3_expand(st, c->wide, srec, roff, seg_off, over, hf[0], K, occ, occ_obs, flags + 3);
-    launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over, hf[0], ucount, flags, big);
+    launch_p3_expand(st, c->wide, srec, roff, seg_off, over2, c->n_over, K, occ, occ_obs, flags + 3);
+    launch_p3_bigsort(st, c->wide, occ, occ_obs, aux, aux_obs, seg_off, over2, c->n_over, ucount, flags, big);
     c->d2h(hf, flags, 12);
   }
   c->n_big = hf[1];
@@ -758,7 +764,7 @@ void do_p3(cf1g_ctx* c) {
     hip_check(hipMemsetAsync(alive, 0xFF, alive_words * 8, st), "memset");
     launch_bbh_big(st, c->wide, occ, occ_obs, vpad, seg_off, ucount, big, c->n_big, nblk, blocks_pad, scratch, alive, occ, flags + 2);
   }
-  if (hf[0] || c->n_big) {
+  if (c->n_over || c->n_big) {
     scans();
     hip_check(hipMemcpyAsync(hf, flags, 12, hipMemcpyDeviceToHost, st), "D2H flags");
     hip_check(hipMemcpyAsync(&tot[1], key_off + ns, 8, hipMemcpyDeviceToHost, st), "D2H D");

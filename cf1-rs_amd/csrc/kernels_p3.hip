@@ -717,20 +717,24 @@ struct SegCfg<Key128> {
   static constexpr int KPT = 8;
 };
 constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12 bits (k <= 5)
-// k_p3_dedup: DD_WG threads, DD_KPT occurrences each, at most DD_CAP per segment
+// k_p3_dedup: DD_WG threads, DD_KPT occurrences each, at most DD_CAP per
+// segment; k_p3_dedup_list takes the segments of up to DD2_CAP occurrences
+// that exceed it, DD2_WG threads each (the 300 Mbp genome's largest segment
+// holds ~2,600)
 constexpr int DD_WG = 256;
 constexpr int DD_KPT = 8;
 constexpr int DD_CAP = DD_WG * DD_KPT;
+constexpr int DD2_WG = 512;
+constexpr int DD2_CAP = DD2_WG * DD_KPT;
 constexpr int DEC_PK = 4;  // k-mers per lane in the dedup's record decode
 
 // A segment's MPHF is built by k_bbh_big when its distinct keys exceed the
 // register capacity, or when it was sorted by k_p3_bigsort (more
 // occurrences than capacity, keys too wide for the direct-address table).
-template <typename KeyT>
+template <typename KeyT, int CAP>
 __device__ __forceinline__ bool seg_is_big(uint64_t n, uint64_t u, int K2) {
-  constexpr uint64_t CAP = DD_CAP;
   const bool table = sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS;
-  return u > CAP || (n > CAP && !table);
+  return u > (uint64_t)CAP || (n > (uint64_t)CAP && !table);
 }
 
 // Each segment builds its BBHash blocks into a padded slot of
@@ -971,18 +975,19 @@ __device__ __forceinline__ bool dd_same(const Key128& h, const Key128& k) { retu
 
 // Position of a held key in the dedup's occurrence filter of FILT_BITS bits
 // (the top 16 of its bits), its slot in the segment's hash table of 2^tb
-// entries (the next tb bits) and its 10-bit fingerprint (the 10 after).
+// entries (the next tb bits) and its FPB-bit fingerprint (the FPB after).
 constexpr int FILT_BITS = 1 << 16;
 template <typename HeldT>
 __device__ __forceinline__ uint32_t filt_pos(const HeldT& k, int sb) {
   return (uint32_t)(dd_bits(k, sb) >> 48);
 }
-template <typename HeldT>
+template <int FPB, typename HeldT>
 __device__ __forceinline__ void tab_slot(const HeldT& k, int sb, int tb, uint32_t& slot, uint32_t& fp) {
   const uint64_t h = dd_bits(k, sb) << 16;
   slot = (uint32_t)(h >> (64 - tb));
-  fp = (uint32_t)((h << tb) >> 54);
+  fp = (uint32_t)((h << tb) >> (64 - FPB));
 }
+constexpr int ceil_log2(int x) { return x <= 1 ? 0 : 1 + ceil_log2((x + 1) / 2); }
 
 // Work-groups per CU the dedup kernel is compiled for: 8 with u64 keys (64
 // VGPRs and 8 x 20 KiB of LDS fill a CU; 3.99 -> 3.60 ms on the 300 Mbp
@@ -996,39 +1001,41 @@ struct DdOccupancy {
   static constexpr int WGS_PER_CU = sizeof(KeyT) == 8 ? 8 : 6;
 };
 
-// Dedup of one segment and its BBHash, DD_WG threads holding DD_KPT
-// occurrences each (element e = slot * DD_WG + thread). Each occurrence
-// carries its 10-bit observation mask; the masks of equal keys are OR-joined
-// (the DFA join of classify.rs:147-518). An occurrence filter (2^16
-// positions, seen / shared bits in LDS) settles every occurrence alone at its
-// position: its key occurs once. The rest (repeated keys and ~n / 2^16
-// false sharers) are deduplicated with an LDS hash table (linear probing,
-// entries = representative occurrence + 1 | joined mask << 12 | 10-bit key
-// fingerprint << 22; keys are compared, from occ, only when the fingerprints
-// match); which occurrence represents a key
-// depends on timing, but the set of keys, their joined masks and the BBHash
-// built over them do not. Output: kout[start + e] = the key of occurrence e
-// if it represents its key, else the pad (all ones; the dense key array is
-// built from these on request, k_p3_keys); the joined masks in MPHF value
-// order at vpad[start + local rank], their
-// count in ucount, the BBHash blocks in the segment's padded slot of
-// blocks_pad and their count in nblk. Segments with more occurrences than
-// DD_CAP go to over_list (sorted by k_p3_bigsort); segments whose distinct
-// keys exceed DD_CAP go to big_list (BBHash built by k_bbh_big). Keys of <= 12
-// bits (k <= 6) use a direct-address mask table instead, which takes any
-// number of occurrences.
-template <typename KeyT>
-__global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
-                                                    const uint64_t* __restrict__ srec, const uint64_t* __restrict__ roff,
-                                                    const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K, int sb,
-                                                    uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
-                                                    uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
-                                                    uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad,
-                                                    uint16_t* __restrict__ vpad) {
-  constexpr int WG = DD_WG;
-  constexpr int KPT = DD_KPT;
-  constexpr int CAP = DD_CAP;
-  constexpr int TAB_BITS = 12;
+// Dedup of one segment and its BBHash, WG threads holding KPT occurrences
+// each (element e = slot * WG + thread), at most CAP = WG * KPT per segment.
+// Each occurrence carries its 10-bit observation mask; the masks of equal
+// keys are OR-joined (the DFA join of classify.rs:147-518). An occurrence
+// filter (2^16 positions, seen / shared bits in LDS) settles every
+// occurrence alone at its position: its key occurs once. The rest (repeated
+// keys and ~n / 2^16 false sharers) are deduplicated with an LDS hash table
+// of 2 CAP entries (linear probing; entries = representative occurrence + 1
+// in EB = log2(2 CAP) bits | joined mask << EB | (22 - EB)-bit key
+// fingerprint << (EB + 10); keys are compared, from occ, only when the
+// fingerprints match); which occurrence represents a key depends on timing,
+// but the set of keys, their joined masks and the BBHash built over them do
+// not. Output: occ[start + e] = the key of occurrence e (equal keys land on
+// one MPHF value, so the dense key array is built from these on request,
+// k_p3_keys); the joined masks in MPHF value order at vpad[start + local
+// rank], their count in ucount, the BBHash blocks in the segment's padded
+// slot of blocks_pad and their count in nblk. Segments with more
+// occurrences than CAP, or a record longer than the decode's owner map, go
+// to over_list (count flags[0], flags[4] with LIST); segments whose distinct keys exceed CAP go
+// to big_list (BBHash built by k_bbh_big). Keys of <= 12 bits (k <= 6) use
+// a direct-address mask table instead, which takes any number of
+// occurrences (not with LIST: that variant sees only the hash path's
+// oversized segments).
+template <typename KeyT, int WG, int KPT, bool LIST>
+__device__ __forceinline__ void dd_segment(const uint64_t seg, KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+                                           const uint64_t* __restrict__ srec, const uint64_t* __restrict__ roff,
+                                           const uint64_t* __restrict__ seg_off, int K, int sb, uint64_t* __restrict__ ucount,
+                                           uint64_t* __restrict__ nblk, uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
+                                           uint32_t* __restrict__ big_list,
+                                           uint64_t* __restrict__ blocks_pad, uint16_t* __restrict__ vpad) {
+  constexpr int CAP = WG * KPT;
+  constexpr int TAB_BITS = ceil_log2(2 * CAP);
+  constexpr int EB = TAB_BITS;      // entry: element + 1
+  constexpr int FPB = 22 - EB;      // entry: fingerprint bits above the 10-bit mask
+  constexpr uint32_t EMASK = (1u << EB) - 1;
   constexpr int SEEN_W = (2 * CAP) / 64;  // level 0 has roundup64(2u) <= 2 CAP bits
   constexpr int REG_W = (4 * CAP + 8192) / 64;
   constexpr int TAB_BYTES = 4 << TAB_BITS;
@@ -1045,7 +1052,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   constexpr int A_SH = REG_W * 8, A_BSH = A_SH + 8 * (WG / 64);
   constexpr int A_END = A_BSH + (int)sizeof(BbhShared<WG>);
   constexpr int AUX_BYTES = A_END > CAP * 2 ? A_END : CAP * 2;
-  static_assert(2 * CAP <= (1 << TAB_BITS) && CAP < (1 << 12), "hash table entries hold element + 1 in 12 bits");
+  static_assert(2 * CAP == (1 << TAB_BITS) && CAP < (1 << EB) && FPB >= 8, "hash table entries hold element + 1, mask and fingerprint");
   static_assert(2 * (FILT_BITS / 8) <= TAB_BYTES, "the occurrence filter fits the table space");
   static_assert(T_END <= T_MJ && (1 << BM_MAX_BITS) * 4 <= TAB_BYTES, "LDS reuse fits");
   static_assert(REG_W / 7 + 2 <= SEEN_W && CAP <= 65536 && REG_W * 64 <= 65536 && T_SEEN % 8 == 0 && T_LIST % 8 == 0,
@@ -1062,11 +1069,9 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   uint16_t* mj = (uint16_t*)(tab_b + T_MJ);
   constexpr int RW = sizeof(KeyT) == 8 ? 2 : 4;
   const int K2 = 2 * K;
-  unsigned int* n_over = flags;
+  unsigned int* n_over = flags + (LIST ? 4 : 0);
   unsigned int* n_big = flags + 1;
   unsigned int* err = flags + 2;
-  const uint64_t seg = blockIdx.x;
-  if (seg >= n_segs || flags[3]) return;  // flags[3]: the occurrence counts disagreed with W (k_p3_gate)
   const uint64_t start = seg_off[seg];
   const uint64_t n = seg_off[seg + 1] - start;
   if (n == 0) {
@@ -1083,7 +1088,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   // slots that hold elements (uniform): the slot loops stop there, so a
   // segment of n elements costs ceil(n / WG) slots of work, not KPT
   int ns = KPT;
-  if (sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS) {
+  if (!LIST && sizeof(KeyT) == 8 && K2 <= BM_MAX_BITS) {
     // ---- direct-address table of masks over all 2^K2 keys: any number of
     // occurrences; distinct keys come out sorted at occ / obs[start...].
     const uint32_t nt = 1u << K2;
@@ -1108,7 +1113,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
     }
     u = tot;
     if (threadIdx.x == 0) ucount[seg] = u;
-    if (seg_is_big<KeyT>(n, u, K2)) {
+    if (seg_is_big<KeyT, CAP>(n, u, K2)) {
       if (threadIdx.x == 0) {
         big_list[atomicAdd(n_big, 1u)] = (uint32_t)seg;
         nblk[seg] = 0;
@@ -1270,9 +1275,9 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
       if (s >= ns) break;
       uint32_t slot = 0, fp = 0;
       const bool v = (pend >> s) & 1;
-      if (v) tab_slot(key[s], sb, tb, slot, fp);
+      if (v) tab_slot<FPB>(key[s], sb, tb, slot, fp);
       hs[s] = v ? slot : 0u;
-      ent[s] = v ? ((uint32_t)(s * WG + threadIdx.x + 1) | (msk[s] << 12) | (fp << 22)) : 0u;
+      ent[s] = v ? ((uint32_t)(s * WG + threadIdx.x + 1) | (msk[s] << EB) | (fp << (EB + 10))) : 0u;
     }
     while (pend) {
       uint32_t old[KPT];
@@ -1288,8 +1293,8 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
           if (old[s] == 0) {
             alive |= 1u << s;
             pend &= ~(1u << s);
-          } else if ((old[s] >> 22) == (ent[s] >> 22) && dd_same(key[s], occ[start + (old[s] & 0xFFFu) - 1])) {
-            atomicOr(&tab[hs[s]], msk[s] << 12);
+          } else if ((old[s] >> (EB + 10)) == (ent[s] >> (EB + 10)) && dd_same(key[s], occ[start + (old[s] & EMASK) - 1])) {
+            atomicOr(&tab[hs[s]], msk[s] << EB);
             pend &= ~(1u << s);
           } else {
             hs[s] = (hs[s] + 1) & tmask;
@@ -1303,7 +1308,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
 #pragma unroll
     for (int s = 0; s < KPT; s++) {
       if (s >= ns) break;
-      if ((alive & cand) >> s & 1) msk[s] = (tab[hs[s]] >> 12) & 0x3FFu;
+      if ((alive & cand) >> s & 1) msk[s] = (tab[hs[s]] >> EB) & 0x3FFu;
     }
     uint64_t tot;
     block_excl_scan<WG>((uint64_t)__popc(alive), sh8, &tot);  // barriers: tab is free after it
@@ -1391,6 +1396,45 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
   }
   __syncthreads();
   for (uint64_t i = threadIdx.x; i < u; i += WG) vpad[start + i] = stage_m[i];
+}
+
+template <typename KeyT>
+__global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_dedup(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+                                                    const uint64_t* __restrict__ srec, const uint64_t* __restrict__ roff,
+                                                    const uint64_t* __restrict__ seg_off, uint64_t n_segs, int K, int sb,
+                                                    uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
+                                                    uint32_t* __restrict__ over_list, unsigned int* __restrict__ flags,
+                                                    uint32_t* __restrict__ big_list, uint64_t* __restrict__ blocks_pad,
+                                                    uint16_t* __restrict__ vpad) {
+  const uint64_t seg = blockIdx.x;
+  if (seg >= n_segs || flags[3]) return;  // flags[3]: the occurrence counts disagreed with W (k_p3_gate)
+  dd_segment<KeyT, DD_WG, DD_KPT, false>(seg, occ, obs, srec, roff, seg_off, K, sb, ucount, nblk, over_list, flags, big_list,
+                                         blocks_pad, vpad);
+}
+
+// The segments k_p3_dedup put in over_list (flags[0] of them), DD2_WG
+// threads each, one work-group per entry over a grid of at most DD2_GRID
+// (the list's length is known on the device only; a grid-stride loop around
+// the segment's dedup would keep its loop-invariant values live through it
+// and spill at any occupancy). Entries beyond the grid, segments above
+// DD2_CAP occurrences and those with a record longer than the decode's owner
+// map go on to over2 (count flags[4]) for k_p3_bigsort.
+constexpr unsigned DD2_GRID = 4096;
+template <typename KeyT>
+__global__ void __launch_bounds__(DD2_WG, 4) k_p3_dedup_list(KeyT* __restrict__ occ, uint16_t* __restrict__ obs,
+                                                             const uint64_t* __restrict__ srec, const uint64_t* __restrict__ roff,
+                                                             const uint64_t* __restrict__ seg_off, int K, int sb,
+                                                             uint64_t* __restrict__ ucount, uint64_t* __restrict__ nblk,
+                                                             const uint32_t* __restrict__ over_list, uint32_t* __restrict__ over2,
+                                                             unsigned int* __restrict__ flags, uint64_t* __restrict__ blocks_pad,
+                                                             uint16_t* __restrict__ vpad) {
+  if (flags[3]) return;
+  const uint32_t n_list = flags[0];
+  if (threadIdx.x == 0)
+    for (uint32_t i = blockIdx.x + gridDim.x; i < n_list; i += gridDim.x) over2[atomicAdd(flags + 4, 1u)] = over_list[i];
+  if (blockIdx.x >= n_list) return;
+  dd_segment<KeyT, DD2_WG, DD_KPT, true>(over_list[blockIdx.x], occ, obs, srec, roff, seg_off, K, sb, ucount, nblk, over2, flags,
+                                         nullptr, blocks_pad, vpad);
 }
 
 // Oversized segments: one work-group per segment runs a stable LSD radix
@@ -1804,6 +1848,18 @@ void launch_p3_dedup(hipStream_t st, bool wide, void* occ, uint16_t* obs, const 
   else
     CF1G_LAUNCH("p3_dedup", k_p3_dedup<Key128>, dim3((unsigned)n_segs), dim3(DD_WG), 0, st, (Key128*)occ, obs, srec, roff, seg_off,
                 n_segs, K, sb, ucount, nblk, over_list, flags, big_list, blocks_pad, vpad);
+}
+void launch_p3_dedup_list(hipStream_t st, bool wide, void* occ, uint16_t* obs, const uint64_t* srec, const uint64_t* roff,
+                          const uint64_t* seg_off, uint64_t n_segs, int K, int sb, uint64_t* ucount, uint64_t* nblk,
+                          const uint32_t* over_list, uint32_t* over2, unsigned int* flags, uint64_t* blocks_pad, uint16_t* vpad) {
+  if (n_segs == 0 || (!wide && 2 * K <= BM_MAX_BITS)) return;  // the direct-address table leaves no list
+  const dim3 g(n_segs < DD2_GRID ? (unsigned)n_segs : DD2_GRID);
+  if (!wide)
+    CF1G_LAUNCH("p3_dedup_list", k_p3_dedup_list<uint64_t>, g, dim3(DD2_WG), 0, st, (uint64_t*)occ, obs, srec, roff, seg_off, K, sb,
+                ucount, nblk, over_list, over2, flags, blocks_pad, vpad);
+  else
+    CF1G_LAUNCH("p3_dedup_list", k_p3_dedup_list<Key128>, g, dim3(DD2_WG), 0, st, (Key128*)occ, obs, srec, roff, seg_off, K, sb,
+                ucount, nblk, over_list, over2, flags, blocks_pad, vpad);
 }
 void launch_p3_bigsort(hipStream_t st, bool wide, void* occ, uint16_t* obs, void* aux, uint16_t* aux_obs, const uint64_t* seg_off,
                        const uint32_t* over_list, uint32_t n_over, uint64_t* ucount, unsigned int* flags, uint32_t* big_list) {

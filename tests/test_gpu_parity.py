@@ -58,7 +58,7 @@ def kmers_of(strs, k):
     return sorted(out)
 
 
-def run_pair(seqs, k, num_bins=128, names=None, p5=True, bins=True, threads=4, target_seg=0):
+def run_pair(seqs, k, num_bins=128, names=None, p5=True, bins=True, threads=4, target_seg=0, on_done=None):
     names = names or [f"r{i}" for i in range(len(seqs))]
     ref = RefPipeline(k, num_bins=num_bins, threads=threads, target_seg=target_seg)
     ref.load(seqs, names)
@@ -108,6 +108,8 @@ def run_pair(seqs, k, num_bins=128, names=None, p5=True, bins=True, threads=4, t
             jr["parameters info"].pop("output prefix")
             assert jg == jr
             assert mg == ref.meta()
+    if on_done:
+        on_done(g, ref)
     g.close()
     return d
 
@@ -145,6 +147,26 @@ def test_heavy_duplicates_bigsort():
     seqs = [b"A" * 120000, b"ACGT" * 30000, b"C" * 5000 + b"N" + b"G" * 5000]
     run_pair(seqs, 31, num_bins=4, bins=False)
     run_pair(seqs, 63, num_bins=4, bins=False)
+
+
+@pytest.mark.parametrize("k", [31, 63])
+def test_segments_above_first_dedup_capacity(k):
+    # 2 bins and a 4,000 target give 8 segments of ~1,800-2,900 distinct keys
+    # (~1.17 occurrences each): most exceed the 256-thread dedup's 2,048
+    # occurrences and take the 512-thread list dedup (up to 4,096), so none
+    # is sorted through global memory
+    seqs = random_genome(seed=78, n_rec=4, max_len=20000)
+    got = {}
+
+    def check(g, ref):
+        w = ref.mphf_export()
+        nseg, off = int(w[5]), 16 + 65536 // 4
+        got["u"] = np.diff(w[off:off + nseg + 1].astype(np.int64))
+        got["info"] = g.info()
+
+    run_pair(seqs, k, num_bins=2, target_seg=4000, p5=False, bins=False, on_done=check)
+    assert (got["u"] > 2048).sum() >= 6 and got["u"].max() < 3000
+    assert got["info"]["n_oversized"] == 0
 
 
 def test_transcriptome_small():
