@@ -320,17 +320,24 @@ __host__ __device__ __forceinline__ uint32_t obs_mask(uint32_t type, bool fwd, u
   const uint32_t b = fwd ? (hn ? nb : LBL_NONE) : (hp ? 3 - pb : LBL_NONE);
   return (1u << a) | (1u << (5 + b));
 }
+// Orders one wave's LDS accesses before and after it (a wave needs no block
+// barrier to see its own LDS writes).
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 // State::from_vertex (state.rs:92-113) of the joined label sets.
+// Arithmetic only (lanes of a wave hold different masks, and the compiler
+// turns chained conditions into branches that run every path): a side is
+// single when its label set is one of bits 0-3 (bits 1, 2, 4, 8 of 0x116);
+// the base of the state comes from (single in, single out) as a byte of
+// 0x10040803, and the labels are added on the single sides.
 __host__ __device__ __forceinline__ uint32_t mask_state(uint32_t m) {
-  if (m == 0) return 0;
   const uint32_t I = m & 31u, O = (m >> 5) & 31u;
-  const bool si = I != 0 && (I & (I - 1)) == 0 && I != 16u;
-  const bool so = O != 0 && (O & (O - 1)) == 0 && O != 16u;
-  const uint32_t f = si ? (uint32_t)__builtin_ctz(I) : 0u, b = so ? (uint32_t)__builtin_ctz(O) : 0u;
-  if (si && so) return 16u | (f << 2) | b;
-  if (so) return 4u | b;
-  if (si) return 8u | f;
-  return 3u;
+  const uint32_t si = (0x116u >> I) & 1u, so = (0x116u >> O) & 1u;
+  const uint32_t f = (uint32_t)__builtin_ctz(I | 32u) & 3u, b = (uint32_t)__builtin_ctz(O | 32u) & 3u;
+  const uint32_t s = ((0x10040803u >> (8 * (si | (so << 1)))) & 0xFFu) | ((f * si) << (2 * so)) | (b * so);
+  return (uint32_t)(m != 0) * s;
 }
 // The label sets a 5-bit state stands for, as a mask that joins like the
 // DFA: a single side keeps its label bit, a multi side is the "none" bit

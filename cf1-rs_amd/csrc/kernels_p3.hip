@@ -2,17 +2,20 @@
 // dedup_bin) and the BBHash build (the project's replacement for ph::fmph,
 // mphf.rs:144-190).
 //
-// P3 layout: the k-mers of bin b are split into 2^sb segments (b, s) by the
-// top sb bits of a multiplicative hash of the canonical key (seg_sub), so
-// segments are evenly sized. Segments are counted (k_p3_count) and scattered
-// into one occurrence array with each occurrence's 10-bit observation mask
-// (k_p3_scatter, k_p3_split). k_p3_dedup deduplicates one segment per
-// work-group (LDS hash table, or a direct-address table for keys of <= 12
-// bits), OR-joins the masks of equal keys and builds the segment's BBHash
-// into a padded block slot, writing keys and masks in MPHF value order;
-// segments above the LDS capacity take a global-memory radix sort
-// (k_p3_bigsort) and k_bbh_big. k_p3_compact then packs keys and masks (both
-// in value order) and the blocks densely.
+// P3 layout: bin b's records (super k-mers) are split into 2^sb segments
+// (b, s) by sb bits of their minimizer's hash, so that a segment is a
+// function of the super k-mer and its k-mers never span two segments.
+// k_p3_rcount counts records and k-mers per segment, k_p3_gate checks the
+// counted k-mers against the P1 total, and k_p3_rscatter sorts the records
+// into their segments' runs. k_p3_dedup deduplicates one segment per
+// work-group, decoding its records itself (LDS hash table, or a
+// direct-address table for keys of <= 12 bits), OR-joins the observation
+// masks of equal keys and builds the segment's BBHash into a padded block
+// slot, writing the masks in MPHF value order; k_p3_dedup_list does the
+// same at 512 threads for the segments above its capacity, and larger ones
+// take a global-memory radix sort (k_p3_expand, k_p3_bigsort) and
+// k_bbh_big. k_p3_compact then packs the masks (value order) and the blocks
+// densely.
 #include <algorithm>
 
 #include "block.hpp"
@@ -716,16 +719,14 @@ template <>
 struct SegCfg<Key128> {
   static constexpr int KPT = 8;
 };
-constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12 bits (k <= 5)
-// k_p3_dedup: DD_WG threads, DD_KPT occurrences each, at most DD_CAP per
-// segment; k_p3_dedup_list takes the segments of up to DD2_CAP occurrences
+constexpr int BM_MAX_BITS = 12;   // direct-address mask table for keys of <= 12 bits (k <= 6)
+// k_p3_dedup: DD_WG threads, DD_KPT occurrences each, at most 2,048 per
+// segment; k_p3_dedup_list takes the segments of up to 4,096 occurrences
 // that exceed it, DD2_WG threads each (the 300 Mbp genome's largest segment
 // holds ~2,600)
 constexpr int DD_WG = 256;
 constexpr int DD_KPT = 8;
-constexpr int DD_CAP = DD_WG * DD_KPT;
 constexpr int DD2_WG = 512;
-constexpr int DD2_CAP = DD2_WG * DD_KPT;
 constexpr int DEC_PK = 4;  // k-mers per lane in the dedup's record decode
 
 // A segment's MPHF is built by k_bbh_big when its distinct keys exceed the
@@ -786,10 +787,6 @@ __device__ __forceinline__ uint32_t wave_prefix4(uint32_t c) {
 #pragma unroll
   for (int b = 0; b < 4; b++) s += (uint32_t)__popcll(__ballot((c >> b) & 1) & below) << b;
   return s;
-}
-__device__ __forceinline__ void wave_lds_order() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
 }
 
 template <int KPT, int WG>
@@ -1417,7 +1414,7 @@ __global__ void __launch_bounds__(DD_WG, DdOccupancy<KeyT>::WGS_PER_CU) k_p3_ded
 // (the list's length is known on the device only; a grid-stride loop around
 // the segment's dedup would keep its loop-invariant values live through it
 // and spill at any occupancy). Entries beyond the grid, segments above
-// DD2_CAP occurrences and those with a record longer than the decode's owner
+// DD2_WG * DD_KPT occurrences and those with a record longer than the decode's owner
 // map go on to over2 (count flags[4]) for k_p3_bigsort.
 constexpr unsigned DD2_GRID = 4096;
 template <typename KeyT>
@@ -1603,11 +1600,11 @@ __global__ void __launch_bounds__(CP_WG) k_p3_compact(const uint16_t* __restrict
                                                       uint64_t n_segs, uint16_t* __restrict__ vmask, uint64_t* __restrict__ blocks) {
   const uint64_t seg = (uint64_t)blockIdx.x * (CP_WG / 64) + (threadIdx.x >> 6);
   if (seg >= n_segs) return;
-  const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
-  if (u == 0) return;
+  // the segment's words loaded together: no test between them (an empty
+  // segment copies nothing)
+  const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg], b0 = blk_off[seg], b1 = blk_off[seg + 1];
   wave_copy(vpad + s0, vmask + ko, u);
-  const uint64_t b0 = blk_off[seg], nw = (blk_off[seg + 1] - b0) * 8;
-  wave_copy(blocks_pad + pad_slot_off(s0, seg) * 8, blocks + b0 * 8, nw);
+  wave_copy(blocks_pad + pad_slot_off(s0, seg) * 8, blocks + b0 * 8, (b1 - b0) * 8);
 }
 
 // The dense key array on request: every key P3 left in its segment's run of
