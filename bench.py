@@ -65,7 +65,7 @@ PHASE_OF = {
     "p3_gate": "P3", "p3_rcount": "P3", "p3_rscatter": "P3", "p3_expand": "P3", "p3_dedup": "P3", "p3_dedup_list": "P3", "p3_bigsort": "P3",
     "p3_compact": "P3", "p1_excl": "P1", "scan_total": "scan", "zero": "clear",
     "scan_tile": "scan", "scan_add": "scan", "p3b_bbh_big": "P3", "p3b_bbh_huge": "P3",
-    "p4_classify": "P4",
+    "p4_states": "P4", "p4_classify": "P4",
 }
 
 
@@ -120,9 +120,11 @@ def algorithmic_bytes(info, kb, rw):
                    4-byte destination per record out
       p3_dedup   : records in; D distinct keys and D masks (local-rank
                    order) and the BBHash blocks out
-      p3_compact : D masks, B in; the same out (dense; the dense key array is
-                   compacted only when a caller asks for the keys)
-      p4_classify: D masks in, ceil(D/12) state words out
+      p3_compact : the BBHash blocks B in and out (dense; the masks stay in
+                   their segments' runs, and the dense key array is built only
+                   when a caller asks for the keys)
+      p4_states  : D masks (segment runs) in, D state bytes (dense) out
+      p4_classify: D state bytes in, ceil(D/12) state words out
     """
     NG = info["stream_len"]
     S, D = info["n_superkmers"], info["n_distinct"]
@@ -137,8 +139,9 @@ def algorithmic_bytes(info, kb, rw):
         "p3_rcount": 10 * S,
         "p3_rscatter": 2 * rec + 6 * S,
         "p3_dedup": rec + D * (kb + 2) + B,
-        "p3_compact": 2 * (2 * D + B),
-        "p4_classify": 2 * D + 8 * ((D + 11) // 12),
+        "p3_compact": 2 * B,
+        "p4_states": 2 * D + D,
+        "p4_classify": D + 8 * ((D + 11) // 12),
     }
 
 

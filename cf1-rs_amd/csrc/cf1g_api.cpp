@@ -67,11 +67,11 @@ void launch_p3_bigsort(hipStream_t, bool, void*, uint16_t*, void*, uint16_t*, co
                        unsigned int*, uint32_t*);
 void launch_bbh_big(hipStream_t, bool, const void*, const uint16_t*, uint16_t*, const uint64_t*, const uint64_t*, const uint32_t*,
                     uint32_t, uint64_t*, uint64_t*, uint64_t*, uint64_t*, void*, unsigned int*);
-void launch_p3_compact(hipStream_t, bool, const uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
-                       const uint64_t*, uint64_t, uint16_t*, uint64_t*);
+void launch_p3_compact(hipStream_t, bool, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, uint64_t*);
 void launch_p3_keys(hipStream_t, bool, const void*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*, const uint64_t*,
                     uint64_t, void*, unsigned int*);
-void launch_p4_pack(hipStream_t, const uint16_t*, uint64_t, uint64_t*);
+void launch_p4_pack(hipStream_t, const uint16_t*, const uint64_t*, const uint64_t*, const uint64_t*, uint64_t, uint64_t, uint8_t*,
+                    uint64_t*);
 void launch_stream_digest(hipStream_t, const uint64_t*, const uint64_t*, uint64_t, unsigned long long*);
 void noalloc_setup(int device);
 void* noalloc_scratch(int device, size_t* bytes);
@@ -775,13 +775,13 @@ void do_p3(cf1g_ctx* c) {
   c->D = D;
   c->n_blocks = tot[2];
   const uint64_t nsw = (D + 11) / 12;
-  uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
   uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
-  // masks and blocks now; the dense key array (which the reference's MPHF
-  // does not keep) is built from occ on first use, dense_keys()
+  // dense blocks now (the masks stay in the segments' runs of vpad for P4);
+  // the dense key array (which the reference's MPHF does not keep) is built
+  // from occ on first use, dense_keys()
   c->seg("p3c", {ns, D, c->n_blocks}, [&] {
-    launch_p3_compact(st, c->wide, vpad, seg_off, ucount, key_off, blk_off, blocks_pad, ns, vmask, blocks);
+    launch_p3_compact(st, c->wide, seg_off, blk_off, blocks_pad, ns, blocks);
     c->zero(states, (nsw + 1) * 8);
   });
   c->keys_ready = false;
@@ -814,13 +814,18 @@ void* dense_keys(cf1g_ctx* c) {
   return keys;
 }
 
-// P4: the joined masks, already in MPHF-value order, become packed states.
+// P4: the joined masks, in MPHF-value order within each segment's run of
+// vpad, become one state byte per value (dense), then packed states.
 void do_p4(cf1g_ctx* c) {
   if (c->restored) fail(CF1G_E_ORDER, "a restored MPHF keeps no joined observations: classify with cf1g_p4_classify_cas");
-  const uint64_t nsw = (c->D + 11) / 12;
-  uint16_t* vmask = c->getT<uint16_t>("vmask", 12 * nsw + 16);
+  const uint64_t nsw = (c->D + 11) / 12, ns = c->n_segs;
+  uint8_t* vstate = c->getT<uint8_t>("vstate", 12 * nsw + 16);
   uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
-  c->seg("p4", {c->D}, [&] { launch_p4_pack(c->st, vmask, c->D, states); });
+  const uint16_t* vpad = c->getT<uint16_t>("vpad", c->W + 8);
+  const uint64_t* seg_off = c->getT<uint64_t>("seg_off", ns + 1);
+  const uint64_t* ucount = c->getT<uint64_t>("seg_ucount", ns + 1);
+  const uint64_t* key_off = c->getT<uint64_t>("seg_out_off", ns + 1);
+  c->seg("p4", {ns, c->D}, [&] { launch_p4_pack(c->st, vpad, seg_off, ucount, key_off, ns, c->D, vstate, states); });
   c->sync();
   c->set_stage(5);
 }

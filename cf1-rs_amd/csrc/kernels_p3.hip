@@ -14,8 +14,8 @@
 // slot, writing the masks in MPHF value order; k_p3_dedup_list does the
 // same at 512 threads for the segments above its capacity, and larger ones
 // take a global-memory radix sort (k_p3_expand, k_p3_bigsort) and
-// k_bbh_big. k_p3_compact then packs the masks (value order) and the blocks
-// densely.
+// k_bbh_big. k_p3_compact then packs the blocks densely; P4 turns the masks,
+// in value order within each segment's run, into states.
 #include <algorithm>
 
 #include "block.hpp"
@@ -1572,13 +1572,11 @@ __global__ void __launch_bounds__(RK_WG) k_p3_bigsort(KeyT* __restrict__ occ, ui
 }
 
 // ------------------------------------------------------------------ compaction
-// One work-group per segment copies the segment's masks (local-rank order,
-// vpad) to their MPHF values in vmask and its BBHash blocks from the padded
-// slot to the dense block order (blk_off).
-// One wave per segment (CP_WG / 64 segments per work-group): the masks and
-// the blocks are copied 8 loads deep per lane, loads before stores, so that a
-// wave keeps several requests in flight (with one work-group per segment the
-// copies were bound by work-group turnover).
+// A segment's BBHash blocks go from its padded slot to the dense block order
+// (blk_off), one wave per segment (CP_WG / 64 segments per work-group),
+// copied 8 loads deep per lane, loads before stores, so that a wave keeps
+// several requests in flight (with one work-group per segment the copies
+// were bound by work-group turnover).
 constexpr int CP_WG = 256;
 constexpr int CP_DEPTH = 8;
 template <typename T>
@@ -1594,16 +1592,14 @@ __device__ __forceinline__ void wave_copy(const T* __restrict__ src, T* __restri
   for (; i < n; i += 64) dst[i] = src[i];
 }
 template <typename KeyT>
-__global__ void __launch_bounds__(CP_WG) k_p3_compact(const uint16_t* __restrict__ vpad, const uint64_t* __restrict__ seg_off,
-                                                      const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
-                                                      const uint64_t* __restrict__ blk_off, const uint64_t* __restrict__ blocks_pad,
-                                                      uint64_t n_segs, uint16_t* __restrict__ vmask, uint64_t* __restrict__ blocks) {
+__global__ void __launch_bounds__(CP_WG) k_p3_compact(const uint64_t* __restrict__ seg_off, const uint64_t* __restrict__ blk_off,
+                                                      const uint64_t* __restrict__ blocks_pad, uint64_t n_segs,
+                                                      uint64_t* __restrict__ blocks) {
   const uint64_t seg = (uint64_t)blockIdx.x * (CP_WG / 64) + (threadIdx.x >> 6);
   if (seg >= n_segs) return;
   // the segment's words loaded together: no test between them (an empty
   // segment copies nothing)
-  const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg], b0 = blk_off[seg], b1 = blk_off[seg + 1];
-  wave_copy(vpad + s0, vmask + ko, u);
+  const uint64_t s0 = seg_off[seg], b0 = blk_off[seg], b1 = blk_off[seg + 1];
   wave_copy(blocks_pad + pad_slot_off(s0, seg) * 8, blocks + b0 * 8, (b1 - b0) * 8);
 }
 
@@ -1889,17 +1885,14 @@ void launch_bbh_big(hipStream_t st, bool wide, const void* occ, const uint16_t* 
                   nblk, blocks_pad, scratch, alive_g, (Key128*)kout, err);
   }
 }
-void launch_p3_compact(hipStream_t st, bool wide, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount,
-                       const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks_pad, uint64_t n_segs, uint16_t* vmask,
-                       uint64_t* blocks) {
+void launch_p3_compact(hipStream_t st, bool wide, const uint64_t* seg_off, const uint64_t* blk_off, const uint64_t* blocks_pad,
+                       uint64_t n_segs, uint64_t* blocks) {
   if (n_segs == 0) return;
   const dim3 grid((unsigned)((n_segs + CP_WG / 64 - 1) / (CP_WG / 64)));
   if (!wide)
-    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, grid, dim3(CP_WG), 0, st, vpad, seg_off, ucount, key_off, blk_off, blocks_pad,
-                n_segs, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<uint64_t>, grid, dim3(CP_WG), 0, st, seg_off, blk_off, blocks_pad, n_segs, blocks);
   else
-    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, grid, dim3(CP_WG), 0, st, vpad, seg_off, ucount, key_off, blk_off, blocks_pad,
-                n_segs, vmask, blocks);
+    CF1G_LAUNCH("p3_compact", k_p3_compact<Key128>, grid, dim3(CP_WG), 0, st, seg_off, blk_off, blocks_pad, n_segs, blocks);
 }
 void launch_p3_keys(hipStream_t st, bool wide, const void* kout, const uint64_t* seg_off, const uint64_t* ucount,
                     const uint64_t* key_off, const uint64_t* blk_off, const uint64_t* blocks, uint64_t n_segs, void* keys,

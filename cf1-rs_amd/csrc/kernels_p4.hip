@@ -27,20 +27,48 @@
 
 namespace cf1g {
 
-// One thread per state word: the 12 joined masks of values [12w, 12w+12)
-// (three 8-byte loads; the mask array is value-ordered by P3) become 5-bit
-// states, packed 12 per word with bits 60-63 zero (state_vector.rs:36-71).
-__global__ void __launch_bounds__(256) k_p4_pack(const uint64_t* __restrict__ vmask, uint64_t D, uint64_t n_words,
+// One wave per P3 segment: its joined masks (vpad[seg_off ...], value order
+// within the segment) become 5-bit states, one byte each, at vstate[key_off
+// ...]: the dense state of every value, copied eight loads deep. The states
+// of all 2^10 masks sit in an LDS table (one entry per thread).
+constexpr int PS_WG = 1024;
+__global__ void __launch_bounds__(PS_WG) k_p4_states(const uint16_t* __restrict__ vpad, const uint64_t* __restrict__ seg_off,
+                                                     const uint64_t* __restrict__ ucount, const uint64_t* __restrict__ key_off,
+                                                     uint64_t n_segs, uint8_t* __restrict__ vstate) {
+  __shared__ uint8_t st_of[1024];
+  st_of[threadIdx.x] = (uint8_t)mask_state(threadIdx.x);
+  __syncthreads();
+  const uint64_t seg = (uint64_t)blockIdx.x * (PS_WG / 64) + (threadIdx.x >> 6);
+  if (seg >= n_segs) return;
+  // the segment's words loaded together: no test between them
+  const uint64_t u = ucount[seg], s0 = seg_off[seg], ko = key_off[seg];
+  const uint16_t* src = vpad + s0;
+  uint8_t* dst = vstate + ko;
+  uint64_t i = __lane_id();
+  for (; i + 7 * 64 < u; i += 8 * 64) {
+    uint32_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = src[i + q * 64];
+#pragma unroll
+    for (int q = 0; q < 8; q++) dst[i + q * 64] = st_of[v[q] & 1023u];
+  }
+  for (; i < u; i += 64) dst[i] = st_of[src[i] & 1023u];
+}
+
+// One thread per state word: the 12 states of values [12w, 12w+12) (three
+// 4-byte loads) packed 12 per word with bits 60-63 zero
+// (state_vector.rs:36-71).
+__global__ void __launch_bounds__(256) k_p4_pack(const uint32_t* __restrict__ vstate, uint64_t D, uint64_t n_words,
                                                  uint64_t* __restrict__ states) {
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= n_words) return;
-  uint64_t m[3] = {vmask[3 * w], vmask[3 * w + 1], vmask[3 * w + 2]};
+  const uint32_t s[3] = {vstate[3 * w], vstate[3 * w + 1], vstate[3 * w + 2]};
   const uint64_t valid = D - 12 * w;  // >= 1
   uint64_t out = 0;
 #pragma unroll
   for (int i = 0; i < 12; i++) {
-    const uint32_t mk = (uint32_t)(m[i >> 2] >> (16 * (i & 3))) & 0xFFFFu;
-    if ((uint64_t)i < valid) out |= (uint64_t)mask_state(mk) << (5 * i);
+    const uint64_t st = (s[i >> 2] >> (8 * (i & 3))) & 31u;
+    out |= (uint64_t)i < valid ? st << (5 * i) : 0;
   }
   states[w] = out;
 }
@@ -277,10 +305,14 @@ __global__ void __launch_bounds__(256) k_lookup(const KeyT* __restrict__ keys, u
   }
 }
 
-void launch_p4_pack(hipStream_t st, const uint16_t* vmask, uint64_t D, uint64_t* states) {
+// vstate: 12 * ceil(D/12) bytes, 4-byte aligned.
+void launch_p4_pack(hipStream_t st, const uint16_t* vpad, const uint64_t* seg_off, const uint64_t* ucount, const uint64_t* key_off,
+                    uint64_t n_segs, uint64_t D, uint8_t* vstate, uint64_t* states) {
   const uint64_t nw = (D + 11) / 12;
   if (nw == 0) return;
-  CF1G_LAUNCH("p4_classify", k_p4_pack, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (const uint64_t*)vmask, D, nw, states);
+  CF1G_LAUNCH("p4_states", k_p4_states, dim3((unsigned)((n_segs + PS_WG / 64 - 1) / (PS_WG / 64))), dim3(PS_WG), 0, st, vpad, seg_off,
+              ucount, key_off, n_segs, vstate);
+  CF1G_LAUNCH("p4_classify", k_p4_pack, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (const uint32_t*)vstate, D, nw, states);
 }
 // counters: 5 zeroed words (see k_p4_cas). Chunks cover stream positions
 // [1, n_pos); grid = persistent work-groups.
