@@ -776,14 +776,12 @@ void do_p3(cf1g_ctx* c) {
   c->n_blocks = tot[2];
   const uint64_t nsw = (D + 11) / 12;
   uint64_t* blocks = c->getT<uint64_t>("blocks", c->n_blocks * 8 + 8);
-  uint64_t* states = c->getT<uint64_t>("states", nsw + 1);
+  c->getT<uint64_t>("states", nsw + 1);  // allocated with the MPHF; P4 writes every word
+  c->getT<uint8_t>("vstate", 12 * nsw + 16);
   // dense blocks now (the masks stay in the segments' runs of vpad for P4);
   // the dense key array (which the reference's MPHF does not keep) is built
   // from occ on first use, dense_keys()
-  c->seg("p3c", {ns, D, c->n_blocks}, [&] {
-    launch_p3_compact(st, c->wide, seg_off, blk_off, blocks_pad, ns, blocks);
-    c->zero(states, (nsw + 1) * 8);
-  });
+  c->seg("p3c", {ns, D, c->n_blocks}, [&] { launch_p3_compact(st, c->wide, seg_off, blk_off, blocks_pad, ns, blocks); });
   c->keys_ready = false;
   c->lidx_ready = c->lidx_states = false;
   c->use_index = 0;
